@@ -1,0 +1,148 @@
+#!/usr/bin/env python3
+"""Throughput of the explicit 3D-3V PIC-MCC step on one MI355X.
+
+Workload: 128^3-cell grounded box, 64 particles/cell/species of e- and Ar+
+(268,435,456 particles), uniform Bz = 0.01 T, argon background at 1 Pa/300 K
+with the analytic cross sections, null-collision MCC every 10 steps (Philox,
+parallel), cell re-sort every 20 steps, FP64 particles, FP64 multigrid
+Poisson solve to 1e-8.  Particles are synthetic (device loader).
+
+Prints one JSON line: particle-steps/s of the whole step, the roofline of the
+fused particle kernel (HIP events around its launches, on the stream it runs
+on) and a CPU baseline from the oracle timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_PARTICLE_STEP = 104   # FP64 fused gather+push+BC+deposit, state read and written once
+
+
+def cpu_baseline(case, sample_particles=2_000_000, seconds=20.0):
+    """Oracle (C restatement of the reference, 1 thread) on a bounded sample:
+    push+BC+deposit per particle on the 128^3 grid, one warm Poisson solve,
+    E and rho at 128^3; composed into particle-steps/s of the full workload."""
+    import numpy as np
+    from oracle import oracle as O
+    O.build()
+    n, h = case.n, case.h
+    rng = np.random.default_rng(7)
+    L = np.array([n[d] * h[d] for d in range(3)])
+    m = sample_particles
+    pos = rng.random((m, 3)) * L
+    vel = 1e6 * rng.standard_normal((m, 3))
+    p = O.Particles(m + 16).set(pos[:, 0], pos[:, 1], pos[:, 2], vel[:, 0], vel[:, 1], vel[:, 2])
+    bc = case.bcnd
+    kq = case.kq()
+    E4 = np.zeros(bc.shape + (4,))
+    E4[..., :3] = 1e3 * rng.standard_normal(bc.shape + (3,))
+    sp = case.species[0]
+    t0 = time.perf_counter()
+    O.move_boris(n, h, p, E4, 1, case.B_uniform, None, sp.charge, sp.mass, case.dt)
+    O.apply_bc(n, h, p, bc, 0, 0, 1, -1, case.Nm[0] * sp.charge, sp.mass, case.Nm[0], case.ngrid)
+    O.deposit(n, h, p, kq, case.Nm[0])
+    t_part = (time.perf_counter() - t0) / m
+    # grid phase: rho from a smooth density, then one solve warm-started from its own solution
+    rho = np.zeros(bc.shape)
+    rho[bc == -1] = 1e-5 * rng.standard_normal(int((bc == -1).sum()))
+    phi = np.zeros(bc.shape)
+    O.pdesolver(n, h, phi, rho, bc, ng=case.ng, eps=case.eps, omega=case.omega)
+    rho *= 1.001
+    t0 = time.perf_counter()
+    O.pdesolver(n, h, phi, rho, bc, ng=case.ng, eps=case.eps, omega=case.omega)
+    t_mg = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.efield(n, h, phi, bc)
+    npt = np.zeros((2,) + bc.shape)
+    O.build_rho(n, O.reduce_density(n, bc, npt), [-1.6e-19, 1.6e-19])
+    t_grid = time.perf_counter() - t0
+    N = 2 * case.np_cell * n[0] * n[1] * n[2]
+    step = N * t_part + t_mg + t_grid
+    return {"value": N / step, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle, 1 thread: Boris push+wall BC+deposit of {m} particles on the "
+                       f"{n[0]}^3 grid ({t_part * 1e9:.0f} ns/particle), one warm MG solve "
+                       f"({t_mg * 1e3:.0f} ms) and E+rho ({t_grid * 1e3:.0f} ms), composed to "
+                       f"the {N}-particle step")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--ppc", type=int, default=64)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+    from essaitemporaire_lepic_module_amd import _native
+    from essaitemporaire_lepic_module_amd import config as cfg
+    from essaitemporaire_lepic_module_amd.simulation import Simulation
+
+    case = cfg.cfg2_pic_mcc(n=args.n, np_cell=args.ppc)
+    torch.cuda.init()
+    sim = Simulation(case, "cuda:0", fused=True, loader="device", mcc_mode=_native.MCC_PHILOX,
+                     capacity_factor=1.25)
+    istep = 1
+    for _ in range(args.warmup):
+        sim.advance_one_step(istep)
+        istep += 1
+    n_before = sim.particle_count()
+    torch.cuda.synchronize()
+    sim.timers = {}
+    t0 = time.perf_counter()
+    particle_steps = 0
+    for _ in range(args.steps):
+        particle_steps += sim.particle_count()
+        sim.advance_one_step(istep)
+        istep += 1
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    phases = sim.phase_times()
+    fused_ms, fused_launches, fused_units = phases.get("push_bc_deposit", (0.0, 0, 0))
+    achieved = BYTES_PER_PARTICLE_STEP * fused_units / (fused_ms * 1e-3) / 1e9 if fused_ms else 0.0
+    mg_ms, mg_calls, _ = phases.get("poisson", (0.0, 1, 0))
+    line = {
+        "metric": "particle-steps/s of the 128^3 x 64 ppc x 2 species PIC-MCC step",
+        "value": particle_steps / wall,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (device Philox loader, Maxwellian e- 3 eV / Ar+ 0.03 eV)",
+        "config": {"workload": "PIC-MCC at scale: 128^3 cells, 64 ppc x 2 species, Bz=0.01 T, "
+                               "Ar 1 Pa, MCC every 10, sort every 20, FP64",
+                   "cells": args.n ** 3, "particles": n_before, "ppc": args.ppc,
+                   "parallelism": "1 GPU"},
+        "mg_solve_ms": mg_ms / max(mg_calls, 1),
+        "phase_ms_per_step": {k: v[0] / args.steps for k, v in phases.items()},
+        "roofline": {"bound": "hbm", "kernel": "k_push_bc_deposit",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_particle": BYTES_PER_PARTICLE_STEP,
+                     "avg_launch_ms": fused_ms / max(fused_launches, 1)},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(case)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

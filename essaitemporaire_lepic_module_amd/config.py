@@ -1,0 +1,259 @@
+"""Host-side set-up of the PIC-MCC step: constants, grids, boundary labels,
+species parameters and the reaction tables MCC consumes.
+
+This is one-time host work (not on the timed path).  It restates the pieces
+of the reference's set-up that define the step's inputs:
+
+* physical constants            -- modules/core/mod_constants.f90:7-12,
+                                   modules/particles/mod_reactions.f90:107-110
+* vt0, Nm, dt                   -- modules/simulation/mod_simParams.f90:98-130
+* kq (charge weights)           -- modules/fields/mod_charge_weights.f90:9-33
+* common energy grid, sig,
+  sig_list, sigv_mx, p_ncol     -- modules/particles/mod_reactions.f90:414-558
+* cadences and nu_uplim         -- modules/simulation/mod_simParams.f90:132-156
+
+Grid arrays are numpy arrays of shape (n3+3, n2+3, n1+3): node (i,j,k) of the
+reference's 0:n+2 index space is element [k, j, i] (x fastest in memory).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+# mod_constants.f90 / mod_reactions.f90
+QE = 1.60217646e-19
+ME = 9.10938188e-31
+AMU = 1.66053886e-27
+EPS0 = 8.854187817e-12
+KB = 1.380649e-23
+MAX_COL = 128
+NU_UPLIM_ELECTRON = 5.0e8      # mod_simParams.f90:155
+NU_UPLIM_OTHER = 1.0e7         # mod_simParams.f90:156
+
+
+def node_shape(n):
+    return (int(n[2]) + 3, int(n[1]) + 3, int(n[0]) + 3)
+
+
+def grounded_box_bcnd(n, label: int = 1) -> np.ndarray:
+    """Box whose six faces are one electrode: label on nodes {0,1,n+1,n+2} of
+    every dimension, plasma (-1) on nodes 2..n."""
+    b = np.full(node_shape(n), -1, np.int32)
+    for axis, nd in zip((2, 1, 0), n):
+        sl = [slice(None)] * 3
+        for idx in (0, 1, nd + 1, nd + 2):
+            sl[axis] = idx
+            b[tuple(sl)] = label
+    return b
+
+
+def biased_wall_bcnd(n, wall_label: int = 2) -> np.ndarray:
+    """Grounded box (label 1) whose x = L face (nodes n1+1, n1+2) carries its
+    own electrode label."""
+    b = grounded_box_bcnd(n, 1)
+    b[:, :, n[0] + 1:] = wall_label
+    return b
+
+
+def build_kq(bcnd: np.ndarray) -> np.ndarray:
+    """kq = 2 on every node with bcnd /= -1, else 1 (mod_charge_weights.f90:22-30)."""
+    return np.where(bcnd != -1, 2.0, 1.0)
+
+
+def omega_pe(n0: float) -> float:
+    return math.sqrt(n0 * QE * QE / (EPS0 * ME))
+
+
+def thermal_speed(T_eV: float, mass: float) -> float:
+    """vt0 = sqrt(2 q T / |m|) (mod_simParams.f90:112-116)."""
+    return math.sqrt(2.0 * QE * T_eV / abs(mass))
+
+
+@dataclasses.dataclass
+class Species:
+    name: str
+    mass: float          # kg, neutrals negative as in the reference
+    charge: float        # C
+    T_eV: float
+    tracked: bool = True
+
+
+@dataclasses.dataclass
+class Case:
+    """One run configuration: grid, boundary, species and step parameters."""
+    name: str
+    n: tuple
+    h: tuple
+    bcnd: np.ndarray
+    ngrid: int
+    electrode_V: dict
+    species: list
+    n0: float
+    np_cell: int
+    dt: float
+    steps: int
+    eps: float = 1e-8
+    omega: float = 1.4
+    ng: int = 5
+    ncycl: int = 20000
+    flag_pbc: int = 0
+    flag_nmn: int = 0
+    B_uniform: tuple | None = None
+    B_map: np.ndarray | None = None
+    ions_magnetized: bool = True
+    ngas: float = 0.0
+    nb_step_sort: int = 10
+    nb_step_collisions: int = 10
+    chemistry: dict | None = None
+
+    @property
+    def tracked(self):
+        return [s for s in self.species if s.tracked]
+
+    @property
+    def vt0(self):
+        return [thermal_speed(s.T_eV, s.mass) for s in self.species]
+
+    @property
+    def Nm(self):
+        """Macro-particle weight (mod_simParams.f90:118-122)."""
+        vol = self.h[0] * self.h[1] * self.h[2]
+        return [(self.n0 if s.charge != 0.0 else self.ngas) * vol / abs(self.np_cell)
+                for s in self.species]
+
+    def phi0(self) -> np.ndarray:
+        """Initial potential: electrode voltages on their labelled nodes."""
+        phi = np.zeros(node_shape(self.n))
+        for lab, v in self.electrode_V.items():
+            phi[self.bcnd == lab] = v
+        return phi
+
+    def kq(self):
+        return build_kq(self.bcnd)
+
+
+def argon_species(Te=3.0, Ti=0.03, Tgas_K=300.0):
+    return [Species("e", ME, -QE, Te), Species("Ar+", 39.95 * AMU, QE, Ti),
+            Species("Ar", -39.95 * AMU, 0.0, Tgas_K * KB / QE, tracked=False)]
+
+
+def cfg1_plain_box(n: int = 32, side: float = 0.01, np_cell: int = 20, steps: int = 100,
+                   ng: int = 5) -> Case:
+    """Plain PIC: grounded cubic box, e- + Ar+, no B, no collisions,
+    dt = 0.1/omega_pe, MG to 1e-8 (reference L1/alpha residual)."""
+    nn = (n, n, n)
+    h = (side / n,) * 3
+    n0 = 1e15
+    sp = argon_species()[:2]
+    return Case("plain-box", nn, h, grounded_box_bcnd(nn), 1, {1: 0.0}, sp, n0, np_cell,
+                0.1 / omega_pe(n0), steps, ng=ng, nb_step_collisions=0)
+
+
+def cfg2_pic_mcc(n: int = 128, np_cell: int = 64, steps: int = 1000, ng: int = 7,
+                 pressure_Pa: float = 1.0, Bz: float = 0.01) -> Case:
+    """PIC-MCC at scale: h = 3.125e-4 m (4 cm box at 128^3), uniform Bz,
+    argon background, analytic cross sections, MCC every 10 steps, sort every 20."""
+    nn = (n, n, n)
+    h = (3.125e-4,) * 3
+    n0 = 1e15
+    sp = argon_species()
+    ngas = pressure_Pa / (KB * 300.0)
+    c = Case("pic-mcc", nn, h, grounded_box_bcnd(nn), 1, {1: 0.0}, sp, n0, np_cell,
+             0.1 / omega_pe(n0), steps, ng=ng, B_uniform=(0.0, 0.0, Bz), ngas=ngas,
+             nb_step_sort=20, nb_step_collisions=10)
+    c.chemistry = argon_chemistry(sp, ngas)
+    return c
+
+
+# ---------------------------------------------------------------------------
+# Reaction tables (mod_reactions.f90:414-558 "ordering")
+# ---------------------------------------------------------------------------
+
+S0_EXCITATION = 3.6e-18   # eV^2 m^2: peak ~1.0e-20 m^2 at e*11.5 eV
+S0_IONIZATION = 1.9e-17   # eV^2 m^2: peak ~2.8e-20 m^2 at e*15.8 eV
+
+
+def lotz_like(E: np.ndarray, s0: float, Eth: float) -> np.ndarray:
+    """sigma(E) = s0 ln(E/Eth) / (E Eth) above threshold, 0 below."""
+    out = np.zeros_like(E)
+    m = E > Eth
+    out[m] = s0 * np.log(E[m] / Eth) / (E[m] * Eth)
+    return out
+
+
+def argon_energy_grid(npts: int = 200, extra=(11.5, 15.8)) -> np.ndarray:
+    grid = np.concatenate([[0.0], np.logspace(-2.0, 3.0, npts), np.asarray(extra, float)])
+    return np.unique(grid)
+
+
+def argon_chemistry(species, ngas: float, npts: int = 200) -> dict:
+    """Synthetic argon set: e+Ar elastic (1e-19 m^2), excitation (11.5 eV) and
+    ionization (15.8 eV) as s0 ln(E/Eth)/(E Eth), Ar+ + Ar charge exchange
+    (5e-19 m^2).  Species indices are 0-based: e=0, Ar+=1, Ar=2."""
+    E = argon_energy_grid(npts)
+    reactions = [
+        # (reactants, byproducts, Eth, sigma(E))
+        ((0, 2), (0, 2), 0.0, np.full_like(E, 1e-19)),
+        ((0, 2), (0, 2), 11.5, lotz_like(E, S0_EXCITATION, 11.5)),
+        ((0, 2), (0, 1, 0), 15.8, lotz_like(E, S0_IONIZATION, 15.8)),
+        ((1, 2), (1, 2), 0.0, np.full_like(E, 5e-19)),
+    ]
+    tables = [(E.copy(), r[3]) for r in reactions]
+    return build_reaction_tables(species, ngas, reactions, tables)
+
+
+def build_reaction_tables(species, ngas, reactions, tables) -> dict:
+    """Common energy grid, interpolated sig, sig_list, p_ncol and sigv_mx as
+    mod_reactions.f90:414-558 builds them."""
+    ntot = len(species)
+    ntrk = sum(1 for s in species if s.tracked)
+    ncol = len(reactions)
+    merged = np.unique(np.concatenate([t[0] for t in tables]))   # sort + drop duplicates
+    npt = merged.size
+    sig = np.zeros((ncol, npt))
+    for icol, (Ek, sg) in enumerate(tables):
+        if Ek.size < 2:
+            continue
+        sig_p = 0.0
+        for ipt in range(npt):
+            Ekp = merged[ipt]
+            for j in range(Ek.size - 1):
+                EkL, EkR, sL, sR = Ek[j], Ek[j + 1], sg[j], sg[j + 1]
+                if Ekp < EkL:
+                    sig_p = sL
+                    break
+                if EkL <= Ekp <= EkR:
+                    sig_p = sL + (Ekp - EkL) * (sR - sL) / (EkR - EkL)
+                    break
+            sig[icol, ipt] = sig_p
+    col_info = np.zeros((ncol, 10), np.int32)
+    sig_Eth = np.zeros(ncol)
+    sig_list = np.zeros((ntot, MAX_COL), np.int32)
+    p_ncol = np.zeros(ntot, np.int32)
+    sigv_mx = np.zeros((ntot, ncol))
+    masses = np.array([s.mass for s in species])
+    for icol, (re, by, Eth, _) in enumerate(reactions):
+        col_info[icol, 0] = len(re)
+        col_info[icol, 1] = len(by)
+        col_info[icol, 2:2 + len(re)] = re
+        col_info[icol, 2 + len(re):2 + len(re) + len(by)] = by
+        sig_Eth[icol] = Eth
+        r1, r2 = re[0], re[1]
+        sig_list[r1, p_ncol[r1]] = icol
+        p_ncol[r1] += 1
+        mu = abs(masses[r1]) * abs(masses[r2]) / (abs(masses[r1]) + abs(masses[r2]))
+        for ipt in range(npt):
+            vr = math.sqrt(2.0 * merged[ipt] * QE / mu)
+            if merged[ipt] >= 1.0:
+                sigv_mx[r1, icol] = max(sigv_mx[r1, icol], vr * sig[icol, ipt])
+    nu_uplim = np.array([NU_UPLIM_ELECTRON] + [NU_UPLIM_OTHER] * (ntot - 1))
+    vt0 = np.array([thermal_speed(s.T_eV, s.mass) for s in species])
+    return {
+        "ntype_tracked": ntrk, "ntype_total": ntot, "mass": masses, "vt0": vt0,
+        "neutral_density": np.array([ngas for s in species if not s.tracked], float),
+        "nu_uplim": nu_uplim, "p_ncol": p_ncol, "sig_list": sig_list.reshape(-1),
+        "ncol": ncol, "npt": npt, "sig_Er": merged, "sig": sig.reshape(-1), "sig_Eth": sig_Eth,
+        "col_info": col_info.reshape(-1), "sigv_mx": sigv_mx.reshape(-1),
+    }

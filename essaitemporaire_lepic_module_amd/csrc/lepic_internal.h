@@ -1,0 +1,193 @@
+// lepic_internal.h -- shared device helpers and the context of liblepic_hip.so.
+//
+// All arithmetic on the FP64 path keeps the reference's operation order and
+// is compiled with -ffp-contract=off, so each product and sum rounds once as
+// in a strict build of the reference.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/lepic.h"
+
+#define LEPIC_WAVE 64
+
+struct lepic_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // generic growable device scratch (sort keys, look-back status, ...)
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+  void *scratch2 = nullptr;
+  size_t scratch2_bytes = 0;
+  // small device control block: counters, flags, partial sums
+  int64_t *d_ctrl = nullptr;          // 4096 int64 words
+  int64_t *h_pinned = nullptr;        // 4096 pinned host words for readbacks
+  uint32_t epoch = 0;                 // look-back epoch (16 bits used)
+  uint64_t *d_status = nullptr;       // look-back status words
+  int64_t status_len = 0;
+  // multigrid level storage
+  void *mg_mem = nullptr;
+  size_t mg_bytes = 0;
+  int32_t mg_n[3] = {0, 0, 0};
+  int32_t mg_ng = 0;
+};
+
+// control block word offsets (int64 units)
+enum : int {
+  CTRL_TILE_COUNTER = 0,   // int32 used
+  CTRL_COUNT_OUT = 8,      // compaction output count
+  CTRL_SPIN_FAIL = 16,     // set when a bounded wait gives up
+  CTRL_MCC = 64,           // MCC counters (256 words)
+  CTRL_MG = 512,           // MG per-level control (12 levels x 16 words)
+  CTRL_PARTIALS = 1024,    // reduction partials (3072 words)
+  CTRL_WORDS = 4096
+};
+
+int lepic_fail(lepic_ctx *ctx, int code, const std::string &msg);
+int lepic_hip_check(lepic_ctx *ctx, hipError_t e, const char *what);
+void *lepic_scratch(lepic_ctx *ctx, size_t bytes, int which);
+uint32_t lepic_next_epoch(lepic_ctx *ctx, int64_t ntiles);
+
+#define LEPIC_HIP(ctx, call)                                    \
+  do {                                                          \
+    hipError_t e__ = (call);                                    \
+    if (e__ != hipSuccess) return lepic_hip_check(ctx, e__, #call); \
+  } while (0)
+
+#define LEPIC_LAUNCH_CHECK(ctx, what)                           \
+  do {                                                          \
+    hipError_t e__ = hipGetLastError();                         \
+    if (e__ != hipSuccess) return lepic_hip_check(ctx, e__, what); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Grid geometry handed to kernels by value.
+// ---------------------------------------------------------------------------
+struct KGrid {
+  int n1, n2, n3;
+  int64_t sy, sz;   // strides of j and k (x stride is 1)
+  double h1, h2, h3;
+};
+
+static inline KGrid make_kgrid(const lepic_grid *g) {
+  KGrid k;
+  k.n1 = g->n[0]; k.n2 = g->n[1]; k.n3 = g->n[2];
+  k.sy = (int64_t)g->n[0] + 3;
+  k.sz = ((int64_t)g->n[0] + 3) * ((int64_t)g->n[1] + 3);
+  k.h1 = g->h[0]; k.h2 = g->h[1]; k.h3 = g->h[2];
+  return k;
+}
+
+static inline int64_t grid_nodes(const lepic_grid *g) {
+  return ((int64_t)g->n[0] + 3) * ((int64_t)g->n[1] + 3) * ((int64_t)g->n[2] + 3);
+}
+
+__host__ __device__ inline int64_t gidx(const KGrid &g, int i, int j, int k) {
+  return (int64_t)i + g.sy * (int64_t)j + g.sz * (int64_t)k;
+}
+
+__device__ inline int imin(int a, int b) { return a < b ? a : b; }
+__device__ inline int imax(int a, int b) { return a > b ? a : b; }
+
+// Trilinear stencil of mod_particleMover.f90:104-127: lower node clamped to
+// [0, n+1], weights w1..w8 over nodes (ix,iy,iz), (ix+1,iy,iz), (ix+1,iy+1,iz),
+// (ix,iy+1,iz) and the same at iz+1.
+struct Stencil {
+  int64_t base;      // node (ix,iy,iz)
+  double w[8];
+};
+
+__device__ inline void make_stencil(const KGrid &g, double xp, double yp, double zp,
+                                    Stencil &s) {
+  int ix = (int)(xp / g.h1) + 1;
+  int iy = (int)(yp / g.h2) + 1;
+  int iz = (int)(zp / g.h3) + 1;
+  ix = imax(0, imin(g.n1 + 1, ix));
+  iy = imax(0, imin(g.n2 + 1, iy));
+  iz = imax(0, imin(g.n3 + 1, iz));
+  double px = ((double)ix * g.h1 - xp) / g.h1;
+  double py = ((double)iy * g.h2 - yp) / g.h2;
+  double pz = ((double)iz * g.h3 - zp) / g.h3;
+  double wx2 = 1.0 - px, wy2 = 1.0 - py, wz2 = 1.0 - pz;
+  s.w[0] = px * py * pz;
+  s.w[1] = wx2 * py * pz;
+  s.w[2] = wx2 * wy2 * pz;
+  s.w[3] = px * wy2 * pz;
+  s.w[4] = px * py * wz2;
+  s.w[5] = wx2 * py * wz2;
+  s.w[6] = wx2 * wy2 * wz2;
+  s.w[7] = px * wy2 * wz2;
+  s.base = gidx(g, ix, iy, iz);
+}
+
+// Gather of a 4-wide node field (E or B map) in the reference's summation order.
+__device__ inline void gather3(const double4 *__restrict__ F, const KGrid &g, const Stencil &s,
+                               double &ox, double &oy, double &oz) {
+  const int64_t b0 = s.base, b1 = s.base + g.sy, b2 = s.base + g.sz, b3 = s.base + g.sz + g.sy;
+  const double4 f1 = F[b0], f2 = F[b0 + 1], f3 = F[b1 + 1], f4 = F[b1];
+  const double4 f5 = F[b2], f6 = F[b2 + 1], f7 = F[b3 + 1], f8 = F[b3];
+  ox = s.w[0] * f1.x + s.w[1] * f2.x + s.w[2] * f3.x + s.w[3] * f4.x + s.w[4] * f5.x +
+       s.w[5] * f6.x + s.w[6] * f7.x + s.w[7] * f8.x;
+  oy = s.w[0] * f1.y + s.w[1] * f2.y + s.w[2] * f3.y + s.w[3] * f4.y + s.w[4] * f5.y +
+       s.w[5] * f6.y + s.w[6] * f7.y + s.w[7] * f8.y;
+  oz = s.w[0] * f1.z + s.w[1] * f2.z + s.w[2] * f3.z + s.w[3] * f4.z + s.w[4] * f5.z +
+       s.w[5] * f6.z + s.w[6] * f7.z + s.w[7] * f8.z;
+}
+
+// Velocity update of part_mover (Src/part_expmover.f90:139-240) given the
+// gathered E; bmode 0 leapfrog, 1 uniform B, 2 map gated at the lower node.
+struct KBField {
+  int mode;
+  int magnetized;
+  double bx, by, bz;
+  const double4 *B4;
+};
+
+__device__ inline void velocity_update(const KGrid &g, const KBField &bf, const Stencil &s,
+                                       double k1, double Ex, double Ey, double Ez, double &vx,
+                                       double &vy, double &vz) {
+  int use_b = 0;
+  double Bx = 0.0, By = 0.0, Bz = 0.0;
+  if (bf.mode == 1) {
+    Bx = bf.bx; By = bf.by; Bz = bf.bz;
+    use_b = 1;
+  } else if (bf.mode == 2) {
+    if (bf.B4[s.base].w > 1.0e-5) {
+      gather3(bf.B4, g, s, Bx, By, Bz);
+      use_b = 1;
+    }
+  }
+  if (!bf.magnetized) use_b = 0;
+  if (use_b) {
+    double Bt = sqrt(Bx * Bx + By * By + Bz * Bz);
+    double k2 = k1 * (2.0 / (1.0 + (k1 * Bt) * (k1 * Bt)));
+    double v1x = vx + k1 * Ex, v1y = vy + k1 * Ey, v1z = vz + k1 * Ez;
+    double v3x = v1x + k1 * (v1y * Bz - v1z * By);
+    double v3y = v1y + k1 * (v1z * Bx - v1x * Bz);
+    double v3z = v1z + k1 * (v1x * By - v1y * Bx);
+    double v2x = v1x + k2 * (v3y * Bz - v3z * By);
+    double v2y = v1y + k2 * (v3z * Bx - v3x * Bz);
+    double v2z = v1z + k2 * (v3x * By - v3y * Bx);
+    vx = v2x + k1 * Ex; vy = v2y + k1 * Ey; vz = v2z + k1 * Ez;
+  } else {
+    vx = vx + 2.0 * k1 * Ex; vy = vy + 2.0 * k1 * Ey; vz = vz + 2.0 * k1 * Ez;
+  }
+}
+
+static inline KBField make_kbfield(const lepic_bfield *bf) {
+  KBField k;
+  k.mode = bf ? bf->mode : 0;
+  k.magnetized = bf ? bf->magnetized : 1;
+  k.bx = bf ? bf->B[0] : 0.0; k.by = bf ? bf->B[1] : 0.0; k.bz = bf ? bf->B[2] : 0.0;
+  k.B4 = bf ? reinterpret_cast<const double4 *>(bf->B4) : nullptr;
+  return k;
+}
+
+static inline unsigned grid_blocks(int64_t n, int block) {
+  int64_t b = (n + block - 1) / block;
+  return (unsigned)(b > 0 ? b : 1);
+}

@@ -1,0 +1,520 @@
+// lepic_mcc.hip -- null-collision Monte Carlo collisions (a10):
+// perform_collisions_step (modules/particles/mod_collisions.f90:214-739).
+//
+// Two modes share one trial routine:
+//  * REPLAY_RAN2 -- the reference's sequential Markov chain (trial order, ran2
+//    consumption, n growing with appended products, immediate kills) run by a
+//    single device thread, for exact parity with the reference;
+//  * PHILOX -- one thread per particle draws k ~ Poisson(Pmax) trials (the
+//    large-N law of Nc = N Pmax picks with replacement) from Philox-4x32-10
+//    keyed by (key, counter, species, particle), products appended with an
+//    atomic slot counter.  Statistically the reference's process, fully
+//    parallel.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "lepic_internal.h"
+#include "lepic_philox.h"
+
+namespace {
+
+constexpr int BLOCK = 256;
+constexpr int MAXR = 16;   // reactions per projectile handled by the kernels
+constexpr double QE_SI = 1.60217646e-19;
+constexpr double PI_F = 3.14159265358979311600;
+
+struct KParts {
+  double *x[LEPIC_MAX_SPECIES], *y[LEPIC_MAX_SPECIES], *z[LEPIC_MAX_SPECIES];
+  double *vx[LEPIC_MAX_SPECIES], *vy[LEPIC_MAX_SPECIES], *vz[LEPIC_MAX_SPECIES];
+  int8_t *dead[LEPIC_MAX_SPECIES];
+  int64_t cap[LEPIC_MAX_SPECIES];
+};
+
+// device counters in the control block (int64 words from CTRL_MCC)
+struct MccCounters {
+  int64_t n[LEPIC_MAX_SPECIES];          // live count per species
+  int64_t accepted[LEPIC_MAX_SPECIES];
+  int64_t err_coll[LEPIC_MAX_SPECIES];
+  int64_t skipped_dead[LEPIC_MAX_SPECIES];
+  int64_t appended[LEPIC_MAX_SPECIES];
+  int64_t overflow;
+  int64_t seed;
+};
+
+struct KChem {
+  lepic_chem c;
+  double np_mx_all[LEPIC_MAX_SPECIES];
+  double nu_max_local[LEPIC_MAX_SPECIES];
+};
+
+struct Ran2Stream {
+  int32_t s;
+  __device__ double uniform() {
+    const int32_t ia = 16807, im = 2147483647, iq = 127773, ir = 2836;
+    const double am = 1.0 / (double)im;
+    int32_t k = s / iq;
+    s = ia * (s - k * iq) - ir * k;
+    if (s < 0) s += im;
+    return am * (double)s;
+  }
+};
+
+__device__ inline void box_muller(double &v1, double &v2, double vt, double r1_in, double r2_in) {
+  // mod_collisions.f90:742-753
+  double r1 = r1_in > 1.0e-14 ? r1_in : 1.0e-14;
+  double fac = vt * sqrt(-log(1.0 - r1));
+  v1 = fac * cos(2.0 * PI_F * r2_in);
+  v2 = fac * sin(2.0 * PI_F * r2_in);
+}
+
+__device__ inline void scatter_vector(double &vx1, double &vy1, double &vz1, double vx, double vy,
+                                      double vz, double costheta, double phi) {
+  // mod_collisions.f90:64-92
+  double s = 1.0 - costheta * costheta;
+  double sintheta = sqrt(s > 0.0 ? s : 0.0);
+  double sinphi = sin(phi), cosphi = cos(phi);
+  double v = sqrt(vx * vx + vy * vy + vz * vz);
+  vx1 = vx; vy1 = vy; vz1 = vz;
+  if (v == 0.0) return;
+  if (fabs(vy) > fabs(vz)) {
+    double vv = sqrt(vx * vx + vy * vy);
+    vx1 = vx * costheta + (vy * v * sinphi + vx * vz * cosphi) / vv * sintheta;
+    vy1 = vy * costheta + (-vx * v * sinphi + vy * vz * cosphi) / vv * sintheta;
+    vz1 = vz * costheta - vv * cosphi * sintheta;
+  } else {
+    double vv = sqrt(vx * vx + vz * vz);
+    vx1 = vx * costheta + (vz * v * sinphi - vy * vx * cosphi) / vv * sintheta;
+    vy1 = vy * costheta + vv * cosphi * sintheta;
+    vz1 = vz * costheta - (vx * v * sinphi + vy * vz * cosphi) / vv * sintheta;
+  }
+}
+
+// First bracket Er[ip-1] < E <= Er[ip] (mod_collisions.f90:512-546 returns
+// this unique bracket on a strictly increasing grid).
+__device__ inline int find_bracket(const double *Er, int npt, double E) {
+  if (npt < 2) return -1;
+  if (!(E > Er[0]) || E > Er[npt - 1]) return -1;
+  int lo = 0, hi = npt - 1;
+  while (hi - lo > 1) {
+    int mid = lo + (hi - lo) / 2;
+    if (E <= Er[mid]) hi = mid;
+    else lo = mid;
+  }
+  return hi;
+}
+
+// Appending a product: the replay chain bumps n in order, the parallel mode
+// takes an atomic slot.
+struct SerialAppend {
+  MccCounters *cnt;
+  __device__ int64_t slot(int bt) { return cnt->n[bt]; }
+  __device__ void commit(int bt, int64_t ib) {
+    if (ib >= cnt->n[bt]) cnt->n[bt] = ib + 1;
+  }
+  __device__ void add(double *a, double v) { *a = *a + v; }
+  __device__ void count(int64_t *a) { *a += 1; }
+};
+
+struct AtomicAppend {
+  MccCounters *cnt;
+  __device__ int64_t slot(int bt) {
+    return (int64_t)atomicAdd((unsigned long long *)&cnt->n[bt], 1ull);
+  }
+  __device__ void commit(int, int64_t) {}
+  __device__ void add(double *a, double v) { atomicAdd(a, v); }
+  __device__ void count(int64_t *a) { atomicAdd((unsigned long long *)a, 1ull); }
+};
+
+// One null-collision trial on particle ip of species pt
+// (mod_collisions.f90:406-726).
+template <class RNG, class APP>
+__device__ void trial(const KGrid &g, const KParts &P, const KChem &K, const double *np_red,
+                      int64_t nodes, int pt, int64_t ip, RNG &rng, APP &app, MccCounters *cnt,
+                      double *Pcoll) {
+  const lepic_chem &c = K.c;
+  const int NT = c.ntype_tracked, NA = c.ntype_total;
+  if (P.dead[pt][ip] != 0) {
+    app.count(&cnt->skipped_dead[pt]);
+    return;
+  }
+  int ix = (int)(P.x[pt][ip] / g.h1) + 1;
+  int iy = (int)(P.y[pt][ip] / g.h2) + 1;
+  int iz = (int)(P.z[pt][ip] / g.h3) + 1;
+  ix = imax(1, imin(g.n1, ix));
+  iy = imax(1, imin(g.n2, iy));
+  iz = imax(1, imin(g.n3, iz));
+  const double vx1 = P.vx[pt][ip], vy1 = P.vy[pt][ip], vz1 = P.vz[pt][ip];
+  const int ncl = c.p_ncol[pt] < MAXR ? c.p_ncol[pt] : MAXR;
+  double nu[MAXR];
+  double sum_nu = 0.0;
+  unsigned tsel = 0, tfound = 0;
+  double svr[LEPIC_MAX_SPECIES], sEkr[LEPIC_MAX_SPECIES], smu[LEPIC_MAX_SPECIES];
+  double scx[LEPIC_MAX_SPECIES], scy[LEPIC_MAX_SPECIES], scz[LEPIC_MAX_SPECIES];
+  for (int a = 0; a < ncl; ++a) nu[a] = 0.0;
+  for (int ic2 = 0; ic2 < ncl; ++ic2) {
+    const int col = c.sig_list[pt * LEPIC_MAX_COL + ic2];
+    const int n_re = c.col_info[col * 10 + 0];
+    const int tt = c.col_info[col * 10 + 1 + n_re];
+    if (tt < 0 || tt >= NA) continue;
+    if (!(tsel & (1u << tt))) {
+      tsel |= 1u << tt;
+      if (tt >= NT) {
+        double vx2, vy2, vz2, dummy;
+        double r1 = rng.uniform(), r2 = rng.uniform();
+        box_muller(vx2, vy2, c.vt0[tt], r1, r2);
+        r1 = rng.uniform(); r2 = rng.uniform();
+        box_muller(vz2, dummy, c.vt0[tt], r1, r2);
+        const double mp = fabs(c.mass[pt]), mt = fabs(c.mass[tt]);
+        smu[tt] = mp * mt / (mp + mt);
+        const double dx = vx1 - vx2, dy = vy1 - vy2, dz = vz1 - vz2;
+        svr[tt] = sqrt(dx * dx + dy * dy + dz * dz);
+        sEkr[tt] = 0.5 * smu[tt] * (svr[tt] * svr[tt]) / QE_SI;
+        scx[tt] = (mp * vx1 + mt * vx2) / (mp + mt);
+        scy[tt] = (mp * vy1 + mt * vy2) / (mp + mt);
+        scz[tt] = (mp * vz1 + mt * vz2) / (mp + mt);
+        tfound |= 1u << tt;
+      } else {
+        // charged target: the cell tables are cleared before MCC runs
+        // (mod_particleMover.f90:153-155, mod_particleBC.f90:255-257), so the
+        // neighbour-cell pick always fails (mod_collisions.f90:473-478)
+        app.count(&cnt->err_coll[pt]);
+      }
+    }
+    if (!(tfound & (1u << tt))) continue;
+    const double vr = svr[tt], Ekr = sEkr[tt];
+    const int ipt = find_bracket(c.sig_Er, c.npt, Ekr);
+    if (ipt > 0) {
+      const double EkL = c.sig_Er[ipt - 1], EkR = c.sig_Er[ipt];
+      const double sL = c.sig[(int64_t)col * c.npt + ipt - 1];
+      const double sR = c.sig[(int64_t)col * c.npt + ipt];
+      const double sig_p = sL + (Ekr - EkL) * (sR - sL) / (EkR - EkL);
+      double np_t;
+      if (tt < NT) {
+        // density_average_at_particle_cell on a 1-based dummy: nodes ix-1..ix
+        const double *d = np_red + (int64_t)tt * nodes;
+        np_t = 0.125 * (d[gidx(g, ix - 1, iy - 1, iz - 1)] + d[gidx(g, ix, iy - 1, iz - 1)] +
+                        d[gidx(g, ix - 1, iy, iz - 1)] + d[gidx(g, ix, iy, iz - 1)] +
+                        d[gidx(g, ix - 1, iy - 1, iz)] + d[gidx(g, ix, iy - 1, iz)] +
+                        d[gidx(g, ix - 1, iy, iz)] + d[gidx(g, ix, iy, iz)]);
+      } else {
+        np_t = K.np_mx_all[tt];
+      }
+      nu[ic2] = np_t * sig_p * vr / K.nu_max_local[pt];
+      sum_nu = sum_nu + nu[ic2];
+    }
+  }
+  // indexx_small: ascending exchange sort (mod_collisions.f90:756-776)
+  int indx[MAXR];
+  for (int a = 0; a < ncl; ++a) indx[a] = a;
+  for (int a = 0; a < ncl - 1; ++a)
+    for (int b = a + 1; b < ncl; ++b)
+      if (nu[indx[b]] < nu[indx[a]]) {
+        int t = indx[a]; indx[a] = indx[b]; indx[b] = t;
+      }
+  const double rnd = rng.uniform();
+  int c_ind = -1;
+  if (rnd <= sum_nu) {
+    double s = 0.0;
+    for (int a = 0; a < ncl; ++a) {
+      s = s + nu[indx[a]];
+      if (rnd <= s) {
+        c_ind = c.sig_list[pt * LEPIC_MAX_COL + indx[a]];
+        break;
+      }
+    }
+  }
+  if (c_ind < 0) return;
+  app.count(&cnt->accepted[pt]);
+  const int n_re = c.col_info[c_ind * 10 + 0];
+  const int n_by = c.col_info[c_ind * 10 + 1];
+  const int tt = c.col_info[c_ind * 10 + 1 + n_re];
+  if (tt < 0 || tt >= NA) return;
+  if (!(tfound & (1u << tt))) return;
+  const double mu = smu[tt], vr = svr[tt];
+  const double vxc = scx[tt], vyc = scy[tt], vzc = scz[tt];
+  const double Eth = c.sig_Eth[c_ind];
+  const double Ee = 0.5 * mu * vr * vr - Eth * QE_SI;
+  if (Ee < 0.0) return;
+  double smi = 0.0;
+  for (int ib = 0; ib < n_by; ++ib) {
+    const int bt = c.col_info[c_ind * 10 + 2 + n_re + ib];
+    if (bt >= 0 && bt < NA && c.mass[bt] > 0.0) smi = smi + 1.0 / fabs(c.mass[bt]);
+  }
+  if (smi <= 0.0) return;
+  const double costh = 1.0 - 2.0 * rng.uniform();
+  const double phi = 2.0 * PI_F * rng.uniform();
+  const double costh_s = 1.0 - 2.0 * rng.uniform();
+  const double phi_s = 2.0 * PI_F * rng.uniform();
+  const double th_add = 2.0 * PI_F / (double)(n_by > 1 ? n_by : 1);
+  const double x0 = P.x[pt][ip], y0 = P.y[pt][ip], z0 = P.z[pt][ip];
+  int flag_add[10];
+  for (int a = 0; a < 10; ++a) flag_add[a] = a < n_re ? -1 : 0;
+  for (int ib = 0; ib < n_by; ++ib) {
+    const int bt = c.col_info[c_ind * 10 + 2 + n_re + ib];
+    if (bt < 0 || bt >= NT) continue;
+    if (c.mass[bt] <= 0.0) continue;
+    const double s2 = 1.0 - costh * costh;
+    const double st = sqrt(0.0 > s2 ? 0.0 : s2);
+    const double ex = costh;
+    const double ey = st * sin(phi + th_add * (double)ib);
+    const double ez = st * cos(phi + th_add * (double)ib);
+    double ex1, ey1, ez1;
+    scatter_vector(ex1, ey1, ez1, ex, ey, ez, costh_s, phi_s);
+    const double mb = fabs(c.mass[bt]);
+    const double vp = sqrt(2.0 * (Ee / smi) / (mb * mb));
+    int64_t ibx;
+    bool fresh;
+    if (ib == 0 && bt == pt) {
+      ibx = ip;
+      flag_add[0] = 0;
+      fresh = false;
+    } else if (bt == tt && tt < NT) {
+      flag_add[1] = 0;
+      continue;  // target index of a failed charged pick (never reached)
+    } else {
+      ibx = app.slot(bt);
+      flag_add[n_re + ib] = 1;
+      fresh = true;
+    }
+    if (ibx >= P.cap[bt]) {
+      atomicExch((unsigned long long *)&cnt->overflow, 1ull);
+      continue;
+    }
+    double v2old = 0.0;
+    if (!fresh)
+      v2old = P.vx[bt][ibx] * P.vx[bt][ibx] + P.vy[bt][ibx] * P.vy[bt][ibx] +
+              P.vz[bt][ibx] * P.vz[bt][ibx];
+    app.commit(bt, ibx);
+    P.x[bt][ibx] = x0; P.y[bt][ibx] = y0; P.z[bt][ibx] = z0;
+    const double nvx = vxc + vp * ex1, nvy = vyc + vp * ey1, nvz = vzc + vp * ez1;
+    P.vx[bt][ibx] = nvx; P.vy[bt][ibx] = nvy; P.vz[bt][ibx] = nvz;
+    P.dead[bt][ibx] = 0;
+    if (fresh) app.count(&cnt->appended[bt]);
+    const double v2new = nvx * nvx + nvy * nvy + nvz * nvz;
+    app.add(&Pcoll[bt], 0.5 * c.Nm[bt] * c.mass[bt] * (v2new - v2old));
+  }
+  for (int ir = 0; ir < n_re; ++ir) {
+    const int ct = c.col_info[c_ind * 10 + 2 + ir];
+    if (ct < 0 || ct >= NT) continue;
+    if (c.mass[ct] <= 0.0) continue;
+    if (flag_add[ir] == -1 && ct == pt) P.dead[pt][ip] = 1;
+  }
+}
+
+// REPLAY_RAN2: the whole chain on one device thread (trial order and ran2
+// consumption of mod_collisions.f90:391-728 for one virtual thread).
+__global__ void k_mcc_replay(KGrid g, KParts P, KChem K, const double *np_red, int64_t nodes,
+                             int32_t Nc0, int32_t Nc1, int32_t Nc2, int32_t Nc3, MccCounters *cnt,
+                             double *Pcoll) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  Ran2Stream rng;
+  rng.s = (int32_t)cnt->seed;
+  SerialAppend app{cnt};
+  const int32_t Ncs[4] = {Nc0, Nc1, Nc2, Nc3};
+  const lepic_chem &c = K.c;
+  for (int pt = 0; pt < c.ntype_tracked && pt < 4; ++pt) {
+    if (c.p_ncol[pt] == 0 || c.mass[pt] <= 0.0) continue;
+    if (Ncs[pt] == 0) continue;
+    if (cnt->n[pt] == 0) continue;
+    if (K.nu_max_local[pt] <= 0.0) continue;
+    for (int32_t ic = 0; ic < Ncs[pt]; ++ic) {
+      const double rnd = rng.uniform();
+      const int64_t n = cnt->n[pt];
+      int64_t ip = (int64_t)((double)n * rnd) + 1;
+      if (ip > n) ip = n;
+      trial(g, P, K, np_red, nodes, pt, ip - 1, rng, app, cnt, Pcoll);
+    }
+  }
+  cnt->seed = rng.s;
+}
+
+// PHILOX: one thread per particle of species pt, k ~ Poisson(lambda) trials.
+__global__ __launch_bounds__(BLOCK) void k_mcc_philox(KGrid g, KParts P, KChem K,
+                                                      const double *np_red, int64_t nodes, int pt,
+                                                      int64_t n0, double lambda, uint64_t key,
+                                                      uint64_t counter, MccCounters *cnt,
+                                                      double *Pcoll) {
+  int64_t ip = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+  if (ip >= n0) return;
+  PhiloxStream rng;
+  rng.init(key ^ ((uint64_t)pt << 56), (uint32_t)ip, (uint32_t)(ip >> 32), (uint32_t)counter);
+  rng.c3 ^= (uint32_t)(counter >> 32) * 0x9E3779B9u;
+  // inversion sampling of Poisson(lambda), lambda <= 1
+  double u = rng.uniform();
+  double p = exp(-lambda), cdf = p;
+  int k = 0;
+  while (u > cdf && k < 32) {
+    ++k;
+    p = p * lambda / (double)k;
+    cdf = cdf + p;
+  }
+  AtomicAppend app{cnt};
+  for (int t = 0; t < k; ++t) trial(g, P, K, np_red, nodes, pt, ip, rng, app, cnt, Pcoll);
+}
+
+// np_mx of the reference: max over stored nodes 0..n of each dimension
+// (maxval(np_red(1:n+1,...)) through a 1-based dummy, mod_collisions.f90:293).
+__global__ __launch_bounds__(BLOCK) void k_npmax(KGrid g, const double *np, int64_t nodes,
+                                                 double *partials) {
+  __shared__ double red[BLOCK];
+  int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+  int64_t nx = g.n1 + 1, ny = g.n2 + 1, nz = g.n3 + 1;
+  double m = -INFINITY;
+  if (t < nx * ny * nz) {
+    int i = (int)(t % nx);
+    int64_t r = t / nx;
+    int j = (int)(r % ny), k = (int)(r / ny);
+    m = np[gidx(g, i, j, k)];
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = BLOCK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+}  // namespace
+
+extern "C" int lepic_collisions(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *parts,
+                                const double *np_red, const lepic_chem *chem, int32_t ns_coll,
+                                double dt, int32_t mode, int32_t *iseed, uint64_t philox_key,
+                                uint64_t philox_counter, double *Pcoll, lepic_mcc_stats *stats) {
+  if (!ctx) return LEPIC_ERR_ARG;
+  if (!g || !parts || !np_red || !chem || !Pcoll || (mode == LEPIC_MCC_REPLAY_RAN2 && !iseed))
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions: null argument");
+  const lepic_chem &c = *chem;
+  const int NT = c.ntype_tracked, NA = c.ntype_total;
+  if (NT < 1 || NT > 4 || NA < NT || NA > LEPIC_MAX_SPECIES)
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions: species counts out of range");
+  for (int t = 0; t < NA; ++t)
+    if (c.p_ncol[t] > MAXR)
+      return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions: more than 16 reactions per projectile");
+  if (!c.sig_list || !c.sig_Er || !c.sig || !c.sig_Eth || !c.col_info || !c.sigv_mx)
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions: null reaction table");
+  KGrid kg = make_kgrid(g);
+  const int64_t nodes = grid_nodes(g);
+  KChem K;
+  std::memset(&K, 0, sizeof(K));
+  K.c = c;
+  // np_mx for tracked species (device max), neutral densities for the rest
+  {
+    int64_t cnt = (int64_t)(g->n[0] + 1) * (g->n[1] + 1) * (g->n[2] + 1);
+    int64_t nb = (cnt + BLOCK - 1) / BLOCK;
+    double *part = reinterpret_cast<double *>(lepic_scratch(ctx, sizeof(double) * (size_t)(nb * NT), 1));
+    if (!part) return lepic_fail(ctx, LEPIC_ERR_HIP, "collisions: scratch");
+    for (int t = 0; t < NT; ++t)
+      hipLaunchKernelGGL(k_npmax, dim3((unsigned)nb), dim3(BLOCK), 0, ctx->stream, kg,
+                         np_red + (int64_t)t * nodes, nodes, part + t * nb);
+    std::vector<double> h((size_t)(nb * NT));
+    LEPIC_HIP(ctx, hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+    LEPIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    for (int t = 0; t < NT; ++t) {
+      double m = -INFINITY;
+      for (int64_t b = 0; b < nb; ++b) m = std::max(m, h[(size_t)(t * nb + b)]);
+      K.np_mx_all[t] = m;
+    }
+    for (int t = NT; t < NA; ++t) K.np_mx_all[t] = c.neutral_density[t - NT];
+  }
+  // nu_max, Pmax, Nc (mod_collisions.f90:317-372); sigv_mx is a small device table
+  std::vector<int32_t> hsl((size_t)NA * LEPIC_MAX_COL), hci((size_t)c.ncol * 10);
+  std::vector<double> hsv((size_t)NA * c.ncol);
+  LEPIC_HIP(ctx, hipMemcpyAsync(hsl.data(), c.sig_list, sizeof(int32_t) * hsl.size(),
+                                hipMemcpyDeviceToHost, ctx->stream));
+  LEPIC_HIP(ctx, hipMemcpyAsync(hci.data(), c.col_info, sizeof(int32_t) * hci.size(),
+                                hipMemcpyDeviceToHost, ctx->stream));
+  LEPIC_HIP(ctx, hipMemcpyAsync(hsv.data(), c.sigv_mx, sizeof(double) * hsv.size(),
+                                hipMemcpyDeviceToHost, ctx->stream));
+  LEPIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  // validate the tables before any kernel indexes them
+  for (int t = 0; t < NA; ++t)
+    for (int ic = 0; ic < c.p_ncol[t]; ++ic) {
+      int col = hsl[(size_t)t * LEPIC_MAX_COL + ic];
+      if (col < 0 || col >= c.ncol)
+        return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions: sig_list entry out of range");
+    }
+  for (int col = 0; col < c.ncol; ++col) {
+    int n_re = hci[(size_t)col * 10], n_by = hci[(size_t)col * 10 + 1];
+    if (n_re < 1 || n_by < 0 || n_re + n_by > 8)
+      return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions: col_info reaction shape out of range");
+  }
+  if (c.npt < 2) return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions: energy grid too short");
+  int32_t Nc[4] = {0, 0, 0, 0};
+  double lambda[4] = {0, 0, 0, 0};
+  for (int t = 0; t < NT; ++t) {
+    if (c.p_ncol[t] == 0 || c.mass[t] <= 0.0) continue;
+    double nu_max = 0.0;
+    for (int ic = 0; ic < c.p_ncol[t]; ++ic) {
+      int col = hsl[(size_t)t * LEPIC_MAX_COL + ic];
+      int n_re = hci[(size_t)col * 10];
+      int tt = hci[(size_t)col * 10 + 1 + n_re];
+      if (tt < 0 || tt >= NA) continue;
+      nu_max = nu_max + K.np_mx_all[tt] * hsv[(size_t)t * c.ncol + col];
+    }
+    if (nu_max > c.nu_uplim[t]) nu_max = c.nu_uplim[t];
+    double Pmax = nu_max * (double)ns_coll * dt;
+    if (Pmax > 1.0) return lepic_fail(ctx, LEPIC_ERR_PMAX, "collisions: Pmax > 1");
+    if (mode == LEPIC_MCC_REPLAY_RAN2) {
+      double dNc = (double)parts[t].n * Pmax / 1.0;
+      int32_t nct = (int32_t)dNc;
+      double rnd = lepic_ran2(&iseed[0]);
+      if (rnd <= (dNc - (double)nct)) nct = nct + 1;
+      Nc[t] = nct;
+      int64_t nn = parts[t].n > 1 ? parts[t].n : 1;
+      K.nu_max_local[t] = (double)nct / (double)nn / ((double)ns_coll * dt);
+    } else {
+      lambda[t] = Pmax;
+      Nc[t] = (int32_t)std::llround((double)parts[t].n * Pmax);
+      K.nu_max_local[t] = Pmax / ((double)ns_coll * dt);
+    }
+  }
+  KParts P;
+  std::memset(&P, 0, sizeof(P));
+  for (int t = 0; t < NT; ++t) {
+    P.x[t] = parts[t].x; P.y[t] = parts[t].y; P.z[t] = parts[t].z;
+    P.vx[t] = parts[t].vx; P.vy[t] = parts[t].vy; P.vz[t] = parts[t].vz;
+    P.dead[t] = parts[t].dead;
+    P.cap[t] = parts[t].cap;
+  }
+  MccCounters hc;
+  std::memset(&hc, 0, sizeof(hc));
+  for (int t = 0; t < NT; ++t) hc.n[t] = parts[t].n;
+  hc.seed = iseed ? iseed[0] : 0;
+  MccCounters *dc = reinterpret_cast<MccCounters *>(ctx->d_ctrl + CTRL_MCC);
+  static_assert(sizeof(MccCounters) <= 256 * sizeof(int64_t), "MCC counters fit the control block");
+  LEPIC_HIP(ctx, hipMemcpyAsync(dc, &hc, sizeof(hc), hipMemcpyHostToDevice, ctx->stream));
+  if (mode == LEPIC_MCC_REPLAY_RAN2) {
+    hipLaunchKernelGGL(k_mcc_replay, dim3(1), dim3(64), 0, ctx->stream, kg, P, K, np_red, nodes,
+                       Nc[0], Nc[1], Nc[2], Nc[3], dc, Pcoll);
+    LEPIC_LAUNCH_CHECK(ctx, "k_mcc_replay");
+  } else if (mode == LEPIC_MCC_PHILOX) {
+    for (int t = 0; t < NT; ++t) {
+      if (c.p_ncol[t] == 0 || c.mass[t] <= 0.0 || lambda[t] <= 0.0 || parts[t].n == 0) continue;
+      hipLaunchKernelGGL(k_mcc_philox, dim3(grid_blocks(parts[t].n, BLOCK)), dim3(BLOCK), 0,
+                         ctx->stream, kg, P, K, np_red, nodes, t, parts[t].n, lambda[t],
+                         philox_key, philox_counter, dc, Pcoll);
+      LEPIC_LAUNCH_CHECK(ctx, "k_mcc_philox");
+    }
+  } else {
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions: unknown mode");
+  }
+  LEPIC_HIP(ctx, hipMemcpyAsync(&hc, dc, sizeof(hc), hipMemcpyDeviceToHost, ctx->stream));
+  LEPIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int t = 0; t < NT; ++t) parts[t].n = hc.n[t] < parts[t].cap ? hc.n[t] : parts[t].cap;
+  if (mode == LEPIC_MCC_REPLAY_RAN2) iseed[0] = (int32_t)hc.seed;
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    for (int t = 0; t < NT; ++t) {
+      stats->Nc[t] = Nc[t];
+      stats->accepted[t] = hc.accepted[t];
+      stats->err_coll[t] = hc.err_coll[t];
+      stats->skipped_dead[t] = hc.skipped_dead[t];
+      stats->appended[t] = hc.appended[t];
+    }
+  }
+  if (hc.overflow) return lepic_fail(ctx, LEPIC_ERR_CAPACITY, "collisions: product capacity exceeded");
+  return LEPIC_OK;
+}

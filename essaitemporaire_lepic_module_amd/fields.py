@@ -1,0 +1,88 @@
+"""Grid side of the step -- the call surface of modules/fields and the legacy
+Poisson solver.
+
+Grid arrays are FP64 device tensors of shape (n3+3, n2+3, n1+3) over the
+reference's 0:n+2 node range (x fastest); E is (n3+3, n2+3, n1+3, 4) with
+components (Ex, Ey, Ez, pad); bcnd is int32 of the node shape; np_thread and
+np_red are (ntype, n3+3, n2+3, n1+3).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _native
+from .device import context, ptr
+from .particles import ParticleSet
+
+EPS0 = 8.854187817e-12
+
+
+def build_kq(bcnd: torch.Tensor, n, kq: torch.Tensor | None = None) -> torch.Tensor:
+    """kq = 2 where bcnd /= -1, else 1 (mod_charge_weights.f90:9-33)."""
+    ctx = context(bcnd.device)
+    kq = torch.empty(bcnd.shape, dtype=torch.float64, device=bcnd.device) if kq is None else kq
+    g = _native.grid(n, (1.0, 1.0, 1.0))
+    ctx.check(ctx.lib.lepic_build_kq(ctx.h, C.byref(g), ptr(bcnd), ptr(kq)))
+    return kq
+
+
+def clear_np_thread(np_thread: torch.Tensor):
+    """np_thread = 0 (mod_chargeDeposition.f90:16-22)."""
+    ctx = context(np_thread.device)
+    ctx.check(ctx.lib.lepic_fill(ctx.h, ptr(np_thread), np_thread.numel(), 0.0))
+
+
+def deposit_particle_set_to_np_thread(part: ParticleSet, n, h, kq: torch.Tensor, Nm_species: float,
+                                      np_local: torch.Tensor):
+    """Trilinear charge deposition (mod_chargeDeposition.f90:25-133)."""
+    ctx = context(part.device)
+    g = _native.grid(n, h)
+    p = part.pset()
+    ctx.check(ctx.lib.lepic_deposit(ctx.h, C.byref(g), C.byref(p), ptr(kq), float(Nm_species),
+                                    ptr(np_local)))
+
+
+def reduce_species_density(n, bcnd: torch.Tensor, np_thread: torch.Tensor, ntype: int,
+                           np_red: torch.Tensor):
+    """Periodic stitching (in place on np_thread) and the sum over the one
+    virtual thread onto nodes 1..n+1 (mod_density.f90:31-73)."""
+    ctx = context(np_thread.device)
+    g = _native.grid(n, (1.0, 1.0, 1.0))
+    ctx.check(ctx.lib.lepic_reduce_density(ctx.h, C.byref(g), ptr(bcnd), int(ntype), ptr(np_thread),
+                                           ptr(np_red)))
+
+
+def build_rho_from_np(n, np_red: torch.Tensor, charge, ntype: int, rho: torch.Tensor):
+    """rho = -sum_s q_s np_s on nodes 1..n+1 (mod_density.f90:76-107)."""
+    ctx = context(np_red.device)
+    g = _native.grid(n, (1.0, 1.0, 1.0))
+    q = (C.c_double * _native.MAX_SPECIES)(*[float(v) for v in charge])
+    ctx.check(ctx.lib.lepic_build_rho(ctx.h, C.byref(g), int(ntype), ptr(np_red), q, ptr(rho)))
+
+
+def solve_poisson_legacy(phi: torch.Tensor, bcnd: torch.Tensor, rhs: torch.Tensor, h, n,
+                         ncycl: int, eps: float, omega: float, ng: int, flag_pbc: int,
+                         flag_nmn: int, eps0: float = EPS0) -> dict:
+    """V-cycle multigrid / red-black SOR, phi updated in place from its
+    current value (mod_PoissonSolver_legacy.f90:27-101 -> pdesolver.f90).
+    Returns the V-cycle count, equivalent sweeps and final L1/alpha residual."""
+    ctx = context(phi.device)
+    g = _native.grid(n, h)
+    mp = _native.MGParams()
+    mp.flag_pbc, mp.flag_nmn, mp.eps0 = int(flag_pbc), int(flag_nmn), float(eps0)
+    mp.ncycl, mp.eps, mp.omega, mp.ng = int(ncycl), float(eps), float(omega), int(ng)
+    st = _native.MGStats()
+    ctx.check(ctx.lib.lepic_poisson_solve(ctx.h, C.byref(g), ptr(phi), ptr(rhs), ptr(bcnd),
+                                          C.byref(mp), C.byref(st)))
+    return {"cycles": st.cycles, "ktot": st.ktot, "res": st.res,
+            "sweeps": [st.sweeps[i] for i in range(int(ng))]}
+
+
+def calc_Efield_modular(n, h, phi: torch.Tensor, E4: torch.Tensor, bcnd: torch.Tensor):
+    """E = -grad(phi) with the reference's wall extrapolation and periodic
+    ghost copies (mod_electricField.f90:9-164)."""
+    ctx = context(phi.device)
+    g = _native.grid(n, h)
+    ctx.check(ctx.lib.lepic_efield(ctx.h, C.byref(g), ptr(phi), ptr(bcnd), ptr(E4)))

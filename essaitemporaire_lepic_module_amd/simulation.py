@@ -1,0 +1,190 @@
+"""Simulation%advance_one_step (modules/simulation/mod_simulation.f90:340-472)
+on one GPU: every phase is a liblepic_hip.so launch on torch's current stream.
+
+Step order (the modular driver, with the legacy Boris rotation in the
+velocity update when B is on):
+  sort (istep mod nb_step_sort == 1) -> reduce density -> rho -> Poisson ->
+  E -> push -> wall BC -> MCC (istep mod nb_step_collisions == 0) -> deposit
+  -> reduce density.
+With fused=True the push, the wall BC and (on steps without MCC) the
+deposit run as one kernel per species.
+"""
+from __future__ import annotations
+
+import contextlib
+
+import torch
+
+from . import _native
+from . import fields as F
+from . import particles as P
+from .config import Case
+from .device import as_device, node_shape
+
+
+class Simulation:
+    def __init__(self, case: Case, device=None, fused: bool = True, loader: str = "host",
+                 mcc_mode: int = _native.MCC_REPLAY_RAN2, philox_key: int = 0x5EED5EED,
+                 capacity_factor: float = 1.3, parts=None, np_thread=None):
+        self.case = case
+        self.dev = torch.device(device if device is not None else "cuda:0")
+        self.fused = fused
+        self.mcc_mode = mcc_mode
+        self.philox_key = philox_key
+        n, h = case.n, case.h
+        self.n, self.h = n, h
+        dev = self.dev
+        self.bcnd = as_device(case.bcnd, torch.int32, dev)
+        self.kq = F.build_kq(self.bcnd, n)
+        self.phi = as_device(case.phi0(), torch.float64, dev)
+        shape = node_shape(n)
+        self.rho = torch.zeros(shape, dtype=torch.float64, device=dev)
+        self.E4 = torch.zeros(shape + (4,), dtype=torch.float64, device=dev)
+        tr = case.tracked
+        self.ntype = len(tr)
+        self.charge = [s.charge for s in tr]
+        self.mass = [s.mass for s in tr]
+        self.Nm = case.Nm[: self.ntype]
+        self.np_red = torch.zeros((self.ntype,) + shape, dtype=torch.float64, device=dev)
+        self.iseed = [P.seed_initialization(1, 0)]
+        ncells = n[0] * n[1] * n[2]
+        cap = int(capacity_factor * case.np_cell * ncells) + 1024
+        if parts is not None:
+            self.parts, self.np_thread = parts, np_thread
+        elif loader == "host":
+            self.parts, self.np_thread = P.load_particles_modular(
+                n, h, case.bcnd, case.kq(), case.np_cell, case.vt0[: self.ntype], self.Nm,
+                [1.0] * self.ntype, self.iseed, cap=cap, device=dev)
+        elif loader == "device":
+            self.parts = P.load_uniform_device(n, h, self.bcnd, case.np_cell, case.vt0[: self.ntype],
+                                               philox_key, cap, dev)
+            self.np_thread = torch.zeros((self.ntype,) + shape, dtype=torch.float64, device=dev)
+            for s, p in enumerate(self.parts):
+                F.deposit_particle_set_to_np_thread(p, n, h, self.kq, self.Nm[s], self.np_thread[s])
+        else:
+            raise ValueError(f"unknown loader {loader!r}")
+        F.reduce_species_density(n, self.bcnd, self.np_thread, self.ntype, self.np_red)
+        self.p_mac = torch.zeros((self.ntype, case.ngrid + 1, 2), dtype=torch.float64, device=dev)
+        self.P_loss = torch.zeros(self.ntype, dtype=torch.float64, device=dev)
+        self.tables = None
+        if case.chemistry is not None:
+            self.Pcoll = torch.zeros(case.chemistry["ntype_total"], dtype=torch.float64, device=dev)
+            self.tables = P.ReactionTables(case.chemistry, case.Nm, dev)
+        self.tag_neg = -1
+        for i, q in enumerate(self.charge):
+            if q < 0.0 and i != 0:
+                self.tag_neg = i + 1
+                break
+        self.bf = []
+        for q in self.charge:
+            if case.B_map is not None:
+                self.B4 = as_device(case.B_map, torch.float64, dev)
+                self.bf.append(P.bfield(2, B4=self.B4, magnetized=(q <= 0 or case.ions_magnetized)))
+            elif case.B_uniform is not None:
+                self.bf.append(P.bfield(1, case.B_uniform, magnetized=(q <= 0 or case.ions_magnetized)))
+            else:
+                self.bf.append(P.bfield(0))
+        self.bp = [P.bc_params(n, h, case.flag_pbc, case.flag_nmn, s + 1, self.tag_neg,
+                               self.Nm[s] * self.charge[s], self.mass[s], self.Nm[s], case.ngrid)
+                   for s in range(self.ntype)]
+        self.last_mg = None
+        self.last_mcc = None
+        self.lost = [0] * self.ntype
+        self.timers = None   # {phase: [(start_event, end_event, units), ...]} when timing
+
+    @contextlib.contextmanager
+    def _phase(self, name: str, units: int = 0):
+        """Record HIP events around a phase on torch's current stream (the
+        stream every lepic call is bound to)."""
+        if self.timers is None:
+            yield
+            return
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        yield
+        b.record()
+        self.timers.setdefault(name, []).append((a, b, units))
+
+    def phase_times(self) -> dict:
+        """{phase: (total_ms, launches, units)} of the recorded events."""
+        torch.cuda.synchronize()
+        out = {}
+        for k, lst in (self.timers or {}).items():
+            out[k] = (sum(a.elapsed_time(b) for a, b, _ in lst), len(lst), sum(u for _, _, u in lst))
+        return out
+
+    # ------------------------------------------------------------------
+    def ensure_headroom(self, factor: float = 0.3):
+        for p in self.parts:
+            if p.n + int(factor * p.n) + 1024 > p.nmax:
+                p.ensure_capacity(p.n + int(factor * p.n) + 1024)
+
+    def advance_one_step(self, istep: int):
+        c = self.case
+        n, h = self.n, self.h
+        if istep % c.nb_step_sort == 1:
+            with self._phase("sort"):
+                for p in self.parts:
+                    if p.n > 1:
+                        P.sort_particles_by_cell(p, n, h, cell_tables=False)
+        with self._phase("rho"):
+            F.reduce_species_density(n, self.bcnd, self.np_thread, self.ntype, self.np_red)
+            F.build_rho_from_np(n, self.np_red, self.charge, self.ntype, self.rho)
+        with self._phase("poisson"):
+            self.last_mg = F.solve_poisson_legacy(self.phi, self.bcnd, self.rho, h, n, c.ncycl,
+                                                  c.eps, c.omega, c.ng, c.flag_pbc, c.flag_nmn)
+        with self._phase("efield"):
+            F.calc_Efield_modular(n, h, self.phi, self.E4, self.bcnd)
+        collide = (self.tables is not None and c.nb_step_collisions > 0
+                   and istep % c.nb_step_collisions == 0)
+        if self.fused:
+            F.clear_np_thread(self.np_thread)
+            for s, p in enumerate(self.parts):
+                with self._phase("push_bc_deposit" if not collide else "push_bc", p.n):
+                    self.lost[s] = P.push_bc_deposit(
+                        p, n, h, self.E4, self.bf[s], self.charge[s], self.mass[s], c.dt,
+                        self.bcnd, self.bp[s], self.kq, None if collide else self.np_thread[s],
+                        self.p_mac[s], self.P_loss[s:s + 1])
+        else:
+            with self._phase("push"):
+                for s, p in enumerate(self.parts):
+                    if self.bf[s].mode == 0:
+                        P.move_particles_electrostatic(p, n, h, self.E4, self.charge[s],
+                                                       self.mass[s], c.dt)
+                    else:
+                        P.move_particles_boris(p, n, h, self.E4, self.bf[s], self.charge[s],
+                                               self.mass[s], c.dt)
+            with self._phase("bc"):
+                for s, p in enumerate(self.parts):
+                    self.lost[s] = P.apply_particle_bc_legacy(
+                        p, n, h, self.bcnd, c.flag_pbc, c.flag_nmn, s + 1, self.tag_neg, 0,
+                        self.Nm[s] * self.charge[s], self.p_mac[s], self.mass[s],
+                        self.P_loss[s:s + 1], self.Nm[s])
+        if collide:
+            self.ensure_headroom()
+            with self._phase("mcc"):
+                self.last_mcc = P.perform_collisions_step(
+                    self.parts, n, h, self.np_red, self.tables, c.nb_step_collisions, c.dt,
+                    self.iseed, self.Pcoll, mode=self.mcc_mode, philox_key=self.philox_key,
+                    philox_counter=istep)
+        if collide or not self.fused:
+            with self._phase("deposit"):
+                F.clear_np_thread(self.np_thread)
+                for s, p in enumerate(self.parts):
+                    F.deposit_particle_set_to_np_thread(p, n, h, self.kq, self.Nm[s],
+                                                        self.np_thread[s])
+        with self._phase("rho"):
+            F.reduce_species_density(n, self.bcnd, self.np_thread, self.ntype, self.np_red)
+
+    def run(self, nsteps: int, start: int = 1):
+        for istep in range(start, start + nsteps):
+            self.advance_one_step(istep)
+
+    # ------------------------------------------------------------------
+    def particle_count(self) -> int:
+        return sum(p.n for p in self.parts)
+
+    def total_charge(self) -> float:
+        """sum_s q_s Nm_s N_s."""
+        return float(sum(q * nm * p.n for q, nm, p in zip(self.charge, self.Nm, self.parts)))

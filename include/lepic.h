@@ -1,0 +1,240 @@
+/*
+ * lepic.h -- C ABI of the MI355X (gfx950) PIC-MCC step library, liblepic_hip.so.
+ *
+ * Plain C: device pointers, sizes and small POD structs; no torch or HIP types
+ * in the signatures (streams travel as void*).  Every entry point returns an
+ * int status (LEPIC_OK or an LEPIC_ERR_* code, message in lepic_last_error())
+ * and never exits the process.  Calls are enqueued on the context's stream;
+ * the few that must return a count to the host (apply_bc, the fused step,
+ * poisson_solve, collisions) synchronise that stream before returning.
+ *
+ * Each entry point replaces one routine of the reference (Fortran, cited as
+ * path:line relative to the reference repository).  The reference has no FFI
+ * of its own; the Python package essaitemporaire_lepic_module_amd binds these
+ * symbols with ctypes under the reference's module/procedure names.
+ *
+ * Index conventions (identical to the reference's index space, 0-based here):
+ *  - grid n = (n1,n2,n3) cells, nodes 0..n+2 per dimension (Fortran 0:n+2),
+ *    node (i,j,k) at ((i-1)h1,(j-1)h2,(k-1)h3); arrays are x-fastest with
+ *    idx = i + (n1+3)*(j + (n2+3)*k).  b/rho use the same strides.
+ *  - cell id c = (ix-1) + n1*((iy-1) + n2*(iz-1)), ix in [1,n1] (the
+ *    reference stores c+1).  Particle indices are 0-based.
+ *  - E fields are 4 doubles per node (Ex,Ey,Ez,pad); B maps are 4 doubles per
+ *    node (Bx,By,Bz,|B|) like the reference's Bi(4,...).
+ *  - bcnd labels: -1 plasma, 0 periodic, -2 Neumann, >= 1 electrode label.
+ */
+#ifndef LEPIC_H
+#define LEPIC_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  LEPIC_OK = 0,
+  LEPIC_ERR_ARG = 1,       /* bad argument / unsupported shape          */
+  LEPIC_ERR_CAPACITY = 2,  /* destination too small; see lepic_last_error */
+  LEPIC_ERR_DIVERGED = 3,  /* Poisson res/res_s > 10 (pdesolver.f90:103)  */
+  LEPIC_ERR_PMAX = 4,      /* MCC Pmax > 1 (mod_collisions.f90:354)        */
+  LEPIC_ERR_HIP = 5,       /* HIP runtime error                           */
+  LEPIC_ERR_SPIN = 6       /* a bounded device wait gave up                */
+};
+
+typedef struct lepic_ctx lepic_ctx;
+
+typedef struct {
+  int32_t n[3];
+  double h[3];
+} lepic_grid;
+
+/* Device-resident structure-of-arrays particle set of one species (FP64).
+ * Replaces ParticleSet (modules/particles/mod_particles.f90:7-34); the
+ * reference's w, sp and flag_cex carry no information on this path. */
+typedef struct {
+  double *x, *y, *z, *vx, *vy, *vz;
+  int32_t *cell;   /* cell id, valid after lepic_sort_particles_by_cell */
+  int8_t *dead;    /* flag_dead: 1 = killed by MCC                      */
+  int64_t n;       /* live particles (host copy)                        */
+  int64_t cap;     /* length of every array                             */
+} lepic_pset;
+
+/* Wall-BC parameters (apply_particle_bc_legacy arguments,
+ * modules/particles/mod_particleBC.f90:42-62); dielectric walls are not
+ * supported (flag_die must be 0). */
+typedef struct {
+  double xmax, ymax, zmax;
+  int32_t flag_pbc, flag_nmn;
+  int32_t ptype, tag_neg;  /* 1-based species id, negative-ion species or -1 */
+  double qmacro;           /* Nm*q */
+  double mass, Nm;
+  int32_t ngrid;           /* highest electrode label                 */
+} lepic_bc_params;
+
+/* Magnetic field of the Boris push (Src/part_expmover.f90:139-212). */
+typedef struct {
+  int32_t mode;            /* 0 none (leapfrog), 1 uniform, 2 map on the E grid */
+  double B[3];             /* uniform field (mode 1)                  */
+  const double *B4;        /* device map, 4 doubles per node (mode 2) */
+  int32_t magnetized;      /* 0: this species ignores B (flag_B_pos)  */
+} lepic_bfield;
+
+typedef struct {
+  int32_t flag_pbc, flag_nmn;
+  double eps0;
+  int32_t ncycl;           /* V-cycle cap (20000 in mod_simulation.f90:392) */
+  double eps;              /* tolerance on L1 residual / alpha            */
+  double omega;            /* SOR over-relaxation                          */
+  int32_t ng;              /* multigrid levels (<= 12)                     */
+} lepic_mg_params;
+
+typedef struct {
+  int32_t cycles;          /* V-cycles executed                        */
+  double ktot;             /* equivalent fine-grid sweeps              */
+  double res;              /* final L1 residual / alpha                */
+  int64_t sweeps[12];      /* SOR sweeps per level                     */
+} lepic_mg_stats;
+
+/* MCC reaction tables, device pointers, layout as documented per field
+ * (built on the host as modules/particles/mod_reactions.f90:414-558 does). */
+#define LEPIC_MAX_COL 128
+#define LEPIC_MAX_SPECIES 16
+typedef struct {
+  int32_t ntype_tracked, ntype_total;
+  double mass[LEPIC_MAX_SPECIES];       /* neutrals negative              */
+  double vt0[LEPIC_MAX_SPECIES];
+  double Nm[LEPIC_MAX_SPECIES];
+  double neutral_density[LEPIC_MAX_SPECIES];
+  double nu_uplim[LEPIC_MAX_SPECIES];
+  int32_t p_ncol[LEPIC_MAX_SPECIES];
+  int32_t ncol, npt;
+  const int32_t *sig_list;  /* device [ntype_total][LEPIC_MAX_COL]          */
+  const double *sig_Er;     /* device [npt], strictly increasing (eV)       */
+  const double *sig;        /* device [ncol][npt] (m^2)                      */
+  const double *sig_Eth;    /* device [ncol] threshold (eV)                  */
+  const int32_t *col_info;  /* device [ncol][10]: n_re, n_by, species...    */
+  const double *sigv_mx;    /* device [ntype_total][ncol]                    */
+} lepic_chem;
+
+enum { LEPIC_MCC_REPLAY_RAN2 = 0, LEPIC_MCC_PHILOX = 1 };
+
+typedef struct {
+  int32_t Nc[LEPIC_MAX_SPECIES];
+  int64_t accepted[LEPIC_MAX_SPECIES];
+  int64_t err_coll[LEPIC_MAX_SPECIES];
+  int64_t skipped_dead[LEPIC_MAX_SPECIES];
+  int64_t appended[LEPIC_MAX_SPECIES];
+} lepic_mcc_stats;
+
+/* ---------------- context ---------------- */
+int lepic_create(int32_t device, lepic_ctx **out);
+int lepic_destroy(lepic_ctx *ctx);
+const char *lepic_last_error(lepic_ctx *ctx);
+int lepic_set_stream(lepic_ctx *ctx, void *hip_stream); /* NULL = null stream */
+int lepic_synchronize(lepic_ctx *ctx);
+const char *lepic_version(void);
+
+/* ---------------- a11: RNG (modules/core/mod_RNG.f90:13-45) ---------------- */
+double lepic_ran2(int32_t *irand);                          /* host, one draw   */
+int32_t lepic_seed(int32_t iproc, int32_t mpi_rank);        /* mod_simParams.f90:202 */
+
+/* ---------------- a13: loaders ---------------- */
+/* load_particles_modular (mod_particle_loader.f90:14-246) for one virtual
+ * thread, on the host with ran2, written to host SoA arrays given in hp[]
+ * (ntype sets); np_thread (host, ntype*nodes) receives the initial deposit. */
+int lepic_load_particles_host(const lepic_grid *g, const int32_t *bcnd_host,
+                              const double *kq_host, double x_load, int32_t np_cell,
+                              int32_t ntype, const double *vt0, const double *Nm,
+                              const double *ni0, int32_t *iseed, lepic_pset *hp,
+                              double *np_thread_host);
+/* Device loader for large synthetic plasmas: np_cell particles per loadable
+ * cell, uniform in the cell, Maxwellian (vt) velocities, counter-based
+ * Philox-4x32-10 stream (key, species) -- statistically the reference's
+ * loader, not its ran2 sequence.  Species share positions when
+ * share_positions != 0 (as the reference loader does). */
+int lepic_load_uniform_device(lepic_ctx *ctx, const lepic_grid *g, const int32_t *bcnd,
+                              int32_t np_cell, int32_t ntype, const double *vt0,
+                              uint64_t key, lepic_pset *dp);
+
+/* ---------------- a2: sort (mod_particle_sorting.f90:16-181) ---------------- */
+/* Stable counting sort by cell of src into dst (dst->cap >= src->n).
+ * cell_count/cell_start (device, ncells int32, 0-based start) may be NULL. */
+int lepic_sort_particles_by_cell(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *src,
+                                 lepic_pset *dst, int32_t *cell_count, int32_t *cell_start);
+
+/* ---------------- a3/a4: movers ---------------- */
+/* move_particles_electrostatic (mod_particleMover.f90:72-156), in place. */
+int lepic_move_electrostatic(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *p,
+                             const double *E4, double q, double m, double dt);
+/* part_mover's Boris/leapfrog velocity update (Src/part_expmover.f90:51-246)
+ * on the modular gather, in place. */
+int lepic_move_boris(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *p, const double *E4,
+                     const lepic_bfield *bf, double q, double m, double dt);
+
+/* ---------------- a5: wall BC + stable compaction ---------------- */
+/* apply_particle_bc_legacy (mod_particleBC.f90:42-259): src -> dst, survivors
+ * in order.  p_mac (device, 2*(ngrid+1), indexed by label) and P_loss
+ * (device, 1) accumulate; *np_lost (host) receives the removed count. */
+int lepic_apply_bc(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *src, lepic_pset *dst,
+                   const int32_t *bcnd, const lepic_bc_params *bp, double *p_mac,
+                   double *P_loss, int64_t *np_lost);
+
+/* ---------------- a6/a7: deposition, density, rho ---------------- */
+int lepic_build_kq(lepic_ctx *ctx, const lepic_grid *g, const int32_t *bcnd, double *kq);
+/* clear_np_thread (mod_chargeDeposition.f90:16-22) */
+int lepic_fill(lepic_ctx *ctx, double *a, int64_t count, double value);
+/* deposit_particle_set_to_np_thread (mod_chargeDeposition.f90:25-133):
+ * np_local += kq*Nm/(h1 h2 h3)*w over the 8 nodes (FP64 atomics). */
+int lepic_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *p, const double *kq,
+                  double Nm, double *np_local);
+/* reduce_species_density (mod_density.f90:31-73), one rank, one virtual
+ * thread: periodic stitching in place on np_thread [ntype][nodes], then
+ * np_red = np_thread on nodes 1..n+1 and 0 elsewhere. */
+int lepic_reduce_density(lepic_ctx *ctx, const lepic_grid *g, const int32_t *bcnd,
+                         int32_t ntype, double *np_thread, double *np_red);
+/* build_rho_from_np (mod_density.f90:76-107); charge[] is host memory. */
+int lepic_build_rho(lepic_ctx *ctx, const lepic_grid *g, int32_t ntype, const double *np_red,
+                    const double *charge, double *rho);
+
+/* ---------------- a8: Poisson ---------------- */
+/* solve_poisson_legacy -> pdesolver -> mg -> sor_rb/restriction/prolongation
+ * (mod_PoissonSolver_legacy.f90:27-101, pdesolver.f90:1-135, mg.f90:1-940,
+ * sors.f90:1-535), one rank.  phi is updated in place (warm start). */
+int lepic_poisson_solve(lepic_ctx *ctx, const lepic_grid *g, double *phi, const double *rho,
+                        const int32_t *bcnd, const lepic_mg_params *mp, lepic_mg_stats *stats);
+
+/* ---------------- a9: E field ---------------- */
+/* calc_Efield_modular (mod_electricField.f90:9-164). */
+int lepic_efield(lepic_ctx *ctx, const lepic_grid *g, const double *phi, const int32_t *bcnd,
+                 double *E4);
+
+/* ---------------- a10: null-collision MCC ---------------- */
+/* perform_collisions_step (mod_collisions.f90:214-739) for one rank and one
+ * virtual thread.  parts[ntype_tracked] are updated in place (velocities,
+ * flag_dead, appended products up to cap).  np_red is [ntype][nodes].
+ * mode LEPIC_MCC_REPLAY_RAN2 replays the reference's sequential chain with
+ * the ran2 stream iseed (host in/out); LEPIC_MCC_PHILOX runs all particles in
+ * parallel with a Poisson(Pmax) trial count each, drawing from Philox-4x32-10
+ * (key, counter).  Pcoll (device, ntype_total) accumulates. */
+int lepic_collisions(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *parts,
+                     const double *np_red, const lepic_chem *chem, int32_t ns_coll, double dt,
+                     int32_t mode, int32_t *iseed, uint64_t philox_key, uint64_t philox_counter,
+                     double *Pcoll, lepic_mcc_stats *stats);
+
+/* ---------------- fused particle phase ---------------- */
+/* One launch per species of gather + (Boris|leapfrog) push + wall BC +
+ * stable compaction + deposit: the mover, apply_particle_bc_legacy and
+ * deposit_particle_set_to_np_thread of one modular step
+ * (mod_simulation.f90:425-451) without the intermediate passes over the
+ * particles.  Valid when no MCC runs between the BC and the deposit.
+ * np_local may be NULL (no deposit). */
+int lepic_push_bc_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *src,
+                          lepic_pset *dst, const double *E4, const lepic_bfield *bf, double q,
+                          double m, double dt, const int32_t *bcnd, const lepic_bc_params *bp,
+                          const double *kq, double *np_local, double *p_mac, double *P_loss,
+                          int64_t *np_lost);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
