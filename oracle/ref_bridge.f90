@@ -1,0 +1,156 @@
+! ref_bridge.f90 -- C entry points onto the reference's own Fortran modules.
+!
+! TEST INFRASTRUCTURE ONLY.  Built by oracle/ref.mk together with the
+! reference sources read in place (nothing of the reference is copied), into
+! oracle/_ref/.  Each routine loads host arrays into a reference ParticleSet,
+! calls the reference procedure unchanged and copies the result back, so the
+! tests can hold the oracle against the reference itself.
+module ref_bridge
+  use iso_c_binding
+  use iso_fortran_env, only: real64, int32, int8
+  use mod_particles, only: ParticleSet
+  use mod_particle_sorting, only: sort_particles_by_cell
+  use mod_particleMover, only: move_particles_electrostatic
+  use mod_particleBC, only: apply_particle_bc_legacy
+  use mod_chargeDeposition, only: deposit_particle_set_to_np_thread
+  use mod_charge_weights, only: build_kq
+  use mod_electricField, only: calc_Efield_modular
+  use mod_rng, only: ran2
+  implicit none
+contains
+
+  subroutine to_pset(p, np, x, y, z, vx, vy, vz, dead)
+    type(ParticleSet), intent(inout) :: p
+    integer(c_int64_t), intent(in) :: np
+    real(c_double), intent(in) :: x(np), y(np), z(np), vx(np), vy(np), vz(np)
+    integer(c_int8_t), intent(in) :: dead(np)
+    call p%allocate_pset(int(max(np, 1_c_int64_t), int32))
+    p%n = int(np, int32)
+    if (np > 0) then
+      p%x(1:np) = x; p%y(1:np) = y; p%z(1:np) = z
+      p%vx(1:np) = vx; p%vy(1:np) = vy; p%vz(1:np) = vz
+      p%flag_dead(1:np) = int(dead, int8)
+    end if
+  end subroutine to_pset
+
+  subroutine from_pset(p, np, x, y, z, vx, vy, vz, dead)
+    type(ParticleSet), intent(in) :: p
+    integer(c_int64_t), intent(inout) :: np
+    real(c_double), intent(inout) :: x(*), y(*), z(*), vx(*), vy(*), vz(*)
+    integer(c_int8_t), intent(inout) :: dead(*)
+    integer :: m
+    m = p%n
+    np = m
+    if (m > 0) then
+      x(1:m) = p%x(1:m); y(1:m) = p%y(1:m); z(1:m) = p%z(1:m)
+      vx(1:m) = p%vx(1:m); vy(1:m) = p%vy(1:m); vz(1:m) = p%vz(1:m)
+      dead(1:m) = int(p%flag_dead(1:m), c_int8_t)
+    end if
+  end subroutine from_pset
+
+  real(c_double) function ref_ran2(irand) bind(C, name="ref_ran2")
+    integer(c_int32_t), intent(inout) :: irand
+    ref_ran2 = ran2(irand)
+  end function ref_ran2
+
+  subroutine ref_sort(n, h, np, x, y, z, vx, vy, vz, dead, cell, ccount, cstart) bind(C, name="ref_sort")
+    integer(c_int32_t), intent(in) :: n(3)
+    real(c_double), intent(in) :: h(3)
+    integer(c_int64_t), value :: np
+    real(c_double), intent(inout) :: x(np), y(np), z(np), vx(np), vy(np), vz(np)
+    integer(c_int8_t), intent(inout) :: dead(np)
+    integer(c_int32_t), intent(out) :: cell(np), ccount(*), cstart(*)
+    type(ParticleSet) :: p
+    integer(c_int64_t) :: m
+    integer :: nc
+    call to_pset(p, np, x, y, z, vx, vy, vz, dead)
+    call sort_particles_by_cell(p, n, h)
+    m = np
+    call from_pset(p, m, x, y, z, vx, vy, vz, dead)
+    nc = n(1) * n(2) * n(3)
+    if (np > 0) cell(1:np) = p%cell_id(1:np)
+    ccount(1:nc) = p%cell_count(1:nc)
+    cstart(1:nc) = p%cell_start(1:nc)
+  end subroutine ref_sort
+
+  subroutine ref_move_es(n, h, np, x, y, z, vx, vy, vz, E3, q, m, dt) bind(C, name="ref_move_es")
+    integer(c_int32_t), intent(in) :: n(3)
+    real(c_double), intent(in) :: h(3)
+    integer(c_int64_t), value :: np
+    real(c_double), intent(inout) :: x(np), y(np), z(np), vx(np), vy(np), vz(np)
+    real(c_double), intent(in) :: E3(3, 0:n(1)+2, 0:n(2)+2, 0:n(3)+2)
+    real(c_double), value :: q, m, dt
+    type(ParticleSet) :: p
+    integer(c_int8_t), allocatable :: dead(:)
+    integer(c_int64_t) :: k
+    allocate(dead(max(np, 1_c_int64_t)))
+    dead = 0
+    call to_pset(p, np, x, y, z, vx, vy, vz, dead)
+    call move_particles_electrostatic(p, n, h, E3, q, m, dt)
+    k = np
+    call from_pset(p, k, x, y, z, vx, vy, vz, dead)
+  end subroutine ref_move_es
+
+  subroutine ref_apply_bc(n, h, np, x, y, z, vx, vy, vz, dead, bcnd, flag_pbc, flag_nmn, ptype, &
+                          tag_neg, qmacro, mass, Nm, ngrid, p_mac, P_loss) bind(C, name="ref_apply_bc")
+    integer(c_int32_t), intent(in) :: n(3)
+    real(c_double), intent(in) :: h(3)
+    integer(c_int64_t), intent(inout) :: np
+    real(c_double), intent(inout) :: x(*), y(*), z(*), vx(*), vy(*), vz(*)
+    integer(c_int8_t), intent(inout) :: dead(*)
+    integer(c_int32_t), intent(in) :: bcnd(0:n(1)+2, 0:n(2)+2, 0:n(3)+2)
+    integer(c_int32_t), value :: flag_pbc, flag_nmn, ptype, tag_neg, ngrid
+    real(c_double), value :: qmacro, mass, Nm
+    real(c_double), intent(inout) :: p_mac(2, ngrid+1)
+    real(c_double), intent(inout) :: P_loss
+    type(ParticleSet) :: p
+    integer(int32) :: dtype(ngrid+1)
+    real(real64), allocatable :: sxz(:,:), syz(:,:,:)
+    allocate(sxz(0:n(1)+2, 0:n(3)+2), syz(2, 0:n(2)+2, 0:n(3)+2))
+    sxz = 0.0_real64
+    syz = 0.0_real64
+    dtype = 0
+    call to_pset(p, np, x(1:np), y(1:np), z(1:np), vx(1:np), vy(1:np), vz(1:np), dead(1:np))
+    call apply_particle_bc_legacy(p, n, h, bcnd, real(n(1),real64)*h(1), real(n(2),real64)*h(2), &
+         real(n(3),real64)*h(3), flag_pbc, flag_nmn, ptype, tag_neg, 0_int32, dtype, qmacro, &
+         sxz, syz, p_mac, mass, P_loss, Nm)
+    call from_pset(p, np, x, y, z, vx, vy, vz, dead)
+  end subroutine ref_apply_bc
+
+  subroutine ref_deposit(n, h, np, x, y, z, dead, kq, Nm, np_local) bind(C, name="ref_deposit")
+    integer(c_int32_t), intent(in) :: n(3)
+    real(c_double), intent(in) :: h(3)
+    integer(c_int64_t), value :: np
+    real(c_double), intent(in) :: x(np), y(np), z(np)
+    integer(c_int8_t), intent(in) :: dead(np)
+    real(c_double), intent(in) :: kq(0:n(1)+2, 0:n(2)+2, 0:n(3)+2)
+    real(c_double), value :: Nm
+    real(c_double), intent(inout) :: np_local(0:n(1)+2, 0:n(2)+2, 0:n(3)+2)
+    type(ParticleSet) :: p
+    real(c_double), allocatable :: zero(:)
+    allocate(zero(max(np, 1_c_int64_t)))
+    zero = 0.0_c_double
+    call to_pset(p, np, x, y, z, zero, zero, zero, dead)
+    call deposit_particle_set_to_np_thread(p, n, h, kq, Nm, np_local)
+  end subroutine ref_deposit
+
+  subroutine ref_build_kq(n, bcnd, kq) bind(C, name="ref_build_kq")
+    integer(c_int32_t), intent(in) :: n(3)
+    integer(c_int32_t), intent(in) :: bcnd(0:n(1)+2, 0:n(2)+2, 0:n(3)+2)
+    real(c_double), intent(out) :: kq(0:n(1)+2, 0:n(2)+2, 0:n(3)+2)
+    integer, allocatable :: b(:,:,:)
+    allocate(b(0:n(1)+2, 0:n(2)+2, 0:n(3)+2))
+    b = bcnd
+    call build_kq(b, kq)
+  end subroutine ref_build_kq
+
+  subroutine ref_efield(n, h, phi, E3, bcnd) bind(C, name="ref_efield")
+    integer(c_int32_t), intent(in) :: n(3)
+    real(c_double), intent(in) :: h(3)
+    real(c_double), intent(in) :: phi(0:n(1)+2, 0:n(2)+2, 0:n(3)+2)
+    real(c_double), intent(inout) :: E3(3, 0:n(1)+2, 0:n(2)+2, 0:n(3)+2)
+    integer(c_int32_t), intent(in) :: bcnd(0:n(1)+2, 0:n(2)+2, 0:n(3)+2)
+    call calc_Efield_modular(n, h, phi, E3, bcnd)
+  end subroutine ref_efield
+
+end module ref_bridge
