@@ -27,7 +27,8 @@ EXPORTS = (
     "lepic_load_uniform_device", "lepic_sort_particles_by_cell", "lepic_move_electrostatic",
     "lepic_move_boris", "lepic_apply_bc", "lepic_build_kq", "lepic_fill", "lepic_deposit",
     "lepic_reduce_density", "lepic_build_rho", "lepic_poisson_solve", "lepic_efield",
-    "lepic_collisions", "lepic_push_bc_deposit",
+    "lepic_collisions", "lepic_push_bc_deposit", "lepic_push_bc_deposit_inplace",
+    "lepic_sort_compact",
 )
 
 
@@ -138,6 +139,11 @@ def load() -> C.CDLL:
             "lepic_push_bc_deposit": (C.c_int, [vp, G, PS, PS, vp, C.POINTER(BField), C.c_double,
                                                 C.c_double, C.c_double, vp, C.POINTER(BCParams),
                                                 vp, vp, vp, vp, C.POINTER(C.c_int64)]),
+            "lepic_push_bc_deposit_inplace": (C.c_int, [vp, G, PS, vp, C.POINTER(BField),
+                                                        C.c_double, C.c_double, C.c_double, vp,
+                                                        C.POINTER(BCParams), vp, vp, vp, vp,
+                                                        C.POINTER(C.c_int64)]),
+            "lepic_sort_compact": (C.c_int, [vp, G, PS, PS]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -34,7 +34,31 @@ struct lepic_ctx {
   size_t mg_bytes = 0;
   int32_t mg_n[3] = {0, 0, 0};
   int32_t mg_ng = 0;
+  // per-XCD deposit replicas (LEPIC_XCDS x nodes doubles, kept zeroed)
+  double *rep = nullptr;
+  int64_t rep_nodes = 0;
 };
+
+// MI355X has 8 XCDs, each with its own L2.  Deposition accumulates into one
+// node-array replica per XCD with L2-resident (workgroup-scope) atomics; the
+// replicas are summed afterwards.  The XCD of a running wave comes from the
+// XCC_ID hardware register, so the replica choice never depends on how the
+// dispatcher spreads workgroups.
+#define LEPIC_XCDS 8
+
+__device__ inline int xcc_id() {
+  // s_getreg_b32 hwreg(HW_REG_XCC_ID=20, offset 0, size 4)
+  return (int)(__builtin_amdgcn_s_getreg((3 << 11) | 20) & (LEPIC_XCDS - 1));
+}
+
+// Atomic add performed in this XCD's L2 (no cross-XCD coherence): valid only
+// on memory that no other XCD touches during the kernel.
+__device__ inline void l2_atomic_add(double *p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+double *lepic_replicas(lepic_ctx *ctx, int64_t nodes);
+int lepic_replica_flush(lepic_ctx *ctx, int64_t nodes, double *np_local);
 
 // control block word offsets (int64 units)
 enum : int {

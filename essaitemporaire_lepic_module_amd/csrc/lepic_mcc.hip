@@ -116,14 +116,30 @@ struct SerialAppend {
   __device__ void count(int64_t *a) { *a += 1; }
 };
 
+// One atomic per wave: the active lanes of the calling wave reserve
+// consecutive values of *ctr.
+__device__ inline unsigned long long wave_reserve(unsigned long long *ctr) {
+  const unsigned long long mask = __ballot(1);
+  const int lane = __lane_id();
+  const int leader = __ffsll(mask) - 1;
+  const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(ctr, (unsigned long long)__popcll(mask));
+  base = __shfl(base, leader);
+  return base + (unsigned long long)rank;
+}
+
 struct AtomicAppend {
   MccCounters *cnt;
+  double pcoll[4];   // tracked species (at most 4)
   __device__ int64_t slot(int bt) {
-    return (int64_t)atomicAdd((unsigned long long *)&cnt->n[bt], 1ull);
+    return (int64_t)wave_reserve((unsigned long long *)&cnt->n[bt]);
   }
   __device__ void commit(int, int64_t) {}
-  __device__ void add(double *a, double v) { atomicAdd(a, v); }
-  __device__ void count(int64_t *a) { atomicAdd((unsigned long long *)a, 1ull); }
+  // Pcoll sums per thread; flushed once per wave at the end of the kernel
+  __device__ void add(double *a, double v) { pcoll[a - pcoll_base] += v; }
+  __device__ void count(int64_t *a) { (void)wave_reserve((unsigned long long *)a); }
+  double *pcoll_base;
 };
 
 // One null-collision trial on particle ip of species pt
@@ -335,22 +351,33 @@ __global__ __launch_bounds__(BLOCK) void k_mcc_philox(KGrid g, KParts P, KChem K
                                                       int64_t n0, double lambda, uint64_t key,
                                                       uint64_t counter, MccCounters *cnt,
                                                       double *Pcoll) {
-  int64_t ip = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
-  if (ip >= n0) return;
-  PhiloxStream rng;
-  rng.init(key ^ ((uint64_t)pt << 56), (uint32_t)ip, (uint32_t)(ip >> 32), (uint32_t)counter);
-  rng.c3 ^= (uint32_t)(counter >> 32) * 0x9E3779B9u;
-  // inversion sampling of Poisson(lambda), lambda <= 1
-  double u = rng.uniform();
-  double p = exp(-lambda), cdf = p;
-  int k = 0;
-  while (u > cdf && k < 32) {
-    ++k;
-    p = p * lambda / (double)k;
-    cdf = cdf + p;
+  const int64_t ip = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+  AtomicAppend app;
+  app.cnt = cnt;
+  app.pcoll_base = Pcoll;
+  for (int t = 0; t < 4; ++t) app.pcoll[t] = 0.0;
+  // holes and MCC-killed particles take no trials (no draws either)
+  if (ip < n0 && P.dead[pt][ip] == 0) {
+    PhiloxStream rng;
+    rng.init(key ^ ((uint64_t)pt << 56), (uint32_t)ip, (uint32_t)(ip >> 32), (uint32_t)counter);
+    rng.c3 ^= (uint32_t)(counter >> 32) * 0x9E3779B9u;
+    // inversion sampling of Poisson(lambda), lambda <= 1
+    const double u = rng.uniform();
+    double p = exp(-lambda), cdf = p;
+    int k = 0;
+    while (u > cdf && k < 32) {
+      ++k;
+      p = p * lambda / (double)k;
+      cdf = cdf + p;
+    }
+    for (int t = 0; t < k; ++t) trial(g, P, K, np_red, nodes, pt, ip, rng, app, cnt, Pcoll);
   }
-  AtomicAppend app{cnt};
-  for (int t = 0; t < k; ++t) trial(g, P, K, np_red, nodes, pt, ip, rng, app, cnt, Pcoll);
+  // Pcoll: wave sums, one atomic per wave and species
+  for (int t = 0; t < K.c.ntype_tracked; ++t) {
+    double v = app.pcoll[t];
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
+    if (__lane_id() == 0 && v != 0.0) atomicAdd(&Pcoll[t], v);
+  }
 }
 
 // np_mx of the reference: max over stored nodes 0..n of each dimension

@@ -295,72 +295,249 @@ __device__ inline bool deposit_weights(const KGrid &g, double k1, double x, doub
   return true;
 }
 
+// Block deposition.  Each block owns an LDS box of nodes covering its
+// particles' deposit stencils.  Within a wave, lanes whose particles share a
+// lower node (runs of one cell in sorted data) are first summed with a
+// segmented shuffle scan, so one lane per run issues the 8 LDS atomics; the
+// box is then flushed with one L2-resident atomic per touched node into this
+// XCD's replica.  Particles whose stencil leaves the box go straight to the
+// replica.  kq is 1 or 2, so kq*sum(w) equals sum(kq*w) term for term.
 struct DepShared {
   int mn[3], mx[3];
+  int x0, y0, z0, bx, by, bz, vol;
   double box[LDS_BOX];
 };
 
-// Block deposit: accumulate the block's weights in an LDS box spanning its
-// node range and flush each touched node with one global atomic, or fall back
-// to per-particle global atomics when the box does not fit.  kq is 1 or 2, so
-// kq*sum(w) equals sum(kq*w) term for term.
-__device__ inline void block_deposit(const KGrid &g, DepShared &ds, bool has, int ix, int iy,
-                                     int iz, const double w[8], const double *__restrict__ kq,
-                                     double *np) {
+__device__ inline int wave_min(int v) {
+  for (int d = 32; d > 0; d >>= 1) v = min(v, __shfl_xor(v, d));
+  return v;
+}
+__device__ inline int wave_max(int v) {
+  for (int d = 32; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d));
+  return v;
+}
+
+// Inclusive segmented sum over runs of equal keys in lane order; returns true
+// on the last lane of each run, which then holds the run's totals.
+__device__ inline bool seg_sum8(int key, double v[8]) {
+  const int lane = __lane_id();
+  const int prev = __shfl_up(key, 1);
+  const int next = __shfl_down(key, 1);
+  const bool head = (lane == 0) || (prev != key);
+  const bool tail = (lane == LEPIC_WAVE - 1) || (next != key);
+  int start = head ? lane : 0;
+  for (int d = 1; d < LEPIC_WAVE; d <<= 1) {
+    int s2 = __shfl_up(start, d);
+    if (lane >= d) start = max(start, s2);
+  }
+  for (int d = 1; d < LEPIC_WAVE; d <<= 1) {
+    const bool take = (lane - d) >= start;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      double u = __shfl_up(v[q], d);
+      if (take) v[q] += u;
+    }
+  }
+  return tail;
+}
+
+// Pass 1: the block's cell range from the particles' current positions,
+// widened by one cell each way for motion within the step.
+__device__ inline void box_begin(const KGrid &g, DepShared &ds, int lo[3], int hi[3]) {
   if (threadIdx.x == 0) {
     ds.mn[0] = ds.mn[1] = ds.mn[2] = 1 << 30;
     ds.mx[0] = ds.mx[1] = ds.mx[2] = -(1 << 30);
   }
   __syncthreads();
-  if (has) {
-    atomicMin(&ds.mn[0], ix); atomicMin(&ds.mn[1], iy); atomicMin(&ds.mn[2], iz);
-    atomicMax(&ds.mx[0], ix); atomicMax(&ds.mx[1], iy); atomicMax(&ds.mx[2], iz);
+  int a0 = wave_min(lo[0]), a1 = wave_min(lo[1]), a2 = wave_min(lo[2]);
+  int b0 = wave_max(hi[0]), b1 = wave_max(hi[1]), b2 = wave_max(hi[2]);
+  if (__lane_id() == 0) {
+    atomicMin(&ds.mn[0], a0); atomicMin(&ds.mn[1], a1); atomicMin(&ds.mn[2], a2);
+    atomicMax(&ds.mx[0], b0); atomicMax(&ds.mx[1], b1); atomicMax(&ds.mx[2], b2);
   }
   __syncthreads();
-  const int x0 = ds.mn[0], y0 = ds.mn[1], z0 = ds.mn[2];
-  if (ds.mx[0] < x0) return;  // nothing to deposit (uniform branch)
-  const int bx = ds.mx[0] - x0 + 2, by = ds.mx[1] - y0 + 2, bz = ds.mx[2] - z0 + 2;
-  const int vol = bx * by * bz;
-  if (vol > LDS_BOX) {
-    if (has) {
-      int64_t b = gidx(g, ix, iy, iz);
-      const int64_t nd[8] = {b, b + 1, b + g.sy + 1, b + g.sy, b + g.sz, b + g.sz + 1,
-                             b + g.sz + g.sy + 1, b + g.sz + g.sy};
-#pragma unroll
-      for (int q = 0; q < 8; ++q) atomicAdd(&np[nd[q]], kq[nd[q]] * w[q]);
+  if (threadIdx.x == 0) {
+    int vol = 0;
+    if (ds.mx[0] >= ds.mn[0]) {
+      ds.x0 = max(ds.mn[0] - 1, 0);
+      ds.y0 = max(ds.mn[1] - 1, 0);
+      ds.z0 = max(ds.mn[2] - 1, 0);
+      ds.bx = min(ds.mx[0] + 2, g.n1 + 2) - ds.x0 + 1;
+      ds.by = min(ds.mx[1] + 2, g.n2 + 2) - ds.y0 + 1;
+      ds.bz = min(ds.mx[2] + 2, g.n3 + 2) - ds.z0 + 1;
+      vol = ds.bx * ds.by * ds.bz;
+      if (vol > LDS_BOX) vol = -1;
     }
-    return;
+    ds.vol = vol;
   }
-  for (int t = threadIdx.x; t < vol; t += BLOCK) ds.box[t] = 0.0;
   __syncthreads();
-  if (has) {
-    int l = (ix - x0) + bx * ((iy - y0) + by * (iz - z0));
-    const int ld[8] = {l, l + 1, l + bx + 1, l + bx, l + bx * by, l + bx * by + 1,
-                       l + bx * by + bx + 1, l + bx * by + bx};
+  for (int t = threadIdx.x; t < ds.vol; t += BLOCK) ds.box[t] = 0.0;
+  __syncthreads();
+}
+
+// Deposit one particle per lane (has = false lanes take part in the shuffles
+// with a unique key and no weight).
+__device__ inline void box_add(const KGrid &g, DepShared &ds, bool has, int ix, int iy, int iz,
+                               double w[8], const double *__restrict__ kq, double *rep) {
+  int l = -1;
+  if (has && ds.vol > 0) {
+    int lx = ix - ds.x0, ly = iy - ds.y0, lz = iz - ds.z0;
+    if (lx >= 0 && ly >= 0 && lz >= 0 && lx + 1 < ds.bx && ly + 1 < ds.by && lz + 1 < ds.bz)
+      l = lx + ds.bx * (ly + ds.by * lz);
+  }
+  if (has && l < 0) {  // outside the box: straight to this XCD's replica
+    int64_t b = gidx(g, ix, iy, iz);
+    const int64_t nd[8] = {b, b + 1, b + g.sy + 1, b + g.sy, b + g.sz, b + g.sz + 1,
+                           b + g.sz + g.sy + 1, b + g.sz + g.sy};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) l2_atomic_add(&rep[nd[q]], kq[nd[q]] * w[q]);
+  }
+  if (ds.vol <= 0) return;  // uniform across the block
+  const int key = (l >= 0) ? l : -1 - __lane_id();
+  if (l < 0) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) w[q] = 0.0;
+  }
+  const bool tail = seg_sum8(key, w);
+  if (tail && l >= 0) {
+    const int bx = ds.bx, bxy = ds.bx * ds.by;
+    const int ld[8] = {l, l + 1, l + bx + 1, l + bx, l + bxy, l + bxy + 1, l + bxy + bx + 1,
+                       l + bxy + bx};
 #pragma unroll
     for (int q = 0; q < 8; ++q) atomicAdd(&ds.box[ld[q]], w[q]);
   }
+}
+
+__device__ inline void box_flush(const KGrid &g, DepShared &ds, const double *__restrict__ kq,
+                                 double *rep) {
   __syncthreads();
-  for (int t = threadIdx.x; t < vol; t += BLOCK) {
+  for (int t = threadIdx.x; t < ds.vol; t += BLOCK) {
     double v = ds.box[t];
     if (v != 0.0) {
-      int lx = t % bx, r = t / bx, ly = r % by, lz = r / by;
-      int64_t node = gidx(g, x0 + lx, y0 + ly, z0 + lz);
-      atomicAdd(&np[node], kq[node] * v);
+      int lx = t % ds.bx, r = t / ds.bx, ly = r % ds.by, lz = r / ds.by;
+      int64_t node = gidx(g, ds.x0 + lx, ds.y0 + ly, ds.z0 + lz);
+      l2_atomic_add(&rep[node], kq[node] * v);
     }
   }
 }
 
+__device__ inline void cell_of(const KGrid &g, double x, double y, double z, int c[3]) {
+  c[0] = imax(0, imin(g.n1 + 1, (int)(x / g.h1) + 1));
+  c[1] = imax(0, imin(g.n2 + 1, (int)(y / g.h2) + 1));
+  c[2] = imax(0, imin(g.n3 + 1, (int)(z / g.h3) + 1));
+}
+
+constexpr int PPT = 4;              // particles per thread in the streaming kernels
+constexpr int TILE = BLOCK * PPT;
+
 __global__ __launch_bounds__(BLOCK) void k_deposit(KGrid g, KPset p, int64_t n,
                                                    const double *__restrict__ kq, double k1,
-                                                   double *np) {
+                                                   double *rep_base, int64_t nodes) {
   __shared__ DepShared ds;
-  int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
-  bool has = false;
-  int ix = 0, iy = 0, iz = 0;
-  double w[8];
-  if (i < n && p.dead[i] == 0) has = deposit_weights(g, k1, p.x[i], p.y[i], p.z[i], ix, iy, iz, w);
-  block_deposit(g, ds, has, ix, iy, iz, w, kq, np);
+  double *rep = rep_base + (int64_t)xcc_id() * nodes;
+  const int64_t base = (int64_t)blockIdx.x * TILE + threadIdx.x;
+  double xs[PPT], ys[PPT], zs[PPT];
+  bool live[PPT];
+  int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+#pragma unroll
+  for (int r = 0; r < PPT; ++r) {
+    const int64_t i = base + (int64_t)r * BLOCK;
+    live[r] = (i < n) && p.dead[i] == 0;
+    if (live[r]) {
+      xs[r] = p.x[i]; ys[r] = p.y[i]; zs[r] = p.z[i];
+      int c[3];
+      cell_of(g, xs[r], ys[r], zs[r], c);
+      for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], c[d]); hi[d] = max(hi[d], c[d]); }
+    }
+  }
+  box_begin(g, ds, lo, hi);
+#pragma unroll
+  for (int r = 0; r < PPT; ++r) {
+    int ix = 0, iy = 0, iz = 0;
+    double w[8];
+    bool has = live[r] && deposit_weights(g, k1, xs[r], ys[r], zs[r], ix, iy, iz, w);
+    box_add(g, ds, has, ix, iy, iz, w, kq, rep);
+  }
+  box_flush(g, ds, kq, rep);
+}
+
+// np_local += sum of the replicas; replicas cleared for the next deposit.
+__global__ void k_rep_flush(double *np, double *rep, int64_t nodes) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nodes) return;
+  double s = 0.0;
+#pragma unroll
+  for (int r = 0; r < LEPIC_XCDS; ++r) {
+    s = s + rep[(int64_t)r * nodes + i];
+    rep[(int64_t)r * nodes + i] = 0.0;
+  }
+  np[i] = np[i] + s;
+}
+
+// Production particle phase, in place: gather + push + wall BC (+ deposit).
+// Lost particles become holes (dead = 2) instead of being compacted; the
+// next sort drops holes, keeping the survivors' relative order, which is the
+// order the reference's per-step stable compaction produces.
+__global__ __launch_bounds__(BLOCK) void k_push_inplace(
+    KGrid g, KPset p, int64_t n, const double4 *__restrict__ E4, KBField bf, double k1, double dt,
+    const int *__restrict__ bcnd, KBC bp, const double *__restrict__ kq, double kdep,
+    double *rep_base, int64_t nodes, int do_dep, double *p_mac, double *P_loss,
+    unsigned long long *lost_ctr) {
+  __shared__ DepShared ds;
+  double *rep = rep_base ? rep_base + (int64_t)xcc_id() * nodes : nullptr;
+  const int64_t base = (int64_t)blockIdx.x * TILE + threadIdx.x;
+  double xs[PPT], ys[PPT], zs[PPT];
+  bool live[PPT];
+  int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+#pragma unroll
+  for (int r = 0; r < PPT; ++r) {
+    const int64_t i = base + (int64_t)r * BLOCK;
+    live[r] = (i < n) && p.dead[i] == 0;
+    if (live[r]) {
+      xs[r] = p.x[i]; ys[r] = p.y[i]; zs[r] = p.z[i];
+      if (do_dep) {
+        int c[3];
+        cell_of(g, xs[r], ys[r], zs[r], c);
+        for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], c[d]); hi[d] = max(hi[d], c[d]); }
+      }
+    }
+  }
+  if (do_dep) box_begin(g, ds, lo, hi);
+  int lost_local = 0;
+#pragma unroll
+  for (int r = 0; r < PPT; ++r) {
+    const int64_t i = base + (int64_t)r * BLOCK;
+    bool has = false;
+    int ix = 0, iy = 0, iz = 0;
+    double w[8];
+    if (live[r]) {
+      double xp = xs[r], yp = ys[r], zp = zs[r];
+      double vx = p.vx[i], vy = p.vy[i], vz = p.vz[i];
+      Stencil s;
+      make_stencil(g, xp, yp, zp, s);
+      double Ex, Ey, Ez;
+      gather3(E4, g, s, Ex, Ey, Ez);
+      velocity_update(g, bf, s, k1, Ex, Ey, Ez, vx, vy, vz);
+      xp = xp + dt * vx; yp = yp + dt * vy; zp = zp + dt * vz;
+      int label;
+      if (wall_test(g, bcnd, bp, xp, yp, zp, label)) {
+        wall_diagnostics(bp, label, vx, vy, vz, p_mac, P_loss);
+        p.dead[i] = 2;
+        ++lost_local;
+      } else {
+        survivor_wrap(bp, xp, yp, zp, vx);
+        p.x[i] = xp; p.y[i] = yp; p.z[i] = zp;
+        p.vx[i] = vx; p.vy[i] = vy; p.vz[i] = vz;
+        if (do_dep) has = deposit_weights(g, kdep, xp, yp, zp, ix, iy, iz, w);
+      }
+    }
+    if (do_dep) box_add(g, ds, has, ix, iy, iz, w, kq, rep);
+  }
+  if (do_dep) box_flush(g, ds, kq, rep);
+  // losses are rare: one atomic per wave that saw any
+  int tot = lost_local;
+  for (int d = 32; d > 0; d >>= 1) tot += __shfl_xor(tot, d);
+  if (__lane_id() == 0 && tot > 0) atomicAdd(lost_ctr, (unsigned long long)tot);
 }
 
 // --------------------------------------------------------------------------
@@ -369,8 +546,9 @@ __global__ __launch_bounds__(BLOCK) void k_deposit(KGrid g, KPset p, int64_t n,
 __global__ __launch_bounds__(BLOCK) void k_push_bc_deposit(
     KGrid g, KPset src, KPset dst, int64_t n, const double4 *__restrict__ E4, KBField bf,
     double k1, double dt, const int *__restrict__ bcnd, KBC bp, const double *__restrict__ kq,
-    double kdep, double *np, double *p_mac, double *P_loss, uint64_t *status, uint32_t epoch,
-    int *counter, int64_t *spin_fail, int64_t *count_out, int64_t ntiles) {
+    double kdep, double *rep_base, int64_t nodes, double *p_mac, double *P_loss,
+    uint64_t *status, uint32_t epoch, int *counter, int64_t *spin_fail, int64_t *count_out,
+    int64_t ntiles) {
   __shared__ LBShared sh;
   __shared__ DepShared ds;
   int64_t tile = take_tile(counter, ntiles, sh);
@@ -401,12 +579,20 @@ __global__ __launch_bounds__(BLOCK) void k_push_bc_deposit(
     dst.vx[o] = vx; dst.vy[o] = vy; dst.vz[o] = vz;
     dst.dead[o] = 0;
   }
-  if (np) {
+  if (rep_base) {
+    double *rep = rep_base + (int64_t)xcc_id() * nodes;
+    int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+    if (keep) {
+      cell_of(g, xp, yp, zp, lo);
+      hi[0] = lo[0]; hi[1] = lo[1]; hi[2] = lo[2];
+    }
+    box_begin(g, ds, lo, hi);
     bool has = false;
     int ix = 0, iy = 0, iz = 0;
     double w[8];
     if (keep) has = deposit_weights(g, kdep, xp, yp, zp, ix, iy, iz, w);
-    block_deposit(g, ds, has, ix, iy, iz, w, kq, np);
+    box_add(g, ds, has, ix, iy, iz, w, kq, rep);
+    box_flush(g, ds, kq, rep);
   }
 }
 
@@ -415,9 +601,22 @@ __global__ __launch_bounds__(BLOCK) void k_push_bc_deposit(
 // --------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_cell_keys(KGrid g, KPset p, int64_t n,
                                                      uint32_t *keys, uint32_t *vals,
-                                                     int32_t *count) {
+                                                     int32_t *count, int drop_dead,
+                                                     unsigned long long *ndead) {
   int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
   if (i >= n) return;
+  if (drop_dead) {
+    const bool d = p.dead[i] != 0;
+    const uint64_t m = __ballot(d);
+    if (d) {
+      // holes sort behind every cell; one atomic per wave counts them
+      keys[i] = (uint32_t)g.n1 * (uint32_t)g.n2 * (uint32_t)g.n3;
+      vals[i] = (uint32_t)i;
+      if (__lane_id() == __ffsll((unsigned long long)m) - 1)
+        atomicAdd(ndead, (unsigned long long)__popcll(m));
+      return;
+    }
+  }
   int ix = (int)(p.x[i] / g.h1) + 1;
   int iy = (int)(p.y[i] / g.h2) + 1;
   int iz = (int)(p.z[i] / g.h3) + 1;
@@ -534,6 +733,33 @@ int run_compaction(lepic_ctx *ctx, int64_t n, Launch launch, int64_t *n_out) {
 
 }  // namespace
 
+double *lepic_replicas(lepic_ctx *ctx, int64_t nodes) {
+  if (ctx->rep && ctx->rep_nodes >= nodes) return ctx->rep;
+  if (ctx->rep) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->rep);
+    ctx->rep = nullptr;
+    ctx->rep_nodes = 0;
+  }
+  size_t bytes = sizeof(double) * (size_t)LEPIC_XCDS * (size_t)nodes;
+  if (hipMalloc((void **)&ctx->rep, bytes) != hipSuccess) return nullptr;
+  if (hipMemsetAsync(ctx->rep, 0, bytes, ctx->stream) != hipSuccess) return nullptr;
+  ctx->rep_nodes = nodes;
+  return ctx->rep;
+}
+
+int lepic_replica_flush(lepic_ctx *ctx, int64_t nodes, double *np_local) {
+  // the deposit kernels stride the replicas by this grid's node count; the
+  // buffer stays all-zero between calls because every written entry is
+  // cleared here
+  if (!ctx->rep || nodes > ctx->rep_nodes)
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "replica flush: replicas smaller than the grid");
+  hipLaunchKernelGGL(k_rep_flush, dim3(grid_blocks(nodes, 256)), dim3(256), 0, ctx->stream,
+                     np_local, ctx->rep, nodes);
+  LEPIC_LAUNCH_CHECK(ctx, "k_rep_flush");
+  return LEPIC_OK;
+}
+
 extern "C" {
 
 int lepic_move_electrostatic(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *p,
@@ -605,10 +831,53 @@ int lepic_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *p, cons
   if ((st = check_grid(ctx, g)) || (st = check_pset(ctx, p, "deposit"))) return st;
   if (!kq || !np_local) return lepic_fail(ctx, LEPIC_ERR_ARG, "deposit: null argument");
   if (p->n == 0) return LEPIC_OK;
+  const int64_t nodes = grid_nodes(g);
+  double *rep = lepic_replicas(ctx, nodes);
+  if (!rep) return lepic_fail(ctx, LEPIC_ERR_HIP, "deposit: replica allocation failed");
   double k1 = Nm / (g->h[0] * g->h[1] * g->h[2]);
-  hipLaunchKernelGGL(k_deposit, dim3(grid_blocks(p->n, BLOCK)), dim3(BLOCK), 0, ctx->stream,
-                     make_kgrid(g), kp(p), p->n, kq, k1, np_local);
+  hipLaunchKernelGGL(k_deposit, dim3(grid_blocks(p->n, TILE)), dim3(BLOCK), 0, ctx->stream,
+                     make_kgrid(g), kp(p), p->n, kq, k1, rep, nodes);
   LEPIC_LAUNCH_CHECK(ctx, "k_deposit");
+  return lepic_replica_flush(ctx, nodes, np_local);
+}
+
+int lepic_push_bc_deposit_inplace(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *p,
+                                  const double *E4, const lepic_bfield *bf, double q, double m,
+                                  double dt, const int32_t *bcnd, const lepic_bc_params *bp,
+                                  const double *kq, double *np_local, double *p_mac,
+                                  double *P_loss, int64_t *np_lost) {
+  if (!ctx) return LEPIC_ERR_ARG;
+  int st;
+  if ((st = check_grid(ctx, g)) || (st = check_pset(ctx, p, "push_bc_deposit_inplace"))) return st;
+  if (!E4 || !bcnd || !bp || !p_mac || !P_loss || (np_local && !kq))
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "push_bc_deposit_inplace: null argument");
+  if (bf && bf->mode == 2 && !bf->B4)
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "push_bc_deposit_inplace: null B map");
+  const int64_t nodes = grid_nodes(g);
+  double *rep = nullptr;
+  if (np_local) {
+    rep = lepic_replicas(ctx, nodes);
+    if (!rep) return lepic_fail(ctx, LEPIC_ERR_HIP, "push_bc_deposit_inplace: replicas");
+  }
+  unsigned long long *lost = reinterpret_cast<unsigned long long *>(ctx->d_ctrl + CTRL_COUNT_OUT);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream,
+                     reinterpret_cast<int32_t *>(lost), (int64_t)2, 0);
+  if (p->n > 0) {
+    double k1 = dt * q / (2.0 * m);
+    double kdep = np_local ? bp->Nm / (g->h[0] * g->h[1] * g->h[2]) : 0.0;
+    hipLaunchKernelGGL(k_push_inplace, dim3(grid_blocks(p->n, TILE)), dim3(BLOCK), 0, ctx->stream,
+                       make_kgrid(g), kp(p), p->n, reinterpret_cast<const double4 *>(E4),
+                       make_kbfield(bf), k1, dt, bcnd, kbc(bp), kq, kdep, rep, nodes,
+                       np_local ? 1 : 0, p_mac, P_loss, lost);
+    LEPIC_LAUNCH_CHECK(ctx, "k_push_inplace");
+  }
+  if (np_local && (st = lepic_replica_flush(ctx, nodes, np_local))) return st;
+  if (np_lost) {
+    LEPIC_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, lost, sizeof(int64_t), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+    LEPIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    *np_lost = ctx->h_pinned[0];
+  }
   return LEPIC_OK;
 }
 
@@ -636,24 +905,31 @@ int lepic_push_bc_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset 
   KBField kbf = make_kbfield(bf);
   double k1 = dt * q / (2.0 * m);
   double kdep = np_local ? bp->Nm / (g->h[0] * g->h[1] * g->h[2]) : 0.0;
+  const int64_t nodes = grid_nodes(g);
+  double *rep = nullptr;
+  if (np_local) {
+    rep = lepic_replicas(ctx, nodes);
+    if (!rep) return lepic_fail(ctx, LEPIC_ERR_HIP, "push_bc_deposit: replicas");
+  }
   st = run_compaction(
       ctx, n,
       [&](dim3 grid, uint32_t epoch, int *counter, int64_t *spin, int64_t *count_out,
           int64_t ntiles) {
         hipLaunchKernelGGL(k_push_bc_deposit, grid, dim3(BLOCK), 0, ctx->stream, kg, kp(src),
                            kp(dst), n, reinterpret_cast<const double4 *>(E4), kbf, k1, dt, bcnd,
-                           kb, kq, kdep, np_local, p_mac, P_loss, ctx->d_status, epoch, counter,
-                           spin, count_out, ntiles);
+                           kb, kq, kdep, rep, nodes, p_mac, P_loss, ctx->d_status, epoch,
+                           counter, spin, count_out, ntiles);
       },
       &nout);
   if (st) return st;
+  if (np_local && (st = lepic_replica_flush(ctx, nodes, np_local))) return st;
   dst->n = nout;
   if (np_lost) *np_lost = n - nout;
   return LEPIC_OK;
 }
 
-int lepic_sort_particles_by_cell(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *src,
-                                 lepic_pset *dst, int32_t *cell_count, int32_t *cell_start) {
+static int sort_impl(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *src, lepic_pset *dst,
+                     int32_t *cell_count, int32_t *cell_start, int drop_dead) {
   if (!ctx) return LEPIC_ERR_ARG;
   int st;
   if ((st = check_grid(ctx, g)) || (st = check_pset(ctx, src, "sort src")) ||
@@ -678,7 +954,7 @@ int lepic_sort_particles_by_cell(lepic_ctx *ctx, const lepic_grid *g, const lepi
   dst->n = n;
   if (n > 0) {
     int nbits = 1;
-    while (((int64_t)1 << nbits) < ncells) ++nbits;
+    while (((int64_t)1 << nbits) <= ncells) ++nbits;   // keys 0..ncells (holes)
     size_t temp_bytes = 0;
     uint32_t *nul = nullptr;
     LEPIC_HIP(ctx, rocprim::radix_sort_pairs(nullptr, temp_bytes, nul, nul, nul, nul, (size_t)n,
@@ -692,15 +968,28 @@ int lepic_sort_particles_by_cell(lepic_ctx *ctx, const lepic_grid *g, const lepi
     uint32_t *keys_out = reinterpret_cast<uint32_t *>(base + 2 * off);
     uint32_t *vals_out = reinterpret_cast<uint32_t *>(base + 3 * off);
     void *temp = base + 4 * off;
+    unsigned long long *ndead = reinterpret_cast<unsigned long long *>(ctx->d_ctrl + CTRL_COUNT_OUT);
+    if (drop_dead)
+      hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream,
+                         reinterpret_cast<int32_t *>(ndead), (int64_t)2, 0);
     hipLaunchKernelGGL(k_cell_keys, dim3(grid_blocks(n, BLOCK)), dim3(BLOCK), 0, ctx->stream, kg,
-                       kp(src), n, keys, vals, count);
+                       kp(src), n, keys, vals, count, drop_dead, ndead);
     LEPIC_LAUNCH_CHECK(ctx, "k_cell_keys");
     LEPIC_HIP(ctx, rocprim::radix_sort_pairs(temp, temp_bytes, keys, keys_out, vals, vals_out,
                                              (size_t)n, 0, nbits, ctx->stream));
     KPset d = kp(dst);
     if (!d.cell) return lepic_fail(ctx, LEPIC_ERR_ARG, "sort: dst has no cell array");
-    hipLaunchKernelGGL(k_gather_sorted, dim3(grid_blocks(n, BLOCK)), dim3(BLOCK), 0, ctx->stream,
-                       kp(src), d, n, keys_out, vals_out);
+    int64_t nkeep = n;
+    if (drop_dead) {
+      LEPIC_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, ndead, sizeof(int64_t), hipMemcpyDeviceToHost,
+                                    ctx->stream));
+      LEPIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      nkeep = n - ctx->h_pinned[0];
+      dst->n = nkeep;
+    }
+    if (nkeep > 0)
+      hipLaunchKernelGGL(k_gather_sorted, dim3(grid_blocks(nkeep, BLOCK)), dim3(BLOCK), 0,
+                         ctx->stream, kp(src), d, nkeep, keys_out, vals_out);
     LEPIC_LAUNCH_CHECK(ctx, "k_gather_sorted");
   }
   if (cell_start) {
@@ -708,15 +997,23 @@ int lepic_sort_particles_by_cell(lepic_ctx *ctx, const lepic_grid *g, const lepi
     LEPIC_HIP(ctx, rocprim::exclusive_scan(nullptr, temp_bytes, count, cell_start, 0,
                                            (size_t)ncells, rocprim::plus<int32_t>(),
                                            ctx->stream));
+    // scratch 0 held the sort keys; the gather above is ordered before the scan
     void *temp = lepic_scratch(ctx, temp_bytes + 1024, 0);
-    if (n > 0) {
-      // scratch 0 holds the sort keys, which are no longer needed
-    }
     if (!temp) return lepic_fail(ctx, LEPIC_ERR_HIP, "sort: scan scratch allocation failed");
     LEPIC_HIP(ctx, rocprim::exclusive_scan(temp, temp_bytes, count, cell_start, 0, (size_t)ncells,
                                            rocprim::plus<int32_t>(), ctx->stream));
   }
   return LEPIC_OK;
+}
+
+int lepic_sort_particles_by_cell(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *src,
+                                 lepic_pset *dst, int32_t *cell_count, int32_t *cell_start) {
+  return sort_impl(ctx, g, src, dst, cell_count, cell_start, 0);
+}
+
+int lepic_sort_compact(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *src,
+                       lepic_pset *dst) {
+  return sort_impl(ctx, g, src, dst, nullptr, nullptr, 1);
 }
 
 int lepic_load_uniform_device(lepic_ctx *ctx, const lepic_grid *g, const int32_t *bcnd,

@@ -229,6 +229,31 @@ def push_bc_deposit(part: ParticleSet, n, h, E4, bf: _native.BField, q, m, dt, b
     return int(lost.value)
 
 
+def push_bc_deposit_inplace(part: ParticleSet, n, h, E4, bf: _native.BField, q, m, dt, bcnd,
+                            bp: _native.BCParams, kq, np_local, p_mac, P_loss) -> int:
+    """Production particle phase in place: lost particles become holes
+    (dead = 2), removed later by sort_compact.  Returns this call's losses."""
+    ctx = context(part.device)
+    g = _native.grid(n, h)
+    p = part.pset()
+    lost = C.c_int64(0)
+    ctx.check(ctx.lib.lepic_push_bc_deposit_inplace(ctx.h, C.byref(g), C.byref(p), ptr(E4),
+                                                    C.byref(bf), q, m, dt, ptr(bcnd), C.byref(bp),
+                                                    ptr(kq), ptr(np_local), ptr(p_mac), ptr(P_loss),
+                                                    C.byref(lost)))
+    return int(lost.value)
+
+
+def sort_compact(part: ParticleSet, n, h):
+    """Stable cell sort that drops holes and MCC-killed particles."""
+    ctx = context(part.device)
+    g = _native.grid(n, h)
+    src, dst = part.pset(), part.pset(other=True)
+    ctx.check(ctx.lib.lepic_sort_compact(ctx.h, C.byref(g), C.byref(src), C.byref(dst)))
+    part.swap()
+    part.n = int(dst.n)
+
+
 # ---------------------------------------------------------------------------
 # a10: MCC
 # ---------------------------------------------------------------------------

@@ -7,7 +7,9 @@ velocity update when B is on):
   E -> push -> wall BC -> MCC (istep mod nb_step_collisions == 0) -> deposit
   -> reduce density.
 With fused=True the push, the wall BC and (on steps without MCC) the
-deposit run as one kernel per species.
+deposit run as one kernel per species.  With lazy=True (production) that
+kernel works in place and leaves holes where particles were lost; the
+periodic sort removes them, keeping the survivors' order.
 """
 from __future__ import annotations
 
@@ -25,10 +27,11 @@ from .device import as_device, node_shape
 class Simulation:
     def __init__(self, case: Case, device=None, fused: bool = True, loader: str = "host",
                  mcc_mode: int = _native.MCC_REPLAY_RAN2, philox_key: int = 0x5EED5EED,
-                 capacity_factor: float = 1.3, parts=None, np_thread=None):
+                 capacity_factor: float = 1.3, parts=None, np_thread=None, lazy: bool = False):
         self.case = case
         self.dev = torch.device(device if device is not None else "cuda:0")
         self.fused = fused
+        self.lazy = lazy and fused
         self.mcc_mode = mcc_mode
         self.philox_key = philox_key
         n, h = case.n, case.h
@@ -90,6 +93,7 @@ class Simulation:
         self.last_mg = None
         self.last_mcc = None
         self.lost = [0] * self.ntype
+        self.holes = [0] * self.ntype   # lazy mode: lost slots not yet compacted
         self.timers = None   # {phase: [(start_event, end_event, units), ...]} when timing
 
     @contextlib.contextmanager
@@ -125,8 +129,12 @@ class Simulation:
         n, h = self.n, self.h
         if istep % c.nb_step_sort == 1:
             with self._phase("sort"):
-                for p in self.parts:
-                    if p.n > 1:
+                for s, p in enumerate(self.parts):
+                    if self.lazy:
+                        if p.n > 0:
+                            P.sort_compact(p, n, h)
+                        self.holes[s] = 0
+                    elif p.n > 1:
                         P.sort_particles_by_cell(p, n, h, cell_tables=False)
         with self._phase("rho"):
             F.reduce_species_density(n, self.bcnd, self.np_thread, self.ntype, self.np_red)
@@ -140,12 +148,16 @@ class Simulation:
                    and istep % c.nb_step_collisions == 0)
         if self.fused:
             F.clear_np_thread(self.np_thread)
+            push = P.push_bc_deposit_inplace if self.lazy else P.push_bc_deposit
             for s, p in enumerate(self.parts):
-                with self._phase("push_bc_deposit" if not collide else "push_bc", p.n):
-                    self.lost[s] = P.push_bc_deposit(
+                with self._phase("push_bc_deposit" if not collide else "push_bc",
+                                 p.n - self.holes[s]):
+                    self.lost[s] = push(
                         p, n, h, self.E4, self.bf[s], self.charge[s], self.mass[s], c.dt,
                         self.bcnd, self.bp[s], self.kq, None if collide else self.np_thread[s],
                         self.p_mac[s], self.P_loss[s:s + 1])
+                if self.lazy:
+                    self.holes[s] += self.lost[s]
         else:
             with self._phase("push"):
                 for s, p in enumerate(self.parts):
@@ -183,7 +195,8 @@ class Simulation:
 
     # ------------------------------------------------------------------
     def particle_count(self) -> int:
-        return sum(p.n for p in self.parts)
+        """Live particles (slots minus holes not yet compacted)."""
+        return sum(p.n - hl for p, hl in zip(self.parts, self.holes))
 
     def total_charge(self) -> float:
         """sum_s q_s Nm_s N_s."""

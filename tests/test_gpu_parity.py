@@ -241,3 +241,68 @@ def test_loader_host_bitexact(oracle):
         for k in ("vx", "vy", "vz"):
             np.testing.assert_allclose(g[k], o[k], rtol=1e-15, atol=0)
     _same(npt.cpu().numpy(), npo)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_fused_inplace_and_sort_compact(gpu, oracle, rng, mode):
+    """Production particle phase: in-place push + wall BC + deposit with holes,
+    then the compacting sort, against the oracle's mover + BC + deposit + sort."""
+    from essaitemporaire_lepic_module_amd import particles as P
+    n, h = H.box((12, 10, 8))
+    bc = H.two_electrode_bcnd(n)
+    kq = cfg.build_kq(bc)
+    count = 30000
+    arr = H.random_particles(rng, n, h, count, outside=0.02, vscale=2e6)
+    dead = (rng.random(count) < 0.03).astype(np.int8)
+    E4 = H.random_E4(rng, n)
+    q, m, dt, Nm = -1.60217646e-19, 9.10938188e-31, 5e-11, 1526.0
+    Bc = (0.0, 0.0, 0.01)
+    # sorted input, as in production (the deposit box path is then exercised)
+    po = _oracle_set(oracle, arr, dead=dead)
+    oracle.sort_particles(n, h, po)
+    start = po.arrays()
+    ps = _gpu_set(P, [start[k] for k in ("x", "y", "z", "vx", "vy", "vz")], dead=start["dead"])
+    bf = P.bfield(mode, Bc)
+    bp = P.bc_params(n, h, 0, 0, 1, -1, Nm * q, m, Nm, int(bc.max()))
+    p_mac = torch.zeros((int(bc.max()) + 1, 2), dtype=torch.float64, device=gpu)
+    P_loss = torch.zeros(1, dtype=torch.float64, device=gpu)
+    npg = torch.zeros(cfg.node_shape(n), dtype=torch.float64, device=gpu)
+    bcg = torch.as_tensor(bc, device=gpu)
+    lost = P.push_bc_deposit_inplace(ps, n, h, torch.as_tensor(E4, device=gpu), bf, q, m, dt, bcg,
+                                     bp, torch.as_tensor(kq, device=gpu), npg, p_mac, P_loss)
+    # oracle: mover, BC (removes MCC-dead too), deposit, sort
+    if mode == 0:
+        oracle.move_electrostatic(n, h, po, E4, q, m, dt)
+    else:
+        oracle.move_boris(n, h, po, E4, mode, Bc, None, q, m, dt)
+    lost_o, pmac_o, Ploss_o = oracle.apply_bc(n, h, po, bc, 0, 0, 1, -1, Nm * q, m, Nm, int(bc.max()))
+    npo = oracle.deposit(n, h, po, kq, Nm)
+    assert lost == lost_o - int(start["dead"].sum())   # MCC-dead holes are not new losses
+    g = ps.to_host()
+    keep = g["dead"] == 0
+    o = po.arrays()
+    for k in ("x", "y", "z", "vx", "vy", "vz"):
+        _same(g[k][keep], o[k])
+    np.testing.assert_allclose(npg.cpu().numpy(), npo, rtol=RTOL_ATOMIC, atol=1e-30)
+    np.testing.assert_allclose(p_mac.cpu().numpy().reshape(-1), pmac_o, rtol=RTOL_ATOMIC, atol=0)
+    P.sort_compact(ps, n, h)
+    oracle.sort_particles(n, h, po)
+    g, o = ps.to_host(), po.arrays()
+    assert ps.n == po.n
+    for k in ("x", "y", "z", "vx", "vy", "vz", "cell"):
+        _same(g[k], o[k])
+
+
+def test_deposit_unsorted_falls_back(gpu, oracle, rng):
+    """Particles spread over the whole grid in every block: the LDS box does
+    not fit and every particle goes straight to the replicas."""
+    from essaitemporaire_lepic_module_amd import particles as P, fields as F
+    n, h = H.box((40, 40, 40))
+    bc = cfg.grounded_box_bcnd(n)
+    kq = cfg.build_kq(bc)
+    arr = H.random_particles(rng, n, h, 20000)
+    ps = _gpu_set(P, arr)
+    npg = torch.zeros(cfg.node_shape(n), dtype=torch.float64, device=gpu)
+    F.deposit_particle_set_to_np_thread(ps, n, h, torch.as_tensor(kq, device=gpu), 3.0, npg)
+    npo = oracle.deposit(n, h, _oracle_set(oracle, arr), kq, 3.0)
+    np.testing.assert_allclose(npg.cpu().numpy(), npo, rtol=RTOL_ATOMIC, atol=1e-30)
