@@ -11,6 +11,7 @@
 #include <string>
 
 #include "../../include/lepic.h"
+#include "lepic_qdiv.h"
 
 #define LEPIC_WAVE 64
 
@@ -95,6 +96,7 @@ struct KGrid {
   int n1, n2, n3;
   int64_t sy, sz;   // strides of j and k (x stride is 1)
   double h1, h2, h3;
+  QDiv q1, q2, q3;  // x/h1, y/h2, z/h3 correctly rounded (lepic_qdiv.h)
 };
 
 static inline KGrid make_kgrid(const lepic_grid *g) {
@@ -103,6 +105,7 @@ static inline KGrid make_kgrid(const lepic_grid *g) {
   k.sy = (int64_t)g->n[0] + 3;
   k.sz = ((int64_t)g->n[0] + 3) * ((int64_t)g->n[1] + 3);
   k.h1 = g->h[0]; k.h2 = g->h[1]; k.h3 = g->h[2];
+  k.q1 = qdiv_make(g->h[0]); k.q2 = qdiv_make(g->h[1]); k.q3 = qdiv_make(g->h[2]);
   return k;
 }
 
@@ -127,15 +130,15 @@ struct Stencil {
 
 __device__ inline void make_stencil(const KGrid &g, double xp, double yp, double zp,
                                     Stencil &s) {
-  int ix = (int)(xp / g.h1) + 1;
-  int iy = (int)(yp / g.h2) + 1;
-  int iz = (int)(zp / g.h3) + 1;
+  int ix = (int)qdiv(xp, g.q1) + 1;
+  int iy = (int)qdiv(yp, g.q2) + 1;
+  int iz = (int)qdiv(zp, g.q3) + 1;
   ix = imax(0, imin(g.n1 + 1, ix));
   iy = imax(0, imin(g.n2 + 1, iy));
   iz = imax(0, imin(g.n3 + 1, iz));
-  double px = ((double)ix * g.h1 - xp) / g.h1;
-  double py = ((double)iy * g.h2 - yp) / g.h2;
-  double pz = ((double)iz * g.h3 - zp) / g.h3;
+  double px = qdiv((double)ix * g.h1 - xp, g.q1);
+  double py = qdiv((double)iy * g.h2 - yp, g.q2);
+  double pz = qdiv((double)iz * g.h3 - zp, g.q3);
   double wx2 = 1.0 - px, wy2 = 1.0 - py, wz2 = 1.0 - pz;
   s.w[0] = px * py * pz;
   s.w[1] = wx2 * py * pz;
@@ -169,6 +172,7 @@ struct KBField {
   int magnetized;
   double bx, by, bz;
   const double4 *B4;
+  double k2u;   // uniform B: k1*2/(1+(k1*|B|)^2), the same for every particle
 };
 
 __device__ inline void velocity_update(const KGrid &g, const KBField &bf, const Stencil &s,
@@ -187,8 +191,13 @@ __device__ inline void velocity_update(const KGrid &g, const KBField &bf, const 
   }
   if (!bf.magnetized) use_b = 0;
   if (use_b) {
-    double Bt = sqrt(Bx * Bx + By * By + Bz * Bz);
-    double k2 = k1 * (2.0 / (1.0 + (k1 * Bt) * (k1 * Bt)));
+    double k2;
+    if (bf.mode == 1) {
+      k2 = bf.k2u;
+    } else {
+      double Bt = sqrt(Bx * Bx + By * By + Bz * Bz);
+      k2 = k1 * (2.0 / (1.0 + (k1 * Bt) * (k1 * Bt)));
+    }
     double v1x = vx + k1 * Ex, v1y = vy + k1 * Ey, v1z = vz + k1 * Ez;
     double v3x = v1x + k1 * (v1y * Bz - v1z * By);
     double v3y = v1y + k1 * (v1z * Bx - v1x * Bz);
@@ -202,12 +211,16 @@ __device__ inline void velocity_update(const KGrid &g, const KBField &bf, const 
   }
 }
 
-static inline KBField make_kbfield(const lepic_bfield *bf) {
+// k1 = dt*q/(2m) of the launch; the uniform-B k2 is evaluated here with the
+// same operations the per-particle path performs.
+static inline KBField make_kbfield(const lepic_bfield *bf, double k1) {
   KBField k;
   k.mode = bf ? bf->mode : 0;
   k.magnetized = bf ? bf->magnetized : 1;
   k.bx = bf ? bf->B[0] : 0.0; k.by = bf ? bf->B[1] : 0.0; k.bz = bf ? bf->B[2] : 0.0;
   k.B4 = bf ? reinterpret_cast<const double4 *>(bf->B4) : nullptr;
+  double Bt = std::sqrt(k.bx * k.bx + k.by * k.by + k.bz * k.bz);
+  k.k2u = k1 * (2.0 / (1.0 + (k1 * Bt) * (k1 * Bt)));
   return k;
 }
 

@@ -5,11 +5,14 @@
 //  * REPLAY_RAN2 -- the reference's sequential Markov chain (trial order, ran2
 //    consumption, n growing with appended products, immediate kills) run by a
 //    single device thread, for exact parity with the reference;
-//  * PHILOX -- one thread per particle draws k ~ Poisson(Pmax) trials (the
-//    large-N law of Nc = N Pmax picks with replacement) from Philox-4x32-10
-//    keyed by (key, counter, species, particle), products appended with an
+//  * PHILOX -- every particle takes k ~ Poisson(Pmax) trials (the large-N
+//    law of Nc = N Pmax picks with replacement), products appended with an
 //    atomic slot counter.  Statistically the reference's process, fully
-//    parallel.
+//    parallel, in two passes: a select pass walks each chunk of particles by
+//    geometric gaps (the particles with k >= 1 form a Bernoulli(1 - e^-Pmax)
+//    sequence) and lists (particle, k) for the live ones; the trial pass runs
+//    one thread per listed particle, its draws from Philox-4x32-10 keyed by
+//    (key, counter, species, particle).
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -21,6 +24,8 @@ namespace {
 
 constexpr int BLOCK = 256;
 constexpr int MAXR = 16;   // reactions per projectile handled by the kernels
+constexpr int MAXT = 4;    // distinct targets per projectile
+constexpr int SEL_CHUNK = 64;   // particles per thread in the select pass
 constexpr double QE_SI = 1.60217646e-19;
 constexpr double PI_F = 3.14159265358979311600;
 
@@ -40,6 +45,7 @@ struct MccCounters {
   int64_t appended[LEPIC_MAX_SPECIES];
   int64_t overflow;
   int64_t seed;
+  int64_t nsel[LEPIC_MAX_SPECIES];       // select-pass list lengths
 };
 
 struct KChem {
@@ -154,9 +160,9 @@ __device__ void trial(const KGrid &g, const KParts &P, const KChem &K, const dou
     app.count(&cnt->skipped_dead[pt]);
     return;
   }
-  int ix = (int)(P.x[pt][ip] / g.h1) + 1;
-  int iy = (int)(P.y[pt][ip] / g.h2) + 1;
-  int iz = (int)(P.z[pt][ip] / g.h3) + 1;
+  int ix = (int)qdiv(P.x[pt][ip], g.q1) + 1;
+  int iy = (int)qdiv(P.y[pt][ip], g.q2) + 1;
+  int iz = (int)qdiv(P.z[pt][ip], g.q3) + 1;
   ix = imax(1, imin(g.n1, ix));
   iy = imax(1, imin(g.n2, iy));
   iz = imax(1, imin(g.n3, iz));
@@ -165,8 +171,15 @@ __device__ void trial(const KGrid &g, const KParts &P, const KChem &K, const dou
   double nu[MAXR];
   double sum_nu = 0.0;
   unsigned tsel = 0, tfound = 0;
-  double svr[LEPIC_MAX_SPECIES], sEkr[LEPIC_MAX_SPECIES], smu[LEPIC_MAX_SPECIES];
-  double scx[LEPIC_MAX_SPECIES], scy[LEPIC_MAX_SPECIES], scz[LEPIC_MAX_SPECIES];
+  // relative-motion data of each sampled neutral target, in first-seen slots
+  int tslot[MAXT];
+  int ntg = 0;
+  double svr[MAXT], sEkr[MAXT], smu[MAXT], scx[MAXT], scy[MAXT], scz[MAXT];
+  auto slot_of = [&](int tt) {
+    int s = 0;
+    while (s < ntg - 1 && tslot[s] != tt) ++s;
+    return s;
+  };
   for (int a = 0; a < ncl; ++a) nu[a] = 0.0;
   for (int ic2 = 0; ic2 < ncl; ++ic2) {
     const int col = c.sig_list[pt * LEPIC_MAX_COL + ic2];
@@ -182,13 +195,16 @@ __device__ void trial(const KGrid &g, const KParts &P, const KChem &K, const dou
         r1 = rng.uniform(); r2 = rng.uniform();
         box_muller(vz2, dummy, c.vt0[tt], r1, r2);
         const double mp = fabs(c.mass[pt]), mt = fabs(c.mass[tt]);
-        smu[tt] = mp * mt / (mp + mt);
+        const int sl = ntg < MAXT ? ntg : MAXT - 1;   // the host caps targets at MAXT
+        tslot[sl] = tt;
+        ntg = sl + 1;
+        smu[sl] = mp * mt / (mp + mt);
         const double dx = vx1 - vx2, dy = vy1 - vy2, dz = vz1 - vz2;
-        svr[tt] = sqrt(dx * dx + dy * dy + dz * dz);
-        sEkr[tt] = 0.5 * smu[tt] * (svr[tt] * svr[tt]) / QE_SI;
-        scx[tt] = (mp * vx1 + mt * vx2) / (mp + mt);
-        scy[tt] = (mp * vy1 + mt * vy2) / (mp + mt);
-        scz[tt] = (mp * vz1 + mt * vz2) / (mp + mt);
+        svr[sl] = sqrt(dx * dx + dy * dy + dz * dz);
+        sEkr[sl] = 0.5 * smu[sl] * (svr[sl] * svr[sl]) / QE_SI;
+        scx[sl] = (mp * vx1 + mt * vx2) / (mp + mt);
+        scy[sl] = (mp * vy1 + mt * vy2) / (mp + mt);
+        scz[sl] = (mp * vz1 + mt * vz2) / (mp + mt);
         tfound |= 1u << tt;
       } else {
         // charged target: the cell tables are cleared before MCC runs
@@ -198,7 +214,8 @@ __device__ void trial(const KGrid &g, const KParts &P, const KChem &K, const dou
       }
     }
     if (!(tfound & (1u << tt))) continue;
-    const double vr = svr[tt], Ekr = sEkr[tt];
+    const int sl = slot_of(tt);
+    const double vr = svr[sl], Ekr = sEkr[sl];
     const int ipt = find_bracket(c.sig_Er, c.npt, Ekr);
     if (ipt > 0) {
       const double EkL = c.sig_Er[ipt - 1], EkR = c.sig_Er[ipt];
@@ -247,8 +264,9 @@ __device__ void trial(const KGrid &g, const KParts &P, const KChem &K, const dou
   const int tt = c.col_info[c_ind * 10 + 1 + n_re];
   if (tt < 0 || tt >= NA) return;
   if (!(tfound & (1u << tt))) return;
-  const double mu = smu[tt], vr = svr[tt];
-  const double vxc = scx[tt], vyc = scy[tt], vzc = scz[tt];
+  const int sl = slot_of(tt);
+  const double mu = smu[sl], vr = svr[sl];
+  const double vxc = scx[sl], vyc = scy[sl], vzc = scz[sl];
   const double Eth = c.sig_Eth[c_ind];
   const double Ee = 0.5 * mu * vr * vr - Eth * QE_SI;
   if (Ee < 0.0) return;
@@ -345,31 +363,65 @@ __global__ void k_mcc_replay(KGrid g, KParts P, KChem K, const double *np_red, i
   cnt->seed = rng.s;
 }
 
-// PHILOX: one thread per particle of species pt, k ~ Poisson(lambda) trials.
-__global__ __launch_bounds__(BLOCK) void k_mcc_philox(KGrid g, KParts P, KChem K,
+// PHILOX select pass: thread t walks particles [t*SEL_CHUNK, (t+1)*SEL_CHUNK)
+// of species pt.  Particles with k >= 1 trials form a Bernoulli(p) sequence,
+// p = 1 - e^-lambda, so the gap to the next one is G = floor(-ln(1-u)/lambda)
+// (geometric, since ln(1-p) = -lambda); k given k >= 1 is zero-truncated
+// Poisson(lambda), drawn by inversion from P(1) = lambda / (e^lambda - 1).
+// Live particles are listed as (index | k << 40); holes and MCC-killed
+// particles take no trials.
+__global__ __launch_bounds__(BLOCK) void k_mcc_select(const int8_t *__restrict__ dead, int pt, int64_t n0,
+                                                      double lambda, double p1, uint64_t key,
+                                                      uint64_t counter, uint64_t *list,
+                                                      int64_t cap, unsigned long long *nsel) {
+  const int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+  const int64_t begin = t * SEL_CHUNK;
+  if (begin >= n0) return;
+  const int64_t end = begin + SEL_CHUNK < n0 ? begin + SEL_CHUNK : n0;
+  PhiloxStream rng;
+  rng.init(key ^ ((uint64_t)pt << 56) ^ 0x5E1EC7A11ull, (uint32_t)t, (uint32_t)(t >> 32),
+           (uint32_t)counter);
+  rng.c3 ^= (uint32_t)(counter >> 32) * 0x9E3779B9u;
+  int64_t j = begin - 1;
+  while (true) {
+    const double gap = -log(1.0 - rng.uniform()) / lambda;   // u in [0,1): finite, >= 0
+    if (gap >= (double)(end - 1 - j)) break;
+    j += 1 + (int64_t)gap;
+    const double u = rng.uniform();
+    double pk = p1, cdf = p1;
+    int k = 1;
+    while (u > cdf && k < 32) {
+      ++k;
+      pk = pk * lambda / (double)k;
+      cdf = cdf + pk;
+    }
+    if (dead[j] != 0) continue;
+    const unsigned long long slot = wave_reserve(nsel);
+    if ((int64_t)slot < cap) list[slot] = (uint64_t)j | ((uint64_t)k << 40);
+  }
+}
+
+// PHILOX trial pass: one thread per listed particle, k trials in a row on the
+// particle's own stream.
+__global__ __launch_bounds__(BLOCK) void k_mcc_trials(KGrid g, KParts P, KChem K,
                                                       const double *np_red, int64_t nodes, int pt,
-                                                      int64_t n0, double lambda, uint64_t key,
-                                                      uint64_t counter, MccCounters *cnt,
-                                                      double *Pcoll) {
-  const int64_t ip = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+                                                      const uint64_t *list,
+                                                      const unsigned long long *nsel, int64_t cap,
+                                                      uint64_t key, uint64_t counter,
+                                                      MccCounters *cnt, double *Pcoll) {
+  const int64_t e = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
   AtomicAppend app;
   app.cnt = cnt;
   app.pcoll_base = Pcoll;
   for (int t = 0; t < 4; ++t) app.pcoll[t] = 0.0;
-  // holes and MCC-killed particles take no trials (no draws either)
-  if (ip < n0 && P.dead[pt][ip] == 0) {
+  const int64_t m = (int64_t)*nsel < cap ? (int64_t)*nsel : cap;
+  if (e < m) {
+    const uint64_t w = list[e];
+    const int64_t ip = (int64_t)(w & ((1ull << 40) - 1));
+    const int k = (int)(w >> 40);
     PhiloxStream rng;
     rng.init(key ^ ((uint64_t)pt << 56), (uint32_t)ip, (uint32_t)(ip >> 32), (uint32_t)counter);
     rng.c3 ^= (uint32_t)(counter >> 32) * 0x9E3779B9u;
-    // inversion sampling of Poisson(lambda), lambda <= 1
-    const double u = rng.uniform();
-    double p = exp(-lambda), cdf = p;
-    int k = 0;
-    while (u > cdf && k < 32) {
-      ++k;
-      p = p * lambda / (double)k;
-      cdf = cdf + p;
-    }
     for (int t = 0; t < k; ++t) trial(g, P, K, np_red, nodes, pt, ip, rng, app, cnt, Pcoll);
   }
   // Pcoll: wave sums, one atomic per wave and species
@@ -378,6 +430,11 @@ __global__ __launch_bounds__(BLOCK) void k_mcc_philox(KGrid g, KParts P, KChem K
     for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
     if (__lane_id() == 0 && v != 0.0) atomicAdd(&Pcoll[t], v);
   }
+}
+
+__global__ void k_fill_u64(uint64_t *a, int64_t n) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
 }
 
 // np_mx of the reference: max over stored nodes 0..n of each dimension
@@ -469,6 +526,20 @@ extern "C" int lepic_collisions(lepic_ctx *ctx, const lepic_grid *g, lepic_pset 
       return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions: col_info reaction shape out of range");
   }
   if (c.npt < 2) return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions: energy grid too short");
+  for (int t = 0; t < NA; ++t) {
+    unsigned seen = 0;
+    int ntg = 0;
+    for (int ic = 0; ic < c.p_ncol[t]; ++ic) {
+      int col = hsl[(size_t)t * LEPIC_MAX_COL + ic];
+      int tt = hci[(size_t)col * 10 + 1 + hci[(size_t)col * 10]];
+      if (tt >= NT && tt < NA && !(seen & (1u << tt))) {
+        seen |= 1u << tt;
+        ++ntg;
+      }
+    }
+    if (ntg > MAXT)
+      return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions: more than 4 neutral targets per projectile");
+  }
   int32_t Nc[4] = {0, 0, 0, 0};
   double lambda[4] = {0, 0, 0, 0};
   for (int t = 0; t < NT; ++t) {
@@ -520,10 +591,38 @@ extern "C" int lepic_collisions(lepic_ctx *ctx, const lepic_grid *g, lepic_pset 
   } else if (mode == LEPIC_MCC_PHILOX) {
     for (int t = 0; t < NT; ++t) {
       if (c.p_ncol[t] == 0 || c.mass[t] <= 0.0 || lambda[t] <= 0.0 || parts[t].n == 0) continue;
-      hipLaunchKernelGGL(k_mcc_philox, dim3(grid_blocks(parts[t].n, BLOCK)), dim3(BLOCK), 0,
-                         ctx->stream, kg, P, K, np_red, nodes, t, parts[t].n, lambda[t],
-                         philox_key, philox_counter, dc, Pcoll);
-      LEPIC_LAUNCH_CHECK(ctx, "k_mcc_philox");
+      const int64_t n0 = parts[t].n;
+      const double lam = lambda[t];
+      const double p1 = lam / std::expm1(lam);
+      const double mean = (double)n0 * -std::expm1(-lam);
+      int64_t cap = (int64_t)(mean + 16.0 * std::sqrt(mean)) + 65536;
+      if (cap > n0) cap = n0;
+      unsigned long long *nsel = reinterpret_cast<unsigned long long *>(&dc->nsel[t]);
+      const int64_t nthreads = (n0 + SEL_CHUNK - 1) / SEL_CHUNK;
+      for (int pass = 0; pass < 2; ++pass) {
+        uint64_t *list = reinterpret_cast<uint64_t *>(lepic_scratch(ctx, sizeof(uint64_t) * (size_t)cap, 0));
+        if (!list) return lepic_fail(ctx, LEPIC_ERR_HIP, "collisions: selection list allocation");
+        hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream,
+                           reinterpret_cast<uint64_t *>(nsel), (int64_t)1);
+        hipLaunchKernelGGL(k_mcc_select, dim3(grid_blocks(nthreads, BLOCK)), dim3(BLOCK), 0,
+                           ctx->stream, P.dead[t], t, n0, lam, p1, philox_key, philox_counter, list,
+                           cap, nsel);
+        LEPIC_LAUNCH_CHECK(ctx, "k_mcc_select");
+        LEPIC_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, nsel, sizeof(int64_t), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+        LEPIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        const int64_t m = ctx->h_pinned[0];
+        if (m > cap) {   // an unlikely tail: list again with room for all (same draws)
+          cap = m;
+          continue;
+        }
+        if (m > 0)
+          hipLaunchKernelGGL(k_mcc_trials, dim3(grid_blocks(m, BLOCK)), dim3(BLOCK), 0,
+                             ctx->stream, kg, P, K, np_red, nodes, t, list, nsel, cap, philox_key,
+                             philox_counter, dc, Pcoll);
+        LEPIC_LAUNCH_CHECK(ctx, "k_mcc_trials");
+        break;
+      }
     }
   } else {
     return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions: unknown mode");

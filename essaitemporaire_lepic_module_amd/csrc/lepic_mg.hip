@@ -77,12 +77,9 @@ __global__ void k_reset_totals(MGCtl *ctl, int n) {
 
 // One colour of sor_rb (sors.f90:87-140 red, 249-302 black).
 // colour 0 = red = (i+j+k) odd, 1 = black.  Threads over (i2, j, k).
-__global__ __launch_bounds__(BLOCK) void k_sor(KLevel L, int colour, double omega,
-                                               const MGCtl *ctl) {
-  if (ctl->active == 0) return;
+__device__ inline void sor_node(const KLevel &L, int colour, double omega, int64_t t) {
   const KGrid &g = L.g;
   const int ni2 = g.n1 / 2 + 1;
-  int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
   int64_t total = (int64_t)ni2 * (g.n2 + 1) * (g.n3 + 1);
   if (t >= total) return;
   int i2 = (int)(t % ni2);
@@ -118,12 +115,16 @@ __global__ __launch_bounds__(BLOCK) void k_sor(KLevel L, int colour, double omeg
              L.ac;
 }
 
+__global__ __launch_bounds__(BLOCK) void k_sor(KLevel L, int colour, double omega,
+                                               const MGCtl *ctl) {
+  if (ctl->active == 0) return;
+  sor_node(L, colour, omega, blockIdx.x * (int64_t)BLOCK + threadIdx.x);
+}
+
 // Periodic ghost copies (sors.f90:184-239, single rank): y planes for every
 // (i, k), then z planes for every (i, j).
-__global__ void k_per_y(KLevel L, const MGCtl *ctl) {
-  if (ctl->active == 0) return;
+__device__ inline void per_y_node(const KLevel &L, int64_t t) {
   const KGrid &g = L.g;
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int64_t ni = g.n1 + 3, nk = g.n3 + 3;
   if (t >= ni * nk) return;
   int i = (int)(t % ni), k = (int)(t / ni);
@@ -132,10 +133,8 @@ __global__ void k_per_y(KLevel L, const MGCtl *ctl) {
   if (b2 == 0 || b2 == -2) L.u[gidx(g, i, g.n2 + 1, k)] = L.u[gidx(g, i, 1, k)];
 }
 
-__global__ void k_per_z(KLevel L, const MGCtl *ctl) {
-  if (ctl->active == 0) return;
+__device__ inline void per_z_node(const KLevel &L, int64_t t) {
   const KGrid &g = L.g;
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int64_t ni = g.n1 + 3, nj = g.n2 + 3;
   if (t >= ni * nj) return;
   int i = (int)(t % ni), j = (int)(t / ni);
@@ -145,12 +144,20 @@ __global__ void k_per_z(KLevel L, const MGCtl *ctl) {
   if (b2 == 0 || b2 == -2) L.u[gidx(g, i, j, m + 1)] = L.u[gidx(g, i, j, 1)];
 }
 
-// L1 residual over bcnd == -1 nodes (sors.f90:409-431), block partials.
-__global__ __launch_bounds__(BLOCK) void k_resid(KLevel L, double *partials, const MGCtl *ctl) {
+__global__ void k_per_y(KLevel L, const MGCtl *ctl) {
   if (ctl->active == 0) return;
-  __shared__ double red[BLOCK];
+  per_y_node(L, blockIdx.x * (int64_t)blockDim.x + threadIdx.x);
+}
+
+__global__ void k_per_z(KLevel L, const MGCtl *ctl) {
+  if (ctl->active == 0) return;
+  per_z_node(L, blockIdx.x * (int64_t)blockDim.x + threadIdx.x);
+}
+
+// L1 residual over bcnd == -1 nodes (sors.f90:409-431), block partials.
+// |b - A u| at interior node t of the residual range (0 outside plasma).
+__device__ inline double resid_node(const KLevel &L, int64_t t) {
   const KGrid &g = L.g;
-  int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
   int64_t nx = g.n1 + 1, ny = g.n2 + 1, nz = g.n3 + 1;
   double a = 0.0;
   if (t < nx * ny * nz) {
@@ -167,6 +174,13 @@ __global__ __launch_bounds__(BLOCK) void k_resid(KLevel L, double *partials, con
       a = fabs(rij);
     }
   }
+  return a;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_resid(KLevel L, double *partials, const MGCtl *ctl) {
+  if (ctl->active == 0) return;
+  __shared__ double red[BLOCK];
+  const double a = resid_node(L, blockIdx.x * (int64_t)BLOCK + threadIdx.x);
   red[threadIdx.x] = a;
   __syncthreads();
   for (int s = BLOCK / 2; s > 0; s >>= 1) {
@@ -209,9 +223,8 @@ __global__ __launch_bounds__(BLOCK) void k_decide(const double *partials, int64_
 // winning; here one thread per entry finds that last writer (largest jr,
 // then largest ir) and applies its branch: getres when bcnd(ir,jr,m+1) <= 0,
 // else 0.  The two branches guard the (ir, jr-1) entry differently.
-__global__ void k_rpbc(KLevel F, double *rp, int flag_nmn) {
+__device__ inline void rpbc_entry(const KLevel &F, double *rp, int flag_nmn, int64_t t) {
   const KGrid &g = F.g;
-  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t rs = g.n1 + 2;
   const int64_t total = rs * (g.n2 + 2);
   if (t >= total) return;
@@ -242,15 +255,18 @@ __global__ void k_rpbc(KLevel F, double *rp, int flag_nmn) {
   rp[t] = val;
 }
 
+__global__ void k_rpbc(KLevel F, double *rp, int flag_nmn) {
+  rpbc_entry(F, rp, flag_nmn, blockIdx.x * (int64_t)blockDim.x + threadIdx.x);
+}
+
 // Restriction (mg.f90:413-718): coarse labels injected from the fine grid and
 // the 27-point full-weighting residual with the reference's neighbour rules.
-__global__ __launch_bounds__(BLOCK) void k_restrict(KLevel F, KLevel C, const double *rp,
-                                                    int flag_nmn) {
+__device__ inline void restrict_node(const KLevel &F, const KLevel &C, const double *rp,
+                                     int flag_nmn, int64_t t) {
   const KGrid &g = F.g;
   const KGrid &gc = C.g;
   const int m = g.n3;
   const int kp = m / 2 + 1;
-  int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
   int64_t ni = g.n1 / 2 + 1, nj = g.n2 / 2 + 1;
   if (t >= ni * nj * kp) return;
   int i = 1 + (int)(t % ni);
@@ -374,13 +390,17 @@ __global__ __launch_bounds__(BLOCK) void k_restrict(KLevel F, KLevel C, const do
   C.b[gidx(gc, i, j, k)] = re;
 }
 
+__global__ __launch_bounds__(BLOCK) void k_restrict(KLevel F, KLevel C, const double *rp,
+                                                    int flag_nmn) {
+  restrict_node(F, C, rp, flag_nmn, blockIdx.x * (int64_t)BLOCK + threadIdx.x);
+}
+
 // Prolongation (mg.f90:886-935): one thread per coarse node updates its 8
 // fine nodes (disjoint between threads).
-__global__ __launch_bounds__(BLOCK) void k_prolong(KLevel F, KLevel C) {
+__device__ inline void prolong_node(const KLevel &F, const KLevel &C, int64_t t) {
   const KGrid &g = F.g;
   const KGrid &gc = C.g;
   const int kp = g.n3 / 2 + 1;
-  int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
   int64_t ni = g.n1 / 2 + 1, nj = g.n2 / 2 + 1;
   if (t >= ni * nj * kp) return;
   int i = 1 + (int)(t % ni);
@@ -414,6 +434,129 @@ __global__ __launch_bounds__(BLOCK) void k_prolong(KLevel F, KLevel C) {
   if (bc[c] <= 0) u[c] = u[c] + (r000 + r00m + rm0m + rm00 + r0m0 + r0mm + rmmm + rmm0) / 8.0;
 }
 
+__global__ __launch_bounds__(BLOCK) void k_prolong(KLevel F, KLevel C) {
+  prolong_node(F, C, blockIdx.x * (int64_t)BLOCK + threadIdx.x);
+}
+
+// --------------------------------------------------------------------------
+// Coarse levels in one workgroup.  Below a few thousand nodes a level's
+// sweep is far shorter than a kernel dispatch, so the part of a V-cycle that
+// only touches levels lc..ng-1 (sor_rb down from lc, the coarsest solve and
+// sor_rb back up to lc, with the restrictions and prolongations between
+// them) runs as one launch: the same per-node routines as the multi-block
+// kernels, phases separated by workgroup barriers, the exit rules of sor_rb
+// evaluated in the workgroup.
+// --------------------------------------------------------------------------
+constexpr int CBLOCK = 1024;
+constexpr int64_t COARSE_MAX_NODES = 12000;
+
+struct CoarseArgs {
+  KLevel L[MAX_LEVELS];
+  int lc, ng;
+  double omega, eps;
+  int flag_pbc, flag_nmn;
+  int first_cycle;
+  double *rp;
+  MGCtl *ctl;
+};
+
+__device__ inline double wg_sum(double a, double *red) {
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int s2 = CBLOCK / 2; s2 > 0; s2 >>= 1) {
+    if ((int)threadIdx.x < s2) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s2];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// sor_rb (sors.f90:1-535) on level l: at most nr sweeps, exit rules as k_decide.
+__device__ void wg_sor_rb(const CoarseArgs &A, int l, int dig, int64_t nr, double *red) {
+  const KLevel &Lv = A.L[l];
+  const KGrid &g = Lv.g;
+  const int64_t nsor = (int64_t)(g.n1 / 2 + 1) * (g.n2 + 1) * (g.n3 + 1);
+  const int64_t nres = (int64_t)(g.n1 + 1) * (g.n2 + 1) * (g.n3 + 1);
+  const int64_t npy = (int64_t)(g.n1 + 3) * (g.n3 + 3), npz = (int64_t)(g.n1 + 3) * (g.n2 + 3);
+  int64_t ksor = 0;
+  double res = 0.0, res_s = 0.0;
+  for (;;) {
+    for (int colour = 0; colour < 2; ++colour) {
+      for (int64_t t = threadIdx.x; t < nsor; t += CBLOCK) sor_node(Lv, colour, A.omega, t);
+      __syncthreads();
+      if (A.flag_pbc) {
+        for (int64_t t = threadIdx.x; t < npy; t += CBLOCK) per_y_node(Lv, t);
+        __syncthreads();
+        for (int64_t t = threadIdx.x; t < npz; t += CBLOCK) per_z_node(Lv, t);
+        __syncthreads();
+      }
+    }
+    double a = 0.0;
+    for (int64_t t = threadIdx.x; t < nres; t += CBLOCK) a = a + resid_node(Lv, t);
+    res = wg_sum(a, red);
+    ++ksor;
+    bool stop;
+    if (dig == 0) stop = (res <= A.eps);
+    else stop = (res_s > 0.0 && (res / res_s) <= 0.6);
+    res_s = res;
+    if (stop || ksor >= nr) break;
+  }
+  if (threadIdx.x == 0) {
+    MGCtl *c = &A.ctl[l];
+    c->active = 0;
+    c->ksor = ksor;
+    c->res = res;
+    c->res_s = res_s;
+    c->total += ksor;
+  }
+}
+
+__device__ void wg_restriction(const CoarseArgs &A, int l) {
+  const KLevel &F = A.L[l];
+  const KLevel &C = A.L[l + 1];
+  const KGrid &g = F.g;
+  const int64_t nc = ((int64_t)C.g.n1 + 3) * ((int64_t)C.g.n2 + 3) * ((int64_t)C.g.n3 + 3);
+  for (int64_t t = threadIdx.x; t < nc; t += CBLOCK) C.u[t] = 0.0;
+  const double *rpp = nullptr;
+  if (A.flag_pbc) {
+    const int64_t nent = ((int64_t)g.n1 + 2) * ((int64_t)g.n2 + 2);
+    for (int64_t t = threadIdx.x; t < nent; t += CBLOCK) rpbc_entry(F, A.rp, A.flag_nmn, t);
+    rpp = A.rp;
+  }
+  __syncthreads();
+  const int64_t nt = (int64_t)(g.n1 / 2 + 1) * (g.n2 / 2 + 1) * (g.n3 / 2 + 1);
+  for (int64_t t = threadIdx.x; t < nt; t += CBLOCK) restrict_node(F, C, rpp, A.flag_nmn, t);
+  __syncthreads();
+}
+
+__device__ void wg_prolongation(const CoarseArgs &A, int l) {  // L[l] += P(L[l+1])
+  const KGrid &g = A.L[l].g;
+  const int64_t nt = (int64_t)(g.n1 / 2 + 1) * (g.n2 / 2 + 1) * (g.n3 / 2 + 1);
+  for (int64_t t = threadIdx.x; t < nt; t += CBLOCK) prolong_node(A.L[l], A.L[l + 1], t);
+  __syncthreads();
+}
+
+// The V-cycle of mg (mg.f90:1-246) restricted to levels lc..ng-1; the
+// prolongation into level lc-1 is left to the caller.
+__global__ __launch_bounds__(CBLOCK) void k_mg_coarse(CoarseArgs A) {
+  __shared__ double red[CBLOCK];
+  const int ng = A.ng;
+  for (int l = A.lc; l <= ng - 2; ++l) {
+    wg_sor_rb(A, l, l + 1 - ng, 4, red);
+    __syncthreads();
+    wg_restriction(A, l);
+  }
+  wg_sor_rb(A, ng - 1, 0, A.first_cycle ? 1000000 : 4, red);
+  __syncthreads();
+  if (ng - 2 >= A.lc) wg_prolongation(A, ng - 2);
+  for (int l = ng - 2; l >= A.lc; --l) {
+    wg_sor_rb(A, l, ng - 1 - l, 4, red);
+    __syncthreads();
+    if (l - 1 >= A.lc) wg_prolongation(A, l - 1);
+  }
+}
+
 __global__ void k_fill_d(double *a, int64_t n, double v) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
@@ -428,6 +571,22 @@ struct Solver {
   double *partials;
   double *rp;
   MGCtl *ctl;  // device, MAX_LEVELS entries
+  int lc;      // first level handled by k_mg_coarse (ng: none)
+
+  void coarse(double eps, int kmg) {
+    CoarseArgs A;
+    for (int l = 0; l < MAX_LEVELS; ++l) A.L[l] = L[l];
+    A.lc = lc;
+    A.ng = ng;
+    A.omega = omega;
+    A.eps = eps;
+    A.flag_pbc = flag_pbc;
+    A.flag_nmn = flag_nmn;
+    A.first_cycle = kmg == 1 ? 1 : 0;
+    A.rp = rp;
+    A.ctl = ctl;
+    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(CBLOCK), 0, ctx->stream, A);
+  }
 
   void sor_sweep(int l, double eps, int dig, int64_t nr) {
     KLevel &Lv = L[l];
@@ -596,21 +755,45 @@ extern "C" int lepic_poisson_solve(lepic_ctx *ctx, const lepic_grid *g, double *
   double res = 0.0, res_s = 0.0;
   int k = 1;
   int status = LEPIC_OK;
+  // Levels lc..ng-1 of every V-cycle run inside k_mg_coarse.  The sequence of
+  // operations is mg.f90's; the stretch from the down-leg sor_rb on lc to the
+  // up-leg sor_rb on lc touches no level above lc and is issued as one launch.
+  S.lc = ng;
+  for (int l = 1; l < ng; ++l)
+    if (Solver::grid_nodes_k(S.L[l].g) <= COARSE_MAX_NODES) {
+      S.lc = l;
+      break;
+    }
   for (;;) {
     // mg (mg.f90:99-239)
     for (int ig = 1; ig <= 2 * ng - 1; ++ig) {
       for (int il = 1; il <= ng - 1 && il <= 11; ++il) {
-        if ((ig < ng && ig == il) || (ig > ng && ig == 2 * ng - il)) {
-          int st = S.sor_rb(il - 1, epsa, k, ig - ng);
-          if (st) return st;
-          if (ig < ng) S.restriction(il - 1);
-          if (ig > ng && il >= 2) S.prolongation(il - 2);
+        const int l = il - 1;
+        if (ig < ng && ig == il) {
+          if (l < S.lc) {
+            int st = S.sor_rb(l, epsa, k, ig - ng);
+            if (st) return st;
+            S.restriction(l);
+          } else if (l == S.lc) {
+            S.coarse(epsa, k);
+          }
+        }
+        if (ig > ng && ig == 2 * ng - il) {
+          if (l < S.lc) {
+            int st = S.sor_rb(l, epsa, k, ig - ng);
+            if (st) return st;
+          }
+          if (l <= S.lc && il >= 2) S.prolongation(il - 2);
         }
       }
       if (ig == ng) {
-        int st = S.sor_rb(ng - 1, epsa, k, 0);
-        if (st) return st;
-        if (ng >= 2) S.prolongation(ng - 2);
+        if (S.lc >= ng) {
+          int st = S.sor_rb(ng - 1, epsa, k, 0);
+          if (st) return st;
+        } else if (S.lc == ng - 1) {
+          S.coarse(epsa, k);
+        }
+        if (ng >= 2 && ng - 2 < S.lc) S.prolongation(ng - 2);
       }
     }
     LEPIC_LAUNCH_CHECK(ctx, "multigrid launch");

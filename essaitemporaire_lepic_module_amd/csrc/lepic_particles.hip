@@ -96,9 +96,9 @@ __global__ __launch_bounds__(BLOCK) void k_move_boris(KGrid g, KPset p, int64_t 
 // --------------------------------------------------------------------------
 __device__ inline int wall_test(const KGrid &g, const int *__restrict__ bcnd, const KBC &bp,
                                 double xp, double yp, double zp, int &label) {
-  int ix = (int)floor(xp / g.h1) + 1;
-  int iy = (int)floor(yp / g.h2) + 1;
-  int iz = (int)floor(zp / g.h3) + 1;
+  int ix = (int)floor(qdiv(xp, g.q1)) + 1;
+  int iy = (int)floor(qdiv(yp, g.q2)) + 1;
+  int iz = (int)floor(qdiv(zp, g.q3)) + 1;
   if (ix < 0) ix = 0;
   if (ix > g.n1 + 1) ix = g.n1 + 1;
   if (iy < 0) iy = 0;
@@ -277,12 +277,12 @@ __device__ inline bool deposit_weights(const KGrid &g, double k1, double x, doub
   if (!(x >= -1.0e-12 && x <= xmax + 1.0e-12)) return false;
   if (!(y >= -1.0e-12 && y <= ymax + 1.0e-12)) return false;
   if (!(z >= -1.0e-12 && z <= zmax + 1.0e-12)) return false;
-  ix = (int)(x / g.h1) + 1;
-  iy = (int)(y / g.h2) + 1;
-  iz = (int)(z / g.h3) + 1;
-  double px = ((double)ix * g.h1 - x) / g.h1;
-  double py = ((double)iy * g.h2 - y) / g.h2;
-  double pz = ((double)iz * g.h3 - z) / g.h3;
+  ix = (int)qdiv(x, g.q1) + 1;
+  iy = (int)qdiv(y, g.q2) + 1;
+  iz = (int)qdiv(z, g.q3) + 1;
+  double px = qdiv((double)ix * g.h1 - x, g.q1);
+  double py = qdiv((double)iy * g.h2 - y, g.q2);
+  double pz = qdiv((double)iz * g.h3 - z, g.q3);
   double wx2 = 1.0 - px, wy2 = 1.0 - py, wz2 = 1.0 - pz;
   w[0] = k1 * px * py * pz;
   w[1] = k1 * wx2 * py * pz;
@@ -422,9 +422,9 @@ __device__ inline void box_flush(const KGrid &g, DepShared &ds, const double *__
 }
 
 __device__ inline void cell_of(const KGrid &g, double x, double y, double z, int c[3]) {
-  c[0] = imax(0, imin(g.n1 + 1, (int)(x / g.h1) + 1));
-  c[1] = imax(0, imin(g.n2 + 1, (int)(y / g.h2) + 1));
-  c[2] = imax(0, imin(g.n3 + 1, (int)(z / g.h3) + 1));
+  c[0] = imax(0, imin(g.n1 + 1, (int)qdiv(x, g.q1) + 1));
+  c[1] = imax(0, imin(g.n2 + 1, (int)qdiv(y, g.q2) + 1));
+  c[2] = imax(0, imin(g.n3 + 1, (int)qdiv(z, g.q3) + 1));
 }
 
 constexpr int PPT = 4;              // particles per thread in the streaming kernels
@@ -474,10 +474,183 @@ __global__ void k_rep_flush(double *np, double *rep, int64_t nodes) {
   np[i] = np[i] + s;
 }
 
+// --------------------------------------------------------------------------
+// Transposed deposition.  Sorted particles come in runs of one cell, but a
+// wave reads them lane by lane, so a run spans lanes and summing it takes a
+// cross-lane scan.  Instead each wave stages its particles' box-local node
+// and fractions (px, py, pz of deposit_particle_set_to_np_thread) in LDS in
+// read order and lane l then deposits particles PPT*l .. PPT*l+PPT-1: a run
+// of one cell is summed in the lane's registers and reaches the LDS box with
+// one set of 8 atomics.
+// --------------------------------------------------------------------------
+struct WaveStage {
+  int key[LEPIC_WAVE * PPT];   // box-local lower node, -1: nothing to deposit
+  double px[LEPIC_WAVE * PPT], py[LEPIC_WAVE * PPT], pz[LEPIC_WAVE * PPT];
+};
+
+// Fractions of deposit_weights (same range guard, same arithmetic).
+__device__ inline bool deposit_fracs(const KGrid &g, double x, double y, double z, int &ix,
+                                     int &iy, int &iz, double &px, double &py, double &pz) {
+  const double xmax = (double)g.n1 * g.h1, ymax = (double)g.n2 * g.h2,
+               zmax = (double)g.n3 * g.h3;
+  if (!(x >= -1.0e-12 && x <= xmax + 1.0e-12)) return false;
+  if (!(y >= -1.0e-12 && y <= ymax + 1.0e-12)) return false;
+  if (!(z >= -1.0e-12 && z <= zmax + 1.0e-12)) return false;
+  ix = (int)qdiv(x, g.q1) + 1;
+  iy = (int)qdiv(y, g.q2) + 1;
+  iz = (int)qdiv(z, g.q3) + 1;
+  px = qdiv((double)ix * g.h1 - x, g.q1);
+  py = qdiv((double)iy * g.h2 - y, g.q2);
+  pz = qdiv((double)iz * g.h3 - z, g.q3);
+  return true;
+}
+
+__device__ inline void box_atomic8(DepShared &ds, int l, const double a[8]) {
+  const int bx = ds.bx, bxy = ds.bx * ds.by;
+  const int ld[8] = {l, l + 1, l + bx + 1, l + bx, l + bxy, l + bxy + 1, l + bxy + bx + 1,
+                     l + bxy + bx};
+#pragma unroll
+  for (int q = 0; q < 8; ++q) atomicAdd(&ds.box[ld[q]], a[q]);
+}
+
+// Called by every lane of the wave after it staged all PPT rounds.
+__device__ inline void wave_deposit_transposed(const WaveStage &st, DepShared &ds, double k1) {
+  const int lane = __lane_id();
+  double acc[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) acc[q] = 0.0;
+  int cur = -1;
+#pragma unroll
+  for (int r = 0; r < PPT; ++r) {
+    const int o = lane * PPT + r;
+    const int key = st.key[o];
+    if (key != cur) {
+      if (cur >= 0) box_atomic8(ds, cur, acc);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) acc[q] = 0.0;
+      cur = key;
+    }
+    if (key >= 0) {
+      const double px = st.px[o], py = st.py[o], pz = st.pz[o];
+      const double wx2 = 1.0 - px, wy2 = 1.0 - py, wz2 = 1.0 - pz;
+      acc[0] += k1 * px * py * pz;
+      acc[1] += k1 * wx2 * py * pz;
+      acc[2] += k1 * wx2 * wy2 * pz;
+      acc[3] += k1 * px * wy2 * pz;
+      acc[4] += k1 * px * py * wz2;
+      acc[5] += k1 * wx2 * py * wz2;
+      acc[6] += k1 * wx2 * wy2 * wz2;
+      acc[7] += k1 * px * wy2 * wz2;
+    }
+  }
+  if (cur >= 0) box_atomic8(ds, cur, acc);
+}
+
 // Production particle phase, in place: gather + push + wall BC (+ deposit).
 // Lost particles become holes (dead = 2) instead of being compacted; the
 // next sort drops holes, keeping the survivors' relative order, which is the
-// order the reference's per-step stable compaction produces.
+// order the reference's per-step stable compaction produces.  Wave w of a
+// block owns particles [w*64*PPT, (w+1)*64*PPT) of the block's tile and reads
+// them in PPT coalesced rounds of 64.
+__global__ __launch_bounds__(BLOCK) void k_push(
+    KGrid g, KPset p, int64_t n, const double4 *__restrict__ E4, KBField bf, double k1, double dt,
+    const int *__restrict__ bcnd, KBC bp, const double *__restrict__ kq, double kdep,
+    double *rep_base, int64_t nodes, int do_dep, double *p_mac, double *P_loss,
+    unsigned long long *lost_ctr) {
+  __shared__ DepShared ds;
+  __shared__ WaveStage stage[NWAVES];
+  double *rep = rep_base ? rep_base + (int64_t)xcc_id() * nodes : nullptr;
+  const int lane = __lane_id();
+  const int wave = threadIdx.x / LEPIC_WAVE;
+  WaveStage &st = stage[wave];
+  const int64_t base = (int64_t)blockIdx.x * TILE + (int64_t)wave * (LEPIC_WAVE * PPT) + lane;
+  double xs[PPT], ys[PPT], zs[PPT], vxs[PPT], vys[PPT], vzs[PPT];
+  bool live[PPT];
+  int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+#pragma unroll
+  for (int r = 0; r < PPT; ++r) {
+    const int64_t i = base + (int64_t)r * LEPIC_WAVE;
+    live[r] = (i < n) && p.dead[i] == 0;
+    if (live[r]) {
+      xs[r] = p.x[i]; ys[r] = p.y[i]; zs[r] = p.z[i];
+      vxs[r] = p.vx[i]; vys[r] = p.vy[i]; vzs[r] = p.vz[i];
+      if (do_dep) {
+        int c[3];
+        cell_of(g, xs[r], ys[r], zs[r], c);
+        for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], c[d]); hi[d] = max(hi[d], c[d]); }
+      }
+    }
+  }
+  if (do_dep) box_begin(g, ds, lo, hi);
+  int lost_local = 0;
+#pragma unroll
+  for (int r = 0; r < PPT; ++r) {
+    const int64_t i = base + (int64_t)r * LEPIC_WAVE;
+    int key = -1;
+    double px = 0.0, py = 0.0, pz = 0.0;
+    if (live[r]) {
+      double xp = xs[r], yp = ys[r], zp = zs[r];
+      double vx = vxs[r], vy = vys[r], vz = vzs[r];
+      Stencil s;
+      make_stencil(g, xp, yp, zp, s);
+      double Ex, Ey, Ez;
+      gather3(E4, g, s, Ex, Ey, Ez);
+      velocity_update(g, bf, s, k1, Ex, Ey, Ez, vx, vy, vz);
+      xp = xp + dt * vx; yp = yp + dt * vy; zp = zp + dt * vz;
+      int label;
+      if (wall_test(g, bcnd, bp, xp, yp, zp, label)) {
+        wall_diagnostics(bp, label, vx, vy, vz, p_mac, P_loss);
+        p.dead[i] = 2;
+        ++lost_local;
+      } else {
+        survivor_wrap(bp, xp, yp, zp, vx);
+        p.x[i] = xp; p.y[i] = yp; p.z[i] = zp;
+        p.vx[i] = vx; p.vy[i] = vy; p.vz[i] = vz;
+        int ix, iy, iz;
+        if (do_dep && deposit_fracs(g, xp, yp, zp, ix, iy, iz, px, py, pz)) {
+          int l = -1;
+          if (ds.vol > 0) {
+            const int lx = ix - ds.x0, ly = iy - ds.y0, lz = iz - ds.z0;
+            if (lx >= 0 && ly >= 0 && lz >= 0 && lx + 1 < ds.bx && ly + 1 < ds.by && lz + 1 < ds.bz)
+              l = lx + ds.bx * (ly + ds.by * lz);
+          }
+          if (l >= 0) {
+            key = l;
+          } else {  // stencil outside the box: straight to this XCD's replica
+            const double wx2 = 1.0 - px, wy2 = 1.0 - py, wz2 = 1.0 - pz;
+            const double w[8] = {kdep * px * py * pz,   kdep * wx2 * py * pz,
+                                 kdep * wx2 * wy2 * pz, kdep * px * wy2 * pz,
+                                 kdep * px * py * wz2,  kdep * wx2 * py * wz2,
+                                 kdep * wx2 * wy2 * wz2, kdep * px * wy2 * wz2};
+            const int64_t b = gidx(g, ix, iy, iz);
+            const int64_t nd[8] = {b, b + 1, b + g.sy + 1, b + g.sy, b + g.sz, b + g.sz + 1,
+                                   b + g.sz + g.sy + 1, b + g.sz + g.sy};
+#pragma unroll
+            for (int q = 0; q < 8; ++q) l2_atomic_add(&rep[nd[q]], kq[nd[q]] * w[q]);
+          }
+        }
+      }
+    }
+    if (do_dep) {
+      const int o = r * LEPIC_WAVE + lane;
+      st.key[o] = key;
+      st.px[o] = px; st.py[o] = py; st.pz[o] = pz;
+    }
+  }
+  if (do_dep) {
+    // the stage is private to this wave: order its writes before the reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_deposit_transposed(st, ds, kdep);
+    box_flush(g, ds, kq, rep);
+  }
+  // losses are rare: one atomic per wave that saw any
+  int tot = lost_local;
+  for (int d = 32; d > 0; d >>= 1) tot += __shfl_xor(tot, d);
+  if (__lane_id() == 0 && tot > 0) atomicAdd(lost_ctr, (unsigned long long)tot);
+}
+
 __global__ __launch_bounds__(BLOCK) void k_push_inplace(
     KGrid g, KPset p, int64_t n, const double4 *__restrict__ E4, KBField bf, double k1, double dt,
     const int *__restrict__ bcnd, KBC bp, const double *__restrict__ kq, double kdep,
@@ -617,9 +790,9 @@ __global__ __launch_bounds__(BLOCK) void k_cell_keys(KGrid g, KPset p, int64_t n
       return;
     }
   }
-  int ix = (int)(p.x[i] / g.h1) + 1;
-  int iy = (int)(p.y[i] / g.h2) + 1;
-  int iz = (int)(p.z[i] / g.h3) + 1;
+  int ix = (int)qdiv(p.x[i], g.q1) + 1;
+  int iy = (int)qdiv(p.y[i], g.q2) + 1;
+  int iz = (int)qdiv(p.z[i], g.q3) + 1;
   ix = imax(1, imin(g.n1, ix));
   iy = imax(1, imin(g.n2, iy));
   iz = imax(1, imin(g.n3, iz));
@@ -700,6 +873,8 @@ int check_grid(lepic_ctx *ctx, const lepic_grid *g) {
   if (!g || g->n[0] < 1 || g->n[1] < 1 || g->n[2] < 1 || !(g->h[0] > 0) || !(g->h[1] > 0) ||
       !(g->h[2] > 0))
     return lepic_fail(ctx, LEPIC_ERR_ARG, "bad grid");
+  if (!qdiv_divisor_ok(g->h[0]) || !qdiv_divisor_ok(g->h[1]) || !qdiv_divisor_ok(g->h[2]))
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "grid step outside [1e-200, 1e200]");
   return LEPIC_OK;
 }
 
@@ -788,7 +963,7 @@ int lepic_move_boris(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *p, const d
   double k1 = dt * q / (2.0 * m);
   hipLaunchKernelGGL(k_move_boris, dim3(grid_blocks(p->n, BLOCK)), dim3(BLOCK), 0, ctx->stream,
                      make_kgrid(g), kp(p), p->n, reinterpret_cast<const double4 *>(E4),
-                     make_kbfield(bf), k1, dt);
+                     make_kbfield(bf, k1), k1, dt);
   LEPIC_LAUNCH_CHECK(ctx, "k_move_boris");
   return LEPIC_OK;
 }
@@ -865,11 +1040,11 @@ int lepic_push_bc_deposit_inplace(lepic_ctx *ctx, const lepic_grid *g, lepic_pse
   if (p->n > 0) {
     double k1 = dt * q / (2.0 * m);
     double kdep = np_local ? bp->Nm / (g->h[0] * g->h[1] * g->h[2]) : 0.0;
-    hipLaunchKernelGGL(k_push_inplace, dim3(grid_blocks(p->n, TILE)), dim3(BLOCK), 0, ctx->stream,
+    hipLaunchKernelGGL(k_push, dim3(grid_blocks(p->n, TILE)), dim3(BLOCK), 0, ctx->stream,
                        make_kgrid(g), kp(p), p->n, reinterpret_cast<const double4 *>(E4),
-                       make_kbfield(bf), k1, dt, bcnd, kbc(bp), kq, kdep, rep, nodes,
+                       make_kbfield(bf, k1), k1, dt, bcnd, kbc(bp), kq, kdep, rep, nodes,
                        np_local ? 1 : 0, p_mac, P_loss, lost);
-    LEPIC_LAUNCH_CHECK(ctx, "k_push_inplace");
+    LEPIC_LAUNCH_CHECK(ctx, "k_push");
   }
   if (np_local && (st = lepic_replica_flush(ctx, nodes, np_local))) return st;
   if (np_lost) {
@@ -902,8 +1077,8 @@ int lepic_push_bc_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset 
   int64_t n = src->n, nout = 0;
   KGrid kg = make_kgrid(g);
   KBC kb = kbc(bp);
-  KBField kbf = make_kbfield(bf);
   double k1 = dt * q / (2.0 * m);
+  KBField kbf = make_kbfield(bf, k1);
   double kdep = np_local ? bp->Nm / (g->h[0] * g->h[1] * g->h[2]) : 0.0;
   const int64_t nodes = grid_nodes(g);
   double *rep = nullptr;

@@ -195,11 +195,19 @@ def test_efield_bitexact(gpu, oracle, rng, bmaker):
     _same(E_g.cpu().numpy(), oracle.efield(n, h, phi, bc))
 
 
-@pytest.mark.parametrize("bmaker,pbc,ng", [(cfg.grounded_box_bcnd, 0, 4), (H.two_electrode_bcnd, 0, 3),
-                                           (H.periodic_yz_bcnd, 1, 3), (cfg.grounded_box_bcnd, 0, 1)])
-def test_poisson_bitexact(gpu, oracle, rng, bmaker, pbc, ng):
+# The solver runs the levels below ~12000 nodes inside one workgroup: the 16^3
+# cases put every level but the finest there, 64^3 and 128^3 keep two and
+# three levels on the multi-block path, ng = 1 has no coarse part at all.
+@pytest.mark.parametrize("bmaker,pbc,ng,size", [(cfg.grounded_box_bcnd, 0, 4, 16),
+                                                (H.two_electrode_bcnd, 0, 3, 16),
+                                                (H.periodic_yz_bcnd, 1, 3, 16),
+                                                (cfg.grounded_box_bcnd, 0, 1, 8),
+                                                (H.periodic_yz_bcnd, 1, 4, 32),
+                                                (H.two_electrode_bcnd, 0, 5, 64),
+                                                (cfg.grounded_box_bcnd, 0, 7, 128)])
+def test_poisson_bitexact(gpu, oracle, rng, bmaker, pbc, ng, size):
     from essaitemporaire_lepic_module_amd import fields as F
-    n, h = H.box((16, 16, 16) if ng > 1 else (8, 8, 8))
+    n, h = H.box((size, size, size))
     bc = bmaker(n)
     rho = np.zeros(cfg.node_shape(n))
     rho[bc <= 0] = 1e-4 * rng.standard_normal(int((bc <= 0).sum()))
@@ -244,11 +252,16 @@ def test_loader_host_bitexact(oracle):
 
 
 @pytest.mark.parametrize("mode", [0, 1])
-def test_fused_inplace_and_sort_compact(gpu, oracle, rng, mode):
+@pytest.mark.parametrize("presort,size", [(True, (12, 10, 8)), (False, (12, 10, 8)),
+                                          (False, (20, 20, 20))])
+def test_fused_inplace_and_sort_compact(gpu, oracle, rng, mode, presort, size):
     """Production particle phase: in-place push + wall BC + deposit with holes,
-    then the compacting sort, against the oracle's mover + BC + deposit + sort."""
+    then the compacting sort, against the oracle's mover + BC + deposit + sort.
+    Sorted input deposits through the LDS box in runs of one cell; unsorted
+    input has no runs, and on the 20^3 grid a block's particles span more
+    nodes than the box holds, so every stencil goes straight to global memory."""
     from essaitemporaire_lepic_module_amd import particles as P
-    n, h = H.box((12, 10, 8))
+    n, h = H.box(size)
     bc = H.two_electrode_bcnd(n)
     kq = cfg.build_kq(bc)
     count = 30000
@@ -257,9 +270,9 @@ def test_fused_inplace_and_sort_compact(gpu, oracle, rng, mode):
     E4 = H.random_E4(rng, n)
     q, m, dt, Nm = -1.60217646e-19, 9.10938188e-31, 5e-11, 1526.0
     Bc = (0.0, 0.0, 0.01)
-    # sorted input, as in production (the deposit box path is then exercised)
     po = _oracle_set(oracle, arr, dead=dead)
-    oracle.sort_particles(n, h, po)
+    if presort:
+        oracle.sort_particles(n, h, po)
     start = po.arrays()
     ps = _gpu_set(P, [start[k] for k in ("x", "y", "z", "vx", "vy", "vz")], dead=start["dead"])
     bf = P.bfield(mode, Bc)
