@@ -141,7 +141,7 @@ def load() -> C.CDLL:
                                                 vp, vp, vp, vp, C.POINTER(C.c_int64)]),
             "lepic_push_bc_deposit_inplace": (C.c_int, [vp, G, PS, vp, C.POINTER(BField),
                                                         C.c_double, C.c_double, C.c_double, vp,
-                                                        C.POINTER(BCParams), vp, vp, vp, vp,
+                                                        C.POINTER(BCParams), vp, vp, vp, vp, vp,
                                                         C.POINTER(C.c_int64)]),
             "lepic_sort_compact": (C.c_int, [vp, G, PS, PS]),
         }

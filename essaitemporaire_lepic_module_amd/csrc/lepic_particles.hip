@@ -430,50 +430,6 @@ __device__ inline void cell_of(const KGrid &g, double x, double y, double z, int
 constexpr int PPT = 4;              // particles per thread in the streaming kernels
 constexpr int TILE = BLOCK * PPT;
 
-__global__ __launch_bounds__(BLOCK) void k_deposit(KGrid g, KPset p, int64_t n,
-                                                   const double *__restrict__ kq, double k1,
-                                                   double *rep_base, int64_t nodes) {
-  __shared__ DepShared ds;
-  double *rep = rep_base + (int64_t)xcc_id() * nodes;
-  const int64_t base = (int64_t)blockIdx.x * TILE + threadIdx.x;
-  double xs[PPT], ys[PPT], zs[PPT];
-  bool live[PPT];
-  int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
-#pragma unroll
-  for (int r = 0; r < PPT; ++r) {
-    const int64_t i = base + (int64_t)r * BLOCK;
-    live[r] = (i < n) && p.dead[i] == 0;
-    if (live[r]) {
-      xs[r] = p.x[i]; ys[r] = p.y[i]; zs[r] = p.z[i];
-      int c[3];
-      cell_of(g, xs[r], ys[r], zs[r], c);
-      for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], c[d]); hi[d] = max(hi[d], c[d]); }
-    }
-  }
-  box_begin(g, ds, lo, hi);
-#pragma unroll
-  for (int r = 0; r < PPT; ++r) {
-    int ix = 0, iy = 0, iz = 0;
-    double w[8];
-    bool has = live[r] && deposit_weights(g, k1, xs[r], ys[r], zs[r], ix, iy, iz, w);
-    box_add(g, ds, has, ix, iy, iz, w, kq, rep);
-  }
-  box_flush(g, ds, kq, rep);
-}
-
-// np_local += sum of the replicas; replicas cleared for the next deposit.
-__global__ void k_rep_flush(double *np, double *rep, int64_t nodes) {
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= nodes) return;
-  double s = 0.0;
-#pragma unroll
-  for (int r = 0; r < LEPIC_XCDS; ++r) {
-    s = s + rep[(int64_t)r * nodes + i];
-    rep[(int64_t)r * nodes + i] = 0.0;
-  }
-  np[i] = np[i] + s;
-}
-
 // --------------------------------------------------------------------------
 // Transposed deposition.  Sorted particles come in runs of one cell, but a
 // wave reads them lane by lane, so a run spans lanes and summing it takes a
@@ -546,6 +502,123 @@ __device__ inline void wave_deposit_transposed(const WaveStage &st, DepShared &d
   if (cur >= 0) box_atomic8(ds, cur, acc);
 }
 
+// Box-local key of one particle's stencil for the transposed deposit, or, if
+// the stencil leaves the box, its 8 contributions straight to the replica.
+__device__ inline int stage_key(const KGrid &g, const DepShared &ds, int ix, int iy, int iz,
+                                double px, double py, double pz, double kdep,
+                                const double *__restrict__ kq, double *rep) {
+  if (ds.vol > 0) {
+    const int lx = ix - ds.x0, ly = iy - ds.y0, lz = iz - ds.z0;
+    if (lx >= 0 && ly >= 0 && lz >= 0 && lx + 1 < ds.bx && ly + 1 < ds.by && lz + 1 < ds.bz)
+      return lx + ds.bx * (ly + ds.by * lz);
+  }
+  const double wx2 = 1.0 - px, wy2 = 1.0 - py, wz2 = 1.0 - pz;
+  const double w[8] = {kdep * px * py * pz,    kdep * wx2 * py * pz,  kdep * wx2 * wy2 * pz,
+                       kdep * px * wy2 * pz,   kdep * px * py * wz2,  kdep * wx2 * py * wz2,
+                       kdep * wx2 * wy2 * wz2, kdep * px * wy2 * wz2};
+  const int64_t b = gidx(g, ix, iy, iz);
+  const int64_t nd[8] = {b, b + 1, b + g.sy + 1, b + g.sy, b + g.sz, b + g.sz + 1,
+                         b + g.sz + g.sy + 1, b + g.sz + g.sy};
+#pragma unroll
+  for (int q = 0; q < 8; ++q) l2_atomic_add(&rep[nd[q]], kq[nd[q]] * w[q]);
+  return -1;
+}
+
+// Orders a wave's stage writes before its transposed reads (the stage is
+// private to the wave).
+__device__ inline void wave_stage_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+
+__global__ __launch_bounds__(BLOCK) void k_deposit(KGrid g, KPset p, int64_t n,
+                                                   const double *__restrict__ kq, double k1,
+                                                   double *rep_base, int64_t nodes) {
+  __shared__ DepShared ds;
+  __shared__ WaveStage stage[NWAVES];
+  double *rep = rep_base + (int64_t)xcc_id() * nodes;
+  const int lane = __lane_id();
+  const int wave = threadIdx.x / LEPIC_WAVE;
+  WaveStage &st = stage[wave];
+  const int64_t base = (int64_t)blockIdx.x * TILE + (int64_t)wave * (LEPIC_WAVE * PPT) + lane;
+  double xs[PPT], ys[PPT], zs[PPT];
+  bool live[PPT];
+  int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+#pragma unroll
+  for (int r = 0; r < PPT; ++r) {
+    const int64_t i = base + (int64_t)r * LEPIC_WAVE;
+    live[r] = (i < n) && p.dead[i] == 0;
+    if (live[r]) {
+      xs[r] = p.x[i]; ys[r] = p.y[i]; zs[r] = p.z[i];
+      int c[3];
+      cell_of(g, xs[r], ys[r], zs[r], c);
+      for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], c[d]); hi[d] = max(hi[d], c[d]); }
+    }
+  }
+  box_begin(g, ds, lo, hi);
+#pragma unroll
+  for (int r = 0; r < PPT; ++r) {
+    int key = -1, ix, iy, iz;
+    double px = 0.0, py = 0.0, pz = 0.0;
+    if (live[r] && deposit_fracs(g, xs[r], ys[r], zs[r], ix, iy, iz, px, py, pz))
+      key = stage_key(g, ds, ix, iy, iz, px, py, pz, k1, kq, rep);
+    const int o = r * LEPIC_WAVE + lane;
+    st.key[o] = key;
+    st.px[o] = px; st.py[o] = py; st.pz[o] = pz;
+  }
+  wave_stage_sync();
+  wave_deposit_transposed(st, ds, k1);
+  box_flush(g, ds, kq, rep);
+}
+
+// np_local += sum of the replicas; replicas cleared for the next deposit.
+__global__ void k_rep_flush(double *np, double *rep, int64_t nodes) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nodes) return;
+  double s = 0.0;
+#pragma unroll
+  for (int r = 0; r < LEPIC_XCDS; ++r) {
+    s = s + rep[(int64_t)r * nodes + i];
+    rep[(int64_t)r * nodes + i] = 0.0;
+  }
+  np[i] = np[i] + s;
+}
+
+// The particle phase of a collision step, in place: gather + push + wall BC
+// only (the deposit follows MCC), one particle per thread, no LDS.
+__global__ __launch_bounds__(BLOCK) void k_push_nodep(
+    KGrid g, KPset p, int64_t n, const double4 *__restrict__ E4, KBField bf, double k1, double dt,
+    const int *__restrict__ bcnd, KBC bp, double *p_mac, double *P_loss,
+    unsigned long long *lost_ctr) {
+  const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+  int lost = 0;
+  if (i < n && p.dead[i] == 0) {
+    double xp = p.x[i], yp = p.y[i], zp = p.z[i];
+    double vx = p.vx[i], vy = p.vy[i], vz = p.vz[i];
+    Stencil s;
+    make_stencil(g, xp, yp, zp, s);
+    double Ex, Ey, Ez;
+    gather3(E4, g, s, Ex, Ey, Ez);
+    velocity_update(g, bf, s, k1, Ex, Ey, Ez, vx, vy, vz);
+    xp = xp + dt * vx; yp = yp + dt * vy; zp = zp + dt * vz;
+    int label;
+    if (wall_test(g, bcnd, bp, xp, yp, zp, label)) {
+      wall_diagnostics(bp, label, vx, vy, vz, p_mac, P_loss);
+      p.dead[i] = 2;
+      lost = 1;
+    } else {
+      survivor_wrap(bp, xp, yp, zp, vx);
+      p.x[i] = xp; p.y[i] = yp; p.z[i] = zp;
+      p.vx[i] = vx; p.vy[i] = vy; p.vz[i] = vz;
+    }
+  }
+  const uint64_t m = __ballot(lost != 0);
+  if (m && __lane_id() == __ffsll((unsigned long long)m) - 1)
+    atomicAdd(lost_ctr, (unsigned long long)__popcll(m));
+}
+
 // Production particle phase, in place: gather + push + wall BC (+ deposit).
 // Lost particles become holes (dead = 2) instead of being compacted; the
 // next sort drops holes, keeping the survivors' relative order, which is the
@@ -607,28 +680,8 @@ __global__ __launch_bounds__(BLOCK) void k_push(
         p.x[i] = xp; p.y[i] = yp; p.z[i] = zp;
         p.vx[i] = vx; p.vy[i] = vy; p.vz[i] = vz;
         int ix, iy, iz;
-        if (do_dep && deposit_fracs(g, xp, yp, zp, ix, iy, iz, px, py, pz)) {
-          int l = -1;
-          if (ds.vol > 0) {
-            const int lx = ix - ds.x0, ly = iy - ds.y0, lz = iz - ds.z0;
-            if (lx >= 0 && ly >= 0 && lz >= 0 && lx + 1 < ds.bx && ly + 1 < ds.by && lz + 1 < ds.bz)
-              l = lx + ds.bx * (ly + ds.by * lz);
-          }
-          if (l >= 0) {
-            key = l;
-          } else {  // stencil outside the box: straight to this XCD's replica
-            const double wx2 = 1.0 - px, wy2 = 1.0 - py, wz2 = 1.0 - pz;
-            const double w[8] = {kdep * px * py * pz,   kdep * wx2 * py * pz,
-                                 kdep * wx2 * wy2 * pz, kdep * px * wy2 * pz,
-                                 kdep * px * py * wz2,  kdep * wx2 * py * wz2,
-                                 kdep * wx2 * wy2 * wz2, kdep * px * wy2 * wz2};
-            const int64_t b = gidx(g, ix, iy, iz);
-            const int64_t nd[8] = {b, b + 1, b + g.sy + 1, b + g.sy, b + g.sz, b + g.sz + 1,
-                                   b + g.sz + g.sy + 1, b + g.sz + g.sy};
-#pragma unroll
-            for (int q = 0; q < 8; ++q) l2_atomic_add(&rep[nd[q]], kq[nd[q]] * w[q]);
-          }
-        }
+        if (do_dep && deposit_fracs(g, xp, yp, zp, ix, iy, iz, px, py, pz))
+          key = stage_key(g, ds, ix, iy, iz, px, py, pz, kdep, kq, rep);
       }
     }
     if (do_dep) {
@@ -638,75 +691,10 @@ __global__ __launch_bounds__(BLOCK) void k_push(
     }
   }
   if (do_dep) {
-    // the stage is private to this wave: order its writes before the reads
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_stage_sync();
     wave_deposit_transposed(st, ds, kdep);
     box_flush(g, ds, kq, rep);
   }
-  // losses are rare: one atomic per wave that saw any
-  int tot = lost_local;
-  for (int d = 32; d > 0; d >>= 1) tot += __shfl_xor(tot, d);
-  if (__lane_id() == 0 && tot > 0) atomicAdd(lost_ctr, (unsigned long long)tot);
-}
-
-__global__ __launch_bounds__(BLOCK) void k_push_inplace(
-    KGrid g, KPset p, int64_t n, const double4 *__restrict__ E4, KBField bf, double k1, double dt,
-    const int *__restrict__ bcnd, KBC bp, const double *__restrict__ kq, double kdep,
-    double *rep_base, int64_t nodes, int do_dep, double *p_mac, double *P_loss,
-    unsigned long long *lost_ctr) {
-  __shared__ DepShared ds;
-  double *rep = rep_base ? rep_base + (int64_t)xcc_id() * nodes : nullptr;
-  const int64_t base = (int64_t)blockIdx.x * TILE + threadIdx.x;
-  double xs[PPT], ys[PPT], zs[PPT];
-  bool live[PPT];
-  int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
-#pragma unroll
-  for (int r = 0; r < PPT; ++r) {
-    const int64_t i = base + (int64_t)r * BLOCK;
-    live[r] = (i < n) && p.dead[i] == 0;
-    if (live[r]) {
-      xs[r] = p.x[i]; ys[r] = p.y[i]; zs[r] = p.z[i];
-      if (do_dep) {
-        int c[3];
-        cell_of(g, xs[r], ys[r], zs[r], c);
-        for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], c[d]); hi[d] = max(hi[d], c[d]); }
-      }
-    }
-  }
-  if (do_dep) box_begin(g, ds, lo, hi);
-  int lost_local = 0;
-#pragma unroll
-  for (int r = 0; r < PPT; ++r) {
-    const int64_t i = base + (int64_t)r * BLOCK;
-    bool has = false;
-    int ix = 0, iy = 0, iz = 0;
-    double w[8];
-    if (live[r]) {
-      double xp = xs[r], yp = ys[r], zp = zs[r];
-      double vx = p.vx[i], vy = p.vy[i], vz = p.vz[i];
-      Stencil s;
-      make_stencil(g, xp, yp, zp, s);
-      double Ex, Ey, Ez;
-      gather3(E4, g, s, Ex, Ey, Ez);
-      velocity_update(g, bf, s, k1, Ex, Ey, Ez, vx, vy, vz);
-      xp = xp + dt * vx; yp = yp + dt * vy; zp = zp + dt * vz;
-      int label;
-      if (wall_test(g, bcnd, bp, xp, yp, zp, label)) {
-        wall_diagnostics(bp, label, vx, vy, vz, p_mac, P_loss);
-        p.dead[i] = 2;
-        ++lost_local;
-      } else {
-        survivor_wrap(bp, xp, yp, zp, vx);
-        p.x[i] = xp; p.y[i] = yp; p.z[i] = zp;
-        p.vx[i] = vx; p.vy[i] = vy; p.vz[i] = vz;
-        if (do_dep) has = deposit_weights(g, kdep, xp, yp, zp, ix, iy, iz, w);
-      }
-    }
-    if (do_dep) box_add(g, ds, has, ix, iy, iz, w, kq, rep);
-  }
-  if (do_dep) box_flush(g, ds, kq, rep);
   // losses are rare: one atomic per wave that saw any
   int tot = lost_local;
   for (int d = 32; d > 0; d >>= 1) tot += __shfl_xor(tot, d);
@@ -812,6 +800,10 @@ __global__ __launch_bounds__(BLOCK) void k_gather_sorted(KPset src, KPset dst, i
   dst.vx[i] = src.vx[j]; dst.vy[i] = src.vy[j]; dst.vz[i] = src.vz[j];
   dst.cell[i] = (int32_t)keys[i];
   dst.dead[i] = src.dead[j];
+}
+
+__global__ void k_add_u64(unsigned long long *acc, const unsigned long long *v) {
+  if (threadIdx.x == 0) *acc = *acc + *v;
 }
 
 __global__ void k_fill_i32(int32_t *a, int64_t n, int32_t v) {
@@ -1020,7 +1012,7 @@ int lepic_push_bc_deposit_inplace(lepic_ctx *ctx, const lepic_grid *g, lepic_pse
                                   const double *E4, const lepic_bfield *bf, double q, double m,
                                   double dt, const int32_t *bcnd, const lepic_bc_params *bp,
                                   const double *kq, double *np_local, double *p_mac,
-                                  double *P_loss, int64_t *np_lost) {
+                                  double *P_loss, uint64_t *lost_acc, int64_t *np_lost) {
   if (!ctx) return LEPIC_ERR_ARG;
   int st;
   if ((st = check_grid(ctx, g)) || (st = check_pset(ctx, p, "push_bc_deposit_inplace"))) return st;
@@ -1034,21 +1026,39 @@ int lepic_push_bc_deposit_inplace(lepic_ctx *ctx, const lepic_grid *g, lepic_pse
     rep = lepic_replicas(ctx, nodes);
     if (!rep) return lepic_fail(ctx, LEPIC_ERR_HIP, "push_bc_deposit_inplace: replicas");
   }
-  unsigned long long *lost = reinterpret_cast<unsigned long long *>(ctx->d_ctrl + CTRL_COUNT_OUT);
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream,
-                     reinterpret_cast<int32_t *>(lost), (int64_t)2, 0);
+  // losses go to the context counter when the host asks for this call's
+  // count, straight to the caller's accumulator otherwise (no host sync)
+  unsigned long long *own = reinterpret_cast<unsigned long long *>(ctx->d_ctrl + CTRL_COUNT_OUT);
+  unsigned long long *ctr = own;
+  if (np_lost || !lost_acc) {
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream,
+                       reinterpret_cast<int32_t *>(own), (int64_t)2, 0);
+  } else {
+    ctr = reinterpret_cast<unsigned long long *>(lost_acc);
+  }
   if (p->n > 0) {
-    double k1 = dt * q / (2.0 * m);
-    double kdep = np_local ? bp->Nm / (g->h[0] * g->h[1] * g->h[2]) : 0.0;
-    hipLaunchKernelGGL(k_push, dim3(grid_blocks(p->n, TILE)), dim3(BLOCK), 0, ctx->stream,
-                       make_kgrid(g), kp(p), p->n, reinterpret_cast<const double4 *>(E4),
-                       make_kbfield(bf, k1), k1, dt, bcnd, kbc(bp), kq, kdep, rep, nodes,
-                       np_local ? 1 : 0, p_mac, P_loss, lost);
-    LEPIC_LAUNCH_CHECK(ctx, "k_push");
+    const double k1 = dt * q / (2.0 * m);
+    const KGrid kg = make_kgrid(g);
+    const KBField kbf = make_kbfield(bf, k1);
+    if (np_local) {
+      const double kdep = bp->Nm / (g->h[0] * g->h[1] * g->h[2]);
+      hipLaunchKernelGGL(k_push, dim3(grid_blocks(p->n, TILE)), dim3(BLOCK), 0, ctx->stream, kg,
+                         kp(p), p->n, reinterpret_cast<const double4 *>(E4), kbf, k1, dt, bcnd,
+                         kbc(bp), kq, kdep, rep, nodes, 1, p_mac, P_loss, ctr);
+      LEPIC_LAUNCH_CHECK(ctx, "k_push");
+    } else {
+      hipLaunchKernelGGL(k_push_nodep, dim3(grid_blocks(p->n, BLOCK)), dim3(BLOCK), 0,
+                         ctx->stream, kg, kp(p), p->n, reinterpret_cast<const double4 *>(E4),
+                         kbf, k1, dt, bcnd, kbc(bp), p_mac, P_loss, ctr);
+      LEPIC_LAUNCH_CHECK(ctx, "k_push_nodep");
+    }
   }
   if (np_local && (st = lepic_replica_flush(ctx, nodes, np_local))) return st;
   if (np_lost) {
-    LEPIC_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, lost, sizeof(int64_t), hipMemcpyDeviceToHost,
+    if (lost_acc)
+      hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(64), 0, ctx->stream,
+                         reinterpret_cast<unsigned long long *>(lost_acc), own);
+    LEPIC_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, own, sizeof(int64_t), hipMemcpyDeviceToHost,
                                   ctx->stream));
     LEPIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     *np_lost = ctx->h_pinned[0];

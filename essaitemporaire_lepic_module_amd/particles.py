@@ -230,9 +230,13 @@ def push_bc_deposit(part: ParticleSet, n, h, E4, bf: _native.BField, q, m, dt, b
 
 
 def push_bc_deposit_inplace(part: ParticleSet, n, h, E4, bf: _native.BField, q, m, dt, bcnd,
-                            bp: _native.BCParams, kq, np_local, p_mac, P_loss) -> int:
+                            bp: _native.BCParams, kq, np_local, p_mac, P_loss,
+                            lost_acc: torch.Tensor | None = None, sync: bool = True):
     """Production particle phase in place: lost particles become holes
-    (dead = 2), removed later by sort_compact.  Returns this call's losses."""
+    (dead = 2), removed later by sort_compact.  np_local None skips the
+    deposit.  lost_acc (1-element int64 device tensor) accumulates the
+    losses on the device; with sync the call waits for the stream and
+    returns this call's losses, otherwise it returns None."""
     ctx = context(part.device)
     g = _native.grid(n, h)
     p = part.pset()
@@ -240,8 +244,8 @@ def push_bc_deposit_inplace(part: ParticleSet, n, h, E4, bf: _native.BField, q, 
     ctx.check(ctx.lib.lepic_push_bc_deposit_inplace(ctx.h, C.byref(g), C.byref(p), ptr(E4),
                                                     C.byref(bf), q, m, dt, ptr(bcnd), C.byref(bp),
                                                     ptr(kq), ptr(np_local), ptr(p_mac), ptr(P_loss),
-                                                    C.byref(lost)))
-    return int(lost.value)
+                                                    ptr(lost_acc), C.byref(lost) if sync else None))
+    return int(lost.value) if sync else None
 
 
 def sort_compact(part: ParticleSet, n, h):

@@ -9,7 +9,16 @@ velocity update when B is on):
 With fused=True the push, the wall BC and (on steps without MCC) the
 deposit run as one kernel per species.  With lazy=True (production) that
 kernel works in place and leaves holes where particles were lost; the
-periodic sort removes them, keeping the survivors' order.
+periodic sort removes them, keeping the survivors' order, and losses are
+counted on the device without a host round trip.
+
+resort_every (production only) re-sorts a species by cell every k steps in
+addition to the reference's cadence: the deposit accumulates cell runs in
+LDS, and fast species (electrons cross a cell in a few steps) lose their
+cell order long before the reference's next sort.  The extra sorts change
+the particle order between the reference's sorts, which nothing in the
+production step depends on (collisions draw from per-particle Philox
+streams and the deposit's summation order is that of its atomics anyway).
 """
 from __future__ import annotations
 
@@ -27,11 +36,15 @@ from .device import as_device, node_shape
 class Simulation:
     def __init__(self, case: Case, device=None, fused: bool = True, loader: str = "host",
                  mcc_mode: int = _native.MCC_REPLAY_RAN2, philox_key: int = 0x5EED5EED,
-                 capacity_factor: float = 1.3, parts=None, np_thread=None, lazy: bool = False):
+                 capacity_factor: float = 1.3, parts=None, np_thread=None, lazy: bool = False,
+                 resort_every=None):
         self.case = case
         self.dev = torch.device(device if device is not None else "cuda:0")
         self.fused = fused
         self.lazy = lazy and fused
+        if resort_every is not None and not self.lazy:
+            raise ValueError("resort_every needs the in-place production path (lazy=True)")
+        self.resort_every = list(resort_every) if resort_every is not None else None
         self.mcc_mode = mcc_mode
         self.philox_key = philox_key
         n, h = case.n, case.h
@@ -93,7 +106,8 @@ class Simulation:
         self.last_mg = None
         self.last_mcc = None
         self.lost = [0] * self.ntype
-        self.holes = [0] * self.ntype   # lazy mode: lost slots not yet compacted
+        # lazy mode: holes (lost slots not yet compacted) per species, on the device
+        self.lost_acc = torch.zeros(self.ntype, dtype=torch.int64, device=dev)
         self.timers = None   # {phase: [(start_event, end_event, units), ...]} when timing
 
     @contextlib.contextmanager
@@ -115,7 +129,8 @@ class Simulation:
         torch.cuda.synchronize()
         out = {}
         for k, lst in (self.timers or {}).items():
-            out[k] = (sum(a.elapsed_time(b) for a, b, _ in lst), len(lst), sum(u for _, _, u in lst))
+            units = sum(int(u) for _, _, u in lst)   # device counts resolve here
+            out[k] = (sum(a.elapsed_time(b) for a, b, _ in lst), len(lst), units)
         return out
 
     # ------------------------------------------------------------------
@@ -131,11 +146,15 @@ class Simulation:
             with self._phase("sort"):
                 for s, p in enumerate(self.parts):
                     if self.lazy:
-                        if p.n > 0:
-                            P.sort_compact(p, n, h)
-                        self.holes[s] = 0
+                        self._compact(s)
                     elif p.n > 1:
                         P.sort_particles_by_cell(p, n, h, cell_tables=False)
+        elif self.resort_every is not None:
+            with self._phase("sort"):
+                for s in range(self.ntype):
+                    k = self.resort_every[s]
+                    if k and (istep - 1) % k == 0:
+                        self._compact(s)
         with self._phase("rho"):
             F.reduce_species_density(n, self.bcnd, self.np_thread, self.ntype, self.np_red)
             F.build_rho_from_np(n, self.np_red, self.charge, self.ntype, self.rho)
@@ -150,14 +169,21 @@ class Simulation:
             F.clear_np_thread(self.np_thread)
             push = P.push_bc_deposit_inplace if self.lazy else P.push_bc_deposit
             for s, p in enumerate(self.parts):
-                with self._phase("push_bc_deposit" if not collide else "push_bc",
-                                 p.n - self.holes[s]):
-                    self.lost[s] = push(
-                        p, n, h, self.E4, self.bf[s], self.charge[s], self.mass[s], c.dt,
-                        self.bcnd, self.bp[s], self.kq, None if collide else self.np_thread[s],
-                        self.p_mac[s], self.P_loss[s:s + 1])
+                np_s = None if collide else self.np_thread[s]
                 if self.lazy:
-                    self.holes[s] += self.lost[s]
+                    # particles pushed = slots - holes before the call (read when timed)
+                    units = (p.n - self.lost_acc[s].clone()) if self.timers is not None else 0
+                    with self._phase("push_bc_deposit" if not collide else "push_bc", units):
+                        P.push_bc_deposit_inplace(
+                            p, n, h, self.E4, self.bf[s], self.charge[s], self.mass[s], c.dt,
+                            self.bcnd, self.bp[s], self.kq, np_s, self.p_mac[s],
+                            self.P_loss[s:s + 1], lost_acc=self.lost_acc[s:s + 1], sync=False)
+                else:
+                    with self._phase("push_bc_deposit" if not collide else "push_bc", p.n):
+                        self.lost[s] = P.push_bc_deposit(
+                            p, n, h, self.E4, self.bf[s], self.charge[s], self.mass[s], c.dt,
+                            self.bcnd, self.bp[s], self.kq, np_s, self.p_mac[s],
+                            self.P_loss[s:s + 1])
         else:
             with self._phase("push"):
                 for s, p in enumerate(self.parts):
@@ -194,9 +220,20 @@ class Simulation:
             self.advance_one_step(istep)
 
     # ------------------------------------------------------------------
+    def _compact(self, s: int):
+        p = self.parts[s]
+        if p.n > 0:
+            P.sort_compact(p, self.n, self.h)
+        self.lost_acc[s].zero_()
+
+    @property
+    def holes(self) -> list:
+        """Lost slots not yet compacted, per species (lazy mode; reads the device)."""
+        return [int(v) for v in self.lost_acc.tolist()]
+
     def particle_count(self) -> int:
         """Live particles (slots minus holes not yet compacted)."""
-        return sum(p.n - hl for p, hl in zip(self.parts, self.holes))
+        return sum(p.n for p in self.parts) - int(self.lost_acc.sum().item())
 
     def total_charge(self) -> float:
         """sum_s q_s Nm_s N_s."""

@@ -239,12 +239,15 @@ int lepic_push_bc_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset 
  * kernel streams each particle once with no cross-block ordering.  Holes are
  * skipped by every later pass and removed by lepic_sort_compact, which keeps
  * the survivors' relative order -- the order the reference's per-step stable
- * compaction gives.  *np_lost receives this call's removals. */
+ * compaction gives.  np_local NULL skips the deposit (collision steps).
+ * lost_acc (device, may be NULL) is incremented by this call's removals
+ * without a host synchronisation; np_lost (host, may be NULL) makes the call
+ * synchronise and return them. */
 int lepic_push_bc_deposit_inplace(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *p,
                                   const double *E4, const lepic_bfield *bf, double q, double m,
                                   double dt, const int32_t *bcnd, const lepic_bc_params *bp,
                                   const double *kq, double *np_local, double *p_mac,
-                                  double *P_loss, int64_t *np_lost);
+                                  double *P_loss, uint64_t *lost_acc, int64_t *np_lost);
 
 /* Stable sort by cell (as lepic_sort_particles_by_cell) that drops every
  * particle with dead != 0; dst->n is the surviving count. */

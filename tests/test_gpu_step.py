@@ -2,13 +2,16 @@
 the GPU against the oracle's OracleSim driving the same step order on the
 host, from the same loaded particles.
 
-Without collisions every particle coordinate is bit-identical after each
-step, in the reference's order (exact compaction) or, on the production
-in-place path, in the same order once holes are skipped; phi and the reduced
-densities agree to the atomic-summation tolerance.  With MCC in the
-reference's ran2 replay mode the event stream is identical and coordinates
-agree to 1e-12 relative (post-collision velocities go through log/sin/cos,
-which the host and GPU math libraries round differently in the last bit)."""
+The first step starts from the oracle's own density, so every particle
+coordinate is bit-identical after it, in the reference's order (exact
+compaction) or, on the production in-place path, in the same order once
+holes are skipped.  The density the step deposits is summed by FP64 atomics,
+whose order is the hardware's, so from the second step on rho, phi and E
+carry last-bit differences into the push; coordinates are then held to 1e-10
+relative, particle counts and multigrid sweep counts stay exact.  With MCC in
+the reference's ran2 replay mode the event stream (counters, seed) is
+identical; post-collision velocities also go through log/sin/cos, which the
+host and GPU math libraries round differently in the last bit."""
 import dataclasses
 
 import numpy as np
@@ -21,6 +24,7 @@ from essaitemporaire_lepic_module_amd import config as cfg
 pytestmark = pytest.mark.gpu
 
 RTOL_ATOMIC = 1e-12
+RTOL_STEPS = 1e-10
 
 
 def _start(oracle, case, P, lazy, mode=_native.MCC_REPLAY_RAN2):
@@ -48,12 +52,13 @@ def _compare(sim, ref, exact=True, step=0):
             if exact:
                 np.testing.assert_array_equal(a[k][live], b[k], err_msg=f"step {step} species {s} {k}")
             else:
-                np.testing.assert_allclose(a[k][live], b[k], rtol=1e-12,
-                                           atol=1e-12 * np.abs(b[k]).max(),
+                np.testing.assert_allclose(a[k][live], b[k], rtol=RTOL_STEPS,
+                                           atol=RTOL_STEPS * np.abs(b[k]).max(),
                                            err_msg=f"step {step} species {s} {k}")
     phi_o = ref.phi
+    tol = 1e-12 if exact else RTOL_STEPS
     np.testing.assert_allclose(sim.phi.cpu().numpy(), phi_o, rtol=0,
-                               atol=1e-12 * max(np.abs(phi_o).max(), 1e-30))
+                               atol=tol * max(np.abs(phi_o).max(), 1e-30))
     np.testing.assert_allclose(sim.np_red.cpu().numpy(), ref.np_red, rtol=RTOL_ATOMIC,
                                atol=1e-12 * ref.np_red.max())
 
@@ -67,7 +72,7 @@ def test_plain_steps_match_oracle(gpu, oracle, lazy):
         sim.advance_one_step(istep)
         ref.advance_one_step(istep)
         torch.cuda.synchronize()
-        _compare(sim, ref, exact=True, step=istep)
+        _compare(sim, ref, exact=istep == 1, step=istep)
         assert sim.last_mg["cycles"] == ref.last_mg["cycles"]
         assert sim.last_mg["sweeps"] == ref.last_mg["sweeps"]
 
@@ -82,8 +87,44 @@ def test_magnetized_mcc_steps_match_oracle(gpu, oracle):
         sim.advance_one_step(istep)
         ref.advance_one_step(istep)
         torch.cuda.synchronize()
-        _compare(sim, ref, exact=istep < 2, step=istep)
+        _compare(sim, ref, exact=istep == 1, step=istep)
         if istep % 2 == 0:
             for k in ("Nc", "accepted", "err_coll", "skipped_dead", "appended"):
                 assert sim.last_mcc[k] == ref.last_mcc[k][:len(sim.last_mcc[k])], (istep, k)
     assert sim.iseed[0] == ref.seed
+
+
+def test_resorted_production_steps_match_oracle(gpu, oracle):
+    """Production path with extra electron sorts: same particles (as a set)
+    and the same fields as the oracle at the reference's sort cadence."""
+    from essaitemporaire_lepic_module_amd import particles as P
+    from essaitemporaire_lepic_module_amd.simulation import Simulation
+    case = dataclasses.replace(cfg.cfg1_plain_box(n=8, np_cell=6, ng=3), nb_step_sort=5)
+    ref = oracle.OracleSim(case)
+    parts = []
+    for p in ref.parts:
+        a = p.arrays()
+        ps = P.ParticleSet(p.cap, "cuda:0")
+        ps.from_host(a["x"], a["y"], a["z"], a["vx"], a["vy"], a["vz"])
+        parts.append(ps)
+    npt = torch.as_tensor(ref.np_thread.copy(), device="cuda:0")
+    sim = Simulation(case, "cuda:0", fused=True, parts=parts, np_thread=npt, lazy=True,
+                     resort_every=[2, 0])
+    for istep in range(1, 7):
+        sim.advance_one_step(istep)
+        ref.advance_one_step(istep)
+        torch.cuda.synchronize()
+        assert sim.particle_count() == sum(p.n for p in ref.parts)
+        for s, (g, o) in enumerate(zip(sim.parts, ref.parts)):
+            a, b = g.to_host(), o.arrays()
+            live = a["dead"] == 0
+            ga = np.stack([a[k][live] for k in ("x", "y", "z", "vx", "vy", "vz")], 1)
+            ob = np.stack([b[k] for k in ("x", "y", "z", "vx", "vy", "vz")], 1)
+            ga = ga[np.lexsort(ga.T[::-1])]
+            ob = ob[np.lexsort(ob.T[::-1])]
+            for col in range(6):
+                np.testing.assert_allclose(ga[:, col], ob[:, col], rtol=RTOL_STEPS,
+                                           atol=RTOL_STEPS * float(np.abs(ob[:, col]).max()),
+                                           err_msg=f"step {istep} species {s} column {col}")
+        np.testing.assert_allclose(sim.phi.cpu().numpy(), ref.phi, rtol=0,
+                                   atol=RTOL_STEPS * max(np.abs(ref.phi).max(), 1e-30))
