@@ -81,7 +81,7 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--ppc", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--resort-electrons", type=int, default=3,
+    ap.add_argument("--resort-electrons", type=int, default=8,
                     help="production path: also re-sort electrons by cell every k steps "
                          "(0: only at the reference's sort cadence)")
     ap.add_argument("--exact-order", action="store_true",

@@ -28,7 +28,7 @@ EXPORTS = (
     "lepic_move_boris", "lepic_apply_bc", "lepic_build_kq", "lepic_fill", "lepic_deposit",
     "lepic_reduce_density", "lepic_build_rho", "lepic_poisson_solve", "lepic_efield",
     "lepic_collisions", "lepic_push_bc_deposit", "lepic_push_bc_deposit_inplace",
-    "lepic_sort_compact",
+    "lepic_sort_compact", "lepic_bin_compact",
 )
 
 
@@ -144,6 +144,7 @@ def load() -> C.CDLL:
                                                         C.POINTER(BCParams), vp, vp, vp, vp, vp,
                                                         C.POINTER(C.c_int64)]),
             "lepic_sort_compact": (C.c_int, [vp, G, PS, PS]),
+            "lepic_bin_compact": (C.c_int, [vp, G, PS, PS]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

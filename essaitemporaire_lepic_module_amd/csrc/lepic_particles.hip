@@ -118,16 +118,81 @@ __device__ inline int wall_test(const KGrid &g, const int *__restrict__ bcnd, co
   return lost;
 }
 
-__device__ inline void wall_diagnostics(const KBC &bp, int label, double vx, double vy,
-                                        double vz, double *p_mac, double *P_loss) {
-  double v2 = vx * vx + vy * vy + vz * vz;
-  double Ek_eV = 0.5 * bp.mass * v2 / 1.60217646e-19;
-  if (label <= bp.ngrid) {
-    atomicAdd(&p_mac[2 * label + 0], bp.qmacro);
-    atomicAdd(&p_mac[2 * label + 1], fabs(bp.qmacro) * Ek_eV);
+// The same test from a per-node table: wt[node] = bcnd[node] when all 8
+// nodes of the cell whose lower corner is node have bcnd >= 1 (a particle
+// there is lost), -1 otherwise.  One load instead of eight.
+__device__ inline int wall_test_tab(const KGrid &g, const int *__restrict__ wt, const KBC &bp,
+                                    double xp, double yp, double zp, int &label) {
+  int ix = (int)floor(qdiv(xp, g.q1)) + 1;
+  int iy = (int)floor(qdiv(yp, g.q2)) + 1;
+  int iz = (int)floor(qdiv(zp, g.q3)) + 1;
+  ix = imax(0, imin(g.n1 + 1, ix));
+  iy = imax(0, imin(g.n2 + 1, iy));
+  iz = imax(0, imin(g.n3 + 1, iz));
+  const int c = wt[gidx(g, ix, iy, iz)];
+  int lost = c >= 0 ? 1 : 0;
+  label = c >= 0 ? c : 0;
+  if (bp.ptype == bp.tag_neg && xp < 0.0 && bp.flag_nmn == 1) {
+    lost = 2;
+    label = 0;
   }
-  double Ek_J = 0.5 * bp.mass * v2;
-  atomicAdd(P_loss, bp.Nm * Ek_J);
+  return lost;
+}
+
+__global__ void k_wall_table(KGrid g, const int *__restrict__ bcnd, int *wt, int64_t nodes) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nodes) return;
+  const int i = (int)(t % g.sy);
+  const int64_t r = t / g.sy;
+  const int j = (int)(r % (g.n2 + 3)), k = (int)(r / (g.n2 + 3));
+  int v = -1;
+  if (i <= g.n1 + 1 && j <= g.n2 + 1 && k <= g.n3 + 1) {
+    const int c0 = bcnd[t], c1 = bcnd[t + 1], c2 = bcnd[t + g.sy + 1], c3 = bcnd[t + g.sy];
+    const int c4 = bcnd[t + g.sz], c5 = bcnd[t + g.sz + 1], c6 = bcnd[t + g.sz + g.sy + 1],
+              c7 = bcnd[t + g.sz + g.sy];
+    if (c0 >= 1 && c1 >= 1 && c2 >= 1 && c3 >= 1 && c4 >= 1 && c5 >= 1 && c6 >= 1 && c7 >= 1)
+      v = c0;
+  }
+  wt[t] = v;
+}
+
+__device__ inline double wave_sum_d(double v) {
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
+  return v;
+}
+
+// Wall diagnostics of apply_particle_bc_legacy (mod_particleBC.f90:195-206):
+// p_mac(1,label) += Nm q, p_mac(2,label) += |Nm q| Ek[eV], P_loss += Nm Ek[J]
+// for each lost particle.  Done for a whole wave: lanes with lost set
+// contribute, and the wave's sums per electrode label go out with one atomic
+// each instead of every lost particle hitting the same words.  Every lane of
+// the wave calls it.
+__device__ inline void wall_diag_wave(const KBC &bp, bool lost, int label, double vx, double vy,
+                                      double vz, double *p_mac, double *P_loss) {
+  const uint64_t any = __ballot(lost);
+  if (any == 0) return;
+  const int lane = __lane_id();
+  double Ek_J = 0.0, Ek_eV = 0.0;
+  if (lost) {
+    const double v2 = vx * vx + vy * vy + vz * vz;
+    Ek_eV = 0.5 * bp.mass * v2 / 1.60217646e-19;
+    Ek_J = 0.5 * bp.mass * v2;
+  }
+  const double pl = wave_sum_d(lost ? bp.Nm * Ek_J : 0.0);
+  if (lane == __ffsll((unsigned long long)any) - 1) atomicAdd(P_loss, pl);
+  uint64_t rem = __ballot(lost && label <= bp.ngrid);
+  while (rem) {
+    const int leader = __ffsll((unsigned long long)rem) - 1;
+    const int L = __shfl(label, leader);
+    const bool mine = lost && label <= bp.ngrid && label == L;
+    const double q = wave_sum_d(mine ? bp.qmacro : 0.0);
+    const double e = wave_sum_d(mine ? fabs(bp.qmacro) * Ek_eV : 0.0);
+    if (lane == leader) {
+      atomicAdd(&p_mac[2 * L + 0], q);
+      atomicAdd(&p_mac[2 * L + 1], e);
+    }
+    rem &= ~__ballot(mine);
+  }
 }
 
 __device__ inline void survivor_wrap(const KBC &bp, double &xp, double &yp, double &zp,
@@ -240,22 +305,21 @@ __global__ __launch_bounds__(BLOCK) void k_apply_bc(KGrid g, KPset src, KPset ds
   __shared__ LBShared sh;
   int64_t tile = take_tile(counter, ntiles, sh);
   int64_t i = tile * BLOCK + threadIdx.x;
-  bool keep = false;
+  bool keep = false, lost = false;
+  int label = 0;
   double xp = 0, yp = 0, zp = 0, vx = 0, vy = 0, vz = 0;
   if (i < n) {
     if (src.dead[i] != 1) {
       xp = src.x[i]; yp = src.y[i]; zp = src.z[i];
       vx = src.vx[i]; vy = src.vy[i]; vz = src.vz[i];
-      int label;
-      int lost = wall_test(g, bcnd, bp, xp, yp, zp, label);
-      if (lost) {
-        wall_diagnostics(bp, label, vx, vy, vz, p_mac, P_loss);
-      } else {
+      lost = wall_test(g, bcnd, bp, xp, yp, zp, label) != 0;
+      if (!lost) {
         survivor_wrap(bp, xp, yp, zp, vx);
         keep = true;
       }
     }
   }
+  wall_diag_wave(bp, lost, label, vx, vy, vz, p_mac, P_loss);
   int64_t o = compact_slot(keep, sh, status, tile, epoch, spin_fail, count_out, ntiles);
   if (keep) {
     dst.x[o] = xp; dst.y[o] = yp; dst.z[o] = zp;
@@ -469,8 +533,25 @@ __device__ inline void box_atomic8(DepShared &ds, int l, const double a[8]) {
   for (int q = 0; q < 8; ++q) atomicAdd(&ds.box[ld[q]], a[q]);
 }
 
-// Called by every lane of the wave after it staged all PPT rounds.
-__device__ inline void wave_deposit_transposed(const WaveStage &st, DepShared &ds, double k1) {
+// Called by every lane of the wave after it staged all PPT rounds.  A run
+// with a box key goes to LDS, a run with a global key to this XCD's replica
+// (kq applied to the run's sum: kq is 1 or 2, so the scaling is exact).
+__device__ inline void flush_run(const KGrid &g, DepShared &ds, int key, const double a[8],
+                                 const double *__restrict__ kq, double *rep) {
+  if (key >= 0) {
+    box_atomic8(ds, key, a);
+  } else if (key <= -2) {
+    const int64_t b = (int64_t)(-(key + 2));
+    const int64_t nd[8] = {b, b + 1, b + g.sy + 1, b + g.sy, b + g.sz, b + g.sz + 1,
+                           b + g.sz + g.sy + 1, b + g.sz + g.sy};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) l2_atomic_add(&rep[nd[q]], kq[nd[q]] * a[q]);
+  }
+}
+
+__device__ inline void wave_deposit_transposed(const KGrid &g, const WaveStage &st, DepShared &ds,
+                                               double k1, const double *__restrict__ kq,
+                                               double *rep) {
   const int lane = __lane_id();
   double acc[8];
 #pragma unroll
@@ -481,12 +562,12 @@ __device__ inline void wave_deposit_transposed(const WaveStage &st, DepShared &d
     const int o = lane * PPT + r;
     const int key = st.key[o];
     if (key != cur) {
-      if (cur >= 0) box_atomic8(ds, cur, acc);
+      flush_run(g, ds, cur, acc, kq, rep);
 #pragma unroll
       for (int q = 0; q < 8; ++q) acc[q] = 0.0;
       cur = key;
     }
-    if (key >= 0) {
+    if (key != -1) {
       const double px = st.px[o], py = st.py[o], pz = st.pz[o];
       const double wx2 = 1.0 - px, wy2 = 1.0 - py, wz2 = 1.0 - pz;
       acc[0] += k1 * px * py * pz;
@@ -499,29 +580,20 @@ __device__ inline void wave_deposit_transposed(const WaveStage &st, DepShared &d
       acc[7] += k1 * px * wy2 * wz2;
     }
   }
-  if (cur >= 0) box_atomic8(ds, cur, acc);
+  flush_run(g, ds, cur, acc, kq, rep);
 }
 
-// Box-local key of one particle's stencil for the transposed deposit, or, if
-// the stencil leaves the box, its 8 contributions straight to the replica.
-__device__ inline int stage_key(const KGrid &g, const DepShared &ds, int ix, int iy, int iz,
-                                double px, double py, double pz, double kdep,
-                                const double *__restrict__ kq, double *rep) {
+// Key of one particle's stencil for the transposed deposit: the box-local
+// lower node (>= 0) when the stencil lies in the block's LDS box, otherwise
+// -(global lower node) - 2, so that runs outside the box are summed in
+// registers too before their global atomics.
+__device__ inline int stage_key(const KGrid &g, const DepShared &ds, int ix, int iy, int iz) {
   if (ds.vol > 0) {
     const int lx = ix - ds.x0, ly = iy - ds.y0, lz = iz - ds.z0;
     if (lx >= 0 && ly >= 0 && lz >= 0 && lx + 1 < ds.bx && ly + 1 < ds.by && lz + 1 < ds.bz)
       return lx + ds.bx * (ly + ds.by * lz);
   }
-  const double wx2 = 1.0 - px, wy2 = 1.0 - py, wz2 = 1.0 - pz;
-  const double w[8] = {kdep * px * py * pz,    kdep * wx2 * py * pz,  kdep * wx2 * wy2 * pz,
-                       kdep * px * wy2 * pz,   kdep * px * py * wz2,  kdep * wx2 * py * wz2,
-                       kdep * wx2 * wy2 * wz2, kdep * px * wy2 * wz2};
-  const int64_t b = gidx(g, ix, iy, iz);
-  const int64_t nd[8] = {b, b + 1, b + g.sy + 1, b + g.sy, b + g.sz, b + g.sz + 1,
-                         b + g.sz + g.sy + 1, b + g.sz + g.sy};
-#pragma unroll
-  for (int q = 0; q < 8; ++q) l2_atomic_add(&rep[nd[q]], kq[nd[q]] * w[q]);
-  return -1;
+  return -(int)gidx(g, ix, iy, iz) - 2;
 }
 
 // Orders a wave's stage writes before its transposed reads (the stage is
@@ -563,13 +635,13 @@ __global__ __launch_bounds__(BLOCK) void k_deposit(KGrid g, KPset p, int64_t n,
     int key = -1, ix, iy, iz;
     double px = 0.0, py = 0.0, pz = 0.0;
     if (live[r] && deposit_fracs(g, xs[r], ys[r], zs[r], ix, iy, iz, px, py, pz))
-      key = stage_key(g, ds, ix, iy, iz, px, py, pz, k1, kq, rep);
+      key = stage_key(g, ds, ix, iy, iz);
     const int o = r * LEPIC_WAVE + lane;
     st.key[o] = key;
     st.px[o] = px; st.py[o] = py; st.pz[o] = pz;
   }
   wave_stage_sync();
-  wave_deposit_transposed(st, ds, k1);
+  wave_deposit_transposed(g, st, ds, k1, kq, rep);
   box_flush(g, ds, kq, rep);
 }
 
@@ -590,31 +662,32 @@ __global__ void k_rep_flush(double *np, double *rep, int64_t nodes) {
 // only (the deposit follows MCC), one particle per thread, no LDS.
 __global__ __launch_bounds__(BLOCK) void k_push_nodep(
     KGrid g, KPset p, int64_t n, const double4 *__restrict__ E4, KBField bf, double k1, double dt,
-    const int *__restrict__ bcnd, KBC bp, double *p_mac, double *P_loss,
+    const int *__restrict__ wt, KBC bp, double *p_mac, double *P_loss,
     unsigned long long *lost_ctr) {
   const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
-  int lost = 0;
+  bool lost = false;
+  int label = 0;
+  double vx = 0.0, vy = 0.0, vz = 0.0;
   if (i < n && p.dead[i] == 0) {
     double xp = p.x[i], yp = p.y[i], zp = p.z[i];
-    double vx = p.vx[i], vy = p.vy[i], vz = p.vz[i];
+    vx = p.vx[i]; vy = p.vy[i]; vz = p.vz[i];
     Stencil s;
     make_stencil(g, xp, yp, zp, s);
     double Ex, Ey, Ez;
     gather3(E4, g, s, Ex, Ey, Ez);
     velocity_update(g, bf, s, k1, Ex, Ey, Ez, vx, vy, vz);
     xp = xp + dt * vx; yp = yp + dt * vy; zp = zp + dt * vz;
-    int label;
-    if (wall_test(g, bcnd, bp, xp, yp, zp, label)) {
-      wall_diagnostics(bp, label, vx, vy, vz, p_mac, P_loss);
+    if (wall_test_tab(g, wt, bp, xp, yp, zp, label)) {
       p.dead[i] = 2;
-      lost = 1;
+      lost = true;
     } else {
       survivor_wrap(bp, xp, yp, zp, vx);
       p.x[i] = xp; p.y[i] = yp; p.z[i] = zp;
       p.vx[i] = vx; p.vy[i] = vy; p.vz[i] = vz;
     }
   }
-  const uint64_t m = __ballot(lost != 0);
+  wall_diag_wave(bp, lost, label, vx, vy, vz, p_mac, P_loss);
+  const uint64_t m = __ballot(lost);
   if (m && __lane_id() == __ffsll((unsigned long long)m) - 1)
     atomicAdd(lost_ctr, (unsigned long long)__popcll(m));
 }
@@ -625,9 +698,17 @@ __global__ __launch_bounds__(BLOCK) void k_push_nodep(
 // order the reference's per-step stable compaction produces.  Wave w of a
 // block owns particles [w*64*PPT, (w+1)*64*PPT) of the block's tile and reads
 // them in PPT coalesced rounds of 64.
-__global__ __launch_bounds__(BLOCK) void k_push(
+// Register-pressure knobs of k_push: with PUSH_REREAD the box pass only
+// reduces the tile's cells and the push pass reads each position again (an L2
+// hit), so nothing is carried across the rounds; PUSH_UNROLL rounds are
+// interleaved; PUSH_WAVES is the occupancy the register budget is set for.
+constexpr bool PUSH_REREAD = true;
+constexpr int PUSH_UNROLL = 4;
+constexpr int PUSH_WAVES = 2;
+
+__global__ __launch_bounds__(BLOCK, PUSH_WAVES) void k_push(
     KGrid g, KPset p, int64_t n, const double4 *__restrict__ E4, KBField bf, double k1, double dt,
-    const int *__restrict__ bcnd, KBC bp, const double *__restrict__ kq, double kdep,
+    const int *__restrict__ wt, KBC bp, const double *__restrict__ kq, double kdep,
     double *rep_base, int64_t nodes, int do_dep, double *p_mac, double *P_loss,
     unsigned long long *lost_ctr) {
   __shared__ DepShared ds;
@@ -637,43 +718,47 @@ __global__ __launch_bounds__(BLOCK) void k_push(
   const int wave = threadIdx.x / LEPIC_WAVE;
   WaveStage &st = stage[wave];
   const int64_t base = (int64_t)blockIdx.x * TILE + (int64_t)wave * (LEPIC_WAVE * PPT) + lane;
-  double xs[PPT], ys[PPT], zs[PPT], vxs[PPT], vys[PPT], vzs[PPT];
-  bool live[PPT];
+  double xs[PPT], ys[PPT], zs[PPT];
   int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+  if (do_dep) {
 #pragma unroll
-  for (int r = 0; r < PPT; ++r) {
-    const int64_t i = base + (int64_t)r * LEPIC_WAVE;
-    live[r] = (i < n) && p.dead[i] == 0;
-    if (live[r]) {
-      xs[r] = p.x[i]; ys[r] = p.y[i]; zs[r] = p.z[i];
-      vxs[r] = p.vx[i]; vys[r] = p.vy[i]; vzs[r] = p.vz[i];
-      if (do_dep) {
+    for (int r = 0; r < PPT; ++r) {
+      const int64_t i = base + (int64_t)r * LEPIC_WAVE;
+      if (i < n && p.dead[i] == 0) {
+        xs[r] = p.x[i]; ys[r] = p.y[i]; zs[r] = p.z[i];
         int c[3];
         cell_of(g, xs[r], ys[r], zs[r], c);
         for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], c[d]); hi[d] = max(hi[d], c[d]); }
       }
     }
+    box_begin(g, ds, lo, hi);
   }
-  if (do_dep) box_begin(g, ds, lo, hi);
   int lost_local = 0;
-#pragma unroll
+#pragma unroll PUSH_UNROLL
   for (int r = 0; r < PPT; ++r) {
     const int64_t i = base + (int64_t)r * LEPIC_WAVE;
     int key = -1;
     double px = 0.0, py = 0.0, pz = 0.0;
-    if (live[r]) {
-      double xp = xs[r], yp = ys[r], zp = zs[r];
-      double vx = vxs[r], vy = vys[r], vz = vzs[r];
+    bool lost = false;
+    int label = 0;
+    double vx = 0.0, vy = 0.0, vz = 0.0;
+    if (i < n && p.dead[i] == 0) {
+      double xp, yp, zp;
+      if (PUSH_REREAD || !do_dep) {
+        xp = p.x[i]; yp = p.y[i]; zp = p.z[i];
+      } else {
+        xp = xs[r]; yp = ys[r]; zp = zs[r];
+      }
+      vx = p.vx[i]; vy = p.vy[i]; vz = p.vz[i];
       Stencil s;
       make_stencil(g, xp, yp, zp, s);
       double Ex, Ey, Ez;
       gather3(E4, g, s, Ex, Ey, Ez);
       velocity_update(g, bf, s, k1, Ex, Ey, Ez, vx, vy, vz);
       xp = xp + dt * vx; yp = yp + dt * vy; zp = zp + dt * vz;
-      int label;
-      if (wall_test(g, bcnd, bp, xp, yp, zp, label)) {
-        wall_diagnostics(bp, label, vx, vy, vz, p_mac, P_loss);
+      if (wall_test_tab(g, wt, bp, xp, yp, zp, label)) {
         p.dead[i] = 2;
+        lost = true;
         ++lost_local;
       } else {
         survivor_wrap(bp, xp, yp, zp, vx);
@@ -681,9 +766,10 @@ __global__ __launch_bounds__(BLOCK) void k_push(
         p.vx[i] = vx; p.vy[i] = vy; p.vz[i] = vz;
         int ix, iy, iz;
         if (do_dep && deposit_fracs(g, xp, yp, zp, ix, iy, iz, px, py, pz))
-          key = stage_key(g, ds, ix, iy, iz, px, py, pz, kdep, kq, rep);
+          key = stage_key(g, ds, ix, iy, iz);
       }
     }
+    wall_diag_wave(bp, lost, label, vx, vy, vz, p_mac, P_loss);
     if (do_dep) {
       const int o = r * LEPIC_WAVE + lane;
       st.key[o] = key;
@@ -692,7 +778,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(
   }
   if (do_dep) {
     wave_stage_sync();
-    wave_deposit_transposed(st, ds, kdep);
+    wave_deposit_transposed(g, st, ds, kdep, kq, rep);
     box_flush(g, ds, kq, rep);
   }
   // losses are rare: one atomic per wave that saw any
@@ -714,7 +800,8 @@ __global__ __launch_bounds__(BLOCK) void k_push_bc_deposit(
   __shared__ DepShared ds;
   int64_t tile = take_tile(counter, ntiles, sh);
   int64_t i = tile * BLOCK + threadIdx.x;
-  bool keep = false;
+  bool keep = false, lost = false;
+  int label = 0;
   double xp = 0, yp = 0, zp = 0, vx = 0, vy = 0, vz = 0;
   if (i < n && src.dead[i] != 1) {
     xp = src.x[i]; yp = src.y[i]; zp = src.z[i];
@@ -725,15 +812,13 @@ __global__ __launch_bounds__(BLOCK) void k_push_bc_deposit(
     gather3(E4, g, s, Ex, Ey, Ez);
     velocity_update(g, bf, s, k1, Ex, Ey, Ez, vx, vy, vz);
     xp = xp + dt * vx; yp = yp + dt * vy; zp = zp + dt * vz;
-    int label;
-    int lost = wall_test(g, bcnd, bp, xp, yp, zp, label);
-    if (lost) {
-      wall_diagnostics(bp, label, vx, vy, vz, p_mac, P_loss);
-    } else {
+    lost = wall_test(g, bcnd, bp, xp, yp, zp, label) != 0;
+    if (!lost) {
       survivor_wrap(bp, xp, yp, zp, vx);
       keep = true;
     }
   }
+  wall_diag_wave(bp, lost, label, vx, vy, vz, p_mac, P_loss);
   int64_t o = compact_slot(keep, sh, status, tile, epoch, spin_fail, count_out, ntiles);
   if (keep) {
     dst.x[o] = xp; dst.y[o] = yp; dst.z[o] = zp;
@@ -802,6 +887,64 @@ __global__ __launch_bounds__(BLOCK) void k_gather_sorted(KPset src, KPset dst, i
   dst.dead[i] = src.dead[j];
 }
 
+// --------------------------------------------------------------------------
+// Binning by cell for the production path: a counting sort whose order within
+// a cell is left open.  Input that is already close to cell order comes in
+// runs of one key along the lanes of a wave; each run takes one atomic.
+// --------------------------------------------------------------------------
+constexpr uint32_t NO_KEY = 0xFFFFFFFFu;
+
+// For the calling lane: the first lane of its run of equal keys and the run's
+// length (valid on the first lane).  Every lane of the wave must call it.
+__device__ inline void wave_runs(uint32_t key, int &head, int &len) {
+  const int lane = __lane_id();
+  const uint32_t prev = (uint32_t)__shfl_up((int)key, 1);
+  const bool is_head = lane == 0 || prev != key;
+  const uint64_t H = __ballot(is_head);
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  head = 63 - __clzll((long long)(H & upto));
+  const uint64_t above = lane == 63 ? 0ull : (H >> (lane + 1));
+  const int next = above ? lane + 1 + __ffsll((long long)above) - 1 : LEPIC_WAVE;
+  len = next - lane;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bin_count(KGrid g, KPset p, int64_t n, uint32_t *keys,
+                                                     int32_t *count) {
+  const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+  uint32_t key = NO_KEY;
+  if (i < n && p.dead[i] == 0) {
+    int ix = (int)qdiv(p.x[i], g.q1) + 1;
+    int iy = (int)qdiv(p.y[i], g.q2) + 1;
+    int iz = (int)qdiv(p.z[i], g.q3) + 1;
+    ix = imax(1, imin(g.n1, ix));
+    iy = imax(1, imin(g.n2, iy));
+    iz = imax(1, imin(g.n3, iz));
+    key = (uint32_t)((ix - 1) + g.n1 * ((iy - 1) + g.n2 * (iz - 1)));
+  }
+  if (i < n) keys[i] = key;
+  int head, len;
+  wave_runs(key, head, len);
+  if (head == __lane_id() && key != NO_KEY) atomicAdd(&count[key], len);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bin_scatter(KPset src, KPset dst, int64_t n,
+                                                       const uint32_t *__restrict__ keys,
+                                                       int32_t *cursor) {
+  const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+  const uint32_t key = i < n ? keys[i] : NO_KEY;
+  int head, len;
+  wave_runs(key, head, len);
+  int base = 0;
+  if (head == __lane_id() && key != NO_KEY) base = atomicAdd(&cursor[key], len);
+  base = __shfl(base, head);
+  if (key == NO_KEY) return;
+  const int64_t o = (int64_t)base + (__lane_id() - head);
+  dst.x[o] = src.x[i]; dst.y[o] = src.y[i]; dst.z[o] = src.z[i];
+  dst.vx[o] = src.vx[i]; dst.vy[o] = src.vy[i]; dst.vz[o] = src.vz[i];
+  dst.cell[o] = (int32_t)key;
+  dst.dead[o] = 0;
+}
+
 __global__ void k_add_u64(unsigned long long *acc, const unsigned long long *v) {
   if (threadIdx.x == 0) *acc = *acc + *v;
 }
@@ -867,6 +1010,8 @@ int check_grid(lepic_ctx *ctx, const lepic_grid *g) {
     return lepic_fail(ctx, LEPIC_ERR_ARG, "bad grid");
   if (!qdiv_divisor_ok(g->h[0]) || !qdiv_divisor_ok(g->h[1]) || !qdiv_divisor_ok(g->h[2]))
     return lepic_fail(ctx, LEPIC_ERR_ARG, "grid step outside [1e-200, 1e200]");
+  if (grid_nodes(g) >= ((int64_t)1 << 31) - 2)   // node ids travel as int32 in the deposit
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "grid has 2^31 nodes or more");
   return LEPIC_OK;
 }
 
@@ -1040,6 +1185,12 @@ int lepic_push_bc_deposit_inplace(lepic_ctx *ctx, const lepic_grid *g, lepic_pse
     const double k1 = dt * q / (2.0 * m);
     const KGrid kg = make_kgrid(g);
     const KBField kbf = make_kbfield(bf, k1);
+    // per-node wall table, rebuilt from bcnd on every call (a few microseconds)
+    int32_t *wt = reinterpret_cast<int32_t *>(lepic_scratch(ctx, sizeof(int32_t) * (size_t)nodes, 1));
+    if (!wt) return lepic_fail(ctx, LEPIC_ERR_HIP, "push_bc_deposit_inplace: wall table");
+    hipLaunchKernelGGL(k_wall_table, dim3(grid_blocks(nodes, 256)), dim3(256), 0, ctx->stream, kg,
+                       bcnd, wt, nodes);
+    bcnd = wt;
     if (np_local) {
       const double kdep = bp->Nm / (g->h[0] * g->h[1] * g->h[2]);
       hipLaunchKernelGGL(k_push, dim3(grid_blocks(p->n, TILE)), dim3(BLOCK), 0, ctx->stream, kg,
@@ -1199,6 +1350,55 @@ int lepic_sort_particles_by_cell(lepic_ctx *ctx, const lepic_grid *g, const lepi
 int lepic_sort_compact(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *src,
                        lepic_pset *dst) {
   return sort_impl(ctx, g, src, dst, nullptr, nullptr, 1);
+}
+
+int lepic_bin_compact(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *src,
+                      lepic_pset *dst) {
+  if (!ctx) return LEPIC_ERR_ARG;
+  int st;
+  if ((st = check_grid(ctx, g)) || (st = check_pset(ctx, src, "bin_compact src")) ||
+      (st = check_pset(ctx, dst, "bin_compact dst")))
+    return st;
+  if (!dst->cell) return lepic_fail(ctx, LEPIC_ERR_ARG, "bin_compact: dst has no cell array");
+  if (dst->cap < src->n) return lepic_fail(ctx, LEPIC_ERR_CAPACITY, "bin_compact: dst too small");
+  if (src->x == dst->x) return lepic_fail(ctx, LEPIC_ERR_ARG, "bin_compact: src and dst must differ");
+  const int64_t n = src->n;
+  const int64_t ncells = (int64_t)g->n[0] * g->n[1] * g->n[2];
+  if (ncells >= ((int64_t)1 << 31) - 1 || n >= ((int64_t)1 << 31))
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "bin_compact: grid or particle count too large");
+  if (n == 0) {
+    dst->n = 0;
+    return LEPIC_OK;
+  }
+  // scratch 0: keys[n] | count[ncells + 1] | start[ncells + 1] | scan temp
+  size_t temp_bytes = 0;
+  int32_t *nul = nullptr;
+  LEPIC_HIP(ctx, rocprim::exclusive_scan(nullptr, temp_bytes, nul, nul, 0, (size_t)ncells + 1,
+                                         rocprim::plus<int32_t>(), ctx->stream));
+  const size_t kb = (sizeof(uint32_t) * (size_t)n + 255) & ~(size_t)255;
+  const size_t cb = (sizeof(int32_t) * (size_t)(ncells + 1) + 255) & ~(size_t)255;
+  char *base = reinterpret_cast<char *>(lepic_scratch(ctx, kb + 2 * cb + temp_bytes + 256, 0));
+  if (!base) return lepic_fail(ctx, LEPIC_ERR_HIP, "bin_compact: scratch allocation failed");
+  uint32_t *keys = reinterpret_cast<uint32_t *>(base);
+  int32_t *count = reinterpret_cast<int32_t *>(base + kb);
+  int32_t *start = reinterpret_cast<int32_t *>(base + kb + cb);
+  void *temp = base + kb + 2 * cb;
+  hipLaunchKernelGGL(k_fill_i32, dim3(grid_blocks(ncells + 1, 256)), dim3(256), 0, ctx->stream,
+                     count, ncells + 1, 0);
+  hipLaunchKernelGGL(k_bin_count, dim3(grid_blocks(n, BLOCK)), dim3(BLOCK), 0, ctx->stream,
+                     make_kgrid(g), kp(src), n, keys, count);
+  LEPIC_LAUNCH_CHECK(ctx, "k_bin_count");
+  // start[c] = particles in cells before c; start[ncells] = live total
+  LEPIC_HIP(ctx, rocprim::exclusive_scan(temp, temp_bytes, count, start, 0, (size_t)ncells + 1,
+                                         rocprim::plus<int32_t>(), ctx->stream));
+  LEPIC_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, start + ncells, sizeof(int32_t),
+                                hipMemcpyDeviceToHost, ctx->stream));
+  hipLaunchKernelGGL(k_bin_scatter, dim3(grid_blocks(n, BLOCK)), dim3(BLOCK), 0, ctx->stream,
+                     kp(src), kp(dst), n, keys, start);
+  LEPIC_LAUNCH_CHECK(ctx, "k_bin_scatter");
+  LEPIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  dst->n = (int64_t)reinterpret_cast<int32_t *>(ctx->h_pinned)[0];
+  return LEPIC_OK;
 }
 
 int lepic_load_uniform_device(lepic_ctx *ctx, const lepic_grid *g, const int32_t *bcnd,

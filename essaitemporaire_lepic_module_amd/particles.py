@@ -248,6 +248,17 @@ def push_bc_deposit_inplace(part: ParticleSet, n, h, E4, bf: _native.BField, q, 
     return int(lost.value) if sync else None
 
 
+def bin_compact(part: ParticleSet, n, h):
+    """Group particles by cell (cell order of sort_particles_by_cell, order
+    inside a cell unspecified) and drop holes and MCC-killed particles."""
+    ctx = context(part.device)
+    g = _native.grid(n, h)
+    src, dst = part.pset(), part.pset(other=True)
+    ctx.check(ctx.lib.lepic_bin_compact(ctx.h, C.byref(g), C.byref(src), C.byref(dst)))
+    part.swap()
+    part.n = int(dst.n)
+
+
 def sort_compact(part: ParticleSet, n, h):
     """Stable cell sort that drops holes and MCC-killed particles."""
     ctx = context(part.device)

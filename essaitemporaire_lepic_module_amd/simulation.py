@@ -221,9 +221,16 @@ class Simulation:
 
     # ------------------------------------------------------------------
     def _compact(self, s: int):
+        """Drop holes and re-sort by cell.  The stable radix sort keeps the
+        reference's order; once extra sorts are asked for, species that are
+        not re-sorted (slow ones, still in long cell runs) are binned instead,
+        which is cheaper on such input."""
         p = self.parts[s]
         if p.n > 0:
-            P.sort_compact(p, self.n, self.h)
+            if self.resort_every is not None and not self.resort_every[s]:
+                P.bin_compact(p, self.n, self.h)
+            else:
+                P.sort_compact(p, self.n, self.h)
         self.lost_acc[s].zero_()
 
     @property

@@ -254,6 +254,14 @@ int lepic_push_bc_deposit_inplace(lepic_ctx *ctx, const lepic_grid *g, lepic_pse
 int lepic_sort_compact(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *src,
                        lepic_pset *dst);
 
+/* Production binning by cell (the cell order of sort_particles_by_cell,
+ * mod_particle_sorting.f90:88-181, with the order inside a cell left open):
+ * a counting sort that drops every particle with dead != 0 and sets dst->cell.
+ * Two streaming passes; for runs that the production path re-sorts more
+ * often than the reference, where the order inside a cell is immaterial. */
+int lepic_bin_compact(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *src,
+                      lepic_pset *dst);
+
 #ifdef __cplusplus
 }
 #endif

@@ -319,3 +319,38 @@ def test_deposit_unsorted_falls_back(gpu, oracle, rng):
     F.deposit_particle_set_to_np_thread(ps, n, h, torch.as_tensor(kq, device=gpu), 3.0, npg)
     npo = oracle.deposit(n, h, _oracle_set(oracle, arr), kq, 3.0)
     np.testing.assert_allclose(npg.cpu().numpy(), npo, rtol=RTOL_ATOMIC, atol=1e-30)
+
+
+@pytest.mark.parametrize("case", ["mixed", "empty", "all_dead", "one_cell"])
+def test_bin_compact_groups_by_cell(gpu, oracle, rng, case):
+    """Binning keeps exactly the live particles, in nondecreasing cell order,
+    each cell holding the same particles as the stable sort puts there."""
+    from essaitemporaire_lepic_module_amd import particles as P
+    n, h = H.box((9, 7, 6))
+    count = {"mixed": 40000, "empty": 0, "all_dead": 3000, "one_cell": 5000}[case]
+    arr = H.random_particles(rng, n, h, count, outside=0.02)
+    if case == "one_cell":
+        arr = [a.copy() for a in arr]
+        for d in range(3):
+            arr[d][:] = 2.5 * h[d] + 0.1 * h[d] * rng.random(count)
+    dead = (rng.random(count) < 0.05).astype(np.int8) * 2
+    if case == "all_dead":
+        dead[:] = 2
+    # nearly sorted input, as in production
+    po = _oracle_set(oracle, arr, dead=np.zeros(count, np.int8))
+    if count:
+        oracle.sort_particles(n, h, po)
+    start = po.arrays()
+    a = _gpu_set(P, [start[k] for k in ("x", "y", "z", "vx", "vy", "vz")], dead=dead)
+    b = _gpu_set(P, [start[k] for k in ("x", "y", "z", "vx", "vy", "vz")], dead=dead)
+    P.bin_compact(a, n, h)
+    P.sort_compact(b, n, h)
+    assert a.n == b.n == int((dead == 0).sum())
+    ga, gb = a.to_host(), b.to_host()
+    assert (ga["dead"] == 0).all()
+    assert (np.diff(ga["cell"]) >= 0).all()
+    np.testing.assert_array_equal(ga["cell"], gb["cell"])
+    cols = ("cell", "x", "y", "z", "vx", "vy", "vz")
+    ka = np.stack([ga[k].astype(np.float64) for k in cols], 1)
+    kb = np.stack([gb[k].astype(np.float64) for k in cols], 1)
+    np.testing.assert_array_equal(ka[np.lexsort(ka.T[::-1])], kb[np.lexsort(kb.T[::-1])])
