@@ -491,23 +491,12 @@ __device__ inline void cell_of(const KGrid &g, double x, double y, double z, int
   c[2] = imax(0, imin(g.n3 + 1, (int)qdiv(z, g.q3) + 1));
 }
 
-constexpr int PPT = 4;              // particles per thread in the streaming kernels
-constexpr int TILE = BLOCK * PPT;
-
 // --------------------------------------------------------------------------
-// Transposed deposition.  Sorted particles come in runs of one cell, but a
-// wave reads them lane by lane, so a run spans lanes and summing it takes a
-// cross-lane scan.  Instead each wave stages its particles' box-local node
-// and fractions (px, py, pz of deposit_particle_set_to_np_thread) in LDS in
-// read order and lane l then deposits particles PPT*l .. PPT*l+PPT-1: a run
-// of one cell is summed in the lane's registers and reaches the LDS box with
-// one set of 8 atomics.
+// Deposit in runs.  Cell-sorted particles come in runs of one cell; a lane
+// sums the 8 weights of consecutive particles with the same key in registers
+// and hands a run to the block's LDS box (or, outside the box, to this XCD's
+// replica) with one set of 8 atomics.
 // --------------------------------------------------------------------------
-struct WaveStage {
-  int key[LEPIC_WAVE * PPT];   // box-local lower node, -1: nothing to deposit
-  double px[LEPIC_WAVE * PPT], py[LEPIC_WAVE * PPT], pz[LEPIC_WAVE * PPT];
-};
-
 // Fractions of deposit_weights (same range guard, same arithmetic).
 __device__ inline bool deposit_fracs(const KGrid &g, double x, double y, double z, int &ix,
                                      int &iy, int &iz, double &px, double &py, double &pz) {
@@ -533,9 +522,8 @@ __device__ inline void box_atomic8(DepShared &ds, int l, const double a[8]) {
   for (int q = 0; q < 8; ++q) atomicAdd(&ds.box[ld[q]], a[q]);
 }
 
-// Called by every lane of the wave after it staged all PPT rounds.  A run
-// with a box key goes to LDS, a run with a global key to this XCD's replica
-// (kq applied to the run's sum: kq is 1 or 2, so the scaling is exact).
+// A run with a box key goes to LDS, a run with a global key to this XCD's
+// replica (kq applied to the run's sum: kq is 1 or 2, so the scaling is exact).
 __device__ inline void flush_run(const KGrid &g, DepShared &ds, int key, const double a[8],
                                  const double *__restrict__ kq, double *rep) {
   if (key >= 0) {
@@ -549,41 +537,7 @@ __device__ inline void flush_run(const KGrid &g, DepShared &ds, int key, const d
   }
 }
 
-__device__ inline void wave_deposit_transposed(const KGrid &g, const WaveStage &st, DepShared &ds,
-                                               double k1, const double *__restrict__ kq,
-                                               double *rep) {
-  const int lane = __lane_id();
-  double acc[8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) acc[q] = 0.0;
-  int cur = -1;
-#pragma unroll
-  for (int r = 0; r < PPT; ++r) {
-    const int o = lane * PPT + r;
-    const int key = st.key[o];
-    if (key != cur) {
-      flush_run(g, ds, cur, acc, kq, rep);
-#pragma unroll
-      for (int q = 0; q < 8; ++q) acc[q] = 0.0;
-      cur = key;
-    }
-    if (key != -1) {
-      const double px = st.px[o], py = st.py[o], pz = st.pz[o];
-      const double wx2 = 1.0 - px, wy2 = 1.0 - py, wz2 = 1.0 - pz;
-      acc[0] += k1 * px * py * pz;
-      acc[1] += k1 * wx2 * py * pz;
-      acc[2] += k1 * wx2 * wy2 * pz;
-      acc[3] += k1 * px * wy2 * pz;
-      acc[4] += k1 * px * py * wz2;
-      acc[5] += k1 * wx2 * py * wz2;
-      acc[6] += k1 * wx2 * wy2 * wz2;
-      acc[7] += k1 * px * wy2 * wz2;
-    }
-  }
-  flush_run(g, ds, cur, acc, kq, rep);
-}
-
-// Key of one particle's stencil for the transposed deposit: the box-local
+// Key of one particle's stencil for the run deposit: the box-local
 // lower node (>= 0) when the stencil lies in the block's LDS box, otherwise
 // -(global lower node) - 2, so that runs outside the box are summed in
 // registers too before their global atomics.
@@ -596,52 +550,154 @@ __device__ inline int stage_key(const KGrid &g, const DepShared &ds, int ix, int
   return -(int)gidx(g, ix, iy, iz) - 2;
 }
 
-// Orders a wave's stage writes before its transposed reads (the stage is
-// private to the wave).
-__device__ inline void wave_stage_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// --------------------------------------------------------------------------
+// Lane-contiguous particle runs: lane t of a block owns particles
+// [tile + CPT*t, tile + CPT*t + CPT).
+// --------------------------------------------------------------------------
+constexpr int CPT = 4;
+constexpr int CTILE = BLOCK * CPT;
+static_assert(CPT == 4, "the vector accesses below move particles in pairs");
+
+__device__ inline void load_pair(const double *a, int64_t i, double &v0, double &v1) {
+  const double2 t = *reinterpret_cast<const double2 *>(a + i);
+  v0 = t.x;
+  v1 = t.y;
 }
 
+__device__ inline void store_pair(double *a, int64_t i, double v0, double v1) {
+  *reinterpret_cast<double2 *>(a + i) = make_double2(v0, v1);
+}
+
+// Loads a lane's run; dead and out-of-range slots come back not live.
+__device__ inline void load_run(const KPset &p, int64_t n, int64_t i0, bool full, double x[CPT],
+                                double y[CPT], double z[CPT], double vx[CPT], double vy[CPT],
+                                double vz[CPT], bool live[CPT]) {
+  if (full) {
+    const uint32_t d4 = *reinterpret_cast<const uint32_t *>(p.dead + i0);
+    load_pair(p.x, i0, x[0], x[1]); load_pair(p.x, i0 + 2, x[2], x[3]);
+    load_pair(p.y, i0, y[0], y[1]); load_pair(p.y, i0 + 2, y[2], y[3]);
+    load_pair(p.z, i0, z[0], z[1]); load_pair(p.z, i0 + 2, z[2], z[3]);
+    load_pair(p.vx, i0, vx[0], vx[1]); load_pair(p.vx, i0 + 2, vx[2], vx[3]);
+    load_pair(p.vy, i0, vy[0], vy[1]); load_pair(p.vy, i0 + 2, vy[2], vy[3]);
+    load_pair(p.vz, i0, vz[0], vz[1]); load_pair(p.vz, i0 + 2, vz[2], vz[3]);
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) live[q] = ((d4 >> (8 * q)) & 0xFFu) == 0u;
+  } else {
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      const int64_t i = i0 + q;
+      live[q] = i < n && p.dead[i] == 0;
+      x[q] = y[q] = z[q] = vx[q] = vy[q] = vz[q] = 0.0;
+      if (live[q]) {
+        x[q] = p.x[i]; y[q] = p.y[i]; z[q] = p.z[i];
+        vx[q] = p.vx[i]; vy[q] = p.vy[i]; vz[q] = p.vz[i];
+      }
+    }
+  }
+}
+
+// Writes a lane's run back: survivors' new state, dead = 2 for the lost.
+// Slots that were not live are written with the values they were read with.
+__device__ inline void store_run(const KPset &p, int64_t n, int64_t i0, bool full,
+                                 const double x[CPT], const double y[CPT], const double z[CPT],
+                                 const double vx[CPT], const double vy[CPT], const double vz[CPT],
+                                 const bool live[CPT], const bool lost[CPT]) {
+  if (full) {
+    store_pair(p.x, i0, x[0], x[1]); store_pair(p.x, i0 + 2, x[2], x[3]);
+    store_pair(p.y, i0, y[0], y[1]); store_pair(p.y, i0 + 2, y[2], y[3]);
+    store_pair(p.z, i0, z[0], z[1]); store_pair(p.z, i0 + 2, z[2], z[3]);
+    store_pair(p.vx, i0, vx[0], vx[1]); store_pair(p.vx, i0 + 2, vx[2], vx[3]);
+    store_pair(p.vy, i0, vy[0], vy[1]); store_pair(p.vy, i0 + 2, vy[2], vy[3]);
+    store_pair(p.vz, i0, vz[0], vz[1]); store_pair(p.vz, i0 + 2, vz[2], vz[3]);
+  } else {
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      const int64_t i = i0 + q;
+      if (live[q] && !lost[q]) {
+        p.x[i] = x[q]; p.y[i] = y[q]; p.z[i] = z[q];
+        p.vx[i] = vx[q]; p.vy[i] = vy[q]; p.vz[i] = vz[q];
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < CPT; ++q)
+    if (lost[q]) p.dead[i0 + q] = 2;
+}
+
+// In-register run of equal deposit keys.
+struct RunAcc {
+  int cur;
+  double a[8];
+  __device__ void reset() {
+    cur = -1;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) a[q] = 0.0;
+  }
+};
+
+__device__ inline void run_add(const KGrid &g, DepShared &ds, RunAcc &ra, int key, double k1,
+                               double px, double py, double pz, const double *__restrict__ kq,
+                               double *rep) {
+  if (key != ra.cur) {
+    flush_run(g, ds, ra.cur, ra.a, kq, rep);
+    ra.reset();
+    ra.cur = key;
+  }
+  const double wx2 = 1.0 - px, wy2 = 1.0 - py, wz2 = 1.0 - pz;
+  ra.a[0] += k1 * px * py * pz;
+  ra.a[1] += k1 * wx2 * py * pz;
+  ra.a[2] += k1 * wx2 * wy2 * pz;
+  ra.a[3] += k1 * px * wy2 * pz;
+  ra.a[4] += k1 * px * py * wz2;
+  ra.a[5] += k1 * wx2 * py * wz2;
+  ra.a[6] += k1 * wx2 * wy2 * wz2;
+  ra.a[7] += k1 * px * wy2 * wz2;
+}
 
 __global__ __launch_bounds__(BLOCK) void k_deposit(KGrid g, KPset p, int64_t n,
                                                    const double *__restrict__ kq, double k1,
                                                    double *rep_base, int64_t nodes) {
   __shared__ DepShared ds;
-  __shared__ WaveStage stage[NWAVES];
   double *rep = rep_base + (int64_t)xcc_id() * nodes;
-  const int lane = __lane_id();
-  const int wave = threadIdx.x / LEPIC_WAVE;
-  WaveStage &st = stage[wave];
-  const int64_t base = (int64_t)blockIdx.x * TILE + (int64_t)wave * (LEPIC_WAVE * PPT) + lane;
-  double xs[PPT], ys[PPT], zs[PPT];
-  bool live[PPT];
-  int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+  const int64_t i0 = (int64_t)blockIdx.x * CTILE + (int64_t)threadIdx.x * CPT;
+  double x[CPT], y[CPT], z[CPT];
+  bool live[CPT];
+  if (i0 + CPT <= n) {
+    const uint32_t d4 = *reinterpret_cast<const uint32_t *>(p.dead + i0);
+    load_pair(p.x, i0, x[0], x[1]); load_pair(p.x, i0 + 2, x[2], x[3]);
+    load_pair(p.y, i0, y[0], y[1]); load_pair(p.y, i0 + 2, y[2], y[3]);
+    load_pair(p.z, i0, z[0], z[1]); load_pair(p.z, i0 + 2, z[2], z[3]);
 #pragma unroll
-  for (int r = 0; r < PPT; ++r) {
-    const int64_t i = base + (int64_t)r * LEPIC_WAVE;
-    live[r] = (i < n) && p.dead[i] == 0;
-    if (live[r]) {
-      xs[r] = p.x[i]; ys[r] = p.y[i]; zs[r] = p.z[i];
-      int c[3];
-      cell_of(g, xs[r], ys[r], zs[r], c);
-      for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], c[d]); hi[d] = max(hi[d], c[d]); }
+    for (int q = 0; q < CPT; ++q) live[q] = ((d4 >> (8 * q)) & 0xFFu) == 0u;
+  } else {
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      const int64_t i = i0 + q;
+      live[q] = i < n && p.dead[i] == 0;
+      x[q] = live[q] ? p.x[i] : 0.0;
+      y[q] = live[q] ? p.y[i] : 0.0;
+      z[q] = live[q] ? p.z[i] : 0.0;
     }
   }
-  box_begin(g, ds, lo, hi);
+  int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
 #pragma unroll
-  for (int r = 0; r < PPT; ++r) {
-    int key = -1, ix, iy, iz;
-    double px = 0.0, py = 0.0, pz = 0.0;
-    if (live[r] && deposit_fracs(g, xs[r], ys[r], zs[r], ix, iy, iz, px, py, pz))
-      key = stage_key(g, ds, ix, iy, iz);
-    const int o = r * LEPIC_WAVE + lane;
-    st.key[o] = key;
-    st.px[o] = px; st.py[o] = py; st.pz[o] = pz;
+  for (int q = 0; q < CPT; ++q)
+    if (live[q]) {
+      int c[3];
+      cell_of(g, x[q], y[q], z[q], c);
+      for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], c[d]); hi[d] = max(hi[d], c[d]); }
+    }
+  box_begin(g, ds, lo, hi);
+  RunAcc ra;
+  ra.reset();
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) {
+    int ix, iy, iz;
+    double px, py, pz;
+    if (live[q] && deposit_fracs(g, x[q], y[q], z[q], ix, iy, iz, px, py, pz))
+      run_add(g, ds, ra, stage_key(g, ds, ix, iy, iz), k1, px, py, pz, kq, rep);
   }
-  wave_stage_sync();
-  wave_deposit_transposed(g, st, ds, k1, kq, rep);
+  flush_run(g, ds, ra.cur, ra.a, kq, rep);
   box_flush(g, ds, kq, rep);
 }
 
@@ -695,90 +751,71 @@ __global__ __launch_bounds__(BLOCK) void k_push_nodep(
 // Production particle phase, in place: gather + push + wall BC (+ deposit).
 // Lost particles become holes (dead = 2) instead of being compacted; the
 // next sort drops holes, keeping the survivors' relative order, which is the
-// order the reference's per-step stable compaction produces.  Wave w of a
-// block owns particles [w*64*PPT, (w+1)*64*PPT) of the block's tile and reads
-// them in PPT coalesced rounds of 64.
-// Register-pressure knobs of k_push: with PUSH_REREAD the box pass only
-// reduces the tile's cells and the push pass reads each position again (an L2
-// hit), so nothing is carried across the rounds; PUSH_UNROLL rounds are
-// interleaved; PUSH_WAVES is the occupancy the register budget is set for.
-constexpr bool PUSH_REREAD = true;
-constexpr int PUSH_UNROLL = 4;
-constexpr int PUSH_WAVES = 2;
-
-__global__ __launch_bounds__(BLOCK, PUSH_WAVES) void k_push(
+// order the reference's per-step stable compaction produces.
+//
+// Each lane owns CPT consecutive particles, read and written with 16-byte
+// vector accesses (a wave still covers one contiguous 2 KB span per array).
+// In cell-sorted data a lane's particles are mostly in one cell, so the
+// deposit sums them in registers and issues one set of 8 atomics per run.
+__global__ __launch_bounds__(BLOCK) void k_push(
     KGrid g, KPset p, int64_t n, const double4 *__restrict__ E4, KBField bf, double k1, double dt,
     const int *__restrict__ wt, KBC bp, const double *__restrict__ kq, double kdep,
     double *rep_base, int64_t nodes, int do_dep, double *p_mac, double *P_loss,
     unsigned long long *lost_ctr) {
   __shared__ DepShared ds;
-  __shared__ WaveStage stage[NWAVES];
   double *rep = rep_base ? rep_base + (int64_t)xcc_id() * nodes : nullptr;
-  const int lane = __lane_id();
-  const int wave = threadIdx.x / LEPIC_WAVE;
-  WaveStage &st = stage[wave];
-  const int64_t base = (int64_t)blockIdx.x * TILE + (int64_t)wave * (LEPIC_WAVE * PPT) + lane;
-  double xs[PPT], ys[PPT], zs[PPT];
+  const int64_t i0 = (int64_t)blockIdx.x * CTILE + (int64_t)threadIdx.x * CPT;
+  const bool full = i0 + CPT <= n;
+  double x[CPT], y[CPT], z[CPT], vx[CPT], vy[CPT], vz[CPT];
+  bool live[CPT];
+  load_run(p, n, i0, full, x, y, z, vx, vy, vz, live);
   int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
   if (do_dep) {
 #pragma unroll
-    for (int r = 0; r < PPT; ++r) {
-      const int64_t i = base + (int64_t)r * LEPIC_WAVE;
-      if (i < n && p.dead[i] == 0) {
-        xs[r] = p.x[i]; ys[r] = p.y[i]; zs[r] = p.z[i];
+    for (int q = 0; q < CPT; ++q)
+      if (live[q]) {
         int c[3];
-        cell_of(g, xs[r], ys[r], zs[r], c);
+        cell_of(g, x[q], y[q], z[q], c);
         for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], c[d]); hi[d] = max(hi[d], c[d]); }
       }
-    }
     box_begin(g, ds, lo, hi);
   }
+  RunAcc ra;
+  ra.reset();
+  bool lostq[CPT];
   int lost_local = 0;
-#pragma unroll PUSH_UNROLL
-  for (int r = 0; r < PPT; ++r) {
-    const int64_t i = base + (int64_t)r * LEPIC_WAVE;
-    int key = -1;
-    double px = 0.0, py = 0.0, pz = 0.0;
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) {
     bool lost = false;
     int label = 0;
-    double vx = 0.0, vy = 0.0, vz = 0.0;
-    if (i < n && p.dead[i] == 0) {
-      double xp, yp, zp;
-      if (PUSH_REREAD || !do_dep) {
-        xp = p.x[i]; yp = p.y[i]; zp = p.z[i];
-      } else {
-        xp = xs[r]; yp = ys[r]; zp = zs[r];
-      }
-      vx = p.vx[i]; vy = p.vy[i]; vz = p.vz[i];
+    if (live[q]) {
+      double xp = x[q], yp = y[q], zp = z[q];
+      double ux = vx[q], uy = vy[q], uz = vz[q];
       Stencil s;
       make_stencil(g, xp, yp, zp, s);
       double Ex, Ey, Ez;
       gather3(E4, g, s, Ex, Ey, Ez);
-      velocity_update(g, bf, s, k1, Ex, Ey, Ez, vx, vy, vz);
-      xp = xp + dt * vx; yp = yp + dt * vy; zp = zp + dt * vz;
+      velocity_update(g, bf, s, k1, Ex, Ey, Ez, ux, uy, uz);
+      xp = xp + dt * ux; yp = yp + dt * uy; zp = zp + dt * uz;
+      vx[q] = ux; vy[q] = uy; vz[q] = uz;
       if (wall_test_tab(g, wt, bp, xp, yp, zp, label)) {
-        p.dead[i] = 2;
         lost = true;
         ++lost_local;
       } else {
-        survivor_wrap(bp, xp, yp, zp, vx);
-        p.x[i] = xp; p.y[i] = yp; p.z[i] = zp;
-        p.vx[i] = vx; p.vy[i] = vy; p.vz[i] = vz;
+        survivor_wrap(bp, xp, yp, zp, vx[q]);
+        x[q] = xp; y[q] = yp; z[q] = zp;
         int ix, iy, iz;
+        double px, py, pz;
         if (do_dep && deposit_fracs(g, xp, yp, zp, ix, iy, iz, px, py, pz))
-          key = stage_key(g, ds, ix, iy, iz);
+          run_add(g, ds, ra, stage_key(g, ds, ix, iy, iz), kdep, px, py, pz, kq, rep);
       }
     }
-    wall_diag_wave(bp, lost, label, vx, vy, vz, p_mac, P_loss);
-    if (do_dep) {
-      const int o = r * LEPIC_WAVE + lane;
-      st.key[o] = key;
-      st.px[o] = px; st.py[o] = py; st.pz[o] = pz;
-    }
+    wall_diag_wave(bp, lost, label, vx[q], vy[q], vz[q], p_mac, P_loss);
+    lostq[q] = lost;
   }
+  store_run(p, n, i0, full, x, y, z, vx, vy, vz, live, lostq);
   if (do_dep) {
-    wave_stage_sync();
-    wave_deposit_transposed(g, st, ds, kdep, kq, rep);
+    flush_run(g, ds, ra.cur, ra.a, kq, rep);
     box_flush(g, ds, kq, rep);
   }
   // losses are rare: one atomic per wave that saw any
@@ -1147,7 +1184,7 @@ int lepic_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *p, cons
   double *rep = lepic_replicas(ctx, nodes);
   if (!rep) return lepic_fail(ctx, LEPIC_ERR_HIP, "deposit: replica allocation failed");
   double k1 = Nm / (g->h[0] * g->h[1] * g->h[2]);
-  hipLaunchKernelGGL(k_deposit, dim3(grid_blocks(p->n, TILE)), dim3(BLOCK), 0, ctx->stream,
+  hipLaunchKernelGGL(k_deposit, dim3(grid_blocks(p->n, CTILE)), dim3(BLOCK), 0, ctx->stream,
                      make_kgrid(g), kp(p), p->n, kq, k1, rep, nodes);
   LEPIC_LAUNCH_CHECK(ctx, "k_deposit");
   return lepic_replica_flush(ctx, nodes, np_local);
@@ -1193,7 +1230,7 @@ int lepic_push_bc_deposit_inplace(lepic_ctx *ctx, const lepic_grid *g, lepic_pse
     bcnd = wt;
     if (np_local) {
       const double kdep = bp->Nm / (g->h[0] * g->h[1] * g->h[2]);
-      hipLaunchKernelGGL(k_push, dim3(grid_blocks(p->n, TILE)), dim3(BLOCK), 0, ctx->stream, kg,
+      hipLaunchKernelGGL(k_push, dim3(grid_blocks(p->n, CTILE)), dim3(BLOCK), 0, ctx->stream, kg,
                          kp(p), p->n, reinterpret_cast<const double4 *>(E4), kbf, k1, dt, bcnd,
                          kbc(bp), kq, kdep, rep, nodes, 1, p_mac, P_loss, ctr);
       LEPIC_LAUNCH_CHECK(ctx, "k_push");
