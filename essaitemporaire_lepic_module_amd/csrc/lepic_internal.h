@@ -175,15 +175,20 @@ struct KBField {
   double k2u;   // uniform B: k1*2/(1+(k1*|B|)^2), the same for every particle
 };
 
+// BM is the B mode when known at compile time (the kernels are instantiated
+// per mode, so a run with uniform B carries no registers for the map
+// gather), -1 to read it from bf.
+template <int BM>
 __device__ inline void velocity_update(const KGrid &g, const KBField &bf, const Stencil &s,
                                        double k1, double Ex, double Ey, double Ez, double &vx,
                                        double &vy, double &vz) {
+  const int mode = BM >= 0 ? BM : bf.mode;
   int use_b = 0;
   double Bx = 0.0, By = 0.0, Bz = 0.0;
-  if (bf.mode == 1) {
+  if (mode == 1) {
     Bx = bf.bx; By = bf.by; Bz = bf.bz;
     use_b = 1;
-  } else if (bf.mode == 2) {
+  } else if (mode == 2) {
     if (bf.B4[s.base].w > 1.0e-5) {
       gather3(bf.B4, g, s, Bx, By, Bz);
       use_b = 1;
@@ -192,7 +197,7 @@ __device__ inline void velocity_update(const KGrid &g, const KBField &bf, const 
   if (!bf.magnetized) use_b = 0;
   if (use_b) {
     double k2;
-    if (bf.mode == 1) {
+    if (mode == 1) {
       k2 = bf.k2u;
     } else {
       double Bt = sqrt(Bx * Bx + By * By + Bz * Bz);

@@ -74,6 +74,7 @@ __global__ __launch_bounds__(BLOCK) void k_move_es(KGrid g, KPset p, int64_t n,
 // --------------------------------------------------------------------------
 // a4: part_mover Boris/leapfrog velocity update (Src/part_expmover.f90:51-246)
 // --------------------------------------------------------------------------
+template <int BM>
 __global__ __launch_bounds__(BLOCK) void k_move_boris(KGrid g, KPset p, int64_t n,
                                                       const double4 *__restrict__ E4, KBField bf,
                                                       double k1, double dt) {
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(BLOCK) void k_move_boris(KGrid g, KPset p, int64_t 
   make_stencil(g, xp, yp, zp, s);
   double Ex, Ey, Ez;
   gather3(E4, g, s, Ex, Ey, Ez);
-  velocity_update(g, bf, s, k1, Ex, Ey, Ez, vx, vy, vz);
+  velocity_update<BM>(g, bf, s, k1, Ex, Ey, Ez, vx, vy, vz);
   p.vx[i] = vx; p.vy[i] = vy; p.vz[i] = vz;
   p.x[i] = xp + dt * vx; p.y[i] = yp + dt * vy; p.z[i] = zp + dt * vz;
 }
@@ -514,12 +515,34 @@ __device__ inline bool deposit_fracs(const KGrid &g, double x, double y, double 
   return true;
 }
 
+// Offset of stencil node q (the reference's order w1..w8) from the lower node.
+__device__ inline int node_off(int q, int bx, int bxy) {
+  const int r = q & 3;
+  const int dx = (r == 1 || r == 2) ? 1 : 0, dy = r >> 1;
+  return dx + dy * bx + (q >> 2) * bxy;
+}
+
+// Adds a run's 8 sums into the LDS box.  Neighbouring lanes of sorted data
+// flush the same cell; each lane starts at node (lane mod 8) of the stencil,
+// so the lanes of one instruction hit different addresses instead of
+// serialising on the same one.
 __device__ inline void box_atomic8(DepShared &ds, int l, const double a[8]) {
   const int bx = ds.bx, bxy = ds.bx * ds.by;
-  const int ld[8] = {l, l + 1, l + bx + 1, l + bx, l + bxy, l + bxy + 1, l + bxy + bx + 1,
-                     l + bxy + bx};
+  const int rot = __lane_id() & 7;
+  double b[8];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) atomicAdd(&ds.box[ld[q]], a[q]);
+  for (int j = 0; j < 8; ++j) b[j] = a[j];
+#pragma unroll
+  for (int stage = 1; stage < 8; stage <<= 1) {   // b[j] <- b[(j + stage) & 7] where bit is set
+    const bool take = (rot & stage) != 0;
+    double t[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) t[j] = b[(j + stage) & 7];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) b[j] = take ? t[j] : b[j];
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) atomicAdd(&ds.box[l + node_off((j + rot) & 7, bx, bxy)], b[j]);
 }
 
 // A run with a box key goes to LDS, a run with a global key to this XCD's
@@ -716,6 +739,7 @@ __global__ void k_rep_flush(double *np, double *rep, int64_t nodes) {
 
 // The particle phase of a collision step, in place: gather + push + wall BC
 // only (the deposit follows MCC), one particle per thread, no LDS.
+template <int BM>
 __global__ __launch_bounds__(BLOCK) void k_push_nodep(
     KGrid g, KPset p, int64_t n, const double4 *__restrict__ E4, KBField bf, double k1, double dt,
     const int *__restrict__ wt, KBC bp, double *p_mac, double *P_loss,
@@ -731,7 +755,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_nodep(
     make_stencil(g, xp, yp, zp, s);
     double Ex, Ey, Ez;
     gather3(E4, g, s, Ex, Ey, Ez);
-    velocity_update(g, bf, s, k1, Ex, Ey, Ez, vx, vy, vz);
+    velocity_update<BM>(g, bf, s, k1, Ex, Ey, Ez, vx, vy, vz);
     xp = xp + dt * vx; yp = yp + dt * vy; zp = zp + dt * vz;
     if (wall_test_tab(g, wt, bp, xp, yp, zp, label)) {
       p.dead[i] = 2;
@@ -757,6 +781,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_nodep(
 // vector accesses (a wave still covers one contiguous 2 KB span per array).
 // In cell-sorted data a lane's particles are mostly in one cell, so the
 // deposit sums them in registers and issues one set of 8 atomics per run.
+template <int BM>
 __global__ __launch_bounds__(BLOCK) void k_push(
     KGrid g, KPset p, int64_t n, const double4 *__restrict__ E4, KBField bf, double k1, double dt,
     const int *__restrict__ wt, KBC bp, const double *__restrict__ kq, double kdep,
@@ -795,7 +820,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(
       make_stencil(g, xp, yp, zp, s);
       double Ex, Ey, Ez;
       gather3(E4, g, s, Ex, Ey, Ez);
-      velocity_update(g, bf, s, k1, Ex, Ey, Ez, ux, uy, uz);
+      velocity_update<BM>(g, bf, s, k1, Ex, Ey, Ez, ux, uy, uz);
       xp = xp + dt * ux; yp = yp + dt * uy; zp = zp + dt * uz;
       vx[q] = ux; vy[q] = uy; vz[q] = uz;
       if (wall_test_tab(g, wt, bp, xp, yp, zp, label)) {
@@ -827,6 +852,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(
 // --------------------------------------------------------------------------
 // Fused particle phase: gather + push + wall BC + stable compaction + deposit.
 // --------------------------------------------------------------------------
+template <int BM>
 __global__ __launch_bounds__(BLOCK) void k_push_bc_deposit(
     KGrid g, KPset src, KPset dst, int64_t n, const double4 *__restrict__ E4, KBField bf,
     double k1, double dt, const int *__restrict__ bcnd, KBC bp, const double *__restrict__ kq,
@@ -847,7 +873,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_bc_deposit(
     make_stencil(g, xp, yp, zp, s);
     double Ex, Ey, Ez;
     gather3(E4, g, s, Ex, Ey, Ez);
-    velocity_update(g, bf, s, k1, Ex, Ey, Ez, vx, vy, vz);
+    velocity_update<BM>(g, bf, s, k1, Ex, Ey, Ez, vx, vy, vz);
     xp = xp + dt * vx; yp = yp + dt * vy; zp = zp + dt * vz;
     lost = wall_test(g, bcnd, bp, xp, yp, zp, label) != 0;
     if (!lost) {
@@ -1135,9 +1161,18 @@ int lepic_move_boris(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *p, const d
   if (bf && (bf->mode < 0 || bf->mode > 2)) return lepic_fail(ctx, LEPIC_ERR_ARG, "move_boris: bad mode");
   if (p->n == 0) return LEPIC_OK;
   double k1 = dt * q / (2.0 * m);
-  hipLaunchKernelGGL(k_move_boris, dim3(grid_blocks(p->n, BLOCK)), dim3(BLOCK), 0, ctx->stream,
+  const int bmode = bf ? bf->mode : 0;
+  switch (bmode) {
+    case 0: hipLaunchKernelGGL((k_move_boris<0>), dim3(grid_blocks(p->n, BLOCK)), dim3(BLOCK), 0, ctx->stream,
                      make_kgrid(g), kp(p), p->n, reinterpret_cast<const double4 *>(E4),
-                     make_kbfield(bf, k1), k1, dt);
+                     make_kbfield(bf, k1), k1, dt); break;
+    case 1: hipLaunchKernelGGL((k_move_boris<1>), dim3(grid_blocks(p->n, BLOCK)), dim3(BLOCK), 0, ctx->stream,
+                     make_kgrid(g), kp(p), p->n, reinterpret_cast<const double4 *>(E4),
+                     make_kbfield(bf, k1), k1, dt); break;
+    default: hipLaunchKernelGGL((k_move_boris<2>), dim3(grid_blocks(p->n, BLOCK)), dim3(BLOCK), 0, ctx->stream,
+                     make_kgrid(g), kp(p), p->n, reinterpret_cast<const double4 *>(E4),
+                     make_kbfield(bf, k1), k1, dt); break;
+  }
   LEPIC_LAUNCH_CHECK(ctx, "k_move_boris");
   return LEPIC_OK;
 }
@@ -1202,6 +1237,9 @@ int lepic_push_bc_deposit_inplace(lepic_ctx *ctx, const lepic_grid *g, lepic_pse
     return lepic_fail(ctx, LEPIC_ERR_ARG, "push_bc_deposit_inplace: null argument");
   if (bf && bf->mode == 2 && !bf->B4)
     return lepic_fail(ctx, LEPIC_ERR_ARG, "push_bc_deposit_inplace: null B map");
+  if (bf && (bf->mode < 0 || bf->mode > 2))
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "push_bc_deposit_inplace: bad B mode");
+  const int bmode = bf ? bf->mode : 0;
   const int64_t nodes = grid_nodes(g);
   double *rep = nullptr;
   if (np_local) {
@@ -1230,14 +1268,30 @@ int lepic_push_bc_deposit_inplace(lepic_ctx *ctx, const lepic_grid *g, lepic_pse
     bcnd = wt;
     if (np_local) {
       const double kdep = bp->Nm / (g->h[0] * g->h[1] * g->h[2]);
-      hipLaunchKernelGGL(k_push, dim3(grid_blocks(p->n, CTILE)), dim3(BLOCK), 0, ctx->stream, kg,
+      switch (bmode) {
+        case 0: hipLaunchKernelGGL((k_push<0>), dim3(grid_blocks(p->n, CTILE)), dim3(BLOCK), 0, ctx->stream, kg,
                          kp(p), p->n, reinterpret_cast<const double4 *>(E4), kbf, k1, dt, bcnd,
-                         kbc(bp), kq, kdep, rep, nodes, 1, p_mac, P_loss, ctr);
+                         kbc(bp), kq, kdep, rep, nodes, 1, p_mac, P_loss, ctr); break;
+        case 1: hipLaunchKernelGGL((k_push<1>), dim3(grid_blocks(p->n, CTILE)), dim3(BLOCK), 0, ctx->stream, kg,
+                         kp(p), p->n, reinterpret_cast<const double4 *>(E4), kbf, k1, dt, bcnd,
+                         kbc(bp), kq, kdep, rep, nodes, 1, p_mac, P_loss, ctr); break;
+        default: hipLaunchKernelGGL((k_push<2>), dim3(grid_blocks(p->n, CTILE)), dim3(BLOCK), 0, ctx->stream, kg,
+                         kp(p), p->n, reinterpret_cast<const double4 *>(E4), kbf, k1, dt, bcnd,
+                         kbc(bp), kq, kdep, rep, nodes, 1, p_mac, P_loss, ctr); break;
+      }
       LEPIC_LAUNCH_CHECK(ctx, "k_push");
     } else {
-      hipLaunchKernelGGL(k_push_nodep, dim3(grid_blocks(p->n, BLOCK)), dim3(BLOCK), 0,
+      switch (bmode) {
+        case 0: hipLaunchKernelGGL((k_push_nodep<0>), dim3(grid_blocks(p->n, BLOCK)), dim3(BLOCK), 0,
                          ctx->stream, kg, kp(p), p->n, reinterpret_cast<const double4 *>(E4),
-                         kbf, k1, dt, bcnd, kbc(bp), p_mac, P_loss, ctr);
+                         kbf, k1, dt, bcnd, kbc(bp), p_mac, P_loss, ctr); break;
+        case 1: hipLaunchKernelGGL((k_push_nodep<1>), dim3(grid_blocks(p->n, BLOCK)), dim3(BLOCK), 0,
+                         ctx->stream, kg, kp(p), p->n, reinterpret_cast<const double4 *>(E4),
+                         kbf, k1, dt, bcnd, kbc(bp), p_mac, P_loss, ctr); break;
+        default: hipLaunchKernelGGL((k_push_nodep<2>), dim3(grid_blocks(p->n, BLOCK)), dim3(BLOCK), 0,
+                         ctx->stream, kg, kp(p), p->n, reinterpret_cast<const double4 *>(E4),
+                         kbf, k1, dt, bcnd, kbc(bp), p_mac, P_loss, ctr); break;
+      }
       LEPIC_LAUNCH_CHECK(ctx, "k_push_nodep");
     }
   }
@@ -1268,6 +1322,9 @@ int lepic_push_bc_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset 
     return lepic_fail(ctx, LEPIC_ERR_ARG, "push_bc_deposit: null argument");
   if (bf && bf->mode == 2 && !bf->B4)
     return lepic_fail(ctx, LEPIC_ERR_ARG, "push_bc_deposit: null B map");
+  if (bf && (bf->mode < 0 || bf->mode > 2))
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "push_bc_deposit: bad B mode");
+  const int bmode = bf ? bf->mode : 0;
   if (dst->cap < src->n)
     return lepic_fail(ctx, LEPIC_ERR_CAPACITY, "push_bc_deposit: dst too small");
   if (src == dst || src->x == dst->x)
@@ -1288,10 +1345,20 @@ int lepic_push_bc_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset 
       ctx, n,
       [&](dim3 grid, uint32_t epoch, int *counter, int64_t *spin, int64_t *count_out,
           int64_t ntiles) {
-        hipLaunchKernelGGL(k_push_bc_deposit, grid, dim3(BLOCK), 0, ctx->stream, kg, kp(src),
+        switch (bmode) {
+          case 0: hipLaunchKernelGGL((k_push_bc_deposit<0>), grid, dim3(BLOCK), 0, ctx->stream, kg, kp(src),
                            kp(dst), n, reinterpret_cast<const double4 *>(E4), kbf, k1, dt, bcnd,
                            kb, kq, kdep, rep, nodes, p_mac, P_loss, ctx->d_status, epoch,
-                           counter, spin, count_out, ntiles);
+                           counter, spin, count_out, ntiles); break;
+          case 1: hipLaunchKernelGGL((k_push_bc_deposit<1>), grid, dim3(BLOCK), 0, ctx->stream, kg, kp(src),
+                           kp(dst), n, reinterpret_cast<const double4 *>(E4), kbf, k1, dt, bcnd,
+                           kb, kq, kdep, rep, nodes, p_mac, P_loss, ctx->d_status, epoch,
+                           counter, spin, count_out, ntiles); break;
+          default: hipLaunchKernelGGL((k_push_bc_deposit<2>), grid, dim3(BLOCK), 0, ctx->stream, kg, kp(src),
+                           kp(dst), n, reinterpret_cast<const double4 *>(E4), kbf, k1, dt, bcnd,
+                           kb, kq, kdep, rep, nodes, p_mac, P_loss, ctx->d_status, epoch,
+                           counter, spin, count_out, ntiles); break;
+        }
       },
       &nout);
   if (st) return st;
