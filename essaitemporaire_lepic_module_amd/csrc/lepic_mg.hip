@@ -5,11 +5,12 @@
 //
 // Each smoother sweep is one launch per colour over that colour's nodes
 // (x fastest), followed by a deterministic block-tree L1 residual and a
-// one-block "decide" kernel that applies sor_rb's exit rules on the device
-// (res <= eps on the coarsest level, res/res_s <= 0.6 elsewhere, at most 4
-// sweeps) and clears the level's active flag.  Sweeps queued after the exit
-// return at once, so a V-cycle needs no host round trip; the host reads the
-// fine residual once per V-cycle for pdesolver's convergence test.
+// one-block kernel that applies sor_rb's exit rules on the device (res <= eps
+// on the coarsest level, res/res_s <= 0.6 elsewhere, at most 4 sweeps) and
+// clears the level's active flag.  Sweeps queued after the exit return at once, so
+// a V-cycle needs no host round trip; the host reads the fine residual once
+// per V-cycle for pdesolver's convergence test.  Levels below a few thousand
+// nodes run inside one workgroup (k_mg_coarse).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -180,8 +181,7 @@ __device__ inline double resid_node(const KLevel &L, int64_t t) {
 __global__ __launch_bounds__(BLOCK) void k_resid(KLevel L, double *partials, const MGCtl *ctl) {
   if (ctl->active == 0) return;
   __shared__ double red[BLOCK];
-  const double a = resid_node(L, blockIdx.x * (int64_t)BLOCK + threadIdx.x);
-  red[threadIdx.x] = a;
+  red[threadIdx.x] = resid_node(L, blockIdx.x * (int64_t)BLOCK + threadIdx.x);
   __syncthreads();
   for (int s = BLOCK / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
@@ -191,7 +191,8 @@ __global__ __launch_bounds__(BLOCK) void k_resid(KLevel L, double *partials, con
 }
 
 // Sum the partials in a fixed order and apply sor_rb's exit rules
-// (sors.f90:446-454).
+// (sors.f90:446-454).  A separate one-block launch: the kernel boundary
+// makes every block's partial visible without a fence per block.
 __global__ __launch_bounds__(BLOCK) void k_decide(const double *partials, int64_t nparts,
                                                   MGCtl *ctl, double eps, int dig, int64_t nr) {
   if (ctl->active == 0) return;
@@ -216,6 +217,7 @@ __global__ __launch_bounds__(BLOCK) void k_decide(const double *partials, int64_
     if (stop || ctl->ksor >= nr) ctl->active = 0;
   }
 }
+
 
 // r_pbc: residual of plane k = m, the periodic image of k = 0
 // (mg.f90:322-392).  The reference loops over coarse (j, i) and each
@@ -460,14 +462,15 @@ struct CoarseArgs {
   MGCtl *ctl;
 };
 
+// Workgroup sum in a fixed order: butterfly within each wave, then the 16
+// wave sums in index order.  Every thread gets the result.
 __device__ inline double wg_sum(double a, double *red) {
-  red[threadIdx.x] = a;
+  for (int d = 32; d > 0; d >>= 1) a += __shfl_xor(a, d);
+  const int wave = threadIdx.x / LEPIC_WAVE;
+  if (__lane_id() == 0) red[wave] = a;
   __syncthreads();
-  for (int s2 = CBLOCK / 2; s2 > 0; s2 >>= 1) {
-    if ((int)threadIdx.x < s2) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s2];
-    __syncthreads();
-  }
-  const double r = red[0];
+  double r = 0.0;
+  for (int w = 0; w < CBLOCK / LEPIC_WAVE; ++w) r = r + red[w];
   __syncthreads();
   return r;
 }
@@ -798,8 +801,10 @@ extern "C" int lepic_poisson_solve(lepic_ctx *ctx, const lepic_grid *g, double *
     }
     LEPIC_LAUNCH_CHECK(ctx, "multigrid launch");
     // the V-cycle's residual is the last sor_rb's: level 1 (or the only level)
-    LEPIC_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, &S.ctl[0], sizeof(MGCtl), hipMemcpyDeviceToHost,
-                                  ctx->stream));
+    // every level's counters come back with the fine residual (no extra
+    // round trip for the statistics)
+    LEPIC_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, S.ctl, sizeof(MGCtl) * MAX_LEVELS,
+                                  hipMemcpyDeviceToHost, ctx->stream));
     LEPIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     double r;
     std::memcpy(&r, &ctx->h_pinned[2], sizeof(double));
@@ -814,9 +819,6 @@ extern "C" int lepic_poisson_solve(lepic_ctx *ctx, const lepic_grid *g, double *
   }
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
-    LEPIC_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, S.ctl, sizeof(MGCtl) * MAX_LEVELS,
-                                  hipMemcpyDeviceToHost, ctx->stream));
-    LEPIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
     double ktot = 0.0;
     for (int l = 0; l < ng; ++l) {
       int64_t tot = ctx->h_pinned[16 * l + 4];

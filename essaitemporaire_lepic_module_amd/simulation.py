@@ -134,10 +134,15 @@ class Simulation:
         return out
 
     # ------------------------------------------------------------------
-    def ensure_headroom(self, factor: float = 0.3):
+    def ensure_headroom(self):
+        """Room for the products MCC appends: the larger of 2^20 slots and
+        1/64 of the set, far above what a collision step creates at the
+        reference's densities (the kernel reports ERR_CAPACITY rather than
+        drop a product if it were ever exceeded)."""
         for p in self.parts:
-            if p.n + int(factor * p.n) + 1024 > p.nmax:
-                p.ensure_capacity(p.n + int(factor * p.n) + 1024)
+            need = p.n + max(1 << 20, p.n // 64)
+            if need > p.nmax:
+                p.ensure_capacity(need)
 
     def advance_one_step(self, istep: int):
         c = self.case
