@@ -24,6 +24,24 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_PARTICLE_STEP = 104   # FP64 fused gather+push+BC+deposit, state read and written once
+COPY_PROBE_BYTES = 1 << 31
+
+
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of the fused particle kernel on this workload, from
+    the committed rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE summary (counter
+    readings scaled by what they read for a plain copy of known size); None
+    when no summary covers this kernel and workload."""
+    if args.n != 128 or args.ppc != 64:
+        return None
+    path = os.path.join(ROOT, "profiles", f"{kernel}_128cube_64ppc_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(case, sample_particles=2_000_000, seconds=20.0):
@@ -81,12 +99,15 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--ppc", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--resort-electrons", type=int, default=8,
+    ap.add_argument("--resort-electrons", type=int, default=4,
                     help="production path: also re-sort electrons by cell every k steps "
                          "(0: only at the reference's sort cadence)")
     ap.add_argument("--exact-order", action="store_true",
                     help="compact every step by look-back (the reference's particle order "
                          "at every step) instead of in place with holes removed at each sort")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed steps, copy 2 GiB device to device once: the "
+                         "calibration kernel for rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -120,8 +141,14 @@ def main():
         istep += 1
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    if args.copy_probe:
+        a = torch.ones(COPY_PROBE_BYTES // 8, dtype=torch.float64, device="cuda:0")
+        torch.empty_like(a).copy_(a)
+        torch.cuda.synchronize()
+        del a
     phases = sim.phase_times()
     fused_ms, fused_launches, fused_units = phases.get("push_bc_deposit", (0.0, 0, 0))
+    kernel = "k_push_bc_deposit" if args.exact_order else "k_push"
     achieved = BYTES_PER_PARTICLE_STEP * fused_units / (fused_ms * 1e-3) / 1e9 if fused_ms else 0.0
     mg_ms, mg_calls, _ = phases.get("poisson", (0.0, 1, 0))
     line = {
@@ -145,11 +172,13 @@ def main():
         "mg_solve_ms": mg_ms / max(mg_calls, 1),
         "mg_vcycles_per_solve": sum(mg_cycles) / max(len(mg_cycles), 1),
         "phase_ms_per_step": {k: v[0] / args.steps for k, v in phases.items()},
-        "roofline": {"bound": "hbm",
-                     "kernel": "k_push_bc_deposit" if args.exact_order else "k_push",
+        "roofline": {"bound": "hbm", "kernel": kernel,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(kernel, args),
                      "bytes_per_particle": BYTES_PER_PARTICLE_STEP,
+                     "algorithmic_bytes_per_launch":
+                         BYTES_PER_PARTICLE_STEP * fused_units / max(fused_launches, 1),
                      "avg_launch_ms": fused_ms / max(fused_launches, 1)},
     }
     if not args.no_cpu_baseline:

@@ -10,7 +10,7 @@ CSRC = os.path.join(HERE, "csrc")
 
 def build(jobs: int = 5, arch: str = "gfx950") -> str:
     env = dict(os.environ)
-    subprocess.run(["make", "-s", f"-j{jobs}", "-C", CSRC, f"ARCH={arch}"], check=True, env=env)
+    subprocess.run(["make", "-s", "-B", f"-j{jobs}", "-C", CSRC, f"ARCH={arch}"], check=True, env=env)
     lib = os.path.join(HERE, "_lib", "liblepic_hip.so")
     if not os.path.exists(lib):
         raise RuntimeError("build finished without producing liblepic_hip.so")
