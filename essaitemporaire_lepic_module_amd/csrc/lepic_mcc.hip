@@ -48,10 +48,23 @@ struct MccCounters {
   int64_t nsel[LEPIC_MAX_SPECIES];       // select-pass list lengths
 };
 
+// Reaction plan of a projectile whose reactions all have neutral targets
+// (at most FR reactions, FT distinct targets), built on the host from
+// sig_list and col_info; ok == 0 sends the projectile to the general trial.
+constexpr int FR = 8;
+constexpr int FT = 2;
+struct FastPlan {
+  int32_t ok, nr, ntg;
+  int32_t tgt[FT];    // target species of each slot, first-seen order
+  int32_t col[FR];    // reactions in sig_list order
+  int32_t slot[FR];   // target slot of each reaction
+};
+
 struct KChem {
   lepic_chem c;
   double np_mx_all[LEPIC_MAX_SPECIES];
   double nu_max_local[LEPIC_MAX_SPECIES];
+  FastPlan fp[4];
 };
 
 struct Ran2Stream {
@@ -122,31 +135,83 @@ struct SerialAppend {
   __device__ void count(int64_t *a) { *a += 1; }
 };
 
-// One atomic per wave: the active lanes of the calling wave reserve
-// consecutive values of *ctr.
-__device__ inline unsigned long long wave_reserve(unsigned long long *ctr) {
-  const unsigned long long mask = __ballot(1);
-  const int lane = __lane_id();
-  const int leader = __ffsll(mask) - 1;
-  const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-  unsigned long long base = 0;
-  if (lane == leader) base = atomicAdd(ctr, (unsigned long long)__popcll(mask));
-  base = __shfl(base, leader);
-  return base + (unsigned long long)rank;
+// Slots for appended products: the active lanes of the calling wave, grouped
+// by the species they append to, take consecutive values of ctr[species], one
+// global atomic per species present in the wave.
+__device__ inline int64_t wave_slot(unsigned long long *ctr, int sp) {
+  unsigned long long todo = __ballot(1);
+  int64_t mine = 0;
+  while (todo) {
+    const int leader = __ffsll(todo) - 1;
+    const int sl = __shfl(sp, leader);
+    const unsigned long long grp = __ballot(sp == sl) & todo;
+    unsigned long long b = 0;
+    if (__lane_id() == leader) b = atomicAdd(&ctr[sl], (unsigned long long)__popcll(grp));
+    b = __shfl(b, leader);
+    if (sp == sl) mine = (int64_t)(b + (unsigned long long)__popcll(grp & ((1ull << __lane_id()) - 1ull)));
+    todo &= ~grp;
+  }
+  return mine;
 }
 
-struct AtomicAppend {
+// Event counters of the trial pass live in LDS (one uint32 per int64 word of
+// MccCounters) and reach the device counters once per block.
+constexpr int CNT_WORDS = (int)(sizeof(MccCounters) / sizeof(int64_t));
+
+struct BlockAppend {
   MccCounters *cnt;
-  double pcoll[4];   // tracked species (at most 4)
+  unsigned *lds;     // CNT_WORDS block counters
+  double pcoll[4];   // tracked species (at most 4); summed per block at the end
+  double *pcoll_base;
   __device__ int64_t slot(int bt) {
-    return (int64_t)wave_reserve((unsigned long long *)&cnt->n[bt]);
+    return wave_slot(reinterpret_cast<unsigned long long *>(cnt->n), bt);
   }
   __device__ void commit(int, int64_t) {}
-  // Pcoll sums per thread; flushed once per wave at the end of the kernel
-  __device__ void add(double *a, double v) { pcoll[a - pcoll_base] += v; }
-  __device__ void count(int64_t *a) { (void)wave_reserve((unsigned long long *)a); }
-  double *pcoll_base;
+  __device__ void add(double *a, double v) {
+    const int t = (int)(a - pcoll_base);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)   // branch-free, static indices: pcoll stays in registers
+      pcoll[i] += i == t ? v : 0.0;
+  }
+  // active lanes may count different words: one LDS atomic per distinct word
+  __device__ void count(int64_t *a) {
+    const int w = (int)(a - reinterpret_cast<int64_t *>(cnt));
+    unsigned long long todo = __ballot(1);
+    while (todo) {
+      const int leader = __ffsll(todo) - 1;
+      const int wl = __shfl(w, leader);
+      const unsigned long long grp = __ballot(w == wl) & todo;
+      if (__lane_id() == leader) atomicAdd(&lds[wl], (unsigned)__popcll(grp));
+      todo &= ~grp;
+    }
+  }
 };
+
+// Exclusive offset of this thread's `count` items in a block-wide run taken
+// from *ctr with one atomic per block.  Every thread of the block calls it.
+__device__ inline int64_t block_reserve(int count, unsigned long long *ctr) {
+  __shared__ int wsum[BLOCK / 64];
+  __shared__ unsigned long long base;
+  const int lane = __lane_id(), w = threadIdx.x >> 6;
+  int v = count;
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(v, d);
+    if (lane >= d) v += o;
+  }
+  if (lane == 63) wsum[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int i = 0; i < BLOCK / 64; ++i) {
+      const int s = wsum[i];
+      wsum[i] = tot;
+      tot += s;
+    }
+    base = tot ? atomicAdd(ctr, (unsigned long long)tot) : 0ull;
+  }
+  __syncthreads();
+  return (int64_t)(base + (unsigned long long)(wsum[w] + v - count));
+}
 
 // One null-collision trial on particle ip of species pt
 // (mod_collisions.f90:406-726).
@@ -336,6 +401,156 @@ __device__ void trial(const KGrid &g, const KParts &P, const KChem &K, const dou
   }
 }
 
+// The trial above for a projectile with a FastPlan, PHILOX mode only.  Same
+// process, arranged for registers: each neutral target is drawn once (first-
+// seen order, 4 draws) and its cross-section bracket found once; the reaction
+// is the first, in ascending nu (ties in sig_list order), whose running sum
+// reaches rnd -- the exchange sort's choice -- found by ranking instead of
+// sorting; rnd is drawn after the targets, as above.
+template <class APP>
+__device__ void trial_fast(const KParts &P, const KChem &K, int pt, int64_t ip,
+                           PhiloxStream &rng, APP &app, MccCounters *cnt) {
+  const lepic_chem &c = K.c;
+  const FastPlan &fp = K.fp[pt];
+  const int NT = c.ntype_tracked, NA = c.ntype_total;
+  if (P.dead[pt][ip] != 0) {
+    app.count(&cnt->skipped_dead[pt]);
+    return;
+  }
+  const double vx1 = P.vx[pt][ip], vy1 = P.vy[pt][ip], vz1 = P.vz[pt][ip];
+  const double mp = fabs(c.mass[pt]);
+  double tvx[FT], tvy[FT], tvz[FT], tvr[FT], tEk[FT];
+  int tip[FT];
+#pragma unroll
+  for (int sl = 0; sl < FT; ++sl) {
+    tvx[sl] = tvy[sl] = tvz[sl] = tvr[sl] = tEk[sl] = 0.0;
+    tip[sl] = -1;
+    if (sl < fp.ntg) {
+      const int tt = fp.tgt[sl];
+      double vx2, vy2, vz2, dummy;
+      double r1 = rng.uniform(), r2 = rng.uniform();
+      box_muller(vx2, vy2, c.vt0[tt], r1, r2);
+      r1 = rng.uniform(); r2 = rng.uniform();
+      box_muller(vz2, dummy, c.vt0[tt], r1, r2);
+      const double mt = fabs(c.mass[tt]);
+      const double mu = mp * mt / (mp + mt);
+      const double dx = vx1 - vx2, dy = vy1 - vy2, dz = vz1 - vz2;
+      const double vr = sqrt(dx * dx + dy * dy + dz * dz);
+      tvx[sl] = vx2; tvy[sl] = vy2; tvz[sl] = vz2;
+      tvr[sl] = vr;
+      tEk[sl] = 0.5 * mu * (vr * vr) / QE_SI;
+      tip[sl] = find_bracket(c.sig_Er, c.npt, tEk[sl]);
+    }
+  }
+  double nu[FR];
+  double sum_nu = 0.0;
+#pragma unroll
+  for (int r = 0; r < FR; ++r) {
+    nu[r] = 0.0;
+    if (r < fp.nr) {
+      const bool s1 = FT > 1 && fp.slot[r] == 1;
+      const int ipt = s1 ? tip[1] : tip[0];
+      if (ipt > 0) {
+        const int col = fp.col[r];
+        const double Ekr = s1 ? tEk[1] : tEk[0];
+        const double vr = s1 ? tvr[1] : tvr[0];
+        const double EkL = c.sig_Er[ipt - 1], EkR = c.sig_Er[ipt];
+        const double sL = c.sig[(int64_t)col * c.npt + ipt - 1];
+        const double sR = c.sig[(int64_t)col * c.npt + ipt];
+        const double sig_p = sL + (Ekr - EkL) * (sR - sL) / (EkR - EkL);
+        nu[r] = K.np_mx_all[fp.tgt[s1 ? 1 : 0]] * sig_p * vr / K.nu_max_local[pt];
+        sum_nu = sum_nu + nu[r];
+      }
+    }
+  }
+  const double rnd = rng.uniform();
+  if (!(rnd <= sum_nu)) return;
+  int pick = -1;
+  double best = INFINITY;
+#pragma unroll
+  for (int a = 0; a < FR; ++a) {
+    if (a < fp.nr) {
+      double run = nu[a];
+#pragma unroll
+      for (int b = 0; b < FR; ++b)
+        if (b < fp.nr && b != a && (nu[b] < nu[a] || (nu[b] == nu[a] && b < a))) run = run + nu[b];
+      if (rnd <= run && run < best) {
+        best = run;
+        pick = a;
+      }
+    }
+  }
+  if (pick < 0) return;
+  app.count(&cnt->accepted[pt]);
+  const int c_ind = fp.col[pick];
+  const bool s1 = FT > 1 && fp.slot[pick] == 1;
+  const int tt = fp.tgt[s1 ? 1 : 0];
+  const double vx2 = s1 ? tvx[1] : tvx[0], vy2 = s1 ? tvy[1] : tvy[0], vz2 = s1 ? tvz[1] : tvz[0];
+  const double vr = s1 ? tvr[1] : tvr[0];
+  const double mt = fabs(c.mass[tt]);
+  const double mu = mp * mt / (mp + mt);
+  const double vxc = (mp * vx1 + mt * vx2) / (mp + mt);
+  const double vyc = (mp * vy1 + mt * vy2) / (mp + mt);
+  const double vzc = (mp * vz1 + mt * vz2) / (mp + mt);
+  const int n_re = c.col_info[c_ind * 10 + 0];
+  const int n_by = c.col_info[c_ind * 10 + 1];
+  const double Eth = c.sig_Eth[c_ind];
+  const double Ee = 0.5 * mu * vr * vr - Eth * QE_SI;
+  if (Ee < 0.0) return;
+  double smi = 0.0;
+  for (int ib = 0; ib < n_by; ++ib) {
+    const int bt = c.col_info[c_ind * 10 + 2 + n_re + ib];
+    if (bt >= 0 && bt < NA && c.mass[bt] > 0.0) smi = smi + 1.0 / fabs(c.mass[bt]);
+  }
+  if (smi <= 0.0) return;
+  const double costh = 1.0 - 2.0 * rng.uniform();
+  const double phi = 2.0 * PI_F * rng.uniform();
+  const double costh_s = 1.0 - 2.0 * rng.uniform();
+  const double phi_s = 2.0 * PI_F * rng.uniform();
+  const double th_add = 2.0 * PI_F / (double)(n_by > 1 ? n_by : 1);
+  bool reused = false;
+  for (int ib = 0; ib < n_by; ++ib) {
+    const int bt = c.col_info[c_ind * 10 + 2 + n_re + ib];
+    if (bt < 0 || bt >= NT) continue;
+    if (c.mass[bt] <= 0.0) continue;
+    const double s2 = 1.0 - costh * costh;
+    const double st = sqrt(0.0 > s2 ? 0.0 : s2);
+    const double ex = costh;
+    const double ey = st * sin(phi + th_add * (double)ib);
+    const double ez = st * cos(phi + th_add * (double)ib);
+    double ex1, ey1, ez1;
+    scatter_vector(ex1, ey1, ez1, ex, ey, ez, costh_s, phi_s);
+    const double mb = fabs(c.mass[bt]);
+    const double vp = sqrt(2.0 * (Ee / smi) / (mb * mb));
+    const double nvx = vxc + vp * ex1, nvy = vyc + vp * ey1, nvz = vzc + vp * ez1;
+    const double v2new = nvx * nvx + nvy * nvy + nvz * nvz;
+    if (ib == 0 && bt == pt) {
+      // the projectile itself carries the first same-species byproduct
+      reused = true;
+      P.vx[pt][ip] = nvx; P.vy[pt][ip] = nvy; P.vz[pt][ip] = nvz;
+      const double v2old = vx1 * vx1 + vy1 * vy1 + vz1 * vz1;
+      app.add(&app.pcoll_base[bt], 0.5 * c.Nm[bt] * c.mass[bt] * (v2new - v2old));
+      continue;
+    }
+    const int64_t ibx = app.slot(bt);
+    if (ibx >= P.cap[bt]) {
+      atomicExch((unsigned long long *)&cnt->overflow, 1ull);
+      continue;
+    }
+    P.x[bt][ibx] = P.x[pt][ip]; P.y[bt][ibx] = P.y[pt][ip]; P.z[bt][ibx] = P.z[pt][ip];
+    P.vx[bt][ibx] = nvx; P.vy[bt][ibx] = nvy; P.vz[bt][ibx] = nvz;
+    P.dead[bt][ibx] = 0;
+    app.count(&cnt->appended[bt]);
+    app.add(&app.pcoll_base[bt], 0.5 * c.Nm[bt] * c.mass[bt] * v2new);
+  }
+  for (int ir = 0; ir < n_re; ++ir) {
+    const int ct = c.col_info[c_ind * 10 + 2 + ir];
+    if (ct < 0 || ct >= NT) continue;
+    if (c.mass[ct] <= 0.0) continue;
+    if (ct == pt && !(ir == 0 && reused)) P.dead[pt][ip] = 1;
+  }
+}
+
 // REPLAY_RAN2: the whole chain on one device thread (trial order and ran2
 // consumption of mod_collisions.f90:391-728 for one virtual thread).
 __global__ void k_mcc_replay(KGrid g, KParts P, KChem K, const double *np_red, int64_t nodes,
@@ -376,42 +591,56 @@ __global__ __launch_bounds__(BLOCK) void k_mcc_select(const int8_t *__restrict__
                                                       int64_t cap, unsigned long long *nsel) {
   const int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
   const int64_t begin = t * SEL_CHUNK;
-  if (begin >= n0) return;
   const int64_t end = begin + SEL_CHUNK < n0 ? begin + SEL_CHUNK : n0;
-  PhiloxStream rng;
-  rng.init(key ^ ((uint64_t)pt << 56) ^ 0x5E1EC7A11ull, (uint32_t)t, (uint32_t)(t >> 32),
-           (uint32_t)counter);
-  rng.c3 ^= (uint32_t)(counter >> 32) * 0x9E3779B9u;
-  int64_t j = begin - 1;
-  while (true) {
-    const double gap = -log(1.0 - rng.uniform()) / lambda;   // u in [0,1): finite, >= 0
-    if (gap >= (double)(end - 1 - j)) break;
-    j += 1 + (int64_t)gap;
-    const double u = rng.uniform();
-    double pk = p1, cdf = p1;
-    int k = 1;
-    while (u > cdf && k < 32) {
-      ++k;
-      pk = pk * lambda / (double)k;
-      cdf = cdf + pk;
+  PhiloxStream rng0;
+  rng0.init(key ^ ((uint64_t)pt << 56) ^ 0x5E1EC7A11ull, (uint32_t)t, (uint32_t)(t >> 32),
+            (uint32_t)counter);
+  rng0.c3 ^= (uint32_t)(counter >> 32) * 0x9E3779B9u;
+  // two walks over the same draws: count the listed particles, take the
+  // block's run of the list, then write them in particle order
+  int64_t slot = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    PhiloxStream rng = rng0;
+    int count = 0;
+    int64_t j = begin - 1;
+    while (begin < end) {
+      const double gap = -log(1.0 - rng.uniform()) / lambda;   // u in [0,1): finite, >= 0
+      if (gap >= (double)(end - 1 - j)) break;
+      j += 1 + (int64_t)gap;
+      const double u = rng.uniform();
+      double pk = p1, cdf = p1;
+      int k = 1;
+      while (u > cdf && k < 32) {
+        ++k;
+        pk = pk * lambda / (double)k;
+        cdf = cdf + pk;
+      }
+      if (dead[j] != 0) continue;
+      if (pass == 1 && slot < cap) list[slot] = (uint64_t)j | ((uint64_t)k << 40);
+      if (pass == 1) ++slot;
+      ++count;
     }
-    if (dead[j] != 0) continue;
-    const unsigned long long slot = wave_reserve(nsel);
-    if ((int64_t)slot < cap) list[slot] = (uint64_t)j | ((uint64_t)k << 40);
+    if (pass == 0) slot = block_reserve(count, nsel);
   }
 }
 
 // PHILOX trial pass: one thread per listed particle, k trials in a row on the
 // particle's own stream.
+template <bool FAST>
 __global__ __launch_bounds__(BLOCK) void k_mcc_trials(KGrid g, KParts P, KChem K,
                                                       const double *np_red, int64_t nodes, int pt,
                                                       const uint64_t *list,
                                                       const unsigned long long *nsel, int64_t cap,
                                                       uint64_t key, uint64_t counter,
                                                       MccCounters *cnt, double *Pcoll) {
+  __shared__ unsigned lcnt[CNT_WORDS];
+  __shared__ double lp[BLOCK / 64][4];
+  for (int i = threadIdx.x; i < CNT_WORDS; i += BLOCK) lcnt[i] = 0u;
+  __syncthreads();
   const int64_t e = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
-  AtomicAppend app;
+  BlockAppend app;
   app.cnt = cnt;
+  app.lds = lcnt;
   app.pcoll_base = Pcoll;
   for (int t = 0; t < 4; ++t) app.pcoll[t] = 0.0;
   const int64_t m = (int64_t)*nsel < cap ? (int64_t)*nsel : cap;
@@ -422,14 +651,29 @@ __global__ __launch_bounds__(BLOCK) void k_mcc_trials(KGrid g, KParts P, KChem K
     PhiloxStream rng;
     rng.init(key ^ ((uint64_t)pt << 56), (uint32_t)ip, (uint32_t)(ip >> 32), (uint32_t)counter);
     rng.c3 ^= (uint32_t)(counter >> 32) * 0x9E3779B9u;
-    for (int t = 0; t < k; ++t) trial(g, P, K, np_red, nodes, pt, ip, rng, app, cnt, Pcoll);
+    for (int t = 0; t < k; ++t) {
+      if constexpr (FAST) trial_fast(P, K, pt, ip, rng, app, cnt);
+      else trial(g, P, K, np_red, nodes, pt, ip, rng, app, cnt, Pcoll);
+    }
   }
-  // Pcoll: wave sums, one atomic per wave and species
-  for (int t = 0; t < K.c.ntype_tracked; ++t) {
+  // Pcoll: wave sums, then one atomic per block and species
+  const int NT = K.c.ntype_tracked;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    if (t >= NT) break;
     double v = app.pcoll[t];
     for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
-    if (__lane_id() == 0 && v != 0.0) atomicAdd(&Pcoll[t], v);
+    if (__lane_id() == 0) lp[threadIdx.x >> 6][t] = v;
   }
+  __syncthreads();
+  if (threadIdx.x < NT) {
+    double v = 0.0;
+    for (int i = 0; i < BLOCK / 64; ++i) v += lp[i][threadIdx.x];
+    if (v != 0.0) atomicAdd(&Pcoll[threadIdx.x], v);
+  }
+  for (int i = threadIdx.x; i < CNT_WORDS; i += BLOCK)
+    if (lcnt[i] != 0u)
+      atomicAdd(reinterpret_cast<unsigned long long *>(cnt) + i, (unsigned long long)lcnt[i]);
 }
 
 __global__ void k_fill_u64(uint64_t *a, int64_t n) {
@@ -540,6 +784,31 @@ extern "C" int lepic_collisions(lepic_ctx *ctx, const lepic_grid *g, lepic_pset 
     if (ntg > MAXT)
       return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions: more than 4 neutral targets per projectile");
   }
+  // register-friendly plans for projectiles whose targets are all neutral
+  for (int t = 0; t < NT; ++t) {
+    FastPlan &fp = K.fp[t];
+    fp.ok = c.p_ncol[t] <= FR;
+    for (int ic = 0; ic < c.p_ncol[t] && fp.ok; ++ic) {
+      const int col = hsl[(size_t)t * LEPIC_MAX_COL + ic];
+      const int tt = hci[(size_t)col * 10 + 1 + hci[(size_t)col * 10]];
+      if (tt < NT || tt >= NA) {   // charged or invalid target: general trial
+        fp.ok = 0;
+        break;
+      }
+      int sl = 0;
+      while (sl < fp.ntg && fp.tgt[sl] != tt) ++sl;
+      if (sl == fp.ntg) {
+        if (fp.ntg == FT) {
+          fp.ok = 0;
+          break;
+        }
+        fp.tgt[fp.ntg++] = tt;
+      }
+      fp.col[fp.nr] = col;
+      fp.slot[fp.nr] = sl;
+      ++fp.nr;
+    }
+  }
   int32_t Nc[4] = {0, 0, 0, 0};
   double lambda[4] = {0, 0, 0, 0};
   for (int t = 0; t < NT; ++t) {
@@ -616,10 +885,16 @@ extern "C" int lepic_collisions(lepic_ctx *ctx, const lepic_grid *g, lepic_pset 
           cap = m;
           continue;
         }
-        if (m > 0)
-          hipLaunchKernelGGL(k_mcc_trials, dim3(grid_blocks(m, BLOCK)), dim3(BLOCK), 0,
-                             ctx->stream, kg, P, K, np_red, nodes, t, list, nsel, cap, philox_key,
-                             philox_counter, dc, Pcoll);
+        if (m > 0) {
+          if (K.fp[t].ok)
+            hipLaunchKernelGGL(k_mcc_trials<true>, dim3(grid_blocks(m, BLOCK)), dim3(BLOCK), 0,
+                               ctx->stream, kg, P, K, np_red, nodes, t, list, nsel, cap, philox_key,
+                               philox_counter, dc, Pcoll);
+          else
+            hipLaunchKernelGGL(k_mcc_trials<false>, dim3(grid_blocks(m, BLOCK)), dim3(BLOCK), 0,
+                               ctx->stream, kg, P, K, np_red, nodes, t, list, nsel, cap, philox_key,
+                               philox_counter, dc, Pcoll);
+        }
         LEPIC_LAUNCH_CHECK(ctx, "k_mcc_trials");
         break;
       }

@@ -36,28 +36,28 @@ __host__ __device__ inline double u01_53(uint32_t a, uint32_t b) {
   return (double)(x >> 11) * (1.0 / 9007199254740992.0);
 }
 
-// A small stream view: key = (k0,k1), counter prefix (c1,c2,c3), draw index in c0.
+// A stream view: key = (k0,k1), counter prefix (c1,c2,c3), block index in c0.
+// Each block of four words gives two uniforms, words (0,1) then (2,3); the
+// second is kept as a double, so the stream holds no indexed word buffer
+// (which the compiler would place in LDS or scratch).
 struct PhiloxStream {
   uint32_t k0, k1, c1, c2, c3;
   uint32_t idx;
-  uint32_t buf[4];
-  int left;
+  double spare;
+  bool has_spare;
   __host__ __device__ void init(uint64_t key, uint32_t a, uint32_t b, uint32_t c) {
     k0 = (uint32_t)key; k1 = (uint32_t)(key >> 32);
     c1 = a; c2 = b; c3 = c;
-    idx = 0; left = 0;
-  }
-  __host__ __device__ uint32_t next32() {
-    if (left == 0) {
-      philox4 r = philox4x32_10(idx++, c1, c2, c3, k0, k1);
-      buf[0] = r.v[0]; buf[1] = r.v[1]; buf[2] = r.v[2]; buf[3] = r.v[3];
-      left = 4;
-    }
-    return buf[4 - (left--)];
+    idx = 0; has_spare = false; spare = 0.0;
   }
   __host__ __device__ double uniform() {
-    uint32_t a = next32();
-    uint32_t b = next32();
-    return u01_53(a, b);
+    if (has_spare) {
+      has_spare = false;
+      return spare;
+    }
+    philox4 r = philox4x32_10(idx++, c1, c2, c3, k0, k1);
+    spare = u01_53(r.v[2], r.v[3]);
+    has_spare = true;
+    return u01_53(r.v[0], r.v[1]);
   }
 };

@@ -192,3 +192,55 @@ def test_mcc_philox_ionization_products(gpu, rng):
     assert all(p in parent for p in prod_e)
     assert sorted(prod_e) == sorted(prod_i)
     assert (ge["dead"][N:] == 0).all() and (gi["dead"][N:] == 0).all()
+
+
+def test_mcc_philox_fast_path_matches_general_trial(gpu, rng):
+    """Projectiles whose targets are all neutral take the register-only trial;
+    adding a zero cross-section reaction on a charged target sends electrons
+    through the general trial instead.  That reaction draws nothing and never
+    fires, so both see the same draws and must make the same collisions."""
+    from essaitemporaire_lepic_module_amd import particles as P
+    c = cfg.cfg2_pic_mcc(n=4, np_cell=1)
+    n, h = c.n, c.h
+    sp = cfg.argon_species()
+    E = cfg.argon_energy_grid(200)
+    base = [((0, 2), (0, 2), 0.0, np.full_like(E, 1e-19)),
+            ((0, 2), (0, 2), 11.5, cfg.lotz_like(E, cfg.S0_EXCITATION, 11.5)),
+            ((0, 2), (0, 1, 0), 15.8, cfg.lotz_like(E, cfg.S0_IONIZATION, 15.8)),
+            ((1, 2), (1, 2), 0.0, np.full_like(E, 5e-19))]
+    dummy = base + [((0, 1), (0, 1), 0.0, np.zeros_like(E))]
+    N = 60_000
+    L = n[0] * h[0]
+    speed = np.sqrt(2 * rng.uniform(1, 200, N) * QE / cfg.ME)
+    dirs = rng.standard_normal((N, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    ve = speed[:, None] * dirs
+    pos = L * rng.random((N, 3))
+    dead = (rng.random(N) < 0.03).astype(np.int8)
+    np_red = torch.as_tensor(np.abs(rng.standard_normal((2,) + cfg.node_shape(n))) * 1e15,
+                             device=gpu)
+    runs = []
+    for reactions in (base, dummy):
+        t = cfg.build_reaction_tables(sp, c.ngas, reactions, [(E.copy(), r[3]) for r in reactions])
+        tables = P.ReactionTables(t, c.Nm, gpu)
+        e = P.ParticleSet(3 * N, "cuda:0")
+        e.from_host(pos[:, 0], pos[:, 1], pos[:, 2], ve[:, 0], ve[:, 1], ve[:, 2], dead=dead)
+        ion = P.ParticleSet(3 * N, "cuda:0")
+        ion.from_host(pos[:, 2], pos[:, 1], pos[:, 0], np.zeros(N), np.zeros(N), np.zeros(N))
+        st, _, Pcoll = _run_gpu(P, [e, ion], n, h, np_red, tables, c.nb_step_collisions, c.dt,
+                                _native.MCC_PHILOX, key=11, counter=4)
+        runs.append((st, Pcoll, e.to_host(), ion.to_host(), e.n, ion.n))
+    (sa, pa, ea, ia, nea, nia), (sb, pb, eb, ib, neb, nib) = runs
+    assert sa["err_coll"][0] == 0 and sb["err_coll"][0] > 0
+    for k in ("accepted", "appended", "skipped_dead"):
+        assert sa[k][:2] == sb[k][:2], k
+    assert sa["accepted"][0] > 0 and sa["appended"][0] > 0
+    assert (nea, nia) == (neb, nib)
+    for k in ("x", "y", "z", "vx", "vy", "vz", "dead"):
+        np.testing.assert_array_equal(ea[k][:N], eb[k][:N], err_msg=k)
+    # products land in atomic slot order: compare them as sets
+    for ga, gb, m in ((ea, eb, nea), (ia, ib, nia)):
+        ka = np.stack([ga[k][N:m] for k in ("x", "y", "z", "vx", "vy", "vz")], 1)
+        kb = np.stack([gb[k][N:m] for k in ("x", "y", "z", "vx", "vy", "vz")], 1)
+        np.testing.assert_array_equal(ka[np.lexsort(ka.T[::-1])], kb[np.lexsort(kb.T[::-1])])
+    np.testing.assert_allclose(pa, pb, rtol=1e-12, atol=0)
