@@ -109,12 +109,13 @@ class Simulation:
         # lazy mode: holes (lost slots not yet compacted) per species, on the device
         self.lost_acc = torch.zeros(self.ntype, dtype=torch.int64, device=dev)
         self.timers = None   # {phase: [(start_event, end_event, units), ...]} when timing
+        self.timed_phases = None   # phases to time (None: all of them)
 
     @contextlib.contextmanager
     def _phase(self, name: str, units: int = 0):
         """Record HIP events around a phase on torch's current stream (the
         stream every lepic call is bound to)."""
-        if self.timers is None:
+        if not self._timing(name):
             yield
             return
         a = torch.cuda.Event(enable_timing=True)
@@ -124,12 +125,16 @@ class Simulation:
         b.record()
         self.timers.setdefault(name, []).append((a, b, units))
 
+    def _timing(self, name: str) -> bool:
+        return self.timers is not None and (self.timed_phases is None or name in self.timed_phases)
+
     def phase_times(self) -> dict:
         """{phase: (total_ms, launches, units)} of the recorded events."""
         torch.cuda.synchronize()
         out = {}
         for k, lst in (self.timers or {}).items():
-            units = sum(int(u) for _, _, u in lst)   # device counts resolve here
+            # (slots, holes tensor) pairs: device hole counts resolve here
+            units = sum(u[0] - int(u[1]) if isinstance(u, tuple) else int(u) for _, _, u in lst)
             out[k] = (sum(a.elapsed_time(b) for a, b, _ in lst), len(lst), units)
         return out
 
@@ -177,8 +182,9 @@ class Simulation:
                 np_s = None if collide else self.np_thread[s]
                 if self.lazy:
                     # particles pushed = slots - holes before the call (read when timed)
-                    units = (p.n - self.lost_acc[s].clone()) if self.timers is not None else 0
-                    with self._phase("push_bc_deposit" if not collide else "push_bc", units):
+                    phase = "push_bc_deposit" if not collide else "push_bc"
+                    units = (p.n, self.lost_acc[s].clone()) if self._timing(phase) else 0
+                    with self._phase(phase, units):
                         P.push_bc_deposit_inplace(
                             p, n, h, self.E4, self.bf[s], self.charge[s], self.mass[s], c.dt,
                             self.bcnd, self.bp[s], self.kq, np_s, self.p_mac[s],
