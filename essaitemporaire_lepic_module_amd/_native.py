@@ -65,7 +65,7 @@ class BField(C.Structure):
 class MGParams(C.Structure):
     _fields_ = [("flag_pbc", C.c_int32), ("flag_nmn", C.c_int32), ("eps0", C.c_double),
                 ("ncycl", C.c_int32), ("eps", C.c_double), ("omega", C.c_double),
-                ("ng", C.c_int32)]
+                ("ng", C.c_int32), ("graph", C.c_int32)]
 
 
 class MGStats(C.Structure):

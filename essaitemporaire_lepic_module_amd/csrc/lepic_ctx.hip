@@ -100,6 +100,9 @@ int lepic_destroy(lepic_ctx *ctx) {
   if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
   if (ctx->d_status) (void)hipFree(ctx->d_status);
   if (ctx->mg_mem) (void)hipFree(ctx->mg_mem);
+  for (hipGraphExec_t &e : ctx->mg_exec)
+    if (e) (void)hipGraphExecDestroy(e);
+  if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
   delete ctx;
   return LEPIC_OK;
 }

@@ -9,6 +9,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "../../include/lepic.h"
 #include "lepic_qdiv.h"
@@ -35,6 +36,10 @@ struct lepic_ctx {
   size_t mg_bytes = 0;
   int32_t mg_n[3] = {0, 0, 0};
   int32_t mg_ng = 0;
+  // captured V-cycles (first and later ones) and the solve they belong to
+  hipGraphExec_t mg_exec[2] = {nullptr, nullptr};
+  std::vector<unsigned char> mg_key;
+  hipStream_t capture_stream = nullptr;
   // per-XCD deposit replicas (LEPIC_XCDS x nodes doubles, kept zeroed)
   double *rep = nullptr;
   int64_t rep_nodes = 0;

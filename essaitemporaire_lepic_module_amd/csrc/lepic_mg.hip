@@ -767,23 +767,23 @@ extern "C" int lepic_poisson_solve(lepic_ctx *ctx, const lepic_grid *g, double *
       S.lc = l;
       break;
     }
-  for (;;) {
-    // mg (mg.f90:99-239)
+  // one V-cycle, mg (mg.f90:99-239)
+  auto vcycle = [&](int kmg) -> int {
     for (int ig = 1; ig <= 2 * ng - 1; ++ig) {
       for (int il = 1; il <= ng - 1 && il <= 11; ++il) {
         const int l = il - 1;
         if (ig < ng && ig == il) {
           if (l < S.lc) {
-            int st = S.sor_rb(l, epsa, k, ig - ng);
+            int st = S.sor_rb(l, epsa, kmg, ig - ng);
             if (st) return st;
             S.restriction(l);
           } else if (l == S.lc) {
-            S.coarse(epsa, k);
+            S.coarse(epsa, kmg);
           }
         }
         if (ig > ng && ig == 2 * ng - il) {
           if (l < S.lc) {
-            int st = S.sor_rb(l, epsa, k, ig - ng);
+            int st = S.sor_rb(l, epsa, kmg, ig - ng);
             if (st) return st;
           }
           if (l <= S.lc && il >= 2) S.prolongation(il - 2);
@@ -791,13 +791,64 @@ extern "C" int lepic_poisson_solve(lepic_ctx *ctx, const lepic_grid *g, double *
       }
       if (ig == ng) {
         if (S.lc >= ng) {
-          int st = S.sor_rb(ng - 1, epsa, k, 0);
+          int st = S.sor_rb(ng - 1, epsa, kmg, 0);
           if (st) return st;
         } else if (S.lc == ng - 1) {
-          S.coarse(epsa, k);
+          S.coarse(epsa, kmg);
         }
         if (ng >= 2 && ng - 2 < S.lc) S.prolongation(ng - 2);
       }
+    }
+    return LEPIC_OK;
+  };
+  // With the coarsest level inside k_mg_coarse a V-cycle's launches are fixed
+  // (every decision is taken on the device), so they are captured once per
+  // solve configuration -- the first V-cycle and the later ones differ in one
+  // argument -- and replayed: one submission per V-cycle instead of ~100.
+  const bool replay = mp->graph && S.lc < ng;
+  if (replay) {
+    const int32_t ints[9] = {ng, g->n[0], g->n[1], g->n[2], S.flag_pbc, S.flag_nmn, S.lc, 0, 0};
+    const void *ptrs[5] = {phi, rho, bcnd, ctx->mg_mem, ctx->d_ctrl};
+    const double dbl[5] = {g->h[0], g->h[1], g->h[2], S.omega, epsa};
+    std::vector<unsigned char> key(sizeof(ints) + sizeof(ptrs) + sizeof(dbl));
+    std::memcpy(key.data(), ints, sizeof(ints));
+    std::memcpy(key.data() + sizeof(ints), ptrs, sizeof(ptrs));
+    std::memcpy(key.data() + sizeof(ints) + sizeof(ptrs), dbl, sizeof(dbl));
+    if (key != ctx->mg_key || !ctx->mg_exec[0] || !ctx->mg_exec[1]) {
+      for (hipGraphExec_t &e : ctx->mg_exec)
+        if (e) {
+          LEPIC_HIP(ctx, hipGraphExecDestroy(e));
+          e = nullptr;
+        }
+      ctx->mg_key.clear();
+      if (!ctx->capture_stream)
+        LEPIC_HIP(ctx, hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+      const hipStream_t eager = ctx->stream;
+      ctx->stream = ctx->capture_stream;   // the Solver launches on ctx->stream
+      int st = LEPIC_OK;
+      for (int which = 0; which < 2 && st == LEPIC_OK; ++which) {
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+          st = vcycle(which == 0 ? 1 : 2);
+          e = hipStreamEndCapture(ctx->capture_stream, &graph);
+        }
+        if (e == hipSuccess && st == LEPIC_OK)
+          e = hipGraphInstantiate(&ctx->mg_exec[which], graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e != hipSuccess && st == LEPIC_OK) st = lepic_fail(ctx, LEPIC_ERR_HIP, hipGetErrorString(e));
+      }
+      ctx->stream = eager;
+      if (st) return st;
+      ctx->mg_key = key;
+    }
+  }
+  for (;;) {
+    if (replay) {
+      LEPIC_HIP(ctx, hipGraphLaunch(ctx->mg_exec[k == 1 ? 0 : 1], ctx->stream));
+    } else {
+      int st = vcycle(k);
+      if (st) return st;
     }
     LEPIC_LAUNCH_CHECK(ctx, "multigrid launch");
     // the V-cycle's residual is the last sor_rb's: level 1 (or the only level)

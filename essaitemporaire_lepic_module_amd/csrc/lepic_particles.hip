@@ -21,7 +21,12 @@ namespace {
 
 constexpr int BLOCK = 256;
 constexpr int NWAVES = BLOCK / LEPIC_WAVE;
-constexpr int LDS_BOX = 4096;            // doubles of the per-block deposit box (32 KB)
+// Doubles of the per-block deposit box.  k_push holds 153 VGPRs (3 waves per
+// SIMD), which leaves 53 KB of LDS per block: its box covers the tiles that
+// straddle a row of cells after electrons drifted for a few steps since the
+// last sort (131 x 7 x 7 nodes); the other deposit kernels keep 32 KB.
+constexpr int LDS_BOX = 4096;
+constexpr int LDS_BOX_PUSH = 6656;
 constexpr long SPIN_LIMIT = 1L << 26;    // bounded look-back wait
 
 struct KPset {
@@ -367,11 +372,15 @@ __device__ inline bool deposit_weights(const KGrid &g, double k1, double x, doub
 // box is then flushed with one L2-resident atomic per touched node into this
 // XCD's replica.  Particles whose stencil leaves the box go straight to the
 // replica.  kq is 1 or 2, so kq*sum(w) equals sum(kq*w) term for term.
-struct DepShared {
+template <int NB>
+struct DepSharedT {
+  static constexpr int CAP = NB;
   int mn[3], mx[3];
   int x0, y0, z0, bx, by, bz, vol;
-  double box[LDS_BOX];
+  double box[NB];
 };
+using DepShared = DepSharedT<LDS_BOX>;
+using DepSharedPush = DepSharedT<LDS_BOX_PUSH>;
 
 __device__ inline int wave_min(int v) {
   for (int d = 32; d > 0; d >>= 1) v = min(v, __shfl_xor(v, d));
@@ -408,7 +417,8 @@ __device__ inline bool seg_sum8(int key, double v[8]) {
 
 // Pass 1: the block's cell range from the particles' current positions,
 // widened by one cell each way for motion within the step.
-__device__ inline void box_begin(const KGrid &g, DepShared &ds, int lo[3], int hi[3]) {
+template <class DS>
+__device__ inline void box_begin(const KGrid &g, DS &ds, int lo[3], int hi[3]) {
   if (threadIdx.x == 0) {
     ds.mn[0] = ds.mn[1] = ds.mn[2] = 1 << 30;
     ds.mx[0] = ds.mx[1] = ds.mx[2] = -(1 << 30);
@@ -431,7 +441,7 @@ __device__ inline void box_begin(const KGrid &g, DepShared &ds, int lo[3], int h
       ds.by = min(ds.mx[1] + 2, g.n2 + 2) - ds.y0 + 1;
       ds.bz = min(ds.mx[2] + 2, g.n3 + 2) - ds.z0 + 1;
       vol = ds.bx * ds.by * ds.bz;
-      if (vol > LDS_BOX) vol = -1;
+      if (vol > DS::CAP) vol = -1;
     }
     ds.vol = vol;
   }
@@ -442,7 +452,8 @@ __device__ inline void box_begin(const KGrid &g, DepShared &ds, int lo[3], int h
 
 // Deposit one particle per lane (has = false lanes take part in the shuffles
 // with a unique key and no weight).
-__device__ inline void box_add(const KGrid &g, DepShared &ds, bool has, int ix, int iy, int iz,
+template <class DS>
+__device__ inline void box_add(const KGrid &g, DS &ds, bool has, int ix, int iy, int iz,
                                double w[8], const double *__restrict__ kq, double *rep) {
   int l = -1;
   if (has && ds.vol > 0) {
@@ -473,7 +484,8 @@ __device__ inline void box_add(const KGrid &g, DepShared &ds, bool has, int ix, 
   }
 }
 
-__device__ inline void box_flush(const KGrid &g, DepShared &ds, const double *__restrict__ kq,
+template <class DS>
+__device__ inline void box_flush(const KGrid &g, DS &ds, const double *__restrict__ kq,
                                  double *rep) {
   __syncthreads();
   for (int t = threadIdx.x; t < ds.vol; t += BLOCK) {
@@ -526,7 +538,8 @@ __device__ inline int node_off(int q, int bx, int bxy) {
 // flush the same cell; each lane starts at node (lane mod 8) of the stencil,
 // so the lanes of one instruction hit different addresses instead of
 // serialising on the same one.
-__device__ inline void box_atomic8(DepShared &ds, int l, const double a[8]) {
+template <class DS>
+__device__ inline void box_atomic8(DS &ds, int l, const double a[8]) {
   const int bx = ds.bx, bxy = ds.bx * ds.by;
   const int rot = __lane_id() & 7;
   double b[8];
@@ -547,7 +560,8 @@ __device__ inline void box_atomic8(DepShared &ds, int l, const double a[8]) {
 
 // A run with a box key goes to LDS, a run with a global key to this XCD's
 // replica (kq applied to the run's sum: kq is 1 or 2, so the scaling is exact).
-__device__ inline void flush_run(const KGrid &g, DepShared &ds, int key, const double a[8],
+template <class DS>
+__device__ inline void flush_run(const KGrid &g, DS &ds, int key, const double a[8],
                                  const double *__restrict__ kq, double *rep) {
   if (key >= 0) {
     box_atomic8(ds, key, a);
@@ -564,7 +578,8 @@ __device__ inline void flush_run(const KGrid &g, DepShared &ds, int key, const d
 // lower node (>= 0) when the stencil lies in the block's LDS box, otherwise
 // -(global lower node) - 2, so that runs outside the box are summed in
 // registers too before their global atomics.
-__device__ inline int stage_key(const KGrid &g, const DepShared &ds, int ix, int iy, int iz) {
+template <class DS>
+__device__ inline int stage_key(const KGrid &g, const DS &ds, int ix, int iy, int iz) {
   if (ds.vol > 0) {
     const int lx = ix - ds.x0, ly = iy - ds.y0, lz = iz - ds.z0;
     if (lx >= 0 && ly >= 0 && lz >= 0 && lx + 1 < ds.bx && ly + 1 < ds.by && lz + 1 < ds.bz)
@@ -658,7 +673,8 @@ struct RunAcc {
   }
 };
 
-__device__ inline void run_add(const KGrid &g, DepShared &ds, RunAcc &ra, int key, double k1,
+template <class DS>
+__device__ inline void run_add(const KGrid &g, DS &ds, RunAcc &ra, int key, double k1,
                                double px, double py, double pz, const double *__restrict__ kq,
                                double *rep) {
   if (key != ra.cur) {
@@ -787,7 +803,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(
     const int *__restrict__ wt, KBC bp, const double *__restrict__ kq, double kdep,
     double *rep_base, int64_t nodes, int do_dep, double *p_mac, double *P_loss,
     unsigned long long *lost_ctr) {
-  __shared__ DepShared ds;
+  __shared__ DepSharedPush ds;
   double *rep = rep_base ? rep_base + (int64_t)xcc_id() * nodes : nullptr;
   const int64_t i0 = (int64_t)blockIdx.x * CTILE + (int64_t)threadIdx.x * CPT;
   const bool full = i0 + CPT <= n;

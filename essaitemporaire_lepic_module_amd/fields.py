@@ -64,15 +64,17 @@ def build_rho_from_np(n, np_red: torch.Tensor, charge, ntype: int, rho: torch.Te
 
 def solve_poisson_legacy(phi: torch.Tensor, bcnd: torch.Tensor, rhs: torch.Tensor, h, n,
                          ncycl: int, eps: float, omega: float, ng: int, flag_pbc: int,
-                         flag_nmn: int, eps0: float = EPS0) -> dict:
+                         flag_nmn: int, eps0: float = EPS0, graph: bool = True) -> dict:
     """V-cycle multigrid / red-black SOR, phi updated in place from its
     current value (mod_PoissonSolver_legacy.f90:27-101 -> pdesolver.f90).
-    Returns the V-cycle count, equivalent sweeps and final L1/alpha residual."""
+    Returns the V-cycle count, equivalent sweeps and final L1/alpha residual.
+    graph: replay each V-cycle's launches from one captured HIP graph."""
     ctx = context(phi.device)
     g = _native.grid(n, h)
     mp = _native.MGParams()
     mp.flag_pbc, mp.flag_nmn, mp.eps0 = int(flag_pbc), int(flag_nmn), float(eps0)
     mp.ncycl, mp.eps, mp.omega, mp.ng = int(ncycl), float(eps), float(omega), int(ng)
+    mp.graph = 1 if graph else 0
     st = _native.MGStats()
     ctx.check(ctx.lib.lepic_poisson_solve(ctx.h, C.byref(g), ptr(phi), ptr(rhs), ptr(bcnd),
                                           C.byref(mp), C.byref(st)))

@@ -86,6 +86,8 @@ typedef struct {
   double eps;              /* tolerance on L1 residual / alpha            */
   double omega;            /* SOR over-relaxation                          */
   int32_t ng;              /* multigrid levels (<= 12)                     */
+  int32_t graph;           /* 1: every V-cycle replays one captured HIP graph
+                              of the same launches (0: launched one by one) */
 } lepic_mg_params;
 
 typedef struct {

@@ -354,3 +354,31 @@ def test_bin_compact_groups_by_cell(gpu, oracle, rng, case):
     ka = np.stack([ga[k].astype(np.float64) for k in cols], 1)
     kb = np.stack([gb[k].astype(np.float64) for k in cols], 1)
     np.testing.assert_array_equal(ka[np.lexsort(ka.T[::-1])], kb[np.lexsort(kb.T[::-1])])
+
+
+@pytest.mark.parametrize("bmaker,pbc,ng,size", [(cfg.grounded_box_bcnd, 0, 5, 64),
+                                                (H.periodic_yz_bcnd, 1, 4, 32)])
+def test_poisson_graph_replay_matches_eager(gpu, rng, bmaker, pbc, ng, size):
+    """V-cycles replayed from the captured HIP graph give the same iterates as
+    the same launches issued one by one: several warm-started solves in a row
+    (each replaying the graph several times), eager work and host syncs
+    between them, and a second right-hand side buffer that forces a new
+    capture."""
+    from essaitemporaire_lepic_module_amd import fields as F
+    n, h = H.box((size, size, size))
+    bc = torch.as_tensor(bmaker(n), device=gpu)
+    phis = {g: torch.zeros(cfg.node_shape(n), dtype=torch.float64, device=gpu) for g in (True, False)}
+    rhos = [torch.zeros(cfg.node_shape(n), dtype=torch.float64, device=gpu) for _ in range(2)]
+    interior = bc <= 0
+    for it in range(6):
+        rho = rhos[it // 3]
+        rho.zero_()
+        rho[interior] = 1e-4 * torch.as_tensor(rng.standard_normal(int(interior.sum())), device=gpu)
+        info = {}
+        for graph in (True, False):
+            info[graph] = F.solve_poisson_legacy(phis[graph], bc, rho, h, n, 20000, 1e-8, 1.4, ng,
+                                                 pbc, 0, graph=graph)
+            torch.cuda.synchronize()
+        assert info[True] == info[False], it
+        assert info[True]["cycles"] >= 2
+        assert torch.equal(phis[True], phis[False]), it
