@@ -156,6 +156,31 @@ __device__ inline void make_stencil(const KGrid &g, double xp, double yp, double
   s.base = gidx(g, ix, iy, iz);
 }
 
+// make_stencil with the quotient range test taken once for the particle
+// (ok = qdiv_coord_ok of x, y and z): the same stencil for every input.
+__device__ inline void make_stencil_pre(const KGrid &g, double xp, double yp, double zp, bool ok,
+                                        Stencil &s) {
+  int ix = (int)qdiv_pre(xp, g.q1, ok) + 1;
+  int iy = (int)qdiv_pre(yp, g.q2, ok) + 1;
+  int iz = (int)qdiv_pre(zp, g.q3, ok) + 1;
+  ix = imax(0, imin(g.n1 + 1, ix));
+  iy = imax(0, imin(g.n2 + 1, iy));
+  iz = imax(0, imin(g.n3 + 1, iz));
+  double px = qdiv_pre((double)ix * g.h1 - xp, g.q1, ok);
+  double py = qdiv_pre((double)iy * g.h2 - yp, g.q2, ok);
+  double pz = qdiv_pre((double)iz * g.h3 - zp, g.q3, ok);
+  double wx2 = 1.0 - px, wy2 = 1.0 - py, wz2 = 1.0 - pz;
+  s.w[0] = px * py * pz;
+  s.w[1] = wx2 * py * pz;
+  s.w[2] = wx2 * wy2 * pz;
+  s.w[3] = px * wy2 * pz;
+  s.w[4] = px * py * wz2;
+  s.w[5] = wx2 * py * wz2;
+  s.w[6] = wx2 * wy2 * wz2;
+  s.w[7] = px * wy2 * wz2;
+  s.base = gidx(g, ix, iy, iz);
+}
+
 // Gather of a 4-wide node field (E or B map) in the reference's summation order.
 __device__ inline void gather3(const double4 *__restrict__ F, const KGrid &g, const Stencil &s,
                                double &ox, double &oy, double &oz) {

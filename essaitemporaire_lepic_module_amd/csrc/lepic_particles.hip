@@ -145,6 +145,25 @@ __device__ inline int wall_test_tab(const KGrid &g, const int *__restrict__ wt, 
   return lost;
 }
 
+// wall_test_tab from the quotients x/h1, y/h2, z/h3 already formed.
+__device__ inline int wall_test_tab_q(const KGrid &g, const int *__restrict__ wt, const KBC &bp,
+                                      double xp, double qx, double qy, double qz, int &label) {
+  int ix = (int)floor(qx) + 1;
+  int iy = (int)floor(qy) + 1;
+  int iz = (int)floor(qz) + 1;
+  ix = imax(0, imin(g.n1 + 1, ix));
+  iy = imax(0, imin(g.n2 + 1, iy));
+  iz = imax(0, imin(g.n3 + 1, iz));
+  const int c = wt[gidx(g, ix, iy, iz)];
+  int lost = c >= 0 ? 1 : 0;
+  label = c >= 0 ? c : 0;
+  if (bp.ptype == bp.tag_neg && xp < 0.0 && bp.flag_nmn == 1) {
+    lost = 2;
+    label = 0;
+  }
+  return lost;
+}
+
 __global__ void k_wall_table(KGrid g, const int *__restrict__ bcnd, int *wt, int64_t nodes) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= nodes) return;
@@ -510,6 +529,25 @@ __device__ inline void cell_of(const KGrid &g, double x, double y, double z, int
 // and hands a run to the block's LDS box (or, outside the box, to this XCD's
 // replica) with one set of 8 atomics.
 // --------------------------------------------------------------------------
+// deposit_fracs given the quotients x/h1, y/h2, z/h3 (ok: the particle's
+// qdiv_coord_ok).
+__device__ inline bool deposit_fracs_q(const KGrid &g, double x, double y, double z, double qx,
+                                       double qy, double qz, bool ok, int &ix, int &iy, int &iz,
+                                       double &px, double &py, double &pz) {
+  const double xmax = (double)g.n1 * g.h1, ymax = (double)g.n2 * g.h2,
+               zmax = (double)g.n3 * g.h3;
+  if (!(x >= -1.0e-12 && x <= xmax + 1.0e-12)) return false;
+  if (!(y >= -1.0e-12 && y <= ymax + 1.0e-12)) return false;
+  if (!(z >= -1.0e-12 && z <= zmax + 1.0e-12)) return false;
+  ix = (int)qx + 1;
+  iy = (int)qy + 1;
+  iz = (int)qz + 1;
+  px = qdiv_pre((double)ix * g.h1 - x, g.q1, ok);
+  py = qdiv_pre((double)iy * g.h2 - y, g.q2, ok);
+  pz = qdiv_pre((double)iz * g.h3 - z, g.q3, ok);
+  return true;
+}
+
 // Fractions of deposit_weights (same range guard, same arithmetic).
 __device__ inline bool deposit_fracs(const KGrid &g, double x, double y, double z, int &ix,
                                      int &iy, int &iz, double &px, double &py, double &pz) {
@@ -815,14 +853,20 @@ __global__ __launch_bounds__(BLOCK) void k_push(
 #pragma unroll
     for (int q = 0; q < CPT; ++q)
       if (live[q]) {
-        int c[3];
-        cell_of(g, x[q], y[q], z[q], c);
+        const bool ok = qdiv_coord_ok(x[q]) && qdiv_coord_ok(y[q]) && qdiv_coord_ok(z[q]);
+        const int c[3] = {imax(0, imin(g.n1 + 1, (int)qdiv_pre(x[q], g.q1, ok) + 1)),
+                          imax(0, imin(g.n2 + 1, (int)qdiv_pre(y[q], g.q2, ok) + 1)),
+                          imax(0, imin(g.n3 + 1, (int)qdiv_pre(z[q], g.q3, ok) + 1))};
         for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], c[d]); hi[d] = max(hi[d], c[d]); }
       }
     box_begin(g, ds, lo, hi);
   }
   RunAcc ra;
   ra.reset();
+  // E at the 8 nodes of the last gathered cell: in cell-sorted runs the next
+  // particle is usually in the same cell and reuses them
+  int64_t eb = -1;
+  double4 f1, f2, f3, f4, f5, f6, f7, f8;
   bool lostq[CPT];
   int lost_local = 0;
 #pragma unroll
@@ -833,21 +877,43 @@ __global__ __launch_bounds__(BLOCK) void k_push(
       double xp = x[q], yp = y[q], zp = z[q];
       double ux = vx[q], uy = vy[q], uz = vz[q];
       Stencil s;
-      make_stencil(g, xp, yp, zp, s);
-      double Ex, Ey, Ez;
-      gather3(E4, g, s, Ex, Ey, Ez);
+      make_stencil_pre(g, xp, yp, zp,
+                       qdiv_coord_ok(xp) && qdiv_coord_ok(yp) && qdiv_coord_ok(zp), s);
+      if (s.base != eb) {
+        eb = s.base;
+        const int64_t b1 = eb + g.sy, b2 = eb + g.sz, b3 = eb + g.sz + g.sy;
+        f1 = E4[eb]; f2 = E4[eb + 1]; f3 = E4[b1 + 1]; f4 = E4[b1];
+        f5 = E4[b2]; f6 = E4[b2 + 1]; f7 = E4[b3 + 1]; f8 = E4[b3];
+      }
+      // gather3's nodes, weights and summation order
+      const double Ex = s.w[0] * f1.x + s.w[1] * f2.x + s.w[2] * f3.x + s.w[3] * f4.x +
+                        s.w[4] * f5.x + s.w[5] * f6.x + s.w[6] * f7.x + s.w[7] * f8.x;
+      const double Ey = s.w[0] * f1.y + s.w[1] * f2.y + s.w[2] * f3.y + s.w[3] * f4.y +
+                        s.w[4] * f5.y + s.w[5] * f6.y + s.w[6] * f7.y + s.w[7] * f8.y;
+      const double Ez = s.w[0] * f1.z + s.w[1] * f2.z + s.w[2] * f3.z + s.w[3] * f4.z +
+                        s.w[4] * f5.z + s.w[5] * f6.z + s.w[6] * f7.z + s.w[7] * f8.z;
       velocity_update<BM>(g, bf, s, k1, Ex, Ey, Ez, ux, uy, uz);
       xp = xp + dt * ux; yp = yp + dt * uy; zp = zp + dt * uz;
       vx[q] = ux; vy[q] = uy; vz[q] = uz;
-      if (wall_test_tab(g, wt, bp, xp, yp, zp, label)) {
+      // one set of quotients of the new position serves the wall test and
+      // the deposit (formed again only if the periodic wrap moved it)
+      bool okn = qdiv_coord_ok(xp) && qdiv_coord_ok(yp) && qdiv_coord_ok(zp);
+      double qx = qdiv_pre(xp, g.q1, okn), qy = qdiv_pre(yp, g.q2, okn),
+             qz = qdiv_pre(zp, g.q3, okn);
+      if (wall_test_tab_q(g, wt, bp, xp, qx, qy, qz, label)) {
         lost = true;
         ++lost_local;
       } else {
+        const double xw = xp, yw = yp, zw = zp;
         survivor_wrap(bp, xp, yp, zp, vx[q]);
+        if (xp != xw || yp != yw || zp != zw) {
+          okn = qdiv_coord_ok(xp) && qdiv_coord_ok(yp) && qdiv_coord_ok(zp);
+          qx = qdiv_pre(xp, g.q1, okn); qy = qdiv_pre(yp, g.q2, okn); qz = qdiv_pre(zp, g.q3, okn);
+        }
         x[q] = xp; y[q] = yp; z[q] = zp;
         int ix, iy, iz;
         double px, py, pz;
-        if (do_dep && deposit_fracs(g, xp, yp, zp, ix, iy, iz, px, py, pz))
+        if (do_dep && deposit_fracs_q(g, xp, yp, zp, qx, qy, qz, okn, ix, iy, iz, px, py, pz))
           run_add(g, ds, ra, stage_key(g, ds, ix, iy, iz), kdep, px, py, pz, kq, rep);
       }
     }

@@ -52,3 +52,25 @@ LEPIC_HD inline double qdiv(double x, const QDiv &d) {
   e = std::fma(-q, d.h, x);
   return std::fma(e, d.y, q);
 }
+
+// Coordinates for which every quotient a particle kernel forms -- x/h and
+// (i*h - x)/h for a cell index i -- is exact on the short path: 0, or a
+// magnitude in [1e-280, 1e279].  (i*h - x is then 0 or, with the divisor
+// inside [1e-200, 1e200], at least ulp(i*h) >= 1e-216 and far below 1e280.)
+// For x = -0 the quotient comes out +0 instead of -0; every use converts it
+// to an integer or adds it to a nonzero term.
+LEPIC_HD inline bool qdiv_coord_ok(double x) {
+  const double ax = std::fabs(x);
+  return x == 0.0 || (ax >= 1.0e-280 && ax <= 1.0e279);
+}
+
+// qdiv with the range test made once per particle: ok from qdiv_coord_ok of
+// the coordinates the quotients come from.
+LEPIC_HD inline double qdiv_pre(double x, const QDiv &d, bool ok) {
+  if (!ok) return x / d.h;
+  double q = x * d.y;
+  double e = std::fma(-q, d.h, x);
+  q = std::fma(e, d.y, q);
+  e = std::fma(-q, d.h, x);
+  return std::fma(e, d.y, q);
+}
