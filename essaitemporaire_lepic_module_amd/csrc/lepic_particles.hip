@@ -1033,61 +1033,295 @@ __global__ __launch_bounds__(BLOCK) void k_gather_sorted(KPset src, KPset dst, i
 }
 
 // --------------------------------------------------------------------------
-// Binning by cell for the production path: a counting sort whose order within
-// a cell is left open.  Input that is already close to cell order comes in
-// runs of one key along the lanes of a wave; each run takes one atomic.
+// Counting sort by cell (production re-sort, holes dropped, order within a
+// cell unspecified).  Lane t of a block owns particles tile + 4t .. 4t+3, as
+// in k_push.  A tile's cells form a small box (particles drift less than a
+// cell per step since the last sort), so the block counts its particles per
+// cell in LDS and touches each of the tile's cells once in global memory:
+// one atomic per (tile, cell) instead of one per particle, and the particles
+// of a cell leave the tile as one run of consecutive slots.  Tiles whose box
+// exceeds BIN_BOX cells (none in sorted data) take one atomic per particle.
 // --------------------------------------------------------------------------
-constexpr uint32_t NO_KEY = 0xFFFFFFFFu;
+constexpr int BIN_BOX = 8192;   // cells (int counters, 32 KB)
 
-// For the calling lane: the first lane of its run of equal keys and the run's
-// length (valid on the first lane).  Every lane of the wave must call it.
-__device__ inline void wave_runs(uint32_t key, int &head, int &len) {
-  const int lane = __lane_id();
-  const uint32_t prev = (uint32_t)__shfl_up((int)key, 1);
-  const bool is_head = lane == 0 || prev != key;
-  const uint64_t H = __ballot(is_head);
-  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-  head = 63 - __clzll((long long)(H & upto));
-  const uint64_t above = lane == 63 ? 0ull : (H >> (lane + 1));
-  const int next = above ? lane + 1 + __ffsll((long long)above) - 1 : LEPIC_WAVE;
-  len = next - lane;
+struct BinShared {
+  int mn[3], mx[3];
+  int x0, y0, z0, bx, bxy, vol;   // vol < 0: per-particle atomics
+  int cnt[BIN_BOX];
+};
+
+// 1-based cell of a live particle, clamped to [1, n] (cell_index_from_position,
+// mod_particle_sorting.f90:16-48)
+__device__ inline void bin_cell(const KGrid &g, double x, double y, double z, int c[3]) {
+  const bool ok = qdiv_coord_ok(x) && qdiv_coord_ok(y) && qdiv_coord_ok(z);
+  c[0] = imax(1, imin(g.n1, (int)qdiv_pre(x, g.q1, ok) + 1));
+  c[1] = imax(1, imin(g.n2, (int)qdiv_pre(y, g.q2, ok) + 1));
+  c[2] = imax(1, imin(g.n3, (int)qdiv_pre(z, g.q3, ok) + 1));
 }
 
-__global__ __launch_bounds__(BLOCK) void k_bin_count(KGrid g, KPset p, int64_t n, uint32_t *keys,
-                                                     int32_t *count) {
-  const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
-  uint32_t key = NO_KEY;
-  if (i < n && p.dead[i] == 0) {
-    int ix = (int)qdiv(p.x[i], g.q1) + 1;
-    int iy = (int)qdiv(p.y[i], g.q2) + 1;
-    int iz = (int)qdiv(p.z[i], g.q3) + 1;
-    ix = imax(1, imin(g.n1, ix));
-    iy = imax(1, imin(g.n2, iy));
-    iz = imax(1, imin(g.n3, iz));
-    key = (uint32_t)((ix - 1) + g.n1 * ((iy - 1) + g.n2 * (iz - 1)));
+__device__ inline uint32_t bin_key(const KGrid &g, const int c[3]) {
+  return (uint32_t)((c[0] - 1) + g.n1 * ((c[1] - 1) + g.n2 * (c[2] - 1)));
+}
+
+// The block's cell box from its lanes' cells; every thread calls it.
+__device__ inline void bin_box(BinShared &bs, const int lo[3], const int hi[3]) {
+  if (threadIdx.x == 0) {
+    bs.mn[0] = bs.mn[1] = bs.mn[2] = 1 << 30;
+    bs.mx[0] = bs.mx[1] = bs.mx[2] = -(1 << 30);
   }
-  if (i < n) keys[i] = key;
-  int head, len;
-  wave_runs(key, head, len);
-  if (head == __lane_id() && key != NO_KEY) atomicAdd(&count[key], len);
+  __syncthreads();
+  const int a0 = wave_min(lo[0]), a1 = wave_min(lo[1]), a2 = wave_min(lo[2]);
+  const int b0 = wave_max(hi[0]), b1 = wave_max(hi[1]), b2 = wave_max(hi[2]);
+  if (__lane_id() == 0) {
+    atomicMin(&bs.mn[0], a0); atomicMin(&bs.mn[1], a1); atomicMin(&bs.mn[2], a2);
+    atomicMax(&bs.mx[0], b0); atomicMax(&bs.mx[1], b1); atomicMax(&bs.mx[2], b2);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int vol = 0;
+    if (bs.mx[0] >= bs.mn[0]) {
+      bs.x0 = bs.mn[0]; bs.y0 = bs.mn[1]; bs.z0 = bs.mn[2];
+      bs.bx = bs.mx[0] - bs.mn[0] + 1;
+      const int by = bs.mx[1] - bs.mn[1] + 1, bz = bs.mx[2] - bs.mn[2] + 1;
+      bs.bxy = bs.bx * by;
+      const int64_t v = (int64_t)bs.bxy * bz;
+      vol = v > BIN_BOX ? -1 : (int)v;
+    }
+    bs.vol = vol;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < bs.vol; t += BLOCK) bs.cnt[t] = 0;
+  __syncthreads();
 }
 
-__global__ __launch_bounds__(BLOCK) void k_bin_scatter(KPset src, KPset dst, int64_t n,
-                                                       const uint32_t *__restrict__ keys,
+__device__ inline int bin_local(const BinShared &bs, const int c[3]) {
+  return (c[0] - bs.x0) + bs.bx * (c[1] - bs.y0) + bs.bxy * (c[2] - bs.z0);
+}
+
+__device__ inline uint32_t bin_key_of_local(const KGrid &g, const BinShared &bs, int t) {
+  const int lz = t / bs.bxy, r = t - lz * bs.bxy;
+  const int ly = r / bs.bx, lx = r - ly * bs.bx;
+  const int c[3] = {bs.x0 + lx, bs.y0 + ly, bs.z0 + lz};
+  return bin_key(g, c);
+}
+
+// Loads positions and the live mask of a lane's run.
+__device__ inline void load_pos_run(const KPset &p, int64_t n, int64_t i0, bool full,
+                                    double x[CPT], double y[CPT], double z[CPT], bool live[CPT]) {
+  if (full) {
+    const uint32_t d4 = *reinterpret_cast<const uint32_t *>(p.dead + i0);
+    load_pair(p.x, i0, x[0], x[1]); load_pair(p.x, i0 + 2, x[2], x[3]);
+    load_pair(p.y, i0, y[0], y[1]); load_pair(p.y, i0 + 2, y[2], y[3]);
+    load_pair(p.z, i0, z[0], z[1]); load_pair(p.z, i0 + 2, z[2], z[3]);
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) live[q] = ((d4 >> (8 * q)) & 0xFFu) == 0u;
+  } else {
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      const int64_t i = i0 + q;
+      live[q] = i < n && p.dead[i] == 0;
+      x[q] = y[q] = z[q] = 0.0;
+      if (live[q]) { x[q] = p.x[i]; y[q] = p.y[i]; z[q] = p.z[i]; }
+    }
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bin_count(KGrid g, KPset p, int64_t n,
+                                                     int32_t *count) {
+  __shared__ BinShared bs;
+  const int64_t i0 = (int64_t)blockIdx.x * CTILE + (int64_t)threadIdx.x * CPT;
+  const bool full = i0 + CPT <= n;
+  double x[CPT], y[CPT], z[CPT];
+  bool live[CPT];
+  load_pos_run(p, n, i0, full, x, y, z, live);
+  int c[CPT][3];
+  int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+#pragma unroll
+  for (int q = 0; q < CPT; ++q)
+    if (live[q]) {
+      bin_cell(g, x[q], y[q], z[q], c[q]);
+      for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], c[q][d]); hi[d] = max(hi[d], c[q][d]); }
+    }
+  bin_box(bs, lo, hi);
+  if (bs.vol > 0) {
+#pragma unroll
+    for (int q = 0; q < CPT; ++q)
+      if (live[q]) atomicAdd(&bs.cnt[bin_local(bs, c[q])], 1);
+    __syncthreads();
+    for (int t = threadIdx.x; t < bs.vol; t += BLOCK) {
+      const int k = bs.cnt[t];
+      if (k) atomicAdd(&count[bin_key_of_local(g, bs, t)], k);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < CPT; ++q)
+      if (live[q]) atomicAdd(&count[bin_key(g, c[q])], 1);
+  }
+}
+
+// cursor[c] starts at the first slot of cell c and is advanced by the tiles.
+// The tile's particles are written through LDS one array at a time, in the
+// order of their destinations (cells of the box in order, each cell's run
+// contiguous), so that neighbouring lanes store to neighbouring addresses.
+constexpr int BIN_STAGE = 4096;   // cells of the staged path (2 int arrays, 32 KB)
+
+struct BinStage {
+  int mn[3], mx[3];
+  int x0, y0, z0, bx, bxy, vol;    // vol < 0: per-particle atomics, direct stores
+  int wsum[NWAVES];
+  int total;
+  int cnt[BIN_STAGE];    // tile count per cell, then first staged slot of the cell
+  int dlt[BIN_STAGE];    // destination of the cell's run minus its first slot
+  int dsti[CTILE];       // destination of each staged slot
+  double buf[CTILE];     // one array of the tile at a time
+};
+
+__global__ __launch_bounds__(BLOCK) void k_bin_scatter(KGrid g, KPset src, KPset dst, int64_t n,
                                                        int32_t *cursor) {
-  const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
-  const uint32_t key = i < n ? keys[i] : NO_KEY;
-  int head, len;
-  wave_runs(key, head, len);
-  int base = 0;
-  if (head == __lane_id() && key != NO_KEY) base = atomicAdd(&cursor[key], len);
-  base = __shfl(base, head);
-  if (key == NO_KEY) return;
-  const int64_t o = (int64_t)base + (__lane_id() - head);
-  dst.x[o] = src.x[i]; dst.y[o] = src.y[i]; dst.z[o] = src.z[i];
-  dst.vx[o] = src.vx[i]; dst.vy[o] = src.vy[i]; dst.vz[o] = src.vz[i];
-  dst.cell[o] = (int32_t)key;
-  dst.dead[o] = 0;
+  __shared__ BinStage bs;
+  const int64_t i0 = (int64_t)blockIdx.x * CTILE + (int64_t)threadIdx.x * CPT;
+  const bool full = i0 + CPT <= n;
+  double x[CPT], y[CPT], z[CPT], vx[CPT], vy[CPT], vz[CPT];
+  bool live[CPT];
+  load_run(src, n, i0, full, x, y, z, vx, vy, vz, live);
+  int c[CPT][3];
+  int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+#pragma unroll
+  for (int q = 0; q < CPT; ++q)
+    if (live[q]) {
+      bin_cell(g, x[q], y[q], z[q], c[q]);
+      for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], c[q][d]); hi[d] = max(hi[d], c[q][d]); }
+    }
+  // the block's cell box
+  if (threadIdx.x == 0) {
+    bs.mn[0] = bs.mn[1] = bs.mn[2] = 1 << 30;
+    bs.mx[0] = bs.mx[1] = bs.mx[2] = -(1 << 30);
+  }
+  __syncthreads();
+  {
+    const int a0 = wave_min(lo[0]), a1 = wave_min(lo[1]), a2 = wave_min(lo[2]);
+    const int b0 = wave_max(hi[0]), b1 = wave_max(hi[1]), b2 = wave_max(hi[2]);
+    if (__lane_id() == 0) {
+      atomicMin(&bs.mn[0], a0); atomicMin(&bs.mn[1], a1); atomicMin(&bs.mn[2], a2);
+      atomicMax(&bs.mx[0], b0); atomicMax(&bs.mx[1], b1); atomicMax(&bs.mx[2], b2);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int vol = 0;
+    if (bs.mx[0] >= bs.mn[0]) {
+      bs.x0 = bs.mn[0]; bs.y0 = bs.mn[1]; bs.z0 = bs.mn[2];
+      bs.bx = bs.mx[0] - bs.mn[0] + 1;
+      const int by = bs.mx[1] - bs.mn[1] + 1, bz = bs.mx[2] - bs.mn[2] + 1;
+      bs.bxy = bs.bx * by;
+      const int64_t v = (int64_t)bs.bxy * bz;
+      vol = v > BIN_STAGE ? -1 : (int)v;
+    }
+    bs.vol = vol;
+  }
+  __syncthreads();
+  const int vol = bs.vol;
+  if (vol < 0) {   // wide tile: one atomic per particle, stores straight to dst
+#pragma unroll
+    for (int q = 0; q < CPT; ++q)
+      if (live[q]) {
+        const uint32_t key = bin_key(g, c[q]);
+        const int64_t d = atomicAdd(&cursor[key], 1);
+        dst.x[d] = x[q]; dst.y[d] = y[q]; dst.z[d] = z[q];
+        dst.vx[d] = vx[q]; dst.vy[d] = vy[q]; dst.vz[d] = vz[q];
+        dst.cell[d] = (int32_t)key;
+        dst.dead[d] = 0;
+      }
+    return;   // uniform across the block
+  }
+  for (int t = threadIdx.x; t < vol; t += BLOCK) bs.cnt[t] = 0;
+  __syncthreads();
+  int l[CPT], rank[CPT];
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) {
+    l[q] = live[q] ? (c[q][0] - bs.x0) + bs.bx * (c[q][1] - bs.y0) + bs.bxy * (c[q][2] - bs.z0) : 0;
+    rank[q] = live[q] ? atomicAdd(&bs.cnt[l[q]], 1) : 0;
+  }
+  __syncthreads();
+  // exclusive scan of the counts over the box (thread t owns a chunk), one
+  // global atomic per nonempty cell for its run of destinations
+  {
+    const int ch = (vol + BLOCK - 1) / BLOCK;
+    const int b = min((int)threadIdx.x * ch, vol), e = min(b + ch, vol);
+    int sum = 0;
+    for (int t = b; t < e; ++t) sum += bs.cnt[t];
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    int inc = sum;
+    for (int d = 1; d < 64; d <<= 1) {
+      const int o = __shfl_up(inc, d);
+      if (lane >= d) inc += o;
+    }
+    if (lane == 63) bs.wsum[w] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int run = 0;
+      for (int i = 0; i < NWAVES; ++i) {
+        const int v = bs.wsum[i];
+        bs.wsum[i] = run;
+        run += v;
+      }
+      bs.total = run;
+    }
+    __syncthreads();
+    int pre = bs.wsum[w] + inc - sum;
+    for (int t = b; t < e; ++t) {
+      const int k = bs.cnt[t];
+      int base = 0;
+      if (k) {
+        const int lz = t / bs.bxy, r = t - lz * bs.bxy;
+        const int ly = r / bs.bx, lx = r - ly * bs.bx;
+        const int cc[3] = {bs.x0 + lx, bs.y0 + ly, bs.z0 + lz};
+        base = atomicAdd(&cursor[bin_key(g, cc)], k);
+      }
+      bs.cnt[t] = pre;
+      bs.dlt[t] = base - pre;
+      pre += k;
+    }
+  }
+  __syncthreads();
+  const int total = bs.total;
+  int slot[CPT];
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) {
+    slot[q] = live[q] ? bs.cnt[l[q]] + rank[q] : 0;
+    if (live[q]) bs.dsti[slot[q]] = slot[q] + bs.dlt[l[q]];
+  }
+  // one array at a time through LDS, stored in destination order
+  for (int f = 0; f < 7; ++f) {
+#pragma unroll
+    for (int q = 0; q < CPT; ++q)
+      if (live[q]) {
+        double v;
+        switch (f) {
+          case 0: v = x[q]; break;
+          case 1: v = y[q]; break;
+          case 2: v = z[q]; break;
+          case 3: v = vx[q]; break;
+          case 4: v = vy[q]; break;
+          case 5: v = vz[q]; break;
+          default: v = __longlong_as_double((long long)bin_key(g, c[q]));
+        }
+        bs.buf[slot[q]] = v;
+      }
+    __syncthreads();
+    double *out = f == 0 ? dst.x : f == 1 ? dst.y : f == 2 ? dst.z : f == 3 ? dst.vx
+                : f == 4 ? dst.vy : dst.vz;
+    for (int s = threadIdx.x; s < total; s += BLOCK) {
+      const int d = bs.dsti[s];
+      if (f < 6) {
+        out[d] = bs.buf[s];
+      } else {
+        dst.cell[d] = (int32_t)__double_as_longlong(bs.buf[s]);
+        dst.dead[d] = 0;
+      }
+    }
+    __syncthreads();
+  }
 }
 
 __global__ void k_add_u64(unsigned long long *acc, const unsigned long long *v) {
@@ -1147,6 +1381,15 @@ int check_pset(lepic_ctx *ctx, const lepic_pset *p, const char *what) {
   if (p->n > ((int64_t)1 << 40))
     return lepic_fail(ctx, LEPIC_ERR_ARG, std::string(what) + ": too many particles");
   return LEPIC_OK;
+}
+
+// The run kernels read and write particles with 16-byte vector accesses and
+// the dead flags 4 at a time.
+static bool aligned_runs(const lepic_pset *p) {
+  const void *a[6] = {p->x, p->y, p->z, p->vx, p->vy, p->vz};
+  for (const void *q : a)
+    if (reinterpret_cast<uintptr_t>(q) % 16 != 0) return false;
+  return reinterpret_cast<uintptr_t>(p->dead) % 4 == 0;
 }
 
 int check_grid(lepic_ctx *ctx, const lepic_grid *g) {
@@ -1295,6 +1538,8 @@ int lepic_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *p, cons
   if (!ctx) return LEPIC_ERR_ARG;
   int st;
   if ((st = check_grid(ctx, g)) || (st = check_pset(ctx, p, "deposit"))) return st;
+  if (!aligned_runs(p))
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "deposit: particle arrays must be 16-byte aligned");
   if (!kq || !np_local) return lepic_fail(ctx, LEPIC_ERR_ARG, "deposit: null argument");
   if (p->n == 0) return LEPIC_OK;
   const int64_t nodes = grid_nodes(g);
@@ -1315,6 +1560,9 @@ int lepic_push_bc_deposit_inplace(lepic_ctx *ctx, const lepic_grid *g, lepic_pse
   if (!ctx) return LEPIC_ERR_ARG;
   int st;
   if ((st = check_grid(ctx, g)) || (st = check_pset(ctx, p, "push_bc_deposit_inplace"))) return st;
+  if (!aligned_runs(p))
+    return lepic_fail(ctx, LEPIC_ERR_ARG,
+                      "push_bc_deposit_inplace: particle arrays must be 16-byte aligned");
   if (!E4 || !bcnd || !bp || !p_mac || !P_loss || (np_local && !kq))
     return lepic_fail(ctx, LEPIC_ERR_ARG, "push_bc_deposit_inplace: null argument");
   if (bf && bf->mode == 2 && !bf->B4)
@@ -1556,31 +1804,32 @@ int lepic_bin_compact(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *src
     dst->n = 0;
     return LEPIC_OK;
   }
-  // scratch 0: keys[n] | count[ncells + 1] | start[ncells + 1] | scan temp
+  if (!aligned_runs(src) || !aligned_runs(dst))
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "bin_compact: particle arrays must be 16-byte aligned");
+  // scratch 0: count[ncells + 1] | start[ncells + 1] | scan temp
   size_t temp_bytes = 0;
   int32_t *nul = nullptr;
   LEPIC_HIP(ctx, rocprim::exclusive_scan(nullptr, temp_bytes, nul, nul, 0, (size_t)ncells + 1,
                                          rocprim::plus<int32_t>(), ctx->stream));
-  const size_t kb = (sizeof(uint32_t) * (size_t)n + 255) & ~(size_t)255;
   const size_t cb = (sizeof(int32_t) * (size_t)(ncells + 1) + 255) & ~(size_t)255;
-  char *base = reinterpret_cast<char *>(lepic_scratch(ctx, kb + 2 * cb + temp_bytes + 256, 0));
+  char *base = reinterpret_cast<char *>(lepic_scratch(ctx, 2 * cb + temp_bytes + 256, 0));
   if (!base) return lepic_fail(ctx, LEPIC_ERR_HIP, "bin_compact: scratch allocation failed");
-  uint32_t *keys = reinterpret_cast<uint32_t *>(base);
-  int32_t *count = reinterpret_cast<int32_t *>(base + kb);
-  int32_t *start = reinterpret_cast<int32_t *>(base + kb + cb);
-  void *temp = base + kb + 2 * cb;
+  int32_t *count = reinterpret_cast<int32_t *>(base);
+  int32_t *start = reinterpret_cast<int32_t *>(base + cb);
+  void *temp = base + 2 * cb;
+  const KGrid kg = make_kgrid(g);
+  const unsigned tiles = (unsigned)((n + CTILE - 1) / CTILE);
   hipLaunchKernelGGL(k_fill_i32, dim3(grid_blocks(ncells + 1, 256)), dim3(256), 0, ctx->stream,
                      count, ncells + 1, 0);
-  hipLaunchKernelGGL(k_bin_count, dim3(grid_blocks(n, BLOCK)), dim3(BLOCK), 0, ctx->stream,
-                     make_kgrid(g), kp(src), n, keys, count);
+  hipLaunchKernelGGL(k_bin_count, dim3(tiles), dim3(BLOCK), 0, ctx->stream, kg, kp(src), n, count);
   LEPIC_LAUNCH_CHECK(ctx, "k_bin_count");
   // start[c] = particles in cells before c; start[ncells] = live total
   LEPIC_HIP(ctx, rocprim::exclusive_scan(temp, temp_bytes, count, start, 0, (size_t)ncells + 1,
                                          rocprim::plus<int32_t>(), ctx->stream));
   LEPIC_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, start + ncells, sizeof(int32_t),
                                 hipMemcpyDeviceToHost, ctx->stream));
-  hipLaunchKernelGGL(k_bin_scatter, dim3(grid_blocks(n, BLOCK)), dim3(BLOCK), 0, ctx->stream,
-                     kp(src), kp(dst), n, keys, start);
+  hipLaunchKernelGGL(k_bin_scatter, dim3(tiles), dim3(BLOCK), 0, ctx->stream, kg, kp(src),
+                     kp(dst), n, start);
   LEPIC_LAUNCH_CHECK(ctx, "k_bin_scatter");
   LEPIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
   dst->n = (int64_t)reinterpret_cast<int32_t *>(ctx->h_pinned)[0];

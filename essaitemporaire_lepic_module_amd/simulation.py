@@ -18,7 +18,9 @@ LDS, and fast species (electrons cross a cell in a few steps) lose their
 cell order long before the reference's next sort.  The extra sorts change
 the particle order between the reference's sorts, which nothing in the
 production step depends on (collisions draw from per-particle Philox
-streams and the deposit's summation order is that of its atomics anyway).
+streams and the deposit's summation order is that of its atomics anyway);
+with them every sort is the counting sort by cell, whose order within a
+cell is unspecified.
 """
 from __future__ import annotations
 
@@ -232,13 +234,13 @@ class Simulation:
 
     # ------------------------------------------------------------------
     def _compact(self, s: int):
-        """Drop holes and re-sort by cell.  The stable radix sort keeps the
-        reference's order; once extra sorts are asked for, species that are
-        not re-sorted (slow ones, still in long cell runs) are binned instead,
-        which is cheaper on such input."""
+        """Drop holes and re-sort by cell.  Without extra sorts the stable
+        radix sort keeps the reference's particle order; once extra sorts are
+        asked for, order within a cell is no longer the reference's anyway and
+        every species is binned by the counting sort, which is cheaper."""
         p = self.parts[s]
         if p.n > 0:
-            if self.resort_every is not None and not self.resort_every[s]:
+            if self.resort_every is not None:
                 P.bin_compact(p, self.n, self.h)
             else:
                 P.sort_compact(p, self.n, self.h)
