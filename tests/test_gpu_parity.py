@@ -321,13 +321,15 @@ def test_deposit_unsorted_falls_back(gpu, oracle, rng):
     np.testing.assert_allclose(npg.cpu().numpy(), npo, rtol=RTOL_ATOMIC, atol=1e-30)
 
 
-@pytest.mark.parametrize("case", ["mixed", "empty", "all_dead", "one_cell"])
+@pytest.mark.parametrize("case", ["mixed", "empty", "all_dead", "one_cell", "wide"])
 def test_bin_compact_groups_by_cell(gpu, oracle, rng, case):
     """Binning keeps exactly the live particles, in nondecreasing cell order,
     each cell holding the same particles as the stable sort puts there."""
     from essaitemporaire_lepic_module_amd import particles as P
-    n, h = H.box((9, 7, 6))
-    count = {"mixed": 40000, "empty": 0, "all_dead": 3000, "one_cell": 5000}[case]
+    # "wide": unsorted input on a larger grid, so that every tile's cell box
+    # exceeds the staged path and takes one atomic per particle
+    n, h = H.box((40, 40, 40) if case == "wide" else (9, 7, 6))
+    count = {"mixed": 40000, "empty": 0, "all_dead": 3000, "one_cell": 5000, "wide": 60000}[case]
     arr = H.random_particles(rng, n, h, count, outside=0.02)
     if case == "one_cell":
         arr = [a.copy() for a in arr]
@@ -338,7 +340,7 @@ def test_bin_compact_groups_by_cell(gpu, oracle, rng, case):
         dead[:] = 2
     # nearly sorted input, as in production
     po = _oracle_set(oracle, arr, dead=np.zeros(count, np.int8))
-    if count:
+    if count and case != "wide":
         oracle.sort_particles(n, h, po)
     start = po.arrays()
     a = _gpu_set(P, [start[k] for k in ("x", "y", "z", "vx", "vy", "vz")], dead=dead)
