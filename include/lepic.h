@@ -244,7 +244,8 @@ int lepic_push_bc_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset 
  * compaction gives.  np_local NULL skips the deposit (collision steps).
  * lost_acc (device, may be NULL) is incremented by this call's removals
  * without a host synchronisation; np_lost (host, may be NULL) makes the call
- * synchronise and return them. */
+ * synchronise and return them.  The particle arrays must be 16-byte aligned
+ * (dead: 4): the kernel moves them with vector accesses. */
 int lepic_push_bc_deposit_inplace(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *p,
                                   const double *E4, const lepic_bfield *bf, double q, double m,
                                   double dt, const int32_t *bcnd, const lepic_bc_params *bp,
@@ -259,8 +260,9 @@ int lepic_sort_compact(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *sr
 /* Production binning by cell (the cell order of sort_particles_by_cell,
  * mod_particle_sorting.f90:88-181, with the order inside a cell left open):
  * a counting sort that drops every particle with dead != 0 and sets dst->cell.
- * Two streaming passes; for runs that the production path re-sorts more
- * often than the reference, where the order inside a cell is immaterial. */
+ * Two streaming passes over tiles of particles; for runs that the production
+ * path re-sorts more often than the reference, where the order inside a cell
+ * is immaterial.  The particle arrays must be 16-byte aligned (dead: 4). */
 int lepic_bin_compact(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *src,
                       lepic_pset *dst);
 
