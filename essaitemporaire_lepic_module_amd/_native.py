@@ -27,8 +27,8 @@ EXPORTS = (
     "lepic_load_uniform_device", "lepic_sort_particles_by_cell", "lepic_move_electrostatic",
     "lepic_move_boris", "lepic_apply_bc", "lepic_build_kq", "lepic_fill", "lepic_deposit",
     "lepic_reduce_density", "lepic_build_rho", "lepic_poisson_solve", "lepic_efield",
-    "lepic_collisions", "lepic_push_bc_deposit", "lepic_push_bc_deposit_inplace",
-    "lepic_sort_compact", "lepic_bin_compact",
+    "lepic_collisions", "lepic_collisions_deposited", "lepic_push_bc_deposit",
+    "lepic_push_bc_deposit_inplace", "lepic_sort_compact", "lepic_bin_compact",
 )
 
 
@@ -136,6 +136,10 @@ def load() -> C.CDLL:
             "lepic_collisions": (C.c_int, [vp, G, PS, vp, C.POINTER(Chem), C.c_int32, C.c_double,
                                            C.c_int32, i32p, C.c_uint64, C.c_uint64, vp,
                                            C.POINTER(MCCStats)]),
+            "lepic_collisions_deposited": (C.c_int, [vp, G, PS, vp, C.POINTER(Chem), C.c_int32,
+                                                     C.c_double, C.c_int32, i32p, C.c_uint64,
+                                                     C.c_uint64, vp, C.POINTER(MCCStats), vp,
+                                                     vp]),
             "lepic_push_bc_deposit": (C.c_int, [vp, G, PS, PS, vp, C.POINTER(BField), C.c_double,
                                                 C.c_double, C.c_double, vp, C.POINTER(BCParams),
                                                 vp, vp, vp, vp, C.POINTER(C.c_int64)]),

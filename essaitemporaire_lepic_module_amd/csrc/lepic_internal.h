@@ -156,6 +156,42 @@ __device__ inline void make_stencil(const KGrid &g, double xp, double yp, double
   s.base = gidx(g, ix, iy, iz);
 }
 
+// Lower node and fractions of deposit_weights (mod_charge_weights.f90:9-33:
+// same range guard, same arithmetic); false: no deposit.
+__device__ inline bool deposit_fracs(const KGrid &g, double x, double y, double z, int &ix,
+                                     int &iy, int &iz, double &px, double &py, double &pz) {
+  const double xmax = (double)g.n1 * g.h1, ymax = (double)g.n2 * g.h2,
+               zmax = (double)g.n3 * g.h3;
+  if (!(x >= -1.0e-12 && x <= xmax + 1.0e-12)) return false;
+  if (!(y >= -1.0e-12 && y <= ymax + 1.0e-12)) return false;
+  if (!(z >= -1.0e-12 && z <= zmax + 1.0e-12)) return false;
+  ix = (int)qdiv(x, g.q1) + 1;
+  iy = (int)qdiv(y, g.q2) + 1;
+  iz = (int)qdiv(z, g.q3) + 1;
+  px = qdiv((double)ix * g.h1 - x, g.q1);
+  py = qdiv((double)iy * g.h2 - y, g.q2);
+  pz = qdiv((double)iz * g.h3 - z, g.q3);
+  return true;
+}
+
+// One particle's deposit, sign * kq[node] * k1 * w per node, by device-scope
+// atomics (k1 = Nm / (h1 h2 h3)); the node order and weights of the run
+// deposit in the particle kernels.
+__device__ inline void deposit_one(const KGrid &g, double *np, const double *__restrict__ kq,
+                                   double k1, double x, double y, double z, double sign) {
+  int ix, iy, iz;
+  double px, py, pz;
+  if (!deposit_fracs(g, x, y, z, ix, iy, iz, px, py, pz)) return;
+  const double wx2 = 1.0 - px, wy2 = 1.0 - py, wz2 = 1.0 - pz;
+  const double w[8] = {px * py * pz,  wx2 * py * pz,  wx2 * wy2 * pz,  px * wy2 * pz,
+                       px * py * wz2, wx2 * py * wz2, wx2 * wy2 * wz2, px * wy2 * wz2};
+  const int64_t b = gidx(g, ix, iy, iz);
+  const int64_t nd[8] = {b, b + 1, b + g.sy + 1, b + g.sy, b + g.sz, b + g.sz + 1,
+                         b + g.sz + g.sy + 1, b + g.sz + g.sy};
+#pragma unroll
+  for (int q = 0; q < 8; ++q) atomicAdd(&np[nd[q]], sign * (kq[nd[q]] * (k1 * w[q])));
+}
+
 // make_stencil with the quotient range test taken once for the particle
 // (ok = qdiv_coord_ok of x, y and z): the same stencil for every input.
 __device__ inline void make_stencil_pre(const KGrid &g, double xp, double yp, double zp, bool ok,

@@ -65,6 +65,11 @@ struct KChem {
   double np_mx_all[LEPIC_MAX_SPECIES];
   double nu_max_local[LEPIC_MAX_SPECIES];
   FastPlan fp[4];
+  // density already deposited for this step ([ntype_tracked][nodes]; null:
+  // none): products are added to it and killed particles taken out
+  double *np_dep;
+  const double *kq;
+  double kdep[4];   // Nm / (h1 h2 h3) per tracked species
 };
 
 struct Ran2Stream {
@@ -389,7 +394,10 @@ __device__ void trial(const KGrid &g, const KParts &P, const KChem &K, const dou
     const double nvx = vxc + vp * ex1, nvy = vyc + vp * ey1, nvz = vzc + vp * ez1;
     P.vx[bt][ibx] = nvx; P.vy[bt][ibx] = nvy; P.vz[bt][ibx] = nvz;
     P.dead[bt][ibx] = 0;
-    if (fresh) app.count(&cnt->appended[bt]);
+    if (fresh) {
+      app.count(&cnt->appended[bt]);
+      if (K.np_dep) deposit_one(g, K.np_dep + (int64_t)bt * nodes, K.kq, K.kdep[bt], x0, y0, z0, 1.0);
+    }
     const double v2new = nvx * nvx + nvy * nvy + nvz * nvz;
     app.add(&Pcoll[bt], 0.5 * c.Nm[bt] * c.mass[bt] * (v2new - v2old));
   }
@@ -397,7 +405,10 @@ __device__ void trial(const KGrid &g, const KParts &P, const KChem &K, const dou
     const int ct = c.col_info[c_ind * 10 + 2 + ir];
     if (ct < 0 || ct >= NT) continue;
     if (c.mass[ct] <= 0.0) continue;
-    if (flag_add[ir] == -1 && ct == pt) P.dead[pt][ip] = 1;
+    if (flag_add[ir] == -1 && ct == pt && P.dead[pt][ip] == 0) {
+      if (K.np_dep) deposit_one(g, K.np_dep + (int64_t)pt * nodes, K.kq, K.kdep[pt], x0, y0, z0, -1.0);
+      P.dead[pt][ip] = 1;
+    }
   }
 }
 
@@ -408,8 +419,8 @@ __device__ void trial(const KGrid &g, const KParts &P, const KChem &K, const dou
 // reaches rnd -- the exchange sort's choice -- found by ranking instead of
 // sorting; rnd is drawn after the targets, as above.
 template <class APP>
-__device__ void trial_fast(const KParts &P, const KChem &K, int pt, int64_t ip,
-                           PhiloxStream &rng, APP &app, MccCounters *cnt) {
+__device__ void trial_fast(const KGrid &g, const KParts &P, const KChem &K, int64_t nodes, int pt,
+                           int64_t ip, PhiloxStream &rng, APP &app, MccCounters *cnt) {
   const lepic_chem &c = K.c;
   const FastPlan &fp = K.fp[pt];
   const int NT = c.ntype_tracked, NA = c.ntype_total;
@@ -537,17 +548,24 @@ __device__ void trial_fast(const KParts &P, const KChem &K, int pt, int64_t ip,
       atomicExch((unsigned long long *)&cnt->overflow, 1ull);
       continue;
     }
-    P.x[bt][ibx] = P.x[pt][ip]; P.y[bt][ibx] = P.y[pt][ip]; P.z[bt][ibx] = P.z[pt][ip];
+    const double x0 = P.x[pt][ip], y0 = P.y[pt][ip], z0 = P.z[pt][ip];
+    P.x[bt][ibx] = x0; P.y[bt][ibx] = y0; P.z[bt][ibx] = z0;
     P.vx[bt][ibx] = nvx; P.vy[bt][ibx] = nvy; P.vz[bt][ibx] = nvz;
     P.dead[bt][ibx] = 0;
     app.count(&cnt->appended[bt]);
+    if (K.np_dep) deposit_one(g, K.np_dep + (int64_t)bt * nodes, K.kq, K.kdep[bt], x0, y0, z0, 1.0);
     app.add(&app.pcoll_base[bt], 0.5 * c.Nm[bt] * c.mass[bt] * v2new);
   }
   for (int ir = 0; ir < n_re; ++ir) {
     const int ct = c.col_info[c_ind * 10 + 2 + ir];
     if (ct < 0 || ct >= NT) continue;
     if (c.mass[ct] <= 0.0) continue;
-    if (ct == pt && !(ir == 0 && reused)) P.dead[pt][ip] = 1;
+    if (ct == pt && !(ir == 0 && reused) && P.dead[pt][ip] == 0) {
+      if (K.np_dep)
+        deposit_one(g, K.np_dep + (int64_t)pt * nodes, K.kq, K.kdep[pt], P.x[pt][ip], P.y[pt][ip],
+                    P.z[pt][ip], -1.0);
+      P.dead[pt][ip] = 1;
+    }
   }
 }
 
@@ -652,7 +670,7 @@ __global__ __launch_bounds__(BLOCK) void k_mcc_trials(KGrid g, KParts P, KChem K
     rng.init(key ^ ((uint64_t)pt << 56), (uint32_t)ip, (uint32_t)(ip >> 32), (uint32_t)counter);
     rng.c3 ^= (uint32_t)(counter >> 32) * 0x9E3779B9u;
     for (int t = 0; t < k; ++t) {
-      if constexpr (FAST) trial_fast(P, K, pt, ip, rng, app, cnt);
+      if constexpr (FAST) trial_fast(g, P, K, nodes, pt, ip, rng, app, cnt);
       else trial(g, P, K, np_red, nodes, pt, ip, rng, app, cnt, Pcoll);
     }
   }
@@ -706,10 +724,11 @@ __global__ __launch_bounds__(BLOCK) void k_npmax(KGrid g, const double *np, int6
 
 }  // namespace
 
-extern "C" int lepic_collisions(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *parts,
-                                const double *np_red, const lepic_chem *chem, int32_t ns_coll,
-                                double dt, int32_t mode, int32_t *iseed, uint64_t philox_key,
-                                uint64_t philox_counter, double *Pcoll, lepic_mcc_stats *stats) {
+static int collisions_impl(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *parts,
+                           const double *np_red, const lepic_chem *chem, int32_t ns_coll,
+                           double dt, int32_t mode, int32_t *iseed, uint64_t philox_key,
+                           uint64_t philox_counter, double *Pcoll, lepic_mcc_stats *stats,
+                           double *np_dep, const double *kq) {
   if (!ctx) return LEPIC_ERR_ARG;
   if (!g || !parts || !np_red || !chem || !Pcoll || (mode == LEPIC_MCC_REPLAY_RAN2 && !iseed))
     return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions: null argument");
@@ -727,6 +746,9 @@ extern "C" int lepic_collisions(lepic_ctx *ctx, const lepic_grid *g, lepic_pset 
   KChem K;
   std::memset(&K, 0, sizeof(K));
   K.c = c;
+  K.np_dep = np_dep;
+  K.kq = kq;
+  for (int t = 0; t < NT; ++t) K.kdep[t] = c.Nm[t] / (g->h[0] * g->h[1] * g->h[2]);
   // np_mx for tracked species (device max), neutral densities for the rest
   {
     int64_t cnt = (int64_t)(g->n[0] + 1) * (g->n[1] + 1) * (g->n[2] + 1);
@@ -918,4 +940,24 @@ extern "C" int lepic_collisions(lepic_ctx *ctx, const lepic_grid *g, lepic_pset 
   }
   if (hc.overflow) return lepic_fail(ctx, LEPIC_ERR_CAPACITY, "collisions: product capacity exceeded");
   return LEPIC_OK;
+}
+
+extern "C" int lepic_collisions(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *parts,
+                                const double *np_red, const lepic_chem *chem, int32_t ns_coll,
+                                double dt, int32_t mode, int32_t *iseed, uint64_t philox_key,
+                                uint64_t philox_counter, double *Pcoll, lepic_mcc_stats *stats) {
+  return collisions_impl(ctx, g, parts, np_red, chem, ns_coll, dt, mode, iseed, philox_key,
+                         philox_counter, Pcoll, stats, nullptr, nullptr);
+}
+
+extern "C" int lepic_collisions_deposited(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *parts,
+                                          const double *np_red, const lepic_chem *chem,
+                                          int32_t ns_coll, double dt, int32_t mode, int32_t *iseed,
+                                          uint64_t philox_key, uint64_t philox_counter,
+                                          double *Pcoll, lepic_mcc_stats *stats, double *np_dep,
+                                          const double *kq) {
+  if (!ctx) return LEPIC_ERR_ARG;
+  if (!np_dep || !kq) return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions_deposited: null density");
+  return collisions_impl(ctx, g, parts, np_red, chem, ns_coll, dt, mode, iseed, philox_key,
+                         philox_counter, Pcoll, stats, np_dep, kq);
 }

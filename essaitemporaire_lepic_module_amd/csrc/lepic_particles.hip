@@ -548,22 +548,6 @@ __device__ inline bool deposit_fracs_q(const KGrid &g, double x, double y, doubl
   return true;
 }
 
-// Fractions of deposit_weights (same range guard, same arithmetic).
-__device__ inline bool deposit_fracs(const KGrid &g, double x, double y, double z, int &ix,
-                                     int &iy, int &iz, double &px, double &py, double &pz) {
-  const double xmax = (double)g.n1 * g.h1, ymax = (double)g.n2 * g.h2,
-               zmax = (double)g.n3 * g.h3;
-  if (!(x >= -1.0e-12 && x <= xmax + 1.0e-12)) return false;
-  if (!(y >= -1.0e-12 && y <= ymax + 1.0e-12)) return false;
-  if (!(z >= -1.0e-12 && z <= zmax + 1.0e-12)) return false;
-  ix = (int)qdiv(x, g.q1) + 1;
-  iy = (int)qdiv(y, g.q2) + 1;
-  iz = (int)qdiv(z, g.q3) + 1;
-  px = qdiv((double)ix * g.h1 - x, g.q1);
-  py = qdiv((double)iy * g.h2 - y, g.q2);
-  pz = qdiv((double)iz * g.h3 - z, g.q3);
-  return true;
-}
 
 // Offset of stencil node q (the reference's order w1..w8) from the lower node.
 __device__ inline int node_off(int q, int bx, int bxy) {

@@ -304,22 +304,30 @@ class ReactionTables:
 def perform_collisions_step(parts: list, n, h, np_red: torch.Tensor, tables: ReactionTables,
                             ns_coll: int, dt: float, iseed: list | None, Pcoll: torch.Tensor,
                             mode: int = _native.MCC_REPLAY_RAN2, philox_key: int = 0,
-                            philox_counter: int = 0) -> dict:
+                            philox_counter: int = 0, np_dep: torch.Tensor | None = None,
+                            kq: torch.Tensor | None = None) -> dict:
     """Null-collision MCC (mod_collisions.f90:214-739) on the tracked species.
 
     mode MCC_REPLAY_RAN2 replays the reference's sequential chain with the
     ran2 seed iseed[0] (updated in place); MCC_PHILOX is the parallel
-    production path.  Returns per-species event counts."""
+    production path.  np_dep (with kq): the step's density, deposited before
+    MCC; products are added to it and killed particles taken out, so that it
+    equals the deposit after MCC.  Returns per-species event counts."""
     ctx = context(parts[0].device)
     g = _native.grid(n, h)
     nt = len(parts)
     arr = (_native.PSet * nt)(*[p.pset() for p in parts])
     seed = C.c_int32(int(iseed[0]) if iseed else 0)
     st = _native.MCCStats()
-    ctx.check(ctx.lib.lepic_collisions(ctx.h, C.byref(g), arr, ptr(np_red), C.byref(tables.c),
-                                       int(ns_coll), float(dt), int(mode), C.byref(seed),
-                                       C.c_uint64(philox_key), C.c_uint64(philox_counter),
-                                       ptr(Pcoll), C.byref(st)))
+    args = (ctx.h, C.byref(g), arr, ptr(np_red), C.byref(tables.c), int(ns_coll), float(dt),
+            int(mode), C.byref(seed), C.c_uint64(philox_key), C.c_uint64(philox_counter),
+            ptr(Pcoll), C.byref(st))
+    if np_dep is not None:
+        if kq is None:
+            raise ValueError("np_dep needs kq")
+        ctx.check(ctx.lib.lepic_collisions_deposited(*args, ptr(np_dep), ptr(kq)))
+    else:
+        ctx.check(ctx.lib.lepic_collisions(*args))
     for p, s in zip(parts, arr):
         p.n = int(s.n)
     if iseed is not None and mode == _native.MCC_REPLAY_RAN2:

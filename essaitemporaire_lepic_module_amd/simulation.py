@@ -7,7 +7,9 @@ velocity update when B is on):
   E -> push -> wall BC -> MCC (istep mod nb_step_collisions == 0) -> deposit
   -> reduce density.
 With fused=True the push, the wall BC and (on steps without MCC) the
-deposit run as one kernel per species.  With lazy=True (production) that
+deposit run as one kernel per species; the production path (lazy=True) also
+deposits in that kernel on MCC steps, and MCC then adds the density of the
+products it appends and takes out that of the particles it kills.  With lazy=True (production) that
 kernel works in place and leaves holes where particles were lost; the
 periodic sort removes them, keeping the survivors' order, and losses are
 counted on the device without a host round trip.
@@ -177,14 +179,17 @@ class Simulation:
             F.calc_Efield_modular(n, h, self.phi, self.E4, self.bcnd)
         collide = (self.tables is not None and c.nb_step_collisions > 0
                    and istep % c.nb_step_collisions == 0)
+        # the production path deposits in the push on collision steps too and
+        # lets MCC add its products to (and take its kills out of) that density
+        dep_first = self.fused and self.lazy
         if self.fused:
             F.clear_np_thread(self.np_thread)
             push = P.push_bc_deposit_inplace if self.lazy else P.push_bc_deposit
             for s, p in enumerate(self.parts):
-                np_s = None if collide else self.np_thread[s]
+                np_s = None if collide and not dep_first else self.np_thread[s]
                 if self.lazy:
                     # particles pushed = slots - holes before the call (read when timed)
-                    phase = "push_bc_deposit" if not collide else "push_bc"
+                    phase = "push_bc_deposit" if np_s is not None else "push_bc"
                     units = (p.n, self.lost_acc[s].clone()) if self._timing(phase) else 0
                     with self._phase(phase, units):
                         P.push_bc_deposit_inplace(
@@ -218,8 +223,9 @@ class Simulation:
                 self.last_mcc = P.perform_collisions_step(
                     self.parts, n, h, self.np_red, self.tables, c.nb_step_collisions, c.dt,
                     self.iseed, self.Pcoll, mode=self.mcc_mode, philox_key=self.philox_key,
-                    philox_counter=istep)
-        if collide or not self.fused:
+                    philox_counter=istep, np_dep=self.np_thread if dep_first else None,
+                    kq=self.kq if dep_first else None)
+        if (collide and not dep_first) or not self.fused:
             with self._phase("deposit"):
                 F.clear_np_thread(self.np_thread)
                 for s, p in enumerate(self.parts):

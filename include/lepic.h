@@ -222,6 +222,18 @@ int lepic_collisions(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *parts,
                      const double *np_red, const lepic_chem *chem, int32_t ns_coll, double dt,
                      int32_t mode, int32_t *iseed, uint64_t philox_key, uint64_t philox_counter,
                      double *Pcoll, lepic_mcc_stats *stats);
+/* lepic_collisions for a step whose deposit already ran before MCC (the
+ * production collision step deposits in the push kernel): np_dep (device,
+ * [ntype_tracked][nodes], as deposit_particle_set_to_np_thread fills it) gets
+ * the deposit of every product MCC appends and loses that of every particle
+ * MCC kills, with the deposit's weights (kq, Nm / (h1 h2 h3)).  It then holds
+ * what the reference's deposit after MCC (mod_simulation.f90:425-451) gives,
+ * summed in another order. */
+int lepic_collisions_deposited(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *parts,
+                               const double *np_red, const lepic_chem *chem, int32_t ns_coll,
+                               double dt, int32_t mode, int32_t *iseed, uint64_t philox_key,
+                               uint64_t philox_counter, double *Pcoll, lepic_mcc_stats *stats,
+                               double *np_dep, const double *kq);
 
 /* ---------------- fused particle phase ---------------- */
 /* One launch per species of gather + (Boris|leapfrog) push + wall BC +

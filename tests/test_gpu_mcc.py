@@ -244,3 +244,55 @@ def test_mcc_philox_fast_path_matches_general_trial(gpu, rng):
         kb = np.stack([gb[k][N:m] for k in ("x", "y", "z", "vx", "vy", "vz")], 1)
         np.testing.assert_array_equal(ka[np.lexsort(ka.T[::-1])], kb[np.lexsort(kb.T[::-1])])
     np.testing.assert_allclose(pa, pb, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("mode", [_native.MCC_PHILOX, _native.MCC_REPLAY_RAN2])
+def test_mcc_deposited_density_matches_deposit_after(gpu, rng, mode):
+    """The production collision step deposits before MCC and lets MCC add its
+    products and take out its kills: the density must equal the deposit of
+    the particles after MCC (the reference's order, mod_simulation.f90:425-451),
+    to the summation-order tolerance of the deposit."""
+    from essaitemporaire_lepic_module_amd import fields as F
+    from essaitemporaire_lepic_module_amd import particles as P
+    c = cfg.cfg2_pic_mcc(n=6, np_cell=1)
+    n, h = c.n, c.h
+    sp = cfg.argon_species()
+    E = cfg.argon_energy_grid(200)
+    reactions = [((0, 2), (0, 2), 0.0, np.full_like(E, 1e-19)),
+                 ((0, 2), (0, 1, 0), 15.8, cfg.lotz_like(E, cfg.S0_IONIZATION, 15.8)),
+                 ((0, 2), (1,), 0.0, np.full_like(E, 2e-20)),      # the electron is lost
+                 ((1, 2), (1, 2), 0.0, np.full_like(E, 5e-19))]
+    t = cfg.build_reaction_tables(sp, c.ngas, reactions, [(E.copy(), r[3]) for r in reactions])
+    tables = P.ReactionTables(t, c.Nm, gpu)
+    N = 30_000 if mode == _native.MCC_PHILOX else 4_000
+    L = n[0] * h[0]
+    speed = np.sqrt(2 * rng.uniform(1, 300, N) * QE / cfg.ME)
+    dirs = rng.standard_normal((N, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    ve = speed[:, None] * dirs
+    pos = L * rng.random((N, 3))
+    ipos = L * rng.random((N, 3))
+    vi = 3e4 * rng.standard_normal((N, 3))
+    dead = (rng.random(N) < 0.02).astype(np.int8) * 2
+    e = P.ParticleSet(3 * N + 64, "cuda:0")
+    e.from_host(pos[:, 0], pos[:, 1], pos[:, 2], ve[:, 0], ve[:, 1], ve[:, 2], dead=dead)
+    ion = P.ParticleSet(3 * N + 64, "cuda:0")
+    ion.from_host(ipos[:, 0], ipos[:, 1], ipos[:, 2], vi[:, 0], vi[:, 1], vi[:, 2])
+    bc = torch.as_tensor(cfg.grounded_box_bcnd(n), device=gpu)
+    kq = F.build_kq(bc, n)
+    shape = (2,) + cfg.node_shape(n)
+    np_dep = torch.zeros(shape, dtype=torch.float64, device=gpu)
+    for s, ps in enumerate((e, ion)):
+        F.deposit_particle_set_to_np_thread(ps, n, h, kq, c.Nm[s], np_dep[s])
+    np_red = torch.as_tensor(np.abs(rng.standard_normal(shape)) * 1e15, device=gpu)
+    Pcoll = torch.zeros(t["ntype_total"], dtype=torch.float64, device=gpu)
+    st = P.perform_collisions_step([e, ion], n, h, np_red, tables, c.nb_step_collisions, c.dt,
+                                   [123456], Pcoll, mode=mode, philox_key=5, philox_counter=7,
+                                   np_dep=np_dep, kq=kq)
+    assert st["appended"][0] > 0 and st["appended"][1] > 0
+    assert (e.to_host()["dead"] == 1).any()          # the loss channel fired
+    want = torch.zeros(shape, dtype=torch.float64, device=gpu)
+    for s, ps in enumerate((e, ion)):
+        F.deposit_particle_set_to_np_thread(ps, n, h, kq, c.Nm[s], want[s])
+    w, g = want.cpu().numpy(), np_dep.cpu().numpy()
+    np.testing.assert_allclose(g, w, rtol=1e-12, atol=1e-12 * np.abs(w).max())
