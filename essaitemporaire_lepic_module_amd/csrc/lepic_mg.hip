@@ -450,10 +450,14 @@ __global__ __launch_bounds__(BLOCK) void k_prolong(KLevel F, KLevel C) {
 // evaluated in the workgroup.
 // --------------------------------------------------------------------------
 constexpr int CBLOCK = 1024;
-constexpr int64_t COARSE_MAX_NODES = 12000;
+// u and b of every level the workgroup handles live in its LDS during the
+// launch (2 x 8704 doubles, 136 KB): each barrier-separated phase then waits
+// on LDS instead of L2.  lc is the first level from which the remaining
+// levels fit.
+constexpr int64_t COARSE_LDS_NODES = 8704;
 
 struct CoarseArgs {
-  KLevel L[MAX_LEVELS];
+  KLevel L[MAX_LEVELS];   // u and b point to global memory; the kernel works on LDS copies
   int lc, ng;
   double omega, eps;
   int flag_pbc, flag_nmn;
@@ -461,6 +465,19 @@ struct CoarseArgs {
   double *rp;
   MGCtl *ctl;
 };
+
+// Level l as the workgroup sees it: u and b in LDS.
+struct CoarseLDS {
+  double *u[MAX_LEVELS];
+  double *b[MAX_LEVELS];
+};
+
+__device__ inline KLevel lds_level(const CoarseArgs &A, const CoarseLDS &S, int l) {
+  KLevel v = A.L[l];
+  v.u = S.u[l];
+  v.b = S.b[l];
+  return v;
+}
 
 // Workgroup sum in a fixed order: butterfly within each wave, then the 16
 // wave sums in index order.  Every thread gets the result.
@@ -476,8 +493,9 @@ __device__ inline double wg_sum(double a, double *red) {
 }
 
 // sor_rb (sors.f90:1-535) on level l: at most nr sweeps, exit rules as k_decide.
-__device__ void wg_sor_rb(const CoarseArgs &A, int l, int dig, int64_t nr, double *red) {
-  const KLevel &Lv = A.L[l];
+__device__ void wg_sor_rb(const CoarseArgs &A, const CoarseLDS &S, int l, int dig, int64_t nr,
+                          double *red) {
+  const KLevel Lv = lds_level(A, S, l);
   const KGrid &g = Lv.g;
   const int64_t nsor = (int64_t)(g.n1 / 2 + 1) * (g.n2 + 1) * (g.n3 + 1);
   const int64_t nres = (int64_t)(g.n1 + 1) * (g.n2 + 1) * (g.n3 + 1);
@@ -515,9 +533,9 @@ __device__ void wg_sor_rb(const CoarseArgs &A, int l, int dig, int64_t nr, doubl
   }
 }
 
-__device__ void wg_restriction(const CoarseArgs &A, int l) {
-  const KLevel &F = A.L[l];
-  const KLevel &C = A.L[l + 1];
+__device__ void wg_restriction(const CoarseArgs &A, const CoarseLDS &S, int l) {
+  const KLevel F = lds_level(A, S, l);
+  const KLevel C = lds_level(A, S, l + 1);
   const KGrid &g = F.g;
   const int64_t nc = ((int64_t)C.g.n1 + 3) * ((int64_t)C.g.n2 + 3) * ((int64_t)C.g.n3 + 3);
   for (int64_t t = threadIdx.x; t < nc; t += CBLOCK) C.u[t] = 0.0;
@@ -533,30 +551,63 @@ __device__ void wg_restriction(const CoarseArgs &A, int l) {
   __syncthreads();
 }
 
-__device__ void wg_prolongation(const CoarseArgs &A, int l) {  // L[l] += P(L[l+1])
-  const KGrid &g = A.L[l].g;
+__device__ void wg_prolongation(const CoarseArgs &A, const CoarseLDS &S, int l) {  // L[l] += P(L[l+1])
+  const KLevel F = lds_level(A, S, l);
+  const KLevel C = lds_level(A, S, l + 1);
+  const KGrid &g = F.g;
   const int64_t nt = (int64_t)(g.n1 / 2 + 1) * (g.n2 / 2 + 1) * (g.n3 / 2 + 1);
-  for (int64_t t = threadIdx.x; t < nt; t += CBLOCK) prolong_node(A.L[l], A.L[l + 1], t);
+  for (int64_t t = threadIdx.x; t < nt; t += CBLOCK) prolong_node(F, C, t);
   __syncthreads();
 }
 
 // The V-cycle of mg (mg.f90:1-246) restricted to levels lc..ng-1; the
 // prolongation into level lc-1 is left to the caller.
+// The host sizes lc so that levels lc..ng-1 hold at most COARSE_LDS_NODES
+// nodes together.  u and b of every level are copied in at the start (the
+// global copies of the levels below lc hold what the global-memory version
+// would read: zeros at the ghost nodes no restriction writes, values that
+// the V-cycle overwrites before reading elsewhere), and u of level lc, the
+// only array read after the launch, is copied back at the end.
 __global__ __launch_bounds__(CBLOCK) void k_mg_coarse(CoarseArgs A) {
-  __shared__ double red[CBLOCK];
+  __shared__ double red[CBLOCK / LEPIC_WAVE];
+  __shared__ double lds[2 * COARSE_LDS_NODES];
+  __shared__ CoarseLDS S;   // level pointers into lds, uniform
   const int ng = A.ng;
-  for (int l = A.lc; l <= ng - 2; ++l) {
-    wg_sor_rb(A, l, l + 1 - ng, 4, red);
-    __syncthreads();
-    wg_restriction(A, l);
+  {
+    double *p = lds;
+    for (int l = A.lc; l < ng; ++l) {
+      const KGrid &g = A.L[l].g;
+      const int64_t nn = ((int64_t)g.n1 + 3) * ((int64_t)g.n2 + 3) * ((int64_t)g.n3 + 3);
+      if (threadIdx.x == 0) {
+        S.u[l] = p;
+        S.b[l] = p + nn;
+      }
+      for (int64_t t = threadIdx.x; t < nn; t += CBLOCK) {
+        p[t] = A.L[l].u[t];
+        p[nn + t] = A.L[l].b[t];
+      }
+      p += 2 * nn;
+    }
   }
-  wg_sor_rb(A, ng - 1, 0, A.first_cycle ? 1000000 : 4, red);
   __syncthreads();
-  if (ng - 2 >= A.lc) wg_prolongation(A, ng - 2);
-  for (int l = ng - 2; l >= A.lc; --l) {
-    wg_sor_rb(A, l, ng - 1 - l, 4, red);
+  for (int l = A.lc; l <= ng - 2; ++l) {
+    wg_sor_rb(A, S, l, l + 1 - ng, 4, red);
     __syncthreads();
-    if (l - 1 >= A.lc) wg_prolongation(A, l - 1);
+    wg_restriction(A, S, l);
+  }
+  wg_sor_rb(A, S, ng - 1, 0, A.first_cycle ? 1000000 : 4, red);
+  __syncthreads();
+  if (ng - 2 >= A.lc) wg_prolongation(A, S, ng - 2);
+  for (int l = ng - 2; l >= A.lc; --l) {
+    wg_sor_rb(A, S, l, ng - 1 - l, 4, red);
+    __syncthreads();
+    if (l - 1 >= A.lc) wg_prolongation(A, S, l - 1);
+  }
+  __syncthreads();
+  {
+    const KGrid &g = A.L[A.lc].g;
+    const int64_t nn = ((int64_t)g.n1 + 3) * ((int64_t)g.n2 + 3) * ((int64_t)g.n3 + 3);
+    for (int64_t t = threadIdx.x; t < nn; t += CBLOCK) A.L[A.lc].u[t] = S.u[A.lc][t];
   }
 }
 
@@ -762,11 +813,14 @@ extern "C" int lepic_poisson_solve(lepic_ctx *ctx, const lepic_grid *g, double *
   // operations is mg.f90's; the stretch from the down-leg sor_rb on lc to the
   // up-leg sor_rb on lc touches no level above lc and is issued as one launch.
   S.lc = ng;
-  for (int l = 1; l < ng; ++l)
-    if (Solver::grid_nodes_k(S.L[l].g) <= COARSE_MAX_NODES) {
+  for (int l = 1; l < ng; ++l) {
+    int64_t tot = 0;
+    for (int m = l; m < ng; ++m) tot += Solver::grid_nodes_k(S.L[m].g);
+    if (tot <= COARSE_LDS_NODES) {
       S.lc = l;
       break;
     }
+  }
   // one V-cycle, mg (mg.f90:99-239)
   auto vcycle = [&](int kmg) -> int {
     for (int ig = 1; ig <= 2 * ng - 1; ++ig) {
