@@ -137,15 +137,19 @@ def main():
     holes = torch.zeros_like(sim.lost_acc)
     slots = 0
     mg_cycles = []
+    marks = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for k in range(args.steps):
+        marks[k].record()
         slots += sum(p.n for p in sim.parts)
         holes += sim.lost_acc
         sim.advance_one_step(istep)
         mg_cycles.append(sim.last_mg["cycles"])
         istep += 1
+    marks[-1].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    step_ms = sorted(a.elapsed_time(b) for a, b in zip(marks[:-1], marks[1:]))
     particle_steps = slots - int(holes.sum().item())
     kernel_phases = sim.phase_times()
     # diagnostic pass (not timed for the value): every phase bracketed by events
@@ -173,6 +177,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": wall * 1e3 / args.steps,
+        "step_ms_median": step_ms[len(step_ms) // 2],
+        "step_ms_p90": step_ms[min(len(step_ms) - 1, (9 * len(step_ms)) // 10)],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

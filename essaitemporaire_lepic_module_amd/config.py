@@ -152,14 +152,24 @@ def cfg1_plain_box(n: int = 32, side: float = 0.01, np_cell: int = 20, steps: in
 
 
 def cfg2_pic_mcc(n: int = 128, np_cell: int = 64, steps: int = 1000, ng: int = 7,
-                 pressure_Pa: float = 1.0, Bz: float = 0.01) -> Case:
+                 pressure_Pa: float = 1.0, Bz: float = 0.01, deck=None) -> Case:
     """PIC-MCC at scale: h = 3.125e-4 m (4 cm box at 128^3), uniform Bz,
-    argon background, analytic cross sections, MCC every 10 steps, sort every 20."""
+    argon background, analytic cross sections, MCC every 10 steps, sort every 20.
+    deck: a chemistry deck in the reference's format (path or text) whose
+    species and reactions replace the analytic argon set."""
     nn = (n, n, n)
     h = (3.125e-4,) * 3
     n0 = 1e15
     sp = argon_species()
     ngas = pressure_Pa / (KB * 300.0)
+    if deck is not None:
+        from . import chemistry
+        chem = chemistry.read_reactions(deck, ngas, Te=sp[0].T_eV)
+        c = Case("pic-mcc", nn, h, grounded_box_bcnd(nn), 1, {1: 0.0}, chem.species, n0,
+                 np_cell, 0.1 / omega_pe(n0), steps, ng=ng, B_uniform=(0.0, 0.0, Bz), ngas=ngas,
+                 nb_step_sort=20, nb_step_collisions=10)
+        c.chemistry = chemistry.build_tables(chem, ngas)
+        return c
     c = Case("pic-mcc", nn, h, grounded_box_bcnd(nn), 1, {1: 0.0}, sp, n0, np_cell,
              0.1 / omega_pe(n0), steps, ng=ng, B_uniform=(0.0, 0.0, Bz), ngas=ngas,
              nb_step_sort=20, nb_step_collisions=10)
@@ -204,9 +214,13 @@ def argon_chemistry(species, ngas: float, npts: int = 200) -> dict:
     return build_reaction_tables(species, ngas, reactions, tables)
 
 
-def build_reaction_tables(species, ngas, reactions, tables) -> dict:
+def build_reaction_tables(species, ngas, reactions, tables, neutral_density=None,
+                          kinds=None) -> dict:
     """Common energy grid, interpolated sig, sig_list, p_ncol and sigv_mx as
-    mod_reactions.f90:414-558 builds them."""
+    mod_reactions.f90:414-558 builds them.  neutral_density: per neutral
+    species (default ngas each); kinds: (reaction type, electrons among the
+    products) per reaction, stored after the species in col_info as the
+    reference's parser does (mod_reactions.f90:255-273)."""
     ntot = len(species)
     ntrk = sum(1 for s in species if s.tracked)
     ncol = len(reactions)
@@ -239,6 +253,9 @@ def build_reaction_tables(species, ngas, reactions, tables) -> dict:
         col_info[icol, 1] = len(by)
         col_info[icol, 2:2 + len(re)] = re
         col_info[icol, 2 + len(re):2 + len(re) + len(by)] = by
+        if kinds is not None and 2 + len(re) + len(by) + 1 < 10:
+            col_info[icol, 2 + len(re) + len(by)] = kinds[icol][0]
+            col_info[icol, 3 + len(re) + len(by)] = kinds[icol][1]
         sig_Eth[icol] = Eth
         r1, r2 = re[0], re[1]
         sig_list[r1, p_ncol[r1]] = icol
@@ -252,7 +269,8 @@ def build_reaction_tables(species, ngas, reactions, tables) -> dict:
     vt0 = np.array([thermal_speed(s.T_eV, s.mass) for s in species])
     return {
         "ntype_tracked": ntrk, "ntype_total": ntot, "mass": masses, "vt0": vt0,
-        "neutral_density": np.array([ngas for s in species if not s.tracked], float),
+        "neutral_density": np.array(neutral_density if neutral_density is not None
+                                    else [ngas for s in species if not s.tracked], float),
         "nu_uplim": nu_uplim, "p_ncol": p_ncol, "sig_list": sig_list.reshape(-1),
         "ncol": ncol, "npt": npt, "sig_Er": merged, "sig": sig.reshape(-1), "sig_Eth": sig_Eth,
         "col_info": col_info.reshape(-1), "sigv_mx": sigv_mx.reshape(-1),

@@ -128,3 +128,38 @@ def test_resorted_production_steps_match_oracle(gpu, oracle):
                                            err_msg=f"step {istep} species {s} column {col}")
         np.testing.assert_allclose(sim.phi.cpu().numpy(), ref.phi, rtol=0,
                                    atol=RTOL_STEPS * max(np.abs(ref.phi).max(), 1e-30))
+
+
+def test_deck_chemistry_production_steps(gpu):
+    """Production steps with the chemistry read from a deck in the
+    reference's format, including an electron-ion reaction (a charged target,
+    whose picks fail at MCC time, mod_collisions.f90:473-478, and which sends
+    electrons through the general trial)."""
+    import numpy as np
+    from essaitemporaire_lepic_module_amd import chemistry as CH
+    from essaitemporaire_lepic_module_amd.simulation import Simulation
+    sp = cfg.argon_species()
+    E = cfg.argon_energy_grid(120)
+    rs = [((0, 1), (0, 1), 0.0, 1, np.full_like(E, 1e-17)),
+          ((0, 2), (0, 2), 0.0, 1, np.full_like(E, 1e-19)),
+          ((0, 2), (0, 1, 0), 15.8, 2, cfg.lotz_like(E, cfg.S0_IONIZATION, 15.8)),
+          ((1, 2), (1, 2), 0.0, 4, np.full_like(E, 5e-19))]
+    reactions = [CH.Reaction("", re, by, Eth, 0.0, kind, sum(1 for p in by if p == 0), E.copy(), sg)
+                 for re, by, Eth, kind, sg in rs]
+    deck = CH.write_reactions(sp, [1.0, 1.0, 1.0], reactions)
+    case = dataclasses.replace(cfg.cfg2_pic_mcc(n=8, np_cell=8, ng=3, deck=deck), nb_step_sort=5)
+    sim = Simulation(case, "cuda:0", fused=True, loader="device", mcc_mode=_native.MCC_PHILOX,
+                     lazy=True, resort_every=[2, 0])
+    n0 = sim.particle_count()
+    for istep in range(1, 12):
+        sim.advance_one_step(istep)
+        if istep == 10:
+            st = sim.last_mcc
+            assert st["err_coll"][0] > 0 and st["accepted"][0] > 0
+    torch.cuda.synchronize()
+    n1 = sim.particle_count()
+    assert 0.5 * n0 < n1 < 1.5 * n0
+    for p in sim.parts:
+        a = p.to_host()
+        live = a["dead"] == 0
+        assert np.isfinite(a["vx"][live]).all()
