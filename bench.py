@@ -44,31 +44,68 @@ def measured_traffic(kernel, args):
     return None
 
 
-def cpu_baseline(case, sample_particles=2_000_000, seconds=20.0):
-    """Oracle (C restatement of the reference, 1 thread) on a bounded sample:
-    push+BC+deposit per particle on the 128^3 grid, one warm Poisson solve,
-    E and rho at 128^3; composed into particle-steps/s of the full workload."""
+def _cpu_chunk(job):
+    """One host core's share of the particle sample: the oracle's Boris push,
+    wall BC and deposit (into its own density array, as the reference's
+    per-thread np_thread) on `count` particles; returns the seconds spent."""
+    import numpy as np
+    from oracle import oracle as O
+    (n, h, bcnd, kq, Bz, q, mass, dt, Nm, ngrid, count, seed, barrier) = job
+    rng = np.random.default_rng(seed)
+    L = np.array([n[d] * h[d] for d in range(3)])
+    pos = rng.random((count, 3)) * L
+    vel = 1e6 * rng.standard_normal((count, 3))
+    p = O.Particles(count + 16).set(pos[:, 0], pos[:, 1], pos[:, 2], vel[:, 0], vel[:, 1],
+                                    vel[:, 2])
+    E4 = np.zeros(bcnd.shape + (4,))
+    E4[..., :3] = 1e3 * rng.standard_normal(bcnd.shape + (3,))
+    barrier.wait()
+    t0 = time.perf_counter()
+    O.move_boris(n, h, p, E4, 1, (0.0, 0.0, Bz), None, q, mass, dt)
+    O.apply_bc(n, h, p, bcnd, 0, 0, 1, -1, Nm * q, mass, Nm, ngrid)
+    O.deposit(n, h, p, kq, Nm)
+    return time.perf_counter() - t0
+
+
+def _cpu_init(barrier):
+    global _BARRIER
+    _BARRIER = barrier
+
+
+def _cpu_chunk_in_pool(job):
+    return _cpu_chunk(job + (_BARRIER,))
+
+
+def cpu_baseline(case, sample_particles=32_000_000):
+    """The oracle (C restatement of the reference) on the host cores of this
+    machine, one process per core (spawned, so none of them touches the GPU):
+    each pushes, wall-tests and deposits its share of a bounded particle
+    sample on the 128^3 grid at the same moment; plus one warm MG solve and
+    E + rho at 128^3 (single-threaded, as the oracle is); composed into
+    particle-steps/s of the full workload."""
+    import multiprocessing as mp
     import numpy as np
     from oracle import oracle as O
     O.build()
     n, h = case.n, case.h
-    rng = np.random.default_rng(7)
-    L = np.array([n[d] * h[d] for d in range(3)])
-    m = sample_particles
-    pos = rng.random((m, 3)) * L
-    vel = 1e6 * rng.standard_normal((m, 3))
-    p = O.Particles(m + 16).set(pos[:, 0], pos[:, 1], pos[:, 2], vel[:, 0], vel[:, 1], vel[:, 2])
-    bc = case.bcnd
-    kq = case.kq()
-    E4 = np.zeros(bc.shape + (4,))
-    E4[..., :3] = 1e3 * rng.standard_normal(bc.shape + (3,))
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(16, cores))
     sp = case.species[0]
-    t0 = time.perf_counter()
-    O.move_boris(n, h, p, E4, 1, case.B_uniform, None, sp.charge, sp.mass, case.dt)
-    O.apply_bc(n, h, p, bc, 0, 0, 1, -1, case.Nm[0] * sp.charge, sp.mass, case.Nm[0], case.ngrid)
-    O.deposit(n, h, p, kq, case.Nm[0])
-    t_part = (time.perf_counter() - t0) / m
+    bc, kq = case.bcnd, case.kq()
+    m = sample_particles
+    per = [m // cores + (1 if k < m % cores else 0) for k in range(cores)]
+    ctx = mp.get_context("spawn")
+    barrier = ctx.Barrier(cores)
+    jobs = [(n, h, bc, kq, case.B_uniform[2], sp.charge, sp.mass, case.dt, case.Nm[0],
+             case.ngrid, c, 7 + k) for k, c in enumerate(per)]
+    with ctx.Pool(cores, initializer=_cpu_init, initargs=(barrier,)) as pool:
+        secs = pool.map(_cpu_chunk_in_pool, jobs)
+    t_part = max(secs) / m              # seconds per particle on `cores` cores together
     # grid phase: rho from a smooth density, then one solve warm-started from its own solution
+    rng = np.random.default_rng(7)
     rho = np.zeros(bc.shape)
     rho[bc == -1] = 1e-5 * rng.standard_normal(int((bc == -1).sum()))
     phi = np.zeros(bc.shape)
@@ -84,11 +121,12 @@ def cpu_baseline(case, sample_particles=2_000_000, seconds=20.0):
     t_grid = time.perf_counter() - t0
     N = 2 * case.np_cell * n[0] * n[1] * n[2]
     step = N * t_part + t_mg + t_grid
-    return {"value": N / step, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle, 1 thread: Boris push+wall BC+deposit of {m} particles on the "
-                       f"{n[0]}^3 grid ({t_part * 1e9:.0f} ns/particle), one warm MG solve "
-                       f"({t_mg * 1e3:.0f} ms) and E+rho ({t_grid * 1e3:.0f} ms), composed to "
-                       f"the {N}-particle step")}
+    return {"value": N / step, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+            "sample": (f"oracle on {cores} host cores (one process each, concurrent): Boris "
+                       f"push+wall BC+deposit of {m} particles on the {n[0]}^3 grid "
+                       f"({t_part * 1e9:.1f} ns/particle for the cores together), one warm MG "
+                       f"solve ({t_mg * 1e3:.0f} ms) and E+rho ({t_grid * 1e3:.0f} ms) on one "
+                       f"core, composed to the {N}-particle step")}
 
 
 def main():
