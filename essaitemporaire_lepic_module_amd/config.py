@@ -107,6 +107,9 @@ class Case:
     nb_step_sort: int = 10
     nb_step_collisions: int = 10
     chemistry: dict | None = None
+    phi_init: np.ndarray | None = None   # initial potential (a deck's generate_boundary u)
+    ni0: list | None = None              # loader acceptance n/n0 per tracked species
+    x_load: float = 0.0                  # loading region x < x_load (m); <= 0: whole box
 
     @property
     def tracked(self):
@@ -118,13 +121,16 @@ class Case:
 
     @property
     def Nm(self):
-        """Macro-particle weight (mod_simParams.f90:118-122)."""
-        vol = self.h[0] * self.h[1] * self.h[2]
-        return [(self.n0 if s.charge != 0.0 else self.ngas) * vol / abs(self.np_cell)
-                for s in self.species]
+        """Macro-particle weight (mod_simParams.f90:118-122), with the
+        reference's order of operations: n * h1 * h2 * h3 / |np_cell|."""
+        return [(self.n0 if s.charge != 0.0 else self.ngas) * self.h[0] * self.h[1] * self.h[2]
+                / abs(self.np_cell) for s in self.species]
 
     def phi0(self) -> np.ndarray:
-        """Initial potential: electrode voltages on their labelled nodes."""
+        """Initial potential: the deck's generate_boundary potential when the
+        case has one, else electrode voltages on their labelled nodes."""
+        if self.phi_init is not None:
+            return np.array(self.phi_init, np.float64)
         phi = np.zeros(node_shape(self.n))
         for lab, v in self.electrode_V.items():
             phi[self.bcnd == lab] = v

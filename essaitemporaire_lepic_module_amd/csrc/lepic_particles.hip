@@ -517,10 +517,21 @@ __device__ inline void box_flush(const KGrid &g, DS &ds, const double *__restric
   }
 }
 
+// Cell of a position for the LDS box bounds only: x * RN(1/h) without the
+// corrections, so the index can be one off next to a cell face.  The bounds
+// decide which deposits go through LDS and which straight to the replica
+// (every deposit tests its own stencil against the box), never what is
+// deposited, so an approximate cell costs at most a few global atomics.
+// Clamped in floating point first so that any input, NaN included, converts.
+__device__ inline int approx_index(double x, const QDiv &d, int n) {
+  const double q = fmin(fmax(x * d.y, -1.0), (double)n + 1.0);
+  return imax(0, imin(n + 1, (int)q + 1));
+}
+
 __device__ inline void cell_of(const KGrid &g, double x, double y, double z, int c[3]) {
-  c[0] = imax(0, imin(g.n1 + 1, (int)qdiv(x, g.q1) + 1));
-  c[1] = imax(0, imin(g.n2 + 1, (int)qdiv(y, g.q2) + 1));
-  c[2] = imax(0, imin(g.n3 + 1, (int)qdiv(z, g.q3) + 1));
+  c[0] = approx_index(x, g.q1, g.n1);
+  c[1] = approx_index(y, g.q2, g.n2);
+  c[2] = approx_index(z, g.q3, g.n3);
 }
 
 // --------------------------------------------------------------------------
@@ -837,10 +848,8 @@ __global__ __launch_bounds__(BLOCK) void k_push(
 #pragma unroll
     for (int q = 0; q < CPT; ++q)
       if (live[q]) {
-        const bool ok = qdiv_coord_ok(x[q]) && qdiv_coord_ok(y[q]) && qdiv_coord_ok(z[q]);
-        const int c[3] = {imax(0, imin(g.n1 + 1, (int)qdiv_pre(x[q], g.q1, ok) + 1)),
-                          imax(0, imin(g.n2 + 1, (int)qdiv_pre(y[q], g.q2, ok) + 1)),
-                          imax(0, imin(g.n3 + 1, (int)qdiv_pre(z[q], g.q3, ok) + 1))};
+        int c[3];
+        cell_of(g, x[q], y[q], z[q], c);
         for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], c[d]); hi[d] = max(hi[d], c[d]); }
       }
     box_begin(g, ds, lo, hi);

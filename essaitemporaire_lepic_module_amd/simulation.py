@@ -68,13 +68,14 @@ class Simulation:
         self.np_red = torch.zeros((self.ntype,) + shape, dtype=torch.float64, device=dev)
         self.iseed = [P.seed_initialization(1, 0)]
         ncells = n[0] * n[1] * n[2]
-        cap = int(capacity_factor * case.np_cell * ncells) + 1024
+        cap = int(capacity_factor * abs(case.np_cell) * ncells) + 1024
         if parts is not None:
             self.parts, self.np_thread = parts, np_thread
         elif loader == "host":
             self.parts, self.np_thread = P.load_particles_modular(
                 n, h, case.bcnd, case.kq(), case.np_cell, case.vt0[: self.ntype], self.Nm,
-                [1.0] * self.ntype, self.iseed, cap=cap, device=dev)
+                list(case.ni0) if case.ni0 is not None else [1.0] * self.ntype, self.iseed,
+                x_load=case.x_load, cap=cap, device=dev)
         elif loader == "device":
             self.parts = P.load_uniform_device(n, h, self.bcnd, case.np_cell, case.vt0[: self.ntype],
                                                philox_key, cap, dev)

@@ -327,10 +327,12 @@ class OracleSim:
         self.seed = seed_value
         if parts is None:
             ncells = self.n[0] * self.n[1] * self.n[2]
-            cap = int(1.3 * case.np_cell * ncells) + 1024
+            cap = int(1.3 * abs(case.np_cell) * ncells) + 1024
+            ni0 = list(case.ni0) if case.ni0 is not None else [1.0] * self.ntype
             parts, np_thread, self.seed = load_particles(self.n, self.h, self.bcnd, self.kq,
                                                          case.np_cell, case.vt0[: self.ntype],
-                                                         self.Nm, [1.0] * self.ntype, seed_value, cap)
+                                                         self.Nm, ni0, seed_value, cap,
+                                                         x_load=case.x_load)
         self.parts, self.np_thread = parts, np.ascontiguousarray(np_thread)
         self.np_red = reduce_density(self.n, self.bcnd, self.np_thread)
         self.p_mac = np.zeros((self.ntype, 2 * (case.ngrid + 1)))

@@ -16,6 +16,9 @@ module ref_bridge
   use mod_charge_weights, only: build_kq
   use mod_electricField, only: calc_Efield_modular
   use mod_rng, only: ran2
+  use mod_config, only: Config
+  use mod_domain, only: Domain
+  use mod_magneticField, only: MagneticField
   implicit none
 contains
 
@@ -152,5 +155,43 @@ contains
     integer(c_int32_t), intent(in) :: bcnd(0:n(1)+2, 0:n(2)+2, 0:n(3)+2)
     call calc_Efield_modular(n, h, phi, E3, bcnd)
   end subroutine ref_efield
+
+  ! MagneticField%build_from_cfg on a Config holding nB field records
+  ! (file/name/info as blank-padded characters) and a Domain holding the box
+  ! extents in m; Bi comes back as (4, 0:nxB+2, 0:nyB+2, 0:nzB+2).
+  subroutine ref_build_bfield(nB, n_B, files, names, infos, scale, B0, dL, x0, y0, z0, box, &
+                              Bi, flag_thr) bind(C, name="ref_build_bfield")
+    integer(c_int32_t), value :: nB
+    integer(c_int32_t), intent(in) :: n_B(3)
+    character(kind=c_char), intent(in) :: files(5), names(20, 5), infos(2, 5)
+    real(c_double), intent(in) :: scale(5), B0(5), dL(5), x0(5), y0(5), z0(5), box(3)
+    real(c_double), intent(out) :: Bi(*)
+    integer(c_int32_t), intent(out) :: flag_thr
+    type(Config) :: cfg
+    type(Domain) :: dom
+    type(MagneticField) :: mf
+    integer :: i, k, m
+    cfg%nB = nB
+    cfg%n_B = n_B
+    cfg%flag_B = 0
+    do i = 1, 5
+      cfg%B_file(i) = files(i)
+      do k = 1, 20
+        cfg%B_name(i)(k:k) = names(k, i)
+      end do
+      do k = 1, 2
+        cfg%B_info(i)(k:k) = infos(k, i)
+      end do
+      if (i <= nB .and. cfg%B_file(i) /= 's' .and. cfg%B_file(i) /= 'S') cfg%flag_B = 1
+    end do
+    cfg%B_scale = scale; cfg%B0 = B0; cfg%dL = dL
+    cfg%x0 = x0; cfg%y0 = y0; cfg%z0 = z0
+    dom%xmax = box(1); dom%ymax = box(2); dom%zmax = box(3)
+    call mf%build_from_cfg(cfg, dom, 1)
+    m = size(mf%Bi)
+    Bi(1:m) = reshape(mf%Bi, [m])
+    flag_thr = mf%flag_thr
+    call mf%destroy()
+  end subroutine ref_build_bfield
 
 end module ref_bridge

@@ -163,3 +163,40 @@ def test_deck_chemistry_production_steps(gpu):
         a = p.to_host()
         live = a["dead"] == 0
         assert np.isfinite(a["vx"][live]).all()
+
+
+def test_deck_case_steps_match_oracle(gpu, oracle, tmp_path):
+    """A run deck in the reference's format -- periodic y-z walls (label 0), a
+    biased electrode, a Gaussian Bz map on the field grid, argon chemistry --
+    drives the same steps on the GPU and in the oracle."""
+    from essaitemporaire_lepic_module_amd import chemistry as CH
+    from essaitemporaire_lepic_module_amd import deck as D
+    from essaitemporaire_lepic_module_amd import particles as P
+    sp = cfg.argon_species()
+    E = cfg.argon_energy_grid(60)
+    reactions = [CH.Reaction("", (0, 2), (0, 2), 0.0, 0.0, 1, 1, E.copy(), np.full_like(E, 2e-19)),
+                 CH.Reaction("", (0, 2), (0, 1, 0), 15.8, 0.0, 2, 2, E.copy(),
+                             cfg.lotz_like(E, cfg.S0_IONIZATION, 15.8))]
+    (tmp_path / "chem.dat").write_text(CH.write_reactions(sp, [1.0, 1.0, 1.0], reactions))
+    n = (16, 8, 8)
+    c = D.Conditions(rname="chem.dat", Te=3.0, ng=3, eps=1e-8, omega=1.4, n=n, kt=0.3, n0=1e15,
+                     ngas=3e20, np_cell=6, ngrid=2, nsav=100, cnt_plt=1, Pabs=-1.0, nu_h=-1.0,
+                     I_inj=-1.0, nB=1, flag_B=1, n_B=n)
+    c.B[0] = D.BSource("n", "gauss", 1.0, 0.02, "Bz", 3.0e-2, 5.0e-2)
+    (tmp_path / "conditions.inp").write_text(D.write_conditions(c))
+    (tmp_path / "geometry.inp").write_text(
+        "10. 2.5 2.5\n0. 0. 0. 10. 2.5 2.5 0\n10. 0. 0. 10. 2.5 2.5 -2\n0. 0. 0. 0. 2.5 2.5 -1\n")
+    (tmp_path / "boundary.inp").write_text(D.write_boundary_potentials([0.0, 4.0]))
+    d = D.load_deck(str(tmp_path))
+    case = dataclasses.replace(d.case(steps=6), nb_step_sort=4, nb_step_collisions=2)
+    assert case.B_map is not None and case.flag_pbc == 1
+    ref, sim = _start(oracle, case, P, lazy=False)
+    for istep in range(1, 7):
+        sim.advance_one_step(istep)
+        ref.advance_one_step(istep)
+        torch.cuda.synchronize()
+        _compare(sim, ref, exact=istep == 1, step=istep)
+        if istep % 2 == 0:
+            for k in ("Nc", "accepted", "err_coll", "skipped_dead", "appended"):
+                assert sim.last_mcc[k] == ref.last_mcc[k][:len(sim.last_mcc[k])], (istep, k)
+    assert sim.iseed[0] == ref.seed
