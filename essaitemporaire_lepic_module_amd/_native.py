@@ -29,6 +29,7 @@ EXPORTS = (
     "lepic_reduce_density", "lepic_build_rho", "lepic_poisson_solve", "lepic_efield",
     "lepic_collisions", "lepic_collisions_deposited", "lepic_push_bc_deposit",
     "lepic_push_bc_deposit_inplace", "lepic_sort_compact", "lepic_bin_compact",
+    "lepic_plane_moments",
 )
 
 
@@ -149,6 +150,8 @@ def load() -> C.CDLL:
                                                         C.POINTER(C.c_int64)]),
             "lepic_sort_compact": (C.c_int, [vp, G, PS, PS]),
             "lepic_bin_compact": (C.c_int, [vp, G, PS, PS]),
+            "lepic_plane_moments": (C.c_int, [vp, G, PS, C.c_double, C.c_double, C.c_int32,
+                                              C.c_int32, C.c_int32, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

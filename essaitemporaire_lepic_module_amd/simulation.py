@@ -115,6 +115,7 @@ class Simulation:
         self.lost_acc = torch.zeros(self.ntype, dtype=torch.int64, device=dev)
         self.timers = None   # {phase: [(start_event, end_event, units), ...]} when timing
         self.timed_phases = None   # phases to time (None: all of them)
+        self.diagnostics = None    # diagnostics.Diagnostics when attached
 
     @contextlib.contextmanager
     def _phase(self, name: str, units: int = 0):
@@ -178,6 +179,8 @@ class Simulation:
                                                   c.eps, c.omega, c.ng, c.flag_pbc, c.flag_nmn)
         with self._phase("efield"):
             F.calc_Efield_modular(n, h, self.phi, self.E4, self.bcnd)
+        if self.diagnostics is not None:
+            self.diagnostics.before_push(istep)
         collide = (self.tables is not None and c.nb_step_collisions > 0
                    and istep % c.nb_step_collisions == 0)
         # the production path deposits in the push on collision steps too and
@@ -234,9 +237,19 @@ class Simulation:
                                                         self.np_thread[s])
         with self._phase("rho"):
             F.reduce_species_density(n, self.bcnd, self.np_thread, self.ntype, self.np_red)
+        if self.diagnostics is not None:
+            self.diagnostics.after_step(istep)
 
-    def run(self, nsteps: int, start: int = 1):
+    def run(self, nsteps: int, start: int = 1, out=None):
+        """Steps start..start+nsteps-1 (mod_simulation.f90:475-494); with
+        diagnostics attached, the summary of print_diagnostics is written to
+        out (a text stream) before every step with mod(istep, nsav) == 1."""
+        d = self.diagnostics
         for istep in range(start, start + nsteps):
+            if d is not None and istep % d.nsav == 1:
+                text = d.print_diagnostics(istep)
+                if out is not None:
+                    out.write(text)
             self.advance_one_step(istep)
 
     # ------------------------------------------------------------------

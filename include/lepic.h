@@ -235,6 +235,23 @@ int lepic_collisions_deposited(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *
                                uint64_t philox_counter, double *Pcoll, lepic_mcc_stats *stats,
                                double *np_dep, const double *kq);
 
+/* ---------------- f3: plane moments ---------------- */
+/* compute_particle_plane_moments_species (mod_planeMoments.f90:13-308) for
+ * one species.  Each live particle, moved back half a step (x - v dt/2),
+ * adds bilinear weights to the xy plane at node iz_plane, the xz plane at
+ * iy_plane and the yz plane at ix_plane (1-based node indices in 1..n+1) when
+ * it lies within one cell of it.  data_xy, data_xz, data_yz (device) hold 5
+ * doubles per plane node, component fastest (np, T, u1, u2, u3), then the
+ * plane's first axis, over the ghosted 0..n+2 index space ((n1+3)(n2+3),
+ * (n1+3)(n3+3) and (n2+3)(n3+3) nodes).  They are updated as the reference
+ * updates them: where a node has weight, np and u1..u3 are overwritten by
+ * this call's count and (previous u + sum w v) / count, and T accumulates
+ * mass <v^2 - u^2> / (3 qe); elsewhere u1..u3 keep their value.  The sums go
+ * through FP64 atomics (summation order is the hardware's). */
+int lepic_plane_moments(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *p, double dt,
+                        double mass, int32_t ix_plane, int32_t iy_plane, int32_t iz_plane,
+                        double *data_xy, double *data_xz, double *data_yz);
+
 /* ---------------- fused particle phase ---------------- */
 /* One launch per species of gather + (Boris|leapfrog) push + wall BC +
  * stable compaction + deposit: the mover, apply_particle_bc_legacy and

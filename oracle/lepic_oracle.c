@@ -445,6 +445,90 @@ void lo_deposit(const lo_grid *g, const lo_pset *p, const double *kq, double Nm,
   }
 }
 
+/* One plane of compute_particle_plane_moments_species: the bilinear
+ * deposit of mod_planeMoments.f90:96-141 (and :146-191, :196-241) into cnt,
+ * data(u1..u3) and the <v^2> sums, nodes (a,b), (a+1,b), (a+1,b+1), (a,b+1). */
+static void plane_add(double *data, double *acc, int64_t stride, int32_t a, int32_t b,
+                      double pplane, double wa1, double wa2, double wb1, double wb2,
+                      double vx, double vy, double vz) {
+  const double w[4] = {pplane * wa1 * wb1, pplane * wa2 * wb1, pplane * wa2 * wb2,
+                       pplane * wa1 * wb2};
+  const int64_t nd[4] = {a + stride * b, (a + 1) + stride * b, (a + 1) + stride * (b + 1),
+                         a + stride * (b + 1)};
+  for (int q = 0; q < 4; ++q) acc[4 * nd[q] + 0] = acc[4 * nd[q] + 0] + w[q];
+  for (int q = 0; q < 4; ++q) data[5 * nd[q] + 2] = data[5 * nd[q] + 2] + w[q] * vx;
+  for (int q = 0; q < 4; ++q) data[5 * nd[q] + 3] = data[5 * nd[q] + 3] + w[q] * vy;
+  for (int q = 0; q < 4; ++q) data[5 * nd[q] + 4] = data[5 * nd[q] + 4] + w[q] * vz;
+  for (int q = 0; q < 4; ++q) acc[4 * nd[q] + 1] = acc[4 * nd[q] + 1] + w[q] * vx * vx;
+  for (int q = 0; q < 4; ++q) acc[4 * nd[q] + 2] = acc[4 * nd[q] + 2] + w[q] * vy * vy;
+  for (int q = 0; q < 4; ++q) acc[4 * nd[q] + 3] = acc[4 * nd[q] + 3] + w[q] * vz * vz;
+}
+
+/* The closing loops of mod_planeMoments.f90:246-302 over one plane. */
+static void plane_close(double *data, const double *acc, int64_t nodes, double mass) {
+  for (int64_t t = 0; t < nodes; ++t) {
+    const double c = acc[4 * t + 0];
+    if (c > 0.0) {
+      double *d = data + 5 * t;
+      d[0] = c;
+      d[2] = d[2] / c;
+      d[3] = d[3] / c;
+      d[4] = d[4] / c;
+      const double u2 = d[2] * d[2] + d[3] * d[3] + d[4] * d[4];
+      const double v2mean = (acc[4 * t + 1] + acc[4 * t + 2] + acc[4 * t + 3]) / c;
+      const double th = v2mean - u2 > 0.0 ? v2mean - u2 : 0.0;
+      d[1] = d[1] + mass * th / (3.0 * QE);
+    }
+  }
+}
+
+/* compute_particle_plane_moments_species (mod_planeMoments.f90:13-308). */
+void lo_plane_moments(const lo_grid *g, const lo_pset *p, double dt, double mass,
+                      int32_t ix_plane, int32_t iy_plane, int32_t iz_plane,
+                      double *data_xy, double *data_xz, double *data_yz) {
+  const int32_t *n = g->n;
+  const double *h = g->h;
+  const int64_t sx = n[0] + 3, sy = n[1] + 3, sz = n[2] + 3;
+  double *axy = (double *)calloc((size_t)(4 * sx * sy), sizeof(double));
+  double *axz = (double *)calloc((size_t)(4 * sx * sz), sizeof(double));
+  double *ayz = (double *)calloc((size_t)(4 * sy * sz), sizeof(double));
+  for (int64_t i = 0; i < p->n; ++i) {
+    if (p->dead[i] != 0) continue;
+    const double x = p->x[i] - p->vx[i] * dt * 0.5;
+    const double y = p->y[i] - p->vy[i] * dt * 0.5;
+    const double z = p->z[i] - p->vz[i] * dt * 0.5;
+    const double vx = p->vx[i], vy = p->vy[i], vz = p->vz[i];
+    const double qx = x / h[0], qy = y / h[1], qz = z / h[2];
+    /* int(q) + 1 in [1, n] exactly when q lies in (-1, n) (truncation) */
+    if (!(qx > -1.0 && qx < (double)n[0])) continue;
+    if (!(qy > -1.0 && qy < (double)n[1])) continue;
+    if (!(qz > -1.0 && qz < (double)n[2])) continue;
+    const int32_t ix = (int32_t)qx + 1, iy = (int32_t)qy + 1, iz = (int32_t)qz + 1;
+    double px = ((double)ix * h[0] - x) / h[0];
+    double py = ((double)iy * h[1] - y) / h[1];
+    double pz = ((double)iz * h[2] - z) / h[2];
+    px = px < 1.0 ? px : 1.0; px = px > 0.0 ? px : 0.0;
+    py = py < 1.0 ? py : 1.0; py = py > 0.0 ? py : 0.0;
+    pz = pz < 1.0 ? pz : 1.0; pz = pz > 0.0 ? pz : 0.0;
+    const double wx1 = px, wx2 = 1.0 - px, wy1 = py, wy2 = 1.0 - py, wz1 = pz, wz2 = 1.0 - pz;
+    if (iz == iz_plane || iz == iz_plane - 1)
+      plane_add(data_xy, axy, sx, ix, iy, iz == iz_plane ? pz : 1.0 - pz, wx1, wx2, wy1, wy2,
+                vx, vy, vz);
+    if (iy == iy_plane || iy == iy_plane - 1)
+      plane_add(data_xz, axz, sx, ix, iz, iy == iy_plane ? py : 1.0 - py, wx1, wx2, wz1, wz2,
+                vx, vy, vz);
+    if (ix == ix_plane || ix == ix_plane - 1)
+      plane_add(data_yz, ayz, sy, iy, iz, ix == ix_plane ? px : 1.0 - px, wy1, wy2, wz1, wz2,
+                vx, vy, vz);
+  }
+  plane_close(data_xy, axy, sx * sy, mass);
+  plane_close(data_xz, axz, sx * sz, mass);
+  plane_close(data_yz, ayz, sy * sz, mass);
+  free(axy);
+  free(axz);
+  free(ayz);
+}
+
 /* apply_periodic_density_bc (mod_density.f90:180-242), y then z stitching. */
 static void periodic_density(const lo_grid *g, const int32_t *bcnd, double *np) {
   const int32_t *n = g->n;

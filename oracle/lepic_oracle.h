@@ -112,6 +112,14 @@ int lo_pdesolver(const lo_grid *g, double *phi, const double *rho,
 void lo_efield(const lo_grid *g, const double *phi, const int32_t *bcnd,
                double *E4);
 
+/* ---- plane moments (modules/particles/mod_planeMoments.f90) ---- */
+/* data_xy: 5 doubles per node of the (n1+3)x(n2+3) plane, component fastest
+ * (np, T, u1, u2, u3), then x, then y; data_xz over (x, z), data_yz over
+ * (y, z).  Dead particles are skipped. */
+void lo_plane_moments(const lo_grid *g, const lo_pset *p, double dt, double mass,
+                      int32_t ix_plane, int32_t iy_plane, int32_t iz_plane,
+                      double *data_xy, double *data_xz, double *data_yz);
+
 /* ---- null-collision MCC (modules/particles/mod_collisions.f90) ---- */
 #define LO_MAX_COL 128
 typedef struct {

@@ -103,6 +103,8 @@ def lib():
                                    C.c_int32, C.c_double, C.c_double, C.c_int32,
                                    C.POINTER(MGStats)]
         L.lo_efield.argtypes = [C.POINTER(Grid), dp, ip, dp]
+        L.lo_plane_moments.argtypes = [C.POINTER(Grid), C.POINTER(PSet), C.c_double, C.c_double,
+                                       C.c_int32, C.c_int32, C.c_int32, dp, dp, dp]
         L.lo_collisions.restype = C.c_int
         L.lo_collisions.argtypes = [C.POINTER(Grid), C.POINTER(PSet), dp, C.POINTER(Chem),
                                     C.c_int32, C.c_double, ip, dp, C.POINTER(MCCStats)]
@@ -275,6 +277,16 @@ def efield(n, h, phi, bcnd):
     E4 = np.zeros(phi.shape + (4,))
     lib().lo_efield(C.byref(g), _p(phi), _p(bcnd, C.c_int32), _p(E4))
     return E4
+
+
+def plane_moments(n, h, p, dt, mass, ix_plane, iy_plane, iz_plane, data_xy, data_xz, data_yz):
+    """compute_particle_plane_moments_species (mod_planeMoments.f90:13-308);
+    data_* are updated in place ((n2+3, n1+3, 5), (n3+3, n1+3, 5), (n3+3, n2+3, 5))."""
+    g = grid(n, h)
+    for d in (data_xy, data_xz, data_yz):
+        assert d.flags.c_contiguous and d.dtype == np.float64
+    lib().lo_plane_moments(C.byref(g), C.byref(p.s), float(dt), float(mass), int(ix_plane),
+                           int(iy_plane), int(iz_plane), _p(data_xy), _p(data_xz), _p(data_yz))
 
 
 class ChemTables:

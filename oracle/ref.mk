@@ -11,7 +11,8 @@ SRCS = $(M)/core/mod_constants.f90 $(M)/core/mod_RNG.f90 $(M)/particles/mod_part
        $(M)/particles/mod_particle_sorting.f90 $(M)/particles/mod_particleMover.f90 \
        $(M)/particles/mod_particleBC.f90 $(M)/fields/mod_chargeDeposition.f90 \
        $(M)/fields/mod_charge_weights.f90 $(M)/fields/mod_electricField.f90 \
-       $(M)/config/mod_config.f90 $(M)/geometry/mod_domain.f90 $(M)/fields/mod_magneticField.f90
+       $(M)/config/mod_config.f90 $(M)/geometry/mod_domain.f90 $(M)/fields/mod_magneticField.f90 \
+       $(M)/io/mod_output_2d.f90
 OBJS = $(patsubst %.f90,$(OUT)/%.o,$(notdir $(SRCS)))
 
 all: $(OUT)/libref_lepic.so

@@ -19,6 +19,8 @@ module ref_bridge
   use mod_config, only: Config
   use mod_domain, only: Domain
   use mod_magneticField, only: MagneticField
+  use mod_output_2d, only: write_plane_xy_scalar_2d, write_plane_xz_scalar_2d, &
+                           write_plane_yz_scalar_2d
   implicit none
 contains
 
@@ -193,5 +195,46 @@ contains
     flag_thr = mf%flag_thr
     call mf%destroy()
   end subroutine ref_build_bfield
+
+  ! mod_output_2d's plane writers: kind 1 xy (f over x, y), 2 xz, 3 yz.
+  subroutine ref_write_plane_2d(kind, fname, nlen, f, n, every) bind(C, name="ref_write_plane_2d")
+    integer(c_int32_t), value :: kind, nlen, every
+    character(kind=c_char), intent(in) :: fname(nlen)
+    real(c_double), intent(in) :: f(*)
+    integer(c_int32_t), intent(in) :: n(3)
+    character(len=nlen) :: name
+    integer :: i
+    do i = 1, nlen
+      name(i:i) = fname(i)
+    end do
+    select case (kind)
+    case (1)
+      call write_plane_xy_scalar_2d(name, f, n, every)
+    case (2)
+      call write_plane_xz_scalar_2d(name, f, n, every)
+    case default
+      call write_plane_yz_scalar_2d(name, f, n, every)
+    end select
+  end subroutine ref_write_plane_2d
+
+  ! An ESw.d or Fw.d edit of v into buf (kind 1 ES, 2 F), blank-padded.
+  subroutine ref_format(kind, v, w, d, buf, blen) bind(C, name="ref_format")
+    integer(c_int32_t), value :: kind, w, d, blen
+    real(c_double), value :: v
+    character(kind=c_char), intent(out) :: buf(blen)
+    character(len=32) :: fmt
+    character(len=64) :: s
+    integer :: i
+    if (kind == 1) then
+      write(fmt, '(a,i0,a,i0,a)') '(ES', w, '.', d, ')'
+    else
+      write(fmt, '(a,i0,a,i0,a)') '(F', w, '.', d, ')'
+    end if
+    s = ' '
+    write(s, fmt) v
+    do i = 1, blen
+      buf(i) = s(i:i)
+    end do
+  end subroutine ref_format
 
 end module ref_bridge
