@@ -29,7 +29,7 @@ EXPORTS = (
     "lepic_reduce_density", "lepic_build_rho", "lepic_poisson_solve", "lepic_efield",
     "lepic_collisions", "lepic_collisions_deposited", "lepic_push_bc_deposit",
     "lepic_push_bc_deposit_inplace", "lepic_sort_compact", "lepic_bin_compact",
-    "lepic_plane_moments",
+    "lepic_plane_moments", "lepic_electron_heating",
 )
 
 
@@ -56,6 +56,15 @@ class BCParams(C.Structure):
                 ("ptype", C.c_int32), ("tag_neg", C.c_int32),
                 ("qmacro", C.c_double), ("mass", C.c_double), ("Nm", C.c_double),
                 ("ngrid", C.c_int32)]
+
+
+class Heating(C.Structure):
+    _fields_ = [("nudt", C.c_double), ("Pabs", C.c_double), ("nu_h", C.c_double),
+                ("xl_pow", C.c_double), ("xr_pow", C.c_double),
+                ("flag_circxh", C.c_int32), ("flag_ahp", C.c_int32),
+                ("R_ahp_moments", C.c_double), ("R_ahp", C.c_double),
+                ("ymax", C.c_double), ("zmax", C.c_double), ("Nm", C.c_double),
+                ("mass", C.c_double), ("vt0", C.c_double)]
 
 
 class BField(C.Structure):
@@ -152,6 +161,8 @@ def load() -> C.CDLL:
             "lepic_bin_compact": (C.c_int, [vp, G, PS, PS]),
             "lepic_plane_moments": (C.c_int, [vp, G, PS, C.c_double, C.c_double, C.c_int32,
                                               C.c_int32, C.c_int32, vp, vp, vp]),
+            "lepic_electron_heating": (C.c_int, [vp, G, PS, C.POINTER(Heating), C.c_int32, i32p,
+                                                 C.c_uint64, C.c_uint64, vp, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

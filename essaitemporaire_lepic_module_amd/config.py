@@ -110,6 +110,7 @@ class Case:
     phi_init: np.ndarray | None = None   # initial potential (a deck's generate_boundary u)
     ni0: list | None = None              # loader acceptance n/n0 per tracked species
     x_load: float = 0.0                  # loading region x < x_load (m); <= 0: whole box
+    heating: dict | None = None          # electron heating (a deck's; None: no heating)
 
     @property
     def tracked(self):

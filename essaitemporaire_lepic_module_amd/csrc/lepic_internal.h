@@ -72,6 +72,7 @@ enum : int {
   CTRL_COUNT_OUT = 8,      // compaction output count
   CTRL_SPIN_FAIL = 16,     // set when a bounded wait gives up
   CTRL_MCC = 64,           // MCC counters (256 words)
+  CTRL_HEAT = 384,         // electron heating: count, energy sum, vt, seed (8 words)
   CTRL_MG = 512,           // MG per-level control (12 levels x 16 words)
   CTRL_PARTIALS = 1024,    // reduction partials (3072 words)
   CTRL_WORDS = 4096

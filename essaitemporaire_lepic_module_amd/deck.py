@@ -884,8 +884,6 @@ class Deck:
         c, b, out = self.conditions, self.boundary, []
         if b.flag_die:
             out.append("dielectric walls")
-        if c.flag_heat and c.Pabs >= 0.0:
-            out.append("electron heating")
         if c.flag_inj:
             out.append("particle injection")
         if c.gam_sec > 0.0:
@@ -933,6 +931,16 @@ class Deck:
         case.phi_init = b.u
         case.ni0 = [chem.density[k] for k, s in enumerate(species) if s.tracked][:len(tracked)]
         case.x_load = c.x_load
+        # the modular driver heats every nb_step_heating steps whatever the
+        # deck says (mod_simulation.f90:441-447); nu_h < 0 gives nudt < 0,
+        # so nothing is re-drawn but the window's draws are still taken.
+        # The heating reads Config's flag_circxh, which keeps its default 0:
+        # build_boundary copies generate_boundary's other outputs to the
+        # domain but not this one (mod_boundary.f90:114-131).
+        case.heating = dict(
+            nb_step_heating=self.params.nb_step_heating, nudt=self.params.nudt, Pabs=c.Pabs,
+            nu_h=c.nu_h, xl_pow=c.xl_pow, xr_pow=c.xr_pow, flag_circxh=0,
+            flag_ahp=c.flag_ahp, R_ahp=c.yr_pow - b.ymax / 2.0, ymax=b.ymax, zmax=b.zmax)
         if chem.reactions and c.ngas > 0.0:
             case.chemistry = chm.build_tables(chem, c.ngas)
         return case

@@ -284,8 +284,9 @@ class Diagnostics:
     def power_diagnostics(self) -> dict:
         """The reductions of print_diagnostics over the accumulators since
         the last call: wall power and current (P_loss, p_mac, from the wall
-        BC), collision power (Pcoll, from MCC); heating and injection are not
-        part of this step and read 0.  Resets the accumulators."""
+        BC), absorbed heating power (P_heat), collision power (Pcoll, from
+        MCC); injection is not part of this step and reads 0.  Resets the
+        accumulators."""
         s = self.sim
         span = np.float64(float(self.nsav - 1) * s.case.dt)   # 0 when nsav = 1, as there
         P_loss = _host(s.P_loss)
@@ -294,14 +295,18 @@ class Diagnostics:
         q1 = _fsum(pm[0, :, 0])
         qi = _fsum(pm[1:, :, 0].reshape(-1))
         absi = _fsum(np.abs(pm[1:, :, 0]).reshape(-1))
+        P_heat = _host(s.P_heat) if getattr(s, "P_heat", None) is not None else np.zeros(1)
         with np.errstate(divide="ignore", invalid="ignore"):
-            out = {"Pwall": float(-np.float64(_fsum(P_loss)) / span), "Pabs": 0.0,
+            out = {"Pwall": float(-np.float64(_fsum(P_loss)) / span),
+                   "Pabs": float(np.float64(_fsum(P_heat)) / span),
                    "Pcoll": float(np.float64(_fsum(Pcoll)) / span), "Pinj": 0.0,
                    "Iw1": float(np.float64(q1) / span), "Iw2": float(np.float64(qi) / span)}
         out["Ekw1"] = _fsum(pm[0, :, 1]) / abs(q1) if abs(q1) > 0.0 else 0.0
         out["Ekw2"] = _fsum(pm[1:, :, 1].reshape(-1)) / absi if absi > 0.0 else 0.0
         s.P_loss.zero_()
         s.p_mac.zero_()
+        if getattr(s, "P_heat", None) is not None:
+            s.P_heat.zero_()
         if getattr(s, "tables", None) is not None:
             s.Pcoll.zero_()
         self.last = out

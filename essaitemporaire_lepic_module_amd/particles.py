@@ -339,6 +339,47 @@ def perform_collisions_step(parts: list, n, h, np_red: torch.Tensor, tables: Rea
 
 
 # ---------------------------------------------------------------------------
+# f4: electron heating
+# ---------------------------------------------------------------------------
+
+def heating_params(nudt: float, Pabs: float, nu_h: float, xl_pow: float, xr_pow: float,
+                   ymax: float, zmax: float, Nm: float, mass: float, vt0: float,
+                   flag_circxh: int = 0, flag_ahp: int = 0, R_ahp_moments: float = 0.0,
+                   R_ahp: float = 0.0) -> _native.Heating:
+    hp = _native.Heating()
+    hp.nudt, hp.Pabs, hp.nu_h = float(nudt), float(Pabs), float(nu_h)
+    hp.xl_pow, hp.xr_pow = float(xl_pow), float(xr_pow)
+    hp.flag_circxh, hp.flag_ahp = int(flag_circxh), int(flag_ahp)
+    hp.R_ahp_moments, hp.R_ahp = float(R_ahp_moments), float(R_ahp)
+    hp.ymax, hp.zmax, hp.Nm, hp.mass, hp.vt0 = (float(v) for v in (ymax, zmax, Nm, mass, vt0))
+    return hp
+
+
+def apply_electron_heating(part: ParticleSet, n, h, hp: _native.Heating, iseed: list | None,
+                           P_heat: torch.Tensor, mode: int = _native.MCC_REPLAY_RAN2,
+                           philox_key: int = 0, philox_counter: int = 0,
+                           stats: bool = True) -> dict | None:
+    """compute_heating_region_moments, update_heating_vt and
+    apply_electron_heating (mod_state.f90:341-373, 524-612; mod_heating.f90:12-82)
+    on the electrons.  MCC_REPLAY_RAN2 consumes the reference's ran2 stream
+    iseed[0] (updated in place); MCC_PHILOX draws per particle.  P_heat
+    (device, 1 double) accumulates the absorbed energy.  Returns the window
+    count, its kinetic energy and the vt used (stats=False: None, and in
+    MCC_PHILOX mode the call then does not wait for the device)."""
+    ctx = context(part.device)
+    seed = C.c_int32(int(iseed[0]) if iseed else 0)
+    out = (C.c_double * 3)() if stats else None
+    ps = part.pset()
+    ctx.check(ctx.lib.lepic_electron_heating(ctx.h, C.byref(_native.grid(n, h)), C.byref(ps),
+                                             C.byref(hp), int(mode), C.byref(seed),
+                                             C.c_uint64(philox_key), C.c_uint64(philox_counter),
+                                             ptr(P_heat), out))
+    if iseed is not None and mode == _native.MCC_REPLAY_RAN2:
+        iseed[0] = seed.value
+    return {"Nh": int(out[0]), "sum_Ek": out[1], "vt": out[2]} if stats else None
+
+
+# ---------------------------------------------------------------------------
 # a11/a13: RNG and loaders
 # ---------------------------------------------------------------------------
 

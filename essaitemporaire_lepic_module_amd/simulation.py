@@ -116,6 +116,11 @@ class Simulation:
         self.timers = None   # {phase: [(start_event, end_event, units), ...]} when timing
         self.timed_phases = None   # phases to time (None: all of them)
         self.diagnostics = None    # diagnostics.Diagnostics when attached
+        # electron heating: absorbed energy (the reference's P_loss(2)) and the
+        # R_ahp the window moments see, which the first heating sets
+        self.P_heat = torch.zeros(1, dtype=torch.float64, device=dev)
+        self.R_ahp_seen = 0.0
+        self.last_heating = None
 
     @contextlib.contextmanager
     def _phase(self, name: str, units: int = 0):
@@ -229,6 +234,10 @@ class Simulation:
                     self.iseed, self.Pcoll, mode=self.mcc_mode, philox_key=self.philox_key,
                     philox_counter=istep, np_dep=self.np_thread if dep_first else None,
                     kq=self.kq if dep_first else None)
+        hc = c.heating
+        if hc is not None and hc["nb_step_heating"] > 0 and istep % hc["nb_step_heating"] == 0:
+            with self._phase("heating"):
+                self._heat(istep)
         if (collide and not dep_first) or not self.fused:
             with self._phase("deposit"):
                 F.clear_np_thread(self.np_thread)
@@ -239,6 +248,22 @@ class Simulation:
             F.reduce_species_density(n, self.bcnd, self.np_thread, self.ntype, self.np_red)
         if self.diagnostics is not None:
             self.diagnostics.after_step(istep)
+
+    def _heat(self, istep: int):
+        """compute_heating_region_moments, update_heating_vt and
+        apply_electron_heating_local (mod_simulation.f90:441-447) on the
+        electrons; velocities only, so the step's deposit stands."""
+        hc = self.case.heating
+        R_app = hc["R_ahp"] if hc["flag_circxh"] == 1 else self.R_ahp_seen
+        hp = P.heating_params(hc["nudt"], hc["Pabs"], hc["nu_h"], hc["xl_pow"], hc["xr_pow"],
+                              hc["ymax"], hc["zmax"], self.Nm[0], self.mass[0],
+                              self.case.vt0[0], hc["flag_circxh"], hc["flag_ahp"],
+                              self.R_ahp_seen, R_app)
+        replay = self.mcc_mode == _native.MCC_REPLAY_RAN2
+        self.last_heating = P.apply_electron_heating(
+            self.parts[0], self.n, self.h, hp, self.iseed if replay else None, self.P_heat,
+            self.mcc_mode, self.philox_key, istep, stats=replay)
+        self.R_ahp_seen = R_app
 
     def run(self, nsteps: int, start: int = 1, out=None):
         """Steps start..start+nsteps-1 (mod_simulation.f90:475-494); with

@@ -235,6 +235,37 @@ int lepic_collisions_deposited(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *
                                uint64_t philox_counter, double *Pcoll, lepic_mcc_stats *stats,
                                double *np_dep, const double *kq);
 
+/* ---------------- f4: electron heating ---------------- */
+/* Parameters of the modular step's electron heating (mod_state.f90:341-373,
+ * 524-612; mod_heating.f90:12-82). */
+typedef struct {
+  double nudt;             /* nu_h * nb_step_heating * dt (mod_simParams.f90:150)  */
+  double Pabs, nu_h;       /* absorbed power (W), heating frequency (conditions)   */
+  double xl_pow, xr_pow;   /* heating window along x (m)                           */
+  int32_t flag_circxh, flag_ahp;
+  double R_ahp_moments;    /* R_ahp the window moments see (0 before the first
+                              heating sets it, mod_state.f90:350)                 */
+  double R_ahp;            /* R_ahp the heating sees                               */
+  double ymax, zmax;       /* domain extents (m)                                   */
+  double Nm, mass;         /* electron macro-particle weight and mass              */
+  double vt0;              /* vt used when the window is empty or Te <= 0          */
+} lepic_heating;
+
+/* compute_heating_region_moments + update_heating_vt + apply_electron_heating
+ * on the electron set p (device): Te from the window's kinetic energy and
+ * Pabs/nu_h, then each electron in the window re-drawn from a Maxwellian of
+ * that temperature with probability nudt.  mode LEPIC_MCC_REPLAY_RAN2: the
+ * reference's ran2 chain on one device thread, iseed (host) read and
+ * advanced, every particle but holes (dead = 2) visited in order;
+ * LEPIC_MCC_PHILOX: live particles only, each with its own draws from
+ * (philox_key, philox_counter, index).  P_heat (device, 1 double)
+ * accumulates 0.5 Nm m (v^2 new - v^2 old) (the reference's P_loss(2)).
+ * out (host, optional): window count, window kinetic energy, vt used. */
+int lepic_electron_heating(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *p,
+                           const lepic_heating *hp, int32_t mode, int32_t *iseed,
+                           uint64_t philox_key, uint64_t philox_counter, double *P_heat,
+                           double *out);
+
 /* ---------------- f3: plane moments ---------------- */
 /* compute_particle_plane_moments_species (mod_planeMoments.f90:13-308) for
  * one species.  Each live particle, moved back half a step (x - v dt/2),

@@ -445,6 +445,71 @@ void lo_deposit(const lo_grid *g, const lo_pset *p, const double *kq, double Nm,
   }
 }
 
+/* load_gauss_local (mod_heating.f90:85-97). */
+static void heat_gauss(double *v1, double *v2, double vt, double r1, double r2) {
+  const double vp = vt * sqrt(-log(1.0 - r1));
+  const double theta = 2.0 * PI_F * r2;
+  *v1 = vp * cos(theta);
+  *v2 = vp * sin(theta);
+}
+
+static int heat_circle_skip(const lo_heating *k, double R, double y, double z) {
+  if (k->flag_circxh != 1) return 0;
+  const double dy = y - k->ymax / 2.0, dz = z - k->zmax / 2.0;
+  const double r2 = dy * dy + dz * dz;
+  return k->flag_ahp == 0 ? r2 > R * R : r2 < R * R;
+}
+
+/* compute_heating_region_moments (mod_state.f90:524-573), update_heating_vt
+ * (:576-612) and apply_electron_heating (mod_heating.f90:12-82). */
+void lo_electron_heating(const lo_grid *g, lo_pset *p, const lo_heating *k,
+                         int32_t *iseed, double *P_heat, double *out) {
+  const double h1 = g->h[0];
+  int32_t Nh = 0;
+  double sum_ek = 0.0;
+  for (int64_t i = 0; i < p->n; ++i) {
+    const int32_t ix = (int32_t)(p->x[i] / h1) + 1;
+    if ((double)ix < k->xl_pow / h1 || (double)ix > k->xr_pow / h1) continue;
+    if (heat_circle_skip(k, k->R_ahp_moments, p->y[i], p->z[i])) continue;
+    const double v2 = p->vx[i] * p->vx[i] + p->vy[i] * p->vy[i] + p->vz[i] * p->vz[i];
+    sum_ek = sum_ek + 0.5 * k->Nm * k->mass * v2;
+    Nh = Nh + 1;
+  }
+  double vt = k->vt0;
+  if (Nh > 0) {
+    const double Te = (2.0 / 3.0) * (sum_ek + k->Pabs / k->nu_h) / (k->Nm * QE * (double)Nh);
+    if (Te > 0.0) vt = sqrt(2.0 * QE * Te / fabs(k->mass));
+  }
+  if (out) {
+    out[0] = (double)Nh;
+    out[1] = sum_ek;
+    out[2] = vt;
+  }
+  const int32_t ixl = (int32_t)(k->xl_pow / h1) + 1, ixr = (int32_t)(k->xr_pow / h1) + 1;
+  double vz_sav = 0.0;
+  for (int64_t i = 0; i < p->n; ++i) {
+    const int32_t ix = (int32_t)(p->x[i] / h1) + 1;
+    if (ix < ixl || ix > ixr) continue;
+    if (heat_circle_skip(k, k->R_ahp, p->y[i], p->z[i])) continue;
+    double r1 = lo_ran2(iseed);
+    if (r1 > k->nudt) continue;
+    const double v2old = p->vx[i] * p->vx[i] + p->vy[i] * p->vy[i] + p->vz[i] * p->vz[i];
+    r1 = lo_ran2(iseed);
+    double r2 = lo_ran2(iseed);
+    heat_gauss(&p->vx[i], &p->vy[i], vt, r1, r2);
+    if (vz_sav == 0.0) {
+      r1 = lo_ran2(iseed);
+      r2 = lo_ran2(iseed);
+      heat_gauss(&p->vz[i], &vz_sav, vt, r1, r2);
+    } else {
+      p->vz[i] = vz_sav;
+      vz_sav = 0.0;
+    }
+    *P_heat = *P_heat + 0.5 * k->Nm * k->mass *
+              (p->vx[i] * p->vx[i] + p->vy[i] * p->vy[i] + p->vz[i] * p->vz[i] - v2old);
+  }
+}
+
 /* One plane of compute_particle_plane_moments_species: the bilinear
  * deposit of mod_planeMoments.f90:96-141 (and :146-191, :196-241) into cnt,
  * data(u1..u3) and the <v^2> sums, nodes (a,b), (a+1,b), (a+1,b+1), (a,b+1). */

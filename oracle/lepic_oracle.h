@@ -112,6 +112,18 @@ int lo_pdesolver(const lo_grid *g, double *phi, const double *rho,
 void lo_efield(const lo_grid *g, const double *phi, const int32_t *bcnd,
                double *E4);
 
+/* ---- electron heating (mod_state.f90:341-373, 524-612; mod_heating.f90) ---- */
+typedef struct {
+  double nudt, Pabs, nu_h, xl_pow, xr_pow;
+  int32_t flag_circxh, flag_ahp;
+  double R_ahp_moments, R_ahp, ymax, zmax, Nm, mass, vt0;
+} lo_heating;
+
+/* Window moments, vt, then the ran2 heating chain over p (all n particles,
+ * as the reference's loops); *P_heat accumulates; out[3]: count, energy, vt. */
+void lo_electron_heating(const lo_grid *g, lo_pset *p, const lo_heating *hp,
+                         int32_t *iseed, double *P_heat, double *out);
+
 /* ---- plane moments (modules/particles/mod_planeMoments.f90) ---- */
 /* data_xy: 5 doubles per node of the (n1+3)x(n2+3) plane, component fastest
  * (np, T, u1, u2, u3), then x, then y; data_xz over (x, z), data_yz over

@@ -59,6 +59,15 @@ class Chem(C.Structure):
     ]
 
 
+class Heating(C.Structure):
+    _fields_ = [("nudt", C.c_double), ("Pabs", C.c_double), ("nu_h", C.c_double),
+                ("xl_pow", C.c_double), ("xr_pow", C.c_double),
+                ("flag_circxh", C.c_int32), ("flag_ahp", C.c_int32),
+                ("R_ahp_moments", C.c_double), ("R_ahp", C.c_double),
+                ("ymax", C.c_double), ("zmax", C.c_double), ("Nm", C.c_double),
+                ("mass", C.c_double), ("vt0", C.c_double)]
+
+
 class MCCStats(C.Structure):
     _fields_ = [("Nc", C.c_int32 * 16), ("accepted", C.c_int64 * 16),
                 ("err_coll", C.c_int64 * 16), ("skipped_dead", C.c_int64 * 16),
@@ -103,6 +112,8 @@ def lib():
                                    C.c_int32, C.c_double, C.c_double, C.c_int32,
                                    C.POINTER(MGStats)]
         L.lo_efield.argtypes = [C.POINTER(Grid), dp, ip, dp]
+        L.lo_electron_heating.argtypes = [C.POINTER(Grid), C.POINTER(PSet), C.POINTER(Heating),
+                                          ip, dp, dp]
         L.lo_plane_moments.argtypes = [C.POINTER(Grid), C.POINTER(PSet), C.c_double, C.c_double,
                                        C.c_int32, C.c_int32, C.c_int32, dp, dp, dp]
         L.lo_collisions.restype = C.c_int
@@ -279,6 +290,19 @@ def efield(n, h, phi, bcnd):
     return E4
 
 
+def electron_heating(n, h, p, hp: dict, seed_value: int, P_heat: float = 0.0):
+    """Heating of the electron set p (host arrays, updated in place); hp holds
+    the lo_heating fields.  Returns (seed, P_heat, (count, energy, vt))."""
+    g = grid(n, h)
+    k = Heating(**{f: hp[f] for f, _ in Heating._fields_})
+    seed = np.array([seed_value], np.int32)
+    P = np.array([P_heat])
+    out = np.zeros(3)
+    lib().lo_electron_heating(C.byref(g), C.byref(p.s), C.byref(k), _p(seed, C.c_int32), _p(P),
+                              _p(out))
+    return int(seed[0]), float(P[0]), (int(out[0]), float(out[1]), float(out[2]))
+
+
 def plane_moments(n, h, p, dt, mass, ix_plane, iy_plane, iz_plane, data_xy, data_xz, data_yz):
     """compute_particle_plane_moments_species (mod_planeMoments.f90:13-308);
     data_* are updated in place ((n2+3, n1+3, 5), (n3+3, n1+3, 5), (n3+3, n2+3, 5))."""
@@ -363,6 +387,9 @@ class OracleSim:
         self.last_mg = None
         self.lost = [0] * self.ntype
         self.last_mcc = None
+        self.P_heat = 0.0
+        self.R_ahp_seen = 0.0
+        self.last_heating = None
 
     def advance_one_step(self, istep):
         c, n, h = self.c, self.n, self.h
@@ -397,6 +424,17 @@ class OracleSim:
             if st != 0:
                 raise RuntimeError(f"collisions status {st}")
             self.Pcoll += Pc
+        hc = c.heating
+        if hc is not None and hc["nb_step_heating"] > 0 and istep % hc["nb_step_heating"] == 0:
+            R_app = hc["R_ahp"] if hc["flag_circxh"] == 1 else self.R_ahp_seen
+            hp = dict(nudt=hc["nudt"], Pabs=hc["Pabs"], nu_h=hc["nu_h"], xl_pow=hc["xl_pow"],
+                      xr_pow=hc["xr_pow"], flag_circxh=hc["flag_circxh"],
+                      flag_ahp=hc["flag_ahp"], R_ahp_moments=self.R_ahp_seen, R_ahp=R_app,
+                      ymax=hc["ymax"], zmax=hc["zmax"], Nm=self.Nm[0], mass=self.mass[0],
+                      vt0=c.vt0[0])
+            self.seed, self.P_heat, self.last_heating = electron_heating(
+                n, h, self.parts[0], hp, self.seed, self.P_heat)
+            self.R_ahp_seen = R_app
         self.np_thread = np.zeros_like(self.np_thread)
         for s, p in enumerate(self.parts):
             deposit(n, h, p, self.kq, self.Nm[s], self.np_thread[s])

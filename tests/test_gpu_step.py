@@ -167,8 +167,9 @@ def test_deck_chemistry_production_steps(gpu):
 
 def test_deck_case_steps_match_oracle(gpu, oracle, tmp_path):
     """A run deck in the reference's format -- periodic y-z walls (label 0), a
-    biased electrode, a Gaussian Bz map on the field grid, argon chemistry --
-    drives the same steps on the GPU and in the oracle."""
+    biased electrode, a Gaussian Bz map on the field grid, argon chemistry,
+    electron heating in a window every 4 steps -- drives the same steps on
+    the GPU and in the oracle (MCC and heating share the ran2 stream)."""
     from essaitemporaire_lepic_module_amd import chemistry as CH
     from essaitemporaire_lepic_module_amd import deck as D
     from essaitemporaire_lepic_module_amd import particles as P
@@ -180,8 +181,8 @@ def test_deck_case_steps_match_oracle(gpu, oracle, tmp_path):
     (tmp_path / "chem.dat").write_text(CH.write_reactions(sp, [1.0, 1.0, 1.0], reactions))
     n = (16, 8, 8)
     c = D.Conditions(rname="chem.dat", Te=3.0, ng=3, eps=1e-8, omega=1.4, n=n, kt=0.3, n0=1e15,
-                     ngas=3e20, np_cell=6, ngrid=2, nsav=100, cnt_plt=1, Pabs=-1.0, nu_h=-1.0,
-                     I_inj=-1.0, nB=1, flag_B=1, n_B=n)
+                     ngas=3e20, np_cell=6, ngrid=2, nsav=100, cnt_plt=1, Pabs=0.05, nu_h=1e8,
+                     xl_pow=2e-2, xr_pow=7e-2, I_inj=-1.0, nB=1, flag_B=1, n_B=n)
     c.B[0] = D.BSource("n", "gauss", 1.0, 0.02, "Bz", 3.0e-2, 5.0e-2)
     (tmp_path / "conditions.inp").write_text(D.write_conditions(c))
     (tmp_path / "geometry.inp").write_text(
@@ -199,4 +200,7 @@ def test_deck_case_steps_match_oracle(gpu, oracle, tmp_path):
         if istep % 2 == 0:
             for k in ("Nc", "accepted", "err_coll", "skipped_dead", "appended"):
                 assert sim.last_mcc[k] == ref.last_mcc[k][:len(sim.last_mcc[k])], (istep, k)
+        if istep == 4:
+            assert sim.last_heating["Nh"] == ref.last_heating[0] > 0
+            assert float(sim.P_heat[0]) == pytest.approx(ref.P_heat, rel=1e-12)
     assert sim.iseed[0] == ref.seed
