@@ -29,7 +29,7 @@ EXPORTS = (
     "lepic_reduce_density", "lepic_build_rho", "lepic_poisson_solve", "lepic_efield",
     "lepic_collisions", "lepic_collisions_deposited", "lepic_push_bc_deposit",
     "lepic_push_bc_deposit_inplace", "lepic_sort_compact", "lepic_bin_compact",
-    "lepic_plane_moments", "lepic_electron_heating",
+    "lepic_plane_moments", "lepic_electron_heating", "lepic_dielectric_phi",
 )
 
 
@@ -55,7 +55,8 @@ class BCParams(C.Structure):
                 ("flag_pbc", C.c_int32), ("flag_nmn", C.c_int32),
                 ("ptype", C.c_int32), ("tag_neg", C.c_int32),
                 ("qmacro", C.c_double), ("mass", C.c_double), ("Nm", C.c_double),
-                ("ngrid", C.c_int32)]
+                ("ngrid", C.c_int32), ("flag_die", C.c_int32), ("dtype", C.c_void_p),
+                ("sum_q_xz", C.c_void_p), ("sum_q_yz", C.c_void_p)]
 
 
 class Heating(C.Structure):
@@ -163,6 +164,8 @@ def load() -> C.CDLL:
                                               C.c_int32, C.c_int32, vp, vp, vp]),
             "lepic_electron_heating": (C.c_int, [vp, G, PS, C.POINTER(Heating), C.c_int32, i32p,
                                                  C.c_uint64, C.c_uint64, vp, dp]),
+            "lepic_dielectric_phi": (C.c_int, [vp, G, vp, vp, C.c_int32, C.c_int32, vp, vp,
+                                               C.c_double, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

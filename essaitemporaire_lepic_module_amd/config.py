@@ -111,6 +111,9 @@ class Case:
     ni0: list | None = None              # loader acceptance n/n0 per tracked species
     x_load: float = 0.0                  # loading region x < x_load (m); <= 0: whole box
     heating: dict | None = None          # electron heating (a deck's; None: no heating)
+    flag_die: int = 0                    # dielectric surface charging
+    dtype: np.ndarray | None = None      # wall type per label 0..ngrid (flag_die)
+    Ca: float = 0.0                      # dielectric capacitance per area (F/m^2)
 
     @property
     def tracked(self):

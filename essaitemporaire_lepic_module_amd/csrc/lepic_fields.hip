@@ -220,6 +220,35 @@ __global__ void k_e_walls(KGrid g, const double *__restrict__ phi,
   if (any) E[c] = e;
 }
 
+// apply_dielectric_bc_to_phi (mod_state.f90:378-430): on every node of a
+// label whose type is a dielectric, phi takes each species' collected
+// surface charge, in species order: the x-z sum of its column over
+// 2 Ca (type 2), the y-z sums of the first or second plane over Ca (3, 4).
+__global__ void k_die_phi(KGrid g, const int32_t *__restrict__ bcnd,
+                          const int32_t *__restrict__ dtype, int ngrid, int ntype, int64_t sxz_n,
+                          int64_t syz_n, const double *__restrict__ sxz,
+                          const double *__restrict__ syz, double Ca, int64_t nodes, double *phi) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= nodes) return;
+  const int igrid = bcnd[t];
+  if (igrid <= 0 || igrid > ngrid) return;   // labels above the table are not walls of it
+  const int dt = dtype[igrid];
+  if (dt < 2 || dt > 4) return;
+  int i, j, k;
+  node_ijk(g, t, i, j, k);
+  double v = phi[t];
+  for (int s = 0; s < ntype; ++s) {
+    // qsum = 0 + the one thread's sum (mod_state.f90:403-406), then / Ca
+    if (dt == 2) {
+      v = v + (0.0 + sxz[s * sxz_n + i + (int64_t)(g.n1 + 3) * k]) / (2.0 * Ca);
+    } else {
+      const int c = dt - 3;
+      v = v + (0.0 + syz[s * syz_n + c + 2 * (j + (int64_t)(g.n2 + 3) * k)]) / Ca;
+    }
+  }
+  phi[t] = v;
+}
+
 }  // namespace
 
 extern "C" {
@@ -270,6 +299,22 @@ int lepic_build_rho(lepic_ctx *ctx, const lepic_grid *g, int32_t ntype, const do
   hipLaunchKernelGGL(k_rho, dim3(grid_blocks(nodes, BLOCK)), dim3(BLOCK), 0, ctx->stream,
                      make_kgrid(g), np_red, ch, ntype, nodes, rho);
   LEPIC_LAUNCH_CHECK(ctx, "k_rho");
+  return LEPIC_OK;
+}
+
+int lepic_dielectric_phi(lepic_ctx *ctx, const lepic_grid *g, const int32_t *bcnd,
+                         const int32_t *dtype, int32_t ngrid, int32_t ntype,
+                         const double *sum_q_xz, const double *sum_q_yz, double Ca_cell,
+                         double *phi) {
+  if (!ctx || !g || !bcnd || !dtype || !sum_q_xz || !sum_q_yz || !phi || ntype < 1 || ngrid < 0)
+    return LEPIC_ERR_ARG;
+  const int64_t nodes = grid_nodes(g);
+  const int64_t sxz_n = (int64_t)(g->n[0] + 3) * (g->n[2] + 3);
+  const int64_t syz_n = 2 * (int64_t)(g->n[1] + 3) * (g->n[2] + 3);
+  hipLaunchKernelGGL(k_die_phi, dim3(grid_blocks(nodes, BLOCK)), dim3(BLOCK), 0, ctx->stream,
+                     make_kgrid(g), bcnd, dtype, (int)ngrid, (int)ntype, sxz_n, syz_n, sum_q_xz, sum_q_yz,
+                     Ca_cell, nodes, phi);
+  LEPIC_LAUNCH_CHECK(ctx, "k_die_phi");
   return LEPIC_OK;
 }
 

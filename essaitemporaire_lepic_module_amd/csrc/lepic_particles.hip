@@ -45,6 +45,9 @@ struct KBC {
   int flag_pbc, flag_nmn, ptype, tag_neg;
   double qmacro, mass, Nm;
   int ngrid;
+  int flag_die;
+  const int32_t *dtype;
+  double *sum_q_xz, *sum_q_yz;
 };
 
 inline KBC kbc(const lepic_bc_params *b) {
@@ -53,6 +56,10 @@ inline KBC kbc(const lepic_bc_params *b) {
   k.flag_pbc = b->flag_pbc; k.flag_nmn = b->flag_nmn;
   k.ptype = b->ptype; k.tag_neg = b->tag_neg;
   k.qmacro = b->qmacro; k.mass = b->mass; k.Nm = b->Nm; k.ngrid = b->ngrid;
+  k.flag_die = b->flag_die;
+  k.dtype = b->dtype;
+  k.sum_q_xz = b->sum_q_xz;
+  k.sum_q_yz = b->sum_q_yz;
   return k;
 }
 
@@ -162,6 +169,52 @@ __device__ inline int wall_test_tab_q(const KGrid &g, const int *__restrict__ wt
     label = 0;
   }
   return lost;
+}
+
+// Surface charge of a particle lost on a dielectric
+// (mod_particleBC.f90:139-188): igrid is the lost particle's wall label
+// (> 0), x, y, z its position after the push.  The position is wrapped
+// along z for periodic runs before the charge is spread; the particle is
+// lost either way.  Called for lost particles only.
+__device__ inline void die_charge(const KGrid &g, const KBC &bp, int igrid, double xp, double yp,
+                                  double zp) {
+  if (igrid > bp.ngrid + 1) return;     // outside the dtype table
+  const int dt = bp.dtype[igrid - 1];   // the routine's 1-based dtype(:) dummy
+  if (dt <= 1) return;
+  int ix = (int)floor(qdiv(xp, g.q1)) + 1;
+  int iy = (int)floor(qdiv(yp, g.q2)) + 1;
+  int iz = (int)floor(qdiv(zp, g.q3)) + 1;
+  if (bp.flag_pbc == 1) {
+    if (zp >= bp.zmax) {
+      zp = zp - bp.zmax;
+      iz = (int)floor(qdiv(zp, g.q3)) + 1;
+    }
+    if (zp <= 0.0) {
+      zp = bp.zmax + zp;
+      iz = (int)floor(qdiv(zp, g.q3)) + 1;
+    }
+  }
+  ix = imax(0, imin(g.n1 + 1, ix));
+  iy = imax(0, imin(g.n2 + 1, iy));
+  iz = imax(0, imin(g.n3 + 1, iz));
+  const double pz = qdiv((double)iz * g.h3 - zp, g.q3);
+  const double q = bp.qmacro;
+  if (dt == 2) {
+    const double px = qdiv((double)ix * g.h1 - xp, g.q1);
+    const int64_t s = g.n1 + 3;
+    atomicAdd(&bp.sum_q_xz[ix + s * iz], q * px * pz);
+    atomicAdd(&bp.sum_q_xz[(ix + 1) + s * iz], q * (1.0 - px) * pz);
+    atomicAdd(&bp.sum_q_xz[(ix + 1) + s * (iz + 1)], q * (1.0 - px) * (1.0 - pz));
+    atomicAdd(&bp.sum_q_xz[ix + s * (iz + 1)], q * px * (1.0 - pz));
+  } else if (dt == 3 || dt == 4) {
+    const double py = qdiv((double)iy * g.h2 - yp, g.q2);
+    const int64_t s = g.n2 + 3;
+    const int c = dt - 3;
+    atomicAdd(&bp.sum_q_yz[c + 2 * (iy + s * iz)], q * py * pz);
+    atomicAdd(&bp.sum_q_yz[c + 2 * ((iy + 1) + s * iz)], q * (1.0 - py) * pz);
+    atomicAdd(&bp.sum_q_yz[c + 2 * ((iy + 1) + s * (iz + 1))], q * (1.0 - py) * (1.0 - pz));
+    atomicAdd(&bp.sum_q_yz[c + 2 * (iy + s * (iz + 1))], q * py * (1.0 - pz));
+  }
 }
 
 __global__ void k_wall_table(KGrid g, const int *__restrict__ bcnd, int *wt, int64_t nodes) {
@@ -341,6 +394,8 @@ __global__ __launch_bounds__(BLOCK) void k_apply_bc(KGrid g, KPset src, KPset ds
       if (!lost) {
         survivor_wrap(bp, xp, yp, zp, vx);
         keep = true;
+      } else if (bp.flag_die == 1 && label > 0) {
+        die_charge(g, bp, label, xp, yp, zp);
       }
     }
   }
@@ -809,6 +864,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_nodep(
     if (wall_test_tab(g, wt, bp, xp, yp, zp, label)) {
       p.dead[i] = 2;
       lost = true;
+      if (bp.flag_die == 1 && label > 0) die_charge(g, bp, label, xp, yp, zp);
     } else {
       survivor_wrap(bp, xp, yp, zp, vx);
       p.x[i] = xp; p.y[i] = yp; p.z[i] = zp;
@@ -896,6 +952,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(
       if (wall_test_tab_q(g, wt, bp, xp, qx, qy, qz, label)) {
         lost = true;
         ++lost_local;
+        if (bp.flag_die == 1 && label > 0) die_charge(g, bp, label, xp, yp, zp);
       } else {
         const double xw = xp, yw = yp, zw = zp;
         survivor_wrap(bp, xp, yp, zp, vx[q]);
@@ -954,6 +1011,8 @@ __global__ __launch_bounds__(BLOCK) void k_push_bc_deposit(
     if (!lost) {
       survivor_wrap(bp, xp, yp, zp, vx);
       keep = true;
+    } else if (bp.flag_die == 1 && label > 0) {
+      die_charge(g, bp, label, xp, yp, zp);
     }
   }
   wall_diag_wave(bp, lost, label, vx, vy, vz, p_mac, P_loss);
@@ -1453,6 +1512,15 @@ int lepic_replica_flush(lepic_ctx *ctx, int64_t nodes, double *np_local) {
   return LEPIC_OK;
 }
 
+// Dielectric charging needs its tables (flag_die = 1).
+static int check_die(lepic_ctx *ctx, const lepic_bc_params *bp, const char *who) {
+  if (bp->flag_die != 1) return LEPIC_OK;
+  if (!bp->dtype || !bp->sum_q_xz || !bp->sum_q_yz || bp->ngrid < 0)
+    return lepic_fail(ctx, LEPIC_ERR_ARG,
+                      std::string(who) + ": flag_die needs dtype, sum_q_xz and sum_q_yz");
+  return LEPIC_OK;
+}
+
 extern "C" {
 
 int lepic_move_electrostatic(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *p,
@@ -1505,6 +1573,7 @@ int lepic_apply_bc(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *src, l
     return st;
   if (!bcnd || !bp || !p_mac || !P_loss)
     return lepic_fail(ctx, LEPIC_ERR_ARG, "apply_bc: null argument");
+  if ((st = check_die(ctx, bp, "apply_bc"))) return st;
   if (dst->cap < src->n) return lepic_fail(ctx, LEPIC_ERR_CAPACITY, "apply_bc: dst too small");
   if (src == dst || src->x == dst->x)
     return lepic_fail(ctx, LEPIC_ERR_ARG, "apply_bc: src and dst must differ");
@@ -1558,6 +1627,7 @@ int lepic_push_bc_deposit_inplace(lepic_ctx *ctx, const lepic_grid *g, lepic_pse
                       "push_bc_deposit_inplace: particle arrays must be 16-byte aligned");
   if (!E4 || !bcnd || !bp || !p_mac || !P_loss || (np_local && !kq))
     return lepic_fail(ctx, LEPIC_ERR_ARG, "push_bc_deposit_inplace: null argument");
+  if ((st = check_die(ctx, bp, "push_bc_deposit_inplace"))) return st;
   if (bf && bf->mode == 2 && !bf->B4)
     return lepic_fail(ctx, LEPIC_ERR_ARG, "push_bc_deposit_inplace: null B map");
   if (bf && (bf->mode < 0 || bf->mode > 2))
@@ -1643,6 +1713,7 @@ int lepic_push_bc_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset 
     return st;
   if (!E4 || !bcnd || !bp || !p_mac || !P_loss || (np_local && !kq))
     return lepic_fail(ctx, LEPIC_ERR_ARG, "push_bc_deposit: null argument");
+  if ((st = check_die(ctx, bp, "push_bc_deposit"))) return st;
   if (bf && bf->mode == 2 && !bf->B4)
     return lepic_fail(ctx, LEPIC_ERR_ARG, "push_bc_deposit: null B map");
   if (bf && (bf->mode < 0 || bf->mode > 2))

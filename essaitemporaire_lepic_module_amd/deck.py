@@ -882,8 +882,6 @@ class Deck:
         """Features this deck turns on that the device step does not carry
         (the PIC-MCC core: push, walls, deposit, Poisson, MCC)."""
         c, b, out = self.conditions, self.boundary, []
-        if b.flag_die:
-            out.append("dielectric walls")
         if c.flag_inj:
             out.append("particle injection")
         if c.gam_sec > 0.0:
@@ -931,6 +929,7 @@ class Deck:
         case.phi_init = b.u
         case.ni0 = [chem.density[k] for k, s in enumerate(species) if s.tracked][:len(tracked)]
         case.x_load = c.x_load
+        case.flag_die, case.dtype, case.Ca = b.flag_die, np.array(b.dtype, np.int32), c.Ca
         # the modular driver heats every nb_step_heating steps whatever the
         # deck says (mod_simulation.f90:441-447); nu_h < 0 gives nudt < 0,
         # so nothing is re-drawn but the window's draws are still taken.

@@ -82,6 +82,20 @@ def solve_poisson_legacy(phi: torch.Tensor, bcnd: torch.Tensor, rhs: torch.Tenso
             "sweeps": [st.sweeps[i] for i in range(int(ng))]}
 
 
+def apply_dielectric_bc_to_phi(n, h, bcnd: torch.Tensor, dtype: torch.Tensor, ngrid: int,
+                               sum_q_xz: torch.Tensor, sum_q_yz: torch.Tensor, Ca_cell: float,
+                               phi: torch.Tensor):
+    """apply_dielectric_bc_to_phi (mod_state.f90:378-430): dielectric nodes
+    take each species' collected surface charge.  sum_q_xz (ntype, n3+3,
+    n1+3), sum_q_yz (ntype, n3+3, n2+3, 2), dtype int32 over labels 0..ngrid;
+    Ca_cell = Ca h1 h3."""
+    ntype = int(sum_q_xz.shape[0])
+    ctx = context(phi.device)
+    ctx.check(ctx.lib.lepic_dielectric_phi(ctx.h, C.byref(_native.grid(n, h)), ptr(bcnd), ptr(dtype),
+                                           int(ngrid), ntype, ptr(sum_q_xz), ptr(sum_q_yz),
+                                           float(Ca_cell), ptr(phi)))
+
+
 def calc_Efield_modular(n, h, phi: torch.Tensor, E4: torch.Tensor, bcnd: torch.Tensor):
     """E = -grad(phi) with the reference's wall extrapolation and periodic
     ghost copies (mod_electricField.f90:9-164)."""

@@ -175,30 +175,53 @@ def move_particles_boris(part: ParticleSet, n, h, E4, bf: _native.BField, q: flo
 # a5: wall BC
 # ---------------------------------------------------------------------------
 
-def bc_params(n, h, flag_pbc, flag_nmn, ptype, tag_neg, qmacro, mass, Nm, ngrid) -> _native.BCParams:
+def bc_params(n, h, flag_pbc, flag_nmn, ptype, tag_neg, qmacro, mass, Nm, ngrid,
+              flag_die: int = 0, dtype: torch.Tensor | None = None,
+              sum_q_xz: torch.Tensor | None = None,
+              sum_q_yz: torch.Tensor | None = None) -> _native.BCParams:
+    """Wall-BC parameters; with flag_die, dtype (device int32 over labels
+    0..ngrid) and this species' surface-charge planes sum_q_xz (n3+3, n1+3)
+    and sum_q_yz (n3+3, n2+3, 2)."""
     bp = _native.BCParams()
     bp.xmax, bp.ymax, bp.zmax = (float(n[d]) * float(h[d]) for d in range(3))
     bp.flag_pbc, bp.flag_nmn = int(flag_pbc), int(flag_nmn)
     bp.ptype, bp.tag_neg = int(ptype), int(tag_neg)
     bp.qmacro, bp.mass, bp.Nm = float(qmacro), float(mass), float(Nm)
     bp.ngrid = int(ngrid)
+    bp.flag_die = int(flag_die)
+    if flag_die:
+        if dtype is None or sum_q_xz is None or sum_q_yz is None:
+            raise ValueError("flag_die needs dtype, sum_q_xz and sum_q_yz")
+        if dtype.dtype != torch.int32 or dtype.numel() != int(ngrid) + 1:
+            raise ValueError("dtype must be int32 over labels 0..ngrid")
+        n1, n2, n3 = (int(v) for v in n)
+        if (tuple(sum_q_xz.shape) != (n3 + 3, n1 + 3) or tuple(sum_q_yz.shape) != (n3 + 3, n2 + 3, 2)
+                or sum_q_xz.dtype != torch.float64 or sum_q_yz.dtype != torch.float64
+                or not sum_q_xz.is_contiguous() or not sum_q_yz.is_contiguous()):
+            raise ValueError("sum_q_xz / sum_q_yz: contiguous float64 (n3+3, n1+3), (n3+3, n2+3, 2)")
+        bp.dtype, bp.sum_q_xz, bp.sum_q_yz = ptr(dtype), ptr(sum_q_xz), ptr(sum_q_yz)
+        bp._keep = (dtype, sum_q_xz, sum_q_yz)
     return bp
 
 
 def apply_particle_bc_legacy(part: ParticleSet, n, h, bcnd: torch.Tensor, flag_pbc: int,
                              flag_nmn: int, ptype: int, tag_neg: int, flag_die: int, qmacro: float,
                              p_mac_boundary: torch.Tensor, mass_species: float,
-                             P_loss_wall: torch.Tensor, Nm_species: float) -> int:
-    """Wall loss, Neumann reflection, periodic wrap and stable compaction
-    (mod_particleBC.f90:42-259).  p_mac_boundary: (ngrid+1, 2) FP64 device
-    tensor indexed by electrode label; P_loss_wall: 1-element FP64 tensor.
-    Returns the number of particles removed."""
-    if flag_die:
-        raise NotImplementedError("dielectric walls (flag_die = 1) are not on this path")
+                             P_loss_wall: torch.Tensor, Nm_species: float,
+                             dtype: torch.Tensor | None = None,
+                             sum_q_xz: torch.Tensor | None = None,
+                             sum_q_yz: torch.Tensor | None = None) -> int:
+    """Wall loss, dielectric surface charging, Neumann reflection, periodic
+    wrap and stable compaction (mod_particleBC.f90:42-259).  p_mac_boundary:
+    (ngrid+1, 2) FP64 device tensor indexed by electrode label; P_loss_wall:
+    1-element FP64 tensor; with flag_die, the domain's dtype (labels
+    0..ngrid) and the species' surface-charge planes.  Returns the number of
+    particles removed."""
     ctx = context(part.device)
     g = _native.grid(n, h)
     ngrid = p_mac_boundary.shape[0] - 1
-    bp = bc_params(n, h, flag_pbc, flag_nmn, ptype, tag_neg, qmacro, mass_species, Nm_species, ngrid)
+    bp = bc_params(n, h, flag_pbc, flag_nmn, ptype, tag_neg, qmacro, mass_species, Nm_species, ngrid,
+                   flag_die, dtype, sum_q_xz, sum_q_yz)
     src, dst = part.pset(), part.pset(other=True)
     lost = C.c_int64(0)
     ctx.check(ctx.lib.lepic_apply_bc(ctx.h, C.byref(g), C.byref(src), C.byref(dst), ptr(bcnd),

@@ -28,6 +28,8 @@ from __future__ import annotations
 
 import contextlib
 
+import numpy as np
+
 import torch
 
 from . import _native
@@ -105,8 +107,22 @@ class Simulation:
                 self.bf.append(P.bfield(1, case.B_uniform, magnetized=(q <= 0 or case.ions_magnetized)))
             else:
                 self.bf.append(P.bfield(0))
+        # dielectric surface charging: per-species charge planes the wall BC
+        # fills and the next step's potential takes (mod_state.f90:148-157)
+        self.dtype_d = self.sum_q_xz = self.sum_q_yz = None
+        if case.flag_die:
+            if case.dtype is None or len(case.dtype) != case.ngrid + 1:
+                raise ValueError("flag_die needs the wall types of labels 0..ngrid")
+            self.dtype_d = torch.as_tensor(np.asarray(case.dtype, np.int32), device=dev)
+            n1, n2, n3 = (int(v) for v in n)
+            self.sum_q_xz = torch.zeros((self.ntype, n3 + 3, n1 + 3), dtype=torch.float64, device=dev)
+            self.sum_q_yz = torch.zeros((self.ntype, n3 + 3, n2 + 3, 2), dtype=torch.float64,
+                                        device=dev)
         self.bp = [P.bc_params(n, h, case.flag_pbc, case.flag_nmn, s + 1, self.tag_neg,
-                               self.Nm[s] * self.charge[s], self.mass[s], self.Nm[s], case.ngrid)
+                               self.Nm[s] * self.charge[s], self.mass[s], self.Nm[s], case.ngrid,
+                               case.flag_die, self.dtype_d,
+                               self.sum_q_xz[s] if case.flag_die else None,
+                               self.sum_q_yz[s] if case.flag_die else None)
                    for s in range(self.ntype)]
         self.last_mg = None
         self.last_mcc = None
@@ -179,6 +195,9 @@ class Simulation:
         with self._phase("rho"):
             F.reduce_species_density(n, self.bcnd, self.np_thread, self.ntype, self.np_red)
             F.build_rho_from_np(n, self.np_red, self.charge, self.ntype, self.rho)
+        if c.flag_die:
+            F.apply_dielectric_bc_to_phi(n, h, self.bcnd, self.dtype_d, c.ngrid, self.sum_q_xz,
+                                         self.sum_q_yz, c.Ca * h[0] * h[2], self.phi)
         with self._phase("poisson"):
             self.last_mg = F.solve_poisson_legacy(self.phi, self.bcnd, self.rho, h, n, c.ncycl,
                                                   c.eps, c.omega, c.ng, c.flag_pbc, c.flag_nmn)
@@ -186,6 +205,9 @@ class Simulation:
             F.calc_Efield_modular(n, h, self.phi, self.E4, self.bcnd)
         if self.diagnostics is not None:
             self.diagnostics.before_push(istep)
+        if c.flag_die:
+            self.sum_q_xz.zero_()
+            self.sum_q_yz.zero_()
         collide = (self.tables is not None and c.nb_step_collisions > 0
                    and istep % c.nb_step_collisions == 0)
         # the production path deposits in the push on collision steps too and
@@ -223,9 +245,11 @@ class Simulation:
             with self._phase("bc"):
                 for s, p in enumerate(self.parts):
                     self.lost[s] = P.apply_particle_bc_legacy(
-                        p, n, h, self.bcnd, c.flag_pbc, c.flag_nmn, s + 1, self.tag_neg, 0,
-                        self.Nm[s] * self.charge[s], self.p_mac[s], self.mass[s],
-                        self.P_loss[s:s + 1], self.Nm[s])
+                        p, n, h, self.bcnd, c.flag_pbc, c.flag_nmn, s + 1, self.tag_neg,
+                        c.flag_die, self.Nm[s] * self.charge[s], self.p_mac[s], self.mass[s],
+                        self.P_loss[s:s + 1], self.Nm[s], self.dtype_d,
+                        self.sum_q_xz[s] if c.flag_die else None,
+                        self.sum_q_yz[s] if c.flag_die else None)
         if collide:
             self.ensure_headroom()
             with self._phase("mcc"):

@@ -60,8 +60,13 @@ typedef struct {
 } lepic_pset;
 
 /* Wall-BC parameters (apply_particle_bc_legacy arguments,
- * modules/particles/mod_particleBC.f90:42-62); dielectric walls are not
- * supported (flag_die must be 0). */
+ * modules/particles/mod_particleBC.f90:42-62).  With flag_die = 1 a particle
+ * lost on a label whose type is a dielectric (> 1) adds its charge Nm*q to
+ * the species' surface charge (mod_particleBC.f90:139-188): bilinear in x-z
+ * into sum_q_xz ((n1+3) x (n3+3), x fastest) for type 2, in y-z into
+ * sum_q_yz (2 x (n2+3) x (n3+3), the plane index 0 or 1 fastest) for types
+ * 3 and 4.  dtype is the domain's table over labels 0..ngrid, read as the
+ * routine's 1-based dummy reads it: label L's type is dtype[L-1]. */
 typedef struct {
   double xmax, ymax, zmax;
   int32_t flag_pbc, flag_nmn;
@@ -69,6 +74,10 @@ typedef struct {
   double qmacro;           /* Nm*q */
   double mass, Nm;
   int32_t ngrid;           /* highest electrode label                 */
+  int32_t flag_die;        /* 1: dielectric surface charging           */
+  const int32_t *dtype;    /* device, ngrid+1 (flag_die = 1)           */
+  double *sum_q_xz;        /* device (flag_die = 1)                    */
+  double *sum_q_yz;        /* device (flag_die = 1)                    */
 } lepic_bc_params;
 
 /* Magnetic field of the Boris push (Src/part_expmover.f90:139-212). */
@@ -234,6 +243,20 @@ int lepic_collisions_deposited(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *
                                double dt, int32_t mode, int32_t *iseed, uint64_t philox_key,
                                uint64_t philox_counter, double *Pcoll, lepic_mcc_stats *stats,
                                double *np_dep, const double *kq);
+
+/* ---------------- f4: dielectric surface charging ---------------- */
+/* apply_dielectric_bc_to_phi (mod_state.f90:378-430): every node whose label
+ * (bcnd > 0) has dtype 2, 3 or 4 takes, species by species, the surface
+ * charge the wall BC collected: sum_q_xz (ntype blocks of (n1+3)(n3+3), the
+ * node's x-z column) / (2 Ca_cell) for type 2, sum_q_yz (ntype blocks of
+ * 2(n2+3)(n3+3), plane 0 for type 3, 1 for type 4) / Ca_cell.  dtype is the
+ * domain's table over labels 0..ngrid (indexed by the label itself);
+ * Ca_cell = Ca h1 h3 (mod_simParams.f90:166-167).  Labels above ngrid are
+ * left alone. */
+int lepic_dielectric_phi(lepic_ctx *ctx, const lepic_grid *g, const int32_t *bcnd,
+                         const int32_t *dtype, int32_t ngrid, int32_t ntype,
+                         const double *sum_q_xz, const double *sum_q_yz, double Ca_cell,
+                         double *phi);
 
 /* ---------------- f4: electron heating ---------------- */
 /* Parameters of the modular step's electron heating (mod_state.f90:341-373,

@@ -345,6 +345,87 @@ int lo_apply_bc(const lo_grid *g, lo_pset *p, const int32_t *bcnd, double xmax,
                 double ymax, double zmax, int flag_pbc, int flag_nmn, int ptype,
                 int tag_neg, double qmacro, double mass, double Nm, int32_t ngrid,
                 double *p_mac, double *P_loss, int64_t *np_lost_out) {
+  return lo_apply_bc_die(g, p, bcnd, xmax, ymax, zmax, flag_pbc, flag_nmn, ptype, tag_neg,
+                         qmacro, mass, Nm, ngrid, 0, NULL, NULL, NULL, p_mac, P_loss,
+                         np_lost_out);
+}
+
+/* The dielectric branch of apply_particle_bc_legacy (mod_particleBC.f90:
+ * 139-188): dtype is read through the routine's 1-based dtype(:) dummy, so
+ * label igrid's type is the domain's dtype[igrid - 1]. */
+static void die_charge(const lo_grid *g, const int32_t *dtype, int flag_pbc, double zmax,
+                       double qmacro, int32_t igrid, int32_t ix, int32_t iy, int32_t iz, double xp,
+                       double yp, double zp, double *sxz, double *syz) {
+  const double *h = g->h;
+  const int32_t *n = g->n;
+  const int32_t dt = dtype[igrid - 1];
+  if (dt <= 1) return;
+  if (flag_pbc == 1) {
+    if (zp >= zmax) {
+      zp = zp - zmax;
+      iz = (int32_t)floor(zp / h[2]) + 1;
+    }
+    if (zp <= 0.0) {
+      zp = zmax + zp;
+      iz = (int32_t)floor(zp / h[2]) + 1;
+    }
+  }
+  if (ix < 0) ix = 0;
+  if (ix > n[0] + 1) ix = n[0] + 1;
+  if (iy < 0) iy = 0;
+  if (iy > n[1] + 1) iy = n[1] + 1;
+  if (iz < 0) iz = 0;
+  if (iz > n[2] + 1) iz = n[2] + 1;
+  const double pz = ((double)iz * h[2] - zp) / h[2];
+  if (dt == 2) {
+    const double px = ((double)ix * h[0] - xp) / h[0];
+    const int64_t s = n[0] + 3;
+    sxz[ix + s * iz] = sxz[ix + s * iz] + qmacro * px * pz;
+    sxz[(ix + 1) + s * iz] = sxz[(ix + 1) + s * iz] + qmacro * (1.0 - px) * pz;
+    sxz[(ix + 1) + s * (iz + 1)] = sxz[(ix + 1) + s * (iz + 1)] + qmacro * (1.0 - px) * (1.0 - pz);
+    sxz[ix + s * (iz + 1)] = sxz[ix + s * (iz + 1)] + qmacro * px * (1.0 - pz);
+  }
+  if (dt == 3 || dt == 4) {
+    const double py = ((double)iy * h[1] - yp) / h[1];
+    const int64_t s = n[1] + 3;
+    const int c = dt - 3;
+    syz[c + 2 * (iy + s * iz)] = syz[c + 2 * (iy + s * iz)] + qmacro * py * pz;
+    syz[c + 2 * ((iy + 1) + s * iz)] = syz[c + 2 * ((iy + 1) + s * iz)] + qmacro * (1.0 - py) * pz;
+    syz[c + 2 * ((iy + 1) + s * (iz + 1))] =
+        syz[c + 2 * ((iy + 1) + s * (iz + 1))] + qmacro * (1.0 - py) * (1.0 - pz);
+    syz[c + 2 * (iy + s * (iz + 1))] = syz[c + 2 * (iy + s * (iz + 1))] + qmacro * py * (1.0 - pz);
+  }
+}
+
+/* apply_dielectric_bc_to_phi (mod_state.f90:378-430), one rank and thread. */
+void lo_dielectric_phi(const lo_grid *g, const int32_t *bcnd, const int32_t *dtype,
+                       int32_t ngrid, int32_t ntype, const double *sxz, const double *syz,
+                       double Ca, double *phi) {
+  const int32_t *n = g->n;
+  const int64_t sxz_n = (int64_t)(n[0] + 3) * (n[2] + 3);
+  const int64_t syz_n = 2 * (int64_t)(n[1] + 3) * (n[2] + 3);
+  for (int32_t k = 0; k <= n[2] + 2; ++k)
+    for (int32_t j = 0; j <= n[1] + 2; ++j)
+      for (int32_t i = 0; i <= n[0] + 2; ++i) {
+        const int64_t c = IDX(g, i, j, k);
+        const int32_t igrid = bcnd[c];
+        if (igrid <= 0 || igrid > ngrid) continue;
+        for (int32_t s = 0; s < ntype; ++s) {
+          if (dtype[igrid] == 2)
+            phi[c] = phi[c] + (0.0 + sxz[s * sxz_n + i + (int64_t)(n[0] + 3) * k]) / (2.0 * Ca);
+          if (dtype[igrid] == 3)
+            phi[c] = phi[c] + (0.0 + syz[s * syz_n + 2 * (j + (int64_t)(n[1] + 3) * k)]) / Ca;
+          if (dtype[igrid] == 4)
+            phi[c] = phi[c] + (0.0 + syz[s * syz_n + 1 + 2 * (j + (int64_t)(n[1] + 3) * k)]) / Ca;
+        }
+      }
+}
+
+int lo_apply_bc_die(const lo_grid *g, lo_pset *p, const int32_t *bcnd, double xmax,
+                    double ymax, double zmax, int flag_pbc, int flag_nmn, int ptype,
+                    int tag_neg, double qmacro, double mass, double Nm, int32_t ngrid,
+                    int flag_die, const int32_t *dtype, double *sum_q_xz, double *sum_q_yz,
+                    double *p_mac, double *P_loss, int64_t *np_lost_out) {
   (void)xmax;
   const double *h = g->h;
   const int32_t *n = g->n;
@@ -367,6 +448,9 @@ int lo_apply_bc(const lo_grid *g, lo_pset *p, const int32_t *bcnd, double xmax,
     if (flag_lost >= 1) {
       int32_t igrid = bcnd[IDX(g, ix, iy, iz)];
       if (flag_lost == 2 && ptype == tag_neg) igrid = 0;
+      if (flag_die == 1 && igrid > 0 && igrid <= ngrid + 1)
+        die_charge(g, dtype, flag_pbc, zmax, qmacro, igrid, ix, iy, iz, xp, yp, zp, sum_q_xz,
+                   sum_q_yz);
       if (igrid < 0) igrid = 0;
       double v2 = vx * vx + vy * vy + vz * vz;
       double Ek_eV = 0.5 * mass * v2 / QE;

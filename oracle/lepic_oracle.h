@@ -86,6 +86,19 @@ int lo_apply_bc(const lo_grid *g, lo_pset *p, const int32_t *bcnd,
                 double mass, double Nm, int32_t ngrid, double *p_mac,
                 double *P_loss, int64_t *np_lost);
 
+/* With dielectric charging (flag_die = 1): dtype over labels 0..ngrid (read
+ * as the routine's 1-based dummy: label L's type is dtype[L-1]); sum_q_xz
+ * ((n1+3)(n3+3)) and sum_q_yz (2(n2+3)(n3+3)) of this species accumulate. */
+int lo_apply_bc_die(const lo_grid *g, lo_pset *p, const int32_t *bcnd, double xmax,
+                    double ymax, double zmax, int flag_pbc, int flag_nmn, int ptype,
+                    int tag_neg, double qmacro, double mass, double Nm, int32_t ngrid,
+                    int flag_die, const int32_t *dtype, double *sum_q_xz, double *sum_q_yz,
+                    double *p_mac, double *P_loss, int64_t *np_lost_out);
+/* apply_dielectric_bc_to_phi (mod_state.f90:378-430); sxz, syz: ntype blocks. */
+void lo_dielectric_phi(const lo_grid *g, const int32_t *bcnd, const int32_t *dtype,
+                       int32_t ngrid, int32_t ntype, const double *sxz, const double *syz,
+                       double Ca, double *phi);
+
 /* ---- deposition, density, rho (modules/fields) ---- */
 void lo_build_kq(const lo_grid *g, const int32_t *bcnd, double *kq);
 void lo_deposit(const lo_grid *g, const lo_pset *p, const double *kq,

@@ -103,6 +103,13 @@ def lib():
                                   C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                   C.c_double, C.c_double, C.c_int32, dp, dp,
                                   C.POINTER(C.c_int64)]
+        L.lo_apply_bc_die.restype = C.c_int
+        L.lo_apply_bc_die.argtypes = [C.POINTER(Grid), C.POINTER(PSet), ip, C.c_double, C.c_double,
+                                      C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                      C.c_double, C.c_double, C.c_int32, C.c_int, ip, dp, dp, dp,
+                                      dp, C.POINTER(C.c_int64)]
+        L.lo_dielectric_phi.argtypes = [C.POINTER(Grid), ip, ip, C.c_int32, C.c_int32, dp, dp,
+                                        C.c_double, dp]
         L.lo_build_kq.argtypes = [C.POINTER(Grid), ip, dp]
         L.lo_deposit.argtypes = [C.POINTER(Grid), C.POINTER(PSet), dp, C.c_double, dp]
         L.lo_reduce_density.argtypes = [C.POINTER(Grid), ip, C.c_int32, C.c_int32, dp, dp]
@@ -230,16 +237,36 @@ def move_boris(n, h, p: Particles, E4, b_mode, Bconst, B4, q, m, dt, magnetized=
                         _p(Bc), _p(B4a), q, m, dt, 1 if magnetized else 0)
 
 
-def apply_bc(n, h, p: Particles, bcnd, flag_pbc, flag_nmn, ptype, tag_neg, qmacro, mass, Nm, ngrid):
+def apply_bc(n, h, p: Particles, bcnd, flag_pbc, flag_nmn, ptype, tag_neg, qmacro, mass, Nm, ngrid,
+             dtype=None, sum_q_xz=None, sum_q_yz=None):
+    """apply_particle_bc_legacy; with dtype (labels 0..ngrid) the dielectric
+    branch runs and adds to sum_q_xz (n3+3, n1+3) and sum_q_yz (n3+3, n2+3, 2)."""
     g = grid(n, h)
     p_mac = np.zeros(2 * (ngrid + 1))
     P_loss = np.zeros(1)
     lost = C.c_int64(0)
     xmax, ymax, zmax = (n[d] * h[d] for d in range(3))
-    lib().lo_apply_bc(C.byref(g), C.byref(p.s), _p(bcnd, C.c_int32), xmax, ymax, zmax,
-                      int(flag_pbc), int(flag_nmn), int(ptype), int(tag_neg), qmacro, mass, Nm,
-                      int(ngrid), _p(p_mac), _p(P_loss), C.byref(lost))
+    if dtype is None:
+        lib().lo_apply_bc(C.byref(g), C.byref(p.s), _p(bcnd, C.c_int32), xmax, ymax, zmax,
+                          int(flag_pbc), int(flag_nmn), int(ptype), int(tag_neg), qmacro, mass, Nm,
+                          int(ngrid), _p(p_mac), _p(P_loss), C.byref(lost))
+    else:
+        dt = np.ascontiguousarray(dtype, np.int32)
+        assert sum_q_xz.flags.c_contiguous and sum_q_yz.flags.c_contiguous
+        lib().lo_apply_bc_die(C.byref(g), C.byref(p.s), _p(bcnd, C.c_int32), xmax, ymax, zmax,
+                              int(flag_pbc), int(flag_nmn), int(ptype), int(tag_neg), qmacro, mass,
+                              Nm, int(ngrid), 1, _p(dt, C.c_int32), _p(sum_q_xz), _p(sum_q_yz),
+                              _p(p_mac), _p(P_loss), C.byref(lost))
     return lost.value, p_mac, float(P_loss[0])
+
+
+def dielectric_phi(n, h, bcnd, dtype, ngrid, sum_q_xz, sum_q_yz, Ca, phi):
+    """apply_dielectric_bc_to_phi on phi (in place); sum_q_* carry ntype first."""
+    g = grid(n, h)
+    dt = np.ascontiguousarray(dtype, np.int32)
+    lib().lo_dielectric_phi(C.byref(g), _p(bcnd, C.c_int32), _p(dt, C.c_int32), int(ngrid),
+                            int(sum_q_xz.shape[0]), _p(np.ascontiguousarray(sum_q_xz)),
+                            _p(np.ascontiguousarray(sum_q_yz)), float(Ca), _p(phi))
 
 
 def build_kq(n, bcnd):
@@ -389,6 +416,10 @@ class OracleSim:
         self.last_mcc = None
         self.P_heat = 0.0
         self.R_ahp_seen = 0.0
+        if case.flag_die:
+            n1, n2, n3 = (int(v) for v in self.n)
+            self.sum_q_xz = np.zeros((self.ntype, n3 + 3, n1 + 3))
+            self.sum_q_yz = np.zeros((self.ntype, n3 + 3, n2 + 3, 2))
         self.last_heating = None
 
     def advance_one_step(self, istep):
@@ -399,6 +430,9 @@ class OracleSim:
                     sort_particles(n, h, p)
         self.np_red = reduce_density(n, self.bcnd, self.np_thread)
         rho = build_rho(n, self.np_red, self.charge)
+        if c.flag_die:
+            dielectric_phi(n, h, self.bcnd, c.dtype, c.ngrid, self.sum_q_xz, self.sum_q_yz,
+                           c.Ca * h[0] * h[2], self.phi)
         st, self.last_mg = pdesolver(n, h, self.phi, rho, self.bcnd, c.flag_pbc, c.flag_nmn,
                                      ncycl=c.ncycl, eps=c.eps, omega=c.omega, ng=c.ng)
         if st != 0:
@@ -412,9 +446,14 @@ class OracleSim:
                 move_boris(n, h, p, E4, 1, c.B_uniform, None, q, m, c.dt, q <= 0 or c.ions_magnetized)
             else:
                 move_electrostatic(n, h, p, E4, q, m, c.dt)
+        if c.flag_die:
+            self.sum_q_xz[:] = 0.0
+            self.sum_q_yz[:] = 0.0
         for s, p in enumerate(self.parts):
+            die = (c.dtype, self.sum_q_xz[s], self.sum_q_yz[s]) if c.flag_die else (None, None, None)
             lost, pm, Pl = apply_bc(n, h, p, self.bcnd, c.flag_pbc, c.flag_nmn, s + 1, self.tag_neg,
-                                    self.Nm[s] * self.charge[s], self.mass[s], self.Nm[s], c.ngrid)
+                                    self.Nm[s] * self.charge[s], self.mass[s], self.Nm[s], c.ngrid,
+                                    *die)
             self.lost[s] = lost
             self.p_mac[s] += pm
             self.P_loss[s] += Pl

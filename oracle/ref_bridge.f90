@@ -122,6 +122,34 @@ contains
     call from_pset(p, np, x, y, z, vx, vy, vz, dead)
   end subroutine ref_apply_bc
 
+  ! apply_particle_bc_legacy with dielectric charging: dtype_in over labels
+  ! 0..ngrid handed over as State hands dom%dtype(0:ngrid) to the routine.
+  subroutine ref_apply_bc_die(n, h, np, x, y, z, vx, vy, vz, dead, bcnd, flag_pbc, flag_nmn, &
+                              ptype, tag_neg, qmacro, mass, Nm, ngrid, dtype_in, sxz, syz, &
+                              p_mac, P_loss) bind(C, name="ref_apply_bc_die")
+    integer(c_int32_t), intent(in) :: n(3)
+    real(c_double), intent(in) :: h(3)
+    integer(c_int64_t), intent(inout) :: np
+    real(c_double), intent(inout) :: x(*), y(*), z(*), vx(*), vy(*), vz(*)
+    integer(c_int8_t), intent(inout) :: dead(*)
+    integer(c_int32_t), intent(in) :: bcnd(0:n(1)+2, 0:n(2)+2, 0:n(3)+2)
+    integer(c_int32_t), value :: flag_pbc, flag_nmn, ptype, tag_neg, ngrid
+    real(c_double), value :: qmacro, mass, Nm
+    integer(c_int32_t), intent(in) :: dtype_in(0:ngrid)
+    real(c_double), intent(inout) :: sxz(0:n(1)+2, 0:n(3)+2), syz(2, 0:n(2)+2, 0:n(3)+2)
+    real(c_double), intent(inout) :: p_mac(2, ngrid+1)
+    real(c_double), intent(inout) :: P_loss
+    type(ParticleSet) :: p
+    integer(int32), allocatable :: dt(:)
+    allocate(dt(0:ngrid))
+    dt = dtype_in
+    call to_pset(p, np, x(1:np), y(1:np), z(1:np), vx(1:np), vy(1:np), vz(1:np), dead(1:np))
+    call apply_particle_bc_legacy(p, n, h, bcnd, real(n(1),real64)*h(1), real(n(2),real64)*h(2), &
+         real(n(3),real64)*h(3), flag_pbc, flag_nmn, ptype, tag_neg, 1_int32, dt, qmacro, &
+         sxz, syz, p_mac, mass, P_loss, Nm)
+    call from_pset(p, np, x, y, z, vx, vy, vz, dead)
+  end subroutine ref_apply_bc_die
+
   subroutine ref_deposit(n, h, np, x, y, z, dead, kq, Nm, np_local) bind(C, name="ref_deposit")
     integer(c_int32_t), intent(in) :: n(3)
     real(c_double), intent(in) :: h(3)
@@ -236,5 +264,46 @@ contains
       buf(i) = s(i:i)
     end do
   end subroutine ref_format
+
+  ! The particle backup as Src/main.f90:1321-1327 writes it for one thread
+  ! (kind 1) or reads it as Src/restart.f90:63-70 (kind 2): the time, np_tot,
+  ! each species' (6, np) block, then flag_cex of species tag_neg if > 0.
+  subroutine ref_backup_io(kind, fname, nlen, time, ntype, np_tot, vxp, tag_neg, cex) &
+      bind(C, name="ref_backup_io")
+    integer(c_int32_t), value :: kind, nlen, ntype, tag_neg
+    character(kind=c_char), intent(in) :: fname(nlen)
+    real(c_double), intent(inout) :: time
+    integer(c_int32_t), intent(inout) :: np_tot(ntype)
+    real(c_double), intent(inout) :: vxp(6, *)
+    integer(c_int32_t), intent(inout) :: cex(*)
+    character(len=nlen) :: name
+    integer :: i, u, t, off
+    do i = 1, nlen
+      name(i:i) = fname(i)
+    end do
+    if (kind == 1) then
+      open(newunit=u, file=name, form='UNFORMATTED')
+      write(u) time
+      write(u) np_tot(1:ntype)
+      off = 0
+      do t = 1, ntype
+        write(u) vxp(:, off + 1:off + np_tot(t))
+        off = off + np_tot(t)
+      end do
+      if (tag_neg > 0) write(u) cex(1:np_tot(tag_neg))
+      close(u)
+    else
+      open(newunit=u, file=name, form='UNFORMATTED', status='old')
+      read(u) time
+      read(u) np_tot(1:ntype)
+      off = 0
+      do t = 1, ntype
+        read(u) vxp(:, off + 1:off + np_tot(t))
+        off = off + np_tot(t)
+      end do
+      if (tag_neg > 0) read(u) cex(1:np_tot(tag_neg))
+      close(u)
+    end if
+  end subroutine ref_backup_io
 
 end module ref_bridge

@@ -364,10 +364,11 @@ def test_reference_deck_params_and_case():
 @need_ref
 def test_unsupported_features_refused():
     d = D.load_deck(os.path.join(REF, "Thruster"))
-    assert "dielectric walls" in d.unsupported()
-    with pytest.raises(D.DeckError, match="dielectric"):
+    assert d.unsupported() == ["particle injection", "B map on its own grid"]
+    with pytest.raises(D.DeckError, match="injection"):
         d.case()
-    assert d.case(allow_unsupported=True).flag_pbc == 1
+    case = d.case(allow_unsupported=True)
+    assert case.flag_pbc == 1 and case.flag_die == 1 and list(case.dtype) == [0, 1, 1, 2, 3, 2]
 
 
 @need_ref

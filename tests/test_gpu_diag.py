@@ -115,3 +115,29 @@ def test_simulation_diagnostics(gpu, tmp_path):
     torch.cuda.synchronize()
     assert seen["cnt"] == 1 and torch.equal(seen["avg"], seen["np"]) and seen["w"] > 0.0
     assert d2.cnt_avg == 0
+
+
+def test_backup_and_restart(gpu, tmp_path):
+    """A production run's backup restarts into a Simulation holding the same
+    live particles in the same order and the same deposited density."""
+    from essaitemporaire_lepic_module_amd import restart as R
+    from essaitemporaire_lepic_module_amd.simulation import Simulation
+    case = cfg.cfg1_plain_box(n=8, np_cell=6, ng=3)
+    sim = Simulation(case, "cuda:0", fused=True, lazy=True)
+    sim.run(3)
+    torch.cuda.synchronize()
+    path = R.save_simulation(sim, str(tmp_path / "DATA.BAK"), time=3 * case.dt)
+    assert os.path.basename(path) == "particles0.bak"
+    sim2, t = R.restart_simulation(case, path, "cuda:0", fused=True, lazy=True)
+    assert t == 3 * case.dt
+    for a, b in zip(sim.parts, sim2.parts):
+        ha, hb = a.to_host(), b.to_host()
+        live = ha["dead"] == 0
+        assert int(live.sum()) == b.n
+        for k in ("x", "y", "z", "vx", "vy", "vz"):
+            np.testing.assert_array_equal(ha[k][live], hb[k])
+    np.testing.assert_allclose(sim2.np_thread.cpu().numpy(), sim.np_thread.cpu().numpy(),
+                               rtol=0, atol=1e-12 * float(sim.np_thread.abs().max()))
+    sim2.run(2, start=4)
+    torch.cuda.synchronize()
+    assert sim2.particle_count() > 0

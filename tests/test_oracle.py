@@ -447,3 +447,32 @@ def test_loader_counts_and_stream(oracle):
     # kq in {1, 2}: the deposited total lies between k1*N and 2*k1*N
     assert k1 * jmax * (1 - 1e-12) <= npt[0].sum() <= 2 * k1 * jmax
     np.testing.assert_array_equal(npt[0], npt[1])
+
+
+def test_dielectric_phi_closed_form(oracle):
+    """apply_dielectric_bc_to_phi: a node of a type-2 label takes sum over
+    species of its x-z column's charge / (2 Ca); types 3 and 4 their y-z
+    plane's / Ca; metals, plasma and labels above ngrid keep phi."""
+    n, h = (4, 3, 5), (1e-3, 2e-3, 3e-3)
+    shape = (n[2] + 3, n[1] + 3, n[0] + 3)
+    bcnd = np.full(shape, -1, np.int32)
+    bcnd[1, 2, 3] = 1      # metal
+    bcnd[2, 1, 4] = 2      # x-z dielectric
+    bcnd[3, 4, 0] = 3      # first y-z dielectric
+    bcnd[4, 0, 2] = 4      # second y-z dielectric
+    bcnd[0, 0, 0] = 9      # above ngrid
+    dtype = np.array([0, 1, 2, 3, 4], np.int32)
+    rng = np.random.default_rng(0)
+    sxz = rng.normal(0, 1e-12, (2, n[2] + 3, n[0] + 3))
+    syz = rng.normal(0, 1e-12, (2, n[2] + 3, n[1] + 3, 2))
+    phi = rng.normal(0, 1, shape)
+    before = phi.copy()
+    Ca = 1e-6 * h[0] * h[2]
+    oracle.dielectric_phi(n, h, bcnd, dtype, 4, sxz, syz, Ca, phi)
+    want = before.copy()
+    want[2, 1, 4] = before[2, 1, 4] + sxz[0, 2, 4] / (2 * Ca) + sxz[1, 2, 4] / (2 * Ca)
+    want[3, 4, 0] = before[3, 4, 0] + syz[0, 3, 4, 0] / Ca + syz[1, 3, 4, 0] / Ca
+    want[4, 0, 2] = before[4, 0, 2] + syz[0, 4, 0, 1] / Ca + syz[1, 4, 0, 1] / Ca
+    np.testing.assert_allclose(phi, want, rtol=1e-15, atol=0)
+    changed = phi != before
+    assert changed.sum() == 3
