@@ -70,7 +70,7 @@ class Heating(C.Structure):
 
 class BField(C.Structure):
     _fields_ = [("mode", C.c_int32), ("B", C.c_double * 3), ("B4", C.c_void_p),
-                ("magnetized", C.c_int32)]
+                ("magnetized", C.c_int32), ("nB", C.c_int32 * 3), ("hB", C.c_double * 3)]
 
 
 class MGParams(C.Structure):

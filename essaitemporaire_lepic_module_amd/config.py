@@ -102,6 +102,7 @@ class Case:
     flag_nmn: int = 0
     B_uniform: tuple | None = None
     B_map: np.ndarray | None = None
+    B_grid: tuple | None = None          # (nB, hB) when B_map lives on its own grid
     ions_magnetized: bool = True
     ngas: float = 0.0
     nb_step_sort: int = 10

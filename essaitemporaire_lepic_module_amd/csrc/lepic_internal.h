@@ -240,15 +240,49 @@ struct KBField {
   double bx, by, bz;
   const double4 *B4;
   double k2u;   // uniform B: k1*2/(1+(k1*|B|)^2), the same for every particle
+  // map on its own grid (lepic_bfield mode 3, run as mode 2 with own_grid)
+  int own_grid;
+  int nB1, nB2, nB3;
+  int64_t sBy, sBz;
+  double hB1, hB2, hB3;
+  QDiv qB1, qB2, qB3;
 };
+
+// The map's stencil at (x, y, z) when it lives on its own grid
+// (Src/part_expmover.f90:149-175, flag_gridB = 1): lower node
+// int(x/hB) + 1, kept inside the map's nodes 0..nB+1 so that the stencil
+// stays in the array (positions inside the domain never need it).
+__device__ inline void b_stencil(const KBField &bf, double xp, double yp, double zp, Stencil &s) {
+  int ix = (int)fmax(fmin(qdiv(xp, bf.qB1), 2.0e9), -2.0e9) + 1;
+  int iy = (int)fmax(fmin(qdiv(yp, bf.qB2), 2.0e9), -2.0e9) + 1;
+  int iz = (int)fmax(fmin(qdiv(zp, bf.qB3), 2.0e9), -2.0e9) + 1;
+  ix = imax(0, imin(bf.nB1 + 1, ix));
+  iy = imax(0, imin(bf.nB2 + 1, iy));
+  iz = imax(0, imin(bf.nB3 + 1, iz));
+  const double px = qdiv((double)ix * bf.hB1 - xp, bf.qB1);
+  const double py = qdiv((double)iy * bf.hB2 - yp, bf.qB2);
+  const double pz = qdiv((double)iz * bf.hB3 - zp, bf.qB3);
+  const double wx2 = 1.0 - px, wy2 = 1.0 - py, wz2 = 1.0 - pz;
+  s.w[0] = px * py * pz;
+  s.w[1] = wx2 * py * pz;
+  s.w[2] = wx2 * wy2 * pz;
+  s.w[3] = px * wy2 * pz;
+  s.w[4] = px * py * wz2;
+  s.w[5] = wx2 * py * wz2;
+  s.w[6] = wx2 * wy2 * wz2;
+  s.w[7] = px * wy2 * wz2;
+  s.base = ix + bf.sBy * iy + bf.sBz * iz;
+}
 
 // BM is the B mode when known at compile time (the kernels are instantiated
 // per mode, so a run with uniform B carries no registers for the map
 // gather), -1 to read it from bf.
+// xp, yp, zp: the position the fields are gathered at (the map on its own
+// grid takes its stencil from it).
 template <int BM>
 __device__ inline void velocity_update(const KGrid &g, const KBField &bf, const Stencil &s,
                                        double k1, double Ex, double Ey, double Ez, double &vx,
-                                       double &vy, double &vz) {
+                                       double &vy, double &vz, double xp, double yp, double zp) {
   const int mode = BM >= 0 ? BM : bf.mode;
   int use_b = 0;
   double Bx = 0.0, By = 0.0, Bz = 0.0;
@@ -256,7 +290,23 @@ __device__ inline void velocity_update(const KGrid &g, const KBField &bf, const 
     Bx = bf.bx; By = bf.by; Bz = bf.bz;
     use_b = 1;
   } else if (mode == 2) {
-    if (bf.B4[s.base].w > 1.0e-5) {
+    if (bf.own_grid) {
+      Stencil sb;
+      b_stencil(bf, xp, yp, zp, sb);
+      if (bf.B4[sb.base].w > 1.0e-5) {
+        const int64_t b0 = sb.base, b1 = sb.base + bf.sBy, b2 = sb.base + bf.sBz,
+                      b3 = sb.base + bf.sBz + bf.sBy;
+        const double4 f1 = bf.B4[b0], f2 = bf.B4[b0 + 1], f3 = bf.B4[b1 + 1], f4 = bf.B4[b1];
+        const double4 f5 = bf.B4[b2], f6 = bf.B4[b2 + 1], f7 = bf.B4[b3 + 1], f8 = bf.B4[b3];
+        Bx = sb.w[0] * f1.x + sb.w[1] * f2.x + sb.w[2] * f3.x + sb.w[3] * f4.x +
+             sb.w[4] * f5.x + sb.w[5] * f6.x + sb.w[6] * f7.x + sb.w[7] * f8.x;
+        By = sb.w[0] * f1.y + sb.w[1] * f2.y + sb.w[2] * f3.y + sb.w[3] * f4.y +
+             sb.w[4] * f5.y + sb.w[5] * f6.y + sb.w[6] * f7.y + sb.w[7] * f8.y;
+        Bz = sb.w[0] * f1.z + sb.w[1] * f2.z + sb.w[2] * f3.z + sb.w[3] * f4.z +
+             sb.w[4] * f5.z + sb.w[5] * f6.z + sb.w[6] * f7.z + sb.w[7] * f8.z;
+        use_b = 1;
+      }
+    } else if (bf.B4[s.base].w > 1.0e-5) {
       gather3(bf.B4, g, s, Bx, By, Bz);
       use_b = 1;
     }
@@ -288,6 +338,21 @@ __device__ inline void velocity_update(const KGrid &g, const KBField &bf, const 
 static inline KBField make_kbfield(const lepic_bfield *bf, double k1) {
   KBField k;
   k.mode = bf ? bf->mode : 0;
+  k.own_grid = 0;
+  k.nB1 = k.nB2 = k.nB3 = 0;
+  k.sBy = k.sBz = 0;
+  k.hB1 = k.hB2 = k.hB3 = 1.0;
+  if (k.mode == 3) {   // the map on its own grid runs in the map kernels
+    k.mode = 2;
+    k.own_grid = 1;
+    k.nB1 = bf->nB[0]; k.nB2 = bf->nB[1]; k.nB3 = bf->nB[2];
+    k.sBy = (int64_t)k.nB1 + 3;
+    k.sBz = ((int64_t)k.nB1 + 3) * ((int64_t)k.nB2 + 3);
+    k.hB1 = bf->hB[0]; k.hB2 = bf->hB[1]; k.hB3 = bf->hB[2];
+  }
+  k.qB1 = qdiv_make(k.hB1);
+  k.qB2 = qdiv_make(k.hB2);
+  k.qB3 = qdiv_make(k.hB3);
   k.magnetized = bf ? bf->magnetized : 1;
   k.bx = bf ? bf->B[0] : 0.0; k.by = bf ? bf->B[1] : 0.0; k.bz = bf ? bf->B[2] : 0.0;
   k.B4 = bf ? reinterpret_cast<const double4 *>(bf->B4) : nullptr;

@@ -98,7 +98,7 @@ __global__ __launch_bounds__(BLOCK) void k_move_boris(KGrid g, KPset p, int64_t 
   make_stencil(g, xp, yp, zp, s);
   double Ex, Ey, Ez;
   gather3(E4, g, s, Ex, Ey, Ez);
-  velocity_update<BM>(g, bf, s, k1, Ex, Ey, Ez, vx, vy, vz);
+  velocity_update<BM>(g, bf, s, k1, Ex, Ey, Ez, vx, vy, vz, xp, yp, zp);
   p.vx[i] = vx; p.vy[i] = vy; p.vz[i] = vz;
   p.x[i] = xp + dt * vx; p.y[i] = yp + dt * vy; p.z[i] = zp + dt * vz;
 }
@@ -859,7 +859,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_nodep(
     make_stencil(g, xp, yp, zp, s);
     double Ex, Ey, Ez;
     gather3(E4, g, s, Ex, Ey, Ez);
-    velocity_update<BM>(g, bf, s, k1, Ex, Ey, Ez, vx, vy, vz);
+    velocity_update<BM>(g, bf, s, k1, Ex, Ey, Ez, vx, vy, vz, xp, yp, zp);
     xp = xp + dt * vx; yp = yp + dt * vy; zp = zp + dt * vz;
     if (wall_test_tab(g, wt, bp, xp, yp, zp, label)) {
       p.dead[i] = 2;
@@ -941,7 +941,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(
                         s.w[4] * f5.y + s.w[5] * f6.y + s.w[6] * f7.y + s.w[7] * f8.y;
       const double Ez = s.w[0] * f1.z + s.w[1] * f2.z + s.w[2] * f3.z + s.w[3] * f4.z +
                         s.w[4] * f5.z + s.w[5] * f6.z + s.w[6] * f7.z + s.w[7] * f8.z;
-      velocity_update<BM>(g, bf, s, k1, Ex, Ey, Ez, ux, uy, uz);
+      velocity_update<BM>(g, bf, s, k1, Ex, Ey, Ez, ux, uy, uz, xp, yp, zp);
       xp = xp + dt * ux; yp = yp + dt * uy; zp = zp + dt * uz;
       vx[q] = ux; vy[q] = uy; vz[q] = uz;
       // one set of quotients of the new position serves the wall test and
@@ -1005,7 +1005,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_bc_deposit(
     make_stencil(g, xp, yp, zp, s);
     double Ex, Ey, Ez;
     gather3(E4, g, s, Ex, Ey, Ez);
-    velocity_update<BM>(g, bf, s, k1, Ex, Ey, Ez, vx, vy, vz);
+    velocity_update<BM>(g, bf, s, k1, Ex, Ey, Ez, vx, vy, vz, xp, yp, zp);
     xp = xp + dt * vx; yp = yp + dt * vy; zp = zp + dt * vz;
     lost = wall_test(g, bcnd, bp, xp, yp, zp, label) != 0;
     if (!lost) {
@@ -1512,6 +1512,15 @@ int lepic_replica_flush(lepic_ctx *ctx, int64_t nodes, double *np_local) {
   return LEPIC_OK;
 }
 
+// A map on its own grid needs a sane grid.
+static int check_bgrid(lepic_ctx *ctx, const lepic_bfield *bf, const char *who) {
+  if (!bf || bf->mode != 3) return LEPIC_OK;
+  for (int d = 0; d < 3; ++d)
+    if (bf->nB[d] < 1 || !qdiv_divisor_ok(bf->hB[d]))
+      return lepic_fail(ctx, LEPIC_ERR_ARG, std::string(who) + ": bad B grid");
+  return LEPIC_OK;
+}
+
 // Dielectric charging needs its tables (flag_die = 1).
 static int check_die(lepic_ctx *ctx, const lepic_bc_params *bp, const char *who) {
   if (bp->flag_die != 1) return LEPIC_OK;
@@ -1543,8 +1552,10 @@ int lepic_move_boris(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *p, const d
   int st;
   if ((st = check_grid(ctx, g)) || (st = check_pset(ctx, p, "move_boris"))) return st;
   if (!E4) return lepic_fail(ctx, LEPIC_ERR_ARG, "move_boris: null E");
-  if (bf && bf->mode == 2 && !bf->B4) return lepic_fail(ctx, LEPIC_ERR_ARG, "move_boris: null B map");
-  if (bf && (bf->mode < 0 || bf->mode > 2)) return lepic_fail(ctx, LEPIC_ERR_ARG, "move_boris: bad mode");
+  if (bf && (bf->mode == 2 || bf->mode == 3) && !bf->B4)
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "move_boris: null B map");
+  if ((st = check_bgrid(ctx, bf, "move_boris"))) return st;
+  if (bf && (bf->mode < 0 || bf->mode > 3)) return lepic_fail(ctx, LEPIC_ERR_ARG, "move_boris: bad mode");
   if (p->n == 0) return LEPIC_OK;
   double k1 = dt * q / (2.0 * m);
   const int bmode = bf ? bf->mode : 0;
@@ -1628,10 +1639,11 @@ int lepic_push_bc_deposit_inplace(lepic_ctx *ctx, const lepic_grid *g, lepic_pse
   if (!E4 || !bcnd || !bp || !p_mac || !P_loss || (np_local && !kq))
     return lepic_fail(ctx, LEPIC_ERR_ARG, "push_bc_deposit_inplace: null argument");
   if ((st = check_die(ctx, bp, "push_bc_deposit_inplace"))) return st;
-  if (bf && bf->mode == 2 && !bf->B4)
+  if (bf && (bf->mode == 2 || bf->mode == 3) && !bf->B4)
     return lepic_fail(ctx, LEPIC_ERR_ARG, "push_bc_deposit_inplace: null B map");
-  if (bf && (bf->mode < 0 || bf->mode > 2))
+  if (bf && (bf->mode < 0 || bf->mode > 3))
     return lepic_fail(ctx, LEPIC_ERR_ARG, "push_bc_deposit_inplace: bad B mode");
+  if ((st = check_bgrid(ctx, bf, "push_bc_deposit_inplace"))) return st;
   const int bmode = bf ? bf->mode : 0;
   const int64_t nodes = grid_nodes(g);
   double *rep = nullptr;
@@ -1714,10 +1726,11 @@ int lepic_push_bc_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset 
   if (!E4 || !bcnd || !bp || !p_mac || !P_loss || (np_local && !kq))
     return lepic_fail(ctx, LEPIC_ERR_ARG, "push_bc_deposit: null argument");
   if ((st = check_die(ctx, bp, "push_bc_deposit"))) return st;
-  if (bf && bf->mode == 2 && !bf->B4)
+  if (bf && (bf->mode == 2 || bf->mode == 3) && !bf->B4)
     return lepic_fail(ctx, LEPIC_ERR_ARG, "push_bc_deposit: null B map");
-  if (bf && (bf->mode < 0 || bf->mode > 2))
+  if (bf && (bf->mode < 0 || bf->mode > 3))
     return lepic_fail(ctx, LEPIC_ERR_ARG, "push_bc_deposit: bad B mode");
+  if ((st = check_bgrid(ctx, bf, "push_bc_deposit"))) return st;
   const int bmode = bf ? bf->mode : 0;
   if (dst->cap < src->n)
     return lepic_fail(ctx, LEPIC_ERR_CAPACITY, "push_bc_deposit: dst too small");

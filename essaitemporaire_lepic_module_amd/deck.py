@@ -894,8 +894,6 @@ class Deck:
             out.append("restart")
         if c.jne > 0.0:
             out.append("negative-ion surface production")
-        if c.flag_B and self.bfield.n_B != (1, 1, 1) and tuple(self.bfield.n_B) != tuple(c.n):
-            out.append("B map on its own grid")
         if c.np_cell <= 0:
             out.append("no initial plasma (np_cell <= 0)")
         return out
@@ -911,19 +909,24 @@ class Deck:
         from . import chemistry as chm
         species = chem.species
         tracked = [s for s in species if s.tracked]
-        B_uniform = B_map = None
+        B_uniform = B_map = B_grid = None
         if c.flag_B:
-            if self.bfield.n_B == (1, 1, 1):
+            # part_expmover.f90:143-175: n_B(1) = 1 is a constant field, a map
+            # of the field grid's size shares its stencil, any other map has
+            # its own (main.f90:368-370)
+            if self.bfield.n_B[0] == 1:
                 B_uniform = tuple(float(v) for v in self.bfield.Bi[1, 1, 1, :3])
-            elif tuple(self.bfield.n_B) == tuple(c.n):
+            else:
                 B_map = np.ascontiguousarray(self.bfield.Bi)
+                if tuple(self.bfield.n_B) != tuple(c.n):
+                    B_grid = (tuple(self.bfield.n_B), tuple(self.bfield.h_B))
         case = cfg.Case(
             name=os.path.basename(os.path.normpath(self.directory)) or "deck",
             n=tuple(c.n), h=b.h, bcnd=b.bcnd, ngrid=c.ngrid,
             electrode_V={k + 1: v for k, v in enumerate(self.V)},
             species=species, n0=c.n0, np_cell=c.np_cell, dt=self.params.dt, steps=steps,
             eps=c.eps, omega=c.omega, ng=c.ng, flag_pbc=b.flag_pbc, flag_nmn=b.flag_nmn,
-            B_uniform=B_uniform, B_map=B_map, ions_magnetized=(c.flag_B_pos == 0),
+            B_uniform=B_uniform, B_map=B_map, B_grid=B_grid, ions_magnetized=(c.flag_B_pos == 0),
             ngas=c.ngas, nb_step_sort=self.params.nb_step_sort,
             nb_step_collisions=self.params.nb_step_collisions)
         case.phi_init = b.u

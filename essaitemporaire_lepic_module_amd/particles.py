@@ -151,13 +151,24 @@ def move_particles_electrostatic(part: ParticleSet, n, h, E4: torch.Tensor, q: f
 
 
 def bfield(mode: int = 0, B=(0.0, 0.0, 0.0), B4: torch.Tensor | None = None,
-           magnetized: bool = True) -> _native.BField:
+           magnetized: bool = True, nB=None, hB=None) -> _native.BField:
+    """Magnetic field of the push: mode 0 none, 1 uniform B, 2 map B4 on the
+    field grid, 3 map B4 on its own grid of nB cells of size hB (B4 then has
+    (nB3+3, nB2+3, nB1+3, 4) doubles)."""
     bf = _native.BField()
     bf.mode = int(mode)
     for i in range(3):
         bf.B[i] = float(B[i])
     bf.B4 = ptr(B4) if B4 is not None else None
     bf.magnetized = 1 if magnetized else 0
+    if mode == 3:
+        if nB is None or hB is None:
+            raise ValueError("a map on its own grid needs nB and hB")
+        if B4 is not None and tuple(B4.shape) != (nB[2] + 3, nB[1] + 3, nB[0] + 3, 4):
+            raise ValueError("B4 must be (nB3+3, nB2+3, nB1+3, 4)")
+        for i in range(3):
+            bf.nB[i] = int(nB[i])
+            bf.hB[i] = float(hB[i])
     bf._keep = B4
     return bf
 

@@ -102,7 +102,12 @@ class Simulation:
         for q in self.charge:
             if case.B_map is not None:
                 self.B4 = as_device(case.B_map, torch.float64, dev)
-                self.bf.append(P.bfield(2, B4=self.B4, magnetized=(q <= 0 or case.ions_magnetized)))
+                if case.B_grid is not None:
+                    self.bf.append(P.bfield(3, B4=self.B4, magnetized=(q <= 0 or case.ions_magnetized),
+                                            nB=case.B_grid[0], hB=case.B_grid[1]))
+                else:
+                    self.bf.append(P.bfield(2, B4=self.B4,
+                                            magnetized=(q <= 0 or case.ions_magnetized)))
             elif case.B_uniform is not None:
                 self.bf.append(P.bfield(1, case.B_uniform, magnetized=(q <= 0 or case.ions_magnetized)))
             else:

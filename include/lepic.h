@@ -82,10 +82,14 @@ typedef struct {
 
 /* Magnetic field of the Boris push (Src/part_expmover.f90:139-212). */
 typedef struct {
-  int32_t mode;            /* 0 none (leapfrog), 1 uniform, 2 map on the E grid */
+  int32_t mode;            /* 0 none (leapfrog), 1 uniform, 2 map on the E grid,
+                              3 map on its own grid (flag_gridB = 1)   */
   double B[3];             /* uniform field (mode 1)                  */
-  const double *B4;        /* device map, 4 doubles per node (mode 2) */
+  const double *B4;        /* device map, 4 doubles per node (modes 2, 3); mode 3:
+                              (nB1+3)(nB2+3)(nB3+3) nodes, x fastest  */
   int32_t magnetized;      /* 0: this species ignores B (flag_B_pos)  */
+  int32_t nB[3];           /* mode 3: cells of the B grid             */
+  double hB[3];            /* mode 3: its cell size                   */
 } lepic_bfield;
 
 typedef struct {

@@ -295,9 +295,9 @@ void lo_move_electrostatic(const lo_grid *g, lo_pset *p, const double *E4, doubl
  * the modular gather above.  b_mode 1 = constant B (n_B == 1, :142-148);
  * b_mode 2 = map on the E grid (flag_gridB = 0) gated by Bi(4) > 1e-5 T at the
  * lower node (:151-207).  magnetized == 0 reproduces flag_B_pos for ions (:212). */
-void lo_move_boris(const lo_grid *g, lo_pset *p, const double *E4, int b_mode,
-                   const double *Bconst, const double *B4, double q, double m,
-                   double dt, int magnetized) {
+static void move_boris(const lo_grid *g, lo_pset *p, const double *E4, int b_mode,
+                       const double *Bconst, const double *B4, const lo_grid *gb, double q,
+                       double m, double dt, int magnetized) {
   double k1 = dt * q / (2.0 * m);
   for (int64_t i = 0; i < p->n; ++i) {
     double xp = p->x[i], yp = p->y[i], zp = p->z[i];
@@ -313,6 +313,16 @@ void lo_move_boris(const lo_grid *g, lo_pset *p, const double *E4, int b_mode,
     } else if (b_mode == 2) {
       if (B4[4 * s.node[0] + 3] > 1.0e-5) {
         Bx = gather4(B4, &s, 0); By = gather4(B4, &s, 1); Bz = gather4(B4, &s, 2);
+        use_b = 1;
+      }
+    } else if (b_mode == 3) {
+      /* the map on its own grid (part_expmover.f90:149-175, flag_gridB = 1):
+       * lower node int(x/hB) + 1 -- inside the domain the same as the
+       * clamped stencil of that grid */
+      stencil8 sb;
+      stencil_at(gb, xp, yp, zp, &sb);
+      if (B4[4 * sb.node[0] + 3] > 1.0e-5) {
+        Bx = gather4(B4, &sb, 0); By = gather4(B4, &sb, 1); Bz = gather4(B4, &sb, 2);
         use_b = 1;
       }
     }
@@ -335,6 +345,17 @@ void lo_move_boris(const lo_grid *g, lo_pset *p, const double *E4, int b_mode,
     p->x[i] = xp + dt * vx; p->y[i] = yp + dt * vy; p->z[i] = zp + dt * vz;
     p->cell[i] = 0;
   }
+}
+
+void lo_move_boris(const lo_grid *g, lo_pset *p, const double *E4, int b_mode,
+                   const double *Bconst, const double *B4, double q, double m,
+                   double dt, int magnetized) {
+  move_boris(g, p, E4, b_mode, Bconst, B4, NULL, q, m, dt, magnetized);
+}
+
+void lo_move_boris_bgrid(const lo_grid *g, lo_pset *p, const double *E4, const double *B4,
+                         const lo_grid *gb, double q, double m, double dt, int magnetized) {
+  move_boris(g, p, E4, 3, NULL, B4, gb, q, m, dt, magnetized);
 }
 
 /* ===================================================================== */

@@ -77,6 +77,10 @@ void lo_move_boris(const lo_grid *g, lo_pset *p, const double *E4, int b_mode,
                    const double *Bconst, const double *B4, double q, double m,
                    double dt, int magnetized);
 
+/* b_mode 3: the map B4 on its own grid gb (4 doubles per node of gb). */
+void lo_move_boris_bgrid(const lo_grid *g, lo_pset *p, const double *E4, const double *B4,
+                         const lo_grid *gb, double q, double m, double dt, int magnetized);
+
 /* ---- wall BC + compaction (modules/particles/mod_particleBC.f90) ---- */
 /* p_mac: 2*(ngrid+1) doubles, p_mac[2*label+0] += qmacro,
  * p_mac[2*label+1] += |qmacro|*Ek_eV.  Returns number removed via *np_lost. */

@@ -98,6 +98,9 @@ def lib():
                                             C.c_double, C.c_double]
         L.lo_move_boris.argtypes = [C.POINTER(Grid), C.POINTER(PSet), dp, C.c_int, dp, dp,
                                     C.c_double, C.c_double, C.c_double, C.c_int]
+        L.lo_move_boris_bgrid.argtypes = [C.POINTER(Grid), C.POINTER(PSet), dp, dp,
+                                          C.POINTER(Grid), C.c_double, C.c_double, C.c_double,
+                                          C.c_int]
         L.lo_apply_bc.restype = C.c_int
         L.lo_apply_bc.argtypes = [C.POINTER(Grid), C.POINTER(PSet), ip, C.c_double, C.c_double,
                                   C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
@@ -236,6 +239,12 @@ def move_boris(n, h, p: Particles, E4, b_mode, Bconst, B4, q, m, dt, magnetized=
     lib().lo_move_boris(C.byref(g), C.byref(p.s), _p(np.ascontiguousarray(E4)), int(b_mode),
                         _p(Bc), _p(B4a), q, m, dt, 1 if magnetized else 0)
 
+
+def move_boris_bgrid(n, h, p, E4, B4, nB, hB, q, m, dt, magnetized=True):
+    """Boris push with the B map on its own grid of nB cells of size hB."""
+    g, gb = grid(n, h), grid(nB, hB)
+    lib().lo_move_boris_bgrid(C.byref(g), C.byref(p.s), _p(E4), _p(np.ascontiguousarray(B4)),
+                              C.byref(gb), q, m, dt, int(bool(magnetized)))
 
 def apply_bc(n, h, p: Particles, bcnd, flag_pbc, flag_nmn, ptype, tag_neg, qmacro, mass, Nm, ngrid,
              dtype=None, sum_q_xz=None, sum_q_yz=None):
@@ -440,7 +449,10 @@ class OracleSim:
         E4 = efield(n, h, self.phi, self.bcnd)
         for s, p in enumerate(self.parts):
             q, m = self.charge[s], self.mass[s]
-            if c.B_map is not None:
+            if c.B_map is not None and c.B_grid is not None:
+                move_boris_bgrid(n, h, p, E4, c.B_map, c.B_grid[0], c.B_grid[1], q, m, c.dt,
+                                 q <= 0 or c.ions_magnetized)
+            elif c.B_map is not None:
                 move_boris(n, h, p, E4, 2, None, c.B_map, q, m, c.dt, q <= 0 or c.ions_magnetized)
             elif c.B_uniform is not None:
                 move_boris(n, h, p, E4, 1, c.B_uniform, None, q, m, c.dt, q <= 0 or c.ions_magnetized)

@@ -364,11 +364,13 @@ def test_reference_deck_params_and_case():
 @need_ref
 def test_unsupported_features_refused():
     d = D.load_deck(os.path.join(REF, "Thruster"))
-    assert d.unsupported() == ["particle injection", "B map on its own grid"]
+    assert d.unsupported() == ["particle injection"]
     with pytest.raises(D.DeckError, match="injection"):
         d.case()
     case = d.case(allow_unsupported=True)
     assert case.flag_pbc == 1 and case.flag_die == 1 and list(case.dtype) == [0, 1, 1, 2, 3, 2]
+    # the Hall-thruster field on its 128 x 256 x 96 map, the field grid being 256 x 512 x 192
+    assert case.B_grid[0] == (128, 256, 96) and case.B_map.shape == (99, 259, 131, 4)
 
 
 @need_ref

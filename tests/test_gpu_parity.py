@@ -70,6 +70,31 @@ def test_move_boris_bitexact(gpu, oracle, rng, mode):
             _same(g[k], o[k])
 
 
+@pytest.mark.parametrize("nB", [(5, 4, 3), (10, 8, 8)])
+def test_move_boris_bmap_own_grid_bitexact(gpu, oracle, rng, nB):
+    """B map on its own grid (lepic_bfield mode 3, the reference's
+    flag_gridB = 1), coarser than the field grid or of the same size."""
+    from essaitemporaire_lepic_module_amd import particles as P
+    n, h = H.box((10, 8, 8))
+    hB = tuple(n[d] * h[d] / nB[d] for d in range(3))
+    arr = H.random_particles(rng, n, h, 4000, vscale=1e6)
+    E4 = H.random_E4(rng, n)
+    shape = (nB[2] + 3, nB[1] + 3, nB[0] + 3)
+    B4 = np.zeros(shape + (4,))
+    B4[..., :3] = 0.02 * rng.standard_normal(shape + (3,))
+    B4[..., 3] = np.sqrt((B4[..., :3] ** 2).sum(-1))
+    B4[..., 3][rng.random(shape) < 0.3] = 0.0
+    for q, m, magnetized in ((-1.6e-19, 9.1e-31, True), (1.6e-19, 6.6e-26, False)):
+        ps = _gpu_set(P, arr)
+        bf = P.bfield(3, B4=torch.as_tensor(B4, device=gpu), magnetized=magnetized, nB=nB, hB=hB)
+        P.move_particles_boris(ps, n, h, torch.as_tensor(E4, device=gpu), bf, q, m, 5e-11)
+        po = _oracle_set(oracle, arr)
+        oracle.move_boris_bgrid(n, h, po, E4, B4, nB, hB, q, m, 5e-11, magnetized)
+        g, o = ps.to_host(), po.arrays()
+        for k in ("x", "y", "z", "vx", "vy", "vz"):
+            _same(g[k], o[k])
+
+
 def test_leapfrog_equals_electrostatic(gpu, rng):
     """Boris with B off is the modular leapfrog bit for bit (2*k1 == dt*q/m)."""
     from essaitemporaire_lepic_module_amd import particles as P

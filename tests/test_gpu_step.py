@@ -165,7 +165,8 @@ def test_deck_chemistry_production_steps(gpu):
         assert np.isfinite(a["vx"][live]).all()
 
 
-def test_deck_case_steps_match_oracle(gpu, oracle, tmp_path):
+@pytest.mark.parametrize("bgrid", [False, True])
+def test_deck_case_steps_match_oracle(gpu, oracle, tmp_path, bgrid):
     """A run deck in the reference's format -- periodic y-z walls (label 0), a
     biased electrode, a Gaussian Bz map on the field grid, argon chemistry,
     electron heating in a window every 4 steps -- drives the same steps on
@@ -182,7 +183,8 @@ def test_deck_case_steps_match_oracle(gpu, oracle, tmp_path):
     n = (16, 8, 8)
     c = D.Conditions(rname="chem.dat", Te=3.0, ng=3, eps=1e-8, omega=1.4, n=n, kt=0.3, n0=1e15,
                      ngas=3e20, np_cell=6, ngrid=2, nsav=100, cnt_plt=1, Pabs=0.05, nu_h=1e8,
-                     xl_pow=2e-2, xr_pow=7e-2, I_inj=-1.0, nB=1, flag_B=1, n_B=n)
+                     xl_pow=2e-2, xr_pow=7e-2, I_inj=-1.0, nB=1, flag_B=1,
+                     n_B=(6, 4, 3) if bgrid else n)
     c.B[0] = D.BSource("n", "gauss", 1.0, 0.02, "Bz", 3.0e-2, 5.0e-2)
     (tmp_path / "conditions.inp").write_text(D.write_conditions(c))
     (tmp_path / "geometry.inp").write_text(
@@ -191,6 +193,7 @@ def test_deck_case_steps_match_oracle(gpu, oracle, tmp_path):
     d = D.load_deck(str(tmp_path))
     case = dataclasses.replace(d.case(steps=6), nb_step_sort=4, nb_step_collisions=2)
     assert case.B_map is not None and case.flag_pbc == 1
+    assert (case.B_grid is not None) == bgrid
     ref, sim = _start(oracle, case, P, lazy=False)
     for istep in range(1, 7):
         sim.advance_one_step(istep)

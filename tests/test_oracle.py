@@ -476,3 +476,35 @@ def test_dielectric_phi_closed_form(oracle):
     np.testing.assert_allclose(phi, want, rtol=1e-15, atol=0)
     changed = phi != before
     assert changed.sum() == 3
+
+
+def test_boris_bmap_on_its_own_grid(oracle):
+    """A B map on its own (coarser, shifted-size) grid: with a field linear in
+    x the trilinear gather returns B at the particle, so one step equals the
+    uniform-field step with that B (to rounding); E = 0."""
+    n, h = (8, 6, 10), (1e-3, 1e-3, 1e-3)
+    nB = (5, 3, 4)
+    hB = (n[0] * h[0] / nB[0], n[1] * h[1] / nB[1], n[2] * h[2] / nB[2])
+    B4 = np.zeros((nB[2] + 3, nB[1] + 3, nB[0] + 3, 4))
+    xb = (np.arange(nB[0] + 3) - 1) * hB[0]
+    B4[..., 2] = 0.05 + 3.0 * xb[None, None, :]        # Bz(x)
+    B4[..., 3] = np.abs(B4[..., 2])
+    E4 = np.zeros(cfg.node_shape(n) + (4,))
+    rng = np.random.default_rng(9)
+    m = 200
+    L = np.array(n) * np.array(h)
+    pos = [rng.uniform(0.05 * L[d], 0.95 * L[d], m) for d in range(3)]
+    vel = [rng.normal(0, 1e5, m) for _ in range(3)]
+    q, mass, dt = -1.60217646e-19, 9.10938188e-31, 1e-11
+    p = oracle.Particles(m)
+    p.set(*pos, *vel)
+    oracle.move_boris_bgrid(n, h, p, E4, B4, nB, hB, q, mass, dt)
+    got = p.arrays()
+    for i in range(0, m, 17):
+        one = oracle.Particles(1)
+        one.set(*([a[i]] for a in pos), *([a[i]] for a in vel))
+        Bz = 0.05 + 3.0 * pos[0][i]
+        oracle.move_boris(n, h, one, E4, 1, (0.0, 0.0, Bz), None, q, mass, dt)
+        o = one.arrays()
+        for k in ("vx", "vy", "vz"):
+            assert got[k][i] == pytest.approx(o[k][0], rel=1e-12, abs=1e-9)
