@@ -927,7 +927,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(
       double ux = vx[q], uy = vy[q], uz = vz[q];
       Stencil s;
       make_stencil_pre(g, xp, yp, zp,
-                       qdiv_coord_ok(xp) && qdiv_coord_ok(yp) && qdiv_coord_ok(zp), s);
+                       qdiv_coords_ok(xp, yp, zp), s);
       if (s.base != eb) {
         eb = s.base;
         const int64_t b1 = eb + g.sy, b2 = eb + g.sz, b3 = eb + g.sz + g.sy;
@@ -946,7 +946,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(
       vx[q] = ux; vy[q] = uy; vz[q] = uz;
       // one set of quotients of the new position serves the wall test and
       // the deposit (formed again only if the periodic wrap moved it)
-      bool okn = qdiv_coord_ok(xp) && qdiv_coord_ok(yp) && qdiv_coord_ok(zp);
+      bool okn = qdiv_coords_ok(xp, yp, zp);
       double qx = qdiv_pre(xp, g.q1, okn), qy = qdiv_pre(yp, g.q2, okn),
              qz = qdiv_pre(zp, g.q3, okn);
       if (wall_test_tab_q(g, wt, bp, xp, qx, qy, qz, label)) {
@@ -957,7 +957,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(
         const double xw = xp, yw = yp, zw = zp;
         survivor_wrap(bp, xp, yp, zp, vx[q]);
         if (xp != xw || yp != yw || zp != zw) {
-          okn = qdiv_coord_ok(xp) && qdiv_coord_ok(yp) && qdiv_coord_ok(zp);
+          okn = qdiv_coords_ok(xp, yp, zp);
           qx = qdiv_pre(xp, g.q1, okn); qy = qdiv_pre(yp, g.q2, okn); qz = qdiv_pre(zp, g.q3, okn);
         }
         x[q] = xp; y[q] = yp; z[q] = zp;
@@ -1105,7 +1105,7 @@ struct BinShared {
 // 1-based cell of a live particle, clamped to [1, n] (cell_index_from_position,
 // mod_particle_sorting.f90:16-48)
 __device__ inline void bin_cell(const KGrid &g, double x, double y, double z, int c[3]) {
-  const bool ok = qdiv_coord_ok(x) && qdiv_coord_ok(y) && qdiv_coord_ok(z);
+  const bool ok = qdiv_coords_ok(x, y, z);
   c[0] = imax(1, imin(g.n1, (int)qdiv_pre(x, g.q1, ok) + 1));
   c[1] = imax(1, imin(g.n2, (int)qdiv_pre(y, g.q2, ok) + 1));
   c[2] = imax(1, imin(g.n3, (int)qdiv_pre(z, g.q3, ok) + 1));

@@ -64,8 +64,19 @@ LEPIC_HD inline bool qdiv_coord_ok(double x) {
   return x == 0.0 || (ax >= 1.0e-280 && ax <= 1.0e279);
 }
 
+// The same test for a particle's three coordinates at once, stricter only in
+// sending a particle with a coordinate exactly 0 to the plain division:
+// every magnitude in [1e-280, 1e279].  (A NaN coordinate may pass; both paths
+// then give NaN.)
+LEPIC_HD inline bool qdiv_coords_ok(double x, double y, double z) {
+  const double ax = std::fabs(x), ay = std::fabs(y), az = std::fabs(z);
+  const double mx = std::fmax(std::fmax(ax, ay), az);
+  const double mn = std::fmin(std::fmin(ax, ay), az);
+  return mn >= 1.0e-280 && mx <= 1.0e279;
+}
+
 // qdiv with the range test made once per particle: ok from qdiv_coord_ok of
-// the coordinates the quotients come from.
+// the coordinates the quotients come from (or qdiv_coords_ok of all three).
 LEPIC_HD inline double qdiv_pre(double x, const QDiv &d, bool ok) {
   if (!ok) return x / d.h;
   double q = x * d.y;

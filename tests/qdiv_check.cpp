@@ -63,6 +63,22 @@ int main(int argc, char **argv) {
     if (!same(qdiv(x, d), x / h)) {
       if (++bad < 10) std::printf("mismatch x=%.17g h=%.17g\n", x, h);
     }
+    // the per-particle form: one range test for three coordinates, then the
+    // quotients x/h and (i h - x)/h of the kernels' stencils
+    const double y = special[next64() % (sizeof(special) / sizeof(special[0]))] * (next64() & 1) +
+                     (u01() - 0.5) * 300.0 * h;
+    const double z = (t % 5 == 0) ? 0.0 : (u01() * 300.0) * h;
+    const bool ok = qdiv_coords_ok(x, y, z);
+    if (!same(qdiv_pre(x, d, ok), x / h)) {
+      if (++bad < 10) std::printf("mismatch (pre) x=%.17g h=%.17g\n", x, h);
+    }
+    const double q = x / h;
+    if (std::isfinite(q) && std::fabs(q) < 1e9) {
+      const double r = (double)((long)q + 1) * h - x;
+      if (!same(qdiv_pre(r, d, ok), r / h)) {
+        if (++bad < 10) std::printf("mismatch (frac) x=%.17g h=%.17g\n", x, h);
+      }
+    }
   }
   std::printf("checked %ld mismatches %ld\n", checked, bad);
   return bad != 0;
