@@ -23,6 +23,7 @@ import dataclasses
 import math
 import os
 import re
+import warnings
 
 import numpy as np
 
@@ -878,10 +879,13 @@ class Deck:
     bfield: MagneticMap
     params: SimParams
 
-    def unsupported(self) -> list:
-        """Features this deck turns on that the device step does not carry
-        (the PIC-MCC core: push, walls, deposit, Poisson, MCC)."""
-        c, b, out = self.conditions, self.boundary, []
+    def legacy_only(self) -> list:
+        """Features this deck sets that only the legacy driver (Src/main.f90)
+        runs: the modular step (mod_simulation.f90:340-472) has no particle
+        injection, secondary emission, RF or iterative electrode potentials,
+        negative-ion surface production or restart, so a deck asking for them
+        runs without them there, and here."""
+        c, out = self.conditions, []
         if c.flag_inj:
             out.append("particle injection")
         if c.gam_sec > 0.0:
@@ -894,17 +898,18 @@ class Deck:
             out.append("restart")
         if c.jne > 0.0:
             out.append("negative-ion surface production")
-        if c.np_cell <= 0:
-            out.append("no initial plasma (np_cell <= 0)")
         return out
 
-    def case(self, steps: int = MODULAR_STEPS, allow_unsupported: bool = False) -> cfg.Case:
-        """The deck as a Case for Simulation.  Features the device step does
-        not carry raise DeckError unless allow_unsupported, which leaves them
-        off."""
-        miss = self.unsupported()
-        if miss and not allow_unsupported:
-            raise DeckError("deck needs " + ", ".join(miss))
+    def case(self, steps: int = MODULAR_STEPS, strict: bool = False) -> cfg.Case:
+        """The deck as a Case for Simulation, run as the modular driver runs
+        it.  Features only the legacy driver has (legacy_only) are left off
+        with a warning, or refused with DeckError when strict."""
+        miss = self.legacy_only()
+        if miss:
+            if strict:
+                raise DeckError("deck asks for " + ", ".join(miss) +
+                                " (legacy driver only)")
+            warnings.warn("left off, as in the modular driver: " + ", ".join(miss), stacklevel=2)
         c, b, chem = self.conditions, self.boundary, self.chemistry
         from . import chemistry as chm
         species = chem.species

@@ -353,7 +353,7 @@ def test_reference_deck_params_and_case():
         assert p.Nm[k] * abs(c.np_cell) / (b.h[0] * b.h[1] * b.h[2]) == pytest.approx(dens, rel=1e-14)
     assert (p.nb_step_sort, p.nb_step_collisions) == (10, 10)
     assert p.nb_step_heating == (4 if c.kt >= 0.1 else 20 if c.kt >= 0.05 else 40)
-    assert d.unsupported() == []
+    assert d.legacy_only() == []
     case = d.case(steps=10)
     assert case.Nm == p.Nm and case.dt == p.dt
     assert case.vt0[:2] == p.vt0[:2]
@@ -362,12 +362,15 @@ def test_reference_deck_params_and_case():
 
 
 @need_ref
-def test_unsupported_features_refused():
+def test_legacy_only_features():
+    """Injection is a legacy-driver feature: the modular step (and a case)
+    runs the Thruster deck without it, with a warning, or refuses when strict."""
     d = D.load_deck(os.path.join(REF, "Thruster"))
-    assert d.unsupported() == ["particle injection"]
+    assert d.legacy_only() == ["particle injection"]
     with pytest.raises(D.DeckError, match="injection"):
-        d.case()
-    case = d.case(allow_unsupported=True)
+        d.case(strict=True)
+    with pytest.warns(UserWarning, match="injection"):
+        case = d.case()
     assert case.flag_pbc == 1 and case.flag_die == 1 and list(case.dtype) == [0, 1, 1, 2, 3, 2]
     # the Hall-thruster field on its 128 x 256 x 96 map, the field grid being 256 x 512 x 192
     assert case.B_grid[0] == (128, 256, 96) and case.B_map.shape == (99, 259, 131, 4)
@@ -406,7 +409,7 @@ def test_written_deck_loads(tmp_path):
     (tmp_path / "geometry.inp").write_text(SLAB_GEO)
     (tmp_path / "boundary.inp").write_text(D.write_boundary_potentials([0.0, 4.0]))
     d = D.load_deck(str(tmp_path))
-    assert d.unsupported() == []
+    assert d.legacy_only() == []
     case = d.case(steps=5)
     assert case.n == (16, 8, 8) and case.flag_pbc == 1
     assert [s.name for s in case.tracked] == ["e", "Ar+"]
