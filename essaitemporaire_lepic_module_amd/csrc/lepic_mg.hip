@@ -178,16 +178,27 @@ __device__ inline double resid_node(const KLevel &L, int64_t t) {
   return a;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_resid(KLevel L, double *partials, const MGCtl *ctl) {
+// At most RESID_BLOCKS blocks, each thread striding over the nodes, so that
+// the one-block decision sums few partials.  The order of the sum is fixed
+// for a level (deterministic from run to run).
+constexpr int RESID_BLOCKS = 1024;
+
+__global__ __launch_bounds__(BLOCK) void k_resid(KLevel L, double *partials, const MGCtl *ctl,
+                                                 int64_t nres) {
   if (ctl->active == 0) return;
-  __shared__ double red[BLOCK];
-  red[threadIdx.x] = resid_node(L, blockIdx.x * (int64_t)BLOCK + threadIdx.x);
+  __shared__ double red[BLOCK / LEPIC_WAVE];
+  double a = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * BLOCK;
+  for (int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x; t < nres; t += stride)
+    a = a + resid_node(L, t);
+  for (int d = LEPIC_WAVE / 2; d > 0; d >>= 1) a = a + __shfl_down(a, d);
+  if (__lane_id() == 0) red[threadIdx.x / LEPIC_WAVE] = a;
   __syncthreads();
-  for (int s = BLOCK / 2; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = 0.0;
+    for (int w = 0; w < BLOCK / LEPIC_WAVE; ++w) b = b + red[w];
+    partials[blockIdx.x] = b;
   }
-  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
 }
 
 // Sum the partials in a fixed order and apply sor_rb's exit rules
@@ -647,7 +658,8 @@ struct Solver {
     const KGrid &g = Lv.g;
     int64_t nsor = (int64_t)(g.n1 / 2 + 1) * (g.n2 + 1) * (g.n3 + 1);
     int64_t nres = (int64_t)(g.n1 + 1) * (g.n2 + 1) * (g.n3 + 1);
-    unsigned bs = grid_blocks(nsor, BLOCK), br = grid_blocks(nres, BLOCK);
+    unsigned bs = grid_blocks(nsor, BLOCK);
+    const unsigned br = std::min<unsigned>(grid_blocks(nres, BLOCK), RESID_BLOCKS);
     hipStream_t s = ctx->stream;
     for (int colour = 0; colour < 2; ++colour) {
       hipLaunchKernelGGL(k_sor, dim3(bs), dim3(BLOCK), 0, s, Lv, colour, omega, &ctl[l]);
@@ -658,7 +670,7 @@ struct Solver {
                            dim3(BLOCK), 0, s, Lv, &ctl[l]);
       }
     }
-    hipLaunchKernelGGL(k_resid, dim3(br), dim3(BLOCK), 0, s, Lv, partials, &ctl[l]);
+    hipLaunchKernelGGL(k_resid, dim3(br), dim3(BLOCK), 0, s, Lv, partials, &ctl[l], nres);
     hipLaunchKernelGGL(k_decide, dim3(1), dim3(BLOCK), 0, s, partials, (int64_t)br, &ctl[l], eps,
                        dig, nr);
   }
