@@ -148,7 +148,7 @@ def main():
                          "calibration kernel for rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE runs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        print(json.dumps({"error": "this library runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
 
     import torch

@@ -101,8 +101,8 @@ def _write_plane(filename: str, f2: np.ndarray, na: int, nb: int, every: int):
     """Rows b = nb+1 down to 1 (step every), values a = 1..na+1 (step every),
     as ES18.10 separated by one blank (the 1X after the last value moves the
     position only, so no record ends in a blank); the header is a
-    list-directed write of the two point counts (written as the Fortran
-    run-time of this build writes it: one blank before each value)."""
+    list-directed write of the two point counts (written as amdflang's
+    run-time writes it: one blank before each value)."""
     rows = [f" {npoints_1_to_np1(na, every)} {npoints_1_to_np1(nb, every)}"]
     for b in range(nb + 1, 0, -every):
         rows.append(" ".join(fortran_es(f2[b, a], 18, 10) for a in range(1, na + 2, every)))
