@@ -64,7 +64,7 @@ __device__ inline void l2_atomic_add(double *p, double v) {
 }
 
 double *lepic_replicas(lepic_ctx *ctx, int64_t nodes);
-int lepic_replica_flush(lepic_ctx *ctx, int64_t nodes, double *np_local);
+int lepic_replica_flush(lepic_ctx *ctx, int64_t nodes, double *np_local, const double *kq);
 
 // control block word offsets (int64 units)
 enum : int {

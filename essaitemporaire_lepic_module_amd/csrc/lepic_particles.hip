@@ -445,7 +445,9 @@ __device__ inline bool deposit_weights(const KGrid &g, double k1, double x, doub
 // segmented shuffle scan, so one lane per run issues the 8 LDS atomics; the
 // box is then flushed with one L2-resident atomic per touched node into this
 // XCD's replica.  Particles whose stencil leaves the box go straight to the
-// replica.  kq is 1 or 2, so kq*sum(w) equals sum(kq*w) term for term.
+// replica.  The replicas hold sum(w); k_rep_flush multiplies each node's
+// total by kq once.  kq is 1 or 2, so kq*sum(w) equals sum(kq*w) term for
+// term.
 template <int NB>
 struct DepSharedT {
   static constexpr int CAP = NB;
@@ -528,7 +530,7 @@ __device__ inline void box_begin(const KGrid &g, DS &ds, int lo[3], int hi[3]) {
 // with a unique key and no weight).
 template <class DS>
 __device__ inline void box_add(const KGrid &g, DS &ds, bool has, int ix, int iy, int iz,
-                               double w[8], const double *__restrict__ kq, double *rep) {
+                               double w[8], double *rep) {
   int l = -1;
   if (has && ds.vol > 0) {
     int lx = ix - ds.x0, ly = iy - ds.y0, lz = iz - ds.z0;
@@ -540,7 +542,7 @@ __device__ inline void box_add(const KGrid &g, DS &ds, bool has, int ix, int iy,
     const int64_t nd[8] = {b, b + 1, b + g.sy + 1, b + g.sy, b + g.sz, b + g.sz + 1,
                            b + g.sz + g.sy + 1, b + g.sz + g.sy};
 #pragma unroll
-    for (int q = 0; q < 8; ++q) l2_atomic_add(&rep[nd[q]], kq[nd[q]] * w[q]);
+    for (int q = 0; q < 8; ++q) l2_atomic_add(&rep[nd[q]], w[q]);
   }
   if (ds.vol <= 0) return;  // uniform across the block
   const int key = (l >= 0) ? l : -1 - __lane_id();
@@ -559,15 +561,14 @@ __device__ inline void box_add(const KGrid &g, DS &ds, bool has, int ix, int iy,
 }
 
 template <class DS>
-__device__ inline void box_flush(const KGrid &g, DS &ds, const double *__restrict__ kq,
-                                 double *rep) {
+__device__ inline void box_flush(const KGrid &g, DS &ds, double *rep) {
   __syncthreads();
   for (int t = threadIdx.x; t < ds.vol; t += BLOCK) {
     double v = ds.box[t];
     if (v != 0.0) {
       int lx = t % ds.bx, r = t / ds.bx, ly = r % ds.by, lz = r / ds.by;
       int64_t node = gidx(g, ds.x0 + lx, ds.y0 + ly, ds.z0 + lz);
-      l2_atomic_add(&rep[node], kq[node] * v);
+      l2_atomic_add(&rep[node], v);
     }
   }
 }
@@ -647,10 +648,10 @@ __device__ inline void box_atomic8(DS &ds, int l, const double a[8]) {
 }
 
 // A run with a box key goes to LDS, a run with a global key to this XCD's
-// replica (kq applied to the run's sum: kq is 1 or 2, so the scaling is exact).
+// replica.
 template <class DS>
 __device__ inline void flush_run(const KGrid &g, DS &ds, int key, const double a[8],
-                                 const double *__restrict__ kq, double *rep) {
+                                 double *rep) {
   if (key >= 0) {
     box_atomic8(ds, key, a);
   } else if (key <= -2) {
@@ -658,7 +659,7 @@ __device__ inline void flush_run(const KGrid &g, DS &ds, int key, const double a
     const int64_t nd[8] = {b, b + 1, b + g.sy + 1, b + g.sy, b + g.sz, b + g.sz + 1,
                            b + g.sz + g.sy + 1, b + g.sz + g.sy};
 #pragma unroll
-    for (int q = 0; q < 8; ++q) l2_atomic_add(&rep[nd[q]], kq[nd[q]] * a[q]);
+    for (int q = 0; q < 8; ++q) l2_atomic_add(&rep[nd[q]], a[q]);
   }
 }
 
@@ -763,10 +764,9 @@ struct RunAcc {
 
 template <class DS>
 __device__ inline void run_add(const KGrid &g, DS &ds, RunAcc &ra, int key, double k1,
-                               double px, double py, double pz, const double *__restrict__ kq,
-                               double *rep) {
+                               double px, double py, double pz, double *rep) {
   if (key != ra.cur) {
-    flush_run(g, ds, ra.cur, ra.a, kq, rep);
+    flush_run(g, ds, ra.cur, ra.a, rep);
     ra.reset();
     ra.cur = key;
   }
@@ -781,8 +781,7 @@ __device__ inline void run_add(const KGrid &g, DS &ds, RunAcc &ra, int key, doub
   ra.a[7] += k1 * px * wy2 * wz2;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_deposit(KGrid g, KPset p, int64_t n,
-                                                   const double *__restrict__ kq, double k1,
+__global__ __launch_bounds__(BLOCK) void k_deposit(KGrid g, KPset p, int64_t n, double k1,
                                                    double *rep_base, int64_t nodes) {
   __shared__ DepShared ds;
   double *rep = rep_base + (int64_t)xcc_id() * nodes;
@@ -822,14 +821,15 @@ __global__ __launch_bounds__(BLOCK) void k_deposit(KGrid g, KPset p, int64_t n,
     int ix, iy, iz;
     double px, py, pz;
     if (live[q] && deposit_fracs(g, x[q], y[q], z[q], ix, iy, iz, px, py, pz))
-      run_add(g, ds, ra, stage_key(g, ds, ix, iy, iz), k1, px, py, pz, kq, rep);
+      run_add(g, ds, ra, stage_key(g, ds, ix, iy, iz), k1, px, py, pz, rep);
   }
-  flush_run(g, ds, ra.cur, ra.a, kq, rep);
-  box_flush(g, ds, kq, rep);
+  flush_run(g, ds, ra.cur, ra.a, rep);
+  box_flush(g, ds, rep);
 }
 
 // np_local += sum of the replicas; replicas cleared for the next deposit.
-__global__ void k_rep_flush(double *np, double *rep, int64_t nodes) {
+__global__ void k_rep_flush(double *np, double *rep, int64_t nodes,
+                            const double *__restrict__ kq) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i >= nodes) return;
   double s = 0.0;
@@ -838,7 +838,7 @@ __global__ void k_rep_flush(double *np, double *rep, int64_t nodes) {
     s = s + rep[(int64_t)r * nodes + i];
     rep[(int64_t)r * nodes + i] = 0.0;
   }
-  np[i] = np[i] + s;
+  np[i] = np[i] + kq[i] * s;
 }
 
 // The particle phase of a collision step, in place: gather + push + wall BC
@@ -889,7 +889,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_nodep(
 template <int BM>
 __global__ __launch_bounds__(BLOCK) void k_push(
     KGrid g, KPset p, int64_t n, const double4 *__restrict__ E4, KBField bf, double k1, double dt,
-    const int *__restrict__ wt, KBC bp, const double *__restrict__ kq, double kdep,
+    const int *__restrict__ wt, KBC bp, double kdep,
     double *rep_base, int64_t nodes, int do_dep, double *p_mac, double *P_loss,
     unsigned long long *lost_ctr) {
   __shared__ DepSharedPush ds;
@@ -964,7 +964,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(
         int ix, iy, iz;
         double px, py, pz;
         if (do_dep && deposit_fracs_q(g, xp, yp, zp, qx, qy, qz, okn, ix, iy, iz, px, py, pz))
-          run_add(g, ds, ra, stage_key(g, ds, ix, iy, iz), kdep, px, py, pz, kq, rep);
+          run_add(g, ds, ra, stage_key(g, ds, ix, iy, iz), kdep, px, py, pz, rep);
       }
     }
     wall_diag_wave(bp, lost, label, vx[q], vy[q], vz[q], p_mac, P_loss);
@@ -972,8 +972,8 @@ __global__ __launch_bounds__(BLOCK) void k_push(
   }
   store_run(p, n, i0, full, x, y, z, vx, vy, vz, live, lostq);
   if (do_dep) {
-    flush_run(g, ds, ra.cur, ra.a, kq, rep);
-    box_flush(g, ds, kq, rep);
+    flush_run(g, ds, ra.cur, ra.a, rep);
+    box_flush(g, ds, rep);
   }
   // losses are rare: one atomic per wave that saw any
   int tot = lost_local;
@@ -987,7 +987,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(
 template <int BM>
 __global__ __launch_bounds__(BLOCK) void k_push_bc_deposit(
     KGrid g, KPset src, KPset dst, int64_t n, const double4 *__restrict__ E4, KBField bf,
-    double k1, double dt, const int *__restrict__ bcnd, KBC bp, const double *__restrict__ kq,
+    double k1, double dt, const int *__restrict__ bcnd, KBC bp,
     double kdep, double *rep_base, int64_t nodes, double *p_mac, double *P_loss,
     uint64_t *status, uint32_t epoch, int *counter, int64_t *spin_fail, int64_t *count_out,
     int64_t ntiles) {
@@ -1034,8 +1034,8 @@ __global__ __launch_bounds__(BLOCK) void k_push_bc_deposit(
     int ix = 0, iy = 0, iz = 0;
     double w[8];
     if (keep) has = deposit_weights(g, kdep, xp, yp, zp, ix, iy, iz, w);
-    box_add(g, ds, has, ix, iy, iz, w, kq, rep);
-    box_flush(g, ds, kq, rep);
+    box_add(g, ds, has, ix, iy, iz, w, rep);
+    box_flush(g, ds, rep);
   }
 }
 
@@ -1500,14 +1500,14 @@ double *lepic_replicas(lepic_ctx *ctx, int64_t nodes) {
   return ctx->rep;
 }
 
-int lepic_replica_flush(lepic_ctx *ctx, int64_t nodes, double *np_local) {
+int lepic_replica_flush(lepic_ctx *ctx, int64_t nodes, double *np_local, const double *kq) {
   // the deposit kernels stride the replicas by this grid's node count; the
   // buffer stays all-zero between calls because every written entry is
   // cleared here
   if (!ctx->rep || nodes > ctx->rep_nodes)
     return lepic_fail(ctx, LEPIC_ERR_ARG, "replica flush: replicas smaller than the grid");
   hipLaunchKernelGGL(k_rep_flush, dim3(grid_blocks(nodes, 256)), dim3(256), 0, ctx->stream,
-                     np_local, ctx->rep, nodes);
+                     np_local, ctx->rep, nodes, kq);
   LEPIC_LAUNCH_CHECK(ctx, "k_rep_flush");
   return LEPIC_OK;
 }
@@ -1620,9 +1620,9 @@ int lepic_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *p, cons
   if (!rep) return lepic_fail(ctx, LEPIC_ERR_HIP, "deposit: replica allocation failed");
   double k1 = Nm / (g->h[0] * g->h[1] * g->h[2]);
   hipLaunchKernelGGL(k_deposit, dim3(grid_blocks(p->n, CTILE)), dim3(BLOCK), 0, ctx->stream,
-                     make_kgrid(g), kp(p), p->n, kq, k1, rep, nodes);
+                     make_kgrid(g), kp(p), p->n, k1, rep, nodes);
   LEPIC_LAUNCH_CHECK(ctx, "k_deposit");
-  return lepic_replica_flush(ctx, nodes, np_local);
+  return lepic_replica_flush(ctx, nodes, np_local, kq);
 }
 
 int lepic_push_bc_deposit_inplace(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *p,
@@ -1676,13 +1676,13 @@ int lepic_push_bc_deposit_inplace(lepic_ctx *ctx, const lepic_grid *g, lepic_pse
       switch (bmode) {
         case 0: hipLaunchKernelGGL((k_push<0>), dim3(grid_blocks(p->n, CTILE)), dim3(BLOCK), 0, ctx->stream, kg,
                          kp(p), p->n, reinterpret_cast<const double4 *>(E4), kbf, k1, dt, bcnd,
-                         kbc(bp), kq, kdep, rep, nodes, 1, p_mac, P_loss, ctr); break;
+                         kbc(bp), kdep, rep, nodes, 1, p_mac, P_loss, ctr); break;
         case 1: hipLaunchKernelGGL((k_push<1>), dim3(grid_blocks(p->n, CTILE)), dim3(BLOCK), 0, ctx->stream, kg,
                          kp(p), p->n, reinterpret_cast<const double4 *>(E4), kbf, k1, dt, bcnd,
-                         kbc(bp), kq, kdep, rep, nodes, 1, p_mac, P_loss, ctr); break;
+                         kbc(bp), kdep, rep, nodes, 1, p_mac, P_loss, ctr); break;
         default: hipLaunchKernelGGL((k_push<2>), dim3(grid_blocks(p->n, CTILE)), dim3(BLOCK), 0, ctx->stream, kg,
                          kp(p), p->n, reinterpret_cast<const double4 *>(E4), kbf, k1, dt, bcnd,
-                         kbc(bp), kq, kdep, rep, nodes, 1, p_mac, P_loss, ctr); break;
+                         kbc(bp), kdep, rep, nodes, 1, p_mac, P_loss, ctr); break;
       }
       LEPIC_LAUNCH_CHECK(ctx, "k_push");
     } else {
@@ -1700,7 +1700,7 @@ int lepic_push_bc_deposit_inplace(lepic_ctx *ctx, const lepic_grid *g, lepic_pse
       LEPIC_LAUNCH_CHECK(ctx, "k_push_nodep");
     }
   }
-  if (np_local && (st = lepic_replica_flush(ctx, nodes, np_local))) return st;
+  if (np_local && (st = lepic_replica_flush(ctx, nodes, np_local, kq))) return st;
   if (np_lost) {
     if (lost_acc)
       hipLaunchKernelGGL(k_add_u64, dim3(1), dim3(64), 0, ctx->stream,
@@ -1755,21 +1755,21 @@ int lepic_push_bc_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset 
         switch (bmode) {
           case 0: hipLaunchKernelGGL((k_push_bc_deposit<0>), grid, dim3(BLOCK), 0, ctx->stream, kg, kp(src),
                            kp(dst), n, reinterpret_cast<const double4 *>(E4), kbf, k1, dt, bcnd,
-                           kb, kq, kdep, rep, nodes, p_mac, P_loss, ctx->d_status, epoch,
+                           kb, kdep, rep, nodes, p_mac, P_loss, ctx->d_status, epoch,
                            counter, spin, count_out, ntiles); break;
           case 1: hipLaunchKernelGGL((k_push_bc_deposit<1>), grid, dim3(BLOCK), 0, ctx->stream, kg, kp(src),
                            kp(dst), n, reinterpret_cast<const double4 *>(E4), kbf, k1, dt, bcnd,
-                           kb, kq, kdep, rep, nodes, p_mac, P_loss, ctx->d_status, epoch,
+                           kb, kdep, rep, nodes, p_mac, P_loss, ctx->d_status, epoch,
                            counter, spin, count_out, ntiles); break;
           default: hipLaunchKernelGGL((k_push_bc_deposit<2>), grid, dim3(BLOCK), 0, ctx->stream, kg, kp(src),
                            kp(dst), n, reinterpret_cast<const double4 *>(E4), kbf, k1, dt, bcnd,
-                           kb, kq, kdep, rep, nodes, p_mac, P_loss, ctx->d_status, epoch,
+                           kb, kdep, rep, nodes, p_mac, P_loss, ctx->d_status, epoch,
                            counter, spin, count_out, ntiles); break;
         }
       },
       &nout);
   if (st) return st;
-  if (np_local && (st = lepic_replica_flush(ctx, nodes, np_local))) return st;
+  if (np_local && (st = lepic_replica_flush(ctx, nodes, np_local, kq))) return st;
   dst->n = nout;
   if (np_lost) *np_lost = n - nout;
   return LEPIC_OK;
