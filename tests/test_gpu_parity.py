@@ -276,18 +276,23 @@ def test_loader_host_bitexact(oracle):
     _same(npt.cpu().numpy(), npo)
 
 
+@pytest.mark.parametrize("neg", [False, True])
 @pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("presort,size", [(True, (12, 10, 8)), (False, (12, 10, 8)),
                                           (False, (20, 20, 20))])
-def test_fused_inplace_and_sort_compact(gpu, oracle, rng, mode, presort, size):
+def test_fused_inplace_and_sort_compact(gpu, oracle, rng, mode, presort, size, neg):
     """Production particle phase: in-place push + wall BC + deposit with holes,
     then the compacting sort, against the oracle's mover + BC + deposit + sort.
     Sorted input deposits through the LDS box in runs of one cell; unsorted
     input has no runs, and on the 20^3 grid a block's particles span more
-    nodes than the box holds, so every stencil goes straight to global memory."""
+    nodes than the box holds, so every stencil goes straight to global memory.
+    neg makes the particles negative ions with a Neumann x = 0 face: those
+    that cross it are lost (lost = 2, no wall label)."""
     from essaitemporaire_lepic_module_amd import particles as P
     n, h = H.box(size)
     bc = H.two_electrode_bcnd(n)
+    if neg:   # a Neumann x = 0 face (-2): the cells next to it are not walls
+        bc[2:n[2] + 1, 2:n[1] + 1, 0:2] = -2
     kq = cfg.build_kq(bc)
     count = 30000
     arr = H.random_particles(rng, n, h, count, outside=0.02, vscale=2e6)
@@ -301,7 +306,8 @@ def test_fused_inplace_and_sort_compact(gpu, oracle, rng, mode, presort, size):
     start = po.arrays()
     ps = _gpu_set(P, [start[k] for k in ("x", "y", "z", "vx", "vy", "vz")], dead=start["dead"])
     bf = P.bfield(mode, Bc)
-    bp = P.bc_params(n, h, 0, 0, 1, -1, Nm * q, m, Nm, int(bc.max()))
+    nmn, ptype, tag_neg = (1, 2, 2) if neg else (0, 1, -1)
+    bp = P.bc_params(n, h, 0, nmn, ptype, tag_neg, Nm * q, m, Nm, int(bc.max()))
     p_mac = torch.zeros((int(bc.max()) + 1, 2), dtype=torch.float64, device=gpu)
     P_loss = torch.zeros(1, dtype=torch.float64, device=gpu)
     npg = torch.zeros(cfg.node_shape(n), dtype=torch.float64, device=gpu)
@@ -313,7 +319,8 @@ def test_fused_inplace_and_sort_compact(gpu, oracle, rng, mode, presort, size):
         oracle.move_electrostatic(n, h, po, E4, q, m, dt)
     else:
         oracle.move_boris(n, h, po, E4, mode, Bc, None, q, m, dt)
-    lost_o, pmac_o, Ploss_o = oracle.apply_bc(n, h, po, bc, 0, 0, 1, -1, Nm * q, m, Nm, int(bc.max()))
+    lost_o, pmac_o, Ploss_o = oracle.apply_bc(n, h, po, bc, 0, nmn, ptype, tag_neg, Nm * q, m, Nm,
+                                              int(bc.max()))
     npo = oracle.deposit(n, h, po, kq, Nm)
     assert lost == lost_o - int(start["dead"].sum())   # MCC-dead holes are not new losses
     g = ps.to_host()
