@@ -26,6 +26,7 @@ EXPORTS = (
     "lepic_version", "lepic_ran2", "lepic_seed", "lepic_load_particles_host",
     "lepic_load_uniform_device", "lepic_sort_particles_by_cell", "lepic_move_electrostatic",
     "lepic_move_boris", "lepic_apply_bc", "lepic_build_kq", "lepic_fill", "lepic_deposit",
+    "lepic_deposit_fixed",
     "lepic_reduce_density", "lepic_build_rho", "lepic_poisson_solve", "lepic_efield",
     "lepic_collisions", "lepic_collisions_deposited", "lepic_push_bc_deposit",
     "lepic_push_bc_deposit_inplace", "lepic_sort_compact", "lepic_bin_compact",
@@ -139,6 +140,7 @@ def load() -> C.CDLL:
             "lepic_build_kq": (C.c_int, [vp, G, vp, vp]),
             "lepic_fill": (C.c_int, [vp, vp, C.c_int64, C.c_double]),
             "lepic_deposit": (C.c_int, [vp, G, PS, vp, C.c_double, vp]),
+            "lepic_deposit_fixed": (C.c_int, [vp, G, PS, vp, C.c_double, C.c_int32, vp]),
             "lepic_reduce_density": (C.c_int, [vp, G, vp, C.c_int32, vp, vp]),
             "lepic_build_rho": (C.c_int, [vp, G, C.c_int32, vp, dp, vp]),
             "lepic_poisson_solve": (C.c_int, [vp, G, vp, vp, vp, C.POINTER(MGParams),

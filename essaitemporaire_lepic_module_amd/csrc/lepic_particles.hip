@@ -1178,6 +1178,81 @@ __device__ inline void load_pos_run(const KPset &p, int64_t n, int64_t i0, bool 
   }
 }
 
+// Fixed-point deposit.  Each of a particle's 8 contributions k1*w (the
+// weights and products of deposit_particle_set_to_np_thread,
+// mod_chargeDeposition.f90:25-133) is rounded to the nearest multiple of
+// 2^-frac_bits and summed as an int64, so the node totals do not depend on
+// the order in which particles arrive, and the charge summed over the grid is
+// exactly the sum of the rounded contributions.  A lane sums its run of one
+// cell in registers; runs go to this XCD's int64 replica (two's complement
+// wrap-around makes unsigned adds of negative values exact).
+__device__ inline void flush_fixed(const KGrid &g, int64_t b, const long long a[8],
+                                   unsigned long long *rep) {
+  if (b < 0) return;
+  const int64_t nd[8] = {b, b + 1, b + g.sy + 1, b + g.sy, b + g.sz, b + g.sz + 1,
+                         b + g.sz + g.sy + 1, b + g.sz + g.sy};
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (a[q] != 0)
+      __hip_atomic_fetch_add(&rep[nd[q]], (unsigned long long)a[q], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ inline long long to_fixed(double w, double scale) {
+  return (long long)rint(w * scale);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_deposit_fixed(KGrid g, KPset p, int64_t n, double k1,
+                                                         double scale,
+                                                         unsigned long long *rep_base,
+                                                         int64_t nodes) {
+  unsigned long long *rep = rep_base + (int64_t)xcc_id() * nodes;
+  const int64_t i0 = (int64_t)blockIdx.x * CTILE + (int64_t)threadIdx.x * CPT;
+  const bool full = i0 + CPT <= n;
+  double x[CPT], y[CPT], z[CPT];
+  bool live[CPT];
+  load_pos_run(p, n, i0, full, x, y, z, live);
+  int64_t cur = -1;
+  long long a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) {
+    int ix, iy, iz;
+    double px, py, pz;
+    if (!(live[q] && deposit_fracs(g, x[q], y[q], z[q], ix, iy, iz, px, py, pz))) continue;
+    const int64_t b = gidx(g, ix, iy, iz);
+    if (b != cur) {
+      flush_fixed(g, cur, a, rep);
+      cur = b;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) a[j] = 0;
+    }
+    const double wx2 = 1.0 - px, wy2 = 1.0 - py, wz2 = 1.0 - pz;
+    a[0] += to_fixed(k1 * px * py * pz, scale);
+    a[1] += to_fixed(k1 * wx2 * py * pz, scale);
+    a[2] += to_fixed(k1 * wx2 * wy2 * pz, scale);
+    a[3] += to_fixed(k1 * px * wy2 * pz, scale);
+    a[4] += to_fixed(k1 * px * py * wz2, scale);
+    a[5] += to_fixed(k1 * wx2 * py * wz2, scale);
+    a[6] += to_fixed(k1 * wx2 * wy2 * wz2, scale);
+    a[7] += to_fixed(k1 * px * wy2 * wz2, scale);
+  }
+  flush_fixed(g, cur, a, rep);
+}
+
+// np += kq * (sum of the replicas) * 2^-frac_bits, clearing the replicas.
+__global__ void k_fixed_flush(double *np, unsigned long long *rep, int64_t nodes,
+                              const double *__restrict__ kq, double inv_scale) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nodes) return;
+  unsigned long long s = 0;
+#pragma unroll
+  for (int r = 0; r < LEPIC_XCDS; ++r) {
+    s += rep[(int64_t)r * nodes + i];
+    rep[(int64_t)r * nodes + i] = 0;
+  }
+  np[i] = np[i] + kq[i] * ((double)(long long)s * inv_scale);
+}
+
 __global__ __launch_bounds__(BLOCK) void k_bin_count(KGrid g, KPset p, int64_t n,
                                                      int32_t *count) {
   __shared__ BinShared bs;
@@ -1623,6 +1698,36 @@ int lepic_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *p, cons
                      make_kgrid(g), kp(p), p->n, k1, rep, nodes);
   LEPIC_LAUNCH_CHECK(ctx, "k_deposit");
   return lepic_replica_flush(ctx, nodes, np_local, kq);
+}
+
+int lepic_deposit_fixed(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *p,
+                        const double *kq, double Nm, int32_t frac_bits, double *np_local) {
+  if (!ctx) return LEPIC_ERR_ARG;
+  int st;
+  if ((st = check_grid(ctx, g)) || (st = check_pset(ctx, p, "deposit_fixed"))) return st;
+  if (!aligned_runs(p))
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "deposit_fixed: particle arrays must be 16-byte aligned");
+  if (!kq || !np_local) return lepic_fail(ctx, LEPIC_ERR_ARG, "deposit_fixed: null argument");
+  const double k1 = Nm / (g->h[0] * g->h[1] * g->h[2]);
+  const double unit = std::ldexp(std::fabs(k1), frac_bits);   // one full weight, in units
+  if (frac_bits < -1000 || frac_bits > 1000 || !(unit >= 1.0 && unit <= 281474976710656.0))
+    return lepic_fail(ctx, LEPIC_ERR_ARG,
+                      "deposit_fixed: |Nm/(h1 h2 h3)| * 2^frac_bits must lie in [1, 2^48]");
+  if (p->n == 0) return LEPIC_OK;
+  const int64_t nodes = grid_nodes(g);
+  double *rep = lepic_replicas(ctx, nodes);
+  if (!rep) return lepic_fail(ctx, LEPIC_ERR_HIP, "deposit_fixed: replica allocation failed");
+  if (nodes > ctx->rep_nodes)
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "deposit_fixed: replicas smaller than the grid");
+  unsigned long long *irep = reinterpret_cast<unsigned long long *>(rep);   // zero bits either way
+  hipLaunchKernelGGL(k_deposit_fixed, dim3(grid_blocks(p->n, CTILE)), dim3(BLOCK), 0,
+                     ctx->stream, make_kgrid(g), kp(p), p->n, k1, std::ldexp(1.0, frac_bits), irep,
+                     nodes);
+  LEPIC_LAUNCH_CHECK(ctx, "k_deposit_fixed");
+  hipLaunchKernelGGL(k_fixed_flush, dim3(grid_blocks(nodes, 256)), dim3(256), 0, ctx->stream,
+                     np_local, irep, nodes, kq, std::ldexp(1.0, -frac_bits));
+  LEPIC_LAUNCH_CHECK(ctx, "k_fixed_flush");
+  return LEPIC_OK;
 }
 
 int lepic_push_bc_deposit_inplace(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *p,

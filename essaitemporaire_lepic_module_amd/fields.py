@@ -9,6 +9,7 @@ np_red are (ntype, n3+3, n2+3, n1+3).
 from __future__ import annotations
 
 import ctypes as C
+import math
 
 import torch
 
@@ -42,6 +43,31 @@ def deposit_particle_set_to_np_thread(part: ParticleSet, n, h, kq: torch.Tensor,
     p = part.pset()
     ctx.check(ctx.lib.lepic_deposit(ctx.h, C.byref(g), C.byref(p), ptr(kq), float(Nm_species),
                                     ptr(np_local)))
+
+
+def fixed_point_bits(n, h, Nm_species: float) -> int:
+    """Fraction bits for deposit_fixed: a full weight |Nm/(h1 h2 h3)| maps
+    to at most 2^44 units, leaving a node's int64 total room for 2^19
+    particles."""
+    k1 = abs(float(Nm_species) / (h[0] * h[1] * h[2]))
+    return 44 - math.ceil(math.log2(k1))
+
+
+def deposit_particle_set_to_np_thread_fixed(part: ParticleSet, n, h, kq: torch.Tensor,
+                                            Nm_species: float, np_local: torch.Tensor,
+                                            frac_bits: int | None = None):
+    """deposit_particle_set_to_np_thread (mod_chargeDeposition.f90:25-133)
+    with every contribution rounded to a multiple of 2^-frac_bits and summed
+    as int64: the density does not depend on particle order, and the grid
+    total is the exact sum of the rounded contributions."""
+    if frac_bits is None:
+        frac_bits = fixed_point_bits(n, h, Nm_species)
+    ctx = context(part.device)
+    g = _native.grid(n, h)
+    p = part.pset()
+    ctx.check(ctx.lib.lepic_deposit_fixed(ctx.h, C.byref(g), C.byref(p), ptr(kq),
+                                          float(Nm_species), int(frac_bits), ptr(np_local)))
+    return frac_bits
 
 
 def reduce_species_density(n, bcnd: torch.Tensor, np_thread: torch.Tensor, ntype: int,

@@ -202,6 +202,19 @@ int lepic_fill(lepic_ctx *ctx, double *a, int64_t count, double value);
  * np_local += kq*Nm/(h1 h2 h3)*w over the 8 nodes (FP64 atomics). */
 int lepic_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *p, const double *kq,
                   double Nm, double *np_local);
+
+/* deposit_particle_set_to_np_thread (mod_chargeDeposition.f90:25-133) in
+ * fixed point: each particle's 8 contributions k1*w (k1 = Nm/(h1 h2 h3), the
+ * reference's weights and products) are rounded to the nearest multiple of
+ * 2^-frac_bits and summed as int64, then np_local += kq * sum * 2^-frac_bits.
+ * The result does not depend on particle order or atomic timing, and the grid
+ * total is exactly the sum of the rounded contributions; each contribution
+ * differs from the FP64 one by at most 2^-(frac_bits+1).  |k1| * 2^frac_bits
+ * must lie in [1, 2^48] (LEPIC_ERR_ARG otherwise), so a node's int64 total
+ * cannot overflow below 2^15 particles touching it (each adds at most one
+ * full weight to a node). */
+int lepic_deposit_fixed(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *p,
+                        const double *kq, double Nm, int32_t frac_bits, double *np_local);
 /* reduce_species_density (mod_density.f90:31-73), one rank, one virtual
  * thread: periodic stitching in place on np_thread [ntype][nodes], then
  * np_red = np_thread on nodes 1..n+1 and 0 elsewhere. */

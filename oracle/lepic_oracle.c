@@ -550,6 +550,53 @@ void lo_deposit(const lo_grid *g, const lo_pset *p, const double *kq, double Nm,
   }
 }
 
+/* lo_deposit in fixed point (the contract of lepic_deposit_fixed): each
+ * contribution k1*w, formed as lo_deposit forms it, is rounded to the nearest
+ * multiple of 2^-frac_bits (rint, ties to even) and summed as a wrapping
+ * 64-bit integer per node; then np += kq * ((double)sum * 2^-frac_bits) at
+ * every node. */
+void lo_deposit_fixed(const lo_grid *g, const lo_pset *p, const double *kq, double Nm,
+                      int32_t frac_bits, double *np_local) {
+  const double *h = g->h;
+  const int64_t nodes = (int64_t)(g->n[0] + 3) * (g->n[1] + 3) * (g->n[2] + 3);
+  uint64_t *acc = (uint64_t *)calloc((size_t)nodes, sizeof(uint64_t));
+  double xmax = (double)g->n[0] * h[0], ymax = (double)g->n[1] * h[1],
+         zmax = (double)g->n[2] * h[2];
+  double k1 = Nm / (h[0] * h[1] * h[2]);
+  const double scale = ldexp(1.0, frac_bits), inv = ldexp(1.0, -frac_bits);
+  for (int64_t i = 0; i < p->n; ++i) {
+    if (p->dead[i] != 0) continue;
+    double x = p->x[i], y = p->y[i], z = p->z[i];
+    if (x < -1.0e-12 || x > xmax + 1.0e-12) continue;
+    if (y < -1.0e-12 || y > ymax + 1.0e-12) continue;
+    if (z < -1.0e-12 || z > zmax + 1.0e-12) continue;
+    int32_t ix = (int32_t)(x / h[0]) + 1;
+    int32_t iy = (int32_t)(y / h[1]) + 1;
+    int32_t iz = (int32_t)(z / h[2]) + 1;
+    double px = ((double)ix * h[0] - x) / h[0];
+    double py = ((double)iy * h[1] - y) / h[1];
+    double pz = ((double)iz * h[2] - z) / h[2];
+    double wx2 = 1.0 - px, wy2 = 1.0 - py, wz2 = 1.0 - pz;
+    double w[8];
+    w[0] = k1 * px * py * pz;
+    w[1] = k1 * wx2 * py * pz;
+    w[2] = k1 * wx2 * wy2 * pz;
+    w[3] = k1 * px * wy2 * pz;
+    w[4] = k1 * px * py * wz2;
+    w[5] = k1 * wx2 * py * wz2;
+    w[6] = k1 * wx2 * wy2 * wz2;
+    w[7] = k1 * px * wy2 * wz2;
+    const int64_t c[8] = {IDX(g, ix, iy, iz),         IDX(g, ix + 1, iy, iz),
+                          IDX(g, ix + 1, iy + 1, iz), IDX(g, ix, iy + 1, iz),
+                          IDX(g, ix, iy, iz + 1),     IDX(g, ix + 1, iy, iz + 1),
+                          IDX(g, ix + 1, iy + 1, iz + 1), IDX(g, ix, iy + 1, iz + 1)};
+    for (int q = 0; q < 8; ++q) acc[c[q]] += (uint64_t)(int64_t)rint(w[q] * scale);
+  }
+  for (int64_t t = 0; t < nodes; ++t)
+    np_local[t] = np_local[t] + kq[t] * ((double)(int64_t)acc[t] * inv);
+  free(acc);
+}
+
 /* load_gauss_local (mod_heating.f90:85-97). */
 static void heat_gauss(double *v1, double *v2, double vt, double r1, double r2) {
   const double vp = vt * sqrt(-log(1.0 - r1));

@@ -115,6 +115,8 @@ def lib():
                                         C.c_double, dp]
         L.lo_build_kq.argtypes = [C.POINTER(Grid), ip, dp]
         L.lo_deposit.argtypes = [C.POINTER(Grid), C.POINTER(PSet), dp, C.c_double, dp]
+        L.lo_deposit_fixed.argtypes = [C.POINTER(Grid), C.POINTER(PSet), dp, C.c_double,
+                                       C.c_int32, dp]
         L.lo_reduce_density.argtypes = [C.POINTER(Grid), ip, C.c_int32, C.c_int32, dp, dp]
         L.lo_build_rho.argtypes = [C.POINTER(Grid), C.c_int32, dp, dp, dp]
         L.lo_pdesolver.restype = C.c_int
@@ -290,6 +292,14 @@ def deposit(n, h, p: Particles, kq, Nm, np_local=None):
     out = np.zeros(kq.shape) if np_local is None else np_local
     assert out.flags.c_contiguous and out.dtype == np.float64
     lib().lo_deposit(C.byref(g), C.byref(p.s), _p(kq), float(Nm), _p(out))
+    return out
+
+
+def deposit_fixed(n, h, p: Particles, kq, Nm, frac_bits, np_local=None):
+    g = grid(n, h)
+    out = np.zeros(kq.shape) if np_local is None else np_local
+    assert out.flags.c_contiguous and out.dtype == np.float64
+    lib().lo_deposit_fixed(C.byref(g), C.byref(p.s), _p(kq), float(Nm), int(frac_bits), _p(out))
     return out
 
 

@@ -338,6 +338,41 @@ def test_fused_inplace_and_sort_compact(gpu, oracle, rng, mode, presort, size, n
         _same(g[k], o[k])
 
 
+@pytest.mark.parametrize("presort", [False, True])
+def test_deposit_fixed_bitexact(gpu, oracle, rng, presort):
+    """The fixed-point deposit is bit-identical to the oracle's (integer sums
+    do not depend on order), also for the particles in another order, and
+    rejects a scale outside [1, 2^48] units per full weight."""
+    from essaitemporaire_lepic_module_amd import fields as F
+    from essaitemporaire_lepic_module_amd import particles as P
+    from essaitemporaire_lepic_module_amd._native import LepicError
+    n, h = H.box((12, 10, 8))
+    bc = H.two_electrode_bcnd(n)
+    kq = cfg.build_kq(bc)
+    count = 30000
+    arr = H.random_particles(rng, n, h, count, outside=0.02)
+    dead = (rng.random(count) < 0.03).astype(np.int8)
+    po = _oracle_set(oracle, arr, dead=dead)
+    if presort:
+        oracle.sort_particles(n, h, po)
+    a = po.arrays()
+    cols = [a[k] for k in ("x", "y", "z", "vx", "vy", "vz")]
+    Nm = 1526.0
+    kqg = torch.as_tensor(kq, device=gpu)
+    outs = []
+    for order in (np.arange(po.n), rng.permutation(po.n)):
+        ps = _gpu_set(P, [c[order] for c in cols], dead=a["dead"][order])
+        npg = torch.zeros(cfg.node_shape(n), dtype=torch.float64, device=gpu)
+        S = F.deposit_particle_set_to_np_thread_fixed(ps, n, h, kqg, Nm, npg)
+        outs.append(npg.cpu().numpy())
+    ref = oracle.deposit_fixed(n, h, po, kq, Nm, S)
+    assert np.abs(ref).sum() > 0
+    _same(outs[0], ref)
+    _same(outs[1], ref)
+    with pytest.raises(LepicError):
+        F.deposit_particle_set_to_np_thread_fixed(ps, n, h, kqg, Nm, npg, frac_bits=S + 10)
+
+
 def test_deposit_unsorted_falls_back(gpu, oracle, rng):
     """Particles spread over the whole grid in every block: the LDS box does
     not fit and every particle goes straight to the replicas."""

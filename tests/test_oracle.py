@@ -150,6 +150,46 @@ def test_deposit_kq_and_skips(oracle, rng):
     np.testing.assert_allclose(dep.sum(), 2.0 * k1 + k1 / 8 * (2 * 7 + 1 * 1), rtol=1e-14)
 
 
+def test_deposit_fixed_point(oracle, rng):
+    """The fixed-point deposit: bitwise the same for any particle order; each
+    node within (its contributions) x 2^-(S+1) x kq of the FP64 deposit;
+    exactly zero where nothing lands; the grid total, divided by kq, within
+    4 units of 2^-S per particle of k1 per particle (the weights of a particle
+    sum to 1)."""
+    n, h = H.box((7, 6, 5))
+    bc = cfg.grounded_box_bcnd(n)
+    kq = oracle.build_kq(n, bc)
+    count = 4000
+    arr = H.random_particles(rng, n, h, count, outside=0.01)
+    dead = (rng.random(count) < 0.05).astype(np.int8)
+    Nm = 1.7e3
+    k1 = Nm / (h[0] * h[1] * h[2])
+    S = 44 - math.ceil(math.log2(k1))
+    p = oracle.Particles(count).set(*arr, dead=dead)
+    fixed = oracle.deposit_fixed(n, h, p, kq, Nm, S)
+    perm = rng.permutation(count)
+    q = oracle.Particles(count).set(*(a[perm] for a in arr), dead=dead[perm])
+    np.testing.assert_array_equal(oracle.deposit_fixed(n, h, q, kq, Nm, S), fixed)
+    ref = oracle.deposit(n, h, p, kq, Nm)
+    # contributions per node: deposit unit weights into a counter grid
+    cnt = np.zeros(kq.shape)
+    g = np.asarray(fixed != 0.0)
+    live = (dead == 0)
+    L = np.array(n) * np.array(h)
+    inside = live & np.all([(arr[d] >= -1e-12) & (arr[d] <= L[d] + 1e-12) for d in range(3)], 0)
+    ix = (arr[0][inside] / h[0]).astype(np.int64) + 1
+    iy = (arr[1][inside] / h[1]).astype(np.int64) + 1
+    iz = (arr[2][inside] / h[2]).astype(np.int64) + 1
+    for di, dj, dk in [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0),
+                       (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]:
+        np.add.at(cnt, (iz + dk, iy + dj, ix + di), 1.0)
+    unit = 2.0 ** -S
+    np.testing.assert_array_less(np.abs(fixed - ref), cnt * kq * unit * 0.5 + 1e-13 * np.abs(ref) + 1e-300)
+    assert (fixed[cnt == 0] == 0.0).all() and g[cnt > 0].mean() > 0.5
+    total = (fixed / kq).sum()
+    assert abs(total - k1 * inside.sum()) <= inside.sum() * (4 * unit + 1e-14 * k1)
+
+
 # ------------------------------------------------------------------- a4: Boris
 def test_boris_rotation_closed_form(oracle, rng):
     """E = 0, uniform Bz: |v| conserved and the gyration angle per step is
