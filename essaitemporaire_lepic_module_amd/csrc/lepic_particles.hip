@@ -1184,18 +1184,35 @@ __device__ inline void load_pos_run(const KPset &p, int64_t n, int64_t i0, bool 
 // 2^-frac_bits and summed as an int64, so the node totals do not depend on
 // the order in which particles arrive, and the charge summed over the grid is
 // exactly the sum of the rounded contributions.  A lane sums its run of one
-// cell in registers; runs go to this XCD's int64 replica (two's complement
-// wrap-around makes unsigned adds of negative values exact).
-__device__ inline void flush_fixed(const KGrid &g, int64_t b, const long long a[8],
-                                   unsigned long long *rep) {
-  if (b < 0) return;
-  const int64_t nd[8] = {b, b + 1, b + g.sy + 1, b + g.sy, b + g.sz, b + g.sz + 1,
-                         b + g.sz + g.sy + 1, b + g.sz + g.sy};
+// cell in registers and adds it to the block's LDS box (or, outside it, to
+// this XCD's int64 replica); two's complement wrap-around makes unsigned adds
+// of negative values exact.
+// The block's cells +-1 get an LDS box of int64 sums (the FP64 deposit's
+// box, 32 KB); stencils outside it go to the replica directly.
+struct DepSharedFixed {
+  static constexpr int CAP = LDS_BOX;
+  int mn[3], mx[3];
+  int x0, y0, z0, bx, by, bz, vol;
+  unsigned long long box[LDS_BOX];
+};
+
+__device__ inline void flush_fixed(const KGrid &g, DepSharedFixed &ds, int key,
+                                   const long long a[8], unsigned long long *rep) {
+  if (key >= 0) {
+    const int bx = ds.bx, bxy = ds.bx * ds.by;
 #pragma unroll
-  for (int q = 0; q < 8; ++q)
-    if (a[q] != 0)
-      __hip_atomic_fetch_add(&rep[nd[q]], (unsigned long long)a[q], __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_WORKGROUP);
+    for (int q = 0; q < 8; ++q)
+      if (a[q] != 0) atomicAdd(&ds.box[key + node_off(q, bx, bxy)], (unsigned long long)a[q]);
+  } else if (key <= -2) {
+    const int64_t b = (int64_t)(-(key + 2));
+    const int64_t nd[8] = {b, b + 1, b + g.sy + 1, b + g.sy, b + g.sz, b + g.sz + 1,
+                           b + g.sz + g.sy + 1, b + g.sz + g.sy};
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (a[q] != 0)
+        __hip_atomic_fetch_add(&rep[nd[q]], (unsigned long long)a[q], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
 }
 
 __device__ inline long long to_fixed(double w, double scale) {
@@ -1206,23 +1223,33 @@ __global__ __launch_bounds__(BLOCK) void k_deposit_fixed(KGrid g, KPset p, int64
                                                          double scale,
                                                          unsigned long long *rep_base,
                                                          int64_t nodes) {
+  __shared__ DepSharedFixed ds;
   unsigned long long *rep = rep_base + (int64_t)xcc_id() * nodes;
   const int64_t i0 = (int64_t)blockIdx.x * CTILE + (int64_t)threadIdx.x * CPT;
   const bool full = i0 + CPT <= n;
   double x[CPT], y[CPT], z[CPT];
   bool live[CPT];
   load_pos_run(p, n, i0, full, x, y, z, live);
-  int64_t cur = -1;
+  int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+#pragma unroll
+  for (int q = 0; q < CPT; ++q)
+    if (live[q]) {
+      int c[3];
+      cell_of(g, x[q], y[q], z[q], c);
+      for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], c[d]); hi[d] = max(hi[d], c[d]); }
+    }
+  box_begin(g, ds, lo, hi);
+  int cur = -1;
   long long a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
   for (int q = 0; q < CPT; ++q) {
     int ix, iy, iz;
     double px, py, pz;
     if (!(live[q] && deposit_fracs(g, x[q], y[q], z[q], ix, iy, iz, px, py, pz))) continue;
-    const int64_t b = gidx(g, ix, iy, iz);
-    if (b != cur) {
-      flush_fixed(g, cur, a, rep);
-      cur = b;
+    const int key = stage_key(g, ds, ix, iy, iz);
+    if (key != cur) {
+      flush_fixed(g, ds, cur, a, rep);
+      cur = key;
 #pragma unroll
       for (int j = 0; j < 8; ++j) a[j] = 0;
     }
@@ -1236,7 +1263,16 @@ __global__ __launch_bounds__(BLOCK) void k_deposit_fixed(KGrid g, KPset p, int64
     a[6] += to_fixed(k1 * wx2 * wy2 * wz2, scale);
     a[7] += to_fixed(k1 * px * wy2 * wz2, scale);
   }
-  flush_fixed(g, cur, a, rep);
+  flush_fixed(g, ds, cur, a, rep);
+  __syncthreads();
+  for (int t = threadIdx.x; t < ds.vol; t += BLOCK) {
+    const unsigned long long v = ds.box[t];
+    if (v != 0) {
+      const int lx = t % ds.bx, r = t / ds.bx, ly = r % ds.by, lz = r / ds.by;
+      const int64_t node = gidx(g, ds.x0 + lx, ds.y0 + ly, ds.z0 + lz);
+      __hip_atomic_fetch_add(&rep[node], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  }
 }
 
 // np += kq * (sum of the replicas) * 2^-frac_bits, clearing the replicas.
