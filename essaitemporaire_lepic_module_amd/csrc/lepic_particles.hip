@@ -1745,6 +1745,7 @@ int lepic_deposit_fixed(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *p
     return lepic_fail(ctx, LEPIC_ERR_ARG, "deposit_fixed: particle arrays must be 16-byte aligned");
   if (!kq || !np_local) return lepic_fail(ctx, LEPIC_ERR_ARG, "deposit_fixed: null argument");
   const double k1 = Nm / (g->h[0] * g->h[1] * g->h[2]);
+  if (k1 == 0.0) return LEPIC_OK;   // every contribution is zero
   const double unit = std::ldexp(std::fabs(k1), frac_bits);   // one full weight, in units
   if (frac_bits < -1000 || frac_bits > 1000 || !(unit >= 1.0 && unit <= 281474976710656.0))
     return lepic_fail(ctx, LEPIC_ERR_ARG,

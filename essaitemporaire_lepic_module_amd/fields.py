@@ -50,6 +50,8 @@ def fixed_point_bits(n, h, Nm_species: float) -> int:
     to at most 2^44 units, leaving a node's int64 total room for 2^19
     particles."""
     k1 = abs(float(Nm_species) / (h[0] * h[1] * h[2]))
+    if k1 == 0.0:
+        return 0        # nothing is deposited
     return 44 - math.ceil(math.log2(k1))
 
 

@@ -210,7 +210,8 @@ int lepic_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *p, cons
  * The result does not depend on particle order or atomic timing, and the grid
  * total is exactly the sum of the rounded contributions; each contribution
  * differs from the FP64 one by at most 2^-(frac_bits+1).  |k1| * 2^frac_bits
- * must lie in [1, 2^48] (LEPIC_ERR_ARG otherwise), so a node's int64 total
+ * must lie in [1, 2^48] (LEPIC_ERR_ARG otherwise; Nm = 0 deposits nothing and
+ * succeeds), so a node's int64 total
  * cannot overflow below 2^15 particles touching it (each adds at most one
  * full weight to a node). */
 int lepic_deposit_fixed(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *p,

@@ -371,6 +371,9 @@ def test_deposit_fixed_bitexact(gpu, oracle, rng, presort):
     _same(outs[1], ref)
     with pytest.raises(LepicError):
         F.deposit_particle_set_to_np_thread_fixed(ps, n, h, kqg, Nm, npg, frac_bits=S + 10)
+    before = npg.clone()
+    F.deposit_particle_set_to_np_thread_fixed(ps, n, h, kqg, 0.0, npg)   # zero weight: no-op
+    assert torch.equal(npg, before)
 
 
 def test_deposit_unsorted_falls_back(gpu, oracle, rng):
