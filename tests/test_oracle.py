@@ -190,6 +190,17 @@ def test_deposit_fixed_point(oracle, rng):
     assert abs(total - k1 * inside.sum()) <= inside.sum() * (4 * unit + 1e-14 * k1)
 
 
+def test_fixed_point_bits():
+    """A full weight |Nm/(h1 h2 h3)| maps to (2^43, 2^44] units; zero weight
+    needs no scale."""
+    from essaitemporaire_lepic_module_amd import fields as F
+    for Nm, h in ((1.7e3, (3.125e-4,) * 3), (2.3e-7, (1e-2, 2e-2, 5e-3)), (-64.0, (1.0, 1.0, 1.0))):
+        k1 = abs(Nm / (h[0] * h[1] * h[2]))
+        S = F.fixed_point_bits((4, 4, 4), h, Nm)
+        assert 2.0 ** 43 < k1 * 2.0 ** S <= 2.0 ** 44
+    assert F.fixed_point_bits((4, 4, 4), (1.0, 1.0, 1.0), 0.0) == 0
+
+
 # ------------------------------------------------------------------- a4: Boris
 def test_boris_rotation_closed_form(oracle, rng):
     """E = 0, uniform Bz: |v| conserved and the gyration angle per step is
