@@ -76,23 +76,37 @@ def _cpu_chunk_in_pool(job):
     return _cpu_chunk(job + (_BARRIER,))
 
 
+def host_cores():
+    """Cores this process may run on: its CPU affinity, capped by the cgroup
+    CPU quota when one is set (a container's share of a larger host)."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        if quota != "max":
+            cores = min(cores, max(1, int(int(quota) // int(period))))
+    except (OSError, ValueError):
+        pass
+    return max(1, cores)
+
+
 def cpu_baseline(case, sample_particles=32_000_000):
-    """The oracle (C restatement of the reference) on the host cores of this
-    machine, one process per core (spawned, so none of them touches the GPU):
-    each pushes, wall-tests and deposits its share of a bounded particle
-    sample on the 128^3 grid at the same moment; plus one warm MG solve and
-    E + rho at 128^3 (single-threaded, as the oracle is); composed into
-    particle-steps/s of the full workload."""
+    """The oracle (C restatement of the reference) on every host core this
+    process may use, one process per core (spawned, so none of them touches
+    the GPU): each pushes, wall-tests and deposits its share of a bounded
+    particle sample on the 128^3 grid at the same moment; one warm MG solve
+    with the multigrid loops on OpenMP threads over the same cores; E + rho
+    at 128^3 on one core; composed into particle-steps/s of the full
+    workload."""
     import multiprocessing as mp
     import numpy as np
     from oracle import oracle as O
     O.build()
     n, h = case.n, case.h
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(16, cores))
+    cores = host_cores()
     sp = case.species[0]
     bc, kq = case.bcnd, case.kq()
     m = sample_particles
@@ -105,14 +119,16 @@ def cpu_baseline(case, sample_particles=32_000_000):
         secs = pool.map(_cpu_chunk_in_pool, jobs)
     t_part = max(secs) / m              # seconds per particle on `cores` cores together
     # grid phase: rho from a smooth density, then one solve warm-started from its own solution
+    os.environ["OMP_NUM_THREADS"] = str(cores)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     rng = np.random.default_rng(7)
     rho = np.zeros(bc.shape)
     rho[bc == -1] = 1e-5 * rng.standard_normal(int((bc == -1).sum()))
     phi = np.zeros(bc.shape)
-    O.pdesolver(n, h, phi, rho, bc, ng=case.ng, eps=case.eps, omega=case.omega)
+    O.pdesolver_threaded(n, h, phi, rho, bc, ng=case.ng, eps=case.eps, omega=case.omega)
     rho *= 1.001
     t0 = time.perf_counter()
-    O.pdesolver(n, h, phi, rho, bc, ng=case.ng, eps=case.eps, omega=case.omega)
+    O.pdesolver_threaded(n, h, phi, rho, bc, ng=case.ng, eps=case.eps, omega=case.omega)
     t_mg = time.perf_counter() - t0
     t0 = time.perf_counter()
     O.efield(n, h, phi, bc)
@@ -125,8 +141,8 @@ def cpu_baseline(case, sample_particles=32_000_000):
             "sample": (f"oracle on {cores} host cores (one process each, concurrent): Boris "
                        f"push+wall BC+deposit of {m} particles on the {n[0]}^3 grid "
                        f"({t_part * 1e9:.1f} ns/particle for the cores together), one warm MG "
-                       f"solve ({t_mg * 1e3:.0f} ms) and E+rho ({t_grid * 1e3:.0f} ms) on one "
-                       f"core, composed to the {N}-particle step")}
+                       f"solve on {cores} OpenMP threads ({t_mg * 1e3:.0f} ms) and E+rho "
+                       f"({t_grid * 1e3:.0f} ms) on one core, composed to the {N}-particle step")}
 
 
 def main():

@@ -193,6 +193,29 @@ __device__ inline void deposit_one(const KGrid &g, double *np, const double *__r
   for (int q = 0; q < 8; ++q) atomicAdd(&np[nd[q]], sign * (kq[nd[q]] * (k1 * w[q])));
 }
 
+// Fixed-point deposit: one particle's full weight U (an integer number of
+// units of 2^-S, held in a double below 2^53) split over its 8 stencil nodes
+// in the order w1..w8 of the trilinear stencil (lower corner, +x, +x+y, +y,
+// then the same at +z).  The weight is divided along x, then y, then z: at
+// each division the upper node takes rint(parent * share) and the lower node
+// the rest, so the 8 parts are integers that sum to U exactly -- charge is
+// conserved bit for bit -- and each part is within 1.5 units of U times its
+// trilinear weight.  ux, uy, uz are the upper nodes' shares (1 - px of the
+// FP64 stencil, the in-cell offset of the FP32 layout).
+__host__ __device__ inline void fixed_split(double U, double ux, double uy, double uz,
+                                            long long a[8]) {
+  const double x1 = rint(U * ux), x0 = U - x1;
+  const double x0y1 = rint(x0 * uy), x0y0 = x0 - x0y1;
+  const double x1y1 = rint(x1 * uy), x1y0 = x1 - x1y1;
+  const double c[4] = {x0y0, x1y0, x1y1, x0y1};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double z1 = rint(c[q] * uz);
+    a[q] = (long long)(c[q] - z1);
+    a[q + 4] = (long long)z1;
+  }
+}
+
 // make_stencil with the quotient range test taken once for the particle
 // (ok = qdiv_coord_ok of x, y and z): the same stencil for every input.
 __device__ inline void make_stencil_pre(const KGrid &g, double xp, double yp, double zp, bool ok,

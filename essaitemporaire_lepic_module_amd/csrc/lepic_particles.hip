@@ -1178,15 +1178,16 @@ __device__ inline void load_pos_run(const KPset &p, int64_t n, int64_t i0, bool 
   }
 }
 
-// Fixed-point deposit.  Each of a particle's 8 contributions k1*w (the
-// weights and products of deposit_particle_set_to_np_thread,
-// mod_chargeDeposition.f90:25-133) is rounded to the nearest multiple of
-// 2^-frac_bits and summed as an int64, so the node totals do not depend on
-// the order in which particles arrive, and the charge summed over the grid is
-// exactly the sum of the rounded contributions.  A lane sums its run of one
-// cell in registers and adds it to the block's LDS box (or, outside it, to
-// this XCD's int64 replica); two's complement wrap-around makes unsigned adds
-// of negative values exact.
+// Fixed-point deposit.  Every particle carries the same full weight U =
+// rint(|k1| 2^frac_bits) units (k1 = Nm / (h1 h2 h3)), split over its 8
+// stencil nodes by fixed_split with the trilinear fractions of
+// deposit_particle_set_to_np_thread (mod_chargeDeposition.f90:90-131): the
+// parts are integers summing to U exactly, so the charge on the grid is N U
+// units for N deposited particles, and int64 node sums do not depend on the
+// order in which particles arrive.  A lane sums its run of one cell in
+// registers and adds it to the block's LDS box (or, outside it, to this
+// XCD's int64 replica); two's complement wrap-around makes unsigned adds of
+// negative values exact.
 // The block's cells +-1 get an LDS box of int64 sums (the FP64 deposit's
 // box, 32 KB); stencils outside it go to the replica directly.
 struct DepSharedFixed {
@@ -1215,12 +1216,9 @@ __device__ inline void flush_fixed(const KGrid &g, DepSharedFixed &ds, int key,
   }
 }
 
-__device__ inline long long to_fixed(double w, double scale) {
-  return (long long)rint(w * scale);
-}
-
-__global__ __launch_bounds__(BLOCK) void k_deposit_fixed(KGrid g, KPset p, int64_t n, double k1,
-                                                         double scale,
+// U: a full weight in units (rint(|k1| 2^S)); neg: k1 < 0.
+__global__ __launch_bounds__(BLOCK) void k_deposit_fixed(KGrid g, KPset p, int64_t n, double U,
+                                                         int neg,
                                                          unsigned long long *rep_base,
                                                          int64_t nodes) {
   __shared__ DepSharedFixed ds;
@@ -1254,14 +1252,10 @@ __global__ __launch_bounds__(BLOCK) void k_deposit_fixed(KGrid g, KPset p, int64
       for (int j = 0; j < 8; ++j) a[j] = 0;
     }
     const double wx2 = 1.0 - px, wy2 = 1.0 - py, wz2 = 1.0 - pz;
-    a[0] += to_fixed(k1 * px * py * pz, scale);
-    a[1] += to_fixed(k1 * wx2 * py * pz, scale);
-    a[2] += to_fixed(k1 * wx2 * wy2 * pz, scale);
-    a[3] += to_fixed(k1 * px * wy2 * pz, scale);
-    a[4] += to_fixed(k1 * px * py * wz2, scale);
-    a[5] += to_fixed(k1 * wx2 * py * wz2, scale);
-    a[6] += to_fixed(k1 * wx2 * wy2 * wz2, scale);
-    a[7] += to_fixed(k1 * px * wy2 * wz2, scale);
+    long long parts[8];
+    fixed_split(U, wx2, wy2, wz2, parts);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[j] += neg ? -parts[j] : parts[j];
   }
   flush_fixed(g, ds, cur, a, rep);
   __syncthreads();
@@ -1747,9 +1741,10 @@ int lepic_deposit_fixed(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *p
   const double k1 = Nm / (g->h[0] * g->h[1] * g->h[2]);
   if (k1 == 0.0) return LEPIC_OK;   // every contribution is zero
   const double unit = std::ldexp(std::fabs(k1), frac_bits);   // one full weight, in units
-  if (frac_bits < -1000 || frac_bits > 1000 || !(unit >= 1.0 && unit <= 281474976710656.0))
+  if (frac_bits < -1000 || frac_bits > 1000 || !(unit >= 1.0 && unit <= 17592186044416.0))
     return lepic_fail(ctx, LEPIC_ERR_ARG,
-                      "deposit_fixed: |Nm/(h1 h2 h3)| * 2^frac_bits must lie in [1, 2^48]");
+                      "deposit_fixed: |Nm/(h1 h2 h3)| * 2^frac_bits must lie in [1, 2^44]");
+  const double U = std::rint(unit);
   if (p->n == 0) return LEPIC_OK;
   const int64_t nodes = grid_nodes(g);
   double *rep = lepic_replicas(ctx, nodes);
@@ -1758,8 +1753,7 @@ int lepic_deposit_fixed(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *p
     return lepic_fail(ctx, LEPIC_ERR_ARG, "deposit_fixed: replicas smaller than the grid");
   unsigned long long *irep = reinterpret_cast<unsigned long long *>(rep);   // zero bits either way
   hipLaunchKernelGGL(k_deposit_fixed, dim3(grid_blocks(p->n, CTILE)), dim3(BLOCK), 0,
-                     ctx->stream, make_kgrid(g), kp(p), p->n, k1, std::ldexp(1.0, frac_bits), irep,
-                     nodes);
+                     ctx->stream, make_kgrid(g), kp(p), p->n, U, k1 < 0.0 ? 1 : 0, irep, nodes);
   LEPIC_LAUNCH_CHECK(ctx, "k_deposit_fixed");
   hipLaunchKernelGGL(k_fixed_flush, dim3(grid_blocks(nodes, 256)), dim3(256), 0, ctx->stream,
                      np_local, irep, nodes, kq, std::ldexp(1.0, -frac_bits));

@@ -59,9 +59,11 @@ def deposit_particle_set_to_np_thread_fixed(part: ParticleSet, n, h, kq: torch.T
                                             Nm_species: float, np_local: torch.Tensor,
                                             frac_bits: int | None = None):
     """deposit_particle_set_to_np_thread (mod_chargeDeposition.f90:25-133)
-    with every contribution rounded to a multiple of 2^-frac_bits and summed
-    as int64: the density does not depend on particle order, and the grid
-    total is the exact sum of the rounded contributions."""
+    in fixed point: each particle's full weight, rint(|Nm/(h1 h2 h3)| *
+    2^frac_bits) units, is split into 8 integer node parts that sum to it
+    exactly and summed as int64, so the density does not depend on particle
+    order and the charge on the grid is exactly the number of deposited
+    particles times one full weight."""
     if frac_bits is None:
         frac_bits = fixed_point_bits(n, h, Nm_species)
     ctx = context(part.device)

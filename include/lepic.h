@@ -204,16 +204,19 @@ int lepic_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *p, cons
                   double Nm, double *np_local);
 
 /* deposit_particle_set_to_np_thread (mod_chargeDeposition.f90:25-133) in
- * fixed point: each particle's 8 contributions k1*w (k1 = Nm/(h1 h2 h3), the
- * reference's weights and products) are rounded to the nearest multiple of
- * 2^-frac_bits and summed as int64, then np_local += kq * sum * 2^-frac_bits.
- * The result does not depend on particle order or atomic timing, and the grid
- * total is exactly the sum of the rounded contributions; each contribution
- * differs from the FP64 one by at most 2^-(frac_bits+1).  |k1| * 2^frac_bits
- * must lie in [1, 2^48] (LEPIC_ERR_ARG otherwise; Nm = 0 deposits nothing and
- * succeeds), so a node's int64 total
- * cannot overflow below 2^15 particles touching it (each adds at most one
- * full weight to a node). */
+ * fixed point: every particle's full weight U = rint(|k1| 2^frac_bits) units
+ * (k1 = Nm/(h1 h2 h3)) is split over its 8 stencil nodes with the
+ * reference's trilinear fractions -- divided along x, then y, then z, the
+ * upper node of each division taking rint(parent * share) and the lower one
+ * the rest -- so the 8 parts are integers that sum to U exactly; they are
+ * summed per node as int64, then np_local += kq * sum * 2^-frac_bits (parts
+ * negated when k1 < 0).  The result does not depend on particle order or
+ * atomic timing, the charge on the grid is exactly N U units for N
+ * deposited particles, and each part is within 1.5 units of U times its
+ * trilinear weight.  |k1| * 2^frac_bits must lie in [1, 2^44] (LEPIC_ERR_ARG
+ * otherwise; Nm = 0 deposits nothing and succeeds).  No part exceeds U, so a
+ * node's int64 sum cannot overflow while fewer than 2^19 particles touch it;
+ * past that the node total wraps without an error. */
 int lepic_deposit_fixed(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *p,
                         const double *kq, double Nm, int32_t frac_bits, double *np_local);
 /* reduce_species_density (mod_density.f90:31-73), one rank, one virtual

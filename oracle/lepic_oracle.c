@@ -21,6 +21,19 @@
 #define NODES(g) (NX(g) * NY(g) * NZ(g))
 #define IDX(g, i, j, k) ((int64_t)(i) + NX(g) * ((int64_t)(j) + NY(g) * (int64_t)(k)))
 
+/* The multigrid loops below are marked for OpenMP.  The parity library is
+ * built without -fopenmp, so they run serially in source order; the host
+ * baseline's copy (liblepic_oracle_omp.so) runs them on every core, with the
+ * residual summed per thread (a different rounding order, used for timing
+ * only). */
+#ifdef _OPENMP
+#define LO_PARALLEL_FOR _Pragma("omp parallel for schedule(static)")
+#define LO_PARALLEL_SUM_RES _Pragma("omp parallel for schedule(static) reduction(+ : res)")
+#else
+#define LO_PARALLEL_FOR
+#define LO_PARALLEL_SUM_RES
+#endif
+
 static const double QE = 1.60217646e-19;           /* mod_constants.f90 qe */
 static const double PI_F = 3.14159265358979311600; /* 4*atan(1) in double  */
 
@@ -550,11 +563,28 @@ void lo_deposit(const lo_grid *g, const lo_pset *p, const double *kq, double Nm,
   }
 }
 
-/* lo_deposit in fixed point (the contract of lepic_deposit_fixed): each
- * contribution k1*w, formed as lo_deposit forms it, is rounded to the nearest
- * multiple of 2^-frac_bits (rint, ties to even) and summed as a wrapping
- * 64-bit integer per node; then np += kq * ((double)sum * 2^-frac_bits) at
- * every node. */
+/* The split of one particle's full weight U (integer units held in a double)
+ * over its 8 stencil nodes, w1..w8 order: divided along x, then y, then z,
+ * the upper node taking rint(parent * share) and the lower node the rest, so
+ * the parts are integers summing to U exactly.  ux, uy, uz: upper shares. */
+void lo_fixed_split(double U, double ux, double uy, double uz, int64_t a[8]) {
+  const double x1 = rint(U * ux), x0 = U - x1;
+  const double x0y1 = rint(x0 * uy), x0y0 = x0 - x0y1;
+  const double x1y1 = rint(x1 * uy), x1y0 = x1 - x1y1;
+  const double c[4] = {x0y0, x1y0, x1y1, x0y1};
+  for (int q = 0; q < 4; ++q) {
+    const double z1 = rint(c[q] * uz);
+    a[q] = (int64_t)(c[q] - z1);
+    a[q + 4] = (int64_t)z1;
+  }
+}
+
+/* lo_deposit in fixed point (the contract of lepic_deposit_fixed): with the
+ * fractions lo_deposit forms (same guard, same arithmetic), each particle's
+ * full weight U = rint(|k1| 2^frac_bits) is split by lo_fixed_split (upper
+ * shares 1 - px, 1 - py, 1 - pz) and the parts are summed per node as
+ * wrapping 64-bit integers (negated when k1 < 0); then
+ * np += kq * ((double)sum * 2^-frac_bits) at every node. */
 void lo_deposit_fixed(const lo_grid *g, const lo_pset *p, const double *kq, double Nm,
                       int32_t frac_bits, double *np_local) {
   const double *h = g->h;
@@ -563,7 +593,8 @@ void lo_deposit_fixed(const lo_grid *g, const lo_pset *p, const double *kq, doub
   double xmax = (double)g->n[0] * h[0], ymax = (double)g->n[1] * h[1],
          zmax = (double)g->n[2] * h[2];
   double k1 = Nm / (h[0] * h[1] * h[2]);
-  const double scale = ldexp(1.0, frac_bits), inv = ldexp(1.0, -frac_bits);
+  const double U = rint(ldexp(fabs(k1), frac_bits)), inv = ldexp(1.0, -frac_bits);
+  const int neg = k1 < 0.0;
   for (int64_t i = 0; i < p->n; ++i) {
     if (p->dead[i] != 0) continue;
     double x = p->x[i], y = p->y[i], z = p->z[i];
@@ -577,20 +608,13 @@ void lo_deposit_fixed(const lo_grid *g, const lo_pset *p, const double *kq, doub
     double py = ((double)iy * h[1] - y) / h[1];
     double pz = ((double)iz * h[2] - z) / h[2];
     double wx2 = 1.0 - px, wy2 = 1.0 - py, wz2 = 1.0 - pz;
-    double w[8];
-    w[0] = k1 * px * py * pz;
-    w[1] = k1 * wx2 * py * pz;
-    w[2] = k1 * wx2 * wy2 * pz;
-    w[3] = k1 * px * wy2 * pz;
-    w[4] = k1 * px * py * wz2;
-    w[5] = k1 * wx2 * py * wz2;
-    w[6] = k1 * wx2 * wy2 * wz2;
-    w[7] = k1 * px * wy2 * wz2;
+    int64_t a[8];
+    lo_fixed_split(U, wx2, wy2, wz2, a);
     const int64_t c[8] = {IDX(g, ix, iy, iz),         IDX(g, ix + 1, iy, iz),
                           IDX(g, ix + 1, iy + 1, iz), IDX(g, ix, iy + 1, iz),
                           IDX(g, ix, iy, iz + 1),     IDX(g, ix + 1, iy, iz + 1),
                           IDX(g, ix + 1, iy + 1, iz + 1), IDX(g, ix, iy + 1, iz + 1)};
-    for (int q = 0; q < 8; ++q) acc[c[q]] += (uint64_t)(int64_t)rint(w[q] * scale);
+    for (int q = 0; q < 8; ++q) acc[c[q]] += (uint64_t)(neg ? -a[q] : a[q]);
   }
   for (int64_t t = 0; t < nodes; ++t)
     np_local[t] = np_local[t] + kq[t] * ((double)(int64_t)acc[t] * inv);
@@ -889,6 +913,7 @@ static void sor_colour(const mg_level *L, int colour, double omega) {
   const double *b = L->b;
   const int32_t *bc = L->bcnd;
   const int m = n[2];
+  LO_PARALLEL_FOR
   for (int k = 1; k <= m + 1; ++k)
     for (int j = 1; j <= n[1] + 1; ++j) {
       int ks = (k % 2 == 0) ? 0 : 1;
@@ -928,6 +953,7 @@ static double sor_residual(const mg_level *L) {
   const int32_t *n = g->n;
   const double *u = L->u, *b = L->b;
   double res = 0.0;
+  LO_PARALLEL_SUM_RES
   for (int k = 1; k <= n[2] + 1; ++k)
     for (int j = 1; j <= n[1] + 1; ++j)
       for (int i = 1; i <= n[0] + 1; ++i) {
@@ -1011,6 +1037,7 @@ static void restriction(const mg_level *F, mg_level *C, int flag_pbc, int flag_n
   }
   memset(C->u, 0, sizeof(double) * (size_t)NODES(gc));
   const int N1 = n[0] + 1, N2 = n[1] + 1, M1 = m + 1;
+  LO_PARALLEL_FOR
   for (int k = km; k <= kp; ++k)
     for (int j = 1; j <= n[1] / 2 + 1; ++j)
       for (int i = 1; i <= n[0] / 2 + 1; ++i) {
@@ -1139,6 +1166,7 @@ static void prolongation(mg_level *F, const mg_level *C) {
   double *u = F->u;
   const double *e = C->u;
   const int32_t *bc = F->bcnd;
+  LO_PARALLEL_FOR
   for (int k = 1; k <= kp; ++k)
     for (int j = 1; j <= n[1] / 2 + 1; ++j)
       for (int i = 1; i <= n[0] / 2 + 1; ++i) {

@@ -107,6 +107,7 @@ void lo_dielectric_phi(const lo_grid *g, const int32_t *bcnd, const int32_t *dty
 void lo_build_kq(const lo_grid *g, const int32_t *bcnd, double *kq);
 void lo_deposit(const lo_grid *g, const lo_pset *p, const double *kq,
                 double Nm, double *np_local);
+void lo_fixed_split(double U, double ux, double uy, double uz, int64_t a[8]);
 void lo_deposit_fixed(const lo_grid *g, const lo_pset *p, const double *kq, double Nm,
                       int32_t frac_bits, double *np_local);
 void lo_reduce_density(const lo_grid *g, const int32_t *bcnd, int32_t ntype,

@@ -117,6 +117,8 @@ def lib():
         L.lo_deposit.argtypes = [C.POINTER(Grid), C.POINTER(PSet), dp, C.c_double, dp]
         L.lo_deposit_fixed.argtypes = [C.POINTER(Grid), C.POINTER(PSet), dp, C.c_double,
                                        C.c_int32, dp]
+        L.lo_fixed_split.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.POINTER(C.c_int64)]
         L.lo_reduce_density.argtypes = [C.POINTER(Grid), ip, C.c_int32, C.c_int32, dp, dp]
         L.lo_build_rho.argtypes = [C.POINTER(Grid), C.c_int32, dp, dp, dp]
         L.lo_pdesolver.restype = C.c_int
@@ -319,6 +321,13 @@ def build_rho(n, np_red, charge):
     return rho
 
 
+def fixed_split(U, ux, uy, uz):
+    """The oracle's split of a full weight U over the 8 stencil nodes."""
+    a = np.zeros(8, np.int64)
+    lib().lo_fixed_split(float(U), float(ux), float(uy), float(uz), _p(a, C.c_int64))
+    return a
+
+
 def pdesolver(n, h, phi, rho, bcnd, flag_pbc=0, flag_nmn=0, eps0=8.854187817e-12, ncycl=20000,
               eps=1e-8, omega=1.4, ng=5):
     g = grid(n, h)
@@ -327,6 +336,35 @@ def pdesolver(n, h, phi, rho, bcnd, flag_pbc=0, flag_nmn=0, eps0=8.854187817e-12
                                 int(flag_nmn), eps0, int(ncycl), eps, omega, int(ng), C.byref(st))
     return status, {"cycles": st.cycles, "ktot": st.ktot, "res": st.res,
                     "sweeps": [st.sweeps[i] for i in range(int(ng))]}
+
+
+_omp = None
+
+
+def pdesolver_threaded(n, h, phi, rho, bcnd, flag_pbc=0, flag_nmn=0, eps0=8.854187817e-12,
+                       ncycl=20000, eps=1e-8, omega=1.4, ng=5):
+    """pdesolver from liblepic_oracle_omp.so: the same source with its
+    multigrid loops on OpenMP threads (OMP_NUM_THREADS, read when the library
+    first starts its threads).  Timing only: its residual sums round in
+    another order, so it is not the parity checker."""
+    global _omp
+    if _omp is None:
+        build()
+        path = os.path.join(_HERE, "_build", "liblepic_oracle_omp.so")
+        if not os.path.exists(path):
+            build(force=True)
+        L = C.CDLL(path)
+        dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int32)
+        L.lo_pdesolver.restype = C.c_int
+        L.lo_pdesolver.argtypes = [C.POINTER(Grid), dp, dp, ip, C.c_int, C.c_int, C.c_double,
+                                   C.c_int32, C.c_double, C.c_double, C.c_int32,
+                                   C.POINTER(MGStats)]
+        _omp = L
+    g = grid(n, h)
+    st = MGStats()
+    status = _omp.lo_pdesolver(C.byref(g), _p(phi), _p(rho), _p(bcnd, C.c_int32), int(flag_pbc),
+                               int(flag_nmn), eps0, int(ncycl), eps, omega, int(ng), C.byref(st))
+    return status, {"cycles": st.cycles, "ktot": st.ktot, "res": st.res}
 
 
 def efield(n, h, phi, bcnd):

@@ -391,6 +391,46 @@ def test_deposit_unsorted_falls_back(gpu, oracle, rng):
     np.testing.assert_allclose(npg.cpu().numpy(), npo, rtol=RTOL_ATOMIC, atol=1e-30)
 
 
+@pytest.mark.parametrize("layout", ["unsorted", "sorted_far"])
+def test_deposit_fixed_replica_path_bitexact(gpu, oracle, rng, layout):
+    """The fixed-point deposit's paths that bypass the LDS box: unsorted
+    particles on a 40^3 grid (every block's box overflows, so every stencil
+    goes to the int64 replicas), and a sorted set salted with far-away
+    particles (their stencils leave the box of a block whose box fits).
+    Bit-identical to the oracle in two particle orders, and the charge on the
+    grid is exactly the deposited count times one full weight."""
+    from essaitemporaire_lepic_module_amd import fields as F
+    from essaitemporaire_lepic_module_amd import particles as P
+    n, h = H.box((40, 40, 40))
+    bc = cfg.grounded_box_bcnd(n)
+    kq = cfg.build_kq(bc)
+    count = 60000
+    arr = [a.copy() for a in H.random_particles(rng, n, h, count)]
+    po = _oracle_set(oracle, arr)
+    if layout == "sorted_far":
+        oracle.sort_particles(n, h, po)
+        a = po.arrays()
+        arr = [a[k].copy() for k in ("x", "y", "z", "vx", "vy", "vz")]
+        far = rng.choice(count, 300, replace=False)
+        L = np.array(n) * np.array(h)
+        for d in range(3):
+            arr[d][far] = L[d] * rng.random(far.size)
+        po = _oracle_set(oracle, arr)
+    Nm = 1526.0
+    kqg = torch.as_tensor(kq, device=gpu)
+    S = None
+    for order in (np.arange(count), rng.permutation(count)):
+        ps = _gpu_set(P, [c[order] for c in arr])
+        npg = torch.zeros(cfg.node_shape(n), dtype=torch.float64, device=gpu)
+        S = F.deposit_particle_set_to_np_thread_fixed(ps, n, h, kqg, Nm, npg)
+        ref = oracle.deposit_fixed(n, h, po, kq, Nm, S)
+        _same(npg.cpu().numpy(), ref)
+    k1 = Nm / (h[0] * h[1] * h[2])
+    U = round(k1 * 2.0 ** S)
+    units = np.rint(ref / kq * 2.0 ** S).astype(np.int64)
+    assert int(units.sum()) == count * U
+
+
 @pytest.mark.parametrize("case", ["mixed", "empty", "all_dead", "one_cell", "wide"])
 def test_bin_compact_groups_by_cell(gpu, oracle, rng, case):
     """Binning keeps exactly the live particles, in nondecreasing cell order,

@@ -152,10 +152,11 @@ def test_deposit_kq_and_skips(oracle, rng):
 
 def test_deposit_fixed_point(oracle, rng):
     """The fixed-point deposit: bitwise the same for any particle order; each
-    node within (its contributions) x 2^-(S+1) x kq of the FP64 deposit;
-    exactly zero where nothing lands; the grid total, divided by kq, within
-    4 units of 2^-S per particle of k1 per particle (the weights of a particle
-    sum to 1)."""
+    node within 1.5 units of 2^-S per contribution (times kq) of the FP64
+    deposit, which is pinned to the reference; exactly zero where nothing
+    lands; and the charge exactly conserved: the grid total divided by kq is
+    the number of deposited particles times one full weight U = rint(k1 2^S),
+    with no rounding at all."""
     n, h = H.box((7, 6, 5))
     bc = cfg.grounded_box_bcnd(n)
     kq = oracle.build_kq(n, bc)
@@ -184,10 +185,34 @@ def test_deposit_fixed_point(oracle, rng):
                        (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]:
         np.add.at(cnt, (iz + dk, iy + dj, ix + di), 1.0)
     unit = 2.0 ** -S
-    np.testing.assert_array_less(np.abs(fixed - ref), cnt * kq * unit * 0.5 + 1e-13 * np.abs(ref) + 1e-300)
+    np.testing.assert_array_less(np.abs(fixed - ref),
+                                 cnt * kq * unit * 1.5 + 1e-13 * np.abs(ref) + 1e-300)
     assert (fixed[cnt == 0] == 0.0).all() and g[cnt > 0].mean() > 0.5
-    total = (fixed / kq).sum()
-    assert abs(total - k1 * inside.sum()) <= inside.sum() * (4 * unit + 1e-14 * k1)
+    # exact conservation, in integer units (every node sum is below 2^53 here)
+    U = round(k1 * 2.0 ** S)
+    units = np.rint(fixed / kq * 2.0 ** S).astype(np.int64)
+    np.testing.assert_array_equal(units.astype(np.float64) * unit, fixed / kq)
+    assert int(units.sum()) == int(inside.sum()) * U
+
+
+def test_fixed_split_exact(oracle, rng):
+    """The split of one full weight over the 8 stencil nodes: integer parts
+    summing to U exactly, each within 1.5 units of U times its trilinear
+    weight (brute force over random shares, the corners 0 and 1 included),
+    upper-node parts zero when the upper shares are zero."""
+    U = float(2 ** 44 - 12345)
+    shares = np.concatenate([rng.random((2000, 3)), np.array([[0.0, 0.0, 0.0], [1.0, 1.0, 1.0],
+                                                             [0.5, 0.0, 1.0]])])
+    for ux, uy, uz in shares:
+        a = oracle.fixed_split(U, ux, uy, uz)
+        assert int(a.sum()) == int(U)
+        wx, wy, wz = (1 - ux, ux), (1 - uy, uy), (1 - uz, uz)
+        order = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1),
+                 (0, 1, 1)]
+        w = np.array([wx[i] * wy[j] * wz[k] for i, j, k in order])
+        assert np.all(np.abs(a - U * w) <= 1.5), (ux, uy, uz)
+    a = oracle.fixed_split(U, 0.0, 0.0, 0.0)
+    assert a[0] == int(U) and not a[1:].any()
 
 
 def test_fixed_point_bits():
