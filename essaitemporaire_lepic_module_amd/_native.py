@@ -31,6 +31,9 @@ EXPORTS = (
     "lepic_collisions", "lepic_collisions_deposited", "lepic_push_bc_deposit",
     "lepic_push_bc_deposit_inplace", "lepic_sort_compact", "lepic_bin_compact",
     "lepic_plane_moments", "lepic_electron_heating", "lepic_dielectric_phi",
+    "lepic_pset_to32", "lepic_pset_from32", "lepic_round_field4", "lepic_push32",
+    "lepic_deposit32", "lepic_fixed_density", "lepic_sort32", "lepic_collisions32",
+    "lepic_poisson_solve_mixed",
 )
 
 
@@ -69,6 +72,17 @@ class Heating(C.Structure):
                 ("mass", C.c_double), ("vt0", C.c_double)]
 
 
+class PSet32(C.Structure):
+    _fields_ = [("cell", C.c_void_p), ("fx", C.c_void_p), ("fy", C.c_void_p),
+                ("fz", C.c_void_p), ("vx", C.c_void_p), ("vy", C.c_void_p),
+                ("vz", C.c_void_p), ("n", C.c_int64), ("cap", C.c_int64)]
+
+
+class BField32(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("B", C.c_double * 3), ("B4", C.c_void_p),
+                ("magnetized", C.c_int32)]
+
+
 class BField(C.Structure):
     _fields_ = [("mode", C.c_int32), ("B", C.c_double * 3), ("B4", C.c_void_p),
                 ("magnetized", C.c_int32), ("nB", C.c_int32 * 3), ("hB", C.c_double * 3)]
@@ -78,6 +92,11 @@ class MGParams(C.Structure):
     _fields_ = [("flag_pbc", C.c_int32), ("flag_nmn", C.c_int32), ("eps0", C.c_double),
                 ("ncycl", C.c_int32), ("eps", C.c_double), ("omega", C.c_double),
                 ("ng", C.c_int32), ("graph", C.c_int32)]
+
+
+class MGMixParams(C.Structure):
+    _fields_ = [("inner_cycles", C.c_int32), ("nu", C.c_int32), ("nu_coarsest", C.c_int32),
+                ("omega", C.c_double)]
 
 
 class MGStats(C.Structure):
@@ -116,7 +135,7 @@ def load() -> C.CDLL:
                               "essaitemporaire_lepic_module_amd.build.build() (hipcc, gfx950)")
         L = C.CDLL(LIB_PATH)
         vp, i32p, dp = C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double)
-        G, PS = C.POINTER(Grid), C.POINTER(PSet)
+        G, PS, P32 = C.POINTER(Grid), C.POINTER(PSet), C.POINTER(PSet32)
         sig = {
             "lepic_create": (C.c_int, [C.c_int32, C.POINTER(vp)]),
             "lepic_destroy": (C.c_int, [vp]),
@@ -145,6 +164,8 @@ def load() -> C.CDLL:
             "lepic_build_rho": (C.c_int, [vp, G, C.c_int32, vp, dp, vp]),
             "lepic_poisson_solve": (C.c_int, [vp, G, vp, vp, vp, C.POINTER(MGParams),
                                               C.POINTER(MGStats)]),
+            "lepic_poisson_solve_mixed": (C.c_int, [vp, G, vp, vp, vp, C.POINTER(MGParams),
+                                                    C.POINTER(MGMixParams), C.POINTER(MGStats)]),
             "lepic_efield": (C.c_int, [vp, G, vp, vp, vp]),
             "lepic_collisions": (C.c_int, [vp, G, PS, vp, C.POINTER(Chem), C.c_int32, C.c_double,
                                            C.c_int32, i32p, C.c_uint64, C.c_uint64, vp,
@@ -168,6 +189,18 @@ def load() -> C.CDLL:
                                                  C.c_uint64, C.c_uint64, vp, dp]),
             "lepic_dielectric_phi": (C.c_int, [vp, G, vp, vp, C.c_int32, C.c_int32, vp, vp,
                                                C.c_double, vp]),
+            "lepic_pset_to32": (C.c_int, [vp, G, PS, P32, C.POINTER(C.c_int64)]),
+            "lepic_pset_from32": (C.c_int, [vp, G, P32, PS]),
+            "lepic_round_field4": (C.c_int, [vp, vp, C.c_int64, vp]),
+            "lepic_push32": (C.c_int, [vp, G, P32, vp, C.POINTER(BField32), C.c_double,
+                                       C.c_double, C.c_double, vp, C.POINTER(BCParams),
+                                       C.c_int32, vp, vp, vp, vp, vp]),
+            "lepic_deposit32": (C.c_int, [vp, G, P32, C.c_double, C.c_int32, vp]),
+            "lepic_fixed_density": (C.c_int, [vp, G, vp, vp, C.c_int32, vp]),
+            "lepic_sort32": (C.c_int, [vp, G, P32, P32]),
+            "lepic_collisions32": (C.c_int, [vp, G, P32, vp, C.POINTER(Chem), C.c_int32,
+                                             C.c_double, C.c_uint64, C.c_uint64, vp,
+                                             C.POINTER(MCCStats), vp, i32p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -188,6 +188,44 @@ def cfg2_pic_mcc(n: int = 128, np_cell: int = 64, steps: int = 1000, ng: int = 7
     return c
 
 
+def cfg3_poisson(n: int = 256, seed: int = 3, h: float = 3.125e-4, nblobs: int = 64):
+    """Mixed-precision Poisson problem: an n^3-cell box whose six faces are
+    electrodes 1..6 (painted x=0, x=L, y=0, y=L, z=0, z=L in that order, so a
+    later face owns the shared edges and corners, as later segments of
+    generate_boundary do) at voltages drawn U[-100, 100] V; rho a sum of
+    nblobs Gaussian blobs (centres uniform in the box, sigma U[2, 10] cells,
+    amplitudes N(0, 1e-4) C/m^3).  Returns (n, h, bcnd, phi0, rhs) with
+    phi0 the electrode voltages (0 elsewhere) and rhs = -rho on plasma nodes
+    (the sign build_rho_from_np gives, mod_density.f90:99)."""
+    rng = np.random.default_rng(seed)
+    nn = (n, n, n)
+    hh = (h, h, h)
+    b = np.full(node_shape(nn), -1, np.int32)
+    V = rng.uniform(-100.0, 100.0, 6)
+    label = 0
+    for axis in (2, 1, 0):            # numpy axes of x, y, z
+        for idx in ((0, 1), (n + 1, n + 2)):
+            label += 1
+            sl = [slice(None)] * 3
+            sl[axis] = list(idx)
+            b[tuple(sl)] = label
+    phi0 = np.zeros(b.shape)
+    for lab in range(1, 7):
+        phi0[b == lab] = V[lab - 1]
+    c = rng.uniform(0.0, n * h, (nblobs, 3))
+    sig = rng.uniform(2.0, 10.0, nblobs) * h
+    A = rng.normal(0.0, 1e-4, nblobs)
+    x = (np.arange(n + 3) - 1) * h
+    rho = np.zeros(b.shape)
+    for k in range(nblobs):
+        gx = np.exp(-(x - c[k, 0]) ** 2 / (2 * sig[k] ** 2))
+        gy = np.exp(-(x - c[k, 1]) ** 2 / (2 * sig[k] ** 2))
+        gz = np.exp(-(x - c[k, 2]) ** 2 / (2 * sig[k] ** 2))
+        rho += A[k] * gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
+    rhs = np.where(b == -1, -rho, 0.0)
+    return nn, hh, b, phi0, rhs
+
+
 # ---------------------------------------------------------------------------
 # Reaction tables (mod_reactions.f90:414-558 "ordering")
 # ---------------------------------------------------------------------------

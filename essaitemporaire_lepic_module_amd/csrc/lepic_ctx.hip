@@ -102,6 +102,8 @@ int lepic_destroy(lepic_ctx *ctx) {
   if (ctx->mg_mem) (void)hipFree(ctx->mg_mem);
   for (hipGraphExec_t &e : ctx->mg_exec)
     if (e) (void)hipGraphExecDestroy(e);
+  if (ctx->mgx_mem) (void)hipFree(ctx->mgx_mem);
+  if (ctx->mgx_exec) (void)hipGraphExecDestroy(ctx->mgx_exec);
   if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
   delete ctx;
   return LEPIC_OK;

@@ -40,6 +40,12 @@ struct lepic_ctx {
   hipGraphExec_t mg_exec[2] = {nullptr, nullptr};
   std::vector<unsigned char> mg_key;
   hipStream_t capture_stream = nullptr;
+  // mixed-precision solve: FP32 correction levels, and the captured inner
+  // V-cycles + correction of one refinement step with the solve they belong to
+  void *mgx_mem = nullptr;
+  size_t mgx_bytes = 0;
+  hipGraphExec_t mgx_exec = nullptr;
+  std::vector<unsigned char> mgx_key;
   // per-XCD deposit replicas (LEPIC_XCDS x nodes doubles, kept zeroed)
   double *rep = nullptr;
   int64_t rep_nodes = 0;

@@ -72,6 +72,89 @@ struct KChem {
   double kdep[4];   // Nm / (h1 h2 h3) per tracked species
 };
 
+// Particle access of the register-only trial, one per storage layout.
+// PA64: the FP64 sets (positions absolute; kills set dead = 1 and, when a
+// deposit already ran, take the particle's FP64 deposit out of np_dep).
+struct PA64 {
+  KParts P;
+  __device__ bool dead(int pt, int64_t ip) const { return P.dead[pt][ip] != 0; }
+  __device__ void vel(int pt, int64_t ip, double &x, double &y, double &z) const {
+    x = P.vx[pt][ip]; y = P.vy[pt][ip]; z = P.vz[pt][ip];
+  }
+  __device__ void set_vel(int pt, int64_t ip, double x, double y, double z) const {
+    P.vx[pt][ip] = x; P.vy[pt][ip] = y; P.vz[pt][ip] = z;
+  }
+  __device__ int64_t cap(int t) const { return P.cap[t]; }
+  __device__ void append(const KGrid &g, const KChem &K, int64_t nodes, int bt, int64_t ib,
+                         int pt, int64_t ip, double vx, double vy, double vz) const {
+    const double x0 = P.x[pt][ip], y0 = P.y[pt][ip], z0 = P.z[pt][ip];
+    P.x[bt][ib] = x0; P.y[bt][ib] = y0; P.z[bt][ib] = z0;
+    P.vx[bt][ib] = vx; P.vy[bt][ib] = vy; P.vz[bt][ib] = vz;
+    P.dead[bt][ib] = 0;
+    if (K.np_dep) deposit_one(g, K.np_dep + (int64_t)bt * nodes, K.kq, K.kdep[bt], x0, y0, z0, 1.0);
+  }
+  __device__ void kill(const KGrid &g, const KChem &K, int64_t nodes, int pt, int64_t ip) const {
+    if (K.np_dep)
+      deposit_one(g, K.np_dep + (int64_t)pt * nodes, K.kq, K.kdep[pt], P.x[pt][ip], P.y[pt][ip],
+                  P.z[pt][ip], -1.0);
+    P.dead[pt][ip] = 1;
+  }
+};
+
+// PA32: the FP32 sets (cell id + in-cell offsets; cell = -1 is no particle).
+// Products take the parent's cell and offsets; with dep set, products add
+// and kills remove their fixed-point deposit (the split of the deposit
+// kernels) in the int64 node sums of their species, by device-scope integer
+// atomics -- exact, whatever the order.
+struct KParts32 {
+  int32_t *cell[4];
+  float *fx[4], *fy[4], *fz[4], *vx[4], *vy[4], *vz[4];
+  int64_t cap[4];
+  long long *dep[4];
+  double U[4];
+  int neg[4];
+};
+
+struct PA32 {
+  KParts32 P;
+  __device__ bool dead(int pt, int64_t ip) const { return P.cell[pt][ip] < 0; }
+  __device__ void vel(int pt, int64_t ip, double &x, double &y, double &z) const {
+    x = (double)P.vx[pt][ip]; y = (double)P.vy[pt][ip]; z = (double)P.vz[pt][ip];
+  }
+  __device__ void set_vel(int pt, int64_t ip, double x, double y, double z) const {
+    P.vx[pt][ip] = (float)x; P.vy[pt][ip] = (float)y; P.vz[pt][ip] = (float)z;
+  }
+  __device__ int64_t cap(int t) const { return P.cap[t]; }
+  __device__ void dep(const KGrid &g, int t, int c, float fx, float fy, float fz, int sign) const {
+    if (!P.dep[t] || c < 0 || (int64_t)c >= (int64_t)g.n1 * g.n2 * g.n3) return;
+    const int cx = c % g.n1, r = c / g.n1, cy = r % g.n2, cz = r / g.n2;
+    const int64_t b = gidx(g, cx + 1, cy + 1, cz + 1);
+    const int64_t nd[8] = {b, b + 1, b + g.sy + 1, b + g.sy, b + g.sz, b + g.sz + 1,
+                           b + g.sz + g.sy + 1, b + g.sz + g.sy};
+    long long a[8];
+    fixed_split(P.U[t], (double)fx, (double)fy, (double)fz, a);
+    const bool flip = (sign < 0) != (P.neg[t] != 0);
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (a[q] != 0)
+        atomicAdd(reinterpret_cast<unsigned long long *>(&P.dep[t][nd[q]]),
+                  (unsigned long long)(flip ? -a[q] : a[q]));
+  }
+  __device__ void append(const KGrid &g, const KChem &, int64_t, int bt, int64_t ib, int pt,
+                         int64_t ip, double vx, double vy, double vz) const {
+    const int c = P.cell[pt][ip];
+    const float fx = P.fx[pt][ip], fy = P.fy[pt][ip], fz = P.fz[pt][ip];
+    P.cell[bt][ib] = c;
+    P.fx[bt][ib] = fx; P.fy[bt][ib] = fy; P.fz[bt][ib] = fz;
+    P.vx[bt][ib] = (float)vx; P.vy[bt][ib] = (float)vy; P.vz[bt][ib] = (float)vz;
+    dep(g, bt, c, fx, fy, fz, 1);
+  }
+  __device__ void kill(const KGrid &g, const KChem &, int64_t, int pt, int64_t ip) const {
+    dep(g, pt, P.cell[pt][ip], P.fx[pt][ip], P.fy[pt][ip], P.fz[pt][ip], -1);
+    P.cell[pt][ip] = -1;
+  }
+};
+
 struct Ran2Stream {
   int32_t s;
   __device__ double uniform() {
@@ -418,17 +501,18 @@ __device__ void trial(const KGrid &g, const KParts &P, const KChem &K, const dou
 // is the first, in ascending nu (ties in sig_list order), whose running sum
 // reaches rnd -- the exchange sort's choice -- found by ranking instead of
 // sorting; rnd is drawn after the targets, as above.
-template <class APP>
-__device__ void trial_fast(const KGrid &g, const KParts &P, const KChem &K, int64_t nodes, int pt,
+template <class PA, class APP>
+__device__ void trial_fast(const KGrid &g, const PA &P, const KChem &K, int64_t nodes, int pt,
                            int64_t ip, PhiloxStream &rng, APP &app, MccCounters *cnt) {
   const lepic_chem &c = K.c;
   const FastPlan &fp = K.fp[pt];
   const int NT = c.ntype_tracked, NA = c.ntype_total;
-  if (P.dead[pt][ip] != 0) {
+  if (P.dead(pt, ip)) {
     app.count(&cnt->skipped_dead[pt]);
     return;
   }
-  const double vx1 = P.vx[pt][ip], vy1 = P.vy[pt][ip], vz1 = P.vz[pt][ip];
+  double vx1, vy1, vz1;
+  P.vel(pt, ip, vx1, vy1, vz1);
   const double mp = fabs(c.mass[pt]);
   double tvx[FT], tvy[FT], tvz[FT], tvr[FT], tEk[FT];
   int tip[FT];
@@ -538,34 +622,25 @@ __device__ void trial_fast(const KGrid &g, const KParts &P, const KChem &K, int6
     if (ib == 0 && bt == pt) {
       // the projectile itself carries the first same-species byproduct
       reused = true;
-      P.vx[pt][ip] = nvx; P.vy[pt][ip] = nvy; P.vz[pt][ip] = nvz;
+      P.set_vel(pt, ip, nvx, nvy, nvz);
       const double v2old = vx1 * vx1 + vy1 * vy1 + vz1 * vz1;
       app.add(&app.pcoll_base[bt], 0.5 * c.Nm[bt] * c.mass[bt] * (v2new - v2old));
       continue;
     }
     const int64_t ibx = app.slot(bt);
-    if (ibx >= P.cap[bt]) {
+    if (ibx >= P.cap(bt)) {
       atomicExch((unsigned long long *)&cnt->overflow, 1ull);
       continue;
     }
-    const double x0 = P.x[pt][ip], y0 = P.y[pt][ip], z0 = P.z[pt][ip];
-    P.x[bt][ibx] = x0; P.y[bt][ibx] = y0; P.z[bt][ibx] = z0;
-    P.vx[bt][ibx] = nvx; P.vy[bt][ibx] = nvy; P.vz[bt][ibx] = nvz;
-    P.dead[bt][ibx] = 0;
+    P.append(g, K, nodes, bt, ibx, pt, ip, nvx, nvy, nvz);
     app.count(&cnt->appended[bt]);
-    if (K.np_dep) deposit_one(g, K.np_dep + (int64_t)bt * nodes, K.kq, K.kdep[bt], x0, y0, z0, 1.0);
     app.add(&app.pcoll_base[bt], 0.5 * c.Nm[bt] * c.mass[bt] * v2new);
   }
   for (int ir = 0; ir < n_re; ++ir) {
     const int ct = c.col_info[c_ind * 10 + 2 + ir];
     if (ct < 0 || ct >= NT) continue;
     if (c.mass[ct] <= 0.0) continue;
-    if (ct == pt && !(ir == 0 && reused) && P.dead[pt][ip] == 0) {
-      if (K.np_dep)
-        deposit_one(g, K.np_dep + (int64_t)pt * nodes, K.kq, K.kdep[pt], P.x[pt][ip], P.y[pt][ip],
-                    P.z[pt][ip], -1.0);
-      P.dead[pt][ip] = 1;
-    }
+    if (ct == pt && !(ir == 0 && reused) && !P.dead(pt, ip)) P.kill(g, K, nodes, pt, ip);
   }
 }
 
@@ -603,7 +678,17 @@ __global__ void k_mcc_replay(KGrid g, KParts P, KChem K, const double *np_red, i
 // Poisson(lambda), drawn by inversion from P(1) = lambda / (e^lambda - 1).
 // Live particles are listed as (index | k << 40); holes and MCC-killed
 // particles take no trials.
-__global__ __launch_bounds__(BLOCK) void k_mcc_select(const int8_t *__restrict__ dead, int pt, int64_t n0,
+struct LiveDead {   // FP64 sets: dead flags
+  const int8_t *d;
+  __device__ bool operator()(int64_t j) const { return d[j] == 0; }
+};
+struct LiveCell {   // FP32 sets: cell id >= 0
+  const int32_t *c;
+  __device__ bool operator()(int64_t j) const { return c[j] >= 0; }
+};
+
+template <class LIVE>
+__global__ __launch_bounds__(BLOCK) void k_mcc_select(LIVE live, int pt, int64_t n0,
                                                       double lambda, double p1, uint64_t key,
                                                       uint64_t counter, uint64_t *list,
                                                       int64_t cap, unsigned long long *nsel) {
@@ -633,7 +718,7 @@ __global__ __launch_bounds__(BLOCK) void k_mcc_select(const int8_t *__restrict__
         pk = pk * lambda / (double)k;
         cdf = cdf + pk;
       }
-      if (dead[j] != 0) continue;
+      if (!live(j)) continue;
       if (pass == 1 && slot < cap) list[slot] = (uint64_t)j | ((uint64_t)k << 40);
       if (pass == 1) ++slot;
       ++count;
@@ -644,8 +729,8 @@ __global__ __launch_bounds__(BLOCK) void k_mcc_select(const int8_t *__restrict__
 
 // PHILOX trial pass: one thread per listed particle, k trials in a row on the
 // particle's own stream.
-template <bool FAST>
-__global__ __launch_bounds__(BLOCK) void k_mcc_trials(KGrid g, KParts P, KChem K,
+template <bool FAST, class PA>
+__global__ __launch_bounds__(BLOCK) void k_mcc_trials(KGrid g, PA P, KChem K,
                                                       const double *np_red, int64_t nodes, int pt,
                                                       const uint64_t *list,
                                                       const unsigned long long *nsel, int64_t cap,
@@ -671,7 +756,7 @@ __global__ __launch_bounds__(BLOCK) void k_mcc_trials(KGrid g, KParts P, KChem K
     rng.c3 ^= (uint32_t)(counter >> 32) * 0x9E3779B9u;
     for (int t = 0; t < k; ++t) {
       if constexpr (FAST) trial_fast(g, P, K, nodes, pt, ip, rng, app, cnt);
-      else trial(g, P, K, np_red, nodes, pt, ip, rng, app, cnt, Pcoll);
+      else trial(g, P.P, K, np_red, nodes, pt, ip, rng, app, cnt, Pcoll);
     }
   }
   // Pcoll: wave sums, then one atomic per block and species
@@ -724,14 +809,35 @@ __global__ __launch_bounds__(BLOCK) void k_npmax(KGrid g, const double *np, int6
 
 }  // namespace
 
-static int collisions_impl(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *parts,
+// The particle sets of one MCC call: FP64 sets (parts) or FP32 sets
+// (parts32, PHILOX mode, neutral targets only), with the FP32 sets' int64
+// deposit sums (dep, [ntype_tracked][nodes], may be null) and their fraction
+// bits.
+struct MccSets {
+  lepic_pset *parts = nullptr;
+  lepic_pset32 *parts32 = nullptr;
+  int64_t *dep = nullptr;
+  const int32_t *frac_bits = nullptr;
+  int64_t n(int t) const { return parts ? parts[t].n : parts32[t].n; }
+  int64_t cap(int t) const { return parts ? parts[t].cap : parts32[t].cap; }
+  void set_n(int t, int64_t v) {
+    if (parts) parts[t].n = v;
+    else parts32[t].n = v;
+  }
+};
+
+static int collisions_impl(lepic_ctx *ctx, const lepic_grid *g, MccSets sets,
                            const double *np_red, const lepic_chem *chem, int32_t ns_coll,
                            double dt, int32_t mode, int32_t *iseed, uint64_t philox_key,
                            uint64_t philox_counter, double *Pcoll, lepic_mcc_stats *stats,
                            double *np_dep, const double *kq) {
   if (!ctx) return LEPIC_ERR_ARG;
-  if (!g || !parts || !np_red || !chem || !Pcoll || (mode == LEPIC_MCC_REPLAY_RAN2 && !iseed))
+  if (!g || (!sets.parts && !sets.parts32) || !np_red || !chem || !Pcoll ||
+      (mode == LEPIC_MCC_REPLAY_RAN2 && !iseed))
     return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions: null argument");
+  const bool fp32 = sets.parts32 != nullptr;
+  if (fp32 && mode != LEPIC_MCC_PHILOX)
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions32: FP32 sets run the PHILOX mode only");
   const lepic_chem &c = *chem;
   const int NT = c.ntype_tracked, NA = c.ntype_total;
   if (NT < 1 || NT > 4 || NA < NT || NA > LEPIC_MAX_SPECIES)
@@ -847,42 +953,68 @@ static int collisions_impl(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *part
     double Pmax = nu_max * (double)ns_coll * dt;
     if (Pmax > 1.0) return lepic_fail(ctx, LEPIC_ERR_PMAX, "collisions: Pmax > 1");
     if (mode == LEPIC_MCC_REPLAY_RAN2) {
-      double dNc = (double)parts[t].n * Pmax / 1.0;
+      double dNc = (double)sets.n(t) * Pmax / 1.0;
       int32_t nct = (int32_t)dNc;
       double rnd = lepic_ran2(&iseed[0]);
       if (rnd <= (dNc - (double)nct)) nct = nct + 1;
       Nc[t] = nct;
-      int64_t nn = parts[t].n > 1 ? parts[t].n : 1;
+      int64_t nn = sets.n(t) > 1 ? sets.n(t) : 1;
       K.nu_max_local[t] = (double)nct / (double)nn / ((double)ns_coll * dt);
     } else {
       lambda[t] = Pmax;
-      Nc[t] = (int32_t)std::llround((double)parts[t].n * Pmax);
+      Nc[t] = (int32_t)std::llround((double)sets.n(t) * Pmax);
       K.nu_max_local[t] = Pmax / ((double)ns_coll * dt);
     }
   }
-  KParts P;
+  if (fp32)
+    for (int t = 0; t < NT; ++t)
+      if (c.p_ncol[t] > 0 && c.mass[t] > 0.0 && !K.fp[t].ok)
+        return lepic_fail(ctx, LEPIC_ERR_ARG,
+                          "collisions32: FP32 sets need every reaction's target to be a neutral");
+  PA64 P;
+  PA32 P32;
   std::memset(&P, 0, sizeof(P));
+  std::memset(&P32, 0, sizeof(P32));
   for (int t = 0; t < NT; ++t) {
-    P.x[t] = parts[t].x; P.y[t] = parts[t].y; P.z[t] = parts[t].z;
-    P.vx[t] = parts[t].vx; P.vy[t] = parts[t].vy; P.vz[t] = parts[t].vz;
-    P.dead[t] = parts[t].dead;
-    P.cap[t] = parts[t].cap;
+    if (fp32) {
+      const lepic_pset32 &q = sets.parts32[t];
+      P32.P.cell[t] = q.cell;
+      P32.P.fx[t] = q.fx; P32.P.fy[t] = q.fy; P32.P.fz[t] = q.fz;
+      P32.P.vx[t] = q.vx; P32.P.vy[t] = q.vy; P32.P.vz[t] = q.vz;
+      P32.P.cap[t] = q.cap;
+      if (sets.dep) {
+        const double k1 = c.Nm[t] / (g->h[0] * g->h[1] * g->h[2]);
+        const double unit = std::ldexp(std::fabs(k1), sets.frac_bits[t]);
+        if (k1 != 0.0 && !(unit >= 1.0 && unit <= 17592186044416.0))
+          return lepic_fail(ctx, LEPIC_ERR_ARG,
+                            "collisions32: |Nm/(h1 h2 h3)| * 2^frac_bits must lie in [1, 2^44]");
+        P32.P.dep[t] = reinterpret_cast<long long *>(sets.dep + (int64_t)t * nodes);
+        P32.P.U[t] = std::rint(unit);
+        P32.P.neg[t] = k1 < 0.0 ? 1 : 0;
+      }
+    } else {
+      const lepic_pset &q = sets.parts[t];
+      P.P.x[t] = q.x; P.P.y[t] = q.y; P.P.z[t] = q.z;
+      P.P.vx[t] = q.vx; P.P.vy[t] = q.vy; P.P.vz[t] = q.vz;
+      P.P.dead[t] = q.dead;
+      P.P.cap[t] = q.cap;
+    }
   }
   MccCounters hc;
   std::memset(&hc, 0, sizeof(hc));
-  for (int t = 0; t < NT; ++t) hc.n[t] = parts[t].n;
+  for (int t = 0; t < NT; ++t) hc.n[t] = sets.n(t);
   hc.seed = iseed ? iseed[0] : 0;
   MccCounters *dc = reinterpret_cast<MccCounters *>(ctx->d_ctrl + CTRL_MCC);
   static_assert(sizeof(MccCounters) <= 256 * sizeof(int64_t), "MCC counters fit the control block");
   LEPIC_HIP(ctx, hipMemcpyAsync(dc, &hc, sizeof(hc), hipMemcpyHostToDevice, ctx->stream));
   if (mode == LEPIC_MCC_REPLAY_RAN2) {
-    hipLaunchKernelGGL(k_mcc_replay, dim3(1), dim3(64), 0, ctx->stream, kg, P, K, np_red, nodes,
+    hipLaunchKernelGGL(k_mcc_replay, dim3(1), dim3(64), 0, ctx->stream, kg, P.P, K, np_red, nodes,
                        Nc[0], Nc[1], Nc[2], Nc[3], dc, Pcoll);
     LEPIC_LAUNCH_CHECK(ctx, "k_mcc_replay");
   } else if (mode == LEPIC_MCC_PHILOX) {
     for (int t = 0; t < NT; ++t) {
-      if (c.p_ncol[t] == 0 || c.mass[t] <= 0.0 || lambda[t] <= 0.0 || parts[t].n == 0) continue;
-      const int64_t n0 = parts[t].n;
+      if (c.p_ncol[t] == 0 || c.mass[t] <= 0.0 || lambda[t] <= 0.0 || sets.n(t) == 0) continue;
+      const int64_t n0 = sets.n(t);
       const double lam = lambda[t];
       const double p1 = lam / std::expm1(lam);
       const double mean = (double)n0 * -std::expm1(-lam);
@@ -895,9 +1027,14 @@ static int collisions_impl(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *part
         if (!list) return lepic_fail(ctx, LEPIC_ERR_HIP, "collisions: selection list allocation");
         hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ctx->stream,
                            reinterpret_cast<uint64_t *>(nsel), (int64_t)1);
-        hipLaunchKernelGGL(k_mcc_select, dim3(grid_blocks(nthreads, BLOCK)), dim3(BLOCK), 0,
-                           ctx->stream, P.dead[t], t, n0, lam, p1, philox_key, philox_counter, list,
-                           cap, nsel);
+        if (fp32)
+          hipLaunchKernelGGL(k_mcc_select<LiveCell>, dim3(grid_blocks(nthreads, BLOCK)),
+                             dim3(BLOCK), 0, ctx->stream, LiveCell{P32.P.cell[t]}, t, n0, lam, p1,
+                             philox_key, philox_counter, list, cap, nsel);
+        else
+          hipLaunchKernelGGL(k_mcc_select<LiveDead>, dim3(grid_blocks(nthreads, BLOCK)),
+                             dim3(BLOCK), 0, ctx->stream, LiveDead{P.P.dead[t]}, t, n0, lam, p1,
+                             philox_key, philox_counter, list, cap, nsel);
         LEPIC_LAUNCH_CHECK(ctx, "k_mcc_select");
         LEPIC_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, nsel, sizeof(int64_t), hipMemcpyDeviceToHost,
                                       ctx->stream));
@@ -908,14 +1045,18 @@ static int collisions_impl(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *part
           continue;
         }
         if (m > 0) {
-          if (K.fp[t].ok)
-            hipLaunchKernelGGL(k_mcc_trials<true>, dim3(grid_blocks(m, BLOCK)), dim3(BLOCK), 0,
-                               ctx->stream, kg, P, K, np_red, nodes, t, list, nsel, cap, philox_key,
-                               philox_counter, dc, Pcoll);
+          if (fp32)
+            hipLaunchKernelGGL((k_mcc_trials<true, PA32>), dim3(grid_blocks(m, BLOCK)), dim3(BLOCK),
+                               0, ctx->stream, kg, P32, K, np_red, nodes, t, list, nsel, cap,
+                               philox_key, philox_counter, dc, Pcoll);
+          else if (K.fp[t].ok)
+            hipLaunchKernelGGL((k_mcc_trials<true, PA64>), dim3(grid_blocks(m, BLOCK)), dim3(BLOCK),
+                               0, ctx->stream, kg, P, K, np_red, nodes, t, list, nsel, cap,
+                               philox_key, philox_counter, dc, Pcoll);
           else
-            hipLaunchKernelGGL(k_mcc_trials<false>, dim3(grid_blocks(m, BLOCK)), dim3(BLOCK), 0,
-                               ctx->stream, kg, P, K, np_red, nodes, t, list, nsel, cap, philox_key,
-                               philox_counter, dc, Pcoll);
+            hipLaunchKernelGGL((k_mcc_trials<false, PA64>), dim3(grid_blocks(m, BLOCK)),
+                               dim3(BLOCK), 0, ctx->stream, kg, P, K, np_red, nodes, t, list, nsel,
+                               cap, philox_key, philox_counter, dc, Pcoll);
         }
         LEPIC_LAUNCH_CHECK(ctx, "k_mcc_trials");
         break;
@@ -926,7 +1067,7 @@ static int collisions_impl(lepic_ctx *ctx, const lepic_grid *g, lepic_pset *part
   }
   LEPIC_HIP(ctx, hipMemcpyAsync(&hc, dc, sizeof(hc), hipMemcpyDeviceToHost, ctx->stream));
   LEPIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  for (int t = 0; t < NT; ++t) parts[t].n = hc.n[t] < parts[t].cap ? hc.n[t] : parts[t].cap;
+  for (int t = 0; t < NT; ++t) sets.set_n(t, hc.n[t] < sets.cap(t) ? hc.n[t] : sets.cap(t));
   if (mode == LEPIC_MCC_REPLAY_RAN2) iseed[0] = (int32_t)hc.seed;
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
@@ -946,7 +1087,9 @@ extern "C" int lepic_collisions(lepic_ctx *ctx, const lepic_grid *g, lepic_pset 
                                 const double *np_red, const lepic_chem *chem, int32_t ns_coll,
                                 double dt, int32_t mode, int32_t *iseed, uint64_t philox_key,
                                 uint64_t philox_counter, double *Pcoll, lepic_mcc_stats *stats) {
-  return collisions_impl(ctx, g, parts, np_red, chem, ns_coll, dt, mode, iseed, philox_key,
+  MccSets sets;
+  sets.parts = parts;
+  return collisions_impl(ctx, g, sets, np_red, chem, ns_coll, dt, mode, iseed, philox_key,
                          philox_counter, Pcoll, stats, nullptr, nullptr);
 }
 
@@ -958,6 +1101,33 @@ extern "C" int lepic_collisions_deposited(lepic_ctx *ctx, const lepic_grid *g, l
                                           const double *kq) {
   if (!ctx) return LEPIC_ERR_ARG;
   if (!np_dep || !kq) return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions_deposited: null density");
-  return collisions_impl(ctx, g, parts, np_red, chem, ns_coll, dt, mode, iseed, philox_key,
+  MccSets sets;
+  sets.parts = parts;
+  return collisions_impl(ctx, g, sets, np_red, chem, ns_coll, dt, mode, iseed, philox_key,
                          philox_counter, Pcoll, stats, np_dep, kq);
+}
+
+extern "C" int lepic_collisions32(lepic_ctx *ctx, const lepic_grid *g, lepic_pset32 *parts,
+                                  const double *np_red, const lepic_chem *chem, int32_t ns_coll,
+                                  double dt, uint64_t philox_key, uint64_t philox_counter,
+                                  double *Pcoll, lepic_mcc_stats *stats, int64_t *dep,
+                                  const int32_t *frac_bits) {
+  if (!ctx) return LEPIC_ERR_ARG;
+  if (!parts || (dep && !frac_bits))
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions32: null argument");
+  if (chem)
+    for (int t = 0; t < chem->ntype_tracked && t < 4; ++t) {
+      const void *a[7] = {parts[t].cell, parts[t].fx, parts[t].fy, parts[t].fz,
+                          parts[t].vx, parts[t].vy, parts[t].vz};
+      for (const void *q : a)
+        if (parts[t].cap > 0 && !q) return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions32: bad set");
+      if (parts[t].n < 0 || parts[t].n > parts[t].cap)
+        return lepic_fail(ctx, LEPIC_ERR_ARG, "collisions32: bad set");
+    }
+  MccSets sets;
+  sets.parts32 = parts;
+  sets.dep = dep;
+  sets.frac_bits = frac_bits;
+  return collisions_impl(ctx, g, sets, np_red, chem, ns_coll, dt, LEPIC_MCC_PHILOX, nullptr,
+                         philox_key, philox_counter, Pcoll, stats, nullptr, nullptr);
 }

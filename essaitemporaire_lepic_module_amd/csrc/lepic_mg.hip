@@ -32,33 +32,39 @@ struct MGCtl {
 };
 static_assert(sizeof(MGCtl) == 16 * sizeof(int64_t), "MGCtl layout");
 
-struct KLevel {
+// One multigrid level: geometry, stencil coefficients and the level's u, b
+// and labels.  T is the storage and arithmetic type of u and b (double for
+// the reference's solver, float for the smoother of the mixed-precision one).
+template <class T>
+struct KLevelT {
   KGrid g;
-  double aw, ae, an, as, at, ab, ac;
-  double *u;
-  double *b;
+  T aw, ae, an, as, at, ab, ac;
+  T *u;
+  T *b;
   int32_t *bc;
 };
+using KLevel = KLevelT<double>;
 
-__device__ inline double getres(const KLevel &L, int ir, int jr, int kr, int flag_nmn) {
+template <class T>
+__device__ inline T getres(const KLevelT<T> &L, int ir, int jr, int kr, int flag_nmn) {
   const KGrid &g = L.g;
-  if (ir < 0 || ir > g.n1 + 1 || jr < 0 || jr > g.n2 + 1 || kr < 0 || kr > g.n3 + 1) return 0.0;
-  const double *u = L.u;
-  double aw = L.aw, ae = L.ae;
+  if (ir < 0 || ir > g.n1 + 1 || jr < 0 || jr > g.n2 + 1 || kr < 0 || kr > g.n3 + 1) return T(0);
+  const T *u = L.u;
+  T aw = L.aw, ae = L.ae;
   const int64_t c = gidx(g, ir, jr, kr);
-  double uij = u[c];
-  double un = u[c + g.sy];
-  double us = u[c - g.sy];
-  double ue = u[c + 1];
-  double ut = u[c + g.sz];
-  double ub = kr >= 1 ? u[c - g.sz] : 0.0;
-  double uw;
+  T uij = u[c];
+  T un = u[c + g.sy];
+  T us = u[c - g.sy];
+  T ue = u[c + 1];
+  T ut = u[c + g.sz];
+  T ub = kr >= 1 ? u[c - g.sz] : T(0);
+  T uw;
   if (ir == 1 && flag_nmn == 1) {
-    uw = 0.0; aw = 0.0; ae = 2.0 * ae;
+    uw = T(0); aw = T(0); ae = T(2) * ae;
   } else {
     uw = u[c - 1];
   }
-  double urhs = L.b[c];
+  T urhs = L.b[c];
   return urhs - (aw * uw + ae * ue + L.an * un + L.as * us + L.ac * uij + L.at * ut + L.ab * ub);
 }
 
@@ -78,7 +84,8 @@ __global__ void k_reset_totals(MGCtl *ctl, int n) {
 
 // One colour of sor_rb (sors.f90:87-140 red, 249-302 black).
 // colour 0 = red = (i+j+k) odd, 1 = black.  Threads over (i2, j, k).
-__device__ inline void sor_node(const KLevel &L, int colour, double omega, int64_t t) {
+template <class T>
+__device__ inline void sor_node(const KLevelT<T> &L, int colour, T omega, int64_t t) {
   const KGrid &g = L.g;
   const int ni2 = g.n1 / 2 + 1;
   int64_t total = (int64_t)ni2 * (g.n2 + 1) * (g.n3 + 1);
@@ -98,10 +105,10 @@ __device__ inline void sor_node(const KLevel &L, int colour, double omega, int64
   const int64_t c = gidx(g, i, j, k);
   const int bcv = L.bc[c];
   if (bcv > 0) return;
-  double *u = L.u;
+  T *u = L.u;
   if (bcv == -2) {
-    u[c] = (1.0 - omega) * u[c] +
-           omega * (L.b[c] - L.an * u[c + g.sy] - L.as * u[c - g.sy] - 2.0 * L.ae * u[c + 1] -
+    u[c] = (T(1) - omega) * u[c] +
+           omega * (L.b[c] - L.an * u[c + g.sy] - L.as * u[c - g.sy] - T(2) * L.ae * u[c + 1] -
                     L.at * u[c + g.sz] - L.ab * u[c - g.sz]) /
                L.ac;
     return;
@@ -110,13 +117,14 @@ __device__ inline void sor_node(const KLevel &L, int colour, double omega, int64
     if (k == g.n3 + 1) return;
     if (j == g.n2 + 1) return;
   }
-  u[c] = (1.0 - omega) * u[c] +
+  u[c] = (T(1) - omega) * u[c] +
          omega * (L.b[c] - L.aw * u[c - 1] - L.an * u[c + g.sy] - L.as * u[c - g.sy] -
                   L.ae * u[c + 1] - L.at * u[c + g.sz] - L.ab * u[c - g.sz]) /
              L.ac;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_sor(KLevel L, int colour, double omega,
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_sor(KLevelT<T> L, int colour, T omega,
                                                const MGCtl *ctl) {
   if (ctl->active == 0) return;
   sor_node(L, colour, omega, blockIdx.x * (int64_t)BLOCK + threadIdx.x);
@@ -124,7 +132,8 @@ __global__ __launch_bounds__(BLOCK) void k_sor(KLevel L, int colour, double omeg
 
 // Periodic ghost copies (sors.f90:184-239, single rank): y planes for every
 // (i, k), then z planes for every (i, j).
-__device__ inline void per_y_node(const KLevel &L, int64_t t) {
+template <class T>
+__device__ inline void per_y_node(const KLevelT<T> &L, int64_t t) {
   const KGrid &g = L.g;
   int64_t ni = g.n1 + 3, nk = g.n3 + 3;
   if (t >= ni * nk) return;
@@ -134,7 +143,8 @@ __device__ inline void per_y_node(const KLevel &L, int64_t t) {
   if (b2 == 0 || b2 == -2) L.u[gidx(g, i, g.n2 + 1, k)] = L.u[gidx(g, i, 1, k)];
 }
 
-__device__ inline void per_z_node(const KLevel &L, int64_t t) {
+template <class T>
+__device__ inline void per_z_node(const KLevelT<T> &L, int64_t t) {
   const KGrid &g = L.g;
   int64_t ni = g.n1 + 3, nj = g.n2 + 3;
   if (t >= ni * nj) return;
@@ -145,19 +155,22 @@ __device__ inline void per_z_node(const KLevel &L, int64_t t) {
   if (b2 == 0 || b2 == -2) L.u[gidx(g, i, j, m + 1)] = L.u[gidx(g, i, j, 1)];
 }
 
-__global__ void k_per_y(KLevel L, const MGCtl *ctl) {
+template <class T>
+__global__ void k_per_y(KLevelT<T> L, const MGCtl *ctl) {
   if (ctl->active == 0) return;
   per_y_node(L, blockIdx.x * (int64_t)blockDim.x + threadIdx.x);
 }
 
-__global__ void k_per_z(KLevel L, const MGCtl *ctl) {
+template <class T>
+__global__ void k_per_z(KLevelT<T> L, const MGCtl *ctl) {
   if (ctl->active == 0) return;
   per_z_node(L, blockIdx.x * (int64_t)blockDim.x + threadIdx.x);
 }
 
 // L1 residual over bcnd == -1 nodes (sors.f90:409-431), block partials.
 // |b - A u| at interior node t of the residual range (0 outside plasma).
-__device__ inline double resid_node(const KLevel &L, int64_t t) {
+template <class T>
+__device__ inline double resid_node(const KLevelT<T> &L, int64_t t) {
   const KGrid &g = L.g;
   int64_t nx = g.n1 + 1, ny = g.n2 + 1, nz = g.n3 + 1;
   double a = 0.0;
@@ -168,11 +181,11 @@ __device__ inline double resid_node(const KLevel &L, int64_t t) {
     int k = 1 + (int)(r / ny);
     const int64_t c = gidx(g, i, j, k);
     if (L.bc[c] == -1) {
-      const double *u = L.u;
-      double rij = L.b[c] - (L.aw * u[c - 1] + L.an * u[c + g.sy] + L.as * u[c - g.sy] +
-                             L.ae * u[c + 1] + L.ac * u[c] + L.at * u[c + g.sz] +
-                             L.ab * u[c - g.sz]);
-      a = fabs(rij);
+      const T *u = L.u;
+      T rij = L.b[c] - (L.aw * u[c - 1] + L.an * u[c + g.sy] + L.as * u[c - g.sy] +
+                        L.ae * u[c + 1] + L.ac * u[c] + L.at * u[c + g.sz] +
+                        L.ab * u[c - g.sz]);
+      a = fabs((double)rij);
     }
   }
   return a;
@@ -183,7 +196,8 @@ __device__ inline double resid_node(const KLevel &L, int64_t t) {
 // for a level (deterministic from run to run).
 constexpr int RESID_BLOCKS = 1024;
 
-__global__ __launch_bounds__(BLOCK) void k_resid(KLevel L, double *partials, const MGCtl *ctl,
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_resid(KLevelT<T> L, double *partials, const MGCtl *ctl,
                                                  int64_t nres) {
   if (ctl->active == 0) return;
   __shared__ double red[BLOCK / LEPIC_WAVE];
@@ -236,14 +250,15 @@ __global__ __launch_bounds__(BLOCK) void k_decide(const double *partials, int64_
 // winning; here one thread per entry finds that last writer (largest jr,
 // then largest ir) and applies its branch: getres when bcnd(ir,jr,m+1) <= 0,
 // else 0.  The two branches guard the (ir, jr-1) entry differently.
-__device__ inline void rpbc_entry(const KLevel &F, double *rp, int flag_nmn, int64_t t) {
+template <class T>
+__device__ inline void rpbc_entry(const KLevelT<T> &F, T *rp, int flag_nmn, int64_t t) {
   const KGrid &g = F.g;
   const int64_t rs = g.n1 + 2;
   const int64_t total = rs * (g.n2 + 2);
   if (t >= total) return;
   const int a = (int)(t % rs), b = (int)(t / rs);
   const int m = g.n3, N1 = g.n1 + 1, N2 = g.n2 + 1;
-  double val = 0.0;
+  T val = T(0);
   bool found = false;
   for (int jr = b + 1; jr >= b - 1 && !found; --jr) {
     if (jr < 1 || jr > N2 || (jr % 2) == 0) continue;
@@ -262,19 +277,21 @@ __device__ inline void rpbc_entry(const KLevel &F, double *rp, int flag_nmn, int
       }
       if (!writes) continue;
       found = true;
-      val = live ? getres(F, a, b, m, flag_nmn) : 0.0;
+      val = live ? getres(F, a, b, m, flag_nmn) : T(0);
     }
   }
   rp[t] = val;
 }
 
-__global__ void k_rpbc(KLevel F, double *rp, int flag_nmn) {
+template <class T>
+__global__ void k_rpbc(KLevelT<T> F, T *rp, int flag_nmn) {
   rpbc_entry(F, rp, flag_nmn, blockIdx.x * (int64_t)blockDim.x + threadIdx.x);
 }
 
 // Restriction (mg.f90:413-718): coarse labels injected from the fine grid and
 // the 27-point full-weighting residual with the reference's neighbour rules.
-__device__ inline void restrict_node(const KLevel &F, const KLevel &C, const double *rp,
+template <class T>
+__device__ inline void restrict_node(const KLevelT<T> &F, const KLevelT<T> &C, const T *rp,
                                      int flag_nmn, int64_t t) {
   const KGrid &g = F.g;
   const KGrid &gc = C.g;
@@ -298,18 +315,18 @@ __device__ inline void restrict_node(const KLevel &F, const KLevel &C, const dou
   if (jr == N2) bc2[gidx(gc, i, g.n2 / 2 + 2, k)] = bc[gidx(g, ir, g.n2 + 2, kr)];
   if (kr == 1) bc2[gidx(gc, i, j, 0)] = bc[gidx(g, ir, jr, 0)];
   if (kr == M1) bc2[gidx(gc, i, j, m / 2 + 2)] = bc[gidx(g, ir, jr, m + 2)];
-  double re = 0.0;
+  T re = T(0);
   if (lab <= 0) {
     const int n2 = g.n2;
     const int64_t rs = g.n1 + 2;
     auto B = [&](int a, int b_, int c) { return bc[gidx(g, a, b_, c)]; };
     auto GR = [&](int a, int b_, int c) { return getres(F, a, b_, c, flag_nmn); };
-    auto RP = [&](int a, int b_) { return rp ? rp[(int64_t)a + rs * b_] : 0.0; };
+    auto RP = [&](int a, int b_) { return rp ? rp[(int64_t)a + rs * b_] : T(0); };
     const bool kb = (kr == 1);
-    double r000 = 0, rpp0 = 0, rp00 = 0, rpm0 = 0, r0m0 = 0, rmm0 = 0, rm00 = 0, rmp0 = 0,
-           r0p0 = 0, r00p = 0, rppp = 0, rp0p = 0, rpmp = 0, r0mp = 0, rmmp = 0, rm0p = 0,
-           rmpp = 0, r0pp = 0, r00m = 0, rppm = 0, rp0m = 0, rpmm = 0, r0mm = 0, rmmm = 0,
-           rm0m = 0, rmpm = 0, r0pm = 0;
+    T r000 = 0, rpp0 = 0, rp00 = 0, rpm0 = 0, r0m0 = 0, rmm0 = 0, rm00 = 0, rmp0 = 0,
+      r0p0 = 0, r00p = 0, rppp = 0, rp0p = 0, rpmp = 0, r0mp = 0, rmmp = 0, rm0p = 0,
+      rmpp = 0, r0pp = 0, r00m = 0, rppm = 0, rp0m = 0, rpmm = 0, r0mm = 0, rmmm = 0,
+      rm0m = 0, rmpm = 0, r0pm = 0;
     r000 = GR(ir, jr, kr);
     if (ir != N1 && jr != N2) { if (B(ir + 1, jr + 1, kr) <= 0) rpp0 = GR(ir + 1, jr + 1, kr); }
     if (ir != N1) { if (B(ir + 1, jr, kr) <= 0) rp00 = GR(ir + 1, jr, kr); }
@@ -395,22 +412,24 @@ __device__ inline void restrict_node(const KLevel &F, const KLevel &C, const dou
       if (kb) { if (jr != N2) r0pm = RP(ir, jr + 1); }
       else { if (jr != N2) r0pm = GR(ir, jr + 1, kr - 1); }
     }
-    re = (8.0 * r000 + 4.0 * (rp00 + rm00 + r0p0 + r0m0 + r00p + r00m) +
-          2.0 * (rpp0 + rmp0 + rpm0 + rmm0 + rp0p + rm0p + r0pp + r0mp + rp0m + rm0m + r0pm + r0mm) +
+    re = (T(8) * r000 + T(4) * (rp00 + rm00 + r0p0 + r0m0 + r00p + r00m) +
+          T(2) * (rpp0 + rmp0 + rpm0 + rmm0 + rp0p + rm0p + r0pp + r0mp + rp0m + rm0m + r0pm + r0mm) +
           rppp + rmpp + rpmp + rmmp + rppm + rmpm + rpmm + rmmm) /
-         64.0;
+         T(64);
   }
   C.b[gidx(gc, i, j, k)] = re;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_restrict(KLevel F, KLevel C, const double *rp,
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_restrict(KLevelT<T> F, KLevelT<T> C, const T *rp,
                                                     int flag_nmn) {
   restrict_node(F, C, rp, flag_nmn, blockIdx.x * (int64_t)BLOCK + threadIdx.x);
 }
 
 // Prolongation (mg.f90:886-935): one thread per coarse node updates its 8
 // fine nodes (disjoint between threads).
-__device__ inline void prolong_node(const KLevel &F, const KLevel &C, int64_t t) {
+template <class T>
+__device__ inline void prolong_node(const KLevelT<T> &F, const KLevelT<T> &C, int64_t t) {
   const KGrid &g = F.g;
   const KGrid &gc = C.g;
   const int kp = g.n3 / 2 + 1;
@@ -420,34 +439,35 @@ __device__ inline void prolong_node(const KLevel &F, const KLevel &C, int64_t t)
   int64_t r = t / ni;
   int j = 1 + (int)(r % nj);
   int k = 1 + (int)(r / nj);
-  const double *e = C.u;
-  double r000 = e[gidx(gc, i, j, k)], r0m0 = e[gidx(gc, i, j - 1, k)],
+  const T *e = C.u;
+  T r000 = e[gidx(gc, i, j, k)], r0m0 = e[gidx(gc, i, j - 1, k)],
          rmm0 = e[gidx(gc, i - 1, j - 1, k)], rm00 = e[gidx(gc, i - 1, j, k)],
          r00m = e[gidx(gc, i, j, k - 1)], r0mm = e[gidx(gc, i, j - 1, k - 1)],
          rmmm = e[gidx(gc, i - 1, j - 1, k - 1)], rm0m = e[gidx(gc, i - 1, j, k - 1)];
   const int ir = 2 * i - 1, jr = 2 * j - 1, kr = 2 * k - 1;
-  double *u = F.u;
+  T *u = F.u;
   const int32_t *bc = F.bc;
   int64_t c;
   c = gidx(g, ir, jr, kr);
   if (bc[c] <= 0) u[c] = u[c] + r000;
   c = gidx(g, ir, jr - 1, kr);
-  if (bc[c] <= 0) u[c] = u[c] + (r000 + r0m0) / 2.0;
+  if (bc[c] <= 0) u[c] = u[c] + (r000 + r0m0) / T(2);
   c = gidx(g, ir - 1, jr - 1, kr);
-  if (bc[c] <= 0) u[c] = u[c] + (r000 + r0m0 + rmm0 + rm00) / 4.0;
+  if (bc[c] <= 0) u[c] = u[c] + (r000 + r0m0 + rmm0 + rm00) / T(4);
   c = gidx(g, ir - 1, jr, kr);
-  if (bc[c] <= 0) u[c] = u[c] + (r000 + rm00) / 2.0;
+  if (bc[c] <= 0) u[c] = u[c] + (r000 + rm00) / T(2);
   c = gidx(g, ir, jr, kr - 1);
-  if (bc[c] <= 0) u[c] = u[c] + (r000 + r00m) / 2.0;
+  if (bc[c] <= 0) u[c] = u[c] + (r000 + r00m) / T(2);
   c = gidx(g, ir, jr - 1, kr - 1);
-  if (bc[c] <= 0) u[c] = u[c] + (r000 + r00m + r0mm + r0m0) / 4.0;
+  if (bc[c] <= 0) u[c] = u[c] + (r000 + r00m + r0mm + r0m0) / T(4);
   c = gidx(g, ir - 1, jr, kr - 1);
-  if (bc[c] <= 0) u[c] = u[c] + (r000 + r00m + rm0m + rm00) / 4.0;
+  if (bc[c] <= 0) u[c] = u[c] + (r000 + r00m + rm0m + rm00) / T(4);
   c = gidx(g, ir - 1, jr - 1, kr - 1);
-  if (bc[c] <= 0) u[c] = u[c] + (r000 + r00m + rm0m + rm00 + r0m0 + r0mm + rmmm + rmm0) / 8.0;
+  if (bc[c] <= 0) u[c] = u[c] + (r000 + r00m + rm0m + rm00 + r0m0 + r0mm + rmmm + rmm0) / T(8);
 }
 
-__global__ __launch_bounds__(BLOCK) void k_prolong(KLevel F, KLevel C) {
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_prolong(KLevelT<T> F, KLevelT<T> C) {
   prolong_node(F, C, blockIdx.x * (int64_t)BLOCK + threadIdx.x);
 }
 
@@ -467,24 +487,30 @@ constexpr int CBLOCK = 1024;
 // levels fit.
 constexpr int64_t COARSE_LDS_NODES = 8704;
 
+template <class T>
 struct CoarseArgs {
-  KLevel L[MAX_LEVELS];   // u and b point to global memory; the kernel works on LDS copies
+  KLevelT<T> L[MAX_LEVELS];   // u and b point to global memory; the kernel works on LDS copies
   int lc, ng;
-  double omega, eps;
+  T omega;
+  double eps;
   int flag_pbc, flag_nmn;
   int first_cycle;
-  double *rp;
+  int fixed;                  // 1: every sor_rb runs its sweep count, no residual exit
+  int nr_fixed, nr_coarsest;  // those counts (fixed = 1)
+  T *rp;
   MGCtl *ctl;
 };
 
 // Level l as the workgroup sees it: u and b in LDS.
+template <class T>
 struct CoarseLDS {
-  double *u[MAX_LEVELS];
-  double *b[MAX_LEVELS];
+  T *u[MAX_LEVELS];
+  T *b[MAX_LEVELS];
 };
 
-__device__ inline KLevel lds_level(const CoarseArgs &A, const CoarseLDS &S, int l) {
-  KLevel v = A.L[l];
+template <class T>
+__device__ inline KLevelT<T> lds_level(const CoarseArgs<T> &A, const CoarseLDS<T> &S, int l) {
+  KLevelT<T> v = A.L[l];
   v.u = S.u[l];
   v.b = S.b[l];
   return v;
@@ -503,10 +529,12 @@ __device__ inline double wg_sum(double a, double *red) {
   return r;
 }
 
-// sor_rb (sors.f90:1-535) on level l: at most nr sweeps, exit rules as k_decide.
-__device__ void wg_sor_rb(const CoarseArgs &A, const CoarseLDS &S, int l, int dig, int64_t nr,
+// sor_rb (sors.f90:1-535) on level l: at most nr sweeps, exit rules as k_decide
+// (with A.fixed, exactly nr sweeps and no residual).
+template <class T>
+__device__ void wg_sor_rb(const CoarseArgs<T> &A, const CoarseLDS<T> &S, int l, int dig, int64_t nr,
                           double *red) {
-  const KLevel Lv = lds_level(A, S, l);
+  const KLevelT<T> Lv = lds_level(A, S, l);
   const KGrid &g = Lv.g;
   const int64_t nsor = (int64_t)(g.n1 / 2 + 1) * (g.n2 + 1) * (g.n3 + 1);
   const int64_t nres = (int64_t)(g.n1 + 1) * (g.n2 + 1) * (g.n3 + 1);
@@ -524,10 +552,14 @@ __device__ void wg_sor_rb(const CoarseArgs &A, const CoarseLDS &S, int l, int di
         __syncthreads();
       }
     }
+    ++ksor;
+    if (A.fixed) {
+      if (ksor >= nr) break;
+      continue;
+    }
     double a = 0.0;
     for (int64_t t = threadIdx.x; t < nres; t += CBLOCK) a = a + resid_node(Lv, t);
     res = wg_sum(a, red);
-    ++ksor;
     bool stop;
     if (dig == 0) stop = (res <= A.eps);
     else stop = (res_s > 0.0 && (res / res_s) <= 0.6);
@@ -544,13 +576,14 @@ __device__ void wg_sor_rb(const CoarseArgs &A, const CoarseLDS &S, int l, int di
   }
 }
 
-__device__ void wg_restriction(const CoarseArgs &A, const CoarseLDS &S, int l) {
-  const KLevel F = lds_level(A, S, l);
-  const KLevel C = lds_level(A, S, l + 1);
+template <class T>
+__device__ void wg_restriction(const CoarseArgs<T> &A, const CoarseLDS<T> &S, int l) {
+  const KLevelT<T> F = lds_level(A, S, l);
+  const KLevelT<T> C = lds_level(A, S, l + 1);
   const KGrid &g = F.g;
   const int64_t nc = ((int64_t)C.g.n1 + 3) * ((int64_t)C.g.n2 + 3) * ((int64_t)C.g.n3 + 3);
-  for (int64_t t = threadIdx.x; t < nc; t += CBLOCK) C.u[t] = 0.0;
-  const double *rpp = nullptr;
+  for (int64_t t = threadIdx.x; t < nc; t += CBLOCK) C.u[t] = T(0);
+  const T *rpp = nullptr;
   if (A.flag_pbc) {
     const int64_t nent = ((int64_t)g.n1 + 2) * ((int64_t)g.n2 + 2);
     for (int64_t t = threadIdx.x; t < nent; t += CBLOCK) rpbc_entry(F, A.rp, A.flag_nmn, t);
@@ -562,9 +595,10 @@ __device__ void wg_restriction(const CoarseArgs &A, const CoarseLDS &S, int l) {
   __syncthreads();
 }
 
-__device__ void wg_prolongation(const CoarseArgs &A, const CoarseLDS &S, int l) {  // L[l] += P(L[l+1])
-  const KLevel F = lds_level(A, S, l);
-  const KLevel C = lds_level(A, S, l + 1);
+template <class T>
+__device__ void wg_prolongation(const CoarseArgs<T> &A, const CoarseLDS<T> &S, int l) {  // L[l] += P(L[l+1])
+  const KLevelT<T> F = lds_level(A, S, l);
+  const KLevelT<T> C = lds_level(A, S, l + 1);
   const KGrid &g = F.g;
   const int64_t nt = (int64_t)(g.n1 / 2 + 1) * (g.n2 / 2 + 1) * (g.n3 / 2 + 1);
   for (int64_t t = threadIdx.x; t < nt; t += CBLOCK) prolong_node(F, C, t);
@@ -579,13 +613,14 @@ __device__ void wg_prolongation(const CoarseArgs &A, const CoarseLDS &S, int l) 
 // would read: zeros at the ghost nodes no restriction writes, values that
 // the V-cycle overwrites before reading elsewhere), and u of level lc, the
 // only array read after the launch, is copied back at the end.
-__global__ __launch_bounds__(CBLOCK) void k_mg_coarse(CoarseArgs A) {
+template <class T>
+__global__ __launch_bounds__(CBLOCK) void k_mg_coarse(CoarseArgs<T> A) {
   __shared__ double red[CBLOCK / LEPIC_WAVE];
-  __shared__ double lds[2 * COARSE_LDS_NODES];
-  __shared__ CoarseLDS S;   // level pointers into lds, uniform
+  __shared__ T lds[2 * COARSE_LDS_NODES];
+  __shared__ CoarseLDS<T> S;   // level pointers into lds, uniform
   const int ng = A.ng;
   {
-    double *p = lds;
+    T *p = lds;
     for (int l = A.lc; l < ng; ++l) {
       const KGrid &g = A.L[l].g;
       const int64_t nn = ((int64_t)g.n1 + 3) * ((int64_t)g.n2 + 3) * ((int64_t)g.n3 + 3);
@@ -601,16 +636,17 @@ __global__ __launch_bounds__(CBLOCK) void k_mg_coarse(CoarseArgs A) {
     }
   }
   __syncthreads();
+  const int64_t nr = A.fixed ? A.nr_fixed : 4;
   for (int l = A.lc; l <= ng - 2; ++l) {
-    wg_sor_rb(A, S, l, l + 1 - ng, 4, red);
+    wg_sor_rb(A, S, l, l + 1 - ng, nr, red);
     __syncthreads();
     wg_restriction(A, S, l);
   }
-  wg_sor_rb(A, S, ng - 1, 0, A.first_cycle ? 1000000 : 4, red);
+  wg_sor_rb(A, S, ng - 1, 0, A.fixed ? A.nr_coarsest : (A.first_cycle ? 1000000 : 4), red);
   __syncthreads();
   if (ng - 2 >= A.lc) wg_prolongation(A, S, ng - 2);
   for (int l = ng - 2; l >= A.lc; --l) {
-    wg_sor_rb(A, S, l, ng - 1 - l, 4, red);
+    wg_sor_rb(A, S, l, ng - 1 - l, nr, red);
     __syncthreads();
     if (l - 1 >= A.lc) wg_prolongation(A, S, l - 1);
   }
@@ -622,24 +658,29 @@ __global__ __launch_bounds__(CBLOCK) void k_mg_coarse(CoarseArgs A) {
   }
 }
 
-__global__ void k_fill_d(double *a, int64_t n, double v) {
+template <class T>
+__global__ void k_fill_d(T *a, int64_t n, T v) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (i < n) a[i] = v;
 }
 
+template <class T>
 struct Solver {
   lepic_ctx *ctx;
-  KLevel L[MAX_LEVELS];
+  KLevelT<T> L[MAX_LEVELS];
   int ng;
-  double omega;
+  T omega;
   int flag_pbc, flag_nmn;
   double *partials;
-  double *rp;
+  T *rp;
   MGCtl *ctl;  // device, MAX_LEVELS entries
   int lc;      // first level handled by k_mg_coarse (ng: none)
+  // fixed sweep counts (the inner smoother of the mixed-precision solve):
+  // nr_fixed per sor_rb, nr_coarsest on the coarsest level, no residuals
+  int fixed = 0, nr_fixed = 2, nr_coarsest = 64;
 
   void coarse(double eps, int kmg) {
-    CoarseArgs A;
+    CoarseArgs<T> A;
     for (int l = 0; l < MAX_LEVELS; ++l) A.L[l] = L[l];
     A.lc = lc;
     A.ng = ng;
@@ -648,13 +689,16 @@ struct Solver {
     A.flag_pbc = flag_pbc;
     A.flag_nmn = flag_nmn;
     A.first_cycle = kmg == 1 ? 1 : 0;
+    A.fixed = fixed;
+    A.nr_fixed = nr_fixed;
+    A.nr_coarsest = nr_coarsest;
     A.rp = rp;
     A.ctl = ctl;
-    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(CBLOCK), 0, ctx->stream, A);
+    hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(CBLOCK), 0, ctx->stream, A);
   }
 
   void sor_sweep(int l, double eps, int dig, int64_t nr) {
-    KLevel &Lv = L[l];
+    KLevelT<T> &Lv = L[l];
     const KGrid &g = Lv.g;
     int64_t nsor = (int64_t)(g.n1 / 2 + 1) * (g.n2 + 1) * (g.n3 + 1);
     int64_t nres = (int64_t)(g.n1 + 1) * (g.n2 + 1) * (g.n3 + 1);
@@ -662,15 +706,16 @@ struct Solver {
     const unsigned br = std::min<unsigned>(grid_blocks(nres, BLOCK), RESID_BLOCKS);
     hipStream_t s = ctx->stream;
     for (int colour = 0; colour < 2; ++colour) {
-      hipLaunchKernelGGL(k_sor, dim3(bs), dim3(BLOCK), 0, s, Lv, colour, omega, &ctl[l]);
+      hipLaunchKernelGGL(k_sor<T>, dim3(bs), dim3(BLOCK), 0, s, Lv, colour, omega, &ctl[l]);
       if (flag_pbc) {
-        hipLaunchKernelGGL(k_per_y, dim3(grid_blocks((int64_t)(g.n1 + 3) * (g.n3 + 3), BLOCK)),
+        hipLaunchKernelGGL(k_per_y<T>, dim3(grid_blocks((int64_t)(g.n1 + 3) * (g.n3 + 3), BLOCK)),
                            dim3(BLOCK), 0, s, Lv, &ctl[l]);
-        hipLaunchKernelGGL(k_per_z, dim3(grid_blocks((int64_t)(g.n1 + 3) * (g.n2 + 3), BLOCK)),
+        hipLaunchKernelGGL(k_per_z<T>, dim3(grid_blocks((int64_t)(g.n1 + 3) * (g.n2 + 3), BLOCK)),
                            dim3(BLOCK), 0, s, Lv, &ctl[l]);
       }
     }
-    hipLaunchKernelGGL(k_resid, dim3(br), dim3(BLOCK), 0, s, Lv, partials, &ctl[l], nres);
+    if (fixed) return;   // a fixed sweep count: no residual, the level stays active
+    hipLaunchKernelGGL(k_resid<T>, dim3(br), dim3(BLOCK), 0, s, Lv, partials, &ctl[l], nres);
     hipLaunchKernelGGL(k_decide, dim3(1), dim3(BLOCK), 0, s, partials, (int64_t)br, &ctl[l], eps,
                        dig, nr);
   }
@@ -678,6 +723,10 @@ struct Solver {
   // sor_rb: returns LEPIC_OK; sweeps are counted on the device.
   int sor_rb(int l, double eps, int kmg, int dig) {
     hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, ctx->stream, &ctl[l]);
+    if (fixed) {
+      for (int c = 0; c < (dig == 0 ? nr_coarsest : nr_fixed); ++c) sor_sweep(l, eps, dig, 4);
+      return LEPIC_OK;
+    }
     if (kmg == 1 && dig == 0) {
       // first V-cycle's coarsest level: up to 1e6 sweeps to res <= eps.
       const int64_t nr = 1000000;
@@ -698,46 +747,100 @@ struct Solver {
   }
 
   void restriction(int l) {
-    KLevel &F = L[l];
-    KLevel &C = L[l + 1];
+    KLevelT<T> &F = L[l];
+    KLevelT<T> &C = L[l + 1];
     const KGrid &g = F.g;
     int64_t nc = grid_nodes_k(C.g);
-    hipLaunchKernelGGL(k_fill_d, dim3(grid_blocks(nc, BLOCK)), dim3(BLOCK), 0, ctx->stream, C.u,
-                       nc, 0.0);
-    const double *rpp = nullptr;
+    hipLaunchKernelGGL(k_fill_d<T>, dim3(grid_blocks(nc, BLOCK)), dim3(BLOCK), 0, ctx->stream, C.u,
+                       nc, T(0));
+    const T *rpp = nullptr;
     if (flag_pbc) {
       int64_t nent = (int64_t)(g.n1 + 2) * (g.n2 + 2);
-      hipLaunchKernelGGL(k_rpbc, dim3(grid_blocks(nent, BLOCK)), dim3(BLOCK), 0, ctx->stream, F,
+      hipLaunchKernelGGL(k_rpbc<T>, dim3(grid_blocks(nent, BLOCK)), dim3(BLOCK), 0, ctx->stream, F,
                          rp, flag_nmn);
       rpp = rp;
     }
     int64_t nt = (int64_t)(g.n1 / 2 + 1) * (g.n2 / 2 + 1) * (g.n3 / 2 + 1);
-    hipLaunchKernelGGL(k_restrict, dim3(grid_blocks(nt, BLOCK)), dim3(BLOCK), 0, ctx->stream, F,
+    hipLaunchKernelGGL(k_restrict<T>, dim3(grid_blocks(nt, BLOCK)), dim3(BLOCK), 0, ctx->stream, F,
                        C, rpp, flag_nmn);
   }
 
   void prolongation(int l) {  // L[l] += P(L[l+1])
     const KGrid &g = L[l].g;
     int64_t nt = (int64_t)(g.n1 / 2 + 1) * (g.n2 / 2 + 1) * (g.n3 / 2 + 1);
-    hipLaunchKernelGGL(k_prolong, dim3(grid_blocks(nt, BLOCK)), dim3(BLOCK), 0, ctx->stream,
+    hipLaunchKernelGGL(k_prolong<T>, dim3(grid_blocks(nt, BLOCK)), dim3(BLOCK), 0, ctx->stream,
                        L[l], L[l + 1]);
   }
 
   static int64_t grid_nodes_k(const KGrid &g) {
     return ((int64_t)g.n1 + 3) * ((int64_t)g.n2 + 3) * ((int64_t)g.n3 + 3);
   }
+
+  // One V-cycle of mg (mg.f90:99-239).  Levels lc..ng-1 run inside
+  // k_mg_coarse: the stretch from the down-leg sor_rb on lc to the up-leg
+  // sor_rb on lc touches no level above lc and is issued as one launch.
+  int vcycle(int kmg, double epsa) {
+    for (int ig = 1; ig <= 2 * ng - 1; ++ig) {
+      for (int il = 1; il <= ng - 1 && il <= 11; ++il) {
+        const int l = il - 1;
+        if (ig < ng && ig == il) {
+          if (l < lc) {
+            int st = sor_rb(l, epsa, kmg, ig - ng);
+            if (st) return st;
+            restriction(l);
+          } else if (l == lc) {
+            coarse(epsa, kmg);
+          }
+        }
+        if (ig > ng && ig == 2 * ng - il) {
+          if (l < lc) {
+            int st = sor_rb(l, epsa, kmg, ig - ng);
+            if (st) return st;
+          }
+          if (l <= lc && il >= 2) prolongation(il - 2);
+        }
+      }
+      if (ig == ng) {
+        if (lc >= ng) {
+          int st = sor_rb(ng - 1, epsa, kmg, 0);
+          if (st) return st;
+        } else if (lc == ng - 1) {
+          coarse(epsa, kmg);
+        }
+        if (ng >= 2 && ng - 2 < lc) prolongation(ng - 2);
+      }
+    }
+    return LEPIC_OK;
+  }
+
+  // lc: the first level from which the remaining levels fit the coarse
+  // workgroup's LDS (ng: none do).
+  void choose_lc() {
+    lc = ng;
+    for (int l = 1; l < ng; ++l) {
+      int64_t tot = 0;
+      for (int m = l; m < ng; ++m) tot += grid_nodes_k(L[m].g);
+      if (tot <= COARSE_LDS_NODES) {
+        lc = l;
+        break;
+      }
+    }
+  }
 };
 
-void set_coeffs(KLevel &L, double eps0) {
-  // mg.h:9-15
+// mg.h:9-15, formed in FP64 (and rounded once for an FP32 level).
+template <class T>
+void set_coeffs(KLevelT<T> &L, double eps0) {
   const KGrid &g = L.g;
-  L.aw = eps0 / (g.h1 * g.h1);
-  L.ae = L.aw;
-  L.an = eps0 / (g.h2 * g.h2);
-  L.as = L.an;
-  L.at = eps0 / (g.h3 * g.h3);
-  L.ab = L.at;
-  L.ac = -(L.an + L.as + L.aw + L.ae + L.at + L.ab);
+  const double aw = eps0 / (g.h1 * g.h1);
+  const double ae = aw;
+  const double an = eps0 / (g.h2 * g.h2);
+  const double as = an;
+  const double at = eps0 / (g.h3 * g.h3);
+  const double ab = at;
+  const double ac = -(an + as + aw + ae + at + ab);
+  L.aw = (T)aw; L.ae = (T)ae; L.an = (T)an; L.as = (T)as; L.at = (T)at; L.ab = (T)ab;
+  L.ac = (T)ac;
 }
 
 }  // namespace
@@ -749,7 +852,7 @@ extern "C" int lepic_poisson_solve(lepic_ctx *ctx, const lepic_grid *g, double *
   if (!g || !phi || !rho || !bcnd || !mp) return lepic_fail(ctx, LEPIC_ERR_ARG, "poisson: null argument");
   const int ng = mp->ng;
   if (ng < 1 || ng > MAX_LEVELS) return lepic_fail(ctx, LEPIC_ERR_ARG, "poisson: ng outside [1,12]");
-  Solver S;
+  Solver<double> S;
   S.ctx = ctx;
   S.ng = ng;
   S.omega = mp->omega;
@@ -804,7 +907,7 @@ extern "C" int lepic_poisson_solve(lepic_ctx *ctx, const lepic_grid *g, double *
   S.L[0].b = const_cast<double *>(rho);
   S.L[0].bc = const_cast<int32_t *>(bcnd);
   for (int l = 1; l < ng; ++l) {
-    int64_t nn = Solver::grid_nodes_k(S.L[l].g);
+    int64_t nn = Solver<double>::grid_nodes_k(S.L[l].g);
     S.L[l].u = lv;
     S.L[l].b = lv + nn;
     lv += 2 * nn;
@@ -821,52 +924,8 @@ extern "C" int lepic_poisson_solve(lepic_ctx *ctx, const lepic_grid *g, double *
   double res = 0.0, res_s = 0.0;
   int k = 1;
   int status = LEPIC_OK;
-  // Levels lc..ng-1 of every V-cycle run inside k_mg_coarse.  The sequence of
-  // operations is mg.f90's; the stretch from the down-leg sor_rb on lc to the
-  // up-leg sor_rb on lc touches no level above lc and is issued as one launch.
-  S.lc = ng;
-  for (int l = 1; l < ng; ++l) {
-    int64_t tot = 0;
-    for (int m = l; m < ng; ++m) tot += Solver::grid_nodes_k(S.L[m].g);
-    if (tot <= COARSE_LDS_NODES) {
-      S.lc = l;
-      break;
-    }
-  }
-  // one V-cycle, mg (mg.f90:99-239)
-  auto vcycle = [&](int kmg) -> int {
-    for (int ig = 1; ig <= 2 * ng - 1; ++ig) {
-      for (int il = 1; il <= ng - 1 && il <= 11; ++il) {
-        const int l = il - 1;
-        if (ig < ng && ig == il) {
-          if (l < S.lc) {
-            int st = S.sor_rb(l, epsa, kmg, ig - ng);
-            if (st) return st;
-            S.restriction(l);
-          } else if (l == S.lc) {
-            S.coarse(epsa, kmg);
-          }
-        }
-        if (ig > ng && ig == 2 * ng - il) {
-          if (l < S.lc) {
-            int st = S.sor_rb(l, epsa, kmg, ig - ng);
-            if (st) return st;
-          }
-          if (l <= S.lc && il >= 2) S.prolongation(il - 2);
-        }
-      }
-      if (ig == ng) {
-        if (S.lc >= ng) {
-          int st = S.sor_rb(ng - 1, epsa, kmg, 0);
-          if (st) return st;
-        } else if (S.lc == ng - 1) {
-          S.coarse(epsa, kmg);
-        }
-        if (ng >= 2 && ng - 2 < S.lc) S.prolongation(ng - 2);
-      }
-    }
-    return LEPIC_OK;
-  };
+  S.choose_lc();
+  auto vcycle = [&](int kmg) -> int { return S.vcycle(kmg, epsa); };
   // With the coarsest level inside k_mg_coarse a V-cycle's launches are fixed
   // (every decision is taken on the device), so they are captured once per
   // solve configuration -- the first V-cycle and the later ones differ in one
@@ -939,6 +998,275 @@ extern "C" int lepic_poisson_solve(lepic_ctx *ctx, const lepic_grid *g, double *
     double ktot = 0.0;
     for (int l = 0; l < ng; ++l) {
       int64_t tot = ctx->h_pinned[16 * l + 4];
+      stats->sweeps[l] = tot;
+      ktot = ktot + (double)tot / std::pow(2.0, 2.0 * l);
+    }
+    stats->cycles = k;
+    stats->ktot = ktot;
+    stats->res = res;
+  }
+  return status;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision: FP32 red-black smoothing V-cycles inside an FP64 iterative
+// refinement.  Each refinement step forms the FP64 residual r = b - A phi at
+// every node the smoother updates (the smoother's own row rules: Neumann rows
+// with 2 ae and no west term, periodic copies left to the ghost update),
+// rounds it to FP32, runs a few fixed-sweep FP32 V-cycles of the same
+// multigrid on A e = r with e = 0 on electrodes, and adds e to phi in FP64.
+// Convergence is the reference's: L1 residual over plasma nodes / alpha
+// below eps (pdesolver.f90:97-115), measured on the FP64 residual.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int MIX_BLOCKS = 1024;
+
+// FP64 residual of the refinement step at ghosted node t; r32, e32 (the
+// FP32 right-hand side and correction) written for every node, the L1 sum
+// over plasma nodes (in the order of sors.f90:409-431's residual) to
+// partials.
+__global__ __launch_bounds__(BLOCK) void k_mix_resid(KLevel L, float *r32, float *e32,
+                                                     double *partials, int64_t nodes) {
+  __shared__ double red[BLOCK / LEPIC_WAVE];
+  const KGrid &g = L.g;
+  double a = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * BLOCK;
+  for (int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x; t < nodes; t += stride) {
+    const int i = (int)(t % g.sy);
+    const int64_t q = t / g.sy;
+    const int j = (int)(q % (g.n2 + 3)), k = (int)(q / (g.n2 + 3));
+    double r = 0.0;
+    if (i >= 1 && i <= g.n1 + 1 && j >= 1 && j <= g.n2 + 1 && k >= 1 && k <= g.n3 + 1) {
+      const int bcv = L.bc[t];
+      const double *u = L.u;
+      if (bcv == -2) {
+        r = L.b[t] - (L.an * u[t + g.sy] + L.as * u[t - g.sy] + 2.0 * L.ae * u[t + 1] +
+                      L.at * u[t + g.sz] + L.ab * u[t - g.sz] + L.ac * u[t]);
+      } else if (bcv == -1 || (bcv == 0 && j != g.n2 + 1 && k != g.n3 + 1)) {
+        r = L.b[t] - (L.aw * u[t - 1] + L.an * u[t + g.sy] + L.as * u[t - g.sy] +
+                      L.ae * u[t + 1] + L.ac * u[t] + L.at * u[t + g.sz] + L.ab * u[t - g.sz]);
+      }
+      if (bcv == -1) a = a + fabs(r);
+    }
+    r32[t] = (float)r;
+    e32[t] = 0.0f;
+  }
+  for (int d = LEPIC_WAVE / 2; d > 0; d >>= 1) a = a + __shfl_down(a, d);
+  if (__lane_id() == 0) red[threadIdx.x / LEPIC_WAVE] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = 0.0;
+    for (int w = 0; w < BLOCK / LEPIC_WAVE; ++w) b = b + red[w];
+    partials[blockIdx.x] = b;
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_mix_sum(const double *partials, int64_t nparts,
+                                                   double *out) {
+  __shared__ double red[BLOCK];
+  double a = 0.0;
+  for (int64_t p = threadIdx.x; p < nparts; p += BLOCK) a = a + partials[p];
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = BLOCK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// phi += e where the smoother may change phi (bcnd <= 0).
+__global__ void k_mix_correct(double *phi, const float *e32, const int32_t *bc, int64_t nodes) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t < nodes && bc[t] <= 0) phi[t] = phi[t] + (double)e32[t];
+}
+
+}  // namespace
+
+extern "C" int lepic_poisson_solve_mixed(lepic_ctx *ctx, const lepic_grid *g, double *phi,
+                                         const double *rho, const int32_t *bcnd,
+                                         const lepic_mg_params *mp, const lepic_mgmix_params *xp,
+                                         lepic_mg_stats *stats) {
+  if (!ctx) return LEPIC_ERR_ARG;
+  if (!g || !phi || !rho || !bcnd || !mp || !xp)
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "poisson_mixed: null argument");
+  const int ng = mp->ng;
+  if (ng < 1 || ng > MAX_LEVELS)
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "poisson_mixed: ng outside [1,12]");
+  if (xp->inner_cycles < 1 || xp->inner_cycles > 64 || xp->nu < 1 || xp->nu > 16 ||
+      xp->nu_coarsest < 1 || xp->nu_coarsest > 100000 || !(xp->omega > 0.0 && xp->omega < 2.0))
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "poisson_mixed: inner V-cycle parameters out of range");
+  Solver<float> S;
+  S.ctx = ctx;
+  S.ng = ng;
+  S.omega = (float)xp->omega;
+  S.flag_pbc = mp->flag_pbc;
+  S.flag_nmn = mp->flag_nmn;
+  S.fixed = 1;
+  S.nr_fixed = xp->nu;
+  S.nr_coarsest = xp->nu_coarsest;
+  KLevel F64;   // the FP64 fine level: phi, rho, labels
+  int64_t coarse_nodes = 0;
+  for (int l = 0; l < ng; ++l) {
+    lepic_grid gl;
+    for (int d = 0; d < 3; ++d) {
+      gl.n[d] = g->n[d] >> l;
+      gl.h[d] = g->h[d] * (double)(1 << l);
+    }
+    if (l < ng - 1)
+      for (int d = 0; d < 3; ++d)
+        if (gl.n[d] < 2 || (gl.n[d] % 2) != 0)
+          return lepic_fail(ctx, LEPIC_ERR_ARG,
+                            "poisson_mixed: every level above the coarsest needs even n >= 2");
+    if (gl.n[0] < 1 || gl.n[1] < 1 || gl.n[2] < 1)
+      return lepic_fail(ctx, LEPIC_ERR_ARG, "poisson_mixed: too many levels for this grid");
+    S.L[l].g = make_kgrid(&gl);
+    set_coeffs(S.L[l], mp->eps0);
+    if (l == 0) {
+      F64.g = S.L[0].g;
+      set_coeffs(F64, mp->eps0);
+    } else {
+      coarse_nodes += grid_nodes(&gl);
+    }
+  }
+  const int64_t nodes = grid_nodes(g);
+  const int64_t rp_len = ((int64_t)g->n[0] + 2) * ((int64_t)g->n[1] + 2);
+  const size_t need = sizeof(double) * (size_t)(MIX_BLOCKS + 8) +
+                      sizeof(float) * (size_t)(2 * nodes + 2 * coarse_nodes + rp_len) +
+                      sizeof(int32_t) * (size_t)coarse_nodes + 4096;
+  if (!ctx->mgx_mem || ctx->mgx_bytes < need) {
+    if (ctx->mgx_mem) {
+      LEPIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      LEPIC_HIP(ctx, hipFree(ctx->mgx_mem));
+      ctx->mgx_mem = nullptr;
+      ctx->mgx_bytes = 0;
+      ctx->mgx_key.clear();
+    }
+    LEPIC_HIP(ctx, hipMalloc(&ctx->mgx_mem, need));
+    LEPIC_HIP(ctx, hipMemsetAsync(ctx->mgx_mem, 0, need, ctx->stream));
+    ctx->mgx_bytes = need;
+  }
+  double *partials = reinterpret_cast<double *>(ctx->mgx_mem);
+  double *res_dev = partials + MIX_BLOCKS;
+  float *fmem = reinterpret_cast<float *>(res_dev + 8);
+  float *e32 = fmem, *r32 = fmem + nodes;
+  float *lv = r32 + nodes;
+  S.rp = lv;
+  lv += rp_len;
+  int32_t *lb = reinterpret_cast<int32_t *>(lv + 2 * coarse_nodes);
+  S.L[0].u = e32;
+  S.L[0].b = r32;
+  S.L[0].bc = const_cast<int32_t *>(bcnd);
+  for (int l = 1; l < ng; ++l) {
+    const int64_t nn = Solver<float>::grid_nodes_k(S.L[l].g);
+    S.L[l].u = lv;
+    S.L[l].b = lv + nn;
+    lv += 2 * nn;
+    S.L[l].bc = lb;
+    lb += nn;
+  }
+  S.partials = partials;   // unused with fixed sweeps
+  S.ctl = reinterpret_cast<MGCtl *>(ctx->d_ctrl + CTRL_MG);
+  S.choose_lc();
+  F64.u = phi;
+  F64.b = const_cast<double *>(rho);
+  F64.bc = const_cast<int32_t *>(bcnd);
+  const double alpha =
+      std::fabs((double)((g->n[0] + 1) * (g->n[1] + 1) * (g->n[2] + 1)) * F64.ac) * 1.0;
+  const unsigned rb = (unsigned)std::min<int64_t>(grid_blocks(nodes, BLOCK), MIX_BLOCKS);
+  // the inner V-cycles and the correction of one refinement step: one
+  // captured graph per solve configuration
+  auto inner = [&]() -> int {
+    for (int c = 0; c < xp->inner_cycles; ++c) {
+      int st = S.vcycle(2, 0.0);
+      if (st) return st;
+    }
+    hipLaunchKernelGGL(k_mix_correct, dim3(grid_blocks(nodes, 256)), dim3(256), 0, ctx->stream,
+                       phi, e32, bcnd, nodes);
+    if (mp->flag_pbc) {   // phi's periodic ghosts from their images (sors.f90:184-239)
+      const KGrid &gg = F64.g;
+      hipLaunchKernelGGL(k_per_y<double>, dim3(grid_blocks((int64_t)(gg.n1 + 3) * (gg.n3 + 3), BLOCK)),
+                         dim3(BLOCK), 0, ctx->stream, F64, &S.ctl[0]);
+      hipLaunchKernelGGL(k_per_z<double>, dim3(grid_blocks((int64_t)(gg.n1 + 3) * (gg.n2 + 3), BLOCK)),
+                         dim3(BLOCK), 0, ctx->stream, F64, &S.ctl[0]);
+    }
+    return LEPIC_OK;
+  };
+  const bool replay = mp->graph != 0;
+  if (replay) {
+    const int32_t ints[10] = {ng, g->n[0], g->n[1], g->n[2], mp->flag_pbc, mp->flag_nmn,
+                              xp->inner_cycles, xp->nu, xp->nu_coarsest, S.lc};
+    const void *ptrs[4] = {phi, bcnd, ctx->mgx_mem, ctx->d_ctrl};
+    const double dbl[5] = {g->h[0], g->h[1], g->h[2], xp->omega, mp->eps0};
+    std::vector<unsigned char> key(sizeof(ints) + sizeof(ptrs) + sizeof(dbl));
+    std::memcpy(key.data(), ints, sizeof(ints));
+    std::memcpy(key.data() + sizeof(ints), ptrs, sizeof(ptrs));
+    std::memcpy(key.data() + sizeof(ints) + sizeof(ptrs), dbl, sizeof(dbl));
+    if (key != ctx->mgx_key || !ctx->mgx_exec) {
+      if (ctx->mgx_exec) {
+        LEPIC_HIP(ctx, hipGraphExecDestroy(ctx->mgx_exec));
+        ctx->mgx_exec = nullptr;
+      }
+      ctx->mgx_key.clear();
+      if (!ctx->capture_stream)
+        LEPIC_HIP(ctx, hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+      const hipStream_t eager = ctx->stream;
+      ctx->stream = ctx->capture_stream;   // the Solver launches on ctx->stream
+      int st = LEPIC_OK;
+      hipGraph_t graph = nullptr;
+      hipError_t e = hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        st = inner();
+        e = hipStreamEndCapture(ctx->capture_stream, &graph);
+      }
+      if (e == hipSuccess && st == LEPIC_OK)
+        e = hipGraphInstantiate(&ctx->mgx_exec, graph, nullptr, nullptr, 0);
+      if (graph) (void)hipGraphDestroy(graph);
+      ctx->stream = eager;
+      if (st) return st;
+      if (e != hipSuccess) return lepic_fail(ctx, LEPIC_ERR_HIP, hipGetErrorString(e));
+      ctx->mgx_key = key;
+    }
+  }
+  hipLaunchKernelGGL(k_reset_totals, dim3(1), dim3(64), 0, ctx->stream, S.ctl, MAX_LEVELS);
+  double res = 0.0, res_s = 0.0;
+  int k = 0;
+  int status = LEPIC_OK;
+  for (;;) {
+    hipLaunchKernelGGL(k_mix_resid, dim3(rb), dim3(BLOCK), 0, ctx->stream, F64, r32, e32,
+                       partials, nodes);
+    hipLaunchKernelGGL(k_mix_sum, dim3(1), dim3(BLOCK), 0, ctx->stream, partials, (int64_t)rb,
+                       res_dev);
+    LEPIC_LAUNCH_CHECK(ctx, "mixed residual");
+    LEPIC_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, res_dev, sizeof(double), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+    LEPIC_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    double r;
+    std::memcpy(&r, &ctx->h_pinned[0], sizeof(double));
+    res = r / alpha;
+    if (k == 0) res_s = res;
+    else if (std::fabs(res / res_s) > 10.0) {
+      status = lepic_fail(ctx, LEPIC_ERR_DIVERGED, "poisson_mixed: res/res_s > 10 (diverging)");
+      break;
+    }
+    if (res < mp->eps || k == mp->ncycl) break;
+    ++k;
+    if (replay) {
+      LEPIC_HIP(ctx, hipGraphLaunch(ctx->mgx_exec, ctx->stream));
+    } else {
+      int st = inner();
+      if (st) return st;
+    }
+    LEPIC_LAUNCH_CHECK(ctx, "mixed refinement step");
+  }
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    double ktot = 0.0;
+    for (int l = 0; l < ng; ++l) {
+      // fixed sweep counts: nu per leg on every level above the coarsest
+      const int64_t per = (l == ng - 1) ? xp->nu_coarsest : 2 * xp->nu;
+      const int64_t tot = per * (int64_t)xp->inner_cycles * k;
       stats->sweeps[l] = tot;
       ktot = ktot + (double)tot / std::pow(2.0, 2.0 * l);
     }

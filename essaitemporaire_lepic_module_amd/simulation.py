@@ -332,3 +332,134 @@ class Simulation:
     def total_charge(self) -> float:
         """sum_s q_s Nm_s N_s."""
         return float(sum(q * nm * p.n for q, nm, p in zip(self.charge, self.Nm, self.parts)))
+
+
+class Simulation32(Simulation):
+    """The step in its precision form: particles as int32 cell + FP32 in-cell
+    offsets + FP32 velocities (particles32.ParticleSet32), the FP32 gather +
+    push + wall BC, the 64-bit fixed-point deposit (density independent of
+    particle order, charge exactly conserved), FP64 fields.  Collisions run
+    in PHILOX mode on the FP32 sets and adjust the fixed-point density for
+    the products they append and the particles they kill.  The order of the
+    step is Simulation's production path; every sort is the stable sort by
+    cell, so a run is reproducible bit for bit.
+
+    poisson: "fp64" (the reference's multigrid) or "mixed" (FP32 smoothing
+    inside FP64 iterative refinement, fields.solve_poisson_mixed)."""
+
+    def __init__(self, case: Case, device=None, philox_key: int = 0x5EED5EED,
+                 capacity_factor: float = 1.3, resort_every=None, poisson: str = "fp64",
+                 parts64=None):
+        from . import particles32 as P32
+        if case.flag_die or case.heating is not None:
+            raise ValueError("the FP32 step has no dielectric charging or electron heating")
+        if case.B_grid is not None:
+            raise ValueError("the FP32 step takes B uniform or as a map on the field grid")
+        super().__init__(case, device, fused=True, loader="device" if parts64 is None else "host",
+                         mcc_mode=_native.MCC_PHILOX, philox_key=philox_key,
+                         capacity_factor=capacity_factor, lazy=True, resort_every=None,
+                         parts=parts64, np_thread=None if parts64 is None else torch.zeros(
+                             (len(case.tracked),) + node_shape(case.n), dtype=torch.float64,
+                             device=device if device is not None else "cuda:0"))
+        self.poisson = poisson
+        self.resort_every32 = list(resort_every) if resort_every is not None else None
+        n, h = self.n, self.h
+        shape = node_shape(n)
+        dev = self.dev
+        self.frac_bits = [F.fixed_point_bits(n, h, nm) for nm in self.Nm]
+        parts32 = []
+        for p in self.parts:
+            q, out = P32.ParticleSet32.from_pset64(p, n, h, cap=max(p.nmax, 16))
+            if out:
+                raise ValueError(f"{out} particles lie outside the cells")
+            P32.sort32(q, n, h)
+            parts32.append(q)
+        self.parts = parts32
+        self.dep = torch.zeros((self.ntype,) + shape, dtype=torch.int64, device=dev)
+        for s, p in enumerate(self.parts):
+            P32.deposit32(p, n, h, self.Nm[s], self.frac_bits[s], self.dep[s])
+        self.np_thread.zero_()
+        for s in range(self.ntype):
+            P32.fixed_density(n, h, self.dep[s], self.kq, self.frac_bits[s], self.np_thread[s])
+        F.reduce_species_density(n, self.bcnd, self.np_thread, self.ntype, self.np_red)
+        self.E4f = torch.zeros(shape + (4,), dtype=torch.float32, device=dev)
+        self.escaped = torch.zeros(self.ntype, dtype=torch.int64, device=dev)
+        self.B4f = P32.round_field4(self.B4) if case.B_map is not None else None
+        self.bf32 = []
+        for q in self.charge:
+            mag = q <= 0 or case.ions_magnetized
+            if case.B_map is not None:
+                self.bf32.append(P32.bfield32(2, B4f=self.B4f, magnetized=mag))
+            elif case.B_uniform is not None:
+                self.bf32.append(P32.bfield32(1, case.B_uniform, magnetized=mag))
+            else:
+                self.bf32.append(P32.bfield32(0))
+        self._P32 = P32
+
+    def ensure_headroom(self):
+        for p in self.parts:
+            need = p.n + max(1 << 20, p.n // 64)
+            if need > p.nmax:
+                p.ensure_capacity(need)
+
+    def _compact(self, s: int):
+        p = self.parts[s]
+        if p.n > 0:
+            self._P32.sort32(p, self.n, self.h)
+        self.lost_acc[s].zero_()
+
+    def check_escaped(self):
+        """Raises when a survivor of a push left the cells (a domain without
+        walls around it); reads the device."""
+        esc = int(self.escaped.sum().item())
+        if esc:
+            raise RuntimeError(f"{esc} particles left the domain without meeting a wall")
+
+    def advance_one_step(self, istep: int):
+        c = self.case
+        n, h = self.n, self.h
+        P32 = self._P32
+        if istep % c.nb_step_sort == 1:
+            with self._phase("sort"):
+                for s in range(self.ntype):
+                    self._compact(s)
+        elif self.resort_every32 is not None:
+            with self._phase("sort"):
+                for s in range(self.ntype):
+                    k = self.resort_every32[s]
+                    if k and (istep - 1) % k == 0:
+                        self._compact(s)
+        with self._phase("rho"):
+            F.reduce_species_density(n, self.bcnd, self.np_thread, self.ntype, self.np_red)
+            F.build_rho_from_np(n, self.np_red, self.charge, self.ntype, self.rho)
+        with self._phase("poisson"):
+            if self.poisson == "mixed":
+                self.last_mg = F.solve_poisson_mixed(self.phi, self.bcnd, self.rho, h, n, c.ncycl,
+                                                     c.eps, c.omega, c.ng, c.flag_pbc, c.flag_nmn)
+            else:
+                self.last_mg = F.solve_poisson_legacy(self.phi, self.bcnd, self.rho, h, n, c.ncycl,
+                                                      c.eps, c.omega, c.ng, c.flag_pbc, c.flag_nmn)
+        with self._phase("efield"):
+            F.calc_Efield_modular(n, h, self.phi, self.E4, self.bcnd)
+            P32.round_field4(self.E4, self.E4f)
+        collide = (self.tables is not None and c.nb_step_collisions > 0
+                   and istep % c.nb_step_collisions == 0)
+        self.dep.zero_()
+        for s, p in enumerate(self.parts):
+            units = (p.n, self.lost_acc[s].clone()) if self._timing("push_bc_deposit") else 0
+            with self._phase("push_bc_deposit", units):
+                P32.push32(p, n, h, self.E4f, self.bf32[s], self.charge[s], self.mass[s], c.dt,
+                           self.bcnd, self.bp[s], self.frac_bits[s], self.dep[s], self.p_mac[s],
+                           self.P_loss[s:s + 1], self.lost_acc[s:s + 1],
+                           self.escaped[s:s + 1])
+        if collide:
+            self.ensure_headroom()
+            with self._phase("mcc"):
+                self.last_mcc = P32.collisions32(self.parts, n, h, self.np_red, self.tables,
+                                                 c.nb_step_collisions, c.dt, self.Pcoll,
+                                                 self.philox_key, istep, self.dep, self.frac_bits)
+        with self._phase("rho"):
+            self.np_thread.zero_()
+            for s in range(self.ntype):
+                P32.fixed_density(n, h, self.dep[s], self.kq, self.frac_bits[s], self.np_thread[s])
+            F.reduce_species_density(n, self.bcnd, self.np_thread, self.ntype, self.np_red)

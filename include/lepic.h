@@ -235,6 +235,28 @@ int lepic_build_rho(lepic_ctx *ctx, const lepic_grid *g, int32_t ntype, const do
 int lepic_poisson_solve(lepic_ctx *ctx, const lepic_grid *g, double *phi, const double *rho,
                         const int32_t *bcnd, const lepic_mg_params *mp, lepic_mg_stats *stats);
 
+/* Precision form of the solve: FP32 red-black smoothing V-cycles inside an
+ * FP64 iterative refinement.  Each refinement step forms the FP64 residual
+ * r = rho - A phi at every node the smoother updates, rounds it to FP32,
+ * runs inner_cycles V-cycles of the multigrid above in FP32 on A e = r (e = 0
+ * on electrodes) with fixed sweep counts -- nu red-black sweeps per level on
+ * each leg at over-relaxation omega, nu_coarsest on the coarsest level --
+ * and adds e to phi in FP64.  It stops, as pdesolver does, when the FP64 L1
+ * residual over plasma nodes / alpha falls below mp->eps (or after
+ * mp->ncycl refinement steps); stats->cycles counts refinement steps and
+ * stats->sweeps the FP32 sweeps per level.  mp->omega is not used.
+ * (No counterpart in the reference, whose solve is FP64 throughout.) */
+typedef struct {
+  int32_t inner_cycles;    /* FP32 V-cycles per refinement step (1..64)  */
+  int32_t nu;              /* sweeps per level per leg (1..16)           */
+  int32_t nu_coarsest;     /* sweeps on the coarsest level               */
+  double omega;            /* over-relaxation of the FP32 smoother       */
+} lepic_mgmix_params;
+
+int lepic_poisson_solve_mixed(lepic_ctx *ctx, const lepic_grid *g, double *phi, const double *rho,
+                              const int32_t *bcnd, const lepic_mg_params *mp,
+                              const lepic_mgmix_params *xp, lepic_mg_stats *stats);
+
 /* ---------------- a9: E field ---------------- */
 /* calc_Efield_modular (mod_electricField.f90:9-164). */
 int lepic_efield(lepic_ctx *ctx, const lepic_grid *g, const double *phi, const int32_t *bcnd,
@@ -369,6 +391,86 @@ int lepic_sort_compact(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *sr
  * is immaterial.  The particle arrays must be 16-byte aligned (dead: 4). */
 int lepic_bin_compact(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *src,
                       lepic_pset *dst);
+
+/* ---------------- precision form: the FP32 particle phase ---------------- */
+/* FP32 particle set of one species: the reference's ParticleSet
+ * (modules/particles/mod_particles.f90:7-34) held as an int32 cell id (as in
+ * lepic_pset.cell; -1: no particle -- lost, killed or never set) plus the
+ * position inside the cell in cell widths (x = ((ix-1) + fx) h1, fx in
+ * [0, 1)) and the velocity, all FP32: 28 bytes per particle.  Every array
+ * must be 16-byte aligned; kernels move them with vector accesses. */
+typedef struct {
+  int32_t *cell;
+  float *fx, *fy, *fz;
+  float *vx, *vy, *vz;
+  int64_t n;       /* slots in use (holes included)                  */
+  int64_t cap;     /* length of every array                          */
+} lepic_pset32;
+
+/* Magnetic field of the FP32 push (Src/part_expmover.f90:139-212): mode 0
+ * none, 1 uniform B, 2 a map on the E grid, 4 floats per node (Bx, By, Bz,
+ * |B|) with the |B| > 1e-5 gate at the lower node. */
+typedef struct {
+  int32_t mode;
+  double B[3];
+  const float *B4;
+  int32_t magnetized;
+} lepic_bfield32;
+
+/* FP64 set -> FP32 set, slot for slot: cell floor(x/h), offset x/h - cell
+ * rounded once to FP32.  Dead particles and particles outside [0, L) become
+ * holes; *n_outside (host, may be NULL: no synchronisation) receives the
+ * latter's count.  (The layout conversion of ParticleSet, no reference
+ * counterpart.) */
+int lepic_pset_to32(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *src,
+                    lepic_pset32 *dst, int64_t *n_outside);
+/* FP32 set -> FP64 set: x = ((ix-1) + fx) h1 in FP64, holes -> dead = 2. */
+int lepic_pset_from32(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset32 *src,
+                      lepic_pset *dst);
+/* F4f = F4 rounded to FP32 (4 values per node: E fields, B maps). */
+int lepic_round_field4(lepic_ctx *ctx, const double *F4, int64_t nodes, float *F4f);
+
+/* The particle phase of one modular step in FP32, in place: gather of the
+ * FP32 E (E4f, 4 floats per node) + part_mover's Boris/leapfrog update
+ * (Src/part_expmover.f90:51-246) + apply_particle_bc_legacy's wall test,
+ * losses, Neumann reflection and periodic wrap (mod_particleBC.f90:42-259)
+ * + the deposit (mod_chargeDeposition.f90:25-133) in fixed point: every
+ * survivor's full weight rint(|Nm/(h1 h2 h3)| 2^frac_bits) split into 8
+ * integer node parts as lepic_deposit_fixed splits it (upper shares = the
+ * new offsets) and added to dep (device int64, one per node, NULL: no
+ * deposit).  Constants are formed in FP64 and rounded once: k1 = dt q/(2m),
+ * dt/h per axis, the uniform-B rotation factor.  Lost particles become holes
+ * (cell = -1) and are added to *lost_acc (device); a survivor left outside
+ * the cells (no wall around the domain) also becomes a hole and is added to
+ * *escaped (device) -- an error for the caller to report.  Wall diagnostics
+ * p_mac / P_loss accumulate in FP64 as lepic_push_bc_deposit_inplace's.
+ * Dielectric charging (flag_die) is not supported here. */
+int lepic_push32(lepic_ctx *ctx, const lepic_grid *g, lepic_pset32 *p, const float *E4f,
+                 const lepic_bfield32 *bf, double q, double m, double dt, const int32_t *bcnd,
+                 const lepic_bc_params *bp, int32_t frac_bits, int64_t *dep, double *p_mac,
+                 double *P_loss, uint64_t *lost_acc, uint64_t *escaped);
+/* The fixed-point deposit (deposit_particle_set_to_np_thread,
+ * mod_chargeDeposition.f90:25-133) of an FP32 set, added to dep. */
+int lepic_deposit32(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset32 *p, double Nm,
+                    int32_t frac_bits, int64_t *dep);
+/* np_local += kq * dep * 2^-frac_bits: the FP64 density of fixed-point sums. */
+int lepic_fixed_density(lepic_ctx *ctx, const lepic_grid *g, const int64_t *dep, const double *kq,
+                        int32_t frac_bits, double *np_local);
+/* sort_particles_by_cell (mod_particle_sorting.f90:88-181) of an FP32 set
+ * into dst: stable (order within a cell = order in src), holes dropped,
+ * dst->n = particles kept.  Synchronises once. */
+int lepic_sort32(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset32 *src,
+                 lepic_pset32 *dst);
+/* perform_collisions_step (mod_collisions.f90:214-739) on FP32 sets, PHILOX
+ * mode (as lepic_collisions), every reaction's target a neutral.  Products
+ * take the parent's cell and offsets; killed particles become holes.  dep
+ * (device, [ntype_tracked][nodes] int64, may be NULL) holds the step's
+ * fixed-point deposit: appended products add theirs and killed particles
+ * remove theirs, with frac_bits[t] (host) per species. */
+int lepic_collisions32(lepic_ctx *ctx, const lepic_grid *g, lepic_pset32 *parts,
+                       const double *np_red, const lepic_chem *chem, int32_t ns_coll, double dt,
+                       uint64_t philox_key, uint64_t philox_counter, double *Pcoll,
+                       lepic_mcc_stats *stats, int64_t *dep, const int32_t *frac_bits);
 
 #ifdef __cplusplus
 }

@@ -1679,3 +1679,294 @@ int lo_advance_one_step(lo_state *s, int32_t istep) {
   lo_reduce_density(g, s->bcnd, s->ntype, 1, s->np_thread, s->np_red);
   return LO_OK;
 }
+
+/* ===================================================================== */
+/* The FP32 particle phase (the precision form; no reference counterpart */
+/* beyond the FP64 routines it restates in single precision):            */
+/* cell id + FP32 in-cell offsets + FP32 velocities.                     */
+/* ===================================================================== */
+
+/* F4f = F4 rounded to FP32. */
+void lo32_round4(const double *F4, int64_t count, float *F4f) {
+  for (int64_t i = 0; i < count; ++i) F4f[i] = (float)F4[i];
+}
+
+/* FP64 -> FP32 set, slot for slot: cell floor(x/h), offset x/h - floor(x/h)
+ * (exact in FP64) rounded once to FP32, a rounding up to 1 moving the
+ * particle to the next cell's lower face.  Dead particles and particles
+ * outside [0, L) become holes (cell -1); returns the count of the latter. */
+int64_t lo32_from64(const lo_grid *g, const lo_pset *src, lo_pset32 *dst) {
+  int64_t outside = 0;
+  for (int64_t i = 0; i < src->n; ++i) {
+    int32_t cell = -1;
+    float f[3] = {0.0f, 0.0f, 0.0f};
+    if (src->dead[i] == 0) {
+      const double x[3] = {src->x[i], src->y[i], src->z[i]};
+      int c[3];
+      int out = 0;
+      for (int d = 0; d < 3; ++d) {
+        const double q = x[d] / g->h[d];
+        const double fl = floor(q);
+        c[d] = -1;
+        if (q >= 0.0 && fl < (double)g->n[d]) {
+          c[d] = (int)fl;
+          f[d] = (float)(q - fl);
+          if (f[d] >= 1.0f) { f[d] = 0.0f; c[d] += 1; }
+          if (c[d] >= g->n[d]) c[d] = -1;
+        }
+        if (c[d] < 0) out = 1;
+      }
+      if (out) {
+        outside++;
+        f[0] = f[1] = f[2] = 0.0f;
+      } else {
+        cell = c[0] + g->n[0] * (c[1] + g->n[1] * c[2]);
+      }
+    }
+    dst->cell[i] = cell;
+    dst->fx[i] = f[0]; dst->fy[i] = f[1]; dst->fz[i] = f[2];
+    dst->vx[i] = (float)src->vx[i]; dst->vy[i] = (float)src->vy[i]; dst->vz[i] = (float)src->vz[i];
+  }
+  dst->n = src->n;
+  return outside;
+}
+
+static void decode32(const lo_grid *g, int32_t c, int *cx, int *cy, int *cz) {
+  *cx = c % g->n[0];
+  const int r = c / g->n[0];
+  *cy = r % g->n[1];
+  *cz = r / g->n[1];
+}
+
+/* FP32 -> FP64 set: x = ((ix-1) + fx) h1 in FP64; holes -> dead = 2. */
+void lo32_to64(const lo_grid *g, const lo_pset32 *src, lo_pset *dst) {
+  for (int64_t i = 0; i < src->n; ++i) {
+    const int32_t c = src->cell[i];
+    int cx = 0, cy = 0, cz = 0;
+    if (c >= 0) decode32(g, c, &cx, &cy, &cz);
+    dst->x[i] = ((double)cx + (double)src->fx[i]) * g->h[0];
+    dst->y[i] = ((double)cy + (double)src->fy[i]) * g->h[1];
+    dst->z[i] = ((double)cz + (double)src->fz[i]) * g->h[2];
+    dst->vx[i] = (double)src->vx[i];
+    dst->vy[i] = (double)src->vy[i];
+    dst->vz[i] = (double)src->vz[i];
+    dst->cell[i] = c;
+    dst->dead[i] = c >= 0 ? 0 : 2;
+  }
+  dst->n = src->n;
+}
+
+static void weights32(float fx, float fy, float fz, float w[8]) {
+  const float ax = 1.0f - fx, ay = 1.0f - fy, az = 1.0f - fz;
+  w[0] = ax * ay * az;
+  w[1] = fx * ay * az;
+  w[2] = fx * fy * az;
+  w[3] = ax * fy * az;
+  w[4] = ax * ay * fz;
+  w[5] = fx * ay * fz;
+  w[6] = fx * fy * fz;
+  w[7] = ax * fy * fz;
+}
+
+/* 4-float node field at the 8 stencil nodes of lower node b, component k,
+ * summed w1..w8 left to right in FP32. */
+static float gather32(const lo_grid *g, const float *F4, int64_t b, const float w[8], int k) {
+  const int64_t sy = NX(g), sz = NX(g) * NY(g);
+  const int64_t nd[8] = {b, b + 1, b + sy + 1, b + sy, b + sz, b + sz + 1, b + sz + sy + 1,
+                         b + sz + sy};
+  float s = w[0] * F4[4 * nd[0] + k];
+  for (int q = 1; q < 8; ++q) s = s + w[q] * F4[4 * nd[q] + k];
+  return s;
+}
+
+static int advance32(float *f, float d) {
+  const float s = *f + d;
+  const float fl = floorf(s);
+  float r = s - fl;
+  int k = (int)fl;
+  if (r >= 1.0f) { r = 0.0f; k += 1; }
+  *f = r;
+  return k;
+}
+
+/* Fixed-point parts of a particle at lower node (ix,iy,iz), offsets f, added
+ * (sign +1) or removed (-1) from dep. */
+static void dep32(const lo_grid *g, int64_t *dep, double U, int neg, int ix, int iy, int iz,
+                  float fx, float fy, float fz, int sign) {
+  int64_t a[8];
+  lo_fixed_split(U, (double)fx, (double)fy, (double)fz, a);
+  const int64_t c[8] = {IDX(g, ix, iy, iz),         IDX(g, ix + 1, iy, iz),
+                        IDX(g, ix + 1, iy + 1, iz), IDX(g, ix, iy + 1, iz),
+                        IDX(g, ix, iy, iz + 1),     IDX(g, ix + 1, iy, iz + 1),
+                        IDX(g, ix + 1, iy + 1, iz + 1), IDX(g, ix, iy + 1, iz + 1)};
+  const int flip = (sign < 0) != (neg != 0);
+  for (int q = 0; q < 8; ++q)
+    dep[c[q]] = (int64_t)((uint64_t)dep[c[q]] + (uint64_t)(flip ? -a[q] : a[q]));
+}
+
+static double unit32(const lo_grid *g, double Nm, int32_t frac_bits, int *neg) {
+  const double k1 = Nm / (g->h[0] * g->h[1] * g->h[2]);
+  *neg = k1 < 0.0;
+  return rint(ldexp(fabs(k1), frac_bits));
+}
+
+/* The FP32 particle phase in place: gather (E4f, FP32), Boris/leapfrog
+ * update (Src/part_expmover.f90:139-240 in FP32), offsets advanced by v dt/h,
+ * the wall test / losses / Neumann reflection / periodic wrap of
+ * apply_particle_bc_legacy (mod_particleBC.f90:96-227) in cell terms, and
+ * the fixed-point deposit of survivors (dep may be NULL).  Lost particles
+ * become holes; so do survivors left outside the cells (counted in
+ * *escaped).  Constants: k1 = dt q/(2m), dt/h, the uniform-B factor k2,
+ * formed in FP64 and rounded once. */
+void lo32_push(const lo_grid *g, lo_pset32 *p, const float *E4f, int bmode, const double *Bc,
+               const float *B4f, int magnetized, double q, double m, double dt,
+               const int32_t *bcnd, int flag_pbc, int flag_nmn, int ptype, int tag_neg,
+               double qmacro, double mass, double Nm, int32_t ngrid, int32_t frac_bits,
+               int64_t *dep, double *p_mac, double *P_loss, int64_t *lost, int64_t *escaped) {
+  const int32_t *n = g->n;
+  const double k1d = dt * q / (2.0 * m);
+  const float k1 = (float)k1d;
+  const float dtx = (float)(dt / g->h[0]), dty = (float)(dt / g->h[1]),
+              dtz = (float)(dt / g->h[2]);
+  const float bx = (float)Bc[0], by = (float)Bc[1], bz = (float)Bc[2];
+  const double Bt = sqrt(Bc[0] * Bc[0] + Bc[1] * Bc[1] + Bc[2] * Bc[2]);
+  const float k2u = (float)(k1d * (2.0 / (1.0 + (k1d * Bt) * (k1d * Bt))));
+  int neg = 0;
+  const double U = dep ? unit32(g, Nm, frac_bits, &neg) : 0.0;
+  for (int64_t i = 0; i < p->n; ++i) {
+    const int32_t c = p->cell[i];
+    if (c < 0) continue;
+    int cx, cy, cz;
+    decode32(g, c, &cx, &cy, &cz);
+    const int ix = cx + 1, iy = cy + 1, iz = cz + 1;
+    float fx = p->fx[i], fy = p->fy[i], fz = p->fz[i];
+    float vx = p->vx[i], vy = p->vy[i], vz = p->vz[i];
+    float w[8];
+    weights32(fx, fy, fz, w);
+    const int64_t b = IDX(g, ix, iy, iz);
+    const float Ex = gather32(g, E4f, b, w, 0), Ey = gather32(g, E4f, b, w, 1),
+                Ez = gather32(g, E4f, b, w, 2);
+    int use_b = 0;
+    float Bx = 0.0f, By = 0.0f, Bz = 0.0f;
+    if (bmode == 1) {
+      Bx = bx; By = by; Bz = bz;
+      use_b = 1;
+    } else if (bmode == 2 && B4f[4 * b + 3] > 1.0e-5f) {
+      Bx = gather32(g, B4f, b, w, 0); By = gather32(g, B4f, b, w, 1);
+      Bz = gather32(g, B4f, b, w, 2);
+      use_b = 1;
+    }
+    if (!magnetized) use_b = 0;
+    if (use_b) {
+      float k2;
+      if (bmode == 1) {
+        k2 = k2u;
+      } else {
+        const float Btf = sqrtf(Bx * Bx + By * By + Bz * Bz);
+        k2 = k1 * (2.0f / (1.0f + (k1 * Btf) * (k1 * Btf)));
+      }
+      const float v1x = vx + k1 * Ex, v1y = vy + k1 * Ey, v1z = vz + k1 * Ez;
+      const float v3x = v1x + k1 * (v1y * Bz - v1z * By);
+      const float v3y = v1y + k1 * (v1z * Bx - v1x * Bz);
+      const float v3z = v1z + k1 * (v1x * By - v1y * Bx);
+      const float v2x = v1x + k2 * (v3y * Bz - v3z * By);
+      const float v2y = v1y + k2 * (v3z * Bx - v3x * Bz);
+      const float v2z = v1z + k2 * (v3x * By - v3y * Bx);
+      vx = v2x + k1 * Ex; vy = v2y + k1 * Ey; vz = v2z + k1 * Ez;
+    } else {
+      vx = vx + 2.0f * k1 * Ex; vy = vy + 2.0f * k1 * Ey; vz = vz + 2.0f * k1 * Ez;
+    }
+    int nx = ix + advance32(&fx, vx * dtx);
+    int ny = iy + advance32(&fy, vy * dty);
+    int nz = iz + advance32(&fz, vz * dtz);
+    /* wall test at the clamped lower node */
+    int ax = nx < 0 ? 0 : (nx > n[0] + 1 ? n[0] + 1 : nx);
+    int ay = ny < 0 ? 0 : (ny > n[1] + 1 ? n[1] + 1 : ny);
+    int az = nz < 0 ? 0 : (nz > n[2] + 1 ? n[2] + 1 : nz);
+    int flag_lost = all_walls(g, bcnd, ax, ay, az) ? 1 : 0;
+    int label = flag_lost ? bcnd[IDX(g, ax, ay, az)] : 0;
+    if (ptype == tag_neg && nx <= 0 && flag_nmn == 1) { flag_lost = 2; label = 0; }
+    if (flag_lost) {
+      if (label < 0) label = 0;
+      const double v2 = (double)vx * (double)vx + (double)vy * (double)vy + (double)vz * (double)vz;
+      const double Ek_eV = 0.5 * mass * v2 / QE;
+      if (label <= ngrid) {
+        p_mac[2 * label + 0] = p_mac[2 * label + 0] + qmacro;
+        p_mac[2 * label + 1] = p_mac[2 * label + 1] + fabs(qmacro) * Ek_eV;
+      }
+      *P_loss = *P_loss + Nm * (0.5 * mass * v2);
+      (*lost)++;
+      p->cell[i] = -1;
+      p->fx[i] = fx; p->fy[i] = fy; p->fz[i] = fz;
+      p->vx[i] = vx; p->vy[i] = vy; p->vz[i] = vz;
+      continue;
+    }
+    if (flag_nmn == 1) {
+      if (nx <= 0) {
+        const float r = 1.0f - fx;
+        nx = 1 - nx;
+        if (r >= 1.0f) { fx = 0.0f; nx += 1; } else { fx = r; }
+        vx = -vx;
+      } else if (nx == 1 && fx == 0.0f) {
+        vx = -vx;
+      }
+    }
+    if (flag_pbc == 1) {
+      if (ny >= n[1] + 1) ny -= n[1];
+      if (ny <= 0) ny += n[1];
+      if (nz >= n[2] + 1) nz -= n[2];
+      if (nz <= 0) nz += n[2];
+    }
+    p->fx[i] = fx; p->fy[i] = fy; p->fz[i] = fz;
+    p->vx[i] = vx; p->vy[i] = vy; p->vz[i] = vz;
+    if (nx < 1 || nx > n[0] || ny < 1 || ny > n[1] || nz < 1 || nz > n[2]) {
+      (*escaped)++;
+      p->cell[i] = -1;
+      continue;
+    }
+    p->cell[i] = (nx - 1) + n[0] * ((ny - 1) + n[1] * (nz - 1));
+    if (dep) dep32(g, dep, U, neg, nx, ny, nz, fx, fy, fz, 1);
+  }
+}
+
+/* The fixed-point deposit of an FP32 set (lepic_deposit_fixed's split with
+ * the offsets as upper shares), added to dep. */
+void lo32_deposit(const lo_grid *g, const lo_pset32 *p, double Nm, int32_t frac_bits,
+                  int64_t *dep) {
+  int neg = 0;
+  const double U = unit32(g, Nm, frac_bits, &neg);
+  for (int64_t i = 0; i < p->n; ++i) {
+    const int32_t c = p->cell[i];
+    if (c < 0) continue;
+    int cx, cy, cz;
+    decode32(g, c, &cx, &cy, &cz);
+    dep32(g, dep, U, neg, cx + 1, cy + 1, cz + 1, p->fx[i], p->fy[i], p->fz[i], 1);
+  }
+}
+
+/* np += kq * dep * 2^-frac_bits. */
+void lo32_fixed_density(const lo_grid *g, const int64_t *dep, const double *kq,
+                        int32_t frac_bits, double *np) {
+  const double inv = ldexp(1.0, -frac_bits);
+  for (int64_t t = 0; t < NODES(g); ++t) np[t] = np[t] + kq[t] * ((double)dep[t] * inv);
+}
+
+/* Stable counting sort by cell of an FP32 set, holes dropped
+ * (sort_particles_by_cell, mod_particle_sorting.f90:88-181). */
+void lo32_sort(const lo_grid *g, const lo_pset32 *src, lo_pset32 *dst) {
+  const int64_t ncells = (int64_t)g->n[0] * g->n[1] * g->n[2];
+  int64_t *start = (int64_t *)calloc((size_t)ncells + 1, sizeof(int64_t));
+  for (int64_t i = 0; i < src->n; ++i)
+    if (src->cell[i] >= 0) start[src->cell[i] + 1]++;
+  for (int64_t c = 0; c < ncells; ++c) start[c + 1] += start[c];
+  for (int64_t i = 0; i < src->n; ++i) {
+    const int32_t c = src->cell[i];
+    if (c < 0) continue;
+    const int64_t d = start[c]++;
+    dst->cell[d] = c;
+    dst->fx[d] = src->fx[i]; dst->fy[d] = src->fy[i]; dst->fz[d] = src->fz[i];
+    dst->vx[d] = src->vx[i]; dst->vy[d] = src->vy[i]; dst->vz[d] = src->vz[i];
+  }
+  dst->n = start[ncells - 1 >= 0 ? ncells - 1 : 0];
+  free(start);
+}

@@ -222,6 +222,28 @@ typedef struct {
 
 int lo_advance_one_step(lo_state *s, int32_t istep);
 
+/* ---- the FP32 particle phase (precision form) ---- */
+typedef struct {
+  int32_t *cell;               /* cell id (0-based), -1: no particle    */
+  float *fx, *fy, *fz;         /* in-cell offsets, cell widths, [0, 1)  */
+  float *vx, *vy, *vz;
+  int64_t n, cap;
+} lo_pset32;
+
+void lo32_round4(const double *F4, int64_t count, float *F4f);
+int64_t lo32_from64(const lo_grid *g, const lo_pset *src, lo_pset32 *dst);
+void lo32_to64(const lo_grid *g, const lo_pset32 *src, lo_pset *dst);
+void lo32_push(const lo_grid *g, lo_pset32 *p, const float *E4f, int bmode, const double *Bc,
+               const float *B4f, int magnetized, double q, double m, double dt,
+               const int32_t *bcnd, int flag_pbc, int flag_nmn, int ptype, int tag_neg,
+               double qmacro, double mass, double Nm, int32_t ngrid, int32_t frac_bits,
+               int64_t *dep, double *p_mac, double *P_loss, int64_t *lost, int64_t *escaped);
+void lo32_deposit(const lo_grid *g, const lo_pset32 *p, double Nm, int32_t frac_bits,
+                  int64_t *dep);
+void lo32_fixed_density(const lo_grid *g, const int64_t *dep, const double *kq,
+                        int32_t frac_bits, double *np);
+void lo32_sort(const lo_grid *g, const lo_pset32 *src, lo_pset32 *dst);
+
 #ifdef __cplusplus
 }
 #endif

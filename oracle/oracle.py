@@ -538,3 +538,145 @@ class OracleSim:
         for s, p in enumerate(self.parts):
             deposit(n, h, p, self.kq, self.Nm[s], self.np_thread[s])
         self.np_red = reduce_density(n, self.bcnd, self.np_thread)
+
+
+# ---------------------------------------------------------------------------
+# The FP32 particle phase (cell id + FP32 in-cell offsets + FP32 velocities)
+# ---------------------------------------------------------------------------
+class PSet32(C.Structure):
+    _fields_ = [("cell", C.POINTER(C.c_int32)),
+                ("fx", C.POINTER(C.c_float)), ("fy", C.POINTER(C.c_float)),
+                ("fz", C.POINTER(C.c_float)), ("vx", C.POINTER(C.c_float)),
+                ("vy", C.POINTER(C.c_float)), ("vz", C.POINTER(C.c_float)),
+                ("n", C.c_int64), ("cap", C.c_int64)]
+
+
+FIELDS32 = ("cell", "fx", "fy", "fz", "vx", "vy", "vz")
+
+
+class Particles32:
+    """Host FP32 particle set (numpy-owned): cell (int32, -1 = no particle),
+    offsets fx fy fz in cell widths and velocities, float32."""
+
+    def __init__(self, cap: int):
+        self.cap = int(cap)
+        self.cell = np.full(cap, -1, np.int32)
+        for k in FIELDS32[1:]:
+            setattr(self, k, np.zeros(cap, np.float32))
+        self.s = PSet32(_p(self.cell, C.c_int32), *[_p(getattr(self, k), C.c_float)
+                                                   for k in FIELDS32[1:]], 0, self.cap)
+
+    @property
+    def n(self) -> int:
+        return int(self.s.n)
+
+    @n.setter
+    def n(self, v: int):
+        self.s.n = int(v)
+
+    def set(self, cell, fx, fy, fz, vx, vy, vz):
+        m = len(cell)
+        assert m <= self.cap
+        for k, v in zip(FIELDS32, (cell, fx, fy, fz, vx, vy, vz)):
+            getattr(self, k)[:m] = v
+        self.s.n = m
+        return self
+
+    def arrays(self):
+        m = self.n
+        return {k: getattr(self, k)[:m].copy() for k in FIELDS32}
+
+    def copy(self) -> "Particles32":
+        o = Particles32(self.cap)
+        for k in FIELDS32:
+            getattr(o, k)[:] = getattr(self, k)
+        o.n = self.n
+        return o
+
+
+_lib32 = None
+
+
+def lib32():
+    global _lib32
+    if _lib32 is None:
+        L = lib()
+        dp, ip, lp = C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+        fp = C.POINTER(C.c_float)
+        L.lo32_round4.argtypes = [dp, C.c_int64, fp]
+        L.lo32_from64.restype = C.c_int64
+        L.lo32_from64.argtypes = [C.POINTER(Grid), C.POINTER(PSet), C.POINTER(PSet32)]
+        L.lo32_to64.argtypes = [C.POINTER(Grid), C.POINTER(PSet32), C.POINTER(PSet)]
+        L.lo32_push.argtypes = [C.POINTER(Grid), C.POINTER(PSet32), fp, C.c_int, dp, fp, C.c_int,
+                                C.c_double, C.c_double, C.c_double, ip, C.c_int, C.c_int, C.c_int,
+                                C.c_int, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                                lp, dp, dp, lp, lp]
+        L.lo32_deposit.argtypes = [C.POINTER(Grid), C.POINTER(PSet32), C.c_double, C.c_int32, lp]
+        L.lo32_fixed_density.argtypes = [C.POINTER(Grid), lp, dp, C.c_int32, dp]
+        L.lo32_sort.argtypes = [C.POINTER(Grid), C.POINTER(PSet32), C.POINTER(PSet32)]
+        _lib32 = L
+    return _lib32
+
+
+def round4(F4):
+    """A 4-per-node field rounded to float32."""
+    a = np.ascontiguousarray(F4, np.float64)
+    out = np.zeros(a.shape, np.float32)
+    lib32().lo32_round4(_p(a), a.size, _p(out, C.c_float))
+    return out
+
+
+def from64(n, h, p: Particles, cap=None) -> tuple:
+    """FP64 set -> FP32 set (slot for slot); returns (set, outside count)."""
+    q = Particles32(cap or p.cap)
+    out = lib32().lo32_from64(C.byref(grid(n, h)), C.byref(p.s), C.byref(q.s))
+    return q, int(out)
+
+
+def to64(n, h, q: Particles32, cap=None) -> Particles:
+    p = Particles(cap or q.cap)
+    lib32().lo32_to64(C.byref(grid(n, h)), C.byref(q.s), C.byref(p.s))
+    return p
+
+
+def push32(n, h, p: Particles32, E4f, bmode, Bconst, B4f, q, m, dt, bcnd, flag_pbc, flag_nmn,
+           ptype, tag_neg, qmacro, mass, Nm, ngrid, frac_bits=0, dep=None, magnetized=True):
+    """The FP32 particle phase in place; dep (int64 per node) receives the
+    fixed-point deposit when given.  Returns (lost, escaped, p_mac, P_loss)."""
+    g = grid(n, h)
+    Bc = np.asarray(Bconst if Bconst is not None else [0.0, 0.0, 0.0], float)
+    B4a = np.ascontiguousarray(B4f, np.float32) if B4f is not None else np.zeros(4, np.float32)
+    E = np.ascontiguousarray(E4f, np.float32)
+    p_mac = np.zeros(2 * (ngrid + 1))
+    P_loss = np.zeros(1)
+    lost, esc = C.c_int64(0), C.c_int64(0)
+    if dep is not None:
+        assert dep.dtype == np.int64 and dep.flags.c_contiguous
+    lib32().lo32_push(C.byref(g), C.byref(p.s), _p(E, C.c_float), int(bmode), _p(Bc),
+                      _p(B4a, C.c_float), 1 if magnetized else 0, q, m, dt,
+                      _p(np.ascontiguousarray(bcnd, np.int32), C.c_int32), int(flag_pbc),
+                      int(flag_nmn), int(ptype), int(tag_neg), qmacro, mass, Nm, int(ngrid),
+                      int(frac_bits), _p(dep, C.c_int64) if dep is not None else None,
+                      _p(p_mac), _p(P_loss), C.byref(lost), C.byref(esc))
+    return lost.value, esc.value, p_mac, float(P_loss[0])
+
+
+def deposit32(n, h, p: Particles32, Nm, frac_bits, dep=None):
+    g = grid(n, h)
+    out = np.zeros(tuple(int(d) + 3 for d in n[::-1]), np.int64) if dep is None else dep
+    lib32().lo32_deposit(C.byref(g), C.byref(p.s), float(Nm), int(frac_bits), _p(out, C.c_int64))
+    return out
+
+
+def fixed_density(n, dep, kq, frac_bits, np_local=None):
+    g = grid(n, (1.0, 1.0, 1.0))
+    out = np.zeros(kq.shape) if np_local is None else np_local
+    lib32().lo32_fixed_density(C.byref(g), _p(np.ascontiguousarray(dep), C.c_int64), _p(kq),
+                               int(frac_bits), _p(out))
+    return out
+
+
+def sort32(n, h, p: Particles32) -> Particles32:
+    q = Particles32(p.cap)
+    lib32().lo32_sort(C.byref(grid(n, h)), C.byref(p.s), C.byref(q.s))
+    return q
