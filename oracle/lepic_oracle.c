@@ -1196,6 +1196,70 @@ static void prolongation(mg_level *F, const mg_level *C) {
       }
 }
 
+/* The multigrid's pieces one at a time, on caller-owned arrays, so that the
+ * tests can pin each against what the mathematics fixes: one colour of the
+ * SOR, the periodic ghost copy, sor_rb with its exit rules, the L1
+ * residual, one restriction and one prolongation. */
+static mg_level hook_level(const lo_grid *g, double *u, const double *b, const int32_t *bcnd,
+                           double eps0) {
+  mg_level L;
+  memset(&L, 0, sizeof(L));
+  L.g = *g;
+  L.u = u;
+  L.b = (double *)b;
+  L.bcnd = (int32_t *)bcnd;
+  coeffs(&L, eps0);
+  return L;
+}
+
+void lo_mg_sor_colour(const lo_grid *g, double *u, const double *b, const int32_t *bcnd,
+                      double eps0, int colour, double omega) {
+  mg_level L = hook_level(g, u, b, bcnd, eps0);
+  sor_colour(&L, colour, omega);
+}
+
+void lo_mg_periodic(const lo_grid *g, double *u, const int32_t *bcnd) {
+  mg_level L = hook_level(g, u, NULL, bcnd, 1.0);
+  periodic_u(&L);
+}
+
+double lo_mg_residual(const lo_grid *g, const double *u, const double *b, const int32_t *bcnd,
+                      double eps0) {
+  mg_level L = hook_level(g, (double *)u, b, bcnd, eps0);
+  return sor_residual(&L);
+}
+
+int lo_mg_sor_rb(const lo_grid *g, double *u, const double *b, const int32_t *bcnd, double eps0,
+                 double omega, double eps, int kmg, int dig, int flag_pbc, double *res_out) {
+  mg_level L = hook_level(g, u, b, bcnd, eps0);
+  return sor_rb(&L, res_out, omega, eps, kmg, dig, flag_pbc);
+}
+
+void lo_mg_restrict(const lo_grid *gf, const double *uf, const double *bf, const int32_t *bcf,
+                    double eps0, int flag_pbc, int flag_nmn, double *b_c, int32_t *bc_c) {
+  mg_level F = hook_level(gf, (double *)uf, bf, bcf, eps0);
+  lo_grid gc;
+  for (int d = 0; d < 3; ++d) {
+    gc.n[d] = gf->n[d] / 2;
+    gc.h[d] = gf->h[d] * 2.0;
+  }
+  double *uc = (double *)calloc((size_t)NODES(&gc), sizeof(double));
+  mg_level C = hook_level(&gc, uc, b_c, bc_c, eps0);
+  restriction(&F, &C, flag_pbc, flag_nmn);
+  free(uc);
+}
+
+void lo_mg_prolong(const lo_grid *gf, double *uf, const int32_t *bcf, const double *uc) {
+  mg_level F = hook_level(gf, uf, NULL, bcf, 1.0);
+  lo_grid gc;
+  for (int d = 0; d < 3; ++d) {
+    gc.n[d] = gf->n[d] / 2;
+    gc.h[d] = gf->h[d] * 2.0;
+  }
+  mg_level C = hook_level(&gc, (double *)uc, NULL, NULL, 1.0);
+  prolongation(&F, &C);
+}
+
 /* pdesolver (pdesolver.f90:1-135) + mg V-cycle (mg.f90:1-246). */
 int lo_pdesolver(const lo_grid *g, double *phi, const double *rho, const int32_t *bcnd,
                  int flag_pbc, int flag_nmn, double eps0, int32_t ncycl, double eps, double omega,
@@ -1408,6 +1472,27 @@ static int find_bracket(const double *Er, int npt, double E) {
   return hi;
 }
 
+/* Optional log of accepted events (tests only): one LO_MCC_REC-double record
+ * per accepted event, written while a log is set. */
+static double *mcc_log = NULL;
+static int64_t mcc_log_cap = 0, mcc_log_n = 0;
+
+void lo_mcc_set_log(double *buf, int64_t cap) {
+  mcc_log = buf;
+  mcc_log_cap = buf ? cap : 0;
+  mcc_log_n = 0;
+}
+
+int64_t lo_mcc_log_count(void) { return mcc_log_n; }
+
+static double *mcc_log_rec(void) {
+  if (!mcc_log || mcc_log_n >= mcc_log_cap) return NULL;
+  double *r = mcc_log + mcc_log_n * LO_MCC_REC;
+  for (int q = 0; q < LO_MCC_REC; ++q) r[q] = -1.0;
+  ++mcc_log_n;
+  return r;
+}
+
 int lo_collisions(const lo_grid *g, lo_pset *parts, const double *np_red, const lo_chem *c,
                   int32_t ns_coll, double dt, int32_t *iseed, double *Pcoll,
                   lo_mcc_stats *stats) {
@@ -1483,7 +1568,7 @@ int lo_collisions(const lo_grid *g, lo_pset *parts, const double *np_red, const 
       double vx1 = P->vx[ip], vy1 = P->vy[ip], vz1 = P->vz[ip];
       double nu[LO_MAX_COL], sum_nu = 0.0;
       int tsel[16] = {0}, tfound[16] = {0};
-      double svr[16], sEkr[16], smu[16], scx[16], scy[16], scz[16];
+      double svr[16], sEkr[16], smu[16], scx[16], scy[16], scz[16], stv[16][3];
       for (int ic2 = 0; ic2 < ncl; ++ic2) nu[ic2] = 0.0;
       for (int ic2 = 0; ic2 < ncl; ++ic2) {
         int col = c->sig_list[pt * LO_MAX_COL + ic2];
@@ -1514,6 +1599,7 @@ int lo_collisions(const lo_grid *g, lo_pset *parts, const double *np_red, const 
           scx[tt] = (mp * vx1 + mt * vx2) / (mp + mt);
           scy[tt] = (mp * vy1 + mt * vy2) / (mp + mt);
           scz[tt] = (mp * vz1 + mt * vz2) / (mp + mt);
+          stv[tt][0] = vx2; stv[tt][1] = vy2; stv[tt][2] = vz2;
           tsel[tt] = 1; tfound[tt] = 1;
         }
         if (!tfound[tt]) continue;
@@ -1558,8 +1644,18 @@ int lo_collisions(const lo_grid *g, lo_pset *parts, const double *np_red, const 
       int n_re = c->col_info[c_ind * 10 + 0];
       int n_by = c->col_info[c_ind * 10 + 1];
       int tt = c->col_info[c_ind * 10 + 1 + n_re];
+      double *rec = mcc_log_rec();
+      if (rec) {
+        rec[0] = pt; rec[1] = (double)ip; rec[2] = c_ind; rec[3] = tt;
+        rec[4] = vx1; rec[5] = vy1; rec[6] = vz1;
+        rec[11] = 0.0;   /* executed */
+      }
       if (tt < 0 || tt >= NA) continue;
       if (!tfound[tt]) continue;
+      if (rec) {
+        rec[7] = stv[tt][0]; rec[8] = stv[tt][1]; rec[9] = stv[tt][2];
+        rec[10] = n_by;
+      }
       double mu = smu[tt], vr = svr[tt];
       double vxc = scx[tt], vyc = scy[tt], vzc = scz[tt];
       double Eth = c->sig_Eth[c_ind];
@@ -1572,6 +1668,7 @@ int lo_collisions(const lo_grid *g, lo_pset *parts, const double *np_red, const 
       }
       if (smi <= 0.0) continue;
       if (stats) stats->executed[pt]++;
+      if (rec) rec[11] = 1.0;
       double costh = 1.0 - 2.0 * lo_ran2(&iseed[0]);
       double phi = 2.0 * PI_F * lo_ran2(&iseed[0]);
       double costh_s = 1.0 - 2.0 * lo_ran2(&iseed[0]);
@@ -1618,6 +1715,11 @@ int lo_collisions(const lo_grid *g, lo_pset *parts, const double *np_red, const 
         B->vz[ib_idx] = vzc + vp * ez1;
         B->dead[ib_idx] = 0;
         B->cell[ib_idx] = 0;
+        if (rec && ib < 3) {
+          double *o = rec + 12 + 5 * ib;
+          o[0] = bt; o[1] = (double)ib_idx;
+          o[2] = B->vx[ib_idx]; o[3] = B->vy[ib_idx]; o[4] = B->vz[ib_idx];
+        }
         double v2new = B->vx[ib_idx] * B->vx[ib_idx] + B->vy[ib_idx] * B->vy[ib_idx] +
                        B->vz[ib_idx] * B->vz[ib_idx];
         Pcoll[bt] = Pcoll[bt] + 0.5 * c->Nm[bt] * c->mass[bt] * (v2new - v2old);

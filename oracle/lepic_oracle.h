@@ -128,6 +128,19 @@ int lo_pdesolver(const lo_grid *g, double *phi, const double *rho,
                  int32_t ncycl, double eps, double omega, int32_t ng,
                  lo_mg_stats *stats);
 
+/* The solver's pieces on one level (coarse level: n/2 cells, 2h), for the
+ * tests that pin them one by one.  colour 0 = red = (i+j+k) odd. */
+void lo_mg_sor_colour(const lo_grid *g, double *u, const double *b, const int32_t *bcnd,
+                      double eps0, int colour, double omega);
+void lo_mg_periodic(const lo_grid *g, double *u, const int32_t *bcnd);
+double lo_mg_residual(const lo_grid *g, const double *u, const double *b, const int32_t *bcnd,
+                      double eps0);
+int lo_mg_sor_rb(const lo_grid *g, double *u, const double *b, const int32_t *bcnd, double eps0,
+                 double omega, double eps, int kmg, int dig, int flag_pbc, double *res_out);
+void lo_mg_restrict(const lo_grid *gf, const double *uf, const double *bf, const int32_t *bcf,
+                    double eps0, int flag_pbc, int flag_nmn, double *b_c, int32_t *bc_c);
+void lo_mg_prolong(const lo_grid *gf, double *uf, const int32_t *bcf, const double *uc);
+
 /* ---- E field (modules/fields/mod_electricField.f90) ---- */
 void lo_efield(const lo_grid *g, const double *phi, const int32_t *bcnd,
                double *E4);
@@ -181,6 +194,15 @@ typedef struct {
   int64_t executed[16];   /* accepted events that reached the scattering draws */
   int64_t appended[16];   /* products appended per species                     */
 } lo_mcc_stats;
+
+/* Test hook: while set, every accepted event appends one record of
+ * LO_MCC_REC doubles: projectile species, index, reaction, target species,
+ * projectile velocity (3), target velocity (3; for neutral targets the
+ * Maxwellian draw), n_by, executed (0/1), then per by-product written (up to
+ * 3): species, index, velocity (3).  Unused fields are -1. */
+#define LO_MCC_REC 27
+void lo_mcc_set_log(double *buf, int64_t cap);
+int64_t lo_mcc_log_count(void);
 
 int lo_collisions(const lo_grid *g, lo_pset *parts, const double *np_red,
                   const lo_chem *c, int32_t ns_coll, double dt,

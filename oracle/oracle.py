@@ -126,6 +126,17 @@ def lib():
                                    C.c_int32, C.c_double, C.c_double, C.c_int32,
                                    C.POINTER(MGStats)]
         L.lo_efield.argtypes = [C.POINTER(Grid), dp, ip, dp]
+        L.lo_mg_sor_colour.argtypes = [C.POINTER(Grid), dp, dp, ip, C.c_double, C.c_int,
+                                       C.c_double]
+        L.lo_mg_periodic.argtypes = [C.POINTER(Grid), dp, ip]
+        L.lo_mg_residual.restype = C.c_double
+        L.lo_mg_residual.argtypes = [C.POINTER(Grid), dp, dp, ip, C.c_double]
+        L.lo_mg_sor_rb.restype = C.c_int
+        L.lo_mg_sor_rb.argtypes = [C.POINTER(Grid), dp, dp, ip, C.c_double, C.c_double,
+                                   C.c_double, C.c_int, C.c_int, C.c_int, dp]
+        L.lo_mg_restrict.argtypes = [C.POINTER(Grid), dp, dp, ip, C.c_double, C.c_int, C.c_int,
+                                     dp, ip]
+        L.lo_mg_prolong.argtypes = [C.POINTER(Grid), dp, ip, dp]
         L.lo_electron_heating.argtypes = [C.POINTER(Grid), C.POINTER(PSet), C.POINTER(Heating),
                                           ip, dp, dp]
         L.lo_plane_moments.argtypes = [C.POINTER(Grid), C.POINTER(PSet), C.c_double, C.c_double,
@@ -338,6 +349,50 @@ def pdesolver(n, h, phi, rho, bcnd, flag_pbc=0, flag_nmn=0, eps0=8.854187817e-12
                     "sweeps": [st.sweeps[i] for i in range(int(ng))]}
 
 
+EPS0 = 8.854187817e-12
+
+
+def mg_sor_colour(n, h, u, b, bcnd, colour, omega, eps0=EPS0):
+    """One colour of the red-black SOR on one level, in place on u."""
+    lib().lo_mg_sor_colour(C.byref(grid(n, h)), _p(u), _p(b), _p(bcnd, C.c_int32), eps0,
+                           int(colour), float(omega))
+
+
+def mg_periodic(n, h, u, bcnd):
+    lib().lo_mg_periodic(C.byref(grid(n, h)), _p(u), _p(bcnd, C.c_int32))
+
+
+def mg_residual(n, h, u, b, bcnd, eps0=EPS0):
+    """The L1 residual over bcnd == -1 nodes that sor_rb's exit tests use."""
+    return lib().lo_mg_residual(C.byref(grid(n, h)), _p(u), _p(b), _p(bcnd, C.c_int32), eps0)
+
+
+def mg_sor_rb(n, h, u, b, bcnd, omega, eps, kmg, dig, flag_pbc=0, eps0=EPS0):
+    """sor_rb on one level, in place on u; returns (sweeps, last residual)."""
+    res = C.c_double(0.0)
+    k = lib().lo_mg_sor_rb(C.byref(grid(n, h)), _p(u), _p(b), _p(bcnd, C.c_int32), eps0,
+                           float(omega), float(eps), int(kmg), int(dig), int(flag_pbc),
+                           C.byref(res))
+    return k, res.value
+
+
+def mg_restrict(n, h, u, b, bcnd, flag_pbc=0, flag_nmn=0, eps0=EPS0):
+    """One restriction of the fine level's residual; returns the coarse
+    level's right-hand side and labels (n/2 cells)."""
+    nc = tuple(int(x) // 2 for x in n)
+    shape = (nc[2] + 3, nc[1] + 3, nc[0] + 3)
+    bc_c = np.zeros(shape, np.int32)
+    b_c = np.zeros(shape)
+    lib().lo_mg_restrict(C.byref(grid(n, h)), _p(u), _p(b), _p(bcnd, C.c_int32), eps0,
+                         int(flag_pbc), int(flag_nmn), _p(b_c), _p(bc_c, C.c_int32))
+    return b_c, bc_c
+
+
+def mg_prolong(n, h, u, bcnd, uc):
+    """u += P(uc) on the fine level, in place."""
+    lib().lo_mg_prolong(C.byref(grid(n, h)), _p(u), _p(bcnd, C.c_int32), _p(uc))
+
+
 _omp = None
 
 
@@ -410,6 +465,27 @@ class ChemTables:
             _p(a["sig"]), _p(a["sig_Eth"]), _p(a["col_info"], C.c_int32), _p(a["sigv_mx"]))
 
 
+MCC_REC = 27
+
+
+def collisions_logged(n, h, parts, np_red, chem, ns_coll, dt, seed_value, cap=200000):
+    """collisions() with the oracle's event log on: returns its result and
+    one row of MCC_REC values per accepted event (see lepic_oracle.h)."""
+    L = lib()
+    if not hasattr(L, "_mcc_log_typed"):
+        L.lo_mcc_set_log.argtypes = [C.POINTER(C.c_double), C.c_int64]
+        L.lo_mcc_log_count.restype = C.c_int64
+        L._mcc_log_typed = True
+    buf = np.zeros((cap, MCC_REC))
+    L.lo_mcc_set_log(_p(buf), cap)
+    try:
+        out = collisions(n, h, parts, np_red, chem, ns_coll, dt, seed_value)
+        m = L.lo_mcc_log_count()
+    finally:
+        L.lo_mcc_set_log(None, 0)
+    return out, buf[:m].copy()
+
+
 def collisions(n, h, parts, np_red, chem: ChemTables, ns_coll, dt, seed_value):
     g = grid(n, h)
     arr = (PSet * len(parts))(*[p.s for p in parts])
@@ -439,6 +515,7 @@ class OracleSim:
         self.bcnd = np.ascontiguousarray(case.bcnd, np.int32)
         self.kq = build_kq(self.n, self.bcnd)
         self.phi = case.phi0()
+        self.rho = np.zeros(self.bcnd.shape)
         tr = case.tracked
         self.ntype = len(tr)
         self.charge = [s.charge for s in tr]
@@ -487,6 +564,7 @@ class OracleSim:
                     sort_particles(n, h, p)
         self.np_red = reduce_density(n, self.bcnd, self.np_thread)
         rho = build_rho(n, self.np_red, self.charge)
+        self.rho = rho
         if c.flag_die:
             dielectric_phi(n, h, self.bcnd, c.dtype, c.ngrid, self.sum_q_xz, self.sum_q_yz,
                            c.Ca * h[0] * h[2], self.phi)

@@ -77,6 +77,26 @@ def test_plain_steps_match_oracle(gpu, oracle, lazy):
         assert sim.last_mg["sweeps"] == ref.last_mg["sweeps"]
 
 
+@pytest.mark.parametrize("lazy", [False, True])
+def test_plain_box_at_full_size(gpu, oracle, lazy):
+    """The plain-PIC workload at its own size: 32^3-cell grounded box, 20
+    particles per cell and species (1.3M particles), MG on 5 levels to 1e-8,
+    three steps against the oracle (the second and third with a nonzero
+    charge density from the first push)."""
+    from essaitemporaire_lepic_module_amd import particles as P
+    case = cfg.cfg1_plain_box()
+    assert case.n == (32, 32, 32) and case.np_cell == 20 and case.ng == 5
+    ref, sim = _start(oracle, case, P, lazy)
+    for istep in range(1, 4):
+        sim.advance_one_step(istep)
+        ref.advance_one_step(istep)
+        torch.cuda.synchronize()
+        _compare(sim, ref, exact=istep == 1, step=istep)
+        assert sim.last_mg["cycles"] == ref.last_mg["cycles"]
+        assert sim.last_mg["sweeps"] == ref.last_mg["sweeps"]
+    assert np.abs(ref.rho).max() > 0
+
+
 def test_magnetized_mcc_steps_match_oracle(gpu, oracle):
     """Uniform Bz, argon MCC every 2 steps in ran2 replay mode (exact compaction)."""
     from essaitemporaire_lepic_module_amd import particles as P

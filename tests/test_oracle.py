@@ -306,37 +306,72 @@ def test_wall_bc_brute_force(oracle, rng):
 
 
 # --------------------------------------------------------------- a7: density
-def test_reduce_density_and_rho_brute_force(oracle, rng):
+def _trapezoid(a, n):
+    """Per x column, the y-z sum of a node field with half weight on the
+    first and last physical planes of each axis (the periodic torus's
+    content counted once)."""
+    wy = np.ones(n[1] + 1)
+    wy[[0, -1]] = 0.5
+    wz = np.ones(n[2] + 1)
+    wz[[0, -1]] = 0.5
+    core = a[1:n[2] + 2, 1:n[1] + 2, :]
+    return np.einsum("kji,j,k->i", core, wy, wz)
+
+
+@pytest.mark.parametrize("labels", ["periodic_yz", "two_electrode"])
+def test_reduce_density_stitching_invariants(oracle, rng, labels):
+    """apply_periodic_density_bc + reduce_species_density
+    (mod_density.f90:31-73, 180-242) by what periodic stitching must do:
+    the two faces of each periodic axis end equal, every x column's
+    trapezoid total over the y-z torus is unchanged (a face node's two halves
+    become one node), nodes off the periodic faces are untouched, np_red is
+    the stitched field on physical nodes and zero on ghosts, and
+    rho = -sum_s q_s np_s."""
     n, h = H.box((5, 6, 4))
-    bc = H.periodic_yz_bcnd(n)
+    bc = H.periodic_yz_bcnd(n) if labels == "periodic_yz" else H.two_electrode_bcnd(n)
     npt = rng.random((2,) + cfg.node_shape(n))
-    ref = npt.copy()
-    for s in range(2):
-        a = ref[s]
-        for iz in range(1, n[2] + 2):
-            for ix in range(1, n[0] + 2):
-                if bc[iz, 1, ix] == 0:
-                    a[iz, 1, ix] = 0.5 * (a[iz, 1, ix] + a[iz, n[1] + 1, ix])
-                    a[iz, 0, ix] = a[iz, n[1], ix]
-                if bc[iz, n[1] + 1, ix] == 0:
-                    a[iz, n[1] + 2, ix] = a[iz, 2, ix]
-                    a[iz, n[1] + 1, ix] = a[iz, 1, ix]
-        for iy in range(1, n[1] + 2):
-            for ix in range(1, n[0] + 2):
-                if bc[1, iy, ix] == 0:
-                    a[1, iy, ix] = 0.5 * (a[1, iy, ix] + a[n[2] + 1, iy, ix])
-                    a[0, iy, ix] = a[n[2], iy, ix]
-                if bc[n[2] + 1, iy, ix] == 0:
-                    a[n[2] + 2, iy, ix] = a[2, iy, ix]
-                    a[n[2] + 1, iy, ix] = a[1, iy, ix]
+    orig = npt.copy()
     red = oracle.reduce_density(n, bc, npt)
-    np.testing.assert_array_equal(npt, ref)
-    expect = np.zeros_like(ref)
-    expect[:, 1:n[2] + 2, 1:n[1] + 2, 1:n[0] + 2] = ref[:, 1:n[2] + 2, 1:n[1] + 2, 1:n[0] + 2]
-    np.testing.assert_array_equal(red, expect)
+    inner = (slice(1, n[2] + 2), slice(1, n[1] + 2), slice(1, n[0] + 2))
+    for s in range(2):
+        a, o = npt[s], orig[s]
+        if labels == "two_electrode":   # nothing periodic: np_thread unchanged
+            np.testing.assert_array_equal(a, o)
+        else:
+            per = slice(2, n[0] + 1)    # x columns whose y/z face nodes are periodic (0)
+            np.testing.assert_array_equal(a[:, 1, per], a[:, n[1] + 1, per])
+            np.testing.assert_array_equal(a[1, 1:n[1] + 2, per], a[n[2] + 1, 1:n[1] + 2, per])
+            t0, t1 = _trapezoid(o, n), _trapezoid(a, n)
+            np.testing.assert_allclose(t1[per], t0[per], rtol=1e-14)
+            # every node off the periodic faces keeps its value
+            np.testing.assert_array_equal(a[2:n[2] + 1, 2:n[1] + 1, :],
+                                          o[2:n[2] + 1, 2:n[1] + 1, :])
+            np.testing.assert_array_equal(a[:, :, :2], o[:, :, :2])
+            np.testing.assert_array_equal(a[:, :, n[0] + 1:], o[:, :, n[0] + 1:])
+            # y ghosts hold their images on the planes the z stitch leaves alone
+            np.testing.assert_array_equal(a[2:n[2] + 1, 0, per], a[2:n[2] + 1, n[1], per])
+            np.testing.assert_array_equal(a[2:n[2] + 1, n[1] + 2, per], a[2:n[2] + 1, 2, per])
+            np.testing.assert_array_equal(a[0, 1:n[1] + 2, per], a[n[2], 1:n[1] + 2, per])
+            np.testing.assert_array_equal(a[n[2] + 2, 1:n[1] + 2, per], a[2, 1:n[1] + 2, per])
+        expect = np.zeros_like(a)
+        expect[inner] = a[inner]
+        np.testing.assert_array_equal(red[s], expect)
     q = [-QE, 2 * QE]
     rho = oracle.build_rho(n, red, q)
     np.testing.assert_allclose(rho, QE * red[0] - 2 * QE * red[1], rtol=1e-15, atol=0)
+
+
+def test_density_stitching_catches_a_sum():
+    """The trapezoid check fails for the plausible mistake of adding the two
+    faces instead of averaging them."""
+    n = (5, 6, 4)
+    rng = np.random.default_rng(3)
+    o = rng.random(cfg.node_shape(n))
+    a = o.copy()
+    s = a[:, 1, :] + a[:, n[1] + 1, :]
+    a[:, 1, :] = s
+    a[:, n[1] + 1, :] = s
+    assert not np.allclose(_trapezoid(a, n), _trapezoid(o, n), rtol=1e-6)
 
 
 # --------------------------------------------------------------- a8: Poisson
@@ -584,3 +619,58 @@ def test_boris_bmap_on_its_own_grid(oracle):
         o = one.arrays()
         for k in ("vx", "vy", "vz"):
             assert got[k][i] == pytest.approx(o[k][0], rel=1e-12, abs=1e-9)
+
+
+def test_boris_map_gate_at_the_lower_node(oracle):
+    """The map push's |B| gate (Src/part_expmover.f90:151-174, :158) reads the
+    stored |B| of the particle's lower stencil node only: with that value at
+    or below 1e-5 T the particle moves by the leapfrog update bit for bit,
+    however large the interpolated B is; above it, the step is the Boris step
+    with the trilinear B, even when every other node holds zero."""
+    n, h = H.box((8, 8, 8))
+    rng = np.random.default_rng(21)
+    E4 = H.random_E4(rng, n, scale=1e4)
+    shape = cfg.node_shape(n)
+    B4 = np.zeros(shape + (4,))
+    B4[..., :3] = 0.05 * rng.standard_normal(shape + (3,))
+    B4[..., 3] = np.linalg.norm(B4[..., :3], axis=-1)
+    # open particles sit in cells with even coordinates (so no open particle's
+    # 2x2x2 stencil holds another's lower node), gated ones elsewhere
+    allc = np.arange(n[0] * n[1] * n[2])
+    cx, cy, cz = allc % n[0], (allc // n[0]) % n[1], allc // (n[0] * n[1])
+    even = allc[(cx % 2 == 0) & (cy % 2 == 0) & (cz % 2 == 0)]
+    odd = np.setdiff1d(allc, even)
+    cells = np.empty(90, np.int64)
+    cells[2::3] = rng.choice(even, 30, replace=False)
+    gated = rng.choice(odd, 60, replace=False)
+    cells[0::3], cells[1::3] = gated[:30], gated[30:]
+    ix, iy, iz = cells % n[0], (cells // n[0]) % n[1], cells // (n[0] * n[1])
+    frac = rng.uniform(0.05, 0.95, (3, 90))
+    pos = [(ix + frac[0]) * h[0], (iy + frac[1]) * h[1], (iz + frac[2]) * h[2]]
+    vel = [rng.normal(0, 1e5, 90) for _ in range(3)]
+    lower = (iz + 1, iy + 1, ix + 1)       # the lower node (1-based) as an array index
+    gate = np.arange(90) % 3               # 0: |B| = 1e-5, 1: |B| = 0, 2: open, others zero
+    B4[lower[0][gate == 0], lower[1][gate == 0], lower[2][gate == 0], 3] = 1e-5
+    B4[lower[0][gate == 1], lower[1][gate == 1], lower[2][gate == 1], 3] = 0.0
+    op = gate == 2
+    for k, j, i in zip(lower[0][op], lower[1][op], lower[2][op]):
+        B4[k:k + 2, j:j + 2, i:i + 2, :] = 0.0   # the open particles' other nodes: zero
+    B4[lower[0][op], lower[1][op], lower[2][op], :] = (0.0, 0.0, 0.05, 2e-5)
+    q, m, dt = -QE, cfg.ME, 2e-11
+    p = oracle.Particles(90).set(*pos, *vel)
+    oracle.move_boris(n, h, p, E4, 2, (0.0, 0.0, 0.0), B4, q, m, dt)
+    got = p.arrays()
+    leap = oracle.Particles(90).set(*pos, *vel)
+    oracle.move_boris(n, h, leap, E4, 0, (0.0, 0.0, 0.0), None, q, m, dt)
+    lf = leap.arrays()
+    for key in ("x", "y", "z", "vx", "vy", "vz"):
+        np.testing.assert_array_equal(got[key][gate < 2], lf[key][gate < 2])
+    for i in np.nonzero(op)[0]:
+        # only the lower node carries B: the trilinear weight of that node
+        w1 = (1 - frac[0][i]) * (1 - frac[1][i]) * (1 - frac[2][i])
+        one = oracle.Particles(1).set(*([a[i]] for a in pos), *([a[i]] for a in vel))
+        oracle.move_boris(n, h, one, E4, 1, (0.0, 0.0, 0.05 * w1), None, q, m, dt)
+        o = one.arrays()
+        for key in ("vx", "vy", "vz"):
+            assert got[key][i] == pytest.approx(o[key][0], rel=1e-12, abs=1e-6)
+        assert got["vx"][i] != lf["vx"][i]
