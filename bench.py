@@ -4,8 +4,15 @@
 Workload: 128^3-cell grounded box, 64 particles/cell/species of e- and Ar+
 (268,435,456 particles), uniform Bz = 0.01 T, argon background at 1 Pa/300 K
 with the analytic cross sections, null-collision MCC every 10 steps (Philox,
-parallel), cell re-sort every 20 steps, FP64 particles, FP64 multigrid
-Poisson solve to 1e-8.  Particles are synthetic (device loader).
+parallel), cell re-sort every 20 steps, FP64 multigrid Poisson solve to 1e-8.
+Particles are synthetic (device loader).
+
+The headline is the step in its precision form, as the workload specifies
+it: FP32 particles (int32 cell + FP32 in-cell offsets + FP32 velocities),
+FP32 gather and push, int64 fixed-point deposit, FP64 fields.  The FP64 step
+is measured beside it in the same run, and the mixed-precision Poisson solve
+(FP32 red-black V-cycles inside FP64 refinement) against the FP64 multigrid
+on its own 256^3 problem.
 
 Prints one JSON line: particle-steps/s of the whole step, the roofline of the
 fused particle kernel (HIP events around its launches, on the stream it runs
@@ -15,6 +22,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -24,6 +32,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_PARTICLE_STEP = 104   # FP64 fused gather+push+BC+deposit, state read and written once
+BYTES_PER_PARTICLE_STEP_F32 = 56   # FP32: int32 cell + 3 FP32 offsets + 3 FP32 velocities, r + w
 COPY_PROBE_BYTES = 1 << 31
 
 
@@ -145,47 +154,35 @@ def cpu_baseline(case, sample_particles=32_000_000):
                        f"({t_grid * 1e3:.0f} ms) on one core, composed to the {N}-particle step")}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--ppc", type=int, default=64)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--resort-electrons", type=int, default=4,
-                    help="production path: also re-sort electrons by cell every k steps "
-                         "(0: only at the reference's sort cadence)")
-    ap.add_argument("--exact-order", action="store_true",
-                    help="compact every step by look-back (the reference's particle order "
-                         "at every step) instead of in place with holes removed at each sort")
-    ap.add_argument("--copy-probe", action="store_true",
-                    help="after the timed steps, copy 2 GiB device to device once: the "
-                         "calibration kernel for rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE runs")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this library runs on one GPU only; --gpus must be 1"}))
-        sys.exit(2)
-
+def _step_bench(args, precision):
+    """W warm-up steps, then K steps timed by wall clock between two device
+    synchronisations (HIP events only around the fused particle kernel),
+    then K more steps with every phase between events (not part of value).
+    precision "f32": the step's precision form (FP32 particles in cell +
+    offset layout, fixed-point deposit, FP64 fields); "f64": the FP64 step."""
     import torch
     from essaitemporaire_lepic_module_amd import _native
     from essaitemporaire_lepic_module_amd import config as cfg
-    from essaitemporaire_lepic_module_amd.simulation import Simulation
+    from essaitemporaire_lepic_module_amd.simulation import Simulation, Simulation32
 
     case = cfg.cfg2_pic_mcc(n=args.n, np_cell=args.ppc)
-    torch.cuda.init()
     resort = None if args.exact_order or args.resort_electrons <= 0 else \
         [args.resort_electrons] + [0] * (len(case.tracked) - 1)
-    sim = Simulation(case, "cuda:0", fused=True, loader="device", mcc_mode=_native.MCC_PHILOX,
-                     capacity_factor=1.25, lazy=not args.exact_order, resort_every=resort)
+    if precision == "f32":
+        sim = Simulation32(case, "cuda:0", capacity_factor=1.25, resort_every=resort)
+        kernel, bpp = "k_push32", BYTES_PER_PARTICLE_STEP_F32
+    else:
+        sim = Simulation(case, "cuda:0", fused=True, loader="device",
+                         mcc_mode=_native.MCC_PHILOX, capacity_factor=1.25,
+                         lazy=not args.exact_order, resort_every=resort)
+        kernel = "k_push_bc_deposit" if args.exact_order else "k_push"
+        bpp = BYTES_PER_PARTICLE_STEP
     istep = 1
     for _ in range(args.warmup):
         sim.advance_one_step(istep)
         istep += 1
     n_before = sim.particle_count()
     torch.cuda.synchronize()
-    # timed region: events only around the fused particle kernel's launches;
-    # live particles = slots (host) - holes (device, summed without a sync)
     sim.timers = {}
     sim.timed_phases = {"push_bc_deposit"}
     holes = torch.zeros_like(sim.lost_acc)
@@ -206,57 +203,167 @@ def main():
     step_ms = sorted(a.elapsed_time(b) for a, b in zip(marks[:-1], marks[1:]))
     particle_steps = slots - int(holes.sum().item())
     kernel_phases = sim.phase_times()
-    # diagnostic pass (not timed for the value): every phase bracketed by events
     sim.timers = {}
     sim.timed_phases = None
     for _ in range(args.steps):
         sim.advance_one_step(istep)
         istep += 1
     torch.cuda.synchronize()
+    phases = sim.phase_times()
+    if precision == "f32":
+        sim.check_escaped()
+    fused_ms, fused_launches, fused_units = kernel_phases.get("push_bc_deposit", (0.0, 0, 0))
+    achieved = bpp * fused_units / (fused_ms * 1e-3) / 1e9 if fused_ms else 0.0
+    mg_ms, mg_calls, _ = phases.get("poisson", (0.0, 1, 0))
+    out = {
+        "value": particle_steps / wall,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "step_ms_median": step_ms[len(step_ms) // 2],
+        "step_ms_p90": step_ms[min(len(step_ms) - 1, (9 * len(step_ms)) // 10)],
+        "particles": n_before,
+        "electron_resort_every": 0 if resort is None else resort[0],
+        "mg_solve_ms": mg_ms / max(mg_calls, 1),
+        "mg_vcycles_per_solve": sum(mg_cycles) / max(len(mg_cycles), 1),
+        "phase_ms_per_step": {k: v[0] / args.steps for k, v in phases.items()},
+        "roofline": {"bound": "hbm", "kernel": kernel,
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(kernel, args),
+                     "bytes_per_particle": bpp,
+                     "algorithmic_bytes_per_launch": bpp * fused_units / max(fused_launches, 1),
+                     "avg_launch_ms": fused_ms / max(fused_launches, 1)},
+    }
+    del sim
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return out
+
+
+def _poisson_cfg3(n):
+    """The mixed-precision Poisson workload: n^3 cells, six faces biased at
+    U[-100, 100] V, 64 Gaussian blobs of charge, solved to 1e-10 (L1 / alpha)
+    from phi = 0 inside, once with the reference's FP64 multigrid and once
+    with FP32 red-black V-cycles inside FP64 refinement; each solve timed
+    after one untimed solve of the same problem (graph capture).  Errors of
+    phi and E (max abs over nodes, relative to the FP64 solution's max)."""
+    import torch
+    from essaitemporaire_lepic_module_amd import config as cfg
+    from essaitemporaire_lepic_module_amd import fields as F
+    n3, h, bc, phi0, rhs = cfg.cfg3_poisson(n=n)
+    dev = "cuda:0"
+    bcg = torch.as_tensor(bc, device=dev)
+    rg = torch.as_tensor(rhs, device=dev)
+    ng = max(2, int(round(math.log2(n))))
+    res = {}
+    phis = {}
+    for name, solve in (("fp64", F.solve_poisson_legacy), ("mixed", F.solve_poisson_mixed)):
+        for rep in range(2):
+            phi = torch.as_tensor(phi0, device=dev).clone()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            info = solve(phi, bcg, rg, h, n3, 20000, 1e-10, 1.4, ng, 0, 0)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+        res[name] = {"ms_per_solve": dt * 1e3, "cycles": info["cycles"], "res": info["res"]}
+        phis[name] = phi
+    E = {}
+    for name, phi in phis.items():
+        E4 = torch.zeros(phi.shape + (4,), dtype=torch.float64, device=dev)
+        F.calc_Efield_modular(n3, h, phi, E4, bcg)
+        E[name] = E4
+    pmax = float(phis["fp64"].abs().max())
+    emax = float(E["fp64"][..., :3].abs().max())
+    res["phi_err_rel"] = float((phis["mixed"] - phis["fp64"]).abs().max()) / pmax
+    res["E_err_rel"] = float((E["mixed"][..., :3] - E["fp64"][..., :3]).abs().max()) / emax
+    res["speedup"] = res["fp64"]["ms_per_solve"] / res["mixed"]["ms_per_solve"]
+    res["workload"] = (f"{n}^3 cells, Dirichlet faces U[-100,100] V, 64 Gaussian blobs, "
+                       f"tolerance 1e-10, {ng} levels, from phi = 0")
+    del phis, E, bcg, rg
+    torch.cuda.empty_cache()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--ppc", type=int, default=64)
+    ap.add_argument("--precision", choices=("f32", "f64", "both"), default="both",
+                    help="f32: the step's precision form (FP32 particles, fixed-point deposit, "
+                         "FP64 Poisson) as the headline; f64: the FP64 step; both: f32 headline "
+                         "with the FP64 step measured beside it")
+    ap.add_argument("--poisson-n", type=int, default=256,
+                    help="grid of the mixed-precision Poisson comparison (0: skip)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--resort-electrons", type=int, default=4,
+                    help="production path: also re-sort electrons by cell every k steps "
+                         "(0: only at the reference's sort cadence)")
+    ap.add_argument("--exact-order", action="store_true",
+                    help="FP64 step: compact every step by look-back (the reference's particle "
+                         "order at every step) instead of in place with holes removed at each sort")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the timed steps, copy 2 GiB device to device once: the "
+                         "calibration kernel for rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE runs")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this library runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+    from essaitemporaire_lepic_module_amd import config as cfg
+
+    torch.cuda.init()
+    head = "f64" if args.precision == "f64" else "f32"
+    main_run = _step_bench(args, head)
+    print(f"[bench] {head} step: {main_run['ms_per_step']:.2f} ms/step", file=sys.stderr,
+          flush=True)
+    other = None
+    if args.precision == "both":
+        other = _step_bench(args, "f64")
+        print(f"[bench] f64 step: {other['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
     if args.copy_probe:
         a = torch.ones(COPY_PROBE_BYTES // 8, dtype=torch.float64, device="cuda:0")
         torch.empty_like(a).copy_(a)
         torch.cuda.synchronize()
         del a
-    phases = sim.phase_times()
-    fused_ms, fused_launches, fused_units = kernel_phases.get("push_bc_deposit", (0.0, 0, 0))
-    kernel = "k_push_bc_deposit" if args.exact_order else "k_push"
-    achieved = BYTES_PER_PARTICLE_STEP * fused_units / (fused_ms * 1e-3) / 1e9 if fused_ms else 0.0
-    mg_ms, mg_calls, _ = phases.get("poisson", (0.0, 1, 0))
+    case = cfg.cfg2_pic_mcc(n=args.n, np_cell=args.ppc)
+    prec = ("FP32 particles (int32 cell + FP32 offsets), int64 fixed-point deposit, FP64 "
+            "Poisson" if head == "f32" else "FP64 particles and fields")
     line = {
         "metric": "particle-steps/s of the 128^3 x 64 ppc x 2 species PIC-MCC step",
-        "value": particle_steps / wall,
+        "value": main_run["value"],
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": wall * 1e3 / args.steps,
-        "step_ms_median": step_ms[len(step_ms) // 2],
-        "step_ms_p90": step_ms[min(len(step_ms) - 1, (9 * len(step_ms)) // 10)],
+        "ms_per_step": main_run["ms_per_step"],
+        "step_ms_median": main_run["step_ms_median"],
+        "step_ms_p90": main_run["step_ms_p90"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": head,
         "data": "synthetic (device Philox loader, Maxwellian e- 3 eV / Ar+ 0.03 eV)",
         "config": {"workload": "PIC-MCC at scale: 128^3 cells, 64 ppc x 2 species, Bz=0.01 T, "
-                               "Ar 1 Pa, MCC every 10, sort every 20, FP64",
-                   "cells": args.n ** 3, "particles": n_before, "ppc": args.ppc,
-                   "electron_resort_every": 0 if resort is None else resort[0],
+                               f"Ar 1 Pa, MCC every 10, sort every 20, {prec}",
+                   "cells": args.n ** 3, "particles": main_run["particles"], "ppc": args.ppc,
+                   "electron_resort_every": main_run["electron_resort_every"],
                    "parallelism": "1 GPU"},
-        "mg_solve_ms": mg_ms / max(mg_calls, 1),
-        "mg_vcycles_per_solve": sum(mg_cycles) / max(len(mg_cycles), 1),
-        "phase_ms_per_step": {k: v[0] / args.steps for k, v in phases.items()},
+        "mg_solve_ms": main_run["mg_solve_ms"],
+        "mg_vcycles_per_solve": main_run["mg_vcycles_per_solve"],
+        "phase_ms_per_step": main_run["phase_ms_per_step"],
         "phase_timing": "second pass of the same number of steps, every phase between HIP "
                         "events (the timed pass brackets only the fused particle kernel)",
-        "roofline": {"bound": "hbm", "kernel": kernel,
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(kernel, args),
-                     "bytes_per_particle": BYTES_PER_PARTICLE_STEP,
-                     "algorithmic_bytes_per_launch":
-                         BYTES_PER_PARTICLE_STEP * fused_units / max(fused_launches, 1),
-                     "avg_launch_ms": fused_ms / max(fused_launches, 1)},
+        "roofline": main_run["roofline"],
     }
+    if other is not None:
+        other["dtype"] = "f64"
+        line["fp64_step"] = other
+    if args.poisson_n:
+        line["poisson_mixed"] = _poisson_cfg3(args.poisson_n)
+        print("[bench] poisson done", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(case)
     print(json.dumps(line))

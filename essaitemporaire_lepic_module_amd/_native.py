@@ -33,7 +33,8 @@ EXPORTS = (
     "lepic_plane_moments", "lepic_electron_heating", "lepic_dielectric_phi",
     "lepic_pset_to32", "lepic_pset_from32", "lepic_round_field4", "lepic_push32",
     "lepic_deposit32", "lepic_fixed_density", "lepic_sort32", "lepic_collisions32",
-    "lepic_poisson_solve_mixed",
+    "lepic_poisson_solve_mixed", "lepic_adj_push", "lepic_adj_efield", "lepic_adj_poisson_rhs",
+    "lepic_adj_bias", "lepic_adj_deposit", "lepic_plane_flux",
 )
 
 
@@ -76,6 +77,12 @@ class PSet32(C.Structure):
     _fields_ = [("cell", C.c_void_p), ("fx", C.c_void_p), ("fy", C.c_void_p),
                 ("fz", C.c_void_p), ("vx", C.c_void_p), ("vy", C.c_void_p),
                 ("vz", C.c_void_p), ("n", C.c_int64), ("cap", C.c_int64)]
+
+
+class AdjSet(C.Structure):
+    _fields_ = [("lx", C.c_void_p), ("ly", C.c_void_p), ("lz", C.c_void_p),
+                ("lvx", C.c_void_p), ("lvy", C.c_void_p), ("lvz", C.c_void_p),
+                ("n", C.c_int64)]
 
 
 class BField32(C.Structure):
@@ -136,6 +143,7 @@ def load() -> C.CDLL:
         L = C.CDLL(LIB_PATH)
         vp, i32p, dp = C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double)
         G, PS, P32 = C.POINTER(Grid), C.POINTER(PSet), C.POINTER(PSet32)
+        AJ = C.POINTER(AdjSet)
         sig = {
             "lepic_create": (C.c_int, [C.c_int32, C.POINTER(vp)]),
             "lepic_destroy": (C.c_int, [vp]),
@@ -201,6 +209,14 @@ def load() -> C.CDLL:
             "lepic_collisions32": (C.c_int, [vp, G, P32, vp, C.POINTER(Chem), C.c_int32,
                                              C.c_double, C.c_uint64, C.c_uint64, vp,
                                              C.POINTER(MCCStats), vp, i32p]),
+            "lepic_adj_push": (C.c_int, [vp, G, PS, AJ, vp, dp, C.c_double, C.c_double,
+                                         C.c_double, vp, vp]),
+            "lepic_adj_efield": (C.c_int, [vp, G, vp, vp, vp]),
+            "lepic_adj_poisson_rhs": (C.c_int, [vp, G, vp, vp, C.c_double, vp]),
+            "lepic_adj_bias": (C.c_int, [vp, G, vp, C.c_int32, vp, vp, C.c_double, vp]),
+            "lepic_adj_deposit": (C.c_int, [vp, G, PS, AJ, vp, vp, C.c_double, C.c_double]),
+            "lepic_plane_flux": (C.c_int, [vp, G, PS, C.c_int32, C.c_double, vp, AJ,
+                                           C.c_double]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

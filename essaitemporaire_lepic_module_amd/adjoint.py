@@ -1,0 +1,307 @@
+"""Differentiation of the PIC step: dL/dtheta of a wall-flux loss with
+respect to an electrode's bias V and the uniform field magnitude B0, by
+hand-written adjoint kernels (lepic_adj_*) run backward over the steps.
+
+The forward is the FP64 step on particles held in place (lost particles
+become holes and keep their slots, so each slot is one particle for the
+whole run): rho from the deposit, the multigrid potential with the biased
+electrode at V, E, the Boris push with B = (0, 0, B0) and the E gather, the
+wall test, the fused deposit for the next step (mod_simulation.f90:340-472
+without sorting or collisions).  The loss is the time-averaged flux of one
+species (Ar+) through the node plane one cell in front of the biased wall,
+
+    L = (1/K) sum_k q Nm / (Ly Lz h1) sum_p vx_p hat(x_p / h1 - (n1 - 1)),
+
+a smooth stand-in for the charge the wall collects per unit time and area
+(apply_particle_bc_legacy's p_mac, mod_particleBC.f90:195-206, counts whole
+particles and has zero derivative almost everywhere).  Which particles the
+walls absorb, and at which step, is frozen in the backward: the gradient is
+the derivative of the run with that event sequence held fixed.
+
+Memory: particle states are kept only at checkpoints every `checkpoint`
+steps (positions, velocities, hole flags, the deposited density and the
+potential); the backward recomputes each segment forward from its
+checkpoint, keeping that segment's states, then walks it backward.
+
+PicLoss is the torch.autograd.Function: forward(V, B0) -> L, backward
+-> (dL/dV, dL/dB0)."""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+import torch
+
+from . import _native
+from . import fields as F
+from . import particles as P
+from .config import EPS0, Case, node_shape
+from .device import context, ptr
+
+_STATE = ("x", "y", "z", "vx", "vy", "vz", "dead")
+
+
+def _pset_of(d: dict, n: int) -> _native.PSet:
+    p = _native.PSet()
+    for f in ("x", "y", "z", "vx", "vy", "vz", "dead"):
+        setattr(p, f, d[f].data_ptr())
+    p.cell = d["cell"].data_ptr() if "cell" in d else None
+    p.n = n
+    p.cap = n
+    return p
+
+
+class AdjointSet:
+    """dL/dx and dL/dv per particle slot of one species (device FP64)."""
+
+    def __init__(self, n: int, device):
+        self.n = n
+        self.a = {k: torch.zeros(max(n, 1), dtype=torch.float64, device=device)
+                  for k in ("lx", "ly", "lz", "lvx", "lvy", "lvz")}
+
+    def zero_(self):
+        for v in self.a.values():
+            v.zero_()
+
+    def struct(self) -> _native.AdjSet:
+        s = _native.AdjSet()
+        for k, v in self.a.items():
+            setattr(s, k, v.data_ptr())
+        s.n = self.n
+        return s
+
+
+class DiffPIC:
+    """The differentiable run of a Case on one device.
+
+    case: plasma (-1) and electrode (>= 1) labels only, B uniform along z or
+    absent, no collisions, dielectrics or heating.  parts: the initial FP64
+    particle sets (host arrays dict per species: x y z vx vy vz), loaded into
+    device sets of exactly that many slots."""
+
+    def __init__(self, case: Case, parts, device=None, bias_label: int = 2, flux_species: int = 1,
+                 checkpoint: int = 20, adjoint_eps: float = 1e-12):
+        b = np.asarray(case.bcnd)
+        if np.any((b == 0) | (b == -2)) or case.flag_pbc or case.flag_nmn:
+            raise ValueError("the adjoint E field takes plasma (-1) and electrode labels only")
+        if case.flag_die or case.heating is not None or case.B_map is not None:
+            raise ValueError("the differentiable step has no dielectrics, heating or B map")
+        if case.B_uniform is not None and (case.B_uniform[0] or case.B_uniform[1]):
+            raise ValueError("the differentiable step takes B along z")
+        self.case = case
+        self.dev = torch.device(device if device is not None else "cuda:0")
+        self.ctx = context(self.dev)
+        self.n, self.h = case.n, case.h
+        self.g = _native.grid(self.n, self.h)
+        self.label = int(bias_label)
+        self.flux_species = int(flux_species)
+        self.K = max(1, int(checkpoint))
+        self.adj_eps = float(adjoint_eps)
+        dev = self.dev
+        self.bcnd = torch.as_tensor(b.astype(np.int32), device=dev)
+        self.kq = torch.as_tensor(case.kq(), device=dev)
+        self.bias_mask = self.bcnd == self.label
+        tr = case.tracked
+        self.ntype = len(tr)
+        self.charge = [s.charge for s in tr]
+        self.mass = [s.mass for s in tr]
+        self.Nm = list(case.Nm[: self.ntype])
+        shape = node_shape(self.n)
+        self.shape = shape
+        self.init = []
+        for a in parts:
+            m = len(a["x"])
+            d = {f: torch.as_tensor(np.asarray(a[f], np.float64), device=dev).clone()
+                 for f in ("x", "y", "z", "vx", "vy", "vz")}
+            d["dead"] = torch.zeros(max(m, 4), dtype=torch.int8, device=dev)[:max(m, 4)]
+            d["cell"] = torch.zeros(max(m, 4), dtype=torch.int32, device=dev)
+            self.init.append((m, d))
+        n1, n2, n3 = self.n
+        self.plane = n1                       # node plane x = (n1 - 1) h1
+        area = n2 * self.h[1] * n3 * self.h[2]
+        self.flux_coef = self.charge[self.flux_species] * self.Nm[self.flux_species] / \
+            (area * self.h[0])
+        self.phi = torch.zeros(shape, dtype=torch.float64, device=dev)
+        self.rho = torch.zeros(shape, dtype=torch.float64, device=dev)
+        self.E4 = torch.zeros(shape + (4,), dtype=torch.float64, device=dev)
+        self.np_thread = torch.zeros((self.ntype,) + shape, dtype=torch.float64, device=dev)
+        self.np_red = torch.zeros_like(self.np_thread)
+        self.p_mac = torch.zeros((self.ntype, 2 * (case.ngrid + 1)), dtype=torch.float64,
+                                 device=dev)
+        self.P_loss = torch.zeros(self.ntype, dtype=torch.float64, device=dev)
+        self.lost = torch.zeros(self.ntype, dtype=torch.int64, device=dev)
+        self.acc = torch.zeros(1, dtype=torch.float64, device=dev)
+        self.timers = None
+
+    # -- forward ----------------------------------------------------------
+    def _fresh(self):
+        """Working particle arrays from the initial sets, the initial deposit
+        and phi = electrode voltages."""
+        st = []
+        for m, d in self.init:
+            w = {k: v.clone() for k, v in d.items()}
+            w["dead"].zero_()
+            st.append((m, w))
+        self.np_thread.zero_()
+        for s, (m, w) in enumerate(st):
+            p = _pset_of(w, m)
+            self.ctx.check(self.ctx.lib.lepic_deposit(self.ctx.h, C.byref(self.g), C.byref(p),
+                                                      ptr(self.kq), float(self.Nm[s]),
+                                                      ptr(self.np_thread[s])))
+        return st
+
+    def _set_electrodes(self, phi, V):
+        for lab, v in self.case.electrode_V.items():
+            phi[self.bcnd == lab] = float(V) if lab == self.label else float(v)
+
+    def _fields(self, V):
+        """rho from the deposited density, phi (warm start), E."""
+        c = self.case
+        F.reduce_species_density(self.n, self.bcnd, self.np_thread, self.ntype, self.np_red)
+        F.build_rho_from_np(self.n, self.np_red, self.charge, self.ntype, self.rho)
+        self._set_electrodes(self.phi, V)
+        F.solve_poisson_legacy(self.phi, self.bcnd, self.rho, self.h, self.n, c.ncycl, c.eps,
+                               c.omega, c.ng, 0, 0)
+        F.calc_Efield_modular(self.n, self.h, self.phi, self.E4, self.bcnd)
+
+    def _push(self, st, B0):
+        c = self.case
+        bf = P.bfield(1, (0.0, 0.0, float(B0))) if B0 is not None else P.bfield(0)
+        self.np_thread.zero_()
+        for s, (m, w) in enumerate(st):
+            p = _pset_of(w, m)
+            bp = P.bc_params(self.n, self.h, 0, 0, s + 1, -1, self.Nm[s] * self.charge[s],
+                             self.mass[s], self.Nm[s], c.ngrid)
+            self.ctx.check(self.ctx.lib.lepic_push_bc_deposit_inplace(
+                self.ctx.h, C.byref(self.g), C.byref(p), ptr(self.E4), C.byref(bf),
+                self.charge[s], self.mass[s], c.dt, ptr(self.bcnd), C.byref(bp), ptr(self.kq),
+                ptr(self.np_thread[s]), ptr(self.p_mac[s]), ptr(self.P_loss[s:s + 1]),
+                ptr(self.lost[s:s + 1]), None))
+
+    def _flux(self, st, acc, adj=None, scale=0.0):
+        m, w = st[self.flux_species]
+        p = _pset_of(w, m)
+        self.ctx.check(self.ctx.lib.lepic_plane_flux(
+            self.ctx.h, C.byref(self.g), C.byref(p), int(self.plane), float(self.flux_coef),
+            ptr(acc), C.byref(adj.struct()) if adj is not None else None, float(scale)))
+
+    @staticmethod
+    def _snap(st):
+        return [(m, {k: v.clone() for k, v in w.items()}) for m, w in st]
+
+    def forward(self, V: float, B0, steps: int, keep: bool = True,
+                record_deaths: bool = False) -> float:
+        """The loss of `steps` steps at bias V and field B0 (None: no B);
+        keep: store the checkpoints the backward needs; record_deaths: keep
+        in self.deaths, per species, the step at which each slot was lost
+        (0: never)."""
+        st = self._fresh()
+        self.phi.zero_()
+        self.acc.zero_()
+        self.ckpt = {}
+        if record_deaths:
+            self.deaths = [torch.zeros(max(m, 1), dtype=torch.int64, device=self.dev)
+                           for m, _ in st]
+        for k in range(1, steps + 1):
+            if keep and (k - 1) % self.K == 0:
+                self.ckpt[k] = (self._snap(st), self.np_thread.clone(), self.phi.clone())
+            self._fields(V)
+            self._push(st, B0)
+            self._flux(st, self.acc)
+            if record_deaths:
+                for s, (m, w) in enumerate(st):
+                    d = self.deaths[s][:m]
+                    d.copy_(torch.where((w["dead"][:m] != 0) & (d == 0), k, d))
+        self.steps = steps
+        self.theta = (float(V), None if B0 is None else float(B0))
+        return float(self.acc.item()) / steps
+
+    # -- backward ---------------------------------------------------------
+    def backward(self, grad_out: float = 1.0):
+        """(dL/dV, dL/dB0) of the last forward, times grad_out."""
+        if not getattr(self, "ckpt", None):
+            raise RuntimeError("backward needs a forward with keep=True")
+        V, B0 = self.theta
+        steps = self.steps
+        ctx, g = self.ctx, self.g
+        dev = self.dev
+        adj = [AdjointSet(m, dev) for m, _ in self.init]
+        lamE = torch.zeros(self.shape + (4,), dtype=torch.float64, device=dev)
+        lamphi = torch.zeros(self.shape, dtype=torch.float64, device=dev)
+        rhs = torch.zeros(self.shape, dtype=torch.float64, device=dev)
+        mu = torch.zeros(self.shape, dtype=torch.float64, device=dev)   # warm start across steps
+        lamB = torch.zeros(3, dtype=torch.float64, device=dev)
+        lamV = torch.zeros(1, dtype=torch.float64, device=dev)
+        plasma = (self.bcnd <= 0)
+        B = None if B0 is None else (C.c_double * 3)(0.0, 0.0, B0)
+        c = self.case
+        scale = grad_out / steps
+        starts = sorted(self.ckpt)
+        for s0 in reversed(starts):
+            s1 = min(s0 + self.K - 1, steps)
+            st_c, np_c, phi_c = self.ckpt[s0]
+            st = self._snap(st_c)
+            self.np_thread.copy_(np_c)
+            self.phi.copy_(phi_c)
+            states, Es = [], []
+            for k in range(s0, s1 + 1):
+                states.append(self._snap(st))
+                self._fields(V)
+                Es.append(self.E4.clone())
+                self._push(st, B0)
+            states.append(self._snap(st))          # after the segment's last push
+            for j in range(s1 - s0, -1, -1):
+                after, before = states[j + 1], states[j]
+                # loss term of this step (state after the push)
+                self._flux(after, None, adj[self.flux_species], scale)
+                lamE.zero_()
+                for s, (m, w) in enumerate(before):
+                    p = _pset_of(w, m)
+                    a = adj[s].struct()
+                    ctx.check(ctx.lib.lepic_adj_push(
+                        ctx.h, C.byref(g), C.byref(p), C.byref(a), ptr(Es[j]), B,
+                        self.charge[s], self.mass[s], c.dt, ptr(lamE), ptr(lamB)))
+                ctx.check(ctx.lib.lepic_adj_efield(ctx.h, C.byref(g), ptr(self.bcnd), ptr(lamE),
+                                                   ptr(lamphi)))
+                # A mu = dL/dphi on plasma nodes, mu = 0 on electrodes, solved
+                # by the forward's multigrid at a right-hand side scaled to O(1)
+                top = float(torch.where(plasma, lamphi.abs(), 0.0).max().item())
+                if top > 0.0:
+                    sc = 1.0 / top
+                    ctx.check(ctx.lib.lepic_adj_poisson_rhs(ctx.h, C.byref(g), ptr(self.bcnd),
+                                                            ptr(lamphi), sc, ptr(rhs)))
+                    mu.mul_(sc)          # the previous step's solution as the first guess
+                    F.solve_poisson_legacy(mu, self.bcnd, rhs, self.h, self.n, c.ncycl,
+                                           self.adj_eps, c.omega, c.ng, 0, 0)
+                    mu.mul_(1.0 / sc)
+                else:
+                    mu.zero_()
+                ctx.check(ctx.lib.lepic_adj_bias(ctx.h, C.byref(g), ptr(self.bcnd), self.label,
+                                                 ptr(lamphi), ptr(mu), EPS0, ptr(lamV)))
+                for s, (m, w) in enumerate(before):
+                    p = _pset_of(w, m)
+                    a = adj[s].struct()
+                    ctx.check(ctx.lib.lepic_adj_deposit(ctx.h, C.byref(g), C.byref(p), C.byref(a),
+                                                        ptr(self.kq), ptr(mu), self.charge[s],
+                                                        self.Nm[s]))
+            del states, Es
+        self.adj = adj
+        return float(lamV.item()), (float(lamB[2].item()) if B0 is not None else 0.0)
+
+
+class PicLoss(torch.autograd.Function):
+    """L(V, B0) of a DiffPIC run as a torch function: the forward runs the
+    steps keeping checkpoints, the backward runs the adjoint kernels."""
+
+    @staticmethod
+    def forward(ctx, V: torch.Tensor, B0: torch.Tensor, sim: DiffPIC, steps: int):
+        L = sim.forward(float(V.item()), float(B0.item()), steps, keep=True)
+        ctx.sim = sim
+        return torch.tensor(L, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, gL):
+        dV, dB = ctx.sim.backward(float(gL.item()))
+        return (torch.tensor(dV, dtype=torch.float64), torch.tensor(dB, dtype=torch.float64),
+                None, None)

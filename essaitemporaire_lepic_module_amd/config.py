@@ -188,6 +188,24 @@ def cfg2_pic_mcc(n: int = 128, np_cell: int = 64, steps: int = 1000, ng: int = 7
     return c
 
 
+def cfg4_differentiable(n: int = 64, np_cell: int = 8, Vb: float = 20.0, B0: float = 0.005,
+                        steps: int = 200, ng: int | None = None) -> Case:
+    """Differentiable PIC: n^3 cells (h = 3.125e-4 m), e- and Ar+ at np_cell
+    particles per cell and species, a grounded box whose x = L face is an
+    electrode of its own (label 2) biased at Vb, uniform Bz = B0 acting on
+    both species, no collisions, dt = 0.1/omega_pe, MG to 1e-10.  The
+    derivatives are taken with respect to Vb and B0."""
+    nn = (n, n, n)
+    h = (3.125e-4,) * 3
+    n0 = 1e15
+    sp = argon_species()[:2]
+    if ng is None:
+        ng = max(2, int(round(math.log2(n))))
+    return Case("differentiable", nn, h, biased_wall_bcnd(nn, 2), 2, {1: 0.0, 2: Vb}, sp, n0,
+                np_cell, 0.1 / omega_pe(n0), steps, eps=1e-10, ng=ng, B_uniform=(0.0, 0.0, B0),
+                nb_step_collisions=0)
+
+
 def cfg3_poisson(n: int = 256, seed: int = 3, h: float = 3.125e-4, nblobs: int = 64):
     """Mixed-precision Poisson problem: an n^3-cell box whose six faces are
     electrodes 1..6 (painted x=0, x=L, y=0, y=L, z=0, z=L in that order, so a

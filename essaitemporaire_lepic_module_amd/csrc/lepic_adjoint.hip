@@ -1,0 +1,522 @@
+// lepic_adjoint.hip -- the adjoint (reverse-mode) kernels of the
+// differentiable step: the Boris push with the E gather, the E field, the
+// charge deposit, the Dirichlet potential coupling and the wall-flux loss.
+//
+// The forward step they differentiate is the FP64 production step on
+// particles held in place (holes instead of compaction, so a particle keeps
+// its slot through the run): deposit of x_k -> rho -> multigrid phi with the
+// electrodes at their voltages -> E = -grad phi (mod_electricField.f90) ->
+// v_{k+1} = Boris(v_k, E(x_k), B) (Src/part_expmover.f90:214-246) ->
+// x_{k+1} = x_k + dt v_{k+1} -> wall test.  Discrete events (which particles
+// a wall absorbs, and when) are frozen: the adjoint is the derivative of the
+// step with that event sequence held fixed.
+//
+// Each kernel is the transpose of its forward operator, element for
+// element: the gather's transpose scatters dL/dE_p with the same trilinear
+// weights (the deposit's form), the deposit's transpose gathers dL/drho with
+// the weights' position derivatives, the E field's transpose is applied as
+// wall-node scatter then plasma-node gather.
+#include <cmath>
+
+#include "lepic_internal.h"
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wunused-function"   // the wall-table kernel is not used here
+#include "lepic_partutil.h"
+#pragma clang diagnostic pop
+
+namespace {
+
+constexpr int BLOCK = 256;
+
+struct KAdj {
+  double *lx, *ly, *lz, *lvx, *lvy, *lvz;
+};
+
+inline KAdj kadj(const lepic_adjoint_set *a) {
+  KAdj k{a->lx, a->ly, a->lz, a->lvx, a->lvy, a->lvz};
+  return k;
+}
+
+// Position derivatives of the trilinear weights, contracted with per-node
+// values e[0..7] (node order w1..w8): returns dL/dpx, dL/dpy, dL/dpz where
+// L = sum_c w_c e_c and px = ix - x/h (so dL/dx = -dL/dpx / h).
+__device__ inline void weight_grads(double px, double py, double pz, const double e[8],
+                                    double &gx, double &gy, double &gz) {
+  const double qx = 1.0 - px, qy = 1.0 - py, qz = 1.0 - pz;
+  gx = py * pz * (e[0] - e[1]) + qy * pz * (e[3] - e[2]) + py * qz * (e[4] - e[5]) +
+       qy * qz * (e[7] - e[6]);
+  gy = px * pz * (e[0] - e[3]) + qx * pz * (e[1] - e[2]) + px * qz * (e[4] - e[7]) +
+       qx * qz * (e[5] - e[6]);
+  gz = px * py * (e[0] - e[4]) + qx * py * (e[1] - e[5]) + qx * qy * (e[2] - e[6]) +
+       px * qy * (e[3] - e[7]);
+}
+
+// The mover's stencil (make_stencil) with its fractions kept.
+__device__ inline void stencil_fracs(const KGrid &g, double xp, double yp, double zp,
+                                     int64_t &base, double &px, double &py, double &pz) {
+  int ix = (int)qdiv(xp, g.q1) + 1;
+  int iy = (int)qdiv(yp, g.q2) + 1;
+  int iz = (int)qdiv(zp, g.q3) + 1;
+  ix = imax(0, imin(g.n1 + 1, ix));
+  iy = imax(0, imin(g.n2 + 1, iy));
+  iz = imax(0, imin(g.n3 + 1, iz));
+  px = qdiv((double)ix * g.h1 - xp, g.q1);
+  py = qdiv((double)iy * g.h2 - yp, g.q2);
+  pz = qdiv((double)iz * g.h3 - zp, g.q3);
+  base = gidx(g, ix, iy, iz);
+}
+
+__device__ inline void node8(const KGrid &g, int64_t b, int64_t nd[8]) {
+  nd[0] = b; nd[1] = b + 1; nd[2] = b + g.sy + 1; nd[3] = b + g.sy;
+  nd[4] = b + g.sz; nd[5] = b + g.sz + 1; nd[6] = b + g.sz + g.sy + 1; nd[7] = b + g.sz + g.sy;
+}
+
+struct KBAdj {
+  int use_b;
+  double B[3];
+  double k2;        // k1 * 2 / (1 + (k1 |B|)^2), formed as the forward forms it
+  double dk2;       // dk2 / d(|B|^2)
+};
+
+// Adjoint of the push of one particle: given dL/dx', dL/dv' of the state
+// after the step (x' = x + dt v', v' = Boris(v, E_p, B)), returns dL/dx and
+// dL/dv of the state before it (the gather's position term included),
+// scatters dL/dE to the node replicas and accumulates dL/dB.
+template <bool REP>
+__global__ __launch_bounds__(BLOCK) void k_adj_push(KGrid g, lepic_pset st, KAdj a, int64_t n,
+                                                    const double4 *__restrict__ E4, KBAdj bf,
+                                                    double k1, double dt, double *lamE,
+                                                    int64_t nodes, double *lamB) {
+  const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+  double lb0 = 0.0, lb1 = 0.0, lb2 = 0.0;
+  if (i < n && st.dead[i] == 0) {
+    const double x = st.x[i], y = st.y[i], z = st.z[i];
+    const double v[3] = {st.vx[i], st.vy[i], st.vz[i]};
+    double lx = a.lx[i], ly = a.ly[i], lz = a.lz[i];
+    double lam[3] = {a.lvx[i] + dt * lx, a.lvy[i] + dt * ly, a.lvz[i] + dt * lz};
+    int64_t b;
+    double px, py, pz;
+    stencil_fracs(g, x, y, z, b, px, py, pz);
+    int64_t nd[8];
+    node8(g, b, nd);
+    const double qx = 1.0 - px, qy = 1.0 - py, qz = 1.0 - pz;
+    const double w[8] = {px * py * pz, qx * py * pz, qx * qy * pz, px * qy * pz,
+                         px * py * qz, qx * py * qz, qx * qy * qz, px * qy * qz};
+    double4 Ec[8];
+    double E[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      Ec[c] = E4[nd[c]];
+      E[0] += w[c] * Ec[c].x;
+      E[1] += w[c] * Ec[c].y;
+      E[2] += w[c] * Ec[c].z;
+    }
+    double lv[3], lE[3];
+    if (bf.use_b) {
+      const double *B = bf.B;
+      const double v1[3] = {v[0] + k1 * E[0], v[1] + k1 * E[1], v[2] + k1 * E[2]};
+      const double v3[3] = {v1[0] + k1 * (v1[1] * B[2] - v1[2] * B[1]),
+                            v1[1] + k1 * (v1[2] * B[0] - v1[0] * B[2]),
+                            v1[2] + k1 * (v1[0] * B[1] - v1[1] * B[0])};
+      // v2 = v1 + k2 (v3 x B), v' = v2 + k1 E
+      const double *l2 = lam;
+      const double l3[3] = {bf.k2 * (B[1] * l2[2] - B[2] * l2[1]),
+                            bf.k2 * (B[2] * l2[0] - B[0] * l2[2]),
+                            bf.k2 * (B[0] * l2[1] - B[1] * l2[0])};
+      const double lk2 = l2[0] * (v3[1] * B[2] - v3[2] * B[1]) +
+                         l2[1] * (v3[2] * B[0] - v3[0] * B[2]) +
+                         l2[2] * (v3[0] * B[1] - v3[1] * B[0]);
+      // dL/dB from v3 x B (through k2's factor) and from k2(|B|^2)
+      lb0 = bf.k2 * (l2[1] * v3[2] - l2[2] * v3[1]) + lk2 * bf.dk2 * 2.0 * B[0];
+      lb1 = bf.k2 * (l2[2] * v3[0] - l2[0] * v3[2]) + lk2 * bf.dk2 * 2.0 * B[1];
+      lb2 = bf.k2 * (l2[0] * v3[1] - l2[1] * v3[0]) + lk2 * bf.dk2 * 2.0 * B[2];
+      // v3 = v1 + k1 (v1 x B)
+      const double l1[3] = {l2[0] + l3[0] + k1 * (B[1] * l3[2] - B[2] * l3[1]),
+                            l2[1] + l3[1] + k1 * (B[2] * l3[0] - B[0] * l3[2]),
+                            l2[2] + l3[2] + k1 * (B[0] * l3[1] - B[1] * l3[0])};
+      lb0 += k1 * (l3[1] * v1[2] - l3[2] * v1[1]);
+      lb1 += k1 * (l3[2] * v1[0] - l3[0] * v1[2]);
+      lb2 += k1 * (l3[0] * v1[1] - l3[1] * v1[0]);
+      for (int d = 0; d < 3; ++d) {
+        lv[d] = l1[d];
+        lE[d] = k1 * lam[d] + k1 * l1[d];
+      }
+    } else {
+      for (int d = 0; d < 3; ++d) {
+        lv[d] = lam[d];
+        lE[d] = 2.0 * k1 * lam[d];
+      }
+    }
+    // gather transpose: dL/dE_c += w_c dL/dE_p; position term of the weights
+    double e[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      e[c] = Ec[c].x * lE[0] + Ec[c].y * lE[1] + Ec[c].z * lE[2];
+      double *dst = lamE + (REP ? (int64_t)xcc_id() * 4 * nodes : 0) + 4 * nd[c];
+      if (REP) {
+        l2_atomic_add(dst + 0, w[c] * lE[0]);
+        l2_atomic_add(dst + 1, w[c] * lE[1]);
+        l2_atomic_add(dst + 2, w[c] * lE[2]);
+      } else {
+        atomicAdd(dst + 0, w[c] * lE[0]);
+        atomicAdd(dst + 1, w[c] * lE[1]);
+        atomicAdd(dst + 2, w[c] * lE[2]);
+      }
+    }
+    double gx, gy, gz;
+    weight_grads(px, py, pz, e, gx, gy, gz);
+    lx -= gx / g.h1;
+    ly -= gy / g.h2;
+    lz -= gz / g.h3;
+    a.lx[i] = lx; a.ly[i] = ly; a.lz[i] = lz;
+    a.lvx[i] = lv[0]; a.lvy[i] = lv[1]; a.lvz[i] = lv[2];
+  }
+  if (bf.use_b) {
+    __shared__ double red[3][BLOCK / LEPIC_WAVE];
+    lb0 = wave_sum_d(lb0); lb1 = wave_sum_d(lb1); lb2 = wave_sum_d(lb2);
+    const int w = threadIdx.x / LEPIC_WAVE;
+    if (__lane_id() == 0) { red[0][w] = lb0; red[1][w] = lb1; red[2][w] = lb2; }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+      double s = 0.0;
+      for (int q = 0; q < BLOCK / LEPIC_WAVE; ++q) s += red[threadIdx.x][q];
+      atomicAdd(&lamB[threadIdx.x], s);
+    }
+  }
+}
+
+// lamE (4 per node) += sum of the per-XCD replicas; replicas cleared.
+__global__ void k_adj_rep_flush(double *lamE, double *rep, int64_t words) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= words) return;
+  double s = lamE[t];
+#pragma unroll
+  for (int r = 0; r < LEPIC_XCDS; ++r) {
+    s += rep[(int64_t)r * words + t];
+    rep[(int64_t)r * words + t] = 0.0;
+  }
+  lamE[t] = s;
+}
+
+__device__ inline bool physical_node(const KGrid &g, int i, int j, int k) {
+  return i >= 1 && i <= g.n1 + 1 && j >= 1 && j <= g.n2 + 1 && k >= 1 && k <= g.n3 + 1;
+}
+
+__device__ inline void ijk_of(const KGrid &g, int64_t c, int &i, int &j, int &k) {
+  i = (int)(c % g.sy);
+  const int64_t r = c / g.sy;
+  j = (int)(r % (g.n2 + 3));
+  k = (int)(r / (g.n2 + 3));
+}
+
+// E-field transpose, part 1: electrode nodes next to plasma.  The forward
+// (mod_electricField.f90:110-152) applies the W, E, S, N, B, T blocks in
+// that order and each sets all three components, so the last applicable
+// block alone defines the node's E: its normal component extrapolated from
+// the pass-1 (central-difference) E of the two nodes on the plasma side,
+// its tangential ones central differences of phi.  dL/dE of the node goes
+// to the pass-1 adjoint lamE1 of those two nodes and to lamphi.
+__global__ void k_adj_e_walls(KGrid g, const int32_t *__restrict__ bcnd,
+                              const double4 *__restrict__ lamE, double *lamE1, double *lamphi,
+                              int64_t nodes) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= nodes) return;
+  int i, j, k;
+  ijk_of(g, c, i, j, k);
+  if (!physical_node(g, i, j, k) || bcnd[c] < 1) return;
+  const int64_t off[6] = {1, -1, g.sy, -g.sy, g.sz, -g.sz};
+  int last = -1;
+  for (int q = 0; q < 6; ++q)
+    if (bcnd[c + off[q]] <= 0) last = q;
+  if (last < 0) return;
+  const double4 l = lamE[c];
+  const double lc[3] = {l.x, l.y, l.z};
+  const double hh[3] = {g.h1, g.h2, g.h3};
+  const int64_t s[3] = {1, g.sy, g.sz};
+  const int ax = last / 2;
+  const int64_t o = off[last];
+  // normal: 2 E1(c + o) - E1(c + 2 o) (E1 outside the array reads as 0)
+  atomicAdd(&lamE1[4 * (c + o) + ax], 2.0 * lc[ax]);
+  if (c + 2 * o >= 0 && c + 2 * o < nodes) atomicAdd(&lamE1[4 * (c + 2 * o) + ax], -lc[ax]);
+  for (int d = 0; d < 3; ++d) {
+    if (d == ax) continue;
+    // E_d = -(phi(c + s) - phi(c - s)) / (2 h)
+    const double f = lc[d] / (2.0 * hh[d]);
+    atomicAdd(&lamphi[c + s[d]], -f);
+    atomicAdd(&lamphi[c - s[d]], f);
+  }
+}
+
+// E-field transpose, part 2: the central differences of pass 1 at plasma
+// nodes (bcnd <= 0, physical), gathered per phi node from its six
+// neighbours; their dL/dE is the final one (pass 2 never writes them) plus
+// what the wall extrapolation read from them.
+__global__ void k_adj_e_interior(KGrid g, const int32_t *__restrict__ bcnd,
+                                 const double4 *__restrict__ lamE,
+                                 const double4 *__restrict__ lamE1, double *lamphi,
+                                 int64_t nodes) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= nodes) return;
+  int i, j, k;
+  ijk_of(g, c, i, j, k);
+  double acc = 0.0;
+  const double hh[3] = {g.h1, g.h2, g.h3};
+  const int64_t s[3] = {1, g.sy, g.sz};
+  const int lo[3] = {i, j, k};
+  const int hi[3] = {g.n1 + 2, g.n2 + 2, g.n3 + 2};
+  for (int d = 0; d < 3; ++d) {
+    for (int sgn = -1; sgn <= 1; sgn += 2) {
+      // neighbour m = c + sgn s: E_d(m) = -(phi(m + s) - phi(m - s)) / (2h)
+      const int idx = lo[d] + sgn;
+      if (idx < 0 || idx > hi[d]) continue;
+      const int64_t m = c + sgn * s[d];
+      int mi, mj, mk;
+      ijk_of(g, m, mi, mj, mk);
+      if (!physical_node(g, mi, mj, mk) || bcnd[m] > 0) continue;
+      const double4 a = lamE[m], b = lamE1[m];
+      const double lm = (d == 0 ? a.x + b.x : d == 1 ? a.y + b.y : a.z + b.z);
+      // c = m - s (sgn = +1): dE/dphi(c) = +1/(2h); c = m + s: -1/(2h)
+      acc += (sgn > 0 ? 1.0 : -1.0) * lm / (2.0 * hh[d]);
+    }
+  }
+  lamphi[c] += acc;
+}
+
+// dL/dV of an electrode at voltage V on every node labelled `label`:
+// sum over those nodes of dL/dphi there minus the coupling of their plasma
+// neighbours, sum_p a_pe mu_p (phi_P = A_PP^-1 (b_P - A_PE phi_E)).  Also
+// returns, in rhs, dL/dphi on plasma nodes and 0 elsewhere (the adjoint
+// solve's right-hand side).
+__global__ void k_adj_bias(KGrid g, const int32_t *__restrict__ bcnd, int label,
+                           const double *__restrict__ lamphi, const double *__restrict__ mu,
+                           double aw, double an, double at, int64_t nodes, double *out) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (c < nodes && bcnd[c] == label) {
+    v = lamphi[c];
+    int i, j, k;
+    ijk_of(g, c, i, j, k);
+    const int64_t s[3] = {1, g.sy, g.sz};
+    const double a[3] = {aw, an, at};
+    const int lo[3] = {i, j, k};
+    const int hi[3] = {g.n1 + 2, g.n2 + 2, g.n3 + 2};
+    for (int d = 0; d < 3; ++d)
+      for (int sgn = -1; sgn <= 1; sgn += 2) {
+        const int idx = lo[d] + sgn;
+        if (idx < 0 || idx > hi[d]) continue;
+        const int64_t m = c + sgn * s[d];
+        int mi, mj, mk;
+        ijk_of(g, m, mi, mj, mk);
+        if (physical_node(g, mi, mj, mk) && bcnd[m] <= 0) v -= a[d] * mu[m];
+      }
+  }
+  __shared__ double red[BLOCK / LEPIC_WAVE];
+  v = wave_sum_d(v);
+  if (__lane_id() == 0) red[threadIdx.x / LEPIC_WAVE] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int q = 0; q < BLOCK / LEPIC_WAVE; ++q) s += red[q];
+    if (s != 0.0) atomicAdd(out, s);
+  }
+}
+
+__global__ void k_adj_rhs(KGrid g, const int32_t *__restrict__ bcnd,
+                          const double *__restrict__ lamphi, double scale, double *rhs,
+                          int64_t nodes) {
+  const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= nodes) return;
+  int i, j, k;
+  ijk_of(g, c, i, j, k);
+  rhs[c] = (physical_node(g, i, j, k) && bcnd[c] <= 0) ? lamphi[c] * scale : 0.0;
+}
+
+// Adjoint of the deposit of one species: dL/dx_p += coef sum_c kq_c lamb_c
+// dw_c/dx (coef = -q Nm / (h1 h2 h3): rho = -sum_s q_s np_s, np = kq k1 w).
+__global__ __launch_bounds__(BLOCK) void k_adj_deposit(KGrid g, lepic_pset st, KAdj a, int64_t n,
+                                                       const double *__restrict__ kq,
+                                                       const double *__restrict__ lamb,
+                                                       double coef) {
+  const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+  if (i >= n || st.dead[i] != 0) return;
+  int ix, iy, iz;
+  double px, py, pz;
+  if (!deposit_fracs(g, st.x[i], st.y[i], st.z[i], ix, iy, iz, px, py, pz)) return;
+  int64_t nd[8];
+  node8(g, gidx(g, ix, iy, iz), nd);
+  double e[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) e[c] = kq[nd[c]] * lamb[nd[c]];
+  double gx, gy, gz;
+  weight_grads(px, py, pz, e, gx, gy, gz);
+  a.lx[i] -= coef * gx / g.h1;
+  a.ly[i] -= coef * gy / g.h2;
+  a.lz[i] -= coef * gz / g.h3;
+}
+
+// The loss term of one step: coef * sum over live particles of vx * S(x),
+// S the cloud-in-cell hat of node plane ip (x = (ip - 1) h1) over h1;
+// with a != NULL also dL/dx, dL/dv += scale * its derivatives.
+__global__ __launch_bounds__(BLOCK) void k_plane_flux(KGrid g, lepic_pset st, int64_t n, int ip,
+                                                      double coef, double *acc, KAdj a,
+                                                      int adj, double scale) {
+  const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+  double v = 0.0;
+  if (i < n && st.dead[i] == 0) {
+    const double u = st.x[i] / g.h1 - (double)(ip - 1);
+    const double au = fabs(u);
+    if (au < 1.0) {
+      const double hat = 1.0 - au;
+      const double vx = st.vx[i];
+      v = coef * vx * hat;
+      if (adj) {
+        a.lvx[i] += scale * coef * hat;
+        a.lx[i] += scale * coef * vx * (u > 0.0 ? -1.0 : 1.0) / g.h1;
+      }
+    }
+  }
+  if (!acc) return;
+  __shared__ double red[BLOCK / LEPIC_WAVE];
+  v = wave_sum_d(v);
+  if (__lane_id() == 0) red[threadIdx.x / LEPIC_WAVE] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int q = 0; q < BLOCK / LEPIC_WAVE; ++q) s += red[q];
+    if (s != 0.0) atomicAdd(acc, s);
+  }
+}
+
+int check_adj(lepic_ctx *ctx, const lepic_pset *p, const lepic_adjoint_set *a, const char *who) {
+  if (!p || !a || p->n < 0 || a->n < p->n)
+    return lepic_fail(ctx, LEPIC_ERR_ARG, std::string(who) + ": adjoint set shorter than the state");
+  if (p->n > 0 && (!p->x || !p->dead || !a->lx || !a->ly || !a->lz || !a->lvx || !a->lvy ||
+                   !a->lvz))
+    return lepic_fail(ctx, LEPIC_ERR_ARG, std::string(who) + ": null array");
+  return LEPIC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lepic_adj_push(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *state,
+                   lepic_adjoint_set *adj, const double *E4, const double *Bunif, double q,
+                   double m, double dt, double *lamE4, double *lamB) {
+  if (!ctx) return LEPIC_ERR_ARG;
+  int st;
+  if ((st = check_grid(ctx, g)) || (st = check_adj(ctx, state, adj, "adj_push"))) return st;
+  if (!E4 || !lamE4 || (Bunif && !lamB))
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "adj_push: null argument");
+  const int64_t n = state->n;
+  if (n == 0) return LEPIC_OK;
+  const int64_t nodes = grid_nodes(g);
+  const double k1 = dt * q / (2.0 * m);
+  KBAdj bf;
+  bf.use_b = Bunif ? 1 : 0;
+  bf.B[0] = Bunif ? Bunif[0] : 0.0;
+  bf.B[1] = Bunif ? Bunif[1] : 0.0;
+  bf.B[2] = Bunif ? Bunif[2] : 0.0;
+  const double B2 = bf.B[0] * bf.B[0] + bf.B[1] * bf.B[1] + bf.B[2] * bf.B[2];
+  const double Bt = std::sqrt(B2);
+  bf.k2 = k1 * (2.0 / (1.0 + (k1 * Bt) * (k1 * Bt)));
+  const double den = 1.0 + k1 * k1 * B2;
+  bf.dk2 = -2.0 * k1 * k1 * k1 / (den * den);
+  double *rep = lepic_replicas(ctx, 4 * nodes);
+  if (!rep) return lepic_fail(ctx, LEPIC_ERR_HIP, "adj_push: replicas");
+  hipLaunchKernelGGL(k_adj_push<true>, dim3(grid_blocks(n, BLOCK)), dim3(BLOCK), 0, ctx->stream,
+                     make_kgrid(g), *state, kadj(adj), n, reinterpret_cast<const double4 *>(E4),
+                     bf, k1, dt, rep, nodes, lamB);
+  LEPIC_LAUNCH_CHECK(ctx, "k_adj_push");
+  hipLaunchKernelGGL(k_adj_rep_flush, dim3(grid_blocks(4 * nodes, 256)), dim3(256), 0,
+                     ctx->stream, lamE4, rep, 4 * nodes);
+  LEPIC_LAUNCH_CHECK(ctx, "k_adj_rep_flush");
+  return LEPIC_OK;
+}
+
+int lepic_adj_efield(lepic_ctx *ctx, const lepic_grid *g, const int32_t *bcnd,
+                     const double *lamE4, double *lamphi) {
+  if (!ctx) return LEPIC_ERR_ARG;
+  int st;
+  if ((st = check_grid(ctx, g))) return st;
+  if (!bcnd || !lamE4 || !lamphi) return lepic_fail(ctx, LEPIC_ERR_ARG, "adj_efield: null argument");
+  const int64_t nodes = grid_nodes(g);
+  double *lamE1 = reinterpret_cast<double *>(lepic_scratch(ctx, sizeof(double) * 4 * nodes, 1));
+  if (!lamE1) return lepic_fail(ctx, LEPIC_ERR_HIP, "adj_efield: scratch");
+  LEPIC_HIP(ctx, hipMemsetAsync(lamE1, 0, sizeof(double) * 4 * nodes, ctx->stream));
+  LEPIC_HIP(ctx, hipMemsetAsync(lamphi, 0, sizeof(double) * nodes, ctx->stream));
+  const KGrid kg = make_kgrid(g);
+  hipLaunchKernelGGL(k_adj_e_walls, dim3(grid_blocks(nodes, 256)), dim3(256), 0, ctx->stream, kg,
+                     bcnd, reinterpret_cast<const double4 *>(lamE4), lamE1, lamphi, nodes);
+  LEPIC_LAUNCH_CHECK(ctx, "k_adj_e_walls");
+  hipLaunchKernelGGL(k_adj_e_interior, dim3(grid_blocks(nodes, 256)), dim3(256), 0, ctx->stream,
+                     kg, bcnd, reinterpret_cast<const double4 *>(lamE4),
+                     reinterpret_cast<const double4 *>(lamE1), lamphi, nodes);
+  LEPIC_LAUNCH_CHECK(ctx, "k_adj_e_interior");
+  return LEPIC_OK;
+}
+
+int lepic_adj_poisson_rhs(lepic_ctx *ctx, const lepic_grid *g, const int32_t *bcnd,
+                          const double *lamphi, double scale, double *rhs) {
+  if (!ctx) return LEPIC_ERR_ARG;
+  int st;
+  if ((st = check_grid(ctx, g))) return st;
+  if (!bcnd || !lamphi || !rhs) return lepic_fail(ctx, LEPIC_ERR_ARG, "adj_poisson_rhs: null argument");
+  const int64_t nodes = grid_nodes(g);
+  hipLaunchKernelGGL(k_adj_rhs, dim3(grid_blocks(nodes, 256)), dim3(256), 0, ctx->stream,
+                     make_kgrid(g), bcnd, lamphi, scale, rhs, nodes);
+  LEPIC_LAUNCH_CHECK(ctx, "k_adj_rhs");
+  return LEPIC_OK;
+}
+
+int lepic_adj_bias(lepic_ctx *ctx, const lepic_grid *g, const int32_t *bcnd, int32_t label,
+                   const double *lamphi, const double *mu, double eps0, double *out) {
+  if (!ctx) return LEPIC_ERR_ARG;
+  int st;
+  if ((st = check_grid(ctx, g))) return st;
+  if (!bcnd || !lamphi || !mu || !out || label < 1)
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "adj_bias: bad argument");
+  const int64_t nodes = grid_nodes(g);
+  const double aw = eps0 / (g->h[0] * g->h[0]), an = eps0 / (g->h[1] * g->h[1]),
+               at = eps0 / (g->h[2] * g->h[2]);
+  hipLaunchKernelGGL(k_adj_bias, dim3(grid_blocks(nodes, BLOCK)), dim3(BLOCK), 0, ctx->stream,
+                     make_kgrid(g), bcnd, label, lamphi, mu, aw, an, at, nodes, out);
+  LEPIC_LAUNCH_CHECK(ctx, "k_adj_bias");
+  return LEPIC_OK;
+}
+
+int lepic_adj_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *state,
+                      lepic_adjoint_set *adj, const double *kq, const double *lamb, double q,
+                      double Nm) {
+  if (!ctx) return LEPIC_ERR_ARG;
+  int st;
+  if ((st = check_grid(ctx, g)) || (st = check_adj(ctx, state, adj, "adj_deposit"))) return st;
+  if (!kq || !lamb) return lepic_fail(ctx, LEPIC_ERR_ARG, "adj_deposit: null argument");
+  if (state->n == 0) return LEPIC_OK;
+  const double coef = -q * (Nm / (g->h[0] * g->h[1] * g->h[2]));
+  hipLaunchKernelGGL(k_adj_deposit, dim3(grid_blocks(state->n, BLOCK)), dim3(BLOCK), 0,
+                     ctx->stream, make_kgrid(g), *state, kadj(adj), state->n, kq, lamb, coef);
+  LEPIC_LAUNCH_CHECK(ctx, "k_adj_deposit");
+  return LEPIC_OK;
+}
+
+int lepic_plane_flux(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *state, int32_t plane,
+                     double coef, double *acc, lepic_adjoint_set *adj, double scale) {
+  if (!ctx) return LEPIC_ERR_ARG;
+  int st;
+  if ((st = check_grid(ctx, g))) return st;
+  if (!state || state->n < 0 || (state->n > 0 && (!state->x || !state->vx || !state->dead)))
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "plane_flux: bad particle set");
+  if (plane < 1 || plane > g->n[0] + 1) return lepic_fail(ctx, LEPIC_ERR_ARG, "plane_flux: plane");
+  if (adj && (st = check_adj(ctx, state, adj, "plane_flux"))) return st;
+  if (!acc && !adj) return LEPIC_OK;
+  if (state->n == 0) return LEPIC_OK;
+  KAdj a = adj ? kadj(adj) : KAdj{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  hipLaunchKernelGGL(k_plane_flux, dim3(grid_blocks(state->n, BLOCK)), dim3(BLOCK), 0,
+                     ctx->stream, make_kgrid(g), *state, state->n, plane, coef, acc, a,
+                     adj ? 1 : 0, scale);
+  LEPIC_LAUNCH_CHECK(ctx, "k_plane_flux");
+  return LEPIC_OK;
+}
+
+}  // extern "C"

@@ -591,7 +591,8 @@ __global__ __launch_bounds__(BLOCK) void k_to32(KGrid g, lepic_pset src, KP32 ds
       }
       if (c[d] < 0) outside = true;
     }
-    if (!outside) cell = c[0] + g.n1 * (c[1] + g.n2 * c[2]);
+    if (outside) f[0] = f[1] = f[2] = 0.0f;   // a hole carries no offsets
+    else cell = c[0] + g.n1 * (c[1] + g.n2 * c[2]);
   }
   if (outside) atomicAdd(ctr, 1ull);
   dst.cell[i] = cell;

@@ -472,6 +472,59 @@ int lepic_collisions32(lepic_ctx *ctx, const lepic_grid *g, lepic_pset32 *parts,
                        uint64_t philox_key, uint64_t philox_counter, double *Pcoll,
                        lepic_mcc_stats *stats, int64_t *dep, const int32_t *frac_bits);
 
+/* ---------------- differentiation: adjoint kernels ---------------- */
+/* The reverse-mode derivative of the FP64 step run in place (holes instead
+ * of compaction, so a particle keeps its slot): deposit -> rho -> multigrid
+ * phi with electrode voltages -> E (mod_electricField.f90) -> Boris push
+ * with the E gather (Src/part_expmover.f90:214-246, uniform B) -> x += dt v
+ * -> wall test.  Which particles the walls absorb, and when, is frozen: the
+ * adjoint is the derivative of the step with that event sequence fixed.
+ * (No reference counterpart: the reference has no derivative of its step.)
+ * An adjoint set holds dL/dx and dL/dv per particle slot (device FP64). */
+typedef struct {
+  double *lx, *ly, *lz, *lvx, *lvy, *lvz;
+  int64_t n;
+} lepic_adjoint_set;
+
+/* Push adjoint of one species: state holds x_k, v_k, dead of the step's
+ * start; adj holds dL/dx_{k+1}, dL/dv_{k+1} on entry and dL/dx_k, dL/dv_k
+ * (push and gather terms) on return.  dL/dE (4 doubles per node, the E4
+ * layout) accumulates the gather's transpose: dL/dE_p spread with the
+ * gather's trilinear weights.  Bunif: the uniform B (3 doubles, host; NULL:
+ * no B, the leapfrog update), whose dL/dB accumulates into lamB (device, 3
+ * doubles). */
+int lepic_adj_push(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *state,
+                   lepic_adjoint_set *adj, const double *E4, const double *Bunif, double q,
+                   double m, double dt, double *lamE4, double *lamB);
+/* lamphi = D^T lamE4, D the E field of calc_Efield_modular
+ * (mod_electricField.f90:9-164) on a grid of plasma (-1) and electrode
+ * (>= 1) labels (no periodic or Neumann labels); lamphi is overwritten. */
+int lepic_adj_efield(lepic_ctx *ctx, const lepic_grid *g, const int32_t *bcnd,
+                     const double *lamE4, double *lamphi);
+/* rhs = scale * lamphi on plasma nodes, 0 elsewhere: the right-hand side of
+ * the adjoint Poisson solve, A mu = rhs with mu = 0 on electrodes, solved by
+ * lepic_poisson_solve (the Dirichlet operator is symmetric). */
+int lepic_adj_poisson_rhs(lepic_ctx *ctx, const lepic_grid *g, const int32_t *bcnd,
+                          const double *lamphi, double scale, double *rhs);
+/* *out (device) += dL/dV of the electrode with this label: sum over its
+ * nodes of lamphi minus sum over their plasma neighbours of a mu, with a the
+ * Laplacian's coefficient eps0/h^2 of that direction. */
+int lepic_adj_bias(lepic_ctx *ctx, const lepic_grid *g, const int32_t *bcnd, int32_t label,
+                   const double *lamphi, const double *mu, double eps0, double *out);
+/* Deposit adjoint of one species of charge q and weight Nm: dL/dx_p +=
+ * -q Nm/(h1 h2 h3) sum_c kq_c lamb_c dw_c/dx (the transpose of
+ * deposit_particle_set_to_np_thread + build_rho_from_np taken as an
+ * operator of the positions), lamb = dL/drho on nodes. */
+int lepic_adj_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *state,
+                      lepic_adjoint_set *adj, const double *kq, const double *lamb, double q,
+                      double Nm);
+/* The wall-flux loss term of one step: *acc (device, may be NULL) +=
+ * coef * sum over live particles of vx * hat(x / h1 - (plane - 1)), the
+ * cloud-in-cell weight of node plane `plane` (1-based); with adj, dL/dx and
+ * dL/dv += scale * the term's derivatives. */
+int lepic_plane_flux(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *state, int32_t plane,
+                     double coef, double *acc, lepic_adjoint_set *adj, double scale);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,331 @@
+"""The adjoint kernels (lepic_adj_*) against torch autograd of the same
+operators written independently in tests/torch_pic.py (FP64, CPU), and the
+whole backward (adjoint.DiffPIC) against the pure-torch 16^3 run and against
+central finite differences of the GPU forward.
+
+Tolerances: single operators 1e-10 relative to the largest entry (FP64
+rounding of differently ordered sums); the Poisson adjoint and whole-run
+gradients 1e-6 (multigrid solves to a residual tolerance, dense solve in
+torch); finite differences as stated in each test (the run is piecewise
+smooth: the cloud-in-cell weights have kinks at cell faces, so a central
+difference carries an error that shrinks with the step)."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from essaitemporaire_lepic_module_amd import _native
+from essaitemporaire_lepic_module_amd import config as cfg
+from tests.torch_pic import TorchPIC
+
+pytestmark = pytest.mark.gpu
+
+QE = cfg.QE
+ME = cfg.ME
+MI = 39.95 * cfg.AMU
+DT = 5.605424054411371e-11
+
+
+def _ctx():
+    from essaitemporaire_lepic_module_amd.device import context
+    return context("cuda:0")
+
+
+def _box(n):
+    h = (3.125e-4,) * 3
+    return tuple(n), h, cfg.biased_wall_bcnd(tuple(n), 2)
+
+
+def _particles(rng, n, h, N, vscale):
+    L = np.array([n[d] * h[d] for d in range(3)])
+    pos = (0.05 + 0.9 * rng.random((N, 3))) * L
+    vel = vscale * rng.standard_normal((N, 3))
+    return pos, vel
+
+
+def _dev_state(pos, vel, dead=None):
+    N = len(pos)
+    d = {k: torch.as_tensor(np.ascontiguousarray(a), device="cuda:0")
+         for k, a in zip(("x", "y", "z", "vx", "vy", "vz"), (*pos.T, *vel.T))}
+    d["dead"] = torch.zeros(N + 4, dtype=torch.int8, device="cuda:0")
+    if dead is not None:
+        d["dead"][:N] = torch.as_tensor(dead)
+    p = _native.PSet()
+    for k in ("x", "y", "z", "vx", "vy", "vz", "dead"):
+        setattr(p, k, d[k].data_ptr())
+    p.cell = None
+    p.n = N
+    p.cap = N
+    return p, d
+
+
+def _adj_set(arrs):
+    t = {k: torch.as_tensor(np.ascontiguousarray(a), device="cuda:0").clone()
+         for k, a in zip(("lx", "ly", "lz", "lvx", "lvy", "lvz"), arrs)}
+    s = _native.AdjSet()
+    for k, v in t.items():
+        setattr(s, k, v.data_ptr())
+    s.n = len(arrs[0])
+    return s, t
+
+
+def _close(a, b, rtol):
+    a, b = np.asarray(a), np.asarray(b)
+    scale = max(np.abs(b).max(), 1e-300)
+    assert np.abs(a - b).max() <= rtol * scale, (np.abs(a - b).max(), scale)
+
+
+@pytest.mark.parametrize("B0", [None, 0.005, 0.05])
+def test_adj_push_matches_autograd(gpu, rng, B0):
+    """Boris + gather + position update of one step: dL/dx, dL/dv, dL/dE and
+    dL/dB0 of L = <lx', x'> + <lv', v'>."""
+    n, h, bc = _box((8, 8, 8))
+    N = 3000
+    q, m = -QE, ME
+    pos, vel = _particles(rng, n, h, N, 1e6)
+    shape = cfg.node_shape(n)
+    E = 1e4 * rng.standard_normal(shape + (3,))
+    E4 = np.zeros(shape + (4,))
+    E4[..., :3] = E
+    lx, lv = rng.standard_normal((N, 3)), rng.standard_normal((N, 3)) * 1e-7
+    ctx = _ctx()
+    g = _native.grid(n, h)
+    p, keep = _dev_state(pos, vel)
+    a, lam = _adj_set((*lx.T, *lv.T))
+    lamE = torch.zeros(shape + (4,), dtype=torch.float64, device=gpu)
+    lamB = torch.zeros(3, dtype=torch.float64, device=gpu)
+    E4d = torch.as_tensor(E4, device=gpu)
+    Bp = None if B0 is None else (C.c_double * 3)(0.0, 0.0, B0)
+    ctx.check(ctx.lib.lepic_adj_push(ctx.h, C.byref(g), C.byref(p), C.byref(a), E4d.data_ptr(), Bp,
+                                     q, m, DT, lamE.data_ptr(), lamB.data_ptr()))
+    torch.cuda.synchronize()
+    model = TorchPIC(n, h, bc, 2, [(q, m, 1.0)], DT)
+    x = torch.tensor(pos, requires_grad=True)
+    v = torch.tensor(vel, requires_grad=True)
+    Et = torch.tensor(E, requires_grad=True)
+    Bt = torch.tensor(0.0 if B0 is None else B0, dtype=torch.float64, requires_grad=True)
+    Ep = model.gather(Et, x)
+    if B0 is None:
+        vn = v + 2.0 * (DT * q / (2.0 * m)) * Ep
+    else:
+        vn = model.boris(v, Ep, Bt, q, m)
+    xn = x + DT * vn
+    L = (torch.tensor(lx) * xn).sum() + (torch.tensor(lv) * vn).sum()
+    gx, gv, gE, gB = torch.autograd.grad(L, [x, v, Et, Bt], allow_unused=True)
+    got_x = np.stack([lam[k].cpu().numpy() for k in ("lx", "ly", "lz")], 1)
+    got_v = np.stack([lam[k].cpu().numpy() for k in ("lvx", "lvy", "lvz")], 1)
+    _close(got_x, gx.numpy(), 1e-10)
+    _close(got_v, gv.numpy(), 1e-10)
+    _close(lamE.cpu().numpy()[..., :3], gE.numpy(), 1e-10)
+    if B0 is not None:
+        assert abs(lamB[2].item() - gB.item()) <= 1e-10 * abs(gB.item())
+
+
+def test_adj_push_skips_holes(gpu, rng):
+    n, h, bc = _box((8, 8, 8))
+    N = 500
+    pos, vel = _particles(rng, n, h, N, 1e5)
+    dead = (np.arange(N) % 3 == 0).astype(np.int8) * 2
+    E4 = torch.as_tensor(np.random.default_rng(1).standard_normal(cfg.node_shape(n) + (4,)),
+                         device=gpu)
+    ctx = _ctx()
+    g = _native.grid(n, h)
+    p, keep = _dev_state(pos, vel, dead)
+    ones = np.ones(N)
+    a, lam = _adj_set((ones, ones, ones, ones, ones, ones))
+    lamE = torch.zeros(cfg.node_shape(n) + (4,), dtype=torch.float64, device=gpu)
+    ctx.check(ctx.lib.lepic_adj_push(ctx.h, C.byref(g), C.byref(p), C.byref(a), E4.data_ptr(), None,
+                                     -QE, ME, DT, lamE.data_ptr(), None))
+    for k in lam:
+        np.testing.assert_array_equal(lam[k].cpu().numpy()[dead != 0], 1.0)
+
+
+def test_adj_efield_is_the_transpose(gpu, rng):
+    """<lamE, D phi> = <D^T lamE, phi> with D = lepic_efield, and D^T lamE
+    equal to torch autograd of the independent E restatement."""
+    from essaitemporaire_lepic_module_amd import fields as F
+    n, h, bc = _box((12, 10, 10))
+    bc[4:6, 4:7, 5:8] = 3               # an electrode block inside, 2+ plasma nodes from walls
+    shape = cfg.node_shape(n)
+    phi = rng.standard_normal(shape)
+    lamE = rng.standard_normal(shape + (4,))
+    ctx = _ctx()
+    g = _native.grid(n, h)
+    bcd = torch.as_tensor(bc, device=gpu)
+    phid = torch.as_tensor(phi, device=gpu)
+    E4 = torch.zeros(shape + (4,), dtype=torch.float64, device=gpu)
+    F.calc_Efield_modular(n, h, phid, E4, bcd)
+    lamEd = torch.as_tensor(lamE, device=gpu)
+    lamphi = torch.zeros(shape, dtype=torch.float64, device=gpu)
+    ctx.check(ctx.lib.lepic_adj_efield(ctx.h, C.byref(g), bcd.data_ptr(), lamEd.data_ptr(),
+                                       lamphi.data_ptr()))
+    lhs = float((lamEd[..., :3] * E4[..., :3]).sum())
+    rhs = float((lamphi * phid).sum())
+    assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + float((lamEd.abs() * E4.abs()).sum()))
+    model = TorchPIC(n, h, bc, 2, [(-QE, ME, 1.0)], DT)
+    pt = torch.tensor(phi.reshape(-1), requires_grad=True)
+    Et = model.efield(pt)
+    _close(E4.cpu().numpy()[..., :3], Et.detach().numpy(), 1e-13)
+    (gphi,) = torch.autograd.grad((torch.tensor(lamE[..., :3]) * Et).sum(), [pt])
+    _close(lamphi.cpu().numpy().reshape(-1), gphi.numpy(), 1e-12)
+
+
+def test_adj_deposit_matches_autograd(gpu, rng):
+    n, h, bc = _box((8, 8, 8))
+    N = 4000
+    q, Nm = QE, 3.8e3
+    pos, vel = _particles(rng, n, h, N, 1e3)
+    shape = cfg.node_shape(n)
+    lamb = rng.standard_normal(shape)
+    kq = cfg.build_kq(bc)
+    ctx = _ctx()
+    g = _native.grid(n, h)
+    p, keep = _dev_state(pos, vel)
+    z = np.zeros(N)
+    a, lam = _adj_set((z, z, z, z, z, z))
+    ctx.check(ctx.lib.lepic_adj_deposit(ctx.h, C.byref(g), C.byref(p), C.byref(a),
+                                        torch.as_tensor(kq, device=gpu).data_ptr(),
+                                        torch.as_tensor(lamb, device=gpu).data_ptr(), q, Nm))
+    model = TorchPIC(n, h, bc, 2, [(q, MI, Nm)], DT)
+    x = torch.tensor(pos, requires_grad=True)
+    rho = -q * model.deposit(x, torch.ones(N, dtype=torch.bool), Nm)
+    (gx,) = torch.autograd.grad((torch.tensor(lamb.reshape(-1)) * rho).sum(), [x])
+    got = np.stack([lam[k].cpu().numpy() for k in ("lx", "ly", "lz")], 1)
+    _close(got, gx.numpy(), 1e-11)
+
+
+def test_plane_flux_value_and_gradient(gpu, rng):
+    n, h, bc = _box((8, 8, 8))
+    N = 20000
+    q, Nm = QE, 3.8e3
+    pos, vel = _particles(rng, n, h, N, 1e3)
+    ctx = _ctx()
+    g = _native.grid(n, h)
+    p, keep = _dev_state(pos, vel)
+    z = np.zeros(N)
+    a, lam = _adj_set((z, z, z, z, z, z))
+    area = n[1] * h[1] * n[2] * h[2]
+    coef = q * Nm / (area * h[0])
+    acc = torch.zeros(1, dtype=torch.float64, device=gpu)
+    ctx.check(ctx.lib.lepic_plane_flux(ctx.h, C.byref(g), C.byref(p), n[0], coef, acc.data_ptr(),
+                                       C.byref(a), 1.0))
+    model = TorchPIC(n, h, bc, 2, [(q, MI, Nm)], DT)
+    x = torch.tensor(pos, requires_grad=True)
+    v = torch.tensor(vel, requires_grad=True)
+    f = model.plane_flux(x, v, torch.ones(N, dtype=torch.bool), q, Nm)
+    gx, gv = torch.autograd.grad(f, [x, v])
+    assert abs(acc.item() - f.item()) <= 1e-12 * abs(f.item())
+    _close(lam["lx"].cpu().numpy(), gx[:, 0].numpy(), 1e-12)
+    _close(lam["lvx"].cpu().numpy(), gv[:, 0].numpy(), 1e-12)
+
+
+def test_poisson_adjoint_matches_dense(gpu, rng):
+    """mu = A^-1 dL/dphi by the forward's multigrid and dL/dV of the biased
+    electrode, against autograd through a dense solve of the same operator."""
+    from essaitemporaire_lepic_module_amd import fields as F
+    n, h, bc = _box((16, 16, 16))
+    shape = cfg.node_shape(n)
+    lamphi = rng.standard_normal(shape)
+    ctx = _ctx()
+    g = _native.grid(n, h)
+    bcd = torch.as_tensor(bc, device=gpu)
+    lp = torch.as_tensor(lamphi, device=gpu)
+    rhs = torch.zeros(shape, dtype=torch.float64, device=gpu)
+    mu = torch.zeros(shape, dtype=torch.float64, device=gpu)
+    sc = 1.0 / float(lp.abs().max())
+    ctx.check(ctx.lib.lepic_adj_poisson_rhs(ctx.h, C.byref(g), bcd.data_ptr(), lp.data_ptr(), sc,
+                                            rhs.data_ptr()))
+    F.solve_poisson_legacy(mu, bcd, rhs, h, n, 20000, 1e-13, 1.4, 4, 0, 0)
+    mu.mul_(1.0 / sc)
+    lamV = torch.zeros(1, dtype=torch.float64, device=gpu)
+    ctx.check(ctx.lib.lepic_adj_bias(ctx.h, C.byref(g), bcd.data_ptr(), 2, lp.data_ptr(),
+                                     mu.data_ptr(), cfg.EPS0, lamV.data_ptr()))
+    model = TorchPIC(n, h, bc, 2, [(QE, MI, 1.0)], DT)
+    b = torch.tensor(rng.standard_normal(int(np.prod(shape))) * 1e-6, requires_grad=True)
+    V = torch.tensor(20.0, requires_grad=True)
+    phi = model.solve(b, V)
+    gb, gV = torch.autograd.grad((torch.tensor(lamphi.reshape(-1)) * phi).sum(), [b, V])
+    plasma = model.plasma.reshape(-1).numpy()
+    _close(mu.cpu().numpy().reshape(-1)[plasma], gb.numpy()[plasma], 1e-8)
+    assert abs(lamV.item() - gV.item()) <= 1e-8 * abs(gV.item())
+
+
+def _initial(rng, case):
+    n, h = case.n, case.h
+    N = case.np_cell * n[0] * n[1] * n[2]
+    L = np.array([n[d] * h[d] for d in range(3)])
+    pos = rng.random((N, 3)) * L
+    parts = []
+    for s in case.tracked:
+        vt = cfg.thermal_speed(s.T_eV, s.mass)
+        vel = rng.standard_normal((N, 3)) * vt / math.sqrt(2.0)
+        parts.append({"x": pos[:, 0], "y": pos[:, 1], "z": pos[:, 2],
+                      "vx": vel[:, 0], "vy": vel[:, 1], "vz": vel[:, 2]})
+    return parts
+
+
+@pytest.mark.parametrize("steps,checkpoint", [(12, 5)])
+def test_gradient_matches_pure_torch_16cube(gpu, steps, checkpoint):
+    """The whole backward at 16^3 x 4 ppc against autograd of the pure-torch
+    run from the same particles, with the wall events the GPU forward saw."""
+    from essaitemporaire_lepic_module_amd.adjoint import DiffPIC
+    rng = np.random.default_rng(4)
+    case = cfg.cfg4_differentiable(n=16, np_cell=4, steps=steps)
+    parts = _initial(rng, case)
+    sim = DiffPIC(case, parts, "cuda:0", checkpoint=checkpoint)
+    L = sim.forward(20.0, 0.005, steps, keep=True, record_deaths=True)
+    dV, dB = sim.backward()
+    sched = [d.cpu()[:len(p["x"])] for d, p in zip(sim.deaths, parts)]
+    sp = [(s.charge, s.mass, nm) for s, nm in zip(case.tracked, case.Nm)]
+    model = TorchPIC(case.n, case.h, case.bcnd, 2, sp, case.dt)
+    xs = [torch.tensor(np.stack([p["x"], p["y"], p["z"]], 1)) for p in parts]
+    vs = [torch.tensor(np.stack([p["vx"], p["vy"], p["vz"]], 1)) for p in parts]
+    V = torch.tensor(20.0, requires_grad=True)
+    B0 = torch.tensor(0.005, dtype=torch.float64, requires_grad=True)
+    Lt, _ = model.run(xs, vs, V, B0, steps, schedule=sched)
+    gV, gB = torch.autograd.grad(Lt, [V, B0])
+    assert abs(L - Lt.item()) <= 1e-8 * abs(Lt.item())
+    assert abs(dV - gV.item()) <= 1e-6 * abs(gV.item()), (dV, gV.item())
+    assert abs(dB - gB.item()) <= 1e-6 * abs(gB.item()), (dB, gB.item())
+
+
+def test_checkpoint_interval_leaves_gradient_unchanged(gpu):
+    from essaitemporaire_lepic_module_amd.adjoint import DiffPIC
+    rng = np.random.default_rng(5)
+    case = cfg.cfg4_differentiable(n=16, np_cell=4, steps=12)
+    parts = _initial(rng, case)
+    out = []
+    for K in (1, 4, 12):
+        sim = DiffPIC(case, parts, "cuda:0", checkpoint=K)
+        sim.forward(20.0, 0.005, 12)
+        out.append(sim.backward())
+    for dV, dB in out[1:]:
+        assert abs(dV - out[0][0]) <= 1e-9 * abs(out[0][0])
+        assert abs(dB - out[0][1]) <= 1e-9 * abs(out[0][1])
+
+
+def test_gradient_matches_central_differences(gpu):
+    """32^3 x 8 ppc, 20 steps: dL/dV against (L(V + h) - L(V - h)) / 2h with
+    h = 1e-3 V, dL/dB0 with h = 1e-4 B0 (the FD runs are the real forward,
+    wall events free)."""
+    from essaitemporaire_lepic_module_amd.adjoint import DiffPIC, PicLoss
+    rng = np.random.default_rng(6)
+    steps = 20
+    case = cfg.cfg4_differentiable(n=32, np_cell=8, steps=steps)
+    parts = _initial(rng, case)
+    sim = DiffPIC(case, parts, "cuda:0", checkpoint=10)
+    V = torch.tensor(20.0, dtype=torch.float64, requires_grad=True)
+    B0 = torch.tensor(0.005, dtype=torch.float64, requires_grad=True)
+    L = PicLoss.apply(V, B0, sim, steps)
+    L.backward()
+    dV, dB = V.grad.item(), B0.grad.item()
+    hV, hB = 20.0 * 1e-3, 0.005 * 1e-4
+    fV = (sim.forward(20.0 + hV, 0.005, steps, keep=False) -
+          sim.forward(20.0 - hV, 0.005, steps, keep=False)) / (2 * hV)
+    fB = (sim.forward(20.0, 0.005 + hB, steps, keep=False) -
+          sim.forward(20.0, 0.005 - hB, steps, keep=False)) / (2 * hB)
+    print(f"dL/dV adjoint {dV:.10e} fd {fV:.10e}; dL/dB0 adjoint {dB:.10e} fd {fB:.10e}")
+    assert abs(dV - fV) <= 1e-4 * abs(fV)
+    assert abs(dB - fB) <= 1e-3 * abs(fB)

@@ -62,7 +62,11 @@ def test_mixed_periodic_and_neumann(gpu, oracle, rng):
     rhs = np.where(bcn == -1, 1e-6 * rng.standard_normal(bcn.shape), 0.0)
     phi_o, phi_g, _, info = _solve_both(oracle, gpu, n, h, bcn, phi0, rhs, 4, 1e-10, flag_nmn=1)
     assert info["res"] < 1e-10
-    assert np.abs(phi_g - phi_o).max() <= 1e-8 * np.abs(phi_o).max()
+    # the ghost plane i = 0 behind the Neumann face is read by no row (the
+    # mirror replaces it) and only collects prolonged corrections, which the
+    # two solvers sum differently: compare the physical nodes
+    phys = (slice(1, n[2] + 2), slice(1, n[1] + 2), slice(1, n[0] + 2))
+    assert np.abs(phi_g[phys] - phi_o[phys]).max() <= 1e-8 * np.abs(phi_o).max()
 
 
 def test_mixed_warm_start_and_graph(gpu, oracle):
