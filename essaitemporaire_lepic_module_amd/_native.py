@@ -34,7 +34,7 @@ EXPORTS = (
     "lepic_pset_to32", "lepic_pset_from32", "lepic_round_field4", "lepic_push32",
     "lepic_deposit32", "lepic_fixed_density", "lepic_sort32", "lepic_collisions32",
     "lepic_poisson_solve_mixed", "lepic_adj_push", "lepic_adj_efield", "lepic_adj_poisson_rhs",
-    "lepic_adj_bias", "lepic_adj_deposit", "lepic_plane_flux",
+    "lepic_adj_bias", "lepic_adj_deposit", "lepic_plane_flux", "lepic_apply_deaths",
 )
 
 
@@ -217,6 +217,7 @@ def load() -> C.CDLL:
             "lepic_adj_deposit": (C.c_int, [vp, G, PS, AJ, vp, vp, C.c_double, C.c_double]),
             "lepic_plane_flux": (C.c_int, [vp, G, PS, C.c_int32, C.c_double, vp, AJ,
                                            C.c_double]),
+            "lepic_apply_deaths": (C.c_int, [vp, PS, vp, C.c_int64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

@@ -190,12 +190,32 @@ class DiffPIC:
     def _snap(st):
         return [(m, {k: v.clone() for k, v in w.items()}) for m, w in st]
 
+    def _push_frozen(self, st, B0, deaths, k):
+        """The push with the wall events of a recorded run: every live slot
+        moves, the slots recorded lost at step k become holes, the rest
+        deposit (the map the adjoint differentiates)."""
+        c = self.case
+        bf = P.bfield(1, (0.0, 0.0, float(B0))) if B0 is not None else P.bfield(0)
+        self.np_thread.zero_()
+        for s, (m, w) in enumerate(st):
+            p = _pset_of(w, m)
+            self.ctx.check(self.ctx.lib.lepic_move_boris(self.ctx.h, C.byref(self.g), C.byref(p),
+                                                         ptr(self.E4), C.byref(bf), self.charge[s],
+                                                         self.mass[s], c.dt))
+            self.ctx.check(self.ctx.lib.lepic_apply_deaths(self.ctx.h, C.byref(p),
+                                                           ptr(deaths[s]), int(k)))
+            self.ctx.check(self.ctx.lib.lepic_deposit(self.ctx.h, C.byref(self.g), C.byref(p),
+                                                      ptr(self.kq), float(self.Nm[s]),
+                                                      ptr(self.np_thread[s])))
+
     def forward(self, V: float, B0, steps: int, keep: bool = True,
-                record_deaths: bool = False) -> float:
+                record_deaths: bool = False, frozen=None) -> float:
         """The loss of `steps` steps at bias V and field B0 (None: no B);
         keep: store the checkpoints the backward needs; record_deaths: keep
         in self.deaths, per species, the step at which each slot was lost
-        (0: never)."""
+        (0: never); frozen: such a record, whose wall events replace the
+        wall test (the run the adjoint differentiates, for checks by finite
+        differences)."""
         st = self._fresh()
         self.phi.zero_()
         self.acc.zero_()
@@ -207,7 +227,10 @@ class DiffPIC:
             if keep and (k - 1) % self.K == 0:
                 self.ckpt[k] = (self._snap(st), self.np_thread.clone(), self.phi.clone())
             self._fields(V)
-            self._push(st, B0)
+            if frozen is not None:
+                self._push_frozen(st, B0, frozen, k)
+            else:
+                self._push(st, B0)
             self._flux(st, self.acc)
             if record_deaths:
                 for s, (m, w) in enumerate(st):

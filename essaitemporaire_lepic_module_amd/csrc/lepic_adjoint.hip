@@ -23,6 +23,7 @@
 #pragma clang diagnostic ignored "-Wunused-function"   // the wall-table kernel is not used here
 #include "lepic_partutil.h"
 #pragma clang diagnostic pop
+#include "lepic_ewall.h"
 
 namespace {
 
@@ -209,41 +210,43 @@ __device__ inline void ijk_of(const KGrid &g, int64_t c, int &i, int &j, int &k)
   k = (int)(r / (g.n2 + 3));
 }
 
-// E-field transpose, part 1: electrode nodes next to plasma.  The forward
-// (mod_electricField.f90:110-152) applies the W, E, S, N, B, T blocks in
-// that order and each sets all three components, so the last applicable
-// block alone defines the node's E: its normal component extrapolated from
-// the pass-1 (central-difference) E of the two nodes on the plasma side,
-// its tangential ones central differences of phi.  dL/dE of the node goes
-// to the pass-1 adjoint lamE1 of those two nodes and to lamphi.
+// E-field transpose, part 1: electrode nodes next to plasma
+// (lepic_ewall.h): the tangential components are central differences of
+// phi, the normal one the chain 2 a0 - (2 a1 - (... - term)) over pre-pass
+// values, so dL/dE of the node goes to lamphi and, with alternating signs,
+// to the pre-pass adjoint lamE1 of the chain's nodes.
 __global__ void k_adj_e_walls(KGrid g, const int32_t *__restrict__ bcnd,
                               const double4 *__restrict__ lamE, double *lamE1, double *lamphi,
                               int64_t nodes) {
   const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (c >= nodes) return;
-  int i, j, k;
-  ijk_of(g, c, i, j, k);
-  if (!physical_node(g, i, j, k) || bcnd[c] < 1) return;
-  const int64_t off[6] = {1, -1, g.sy, -g.sy, g.sz, -g.sz};
-  int last = -1;
-  for (int q = 0; q < 6; ++q)
-    if (bcnd[c + off[q]] <= 0) last = q;
+  if (!ew_physical(g, c) || bcnd[c] < 1) return;
+  const int last = ew_last(g, bcnd, c);
   if (last < 0) return;
+  EChain ch;
+  ew_chain(g, bcnd, c, last, nodes, ch);
   const double4 l = lamE[c];
   const double lc[3] = {l.x, l.y, l.z};
   const double hh[3] = {g.h1, g.h2, g.h3};
   const int64_t s[3] = {1, g.sy, g.sz};
-  const int ax = last / 2;
-  const int64_t o = off[last];
-  // normal: 2 E1(c + o) - E1(c + 2 o) (E1 outside the array reads as 0)
-  atomicAdd(&lamE1[4 * (c + o) + ax], 2.0 * lc[ax]);
-  if (c + 2 * o >= 0 && c + 2 * o < nodes) atomicAdd(&lamE1[4 * (c + 2 * o) + ax], -lc[ax]);
   for (int d = 0; d < 3; ++d) {
-    if (d == ax) continue;
-    // E_d = -(phi(c + s) - phi(c - s)) / (2 h)
-    const double f = lc[d] / (2.0 * hh[d]);
+    if (d == ch.ax) continue;
+    const double f = lc[d] / (2.0 * hh[d]);   // E_d = -(phi(c + s) - phi(c - s)) / (2 h)
     atomicAdd(&lamphi[c + s[d]], -f);
     atomicAdd(&lamphi[c - s[d]], f);
+  }
+  const double ln = lc[ch.ax];
+  double sign = 1.0;
+  for (int j = 0; j < ch.k; ++j) {
+    atomicAdd(&lamE1[4 * ch.a[j] + ch.ax], 2.0 * sign * ln);
+    sign = -sign;
+  }
+  if (ch.term_kind == 1) {
+    atomicAdd(&lamE1[4 * ch.t + ch.ax], sign * ln);
+  } else if (ch.term_kind == 2) {
+    const double f = sign * ln / (2.0 * hh[ch.ax]);
+    atomicAdd(&lamphi[ch.t + s[ch.ax]], -f);
+    atomicAdd(&lamphi[ch.t - s[ch.ax]], f);
   }
 }
 
@@ -387,6 +390,14 @@ __global__ __launch_bounds__(BLOCK) void k_plane_flux(KGrid g, lepic_pset st, in
   }
 }
 
+// Frozen wall events: the slots whose recorded loss step is `step` become
+// holes (dead = 2), whatever their position.
+__global__ void k_apply_deaths(int8_t *dead, const int64_t *__restrict__ deaths, int64_t n,
+                               int64_t step) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n && deaths[i] == step) dead[i] = 2;
+}
+
 int check_adj(lepic_ctx *ctx, const lepic_pset *p, const lepic_adjoint_set *a, const char *who) {
   if (!p || !a || p->n < 0 || a->n < p->n)
     return lepic_fail(ctx, LEPIC_ERR_ARG, std::string(who) + ": adjoint set shorter than the state");
@@ -516,6 +527,17 @@ int lepic_plane_flux(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *stat
                      ctx->stream, make_kgrid(g), *state, state->n, plane, coef, acc, a,
                      adj ? 1 : 0, scale);
   LEPIC_LAUNCH_CHECK(ctx, "k_plane_flux");
+  return LEPIC_OK;
+}
+
+int lepic_apply_deaths(lepic_ctx *ctx, lepic_pset *p, const int64_t *deaths, int64_t step) {
+  if (!ctx) return LEPIC_ERR_ARG;
+  if (!p || p->n < 0 || (p->n > 0 && (!p->dead || !deaths)))
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "apply_deaths: bad argument");
+  if (p->n == 0) return LEPIC_OK;
+  hipLaunchKernelGGL(k_apply_deaths, dim3(grid_blocks(p->n, 256)), dim3(256), 0, ctx->stream,
+                     p->dead, deaths, p->n, step);
+  LEPIC_LAUNCH_CHECK(ctx, "k_apply_deaths");
   return LEPIC_OK;
 }
 

@@ -2,6 +2,7 @@
 // periodic stitching (a7), rho (a7) and the electric field (a9).
 // One thread per node (x fastest), so every pass is a coalesced stream.
 #include "lepic_internal.h"
+#include "lepic_ewall.h"
 
 namespace {
 
@@ -190,34 +191,39 @@ __global__ void k_e_periodic(KGrid g, const double *__restrict__ phi,
 }
 
 // Pass 2b: electrode nodes next to the plasma (mod_electricField.f90:110-152):
-// normal component extrapolated, tangential by central differences; the W,
-// E, S, N, B, T blocks apply in that order so the last one wins.
+// the last applicable block's values (lepic_ewall.h), computed from the
+// pass-1/2a field into `out` so that no thread reads a node another one
+// writes; k_e_walls_store then copies them in.  A chain longer than
+// E_CHAIN_MAX links (electrodes one plasma node apart, 32 in a row) gives
+// NaN.
 __global__ void k_e_walls(KGrid g, const double *__restrict__ phi,
-                          const int32_t *__restrict__ bcnd, double4 *E, int64_t nodes) {
+                          const int32_t *__restrict__ bcnd, const double4 *__restrict__ E,
+                          double4 *out, int64_t nodes) {
   int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (c >= nodes) return;
-  int i, j, k;
-  node_ijk(g, c, i, j, k);
-  if (!physical(g, i, j, k)) return;
-  if (bcnd[c] < 1) return;
-  const int64_t sx = 1, sy = g.sy, sz = g.sz;
-  // second neighbours outside the array read as 0 (the reference indexes past
-  // its bounds there)
-  auto Er = [&](int64_t idx) {
-    return (idx >= 0 && idx < nodes) ? E[idx] : make_double4(0.0, 0.0, 0.0, 0.0);
-  };
-  const double ddx = -(phi[c + sx] - phi[c - sx]) / (2.0 * g.h1);
-  const double ddy = -(phi[c + sy] - phi[c - sy]) / (2.0 * g.h2);
-  const double ddz = -(phi[c + sz] - phi[c - sz]) / (2.0 * g.h3);
-  double4 e = E[c];
-  bool any = false;
-  if (bcnd[c + sx] <= 0) { e.x = 2.0 * E[c + sx].x - Er(c + 2 * sx).x; e.y = ddy; e.z = ddz; any = true; }
-  if (bcnd[c - sx] <= 0) { e.x = 2.0 * E[c - sx].x - Er(c - 2 * sx).x; e.y = ddy; e.z = ddz; any = true; }
-  if (bcnd[c + sy] <= 0) { e.x = ddx; e.y = 2.0 * E[c + sy].y - Er(c + 2 * sy).y; e.z = ddz; any = true; }
-  if (bcnd[c - sy] <= 0) { e.x = ddx; e.y = 2.0 * E[c - sy].y - Er(c - 2 * sy).y; e.z = ddz; any = true; }
-  if (bcnd[c + sz] <= 0) { e.x = ddx; e.y = ddy; e.z = 2.0 * E[c + sz].z - Er(c + 2 * sz).z; any = true; }
-  if (bcnd[c - sz] <= 0) { e.x = ddx; e.y = ddy; e.z = 2.0 * E[c - sz].z - Er(c - 2 * sz).z; any = true; }
-  if (any) E[c] = e;
+  if (!ew_physical(g, c) || bcnd[c] < 1) return;
+  const int last = ew_last(g, bcnd, c);
+  if (last < 0) return;
+  EChain ch;
+  ew_chain(g, bcnd, c, last, nodes, ch);
+  double v;
+  if (ch.term_kind == 0) v = 0.0;
+  else if (ch.term_kind == 1) v = ew_comp(E[ch.t], ch.ax);
+  else v = ew_central(g, phi, ch.t, ch.ax);
+  for (int j = ch.k - 1; j >= 0; --j) v = 2.0 * ew_comp(E[ch.a[j]], ch.ax) - v;
+  if (ch.overflow) v = __builtin_nan("");
+  double e[3];
+  for (int d = 0; d < 3; ++d) e[d] = d == ch.ax ? v : ew_central(g, phi, c, d);
+  out[c] = make_double4(e[0], e[1], e[2], E[c].w);
+}
+
+__global__ void k_e_walls_store(KGrid g, const int32_t *__restrict__ bcnd,
+                                const double4 *__restrict__ out, double4 *E, int64_t nodes) {
+  int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (c >= nodes) return;
+  if (!ew_physical(g, c) || bcnd[c] < 1) return;
+  if (ew_last(g, bcnd, c) < 0) return;
+  E[c] = out[c];
 }
 
 // apply_dielectric_bc_to_phi (mod_state.f90:378-430): on every node of a
@@ -339,7 +345,10 @@ int lepic_efield(lepic_ctx *ctx, const lepic_grid *g, const double *phi, const i
                      phi, bcnd, E, 2);
   hipLaunchKernelGGL(k_e_periodic, dim3(grid_blocks(nxz, BLOCK)), dim3(BLOCK), 0, ctx->stream, kg,
                      phi, bcnd, E, 3);
-  hipLaunchKernelGGL(k_e_walls, gr, dim3(BLOCK), 0, ctx->stream, kg, phi, bcnd, E, nodes);
+  double4 *out = reinterpret_cast<double4 *>(lepic_scratch(ctx, sizeof(double4) * (size_t)nodes, 1));
+  if (!out) return lepic_fail(ctx, LEPIC_ERR_HIP, "efield: scratch");
+  hipLaunchKernelGGL(k_e_walls, gr, dim3(BLOCK), 0, ctx->stream, kg, phi, bcnd, E, out, nodes);
+  hipLaunchKernelGGL(k_e_walls_store, gr, dim3(BLOCK), 0, ctx->stream, kg, bcnd, out, E, nodes);
   LEPIC_LAUNCH_CHECK(ctx, "efield");
   return LEPIC_OK;
 }

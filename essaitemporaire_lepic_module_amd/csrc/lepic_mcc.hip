@@ -131,14 +131,14 @@ struct PA32 {
     const int64_t b = gidx(g, cx + 1, cy + 1, cz + 1);
     const int64_t nd[8] = {b, b + 1, b + g.sy + 1, b + g.sy, b + g.sz, b + g.sz + 1,
                            b + g.sz + g.sy + 1, b + g.sz + g.sy};
-    long long a[8];
-    fixed_split(P.U[t], (double)fx, (double)fy, (double)fz, a);
+    int a[8];
+    fixed_split32((float)P.U[t], fx, fy, fz, a);
     const bool flip = (sign < 0) != (P.neg[t] != 0);
 #pragma unroll
     for (int q = 0; q < 8; ++q)
       if (a[q] != 0)
         atomicAdd(reinterpret_cast<unsigned long long *>(&P.dep[t][nd[q]]),
-                  (unsigned long long)(flip ? -a[q] : a[q]));
+                  (unsigned long long)(long long)(flip ? -a[q] : a[q]));
   }
   __device__ void append(const KGrid &g, const KChem &, int64_t, int bt, int64_t ib, int pt,
                          int64_t ip, double vx, double vy, double vz) const {
@@ -985,9 +985,9 @@ static int collisions_impl(lepic_ctx *ctx, const lepic_grid *g, MccSets sets,
       if (sets.dep) {
         const double k1 = c.Nm[t] / (g->h[0] * g->h[1] * g->h[2]);
         const double unit = std::ldexp(std::fabs(k1), sets.frac_bits[t]);
-        if (k1 != 0.0 && !(unit >= 1.0 && unit <= 17592186044416.0))
+        if (k1 != 0.0 && !(unit >= 1.0 && unit <= LEPIC_FIXED32_MAX))
           return lepic_fail(ctx, LEPIC_ERR_ARG,
-                            "collisions32: |Nm/(h1 h2 h3)| * 2^frac_bits must lie in [1, 2^44]");
+                            "collisions32: |Nm/(h1 h2 h3)| * 2^frac_bits must lie in [1, 2^22]");
         P32.P.dep[t] = reinterpret_cast<long long *>(sets.dep + (int64_t)t * nodes);
         P32.P.U[t] = std::rint(unit);
         P32.P.neg[t] = k1 < 0.0 ? 1 : 0;

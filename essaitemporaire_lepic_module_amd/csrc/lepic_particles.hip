@@ -374,6 +374,10 @@ __device__ inline bool deposit_weights(const KGrid &g, double k1, double x, doub
 template <int NB>
 struct DepSharedT {
   static constexpr int CAP = NB;
+  // extents of a cut box (nodes): x, then y and z
+  static constexpr int CUT_X = 36;
+  static constexpr int CUT_YZ = NB >= 36 * 13 * 13 ? 13 : 10;
+  static_assert(CUT_X * CUT_YZ * CUT_YZ <= NB, "cut box exceeds the box");
   int mn[3], mx[3];
   int x0, y0, z0, bx, by, bz, vol;
   double box[NB];
@@ -406,32 +410,60 @@ __device__ inline bool seg_sum8(int key, double v[8]) {
 }
 
 // Pass 1: the block's cell range from the particles' current positions,
-// widened by one cell each way for motion within the step.
+// widened by one cell below and two above for motion within the step.  When
+// that range exceeds the box, the box is cut to BOXCUT nodes around the
+// block's mean lower node (sm / cnt: the lane's sum of lower nodes and its
+// particle count): a cell-sorted tile spans a row of cells along x and one
+// cell in y and z, and the few particles that drifted beyond the cut box
+// deposit into the replica one by one instead of sending the whole block
+// there.
 template <class DS>
-__device__ inline void box_begin(const KGrid &g, DS &ds, int lo[3], int hi[3]) {
+__device__ inline void box_begin(const KGrid &g, DS &ds, int lo[3], int hi[3], const int sm[3],
+                                 int cnt) {
+  __shared__ int bsum[4];
   if (threadIdx.x == 0) {
     ds.mn[0] = ds.mn[1] = ds.mn[2] = 1 << 30;
     ds.mx[0] = ds.mx[1] = ds.mx[2] = -(1 << 30);
+    bsum[0] = bsum[1] = bsum[2] = bsum[3] = 0;
   }
   __syncthreads();
   int a0 = wave_min(lo[0]), a1 = wave_min(lo[1]), a2 = wave_min(lo[2]);
   int b0 = wave_max(hi[0]), b1 = wave_max(hi[1]), b2 = wave_max(hi[2]);
+  int s0 = sm[0], s1 = sm[1], s2 = sm[2], c = cnt;
+  for (int d = 32; d > 0; d >>= 1) {
+    s0 += __shfl_xor(s0, d); s1 += __shfl_xor(s1, d); s2 += __shfl_xor(s2, d);
+    c += __shfl_xor(c, d);
+  }
   if (__lane_id() == 0) {
     atomicMin(&ds.mn[0], a0); atomicMin(&ds.mn[1], a1); atomicMin(&ds.mn[2], a2);
     atomicMax(&ds.mx[0], b0); atomicMax(&ds.mx[1], b1); atomicMax(&ds.mx[2], b2);
+    atomicAdd(&bsum[0], s0); atomicAdd(&bsum[1], s1); atomicAdd(&bsum[2], s2);
+    atomicAdd(&bsum[3], c);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     int vol = 0;
     if (ds.mx[0] >= ds.mn[0]) {
-      ds.x0 = max(ds.mn[0] - 1, 0);
-      ds.y0 = max(ds.mn[1] - 1, 0);
-      ds.z0 = max(ds.mn[2] - 1, 0);
-      ds.bx = min(ds.mx[0] + 2, g.n1 + 2) - ds.x0 + 1;
-      ds.by = min(ds.mx[1] + 2, g.n2 + 2) - ds.y0 + 1;
-      ds.bz = min(ds.mx[2] + 2, g.n3 + 2) - ds.z0 + 1;
-      vol = ds.bx * ds.by * ds.bz;
-      if (vol > DS::CAP) vol = -1;
+      const int top[3] = {g.n1 + 2, g.n2 + 2, g.n3 + 2};
+      int a[3], e[3];
+      for (int d = 0; d < 3; ++d) {
+        a[d] = max(ds.mn[d] - 1, 0);
+        e[d] = min(ds.mx[d] + 2, top[d]) - a[d] + 1;
+      }
+      if (e[0] * e[1] * e[2] > DS::CAP) {
+        const int lim[3] = {DS::CUT_X, DS::CUT_YZ, DS::CUT_YZ};
+        for (int d = 0; d < 3; ++d) {
+          if (e[d] <= lim[d]) continue;
+          const int mean = bsum[d] / max(bsum[3], 1);
+          int st = mean - lim[d] / 2;
+          st = min(max(st, a[d]), a[d] + e[d] - lim[d]);
+          a[d] = st;
+          e[d] = lim[d];
+        }
+      }
+      ds.x0 = a[0]; ds.y0 = a[1]; ds.z0 = a[2];
+      ds.bx = e[0]; ds.by = e[1]; ds.bz = e[2];
+      vol = e[0] * e[1] * e[2];
     }
     ds.vol = vol;
   }
@@ -713,14 +745,18 @@ __global__ __launch_bounds__(BLOCK) void k_deposit(KGrid g, KPset p, int64_t n, 
     }
   }
   int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+  int sm[3] = {0, 0, 0}, cnt = 0;
 #pragma unroll
   for (int q = 0; q < CPT; ++q)
     if (live[q]) {
       int c[3];
       cell_of(g, x[q], y[q], z[q], c);
-      for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], c[d]); hi[d] = max(hi[d], c[d]); }
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = min(lo[d], c[d]); hi[d] = max(hi[d], c[d]); sm[d] += c[d];
+      }
+      ++cnt;
     }
-  box_begin(g, ds, lo, hi);
+  box_begin(g, ds, lo, hi, sm, cnt);
   RunAcc ra;
   ra.reset();
 #pragma unroll
@@ -808,14 +844,18 @@ __global__ __launch_bounds__(BLOCK) void k_push(
   load_run(p, n, i0, full, x, y, z, vx, vy, vz, live);
   int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
   if (do_dep) {
+    int sm[3] = {0, 0, 0}, cnt = 0;
 #pragma unroll
     for (int q = 0; q < CPT; ++q)
       if (live[q]) {
         int c[3];
         cell_of(g, x[q], y[q], z[q], c);
-        for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], c[d]); hi[d] = max(hi[d], c[d]); }
+        for (int d = 0; d < 3; ++d) {
+          lo[d] = min(lo[d], c[d]); hi[d] = max(hi[d], c[d]); sm[d] += c[d];
+        }
+        ++cnt;
       }
-    box_begin(g, ds, lo, hi);
+    box_begin(g, ds, lo, hi, sm, cnt);
   }
   RunAcc ra;
   ra.reset();
@@ -936,7 +976,7 @@ __global__ __launch_bounds__(BLOCK) void k_push_bc_deposit(
       cell_of(g, xp, yp, zp, lo);
       hi[0] = lo[0]; hi[1] = lo[1]; hi[2] = lo[2];
     }
-    box_begin(g, ds, lo, hi);
+    box_begin(g, ds, lo, hi, lo, keep ? 1 : 0);
     bool has = false;
     int ix = 0, iy = 0, iz = 0;
     double w[8];
@@ -1099,6 +1139,8 @@ __device__ inline void load_pos_run(const KPset &p, int64_t n, int64_t i0, bool 
 // box, 32 KB); stencils outside it go to the replica directly.
 struct DepSharedFixed {
   static constexpr int CAP = LDS_BOX;
+  static constexpr int CUT_X = 36;
+  static constexpr int CUT_YZ = 10;
   int mn[3], mx[3];
   int x0, y0, z0, bx, by, bz, vol;
   unsigned long long box[LDS_BOX];
@@ -1136,14 +1178,18 @@ __global__ __launch_bounds__(BLOCK) void k_deposit_fixed(KGrid g, KPset p, int64
   bool live[CPT];
   load_pos_run(p, n, i0, full, x, y, z, live);
   int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+  int sm[3] = {0, 0, 0}, cnt = 0;
 #pragma unroll
   for (int q = 0; q < CPT; ++q)
     if (live[q]) {
       int c[3];
       cell_of(g, x[q], y[q], z[q], c);
-      for (int d = 0; d < 3; ++d) { lo[d] = min(lo[d], c[d]); hi[d] = max(hi[d], c[d]); }
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = min(lo[d], c[d]); hi[d] = max(hi[d], c[d]); sm[d] += c[d];
+      }
+      ++cnt;
     }
-  box_begin(g, ds, lo, hi);
+  box_begin(g, ds, lo, hi, sm, cnt);
   int cur = -1;
   long long a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll

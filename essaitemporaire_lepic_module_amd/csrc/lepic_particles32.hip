@@ -252,36 +252,70 @@ __device__ inline int wall32(const KG32 &g, const int *__restrict__ wt, const KB
 // replicas for stencils outside it.
 // ---------------------------------------------------------------------------
 struct Box32 {
-  int mn[3], mx[3];
+  int mn[3], mx[3], sum[3], cnt;
   int x0, y0, z0, bx, by, bz, vol;   // vol <= 0: no box (everything to the replicas)
   unsigned long long box[BOX32];
 };
 
-// lo / hi: the lane's lower nodes (1-based) before the move.
-__device__ inline void box32_begin(const KG32 &g, Box32 &ds, const int lo[3], const int hi[3]) {
+// Box extents (nodes) used when the block's full range does not fit BOX32:
+// a cell-sorted tile spans a row of cells along x and one cell in y and z,
+// so the box keeps 36 x 10 x 10 nodes around the tile's mean lower node and
+// the few particles that drifted beyond go to the replicas.
+constexpr int BOXLIM_X = 36, BOXLIM_Y = 10, BOXLIM_Z = 10;
+static_assert(BOXLIM_X * BOXLIM_Y * BOXLIM_Z <= BOX32, "box limits exceed the box");
+
+// lo / hi: the lane's lower nodes (1-based) before the move; sm / cnt: the
+// sum of those nodes over the lane's particles and their number.  The box
+// spans the block's lower nodes widened by one node below and two above
+// (motion within the step); when that exceeds BOX32 nodes, it is cut to
+// BOXLIM around the mean lower node, and particles outside it deposit into
+// the replicas.
+__device__ inline void box32_begin(const KG32 &g, Box32 &ds, const int lo[3], const int hi[3],
+                                   const int sm[3], int cnt) {
   if (threadIdx.x == 0) {
     ds.mn[0] = ds.mn[1] = ds.mn[2] = 1 << 30;
     ds.mx[0] = ds.mx[1] = ds.mx[2] = -(1 << 30);
+    ds.sum[0] = ds.sum[1] = ds.sum[2] = 0;
+    ds.cnt = 0;
   }
   __syncthreads();
   const int a0 = wave_min(lo[0]), a1 = wave_min(lo[1]), a2 = wave_min(lo[2]);
   const int b0 = wave_max(hi[0]), b1 = wave_max(hi[1]), b2 = wave_max(hi[2]);
+  int s0 = sm[0], s1 = sm[1], s2 = sm[2], c = cnt;
+  for (int d = 32; d > 0; d >>= 1) {
+    s0 += __shfl_xor(s0, d); s1 += __shfl_xor(s1, d); s2 += __shfl_xor(s2, d);
+    c += __shfl_xor(c, d);
+  }
   if (__lane_id() == 0) {
     atomicMin(&ds.mn[0], a0); atomicMin(&ds.mn[1], a1); atomicMin(&ds.mn[2], a2);
     atomicMax(&ds.mx[0], b0); atomicMax(&ds.mx[1], b1); atomicMax(&ds.mx[2], b2);
+    atomicAdd(&ds.sum[0], s0); atomicAdd(&ds.sum[1], s1); atomicAdd(&ds.sum[2], s2);
+    atomicAdd(&ds.cnt, c);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     int vol = 0;
     if (ds.mx[0] >= ds.mn[0]) {
-      ds.x0 = max(ds.mn[0] - 1, 0);
-      ds.y0 = max(ds.mn[1] - 1, 0);
-      ds.z0 = max(ds.mn[2] - 1, 0);
-      ds.bx = min(ds.mx[0] + 2, g.n1 + 2) - ds.x0 + 1;
-      ds.by = min(ds.mx[1] + 2, g.n2 + 2) - ds.y0 + 1;
-      ds.bz = min(ds.mx[2] + 2, g.n3 + 2) - ds.z0 + 1;
-      const int64_t v = (int64_t)ds.bx * ds.by * ds.bz;
-      vol = v > BOX32 ? -1 : (int)v;
+      const int top[3] = {g.n1 + 2, g.n2 + 2, g.n3 + 2};
+      const int lim[3] = {BOXLIM_X, BOXLIM_Y, BOXLIM_Z};
+      int a[3], e[3];
+      for (int d = 0; d < 3; ++d) {
+        a[d] = max(ds.mn[d] - 1, 0);
+        e[d] = min(ds.mx[d] + 2, top[d]) - a[d] + 1;
+      }
+      if ((int64_t)e[0] * e[1] * e[2] > BOX32) {
+        for (int d = 0; d < 3; ++d) {
+          if (e[d] <= lim[d]) continue;
+          const int mean = ds.sum[d] / max(ds.cnt, 1);
+          int st = mean - lim[d] / 2;
+          st = min(max(st, a[d]), a[d] + e[d] - lim[d]);
+          a[d] = st;
+          e[d] = lim[d];
+        }
+      }
+      ds.x0 = a[0]; ds.y0 = a[1]; ds.z0 = a[2];
+      ds.bx = e[0]; ds.by = e[1]; ds.bz = e[2];
+      vol = e[0] * e[1] * e[2];
     }
     ds.vol = vol;
   }
@@ -301,7 +335,10 @@ __device__ inline int key32(const KG32 &g, const Box32 &ds, int ix, int iy, int 
   return -(int)node_of(g, ix, iy, iz) - 2;
 }
 
-__device__ inline void flush32(const KG32 &g, Box32 &ds, int key, const long long a[8],
+// A run's 8 node sums (int32: a lane's run holds at most CPT particles of at
+// most 2^22 units) added to the box or, outside it, to this XCD's replica,
+// sign-extended into the int64 node sums.
+__device__ inline void flush32(const KG32 &g, Box32 &ds, int key, const int a[8],
                                unsigned long long *rep) {
   if (key >= 0) {
     const int bx = ds.bx, bxy = ds.bx * ds.by;
@@ -309,10 +346,11 @@ __device__ inline void flush32(const KG32 &g, Box32 &ds, int key, const long lon
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int q = (j + rot) & 7;
-      long long v = a[0];
+      int v = a[0];
 #pragma unroll
       for (int t = 1; t < 8; ++t) v = (q == t) ? a[t] : v;
-      if (v != 0) atomicAdd(&ds.box[key + node_off(q, bx, bxy)], (unsigned long long)v);
+      if (v != 0)
+        atomicAdd(&ds.box[key + node_off(q, bx, bxy)], (unsigned long long)(long long)v);
     }
   } else if (key <= -2) {
     const int64_t b = (int64_t)(-(key + 2));
@@ -321,8 +359,8 @@ __device__ inline void flush32(const KG32 &g, Box32 &ds, int key, const long lon
 #pragma unroll
     for (int q = 0; q < 8; ++q)
       if (a[q] != 0)
-        __hip_atomic_fetch_add(&rep[nd[q]], (unsigned long long)a[q], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&rep[nd[q]], (unsigned long long)(long long)a[q],
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
 }
 
@@ -341,21 +379,21 @@ __device__ inline void box32_flush(const KG32 &g, Box32 &ds, unsigned long long 
 // In-register run of one stencil key.
 struct Run32 {
   int cur;
-  long long a[8];
+  int a[8];
   __device__ void reset() {
     cur = -1;
 #pragma unroll
     for (int q = 0; q < 8; ++q) a[q] = 0;
   }
-  __device__ void add(const KG32 &g, Box32 &ds, int key, double U, int neg, float fx, float fy,
+  __device__ void add(const KG32 &g, Box32 &ds, int key, float U, int neg, float fx, float fy,
                       float fz, unsigned long long *rep) {
     if (key != cur) {
       flush32(g, ds, cur, a, rep);
       reset();
       cur = key;
     }
-    long long parts[8];
-    fixed_split(U, (double)fx, (double)fy, (double)fz, parts);
+    int parts[8];
+    fixed_split32(U, fx, fy, fz, parts);
 #pragma unroll
     for (int q = 0; q < 8; ++q) a[q] += neg ? -parts[q] : parts[q];
   }
@@ -421,7 +459,7 @@ __device__ inline void store32(const KP32 &p, int64_t n, int64_t i0, bool full, 
 template <int BM>
 __global__ __launch_bounds__(BLOCK) void k_push32(
     KG32 g, KP32 p, int64_t n, const float4 *__restrict__ E4, KBF32 bf, float k1,
-    const int *__restrict__ wt, KBC bp, double U, int neg, unsigned long long *rep_base,
+    const int *__restrict__ wt, KBC bp, float U, int neg, unsigned long long *rep_base,
     int64_t nodes, int do_dep, double *p_mac, double *P_loss, unsigned long long *lost_ctr,
     unsigned long long *esc_ctr) {
   __shared__ Box32 ds;
@@ -436,6 +474,7 @@ __global__ __launch_bounds__(BLOCK) void k_push32(
     if (c[q] >= g.ncells) c[q] = -1;
   int ix[CPT], iy[CPT], iz[CPT];
   int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+  int sm[3] = {0, 0, 0}, cnt = 0;
 #pragma unroll
   for (int q = 0; q < CPT; ++q) {
     ix[q] = iy[q] = iz[q] = 0;
@@ -444,9 +483,11 @@ __global__ __launch_bounds__(BLOCK) void k_push32(
       ++ix[q]; ++iy[q]; ++iz[q];
       lo[0] = min(lo[0], ix[q]); lo[1] = min(lo[1], iy[q]); lo[2] = min(lo[2], iz[q]);
       hi[0] = max(hi[0], ix[q]); hi[1] = max(hi[1], iy[q]); hi[2] = max(hi[2], iz[q]);
+      sm[0] += ix[q]; sm[1] += iy[q]; sm[2] += iz[q];
+      ++cnt;
     }
   }
-  if (do_dep) box32_begin(g, ds, lo, hi);
+  if (do_dep) box32_begin(g, ds, lo, hi, sm, cnt);
   Run32 ra;
   ra.reset();
   int64_t eb = -1;
@@ -502,7 +543,7 @@ __global__ __launch_bounds__(BLOCK) void k_push32(
 }
 
 // Fixed-point deposit of a set without moving it.
-__global__ __launch_bounds__(BLOCK) void k_deposit32(KG32 g, KP32 p, int64_t n, double U, int neg,
+__global__ __launch_bounds__(BLOCK) void k_deposit32(KG32 g, KP32 p, int64_t n, float U, int neg,
                                                      unsigned long long *rep_base,
                                                      int64_t nodes) {
   __shared__ Box32 ds;
@@ -517,6 +558,7 @@ __global__ __launch_bounds__(BLOCK) void k_deposit32(KG32 g, KP32 p, int64_t n, 
     if (c[q] >= g.ncells) c[q] = -1;
   int ix[CPT], iy[CPT], iz[CPT];
   int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+  int sm[3] = {0, 0, 0}, cnt = 0;
 #pragma unroll
   for (int q = 0; q < CPT; ++q) {
     ix[q] = iy[q] = iz[q] = 0;
@@ -525,9 +567,11 @@ __global__ __launch_bounds__(BLOCK) void k_deposit32(KG32 g, KP32 p, int64_t n, 
       ++ix[q]; ++iy[q]; ++iz[q];
       lo[0] = min(lo[0], ix[q]); lo[1] = min(lo[1], iy[q]); lo[2] = min(lo[2], iz[q]);
       hi[0] = max(hi[0], ix[q]); hi[1] = max(hi[1], iy[q]); hi[2] = max(hi[2], iz[q]);
+      sm[0] += ix[q]; sm[1] += iy[q]; sm[2] += iz[q];
+      ++cnt;
     }
   }
-  box32_begin(g, ds, lo, hi);
+  box32_begin(g, ds, lo, hi, sm, cnt);
   Run32 ra;
   ra.reset();
 #pragma unroll
@@ -848,8 +892,53 @@ __global__ __launch_bounds__(BLOCK) void k_sort_claim(KG32 g, const int32_t *__r
 }
 
 // Rank: one wave per cell sorts the cell's source indices ascending (they are
-// distinct).  Up to 64: a bitonic sort across the wave's lanes; larger cells:
+// distinct).  Up to 256: a bitonic sort of the cell padded to 64 x R
+// entries, R per lane (position q*64 + lane in register q: partners 64 or
+// more apart in the same lane, nearer ones across lanes); larger cells:
 // each index's rank counted over the cell.
+template <int R>
+__device__ inline void wave_bitonic(int v[R], int lane) {
+#pragma unroll
+  for (int k = 2; k <= 64 * R; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j >= 64) {
+        const int r = j / 64;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          if ((q & r) != 0) continue;
+          const int p = q * 64 + lane;
+          const bool up = (p & k) == 0;
+          const int a = v[q], b = v[q | r];
+          v[q] = up ? min(a, b) : max(a, b);
+          v[q | r] = up ? max(a, b) : min(a, b);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          const int p = q * 64 + lane;
+          const int o = __shfl_xor(v[q], j);
+          const bool up = (p & k) == 0;
+          const bool low = (lane & j) == 0;
+          v[q] = (low == up) ? min(v[q], o) : max(v[q], o);
+        }
+      }
+    }
+  }
+}
+
+template <int R>
+__device__ inline void rank_small(const int32_t *__restrict__ idx, int32_t *sidx, int64_t b, int s,
+                                  int lane) {
+  int v[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) v[q] = q * 64 + lane < s ? idx[b + q * 64 + lane] : 0x7FFFFFFF;
+  wave_bitonic<R>(v, lane);
+#pragma unroll
+  for (int q = 0; q < R; ++q)
+    if (q * 64 + lane < s) sidx[b + q * 64 + lane] = v[q];
+}
+
 __global__ __launch_bounds__(BLOCK) void k_sort_rank(const int32_t *__restrict__ count,
                                                      const int32_t *__restrict__ start,
                                                      int64_t ncells, const int32_t *__restrict__ idx,
@@ -861,22 +950,18 @@ __global__ __launch_bounds__(BLOCK) void k_sort_rank(const int32_t *__restrict__
   const int64_t b = start[c];
   if (s == 0) return;
   if (s <= 64) {
-    int v = lane < s ? idx[b + lane] : 0x7FFFFFFF;
-    for (int k = 2; k <= 64; k <<= 1)
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        const int o = __shfl_xor(v, j);
-        const bool up = (lane & k) == 0;
-        const bool low = (lane & j) == 0;
-        v = (low == up) ? min(v, o) : max(v, o);
-      }
-    if (lane < s) sidx[b + lane] = v;
-    return;
-  }
-  for (int e = lane; e < s; e += 64) {
-    const int v = idx[b + e];
-    int r = 0;
-    for (int j = 0; j < s; ++j) r += idx[b + j] < v ? 1 : 0;
-    sidx[b + r] = v;
+    rank_small<1>(idx, sidx, b, s, lane);
+  } else if (s <= 128) {
+    rank_small<2>(idx, sidx, b, s, lane);
+  } else if (s <= 256) {
+    rank_small<4>(idx, sidx, b, s, lane);
+  } else {
+    for (int e = lane; e < s; e += 64) {
+      const int v = idx[b + e];
+      int r = 0;
+      for (int j = 0; j < s; ++j) r += idx[b + j] < v ? 1 : 0;
+      sidx[b + r] = v;
+    }
   }
 }
 
@@ -926,7 +1011,8 @@ int check_grid32(lepic_ctx *ctx, const lepic_grid *g, const char *who) {
   return LEPIC_OK;
 }
 
-// Full weight in units of 2^-S: rint(|Nm / (h1 h2 h3)| 2^S) within [1, 2^44].
+// Full weight in units of 2^-S: rint(|Nm / (h1 h2 h3)| 2^S) within [1, 2^22]
+// (the FP32 split's exact range).
 int fixed_unit(lepic_ctx *ctx, const lepic_grid *g, double Nm, int32_t frac_bits, double &U,
                int &neg, const char *who) {
   const double k1 = Nm / (g->h[0] * g->h[1] * g->h[2]);
@@ -936,9 +1022,9 @@ int fixed_unit(lepic_ctx *ctx, const lepic_grid *g, double Nm, int32_t frac_bits
     return LEPIC_OK;
   }
   const double unit = std::ldexp(std::fabs(k1), frac_bits);
-  if (frac_bits < -1000 || frac_bits > 1000 || !(unit >= 1.0 && unit <= 17592186044416.0))
+  if (frac_bits < -1000 || frac_bits > 1000 || !(unit >= 1.0 && unit <= LEPIC_FIXED32_MAX))
     return lepic_fail(ctx, LEPIC_ERR_ARG,
-                      std::string(who) + ": |Nm/(h1 h2 h3)| * 2^frac_bits must lie in [1, 2^44]");
+                      std::string(who) + ": |Nm/(h1 h2 h3)| * 2^frac_bits must lie in [1, 2^22]");
   U = std::rint(unit);
   return LEPIC_OK;
 }
@@ -1041,13 +1127,13 @@ int lepic_push32(lepic_ctx *ctx, const lepic_grid *g, lepic_pset32 *p, const flo
   const dim3 grid(grid_blocks(p->n, CTILE)), block(BLOCK);
   switch (bmode) {
     case 0: hipLaunchKernelGGL((k_push32<0>), grid, block, 0, ctx->stream, kg, kp32(p), p->n,
-                               reinterpret_cast<const float4 *>(E4f), kbf, k1, wt, kbc(bp), U, neg,
+                               reinterpret_cast<const float4 *>(E4f), kbf, k1, wt, kbc(bp), (float)U, neg,
                                rep, nodes, dd, p_mac, P_loss, lc, ec); break;
     case 1: hipLaunchKernelGGL((k_push32<1>), grid, block, 0, ctx->stream, kg, kp32(p), p->n,
-                               reinterpret_cast<const float4 *>(E4f), kbf, k1, wt, kbc(bp), U, neg,
+                               reinterpret_cast<const float4 *>(E4f), kbf, k1, wt, kbc(bp), (float)U, neg,
                                rep, nodes, dd, p_mac, P_loss, lc, ec); break;
     default: hipLaunchKernelGGL((k_push32<2>), grid, block, 0, ctx->stream, kg, kp32(p), p->n,
-                                reinterpret_cast<const float4 *>(E4f), kbf, k1, wt, kbc(bp), U, neg,
+                                reinterpret_cast<const float4 *>(E4f), kbf, k1, wt, kbc(bp), (float)U, neg,
                                 rep, nodes, dd, p_mac, P_loss, lc, ec); break;
   }
   LEPIC_LAUNCH_CHECK(ctx, "k_push32");
@@ -1073,7 +1159,7 @@ int lepic_deposit32(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset32 *p, 
   unsigned long long *rep = int_replicas(ctx, nodes);
   if (!rep) return lepic_fail(ctx, LEPIC_ERR_HIP, "deposit32: replicas");
   hipLaunchKernelGGL(k_deposit32, dim3(grid_blocks(p->n, CTILE)), dim3(BLOCK), 0, ctx->stream,
-                     make_kg32(g, 0.0), kp32(p), p->n, U, neg, rep, nodes);
+                     make_kg32(g, 0.0), kp32(p), p->n, (float)U, neg, rep, nodes);
   LEPIC_LAUNCH_CHECK(ctx, "k_deposit32");
   hipLaunchKernelGGL(k_rep_flush_i64, dim3(grid_blocks(nodes, 256)), dim3(256), 0, ctx->stream,
                      reinterpret_cast<long long *>(dep), rep, nodes);

@@ -366,7 +366,7 @@ class Simulation32(Simulation):
         n, h = self.n, self.h
         shape = node_shape(n)
         dev = self.dev
-        self.frac_bits = [F.fixed_point_bits(n, h, nm) for nm in self.Nm]
+        self.frac_bits = [F.fixed_point_bits32(n, h, nm) for nm in self.Nm]
         parts32 = []
         for p in self.parts:
             q, out = P32.ParticleSet32.from_pset64(p, n, h, cap=max(p.nmax, 16))

@@ -525,6 +525,13 @@ int lepic_adj_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *sta
 int lepic_plane_flux(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *state, int32_t plane,
                      double coef, double *acc, lepic_adjoint_set *adj, double scale);
 
+/* Frozen wall events for replays of a differentiable run: the slots whose
+ * recorded loss step (deaths, device int64 per slot, 0: never) equals `step`
+ * become holes (dead = 2).  With lepic_move_boris and lepic_deposit in place
+ * of the fused push it evaluates the run whose derivative the adjoint
+ * computes. */
+int lepic_apply_deaths(lepic_ctx *ctx, lepic_pset *p, const int64_t *deaths, int64_t step);
+
 #ifdef __cplusplus
 }
 #endif

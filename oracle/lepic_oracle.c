@@ -1893,17 +1893,34 @@ static int advance32(float *f, float d) {
 
 /* Fixed-point parts of a particle at lower node (ix,iy,iz), offsets f, added
  * (sign +1) or removed (-1) from dep. */
+/* The FP32 phase's split of a full weight U (an integer of at most 2^22
+ * units) over the 8 stencil nodes, in FP32: along x, then y, then z the
+ * upper node takes rint(parent * offset) and the lower one the rest; every
+ * product stays below 2^22 and every difference is of integers below 2^23,
+ * so the parts are exact integers summing to U. */
+void lo32_fixed_split(float U, float ux, float uy, float uz, int32_t a[8]) {
+  const float x1 = rintf(U * ux), x0 = U - x1;
+  const float x0y1 = rintf(x0 * uy), x0y0 = x0 - x0y1;
+  const float x1y1 = rintf(x1 * uy), x1y0 = x1 - x1y1;
+  const float c[4] = {x0y0, x1y0, x1y1, x0y1};
+  for (int q = 0; q < 4; ++q) {
+    const float z1 = rintf(c[q] * uz);
+    a[q] = (int32_t)(c[q] - z1);
+    a[q + 4] = (int32_t)z1;
+  }
+}
+
 static void dep32(const lo_grid *g, int64_t *dep, double U, int neg, int ix, int iy, int iz,
                   float fx, float fy, float fz, int sign) {
-  int64_t a[8];
-  lo_fixed_split(U, (double)fx, (double)fy, (double)fz, a);
+  int32_t a[8];
+  lo32_fixed_split((float)U, fx, fy, fz, a);
   const int64_t c[8] = {IDX(g, ix, iy, iz),         IDX(g, ix + 1, iy, iz),
                         IDX(g, ix + 1, iy + 1, iz), IDX(g, ix, iy + 1, iz),
                         IDX(g, ix, iy, iz + 1),     IDX(g, ix + 1, iy, iz + 1),
                         IDX(g, ix + 1, iy + 1, iz + 1), IDX(g, ix, iy + 1, iz + 1)};
   const int flip = (sign < 0) != (neg != 0);
   for (int q = 0; q < 8; ++q)
-    dep[c[q]] = (int64_t)((uint64_t)dep[c[q]] + (uint64_t)(flip ? -a[q] : a[q]));
+    dep[c[q]] = (int64_t)((uint64_t)dep[c[q]] + (uint64_t)(int64_t)(flip ? -a[q] : a[q]));
 }
 
 static double unit32(const lo_grid *g, double Nm, int32_t frac_bits, int *neg) {

@@ -252,6 +252,7 @@ typedef struct {
   int64_t n, cap;
 } lo_pset32;
 
+void lo32_fixed_split(float U, float ux, float uy, float uz, int32_t a[8]);
 void lo32_round4(const double *F4, int64_t count, float *F4f);
 int64_t lo32_from64(const lo_grid *g, const lo_pset *src, lo_pset32 *dst);
 void lo32_to64(const lo_grid *g, const lo_pset32 *src, lo_pset *dst);

@@ -50,3 +50,14 @@ def two_electrode_bcnd(n):
     cx, cy = n[0] // 2 + 1, n[1] // 2 + 1
     b[2:n[2] // 2 + 1, cy, cx] = 3
     return b
+
+
+def thin_gap_bcnd(n):
+    """Biased box with electrode plates one plasma node apart along z and
+    y (k = 1 wall, plates at k = 3 and 5; a y plate at j = 3): the E wall
+    pass reads electrode nodes its loop has already visited."""
+    b = cfg.biased_wall_bcnd(n, 2)
+    b[3, 3:n[1] - 1, 3:n[0]] = 3
+    b[5, 4:n[1] - 2, 4:n[0] - 1] = 4
+    b[6:n[2], 3, 2:n[0]] = 5
+    return b

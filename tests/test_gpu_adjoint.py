@@ -142,12 +142,19 @@ def test_adj_push_skips_holes(gpu, rng):
         np.testing.assert_array_equal(lam[k].cpu().numpy()[dead != 0], 1.0)
 
 
-def test_adj_efield_is_the_transpose(gpu, rng):
+@pytest.mark.parametrize("geometry", ["block", "thin_gaps"])
+def test_adj_efield_is_the_transpose(gpu, rng, geometry):
     """<lamE, D phi> = <D^T lamE, phi> with D = lepic_efield, and D^T lamE
-    equal to torch autograd of the independent E restatement."""
+    equal to torch autograd of the independent E restatement; with an
+    electrode block inside, and with plates one plasma node apart (chains of
+    wall-pass reads)."""
     from essaitemporaire_lepic_module_amd import fields as F
+    from tests import helpers as H
     n, h, bc = _box((12, 10, 10))
-    bc[4:6, 4:7, 5:8] = 3               # an electrode block inside, 2+ plasma nodes from walls
+    if geometry == "block":
+        bc[4:6, 4:7, 5:8] = 3
+    else:
+        bc = H.thin_gap_bcnd(n)
     shape = cfg.node_shape(n)
     phi = rng.standard_normal(shape)
     lamE = rng.standard_normal(shape + (4,))
@@ -185,9 +192,11 @@ def test_adj_deposit_matches_autograd(gpu, rng):
     p, keep = _dev_state(pos, vel)
     z = np.zeros(N)
     a, lam = _adj_set((z, z, z, z, z, z))
+    kqd = torch.as_tensor(kq, device=gpu)
+    lambd = torch.as_tensor(lamb, device=gpu)   # both held while the kernel runs
     ctx.check(ctx.lib.lepic_adj_deposit(ctx.h, C.byref(g), C.byref(p), C.byref(a),
-                                        torch.as_tensor(kq, device=gpu).data_ptr(),
-                                        torch.as_tensor(lamb, device=gpu).data_ptr(), q, Nm))
+                                        kqd.data_ptr(), lambd.data_ptr(), q, Nm))
+    torch.cuda.synchronize()
     model = TorchPIC(n, h, bc, 2, [(q, MI, Nm)], DT)
     x = torch.tensor(pos, requires_grad=True)
     rho = -q * model.deposit(x, torch.ones(N, dtype=torch.bool), Nm)
@@ -244,7 +253,7 @@ def test_poisson_adjoint_matches_dense(gpu, rng):
                                      mu.data_ptr(), cfg.EPS0, lamV.data_ptr()))
     model = TorchPIC(n, h, bc, 2, [(QE, MI, 1.0)], DT)
     b = torch.tensor(rng.standard_normal(int(np.prod(shape))) * 1e-6, requires_grad=True)
-    V = torch.tensor(20.0, requires_grad=True)
+    V = torch.tensor(20.0, dtype=torch.float64, requires_grad=True)
     phi = model.solve(b, V)
     gb, gV = torch.autograd.grad((torch.tensor(lamphi.reshape(-1)) * phi).sum(), [b, V])
     plasma = model.plasma.reshape(-1).numpy()
@@ -282,7 +291,7 @@ def test_gradient_matches_pure_torch_16cube(gpu, steps, checkpoint):
     model = TorchPIC(case.n, case.h, case.bcnd, 2, sp, case.dt)
     xs = [torch.tensor(np.stack([p["x"], p["y"], p["z"]], 1)) for p in parts]
     vs = [torch.tensor(np.stack([p["vx"], p["vy"], p["vz"]], 1)) for p in parts]
-    V = torch.tensor(20.0, requires_grad=True)
+    V = torch.tensor(20.0, dtype=torch.float64, requires_grad=True)
     B0 = torch.tensor(0.005, dtype=torch.float64, requires_grad=True)
     Lt, _ = model.run(xs, vs, V, B0, steps, schedule=sched)
     gV, gB = torch.autograd.grad(Lt, [V, B0])
@@ -307,9 +316,12 @@ def test_checkpoint_interval_leaves_gradient_unchanged(gpu):
 
 
 def test_gradient_matches_central_differences(gpu):
-    """32^3 x 8 ppc, 20 steps: dL/dV against (L(V + h) - L(V - h)) / 2h with
-    h = 1e-3 V, dL/dB0 with h = 1e-4 B0 (the FD runs are the real forward,
-    wall events free)."""
+    """32^3 x 8 ppc, 20 steps: dL/dV and dL/dB0 against central differences
+    (steps 1e-3 and 1e-4 relative) of the run with the nominal wall events replayed
+    (the map the adjoint differentiates).  Central differences of the free
+    run, whose wall events move with the parameters, are printed beside
+    them: their difference from the adjoint is the events' own share of the
+    derivative, which the frozen-event gradient leaves out."""
     from essaitemporaire_lepic_module_amd.adjoint import DiffPIC, PicLoss
     rng = np.random.default_rng(6)
     steps = 20
@@ -321,11 +333,22 @@ def test_gradient_matches_central_differences(gpu):
     L = PicLoss.apply(V, B0, sim, steps)
     L.backward()
     dV, dB = V.grad.item(), B0.grad.item()
+    sim.forward(20.0, 0.005, steps, keep=False, record_deaths=True)
+    deaths = [d.clone() for d in sim.deaths]
+    # The frozen-event map is piecewise smooth: a particle crossing a cell
+    # plane between the two runs puts a kink of its deposit weights inside
+    # the difference, an error that shrinks with the step (for dL/dB0 it was
+    # 1e-3 relative at a step of 1e-3).
     hV, hB = 20.0 * 1e-3, 0.005 * 1e-4
-    fV = (sim.forward(20.0 + hV, 0.005, steps, keep=False) -
-          sim.forward(20.0 - hV, 0.005, steps, keep=False)) / (2 * hV)
-    fB = (sim.forward(20.0, 0.005 + hB, steps, keep=False) -
-          sim.forward(20.0, 0.005 - hB, steps, keep=False)) / (2 * hB)
-    print(f"dL/dV adjoint {dV:.10e} fd {fV:.10e}; dL/dB0 adjoint {dB:.10e} fd {fB:.10e}")
+
+    def fd(frozen):
+        f = lambda a, b: sim.forward(a, b, steps, keep=False, frozen=frozen)
+        return ((f(20.0 + hV, 0.005) - f(20.0 - hV, 0.005)) / (2 * hV),
+                (f(20.0, 0.005 + hB) - f(20.0, 0.005 - hB)) / (2 * hB))
+
+    fV, fB = fd(deaths)
+    gV, gB = fd(None)
+    print(f"dL/dV adjoint {dV:.10e} frozen fd {fV:.10e} free fd {gV:.10e}")
+    print(f"dL/dB0 adjoint {dB:.10e} frozen fd {fB:.10e} free fd {gB:.10e}")
     assert abs(dV - fV) <= 1e-4 * abs(fV)
-    assert abs(dB - fB) <= 1e-3 * abs(fB)
+    assert abs(dB - fB) <= 3e-4 * abs(fB)

@@ -209,7 +209,8 @@ def test_kq_reduce_rho_bitexact(gpu, oracle, rng):
         _same(rho_g.cpu().numpy(), oracle.build_rho(n, np_red_o, q))
 
 
-@pytest.mark.parametrize("bmaker", [cfg.grounded_box_bcnd, H.two_electrode_bcnd, H.periodic_yz_bcnd])
+@pytest.mark.parametrize("bmaker", [cfg.grounded_box_bcnd, H.two_electrode_bcnd, H.periodic_yz_bcnd,
+                                    H.thin_gap_bcnd])
 def test_efield_bitexact(gpu, oracle, rng, bmaker):
     from essaitemporaire_lepic_module_amd import fields as F
     n, h = H.box((8, 8, 8))

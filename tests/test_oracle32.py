@@ -112,7 +112,7 @@ def test_deposit32_exact_and_close(oracle, rng):
     p32, p64 = _state(oracle, rng, n, h, count)
     Nm = 1526.0
     k1 = Nm / (h[0] * h[1] * h[2])
-    S = 44 - math.ceil(math.log2(k1))
+    S = 22 - math.ceil(math.log2(k1))
     dep = oracle.deposit32(n, h, p32, Nm, S)
     perm = rng.permutation(count)
     a = p32.arrays()
@@ -139,3 +139,27 @@ def test_sort32_stable(oracle, rng):
     order = np.argsort(np.where(keep, a["cell"], 1 << 30), kind="stable")[: keep.sum()]
     for k in oracle.FIELDS32:
         np.testing.assert_array_equal(s[k], a[k][order])
+
+
+def test_fixed_split32_exact(oracle, rng):
+    """The FP32 split of a full weight of at most 2^22 units: 8 integers that
+    sum to U exactly, each within 1.5 units of U times its trilinear weight;
+    checked against exact rational arithmetic of the same rounding steps."""
+    import ctypes as C
+    lib = oracle.lib32()
+    lib.lo32_fixed_split.argtypes = [C.c_float, C.c_float, C.c_float, C.c_float,
+                                     C.POINTER(C.c_int32)]
+    a = (C.c_int32 * 8)()
+    U = np.concatenate([[1.0, 2.0 ** 22, 2.0 ** 22 - 1], rng.integers(1, 2 ** 22, 2000)])
+    u = rng.random((U.size, 3)).astype(np.float32)
+    u[:5] = [[0, 0, 0], [0.5, 0.5, 0.5], [np.float32(1) - np.float32(2 ** -24)] * 3,
+             [0.25, 0.75, 0.125], [0.999, 0.001, 0.5]]
+    for Ui, (ux, uy, uz) in zip(U, u):
+        lib.lo32_fixed_split(float(Ui), float(ux), float(uy), float(uz), a)
+        parts = np.array(a[:], np.int64)
+        assert parts.sum() == int(Ui)
+        wx, wy, wz = 1 - float(ux), 1 - float(uy), 1 - float(uz)
+        w = np.array([wx * wy * wz, (1 - wx) * wy * wz, (1 - wx) * (1 - wy) * wz,
+                      wx * (1 - wy) * wz, wx * wy * (1 - wz), (1 - wx) * wy * (1 - wz),
+                      (1 - wx) * (1 - wy) * (1 - wz), wx * (1 - wy) * (1 - wz)])
+        assert np.all(np.abs(parts - Ui * w) <= 1.5 + 1e-9 * Ui)

@@ -23,8 +23,19 @@ def _setup(n=(12, 10, 10)):
     return n, h, bc
 
 
-def test_efield_equals_oracle(oracle, rng):
-    n, h, bc = _setup()
+def _thin_gaps(n=(12, 10, 10)):
+    """Electrode plates one plasma node apart along z (k = 1 wall, plates at
+    k = 3 and 5): the wall pass's chains of second-neighbour reads."""
+    h = (3.125e-4,) * 3
+    bc = cfg.biased_wall_bcnd(n, 2)
+    bc[3, 3:7, 3:8] = 3
+    bc[5, 4:6, 4:7] = 4
+    return n, h, bc
+
+
+@pytest.mark.parametrize("geometry", ["block", "thin_gaps"])
+def test_efield_equals_oracle(oracle, rng, geometry):
+    n, h, bc = _setup() if geometry == "block" else _thin_gaps()
     phi = rng.standard_normal(bc.shape)
     model = TorchPIC(n, h, bc, 2, [(-QE, cfg.ME, 1.0)], 1e-11)
     E = model.efield(torch.tensor(phi.reshape(-1))).numpy()
@@ -71,7 +82,7 @@ def test_solve_equals_oracle(oracle, rng):
 
 def test_autograd_gradient_matches_frozen_differences():
     """16^3 x 2 ppc, 8 steps: dL/dV and dL/dB0 against central differences
-    with the wall events of the nominal run (h = 1e-3 relative)."""
+    with the wall events of the nominal run (h = 1e-4 relative)."""
     n = (16, 16, 16)
     h = (3.125e-4,) * 3
     bc = cfg.biased_wall_bcnd(n, 2)
@@ -87,16 +98,20 @@ def test_autograd_gradient_matches_frozen_differences():
     vt = [cfg.thermal_speed(3.0, cfg.ME), cfg.thermal_speed(0.03, 39.95 * cfg.AMU)]
     xs = [torch.tensor(pos), torch.tensor(pos)]
     vs = [torch.tensor(rng.standard_normal((N, 3)) * v / math.sqrt(2)) for v in vt]
-    V = torch.tensor(20.0, requires_grad=True)
+    V = torch.tensor(20.0, dtype=torch.float64, requires_grad=True)
     B0 = torch.tensor(0.005, dtype=torch.float64, requires_grad=True)
     L0, sched = model.run(xs, vs, V, B0, 8)
     gV, gB = torch.autograd.grad(L0, [V, B0])
     assert sum(int((s > 0).sum()) for s in sched) > 0     # walls absorbed some
     with torch.no_grad():
         def f(a, b):
-            return model.run(xs, vs, torch.tensor(a), torch.tensor(b), 8, schedule=sched)[0].item()
-        fV = (f(20.0 * (1 + 1e-3), 0.005) - f(20.0 * (1 - 1e-3), 0.005)) / (2 * 20.0 * 1e-3)
-        fB = (f(20.0, 0.005 * (1 + 1e-3)) - f(20.0, 0.005 * (1 - 1e-3))) / (2 * 0.005 * 1e-3)
-    # h = 1e-3 relative: smaller steps lose to cancellation in the loss sum
-    assert abs(gV.item() - fV) <= 1e-4 * abs(fV)
-    assert abs(gB.item() - fB) <= 1e-3 * abs(fB)
+            return model.run(xs, vs, torch.tensor(a, dtype=torch.float64),
+                             torch.tensor(b, dtype=torch.float64), 8, schedule=sched)[0].item()
+        r = 1e-4
+        fV = (f(20.0 * (1 + r), 0.005) - f(20.0 * (1 - r), 0.005)) / (2 * 20.0 * r)
+        fB = (f(20.0, 0.005 * (1 + r)) - f(20.0, 0.005 * (1 - r))) / (2 * 0.005 * r)
+    # The frozen-event map is piecewise smooth (a particle crossing a cell
+    # plane puts a kink in its deposit weights), so the differences' error
+    # shrinks with the step: 2e-4 of dL/dB0 at r = 1e-3, 2e-7 at r = 1e-4.
+    assert abs(gV.item() - fV) <= 1e-5 * abs(fV)
+    assert abs(gB.item() - fB) <= 1e-5 * abs(fB)

@@ -136,7 +136,11 @@ class TorchPIC:
         Ei = torch.stack([cx, cy, cz], -1) * pl[..., None].double()
         E = torch.zeros(sh + (3,), dtype=torch.float64)
         E[inner] = Ei
-        # wall nodes: blocks in W, E, S, N, B, T order, the last applicable wins
+        # wall nodes: blocks in W, E, S, N, B, T order, the last applicable
+        # wins; the reference's loop visits nodes in index order, so the
+        # second-neighbour read E(c + 2o) sees the final value of an electrode
+        # node visited before (o < 0) and the pass-1 value otherwise: iterate
+        # to the fixed point (one more link of such chains settles per pass)
         b = self.bc
         wall = self.phys & (b >= 1)
         flat = E.reshape(-1, 3)
@@ -146,22 +150,31 @@ class TorchPIC:
         N = b.numel()
         allidx = torch.arange(N)
         bflat = b.reshape(-1)
-        res = flat
         wflat = wall.reshape(-1)
         cflat = cen.reshape(-1, 3)
-        for off, comp in ((sx, 0), (-sx, 0), (sy, 1), (-sy, 1), (sz, 2), (-sz, 2)):
-            nb = allidx + off
-            valid = wflat & (nb >= 0) & (nb < N)
-            nbc = torch.clamp(nb, 0, N - 1)
-            cond = valid & (bflat[nbc] <= 0)
-            nb2 = allidx + 2 * off
-            in2 = (nb2 >= 0) & (nb2 < N)
-            e2 = torch.where(in2, flat[torch.clamp(nb2, 0, N - 1), comp], 0.0)
-            normal = 2.0 * flat[nbc, comp] - e2
-            new = cflat.clone()
-            new[:, comp] = normal
-            res = torch.where(cond[:, None], new, res)
-        return res.reshape(sh + (3,))
+        cur = flat
+        for _ in range(64):
+            res = flat
+            for off, comp in ((sx, 0), (-sx, 0), (sy, 1), (-sy, 1), (sz, 2), (-sz, 2)):
+                nb = allidx + off
+                valid = wflat & (nb >= 0) & (nb < N)
+                nbc = torch.clamp(nb, 0, N - 1)
+                cond = valid & (bflat[nbc] <= 0)
+                nb2 = allidx + 2 * off
+                in2 = (nb2 >= 0) & (nb2 < N)
+                nb2c = torch.clamp(nb2, 0, N - 1)
+                seen = in2 & (off < 0) & wflat[nb2c]
+                e2 = torch.where(seen, cur[nb2c, comp], flat[nb2c, comp])
+                e2 = torch.where(in2, e2, 0.0)
+                normal = 2.0 * flat[nbc, comp] - e2
+                new = cflat.clone()
+                new[:, comp] = normal
+                res = torch.where(cond[:, None], new, res)
+            done = bool(torch.equal(res.detach(), cur.detach()))
+            cur = res
+            if done:
+                break
+        return cur.reshape(sh + (3,))
 
     def gather(self, E, x):
         nodes, w = self.stencil(x)
