@@ -28,6 +28,7 @@ EXPORTS = (
     "lepic_move_boris", "lepic_apply_bc", "lepic_build_kq", "lepic_fill", "lepic_deposit",
     "lepic_deposit_fixed",
     "lepic_reduce_density", "lepic_build_rho", "lepic_poisson_solve", "lepic_efield",
+    "lepic_efield_noalloc",
     "lepic_collisions", "lepic_collisions_deposited", "lepic_push_bc_deposit",
     "lepic_push_bc_deposit_inplace", "lepic_sort_compact", "lepic_bin_compact",
     "lepic_plane_moments", "lepic_electron_heating", "lepic_dielectric_phi",
@@ -175,6 +176,7 @@ def load() -> C.CDLL:
             "lepic_poisson_solve_mixed": (C.c_int, [vp, G, vp, vp, vp, C.POINTER(MGParams),
                                                     C.POINTER(MGMixParams), C.POINTER(MGStats)]),
             "lepic_efield": (C.c_int, [vp, G, vp, vp, vp]),
+            "lepic_efield_noalloc": (C.c_int, [vp, G, vp, vp, vp]),
             "lepic_collisions": (C.c_int, [vp, G, PS, vp, C.POINTER(Chem), C.c_int32, C.c_double,
                                            C.c_int32, i32p, C.c_uint64, C.c_uint64, vp,
                                            C.POINTER(MCCStats)]),

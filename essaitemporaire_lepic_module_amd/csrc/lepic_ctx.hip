@@ -86,6 +86,7 @@ int lepic_create(int32_t device, lepic_ctx **out) {
     delete ctx;
     return LEPIC_ERR_HIP;
   }
+  lepic_efield_scratch_setup(device);
   *out = ctx;
   return LEPIC_OK;
 }

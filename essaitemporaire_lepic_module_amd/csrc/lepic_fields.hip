@@ -1,6 +1,9 @@
 // lepic_fields.hip -- node-field kernels: kq, fills, density reduction with
 // periodic stitching (a7), rho (a7) and the electric field (a9).
 // One thread per node (x fastest), so every pass is a coalesced stream.
+#include <mutex>
+#include <string>
+
 #include "lepic_internal.h"
 #include "lepic_ewall.h"
 
@@ -191,14 +194,15 @@ __global__ void k_e_periodic(KGrid g, const double *__restrict__ phi,
 }
 
 // Pass 2b: electrode nodes next to the plasma (mod_electricField.f90:110-152):
-// the last applicable block's values (lepic_ewall.h), computed from the
-// pass-1/2a field into `out` so that no thread reads a node another one
-// writes; k_e_walls_store then copies them in.  A chain longer than
-// E_CHAIN_MAX links (electrodes one plasma node apart, 32 in a row) gives
-// NaN.
+// the last applicable block's normal component (lepic_ewall.h), computed
+// from the pass-1/2a field into `outv` (one double per node, written only at
+// the nodes the pass sets) so that no thread reads a node another one
+// writes; k_e_walls_store then sets those nodes' E from it and the central
+// differences of phi.  A chain longer than E_CHAIN_MAX links (electrodes one
+// plasma node apart, 32 in a row) gives NaN.
 __global__ void k_e_walls(KGrid g, const double *__restrict__ phi,
                           const int32_t *__restrict__ bcnd, const double4 *__restrict__ E,
-                          double4 *out, int64_t nodes) {
+                          double *outv, int64_t nodes) {
   int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (c >= nodes) return;
   if (!ew_physical(g, c) || bcnd[c] < 1) return;
@@ -212,18 +216,21 @@ __global__ void k_e_walls(KGrid g, const double *__restrict__ phi,
   else v = ew_central(g, phi, ch.t, ch.ax);
   for (int j = ch.k - 1; j >= 0; --j) v = 2.0 * ew_comp(E[ch.a[j]], ch.ax) - v;
   if (ch.overflow) v = __builtin_nan("");
-  double e[3];
-  for (int d = 0; d < 3; ++d) e[d] = d == ch.ax ? v : ew_central(g, phi, c, d);
-  out[c] = make_double4(e[0], e[1], e[2], E[c].w);
+  outv[c] = v;
 }
 
-__global__ void k_e_walls_store(KGrid g, const int32_t *__restrict__ bcnd,
-                                const double4 *__restrict__ out, double4 *E, int64_t nodes) {
+__global__ void k_e_walls_store(KGrid g, const double *__restrict__ phi,
+                                const int32_t *__restrict__ bcnd,
+                                const double *__restrict__ outv, double4 *E, int64_t nodes) {
   int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (c >= nodes) return;
   if (!ew_physical(g, c) || bcnd[c] < 1) return;
-  if (ew_last(g, bcnd, c) < 0) return;
-  E[c] = out[c];
+  const int last = ew_last(g, bcnd, c);
+  if (last < 0) return;
+  const int ax = last / 2;
+  double e[3];
+  for (int d = 0; d < 3; ++d) e[d] = d == ax ? outv[c] : ew_central(g, phi, c, d);
+  E[c] = make_double4(e[0], e[1], e[2], E[c].w);
 }
 
 // apply_dielectric_bc_to_phi (mod_state.f90:378-430): on every node of a
@@ -256,6 +263,69 @@ __global__ void k_die_phi(KGrid g, const int32_t *__restrict__ bcnd,
 }
 
 }  // namespace
+
+// Scratch of the wall pass for lepic_efield_noalloc: one double per node,
+// in a file-scope device array that every GPU the library runs on holds
+// until the process ends (LEPIC_EFIELD_NOALLOC_MAX_NODES doubles, 18 MiB).
+// Host code never passes the array's name to a kernel (on the host it is a
+// stand-in, not a device address); it uses the address hipGetSymbolAddress
+// gives for the device, resolved once per device when a context is created.
+static __device__ __align__(256) double scratch[LEPIC_EFIELD_NOALLOC_MAX_NODES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[MAX_DEVICES] = {};
+bool g_scratch_tried[MAX_DEVICES] = {};
+
+double *scratch_ptr(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
+// E = -grad(phi) (calc_Efield_modular, mod_electricField.f90:9-164): the
+// interior and periodic passes, then the electrode-wall pass through outv.
+int efield_launch(lepic_ctx *ctx, const lepic_grid *g, const double *phi, const int32_t *bcnd,
+                  double *E4, double *outv) {
+  KGrid kg = make_kgrid(g);
+  const int64_t nodes = grid_nodes(g);
+  double4 *E = reinterpret_cast<double4 *>(E4);
+  dim3 gr(grid_blocks(nodes, BLOCK));
+  hipLaunchKernelGGL(k_e_interior, gr, dim3(BLOCK), 0, ctx->stream, kg, phi, bcnd, E, nodes);
+  const int64_t nphys = (int64_t)(g->n[0] + 1) * (g->n[1] + 1) * (g->n[2] + 1);
+  const int64_t nxz = (int64_t)(g->n[0] + 1) * (g->n[2] + 1);
+  const int64_t nxy = (int64_t)(g->n[0] + 1) * (g->n[1] + 1);
+  hipLaunchKernelGGL(k_e_periodic, dim3(grid_blocks(nphys, BLOCK)), dim3(BLOCK), 0, ctx->stream, kg,
+                     phi, bcnd, E, 0);
+  hipLaunchKernelGGL(k_e_periodic, dim3(grid_blocks(nxz, BLOCK)), dim3(BLOCK), 0, ctx->stream, kg,
+                     phi, bcnd, E, 1);
+  hipLaunchKernelGGL(k_e_periodic, dim3(grid_blocks(nxy, BLOCK)), dim3(BLOCK), 0, ctx->stream, kg,
+                     phi, bcnd, E, 2);
+  hipLaunchKernelGGL(k_e_periodic, dim3(grid_blocks(nxz, BLOCK)), dim3(BLOCK), 0, ctx->stream, kg,
+                     phi, bcnd, E, 3);
+  hipLaunchKernelGGL(k_e_walls, gr, dim3(BLOCK), 0, ctx->stream, kg, phi, bcnd, E, outv, nodes);
+  hipLaunchKernelGGL(k_e_walls_store, gr, dim3(BLOCK), 0, ctx->stream, kg, phi, bcnd, outv, E,
+                     nodes);
+  LEPIC_LAUNCH_CHECK(ctx, "efield");
+  return LEPIC_OK;
+}
+
+}  // namespace
+
+// Called by lepic_create with the context's device current.
+void lepic_efield_scratch_setup(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_tried[device]) return;
+  g_scratch_tried[device] = true;
+  void *ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) == hipSuccess && ptr)
+    g_scratch_ptr[device] = reinterpret_cast<double *>(ptr);
+  else
+    (void)hipGetLastError();
+}
 
 extern "C" {
 
@@ -329,28 +399,28 @@ int lepic_efield(lepic_ctx *ctx, const lepic_grid *g, const double *phi, const i
   if (!ctx || !g || !phi || !bcnd || !E4) return LEPIC_ERR_ARG;
   if (g->n[0] < 2 || g->n[1] < 2 || g->n[2] < 2)
     return lepic_fail(ctx, LEPIC_ERR_ARG, "efield: need n >= 2 in every dimension");
-  KGrid kg = make_kgrid(g);
   const int64_t nodes = grid_nodes(g);
-  double4 *E = reinterpret_cast<double4 *>(E4);
-  dim3 gr(grid_blocks(nodes, BLOCK));
-  hipLaunchKernelGGL(k_e_interior, gr, dim3(BLOCK), 0, ctx->stream, kg, phi, bcnd, E, nodes);
-  const int64_t nphys = (int64_t)(g->n[0] + 1) * (g->n[1] + 1) * (g->n[2] + 1);
-  const int64_t nxz = (int64_t)(g->n[0] + 1) * (g->n[2] + 1);
-  const int64_t nxy = (int64_t)(g->n[0] + 1) * (g->n[1] + 1);
-  hipLaunchKernelGGL(k_e_periodic, dim3(grid_blocks(nphys, BLOCK)), dim3(BLOCK), 0, ctx->stream, kg,
-                     phi, bcnd, E, 0);
-  hipLaunchKernelGGL(k_e_periodic, dim3(grid_blocks(nxz, BLOCK)), dim3(BLOCK), 0, ctx->stream, kg,
-                     phi, bcnd, E, 1);
-  hipLaunchKernelGGL(k_e_periodic, dim3(grid_blocks(nxy, BLOCK)), dim3(BLOCK), 0, ctx->stream, kg,
-                     phi, bcnd, E, 2);
-  hipLaunchKernelGGL(k_e_periodic, dim3(grid_blocks(nxz, BLOCK)), dim3(BLOCK), 0, ctx->stream, kg,
-                     phi, bcnd, E, 3);
-  double4 *out = reinterpret_cast<double4 *>(lepic_scratch(ctx, sizeof(double4) * (size_t)nodes, 1));
-  if (!out) return lepic_fail(ctx, LEPIC_ERR_HIP, "efield: scratch");
-  hipLaunchKernelGGL(k_e_walls, gr, dim3(BLOCK), 0, ctx->stream, kg, phi, bcnd, E, out, nodes);
-  hipLaunchKernelGGL(k_e_walls_store, gr, dim3(BLOCK), 0, ctx->stream, kg, bcnd, out, E, nodes);
-  LEPIC_LAUNCH_CHECK(ctx, "efield");
-  return LEPIC_OK;
+  double *outv = reinterpret_cast<double *>(lepic_scratch(ctx, sizeof(double) * (size_t)nodes, 1));
+  if (!outv) return lepic_fail(ctx, LEPIC_ERR_HIP, "efield: scratch");
+  return efield_launch(ctx, g, phi, bcnd, E4, outv);
+}
+
+int lepic_efield_noalloc(lepic_ctx *ctx, const lepic_grid *g, const double *phi,
+                         const int32_t *bcnd, double *E4) {
+  if (!ctx || !g || !phi || !bcnd || !E4) return LEPIC_ERR_ARG;
+  if (g->n[0] < 2 || g->n[1] < 2 || g->n[2] < 2)
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "efield_noalloc: need n >= 2 in every dimension");
+  const int64_t nodes = grid_nodes(g);
+  if (nodes > LEPIC_EFIELD_NOALLOC_MAX_NODES)
+    return lepic_fail(ctx, LEPIC_ERR_ARG,
+                      "efield_noalloc: " + std::to_string(nodes) + " nodes exceed the static "
+                      "scratch of " + std::to_string(LEPIC_EFIELD_NOALLOC_MAX_NODES) + " nodes");
+  double *outv = scratch_ptr(ctx->device);
+  if (!outv)
+    return lepic_fail(ctx, LEPIC_ERR_HIP, "efield_noalloc: the static scratch of device " +
+                                              std::to_string(ctx->device) +
+                                              " was not resolved when the context was created");
+  return efield_launch(ctx, g, phi, bcnd, E4, outv);
 }
 
 }  // extern "C"

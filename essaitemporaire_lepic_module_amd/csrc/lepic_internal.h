@@ -87,6 +87,9 @@ enum : int {
 int lepic_fail(lepic_ctx *ctx, int code, const std::string &msg);
 int lepic_hip_check(lepic_ctx *ctx, hipError_t e, const char *what);
 void *lepic_scratch(lepic_ctx *ctx, size_t bytes, int which);
+// Resolves, once per device, the static scratch lepic_efield_noalloc uses
+// (lepic_fields.hip); the device must be current.
+void lepic_efield_scratch_setup(int device);
 uint32_t lepic_next_epoch(lepic_ctx *ctx, int64_t ntiles);
 
 #define LEPIC_HIP(ctx, call)                                    \

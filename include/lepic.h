@@ -262,6 +262,19 @@ int lepic_poisson_solve_mixed(lepic_ctx *ctx, const lepic_grid *g, double *phi, 
 int lepic_efield(lepic_ctx *ctx, const lepic_grid *g, const double *phi, const int32_t *bcnd,
                  double *E4);
 
+/* lepic_efield for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It has no counterpart in the reference.  Same arguments and the same E,
+ * bit for bit; the wall pass works in a static device array of
+ * LEPIC_EFIELD_NOALLOC_MAX_NODES doubles (18 MiB, held by every GPU the
+ * library runs on until the process ends) instead of context scratch, and
+ * the call allocates nothing.  Grids with more nodes ((n1+3)(n2+3)(n3+3))
+ * give LEPIC_ERR_ARG.  Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (two streams, two contexts) race. */
+#define LEPIC_EFIELD_NOALLOC_MAX_NODES 2359296
+int lepic_efield_noalloc(lepic_ctx *ctx, const lepic_grid *g, const double *phi,
+                         const int32_t *bcnd, double *E4);
+
 /* ---------------- a10: null-collision MCC ---------------- */
 /* perform_collisions_step (mod_collisions.f90:214-739) for one rank and one
  * virtual thread.  parts[ntype_tracked] are updated in place (velocities,
