@@ -182,6 +182,7 @@ def _step_bench(args, precision):
         sim.advance_one_step(istep)
         istep += 1
     n_before = sim.particle_count()
+    first_timed = istep
     torch.cuda.synchronize()
     sim.timers = {}
     sim.timed_phases = {"push_bc_deposit"}
@@ -200,7 +201,8 @@ def _step_bench(args, precision):
     marks[-1].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    step_ms = sorted(a.elapsed_time(b) for a, b in zip(marks[:-1], marks[1:]))
+    step_each = [a.elapsed_time(b) for a, b in zip(marks[:-1], marks[1:])]
+    step_ms = sorted(step_each)
     particle_steps = slots - int(holes.sum().item())
     kernel_phases = sim.phase_times()
     sim.timers = {}
@@ -220,6 +222,8 @@ def _step_bench(args, precision):
         "ms_per_step": wall * 1e3 / args.steps,
         "step_ms_median": step_ms[len(step_ms) // 2],
         "step_ms_p90": step_ms[min(len(step_ms) - 1, (9 * len(step_ms)) // 10)],
+        "step_ms_each": [round(v, 3) for v in step_each],
+        "first_timed_step": first_timed,
         "particles": n_before,
         "electron_resort_every": 0 if resort is None else resort[0],
         "mg_solve_ms": mg_ms / max(mg_calls, 1),
@@ -283,11 +287,59 @@ def _poisson_cfg3(n):
     return res
 
 
+def _adjoint_cfg4(steps, n=64, ppc=8, checkpoint=20):
+    """The differentiable run: n^3 cells x ppc per species, one wall biased
+    at 20 V, Bz = 0.005 T, argon MCC every 10 steps frozen in the backward,
+    `steps` steps with checkpoints every `checkpoint`.  Times the plain
+    forward (no checkpoints), the forward that keeps checkpoints and the
+    backward (recompute + adjoint kernels), each once after an untimed
+    forward + backward of the same run."""
+    import torch
+    import numpy as np
+    from essaitemporaire_lepic_module_amd import config as cfg
+    from essaitemporaire_lepic_module_amd.adjoint import DiffPIC
+    case = cfg.cfg4_differentiable(n=n, np_cell=ppc, steps=steps)
+    rng = np.random.default_rng(0)
+    N = ppc * n ** 3
+    L = np.array([n * case.h[d] for d in range(3)])
+    pos = rng.random((N, 3)) * L
+    parts = []
+    for sp in case.tracked:
+        vel = rng.standard_normal((N, 3)) * cfg.thermal_speed(sp.T_eV, sp.mass) / math.sqrt(2.0)
+        parts.append({"x": pos[:, 0], "y": pos[:, 1], "z": pos[:, 2],
+                      "vx": vel[:, 0], "vy": vel[:, 1], "vz": vel[:, 2]})
+    sim = DiffPIC(case, parts, "cuda:0", checkpoint=checkpoint)
+    sim.forward(20.0, 0.005, steps, keep=True)
+    sim.backward()
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, r
+
+    t_plain, loss = timed(lambda: sim.forward(20.0, 0.005, steps, keep=False))
+    t_keep, _ = timed(lambda: sim.forward(20.0, 0.005, steps, keep=True))
+    t_back, (dV, dB) = timed(sim.backward)
+    ncoll = sum(int(ev["idx"].numel()) for evs in sim.mcc_events.values() for ev in evs)
+    out = {"forward_ms": t_plain * 1e3, "forward_checkpointed_ms": t_keep * 1e3,
+           "backward_ms": t_back * 1e3,
+           "adjoint_over_forward": (t_keep + t_back) / t_plain,
+           "loss": loss, "dL_dV": dV, "dL_dB0": dB, "collided": ncoll,
+           "workload": (f"{n}^3 cells, {ppc} ppc x 2 species, wall biased at 20 V, Bz 0.005 T, "
+                        f"argon MCC every 10 steps (events frozen in the backward), {steps} "
+                        f"steps, checkpoint every {checkpoint}, FP64")}
+    del sim
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--ppc", type=int, default=64)
     ap.add_argument("--precision", choices=("f32", "f64", "both"), default="both",
@@ -297,6 +349,9 @@ def main():
     ap.add_argument("--poisson-n", type=int, default=256,
                     help="grid of the mixed-precision Poisson comparison (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=200,
+                    help="steps of the differentiable 64^3 run whose forward+adjoint / forward "
+                         "cost is reported (0: skip)")
     ap.add_argument("--resort-electrons", type=int, default=4,
                     help="production path: also re-sort electrons by cell every k steps "
                          "(0: only at the reference's sort cadence)")
@@ -341,6 +396,8 @@ def main():
         "ms_per_step": main_run["ms_per_step"],
         "step_ms_median": main_run["step_ms_median"],
         "step_ms_p90": main_run["step_ms_p90"],
+        "step_ms_each": main_run["step_ms_each"],
+        "first_timed_step": main_run["first_timed_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -364,6 +421,9 @@ def main():
     if args.poisson_n:
         line["poisson_mixed"] = _poisson_cfg3(args.poisson_n)
         print("[bench] poisson done", file=sys.stderr, flush=True)
+    if args.adjoint_steps:
+        line["adjoint"] = _adjoint_cfg4(args.adjoint_steps)
+        print("[bench] adjoint done", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(case)
     print(json.dumps(line))

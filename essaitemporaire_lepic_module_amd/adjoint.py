@@ -6,17 +6,23 @@ The forward is the FP64 step on particles held in place (lost particles
 become holes and keep their slots, so each slot is one particle for the
 whole run): rho from the deposit, the multigrid potential with the biased
 electrode at V, E, the Boris push with B = (0, 0, B0) and the E gather, the
-wall test, the fused deposit for the next step (mod_simulation.f90:340-472
-without sorting or collisions).  The loss is the time-averaged flux of one
-species (Ar+) through the node plane one cell in front of the biased wall,
+wall test, the fused deposit for the next step, and on collision steps the
+null-collision MCC (mod_simulation.f90:340-472 without sorting; MCC
+products take slots reserved after the initial particles).  The loss is the
+time-averaged flux of one species (Ar+) through the node plane one cell in
+front of the biased wall,
 
     L = (1/K) sum_k q Nm / (Ly Lz h1) sum_p vx_p hat(x_p / h1 - (n1 - 1)),
 
 a smooth stand-in for the charge the wall collects per unit time and area
 (apply_particle_bc_legacy's p_mac, mod_particleBC.f90:195-206, counts whole
-particles and has zero derivative almost everywhere).  Which particles the
-walls absorb, and at which step, is frozen in the backward: the gradient is
-the derivative of the run with that event sequence held fixed.
+particles and has zero derivative almost everywhere).  The events are
+frozen in the backward: which particles the walls absorb and at which step,
+and the collisions -- which particles collide at which step, the velocity
+each leaves with and the products (their whole state) -- are constants of
+the map the gradient differentiates.  So a collided particle's velocity
+carries no derivative from before its collision (its position does), and a
+product's state none from before its creation.
 
 Memory: particle states are kept only at checkpoints every `checkpoint`
 steps (positions, velocities, hole flags, the deposited density and the
@@ -42,13 +48,14 @@ from .device import context, ptr
 _STATE = ("x", "y", "z", "vx", "vy", "vz", "dead")
 
 
-def _pset_of(d: dict, n: int) -> _native.PSet:
+def _pset_of(d: dict, n: int, cap: int | None = None) -> _native.PSet:
+    """The set of the first n slots of the arrays in d (cap slots)."""
     p = _native.PSet()
     for f in ("x", "y", "z", "vx", "vy", "vz", "dead"):
         setattr(p, f, d[f].data_ptr())
-    p.cell = d["cell"].data_ptr() if "cell" in d else None
+    p.cell = None
     p.n = n
-    p.cap = n
+    p.cap = n if cap is None else cap
     return p
 
 
@@ -76,12 +83,15 @@ class DiffPIC:
     """The differentiable run of a Case on one device.
 
     case: plasma (-1) and electrode (>= 1) labels only, B uniform along z or
-    absent, no collisions, dielectrics or heating.  parts: the initial FP64
-    particle sets (host arrays dict per species: x y z vx vy vz), loaded into
-    device sets of exactly that many slots."""
+    absent, collisions (PHILOX draws, keyed by philox_key and the step) or
+    none, no dielectrics or heating.  parts: the initial FP64 particle sets
+    (host arrays dict per species: x y z vx vy vz), loaded into device sets
+    of that many slots plus, with collisions, room for the products
+    (product_room of the initial count, at least 1024 slots)."""
 
     def __init__(self, case: Case, parts, device=None, bias_label: int = 2, flux_species: int = 1,
-                 checkpoint: int = 20, adjoint_eps: float = 1e-12):
+                 checkpoint: int = 20, adjoint_eps: float = 1e-12, philox_key: int = 0x5EED5EED,
+                 product_room: float = 1.0 / 16):
         b = np.asarray(case.bcnd)
         if np.any((b == 0) | (b == -2)) or case.flag_pbc or case.flag_nmn:
             raise ValueError("the adjoint E field takes plasma (-1) and electrode labels only")
@@ -109,14 +119,25 @@ class DiffPIC:
         self.Nm = list(case.Nm[: self.ntype])
         shape = node_shape(self.n)
         self.shape = shape
+        self.collide_every = int(case.nb_step_collisions) if case.chemistry is not None else 0
+        self.tables = None
+        if self.collide_every > 0:
+            self.tables = P.ReactionTables(case.chemistry, case.Nm, dev)
+            self.Pcoll = torch.zeros(case.chemistry["ntype_total"], dtype=torch.float64,
+                                     device=dev)
+        self.philox_key = int(philox_key)
         self.init = []
         for a in parts:
             m = len(a["x"])
-            d = {f: torch.as_tensor(np.asarray(a[f], np.float64), device=dev).clone()
-                 for f in ("x", "y", "z", "vx", "vy", "vz")}
-            d["dead"] = torch.zeros(max(m, 4), dtype=torch.int8, device=dev)[:max(m, 4)]
-            d["cell"] = torch.zeros(max(m, 4), dtype=torch.int32, device=dev)
-            self.init.append((m, d))
+            cap = max(m + (max(1024, int(m * product_room)) if self.collide_every else 0), 4)
+            d = {}
+            for f in ("x", "y", "z", "vx", "vy", "vz"):
+                d[f] = torch.zeros(cap, dtype=torch.float64, device=dev)
+                d[f][:m] = torch.as_tensor(np.asarray(a[f], np.float64), device=dev)
+            d["dead"] = torch.ones(cap, dtype=torch.int8, device=dev)
+            d["dead"][:m] = 0
+            self.init.append((m, d, cap))
+        self.mcc_events = {}
         n1, n2, n3 = self.n
         self.plane = n1                       # node plane x = (n1 - 1) h1
         area = n2 * self.h[1] * n3 * self.h[2]
@@ -136,13 +157,12 @@ class DiffPIC:
 
     # -- forward ----------------------------------------------------------
     def _fresh(self):
-        """Working particle arrays from the initial sets, the initial deposit
-        and phi = electrode voltages."""
+        """Working particle arrays from the initial sets ([slots in use,
+        arrays] per species) and the initial deposit."""
         st = []
-        for m, d in self.init:
+        for m, d, _ in self.init:
             w = {k: v.clone() for k, v in d.items()}
-            w["dead"].zero_()
-            st.append((m, w))
+            st.append([m, w])
         self.np_thread.zero_()
         for s, (m, w) in enumerate(st):
             p = _pset_of(w, m)
@@ -188,7 +208,63 @@ class DiffPIC:
 
     @staticmethod
     def _snap(st):
-        return [(m, {k: v.clone() for k, v in w.items()}) for m, w in st]
+        return [[m, {k: v.clone() for k, v in w.items()}] for m, w in st]
+
+    def _collides(self, k: int) -> bool:
+        return self.collide_every > 0 and k % self.collide_every == 0
+
+    def _collide(self, st, k: int, events=None):
+        """MCC at step k (perform_collisions_step, mod_collisions.f90:214-739,
+        on all particles with per-particle Philox draws), its density change
+        added to the step's deposit.  events None: run it and return the
+        record (per species: the collided slots and the velocities they
+        leave with, the product slots and their states); else replay that
+        record (the frozen-event run)."""
+        ctx, g = self.ctx, self.g
+        if events is not None:
+            for s, ev in enumerate(events):
+                m, w = st[s]
+                if ev["idx"].numel():
+                    for f in ("vx", "vy", "vz"):
+                        w[f].index_copy_(0, ev["idx"], ev[f])
+                a, b = ev["m0"], ev["m1"]
+                if b > a:
+                    for f in ("x", "y", "z", "vx", "vy", "vz"):
+                        w[f][a:b] = ev["prod"][f]
+                    w["dead"][a:b] = 0
+                    # the record's own (aligned) arrays: the deposit loads 16-byte vectors
+                    p = _pset_of(ev["prod"], b - a)
+                    ctx.check(ctx.lib.lepic_deposit(ctx.h, C.byref(g), C.byref(p), ptr(self.kq),
+                                                    float(self.Nm[s]), ptr(self.np_thread[s])))
+                st[s][0] = b
+            return events
+        c = self.case
+        v0 = [torch.stack([w["vx"][:m], w["vy"][:m], w["vz"][:m]]) for m, w in st]
+        dead0 = [w["dead"][:m].clone() for m, w in st]
+        arr = (_native.PSet * self.ntype)(*[_pset_of(w, m, w["x"].numel()) for m, w in st])
+        seed = C.c_int32(0)
+        stats = _native.MCCStats()
+        ctx.check(ctx.lib.lepic_collisions_deposited(
+            ctx.h, C.byref(g), arr, ptr(self.np_red), C.byref(self.tables.c),
+            int(c.nb_step_collisions), float(c.dt), int(_native.MCC_PHILOX), C.byref(seed),
+            C.c_uint64(self.philox_key), C.c_uint64(k), ptr(self.Pcoll), C.byref(stats),
+            ptr(self.np_thread), ptr(self.kq)))
+        out = []
+        for s, (m, w) in enumerate(st):
+            m1 = int(arr[s].n)
+            if not torch.equal(w["dead"][:m], dead0[s]):
+                raise RuntimeError("collisions removed particles; the differentiable step "
+                                   "freezes velocity changes and products only")
+            v1 = torch.stack([w["vx"][:m], w["vy"][:m], w["vz"][:m]])
+            idx = (v1 != v0[s]).any(0).nonzero().squeeze(1)
+            prod = {f: w[f][m:m1].clone() for f in ("x", "y", "z", "vx", "vy", "vz")}
+            prod["dead"] = torch.zeros(m1 - m + 16, dtype=torch.int8, device=self.dev)
+            ev = {"idx": idx, "m0": m, "m1": m1, "prod": prod}
+            for f in ("vx", "vy", "vz"):
+                ev[f] = w[f].index_select(0, idx)
+            out.append(ev)
+            st[s][0] = m1
+        return out
 
     def _push_frozen(self, st, B0, deaths, k):
         """The push with the wall events of a recorded run: every live slot
@@ -209,20 +285,24 @@ class DiffPIC:
                                                       ptr(self.np_thread[s])))
 
     def forward(self, V: float, B0, steps: int, keep: bool = True,
-                record_deaths: bool = False, frozen=None) -> float:
+                record_deaths: bool = False, frozen=None, frozen_mcc=None) -> float:
         """The loss of `steps` steps at bias V and field B0 (None: no B);
         keep: store the checkpoints the backward needs; record_deaths: keep
         in self.deaths, per species, the step at which each slot was lost
         (0: never); frozen: such a record, whose wall events replace the
-        wall test (the run the adjoint differentiates, for checks by finite
-        differences)."""
+        wall test; frozen_mcc: a run's self.mcc_events, whose collisions
+        replace MCC (with both, the run the adjoint differentiates, for
+        checks by finite differences).  Without frozen_mcc the collisions
+        are drawn and recorded in self.mcc_events."""
         st = self._fresh()
         self.phi.zero_()
         self.acc.zero_()
         self.ckpt = {}
+        if frozen_mcc is None:
+            self.mcc_events = {}
         if record_deaths:
-            self.deaths = [torch.zeros(max(m, 1), dtype=torch.int64, device=self.dev)
-                           for m, _ in st]
+            self.deaths = [torch.zeros(max(w["x"].numel(), 1), dtype=torch.int64,
+                                       device=self.dev) for _, w in st]
         for k in range(1, steps + 1):
             if keep and (k - 1) % self.K == 0:
                 self.ckpt[k] = (self._snap(st), self.np_thread.clone(), self.phi.clone())
@@ -231,6 +311,11 @@ class DiffPIC:
                 self._push_frozen(st, B0, frozen, k)
             else:
                 self._push(st, B0)
+            if self._collides(k):
+                if frozen_mcc is not None:
+                    self._collide(st, k, frozen_mcc[k])
+                else:
+                    self.mcc_events[k] = self._collide(st, k)
             self._flux(st, self.acc)
             if record_deaths:
                 for s, (m, w) in enumerate(st):
@@ -249,7 +334,7 @@ class DiffPIC:
         steps = self.steps
         ctx, g = self.ctx, self.g
         dev = self.dev
-        adj = [AdjointSet(m, dev) for m, _ in self.init]
+        adj = [AdjointSet(cap, dev) for _, _, cap in self.init]
         lamE = torch.zeros(self.shape + (4,), dtype=torch.float64, device=dev)
         lamphi = torch.zeros(self.shape, dtype=torch.float64, device=dev)
         rhs = torch.zeros(self.shape, dtype=torch.float64, device=dev)
@@ -273,11 +358,25 @@ class DiffPIC:
                 self._fields(V)
                 Es.append(self.E4.clone())
                 self._push(st, B0)
-            states.append(self._snap(st))          # after the segment's last push
+                if self._collides(k):
+                    self._collide(st, k, self.mcc_events[k])
+            states.append(self._snap(st))          # after the segment's last step
             for j in range(s1 - s0, -1, -1):
                 after, before = states[j + 1], states[j]
-                # loss term of this step (state after the push)
+                k = s0 + j
+                # loss term of this step (state after the push and MCC)
                 self._flux(after, None, adj[self.flux_species], scale)
+                if self._collides(k):
+                    # frozen collisions: the velocity a collided particle
+                    # leaves with, and a product's state, are constants
+                    for s, ev in enumerate(self.mcc_events[k]):
+                        a = adj[s].a
+                        if ev["idx"].numel():
+                            for f in ("lvx", "lvy", "lvz"):
+                                a[f].index_fill_(0, ev["idx"], 0.0)
+                        if ev["m1"] > ev["m0"]:
+                            for v in a.values():
+                                v[ev["m0"]:ev["m1"]] = 0.0
                 lamE.zero_()
                 for s, (m, w) in enumerate(before):
                     p = _pset_of(w, m)

@@ -189,21 +189,32 @@ def cfg2_pic_mcc(n: int = 128, np_cell: int = 64, steps: int = 1000, ng: int = 7
 
 
 def cfg4_differentiable(n: int = 64, np_cell: int = 8, Vb: float = 20.0, B0: float = 0.005,
-                        steps: int = 200, ng: int | None = None) -> Case:
+                        steps: int = 200, ng: int | None = None, mcc_every: int = 10,
+                        pressure_Pa: float = 1.0) -> Case:
     """Differentiable PIC: n^3 cells (h = 3.125e-4 m), e- and Ar+ at np_cell
     particles per cell and species, a grounded box whose x = L face is an
     electrode of its own (label 2) biased at Vb, uniform Bz = B0 acting on
-    both species, no collisions, dt = 0.1/omega_pe, MG to 1e-10.  The
-    derivatives are taken with respect to Vb and B0."""
+    both species, dt = 0.1/omega_pe, MG to 1e-10; the analytic argon MCC of
+    the PIC-MCC case every mcc_every steps (0: no collisions) on a background
+    at pressure_Pa and 300 K.  The derivatives are taken with respect to Vb
+    and B0."""
     nn = (n, n, n)
     h = (3.125e-4,) * 3
     n0 = 1e15
-    sp = argon_species()[:2]
     if ng is None:
         ng = max(2, int(round(math.log2(n))))
-    return Case("differentiable", nn, h, biased_wall_bcnd(nn, 2), 2, {1: 0.0, 2: Vb}, sp, n0,
-                np_cell, 0.1 / omega_pe(n0), steps, eps=1e-10, ng=ng, B_uniform=(0.0, 0.0, B0),
-                nb_step_collisions=0)
+    if mcc_every <= 0:
+        sp = argon_species()[:2]
+        return Case("differentiable", nn, h, biased_wall_bcnd(nn, 2), 2, {1: 0.0, 2: Vb}, sp,
+                    n0, np_cell, 0.1 / omega_pe(n0), steps, eps=1e-10, ng=ng,
+                    B_uniform=(0.0, 0.0, B0), nb_step_collisions=0)
+    sp = argon_species()
+    ngas = pressure_Pa / (KB * 300.0)
+    c = Case("differentiable", nn, h, biased_wall_bcnd(nn, 2), 2, {1: 0.0, 2: Vb}, sp, n0,
+             np_cell, 0.1 / omega_pe(n0), steps, eps=1e-10, ng=ng, B_uniform=(0.0, 0.0, B0),
+             ngas=ngas, nb_step_collisions=int(mcc_every))
+    c.chemistry = argon_chemistry(sp, ngas)
+    return c
 
 
 def cfg3_poisson(n: int = 256, seed: int = 3, h: float = 3.125e-4, nblobs: int = 64):

@@ -281,7 +281,7 @@ def test_gradient_matches_pure_torch_16cube(gpu, steps, checkpoint):
     run from the same particles, with the wall events the GPU forward saw."""
     from essaitemporaire_lepic_module_amd.adjoint import DiffPIC
     rng = np.random.default_rng(4)
-    case = cfg.cfg4_differentiable(n=16, np_cell=4, steps=steps)
+    case = cfg.cfg4_differentiable(n=16, np_cell=4, steps=steps, mcc_every=0)
     parts = _initial(rng, case)
     sim = DiffPIC(case, parts, "cuda:0", checkpoint=checkpoint)
     L = sim.forward(20.0, 0.005, steps, keep=True, record_deaths=True)
@@ -301,6 +301,8 @@ def test_gradient_matches_pure_torch_16cube(gpu, steps, checkpoint):
 
 
 def test_checkpoint_interval_leaves_gradient_unchanged(gpu):
+    """Checkpoints every 1, 4 and 12 steps give one gradient; MCC at step 10
+    (recomputed segments replay the recorded collisions)."""
     from essaitemporaire_lepic_module_amd.adjoint import DiffPIC
     rng = np.random.default_rng(5)
     case = cfg.cfg4_differentiable(n=16, np_cell=4, steps=12)
@@ -309,6 +311,7 @@ def test_checkpoint_interval_leaves_gradient_unchanged(gpu):
     for K in (1, 4, 12):
         sim = DiffPIC(case, parts, "cuda:0", checkpoint=K)
         sim.forward(20.0, 0.005, 12)
+        assert sum(int(ev["idx"].numel()) for ev in sim.mcc_events[10]) > 0
         out.append(sim.backward())
     for dV, dB in out[1:]:
         assert abs(dV - out[0][0]) <= 1e-9 * abs(out[0][0])
@@ -316,9 +319,10 @@ def test_checkpoint_interval_leaves_gradient_unchanged(gpu):
 
 
 def test_gradient_matches_central_differences(gpu):
-    """32^3 x 8 ppc, 20 steps: dL/dV and dL/dB0 against central differences
-    (steps 1e-3 and 1e-4 relative) of the run with the nominal wall events replayed
-    (the map the adjoint differentiates).  Central differences of the free
+    """32^3 x 8 ppc, 20 steps, MCC at steps 10 and 20: dL/dV and dL/dB0
+    against central differences (steps 1e-3 and 1e-4 relative) of the run with
+    the nominal wall events and collisions replayed (the map the adjoint
+    differentiates).  Central differences of the free
     run, whose wall events move with the parameters, are printed beside
     them: their difference from the adjoint is the events' own share of the
     derivative, which the frozen-event gradient leaves out."""
@@ -335,6 +339,11 @@ def test_gradient_matches_central_differences(gpu):
     dV, dB = V.grad.item(), B0.grad.item()
     sim.forward(20.0, 0.005, steps, keep=False, record_deaths=True)
     deaths = [d.clone() for d in sim.deaths]
+    events = sim.mcc_events
+    ncoll = sum(int(ev["idx"].numel()) for evs in events.values() for ev in evs)
+    nprod = sum(ev["m1"] - ev["m0"] for evs in events.values() for ev in evs)
+    print(f"collided {ncoll} products {nprod}")
+    assert ncoll > 0
     # The frozen-event map is piecewise smooth: a particle crossing a cell
     # plane between the two runs puts a kink of its deposit weights inside
     # the difference, an error that shrinks with the step (for dL/dB0 it was
@@ -342,7 +351,8 @@ def test_gradient_matches_central_differences(gpu):
     hV, hB = 20.0 * 1e-3, 0.005 * 1e-4
 
     def fd(frozen):
-        f = lambda a, b: sim.forward(a, b, steps, keep=False, frozen=frozen)
+        f = lambda a, b: sim.forward(a, b, steps, keep=False, frozen=frozen,
+                                     frozen_mcc=events if frozen is not None else None)
         return ((f(20.0 + hV, 0.005) - f(20.0 - hV, 0.005)) / (2 * hV),
                 (f(20.0, 0.005 + hB) - f(20.0, 0.005 - hB)) / (2 * hB))
 
