@@ -90,7 +90,7 @@ class DiffPIC:
     (product_room of the initial count, at least 1024 slots)."""
 
     def __init__(self, case: Case, parts, device=None, bias_label: int = 2, flux_species: int = 1,
-                 checkpoint: int = 20, adjoint_eps: float = 1e-12, philox_key: int = 0x5EED5EED,
+                 checkpoint: int = 20, adjoint_eps: float = 1e-10, philox_key: int = 0x5EED5EED,
                  product_room: float = 1.0 / 16):
         b = np.asarray(case.bcnd)
         if np.any((b == 0) | (b == -2)) or case.flag_pbc or case.flag_nmn:
@@ -112,6 +112,9 @@ class DiffPIC:
         self.bcnd = torch.as_tensor(b.astype(np.int32), device=dev)
         self.kq = torch.as_tensor(case.kq(), device=dev)
         self.bias_mask = self.bcnd == self.label
+        self.elec_mask = torch.zeros_like(self.bcnd, dtype=torch.bool)
+        for lab in case.electrode_V:
+            self.elec_mask |= self.bcnd == lab
         tr = case.tracked
         self.ntype = len(tr)
         self.charge = [s.charge for s in tr]
@@ -127,8 +130,19 @@ class DiffPIC:
                                      device=dev)
         self.philox_key = int(philox_key)
         self.init = []
+        # slots hold the particles sorted by cell (the loss and its gradient
+        # do not depend on the order; gathers and deposits of cell-sorted
+        # particles reuse stencils); perm[s][slot] is the input index
+        self.perm = []
+        n1, n2, n3 = case.n
         for a in parts:
             m = len(a["x"])
+            cx = np.clip((np.asarray(a["x"]) / case.h[0]).astype(np.int64), 0, n1 - 1)
+            cy = np.clip((np.asarray(a["y"]) / case.h[1]).astype(np.int64), 0, n2 - 1)
+            cz = np.clip((np.asarray(a["z"]) / case.h[2]).astype(np.int64), 0, n3 - 1)
+            order = np.argsort(cx + n1 * (cy + n2 * cz), kind="stable")
+            self.perm.append(order)
+            a = {f: np.asarray(a[f], np.float64)[order] for f in ("x", "y", "z", "vx", "vy", "vz")}
             cap = max(m + (max(1024, int(m * product_room)) if self.collide_every else 0), 4)
             d = {}
             for f in ("x", "y", "z", "vx", "vy", "vz"):
@@ -171,9 +185,24 @@ class DiffPIC:
                                                       ptr(self.np_thread[s])))
         return st
 
+    def by_input(self, s: int, per_slot) -> np.ndarray:
+        """A per-slot array of species s (slots 0..m-1, host or device) in the
+        order of the particles the run was given."""
+        v = per_slot.cpu().numpy() if isinstance(per_slot, torch.Tensor) else np.asarray(per_slot)
+        order = self.perm[s]
+        out = np.empty((len(order),) + v.shape[1:], v.dtype)
+        out[order] = v[:len(order)]
+        return out
+
     def _set_electrodes(self, phi, V):
-        for lab, v in self.case.electrode_V.items():
-            phi[self.bcnd == lab] = float(V) if lab == self.label else float(v)
+        """Electrode nodes take their voltages (the biased label V), without
+        a host synchronisation."""
+        if getattr(self, "_elec_V", None) != float(V):
+            vmap = torch.zeros_like(phi)
+            for lab, v in self.case.electrode_V.items():
+                vmap.masked_fill_(self.bcnd == lab, float(V) if lab == self.label else float(v))
+            self._elec_map, self._elec_V = vmap, float(V)
+        torch.where(self.elec_mask, self._elec_map, phi, out=phi)
 
     def _fields(self, V):
         """rho from the deposited density, phi (warm start), E."""

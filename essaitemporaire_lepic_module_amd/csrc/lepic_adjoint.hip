@@ -82,15 +82,38 @@ struct KBAdj {
 // Adjoint of the push of one particle: given dL/dx', dL/dv' of the state
 // after the step (x' = x + dt v', v' = Boris(v, E_p, B)), returns dL/dx and
 // dL/dv of the state before it (the gather's position term included),
-// scatters dL/dE to the node replicas and accumulates dL/dB.
-template <bool REP>
+// scatters dL/dE to the node replicas and accumulates dL/dB.  A lane takes
+// APT consecutive particles and sums the dL/dE of consecutive particles with
+// the same stencil in registers (one cell of cell-sorted particles), so a run
+// costs one set of 24 atomics instead of one per particle.
+constexpr int APT = 4;
+
+__device__ inline void adj_flush(const KGrid &g, double *dst, int64_t b, const double acc[8][3]) {
+  if (b < 0) return;
+  int64_t nd[8];
+  node8(g, b, nd);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    double *d = dst + 4 * nd[c];
+    l2_atomic_add(d + 0, acc[c][0]);
+    l2_atomic_add(d + 1, acc[c][1]);
+    l2_atomic_add(d + 2, acc[c][2]);
+  }
+}
+
 __global__ __launch_bounds__(BLOCK) void k_adj_push(KGrid g, lepic_pset st, KAdj a, int64_t n,
                                                     const double4 *__restrict__ E4, KBAdj bf,
-                                                    double k1, double dt, double *lamE,
+                                                    double k1, double dt, double *rep,
                                                     int64_t nodes, double *lamB) {
-  const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+  const int64_t i0 = (blockIdx.x * (int64_t)BLOCK + threadIdx.x) * APT;
+  double *dst = rep + (int64_t)xcc_id() * 4 * nodes;
   double lb0 = 0.0, lb1 = 0.0, lb2 = 0.0;
-  if (i < n && st.dead[i] == 0) {
+  double acc[8][3];
+  int64_t cur = -1;
+#pragma unroll
+  for (int q = 0; q < APT; ++q) {
+    const int64_t i = i0 + q;
+    if (i >= n || st.dead[i] != 0) continue;
     const double x = st.x[i], y = st.y[i], z = st.z[i];
     const double v[3] = {st.vx[i], st.vy[i], st.vz[i]};
     double lx = a.lx[i], ly = a.ly[i], lz = a.lz[i];
@@ -128,9 +151,9 @@ __global__ __launch_bounds__(BLOCK) void k_adj_push(KGrid g, lepic_pset st, KAdj
                          l2[1] * (v3[2] * B[0] - v3[0] * B[2]) +
                          l2[2] * (v3[0] * B[1] - v3[1] * B[0]);
       // dL/dB from v3 x B (through k2's factor) and from k2(|B|^2)
-      lb0 = bf.k2 * (l2[1] * v3[2] - l2[2] * v3[1]) + lk2 * bf.dk2 * 2.0 * B[0];
-      lb1 = bf.k2 * (l2[2] * v3[0] - l2[0] * v3[2]) + lk2 * bf.dk2 * 2.0 * B[1];
-      lb2 = bf.k2 * (l2[0] * v3[1] - l2[1] * v3[0]) + lk2 * bf.dk2 * 2.0 * B[2];
+      lb0 += bf.k2 * (l2[1] * v3[2] - l2[2] * v3[1]) + lk2 * bf.dk2 * 2.0 * B[0];
+      lb1 += bf.k2 * (l2[2] * v3[0] - l2[0] * v3[2]) + lk2 * bf.dk2 * 2.0 * B[1];
+      lb2 += bf.k2 * (l2[0] * v3[1] - l2[1] * v3[0]) + lk2 * bf.dk2 * 2.0 * B[2];
       // v3 = v1 + k1 (v1 x B)
       const double l1[3] = {l2[0] + l3[0] + k1 * (B[1] * l3[2] - B[2] * l3[1]),
                             l2[1] + l3[1] + k1 * (B[2] * l3[0] - B[0] * l3[2]),
@@ -148,21 +171,21 @@ __global__ __launch_bounds__(BLOCK) void k_adj_push(KGrid g, lepic_pset st, KAdj
         lE[d] = 2.0 * k1 * lam[d];
       }
     }
-    // gather transpose: dL/dE_c += w_c dL/dE_p; position term of the weights
+    // gather transpose: dL/dE_c += w_c dL/dE_p, summed over the run;
+    // position term of the weights
+    if (b != cur) {
+      adj_flush(g, dst, cur, acc);
+      cur = b;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) acc[c][0] = acc[c][1] = acc[c][2] = 0.0;
+    }
     double e[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       e[c] = Ec[c].x * lE[0] + Ec[c].y * lE[1] + Ec[c].z * lE[2];
-      double *dst = lamE + (REP ? (int64_t)xcc_id() * 4 * nodes : 0) + 4 * nd[c];
-      if (REP) {
-        l2_atomic_add(dst + 0, w[c] * lE[0]);
-        l2_atomic_add(dst + 1, w[c] * lE[1]);
-        l2_atomic_add(dst + 2, w[c] * lE[2]);
-      } else {
-        atomicAdd(dst + 0, w[c] * lE[0]);
-        atomicAdd(dst + 1, w[c] * lE[1]);
-        atomicAdd(dst + 2, w[c] * lE[2]);
-      }
+      acc[c][0] += w[c] * lE[0];
+      acc[c][1] += w[c] * lE[1];
+      acc[c][2] += w[c] * lE[2];
     }
     double gx, gy, gz;
     weight_grads(px, py, pz, e, gx, gy, gz);
@@ -172,6 +195,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_push(KGrid g, lepic_pset st, KAdj
     a.lx[i] = lx; a.ly[i] = ly; a.lz[i] = lz;
     a.lvx[i] = lv[0]; a.lvy[i] = lv[1]; a.lvz[i] = lv[2];
   }
+  adj_flush(g, dst, cur, acc);
   if (bf.use_b) {
     __shared__ double red[3][BLOCK / LEPIC_WAVE];
     lb0 = wave_sum_d(lb0); lb1 = wave_sum_d(lb1); lb2 = wave_sum_d(lb2);
@@ -186,7 +210,6 @@ __global__ __launch_bounds__(BLOCK) void k_adj_push(KGrid g, lepic_pset st, KAdj
   }
 }
 
-// lamE (4 per node) += sum of the per-XCD replicas; replicas cleared.
 __global__ void k_adj_rep_flush(double *lamE, double *rep, int64_t words) {
   const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   if (t >= words) return;
@@ -435,7 +458,7 @@ int lepic_adj_push(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *state,
   bf.dk2 = -2.0 * k1 * k1 * k1 / (den * den);
   double *rep = lepic_replicas(ctx, 4 * nodes);
   if (!rep) return lepic_fail(ctx, LEPIC_ERR_HIP, "adj_push: replicas");
-  hipLaunchKernelGGL(k_adj_push<true>, dim3(grid_blocks(n, BLOCK)), dim3(BLOCK), 0, ctx->stream,
+  hipLaunchKernelGGL(k_adj_push, dim3(grid_blocks(n, BLOCK * APT)), dim3(BLOCK), 0, ctx->stream,
                      make_kgrid(g), *state, kadj(adj), n, reinterpret_cast<const double4 *>(E4),
                      bf, k1, dt, rep, nodes, lamB);
   LEPIC_LAUNCH_CHECK(ctx, "k_adj_push");

@@ -79,9 +79,12 @@ int lepic_create(int32_t device, lepic_ctx **out) {
     delete ctx;
     return LEPIC_ERR_HIP;
   }
+  const size_t wd_bytes = sizeof(double) * LEPIC_XCDS * (2 * LEPIC_WD_LABELS + 1);
   if (hipMalloc((void **)&ctx->d_ctrl, sizeof(int64_t) * CTRL_WORDS) != hipSuccess ||
       hipHostMalloc((void **)&ctx->h_pinned, sizeof(int64_t) * CTRL_WORDS) != hipSuccess ||
       hipMemset(ctx->d_ctrl, 0, sizeof(int64_t) * CTRL_WORDS) != hipSuccess ||
+      hipMalloc((void **)&ctx->wdiag, wd_bytes) != hipSuccess ||
+      hipMemset(ctx->wdiag, 0, wd_bytes) != hipSuccess ||
       hipDeviceSynchronize() != hipSuccess) {
     delete ctx;
     return LEPIC_ERR_HIP;
@@ -98,6 +101,7 @@ int lepic_destroy(lepic_ctx *ctx) {
   if (ctx->scratch) (void)hipFree(ctx->scratch);
   if (ctx->scratch2) (void)hipFree(ctx->scratch2);
   if (ctx->d_ctrl) (void)hipFree(ctx->d_ctrl);
+  if (ctx->wdiag) (void)hipFree(ctx->wdiag);
   if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
   if (ctx->d_status) (void)hipFree(ctx->d_status);
   if (ctx->mg_mem) (void)hipFree(ctx->mg_mem);

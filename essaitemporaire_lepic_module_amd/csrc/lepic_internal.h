@@ -49,6 +49,8 @@ struct lepic_ctx {
   // per-XCD deposit replicas (LEPIC_XCDS x nodes doubles, kept zeroed)
   double *rep = nullptr;
   int64_t rep_nodes = 0;
+  // per-XCD wall-diagnostic replicas (kept zeroed)
+  double *wdiag = nullptr;
 };
 
 // MI355X has 8 XCDs, each with its own L2.  Deposition accumulates into one
@@ -57,6 +59,10 @@ struct lepic_ctx {
 // XCC_ID hardware register, so the replica choice never depends on how the
 // dispatcher spreads workgroups.
 #define LEPIC_XCDS 8
+// electrode labels whose wall diagnostics the production pushes sum per
+// block (lepic_partutil.h); the context keeps LEPIC_XCDS replicas of
+// 2 LEPIC_WD_LABELS + 1 doubles for them
+#define LEPIC_WD_LABELS 16
 
 __device__ inline int xcc_id() {
   // s_getreg_b32 hwreg(HW_REG_XCC_ID=20, offset 0, size 4)

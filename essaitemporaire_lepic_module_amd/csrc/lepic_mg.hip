@@ -35,6 +35,25 @@ static_assert(sizeof(MGCtl) == 16 * sizeof(int64_t), "MGCtl layout");
 // One multigrid level: geometry, stencil coefficients and the level's u, b
 // and labels.  T is the storage and arithmetic type of u and b (double for
 // the reference's solver, float for the smoother of the mixed-precision one).
+// floor(x / d) for x < 2^32 by a 64-bit multiply-high: m = ceil(2^64 / d),
+// and x m / 2^64 exceeds x / d by less than x / 2^64 < 2^-32 <= 1/d, so the
+// floor is exact.  Decodes thread numbers into (i, j, k) without a division.
+struct MDiv {
+  uint64_t m;
+  uint32_t d;
+};
+
+inline MDiv mdiv(uint32_t d) {
+  MDiv v;
+  v.d = d;
+  v.m = d <= 1 ? 0 : (~0ull) / d + 1;
+  return v;
+}
+
+__device__ inline uint32_t mdiv_q(uint32_t x, const MDiv &v) {
+  return v.d <= 1 ? x : (uint32_t)__umul64hi((uint64_t)x, v.m);
+}
+
 template <class T>
 struct KLevelT {
   KGrid g;
@@ -42,7 +61,20 @@ struct KLevelT {
   T *u;
   T *b;
   int32_t *bc;
+  // thread-number decoders: n1/2+1, n2+1, n1+1, n1+3, n1+2, n2/2+1
+  MDiv d_h1, d_n2p1, d_n1p1, d_n1p3, d_n1p2, d_h2;
 };
+
+template <class T>
+inline void set_level_divs(KLevelT<T> &L) {
+  const KGrid &g = L.g;
+  L.d_h1 = mdiv((uint32_t)(g.n1 / 2 + 1));
+  L.d_n2p1 = mdiv((uint32_t)(g.n2 + 1));
+  L.d_n1p1 = mdiv((uint32_t)(g.n1 + 1));
+  L.d_n1p3 = mdiv((uint32_t)(g.n1 + 3));
+  L.d_n1p2 = mdiv((uint32_t)(g.n1 + 2));
+  L.d_h2 = mdiv((uint32_t)(g.n2 / 2 + 1));
+}
 using KLevel = KLevelT<double>;
 
 template <class T>
@@ -90,10 +122,10 @@ __device__ inline void sor_node(const KLevelT<T> &L, int colour, T omega, int64_
   const int ni2 = g.n1 / 2 + 1;
   int64_t total = (int64_t)ni2 * (g.n2 + 1) * (g.n3 + 1);
   if (t >= total) return;
-  int i2 = (int)(t % ni2);
-  int64_t r = t / ni2;
-  int j = 1 + (int)(r % (g.n2 + 1));
-  int k = 1 + (int)(r / (g.n2 + 1));
+  const uint32_t tt = (uint32_t)t, r = mdiv_q(tt, L.d_h1), rk = mdiv_q(r, L.d_n2p1);
+  int i2 = (int)(tt - r * (uint32_t)ni2);
+  int j = 1 + (int)(r - rk * (uint32_t)(g.n2 + 1));
+  int k = 1 + (int)rk;
   int ks = (k % 2 == 0) ? 0 : 1;
   int is;
   if (colour == 0) is = (j % 2 == 0) ? 1 : 2;
@@ -137,7 +169,8 @@ __device__ inline void per_y_node(const KLevelT<T> &L, int64_t t) {
   const KGrid &g = L.g;
   int64_t ni = g.n1 + 3, nk = g.n3 + 3;
   if (t >= ni * nk) return;
-  int i = (int)(t % ni), k = (int)(t / ni);
+  const uint32_t tt = (uint32_t)t, kk = mdiv_q(tt, L.d_n1p3);
+  int i = (int)(tt - kk * (uint32_t)ni), k = (int)kk;
   int b1 = L.bc[gidx(g, i, 1, k)], b2 = L.bc[gidx(g, i, g.n2 + 1, k)];
   if (b1 == 0 || b1 == -2) L.u[gidx(g, i, 0, k)] = L.u[gidx(g, i, g.n2, k)];
   if (b2 == 0 || b2 == -2) L.u[gidx(g, i, g.n2 + 1, k)] = L.u[gidx(g, i, 1, k)];
@@ -148,7 +181,8 @@ __device__ inline void per_z_node(const KLevelT<T> &L, int64_t t) {
   const KGrid &g = L.g;
   int64_t ni = g.n1 + 3, nj = g.n2 + 3;
   if (t >= ni * nj) return;
-  int i = (int)(t % ni), j = (int)(t / ni);
+  const uint32_t tt = (uint32_t)t, jj = mdiv_q(tt, L.d_n1p3);
+  int i = (int)(tt - jj * (uint32_t)ni), j = (int)jj;
   const int m = g.n3;
   int b1 = L.bc[gidx(g, i, j, 1)], b2 = L.bc[gidx(g, i, j, m + 1)];
   if (b1 == 0 || b1 == -2) L.u[gidx(g, i, j, 0)] = L.u[gidx(g, i, j, m)];
@@ -175,10 +209,10 @@ __device__ inline double resid_node(const KLevelT<T> &L, int64_t t) {
   int64_t nx = g.n1 + 1, ny = g.n2 + 1, nz = g.n3 + 1;
   double a = 0.0;
   if (t < nx * ny * nz) {
-    int i = 1 + (int)(t % nx);
-    int64_t r = t / nx;
-    int j = 1 + (int)(r % ny);
-    int k = 1 + (int)(r / ny);
+    const uint32_t tt = (uint32_t)t, r = mdiv_q(tt, L.d_n1p1), rk = mdiv_q(r, L.d_n2p1);
+    int i = 1 + (int)(tt - r * (uint32_t)nx);
+    int j = 1 + (int)(r - rk * (uint32_t)ny);
+    int k = 1 + (int)rk;
     const int64_t c = gidx(g, i, j, k);
     if (L.bc[c] == -1) {
       const T *u = L.u;
@@ -256,7 +290,8 @@ __device__ inline void rpbc_entry(const KLevelT<T> &F, T *rp, int flag_nmn, int6
   const int64_t rs = g.n1 + 2;
   const int64_t total = rs * (g.n2 + 2);
   if (t >= total) return;
-  const int a = (int)(t % rs), b = (int)(t / rs);
+  const uint32_t tt = (uint32_t)t, bb = mdiv_q(tt, F.d_n1p2);
+  const int a = (int)(tt - bb * (uint32_t)rs), b = (int)bb;
   const int m = g.n3, N1 = g.n1 + 1, N2 = g.n2 + 1;
   T val = T(0);
   bool found = false;
@@ -299,10 +334,10 @@ __device__ inline void restrict_node(const KLevelT<T> &F, const KLevelT<T> &C, c
   const int kp = m / 2 + 1;
   int64_t ni = g.n1 / 2 + 1, nj = g.n2 / 2 + 1;
   if (t >= ni * nj * kp) return;
-  int i = 1 + (int)(t % ni);
-  int64_t r = t / ni;
-  int j = 1 + (int)(r % nj);
-  int k = 1 + (int)(r / nj);
+  const uint32_t tt = (uint32_t)t, r = mdiv_q(tt, F.d_h1), rk = mdiv_q(r, F.d_h2);
+  int i = 1 + (int)(tt - r * (uint32_t)ni);
+  int j = 1 + (int)(r - rk * (uint32_t)nj);
+  int k = 1 + (int)rk;
   const int ir = 2 * i - 1, jr = 2 * j - 1, kr = 2 * k - 1;
   const int N1 = g.n1 + 1, N2 = g.n2 + 1, M1 = m + 1;
   const int32_t *bc = F.bc;
@@ -435,10 +470,10 @@ __device__ inline void prolong_node(const KLevelT<T> &F, const KLevelT<T> &C, in
   const int kp = g.n3 / 2 + 1;
   int64_t ni = g.n1 / 2 + 1, nj = g.n2 / 2 + 1;
   if (t >= ni * nj * kp) return;
-  int i = 1 + (int)(t % ni);
-  int64_t r = t / ni;
-  int j = 1 + (int)(r % nj);
-  int k = 1 + (int)(r / nj);
+  const uint32_t tt = (uint32_t)t, r = mdiv_q(tt, F.d_h1), rk = mdiv_q(r, F.d_h2);
+  int i = 1 + (int)(tt - r * (uint32_t)ni);
+  int j = 1 + (int)(r - rk * (uint32_t)nj);
+  int k = 1 + (int)rk;
   const T *e = C.u;
   T r000 = e[gidx(gc, i, j, k)], r0m0 = e[gidx(gc, i, j - 1, k)],
          rmm0 = e[gidx(gc, i - 1, j - 1, k)], rm00 = e[gidx(gc, i - 1, j, k)],
@@ -852,6 +887,8 @@ extern "C" int lepic_poisson_solve(lepic_ctx *ctx, const lepic_grid *g, double *
   if (!g || !phi || !rho || !bcnd || !mp) return lepic_fail(ctx, LEPIC_ERR_ARG, "poisson: null argument");
   const int ng = mp->ng;
   if (ng < 1 || ng > MAX_LEVELS) return lepic_fail(ctx, LEPIC_ERR_ARG, "poisson: ng outside [1,12]");
+  if (grid_nodes(g) > INT32_MAX)   // thread numbers are decoded in 32 bits
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "poisson: more than 2^31 nodes");
   Solver<double> S;
   S.ctx = ctx;
   S.ng = ng;
@@ -874,6 +911,7 @@ extern "C" int lepic_poisson_solve(lepic_ctx *ctx, const lepic_grid *g, double *
     if (gl.n[0] < 1 || gl.n[1] < 1 || gl.n[2] < 1)
       return lepic_fail(ctx, LEPIC_ERR_ARG, "poisson: too many levels for this grid");
     S.L[l].g = make_kgrid(&gl);
+    set_level_divs(S.L[l]);
     set_coeffs(S.L[l], mp->eps0);
     if (l > 0) coarse_nodes += grid_nodes(&gl);
     int64_t nres = (int64_t)(gl.n[0] + 1) * (gl.n[1] + 1) * (gl.n[2] + 1);
@@ -1097,6 +1135,8 @@ extern "C" int lepic_poisson_solve_mixed(lepic_ctx *ctx, const lepic_grid *g, do
   if (xp->inner_cycles < 1 || xp->inner_cycles > 64 || xp->nu < 1 || xp->nu > 16 ||
       xp->nu_coarsest < 1 || xp->nu_coarsest > 100000 || !(xp->omega > 0.0 && xp->omega < 2.0))
     return lepic_fail(ctx, LEPIC_ERR_ARG, "poisson_mixed: inner V-cycle parameters out of range");
+  if (grid_nodes(g) > INT32_MAX)   // thread numbers are decoded in 32 bits
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "poisson_mixed: more than 2^31 nodes");
   Solver<float> S;
   S.ctx = ctx;
   S.ng = ng;
@@ -1122,9 +1162,11 @@ extern "C" int lepic_poisson_solve_mixed(lepic_ctx *ctx, const lepic_grid *g, do
     if (gl.n[0] < 1 || gl.n[1] < 1 || gl.n[2] < 1)
       return lepic_fail(ctx, LEPIC_ERR_ARG, "poisson_mixed: too many levels for this grid");
     S.L[l].g = make_kgrid(&gl);
+    set_level_divs(S.L[l]);
     set_coeffs(S.L[l], mp->eps0);
     if (l == 0) {
       F64.g = S.L[0].g;
+      set_level_divs(F64);
       set_coeffs(F64, mp->eps0);
     } else {
       coarse_nodes += grid_nodes(&gl);

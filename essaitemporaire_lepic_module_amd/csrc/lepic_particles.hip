@@ -380,6 +380,7 @@ struct DepSharedT {
   static_assert(CUT_X * CUT_YZ * CUT_YZ <= NB, "cut box exceeds the box");
   int mn[3], mx[3];
   int x0, y0, z0, bx, by, bz, vol;
+  uint64_t mbx, mby;   // small_div multipliers of bx and by
   double box[NB];
 };
 using DepShared = DepSharedT<LDS_BOX>;
@@ -463,6 +464,8 @@ __device__ inline void box_begin(const KGrid &g, DS &ds, int lo[3], int hi[3], c
       }
       ds.x0 = a[0]; ds.y0 = a[1]; ds.z0 = a[2];
       ds.bx = e[0]; ds.by = e[1]; ds.bz = e[2];
+      ds.mbx = small_div_magic((uint32_t)e[0]);
+      ds.mby = small_div_magic((uint32_t)e[1]);
       vol = e[0] * e[1] * e[2];
     }
     ds.vol = vol;
@@ -512,7 +515,8 @@ __device__ inline void box_flush(const KGrid &g, DS &ds, double *rep) {
   for (int t = threadIdx.x; t < ds.vol; t += BLOCK) {
     double v = ds.box[t];
     if (v != 0.0) {
-      int lx = t % ds.bx, r = t / ds.bx, ly = r % ds.by, lz = r / ds.by;
+      const int r = (int)small_div((uint32_t)t, ds.mbx), lz = (int)small_div((uint32_t)r, ds.mby);
+      const int lx = t - r * ds.bx, ly = r - lz * ds.by;
       int64_t node = gidx(g, ds.x0 + lx, ds.y0 + ly, ds.z0 + lz);
       l2_atomic_add(&rep[node], v);
     }
@@ -833,9 +837,12 @@ template <int BM>
 __global__ __launch_bounds__(BLOCK) void k_push(
     KGrid g, KPset p, int64_t n, const double4 *__restrict__ E4, KBField bf, double k1, double dt,
     const int *__restrict__ wt, KBC bp, double kdep,
-    double *rep_base, int64_t nodes, int do_dep, double *p_mac, double *P_loss,
+    double *rep_base, int64_t nodes, int do_dep, double *p_mac, double *wdrep,
     unsigned long long *lost_ctr) {
   __shared__ DepSharedPush ds;
+  __shared__ WallDiagShared wd;
+  wd_begin(wd);
+  __syncthreads();
   double *rep = rep_base ? rep_base + (int64_t)xcc_id() * nodes : nullptr;
   const int64_t i0 = (int64_t)blockIdx.x * CTILE + (int64_t)threadIdx.x * CPT;
   const bool full = i0 + CPT <= n;
@@ -914,7 +921,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(
           run_add(g, ds, ra, stage_key(g, ds, ix, iy, iz), kdep, px, py, pz, rep);
       }
     }
-    wall_diag_wave(bp, lost, label, vx[q], vy[q], vz[q], p_mac, P_loss);
+    if (lost) wd_lost(wd, bp, label, vx[q], vy[q], vz[q], p_mac);
     lostq[q] = lost;
   }
   store_run(p, n, i0, full, x, y, z, vx, vy, vz, live, lostq);
@@ -922,6 +929,7 @@ __global__ __launch_bounds__(BLOCK) void k_push(
     flush_run(g, ds, ra.cur, ra.a, rep);
     box_flush(g, ds, rep);
   }
+  wd_finish(wd, wdrep);
   // losses are rare: one atomic per wave that saw any
   int tot = lost_local;
   for (int d = 32; d > 0; d >>= 1) tot += __shfl_xor(tot, d);
@@ -1143,6 +1151,7 @@ struct DepSharedFixed {
   static constexpr int CUT_YZ = 10;
   int mn[3], mx[3];
   int x0, y0, z0, bx, by, bz, vol;
+  uint64_t mbx, mby;   // small_div multipliers of bx and by
   unsigned long long box[LDS_BOX];
 };
 
@@ -1215,7 +1224,8 @@ __global__ __launch_bounds__(BLOCK) void k_deposit_fixed(KGrid g, KPset p, int64
   for (int t = threadIdx.x; t < ds.vol; t += BLOCK) {
     const unsigned long long v = ds.box[t];
     if (v != 0) {
-      const int lx = t % ds.bx, r = t / ds.bx, ly = r % ds.by, lz = r / ds.by;
+      const int r = (int)small_div((uint32_t)t, ds.mbx), lz = (int)small_div((uint32_t)r, ds.mby);
+      const int lx = t - r * ds.bx, ly = r - lz * ds.by;
       const int64_t node = gidx(g, ds.x0 + lx, ds.y0 + ly, ds.z0 + lz);
       __hip_atomic_fetch_add(&rep[node], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
@@ -1755,15 +1765,18 @@ int lepic_push_bc_deposit_inplace(lepic_ctx *ctx, const lepic_grid *g, lepic_pse
       switch (bmode) {
         case 0: hipLaunchKernelGGL((k_push<0>), dim3(grid_blocks(p->n, CTILE)), dim3(BLOCK), 0, ctx->stream, kg,
                          kp(p), p->n, reinterpret_cast<const double4 *>(E4), kbf, k1, dt, bcnd,
-                         kbc(bp), kdep, rep, nodes, 1, p_mac, P_loss, ctr); break;
+                         kbc(bp), kdep, rep, nodes, 1, p_mac, ctx->wdiag, ctr); break;
         case 1: hipLaunchKernelGGL((k_push<1>), dim3(grid_blocks(p->n, CTILE)), dim3(BLOCK), 0, ctx->stream, kg,
                          kp(p), p->n, reinterpret_cast<const double4 *>(E4), kbf, k1, dt, bcnd,
-                         kbc(bp), kdep, rep, nodes, 1, p_mac, P_loss, ctr); break;
+                         kbc(bp), kdep, rep, nodes, 1, p_mac, ctx->wdiag, ctr); break;
         default: hipLaunchKernelGGL((k_push<2>), dim3(grid_blocks(p->n, CTILE)), dim3(BLOCK), 0, ctx->stream, kg,
                          kp(p), p->n, reinterpret_cast<const double4 *>(E4), kbf, k1, dt, bcnd,
-                         kbc(bp), kdep, rep, nodes, 1, p_mac, P_loss, ctr); break;
+                         kbc(bp), kdep, rep, nodes, 1, p_mac, ctx->wdiag, ctr); break;
       }
       LEPIC_LAUNCH_CHECK(ctx, "k_push");
+      hipLaunchKernelGGL(k_wd_flush, dim3(1), dim3(64), 0, ctx->stream, ctx->wdiag, p_mac, P_loss,
+                         std::min(bp->ngrid + 1, LEPIC_WD_LABELS));
+      LEPIC_LAUNCH_CHECK(ctx, "k_wd_flush");
     } else {
       switch (bmode) {
         case 0: hipLaunchKernelGGL((k_push_nodep<0>), dim3(grid_blocks(p->n, BLOCK)), dim3(BLOCK), 0,

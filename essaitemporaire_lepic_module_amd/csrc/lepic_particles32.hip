@@ -24,7 +24,7 @@ namespace {
 constexpr int BLOCK = 256;
 constexpr int CPT = 4;              // particles per lane, moved with 16-byte accesses
 constexpr int CTILE = BLOCK * CPT;
-constexpr int BOX32 = 4096;         // int64 nodes of the per-block deposit box (32 KB)
+constexpr int BOX32 = 4000;   // int64 nodes of the per-block deposit box (with its header and the wall sums, 32 KB of LDS: 5 blocks per CU)
 
 struct KP32 {
   int32_t *cell;
@@ -214,13 +214,17 @@ __device__ inline void reflect(int &ix, float &f) {
 // periodic y and z).  A survivor exactly on y = 0 or z = 0 stays there (the
 // reference moves it to the far face y = ymax, which is no cell of the
 // offset layout).
-__device__ inline int wall32(const KG32 &g, const int *__restrict__ wt, const KBC &bp, int &ix,
-                             int &iy, int &iz, float &fx, float &fy, float &fz, float &vx,
-                             int &label) {
+__device__ inline int wall_entry32(const KG32 &g, const int *__restrict__ wt, int ix, int iy,
+                                   int iz) {
   const int cx = imax(0, imin(g.n1 + 1, ix));
   const int cy = imax(0, imin(g.n2 + 1, iy));
   const int cz = imax(0, imin(g.n3 + 1, iz));
-  const int c = wt[node_of(g, cx, cy, cz)];
+  return wt[node_of(g, cx, cy, cz)];
+}
+
+// c: the wall table's entry at the clamped lower node (wall_entry32).
+__device__ inline int wall32(const KG32 &g, int c, const KBC &bp, int &ix, int &iy, int &iz,
+                             float &fx, float &fy, float &fz, float &vx, int &label) {
   int lost = c >= 0 ? 1 : 0;
   label = c >= 0 ? c : 0;
   if (bp.ptype == bp.tag_neg && ix <= 0 && bp.flag_nmn == 1) {   // x < 0
@@ -254,6 +258,7 @@ __device__ inline int wall32(const KG32 &g, const int *__restrict__ wt, const KB
 struct Box32 {
   int mn[3], mx[3], sum[3], cnt;
   int x0, y0, z0, bx, by, bz, vol;   // vol <= 0: no box (everything to the replicas)
+  uint64_t mbx, mby;                 // small_div multipliers of bx and by
   unsigned long long box[BOX32];
 };
 
@@ -315,6 +320,8 @@ __device__ inline void box32_begin(const KG32 &g, Box32 &ds, const int lo[3], co
       }
       ds.x0 = a[0]; ds.y0 = a[1]; ds.z0 = a[2];
       ds.bx = e[0]; ds.by = e[1]; ds.bz = e[2];
+      ds.mbx = small_div_magic((uint32_t)e[0]);
+      ds.mby = small_div_magic((uint32_t)e[1]);
       vol = e[0] * e[1] * e[2];
     }
     ds.vol = vol;
@@ -337,21 +344,36 @@ __device__ inline int key32(const KG32 &g, const Box32 &ds, int ix, int iy, int 
 
 // A run's 8 node sums (int32: a lane's run holds at most CPT particles of at
 // most 2^22 units) added to the box or, outside it, to this XCD's replica,
-// sign-extended into the int64 node sums.
-__device__ inline void flush32(const KG32 &g, Box32 &ds, int key, const int a[8],
+// sign-extended into the int64 node sums.  In the box, lane l starts at
+// stencil node l mod 8, so the lanes of one instruction that flush the same
+// cell (sorted data) hit different addresses; off[j] is the box offset of the
+// lane's j-th node (box32_offsets), and the sums are rotated to match by three
+// conditional shifts.
+__device__ inline void box32_offsets(const Box32 &ds, int off[8]) {
+  const int rot = __lane_id() & 7;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) off[j] = node_off((j + rot) & 7, ds.bx, ds.bx * ds.by);
+}
+
+__device__ inline void flush32(const KG32 &g, Box32 &ds, int key, const int a[8], const int off[8],
                                unsigned long long *rep) {
   if (key >= 0) {
-    const int bx = ds.bx, bxy = ds.bx * ds.by;
-    const int rot = __lane_id() & 7;   // neighbouring lanes start on different nodes
+    const int rot = __lane_id() & 7;
+    int b[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int q = (j + rot) & 7;
-      int v = a[0];
+    for (int j = 0; j < 8; ++j) b[j] = a[j];
 #pragma unroll
-      for (int t = 1; t < 8; ++t) v = (q == t) ? a[t] : v;
-      if (v != 0)
-        atomicAdd(&ds.box[key + node_off(q, bx, bxy)], (unsigned long long)(long long)v);
+    for (int stage = 1; stage < 8; stage <<= 1) {   // b[j] <- b[(j + stage) & 7] where bit is set
+      const bool take = (rot & stage) != 0;
+      int t[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) t[j] = b[(j + stage) & 7];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) b[j] = take ? t[j] : b[j];
     }
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (b[j] != 0) atomicAdd(&ds.box[key + off[j]], (unsigned long long)(long long)b[j]);
   } else if (key <= -2) {
     const int64_t b = (int64_t)(-(key + 2));
     const int64_t nd[8] = {b, b + 1, b + g.sy + 1, b + g.sy, b + g.sz, b + g.sz + 1,
@@ -369,7 +391,8 @@ __device__ inline void box32_flush(const KG32 &g, Box32 &ds, unsigned long long 
   for (int t = threadIdx.x; t < ds.vol; t += BLOCK) {
     const unsigned long long v = ds.box[t];
     if (v != 0ull) {
-      const int lx = t % ds.bx, r = t / ds.bx, ly = r % ds.by, lz = r / ds.by;
+      const int r = (int)small_div((uint32_t)t, ds.mbx), lz = (int)small_div((uint32_t)r, ds.mby);
+      const int lx = t - r * ds.bx, ly = r - lz * ds.by;
       __hip_atomic_fetch_add(&rep[node_of(g, ds.x0 + lx, ds.y0 + ly, ds.z0 + lz)], v,
                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
@@ -380,6 +403,7 @@ __device__ inline void box32_flush(const KG32 &g, Box32 &ds, unsigned long long 
 struct Run32 {
   int cur;
   int a[8];
+  int off[8];   // box offsets of the lane's rotated stencil (box32_offsets)
   __device__ void reset() {
     cur = -1;
 #pragma unroll
@@ -388,7 +412,7 @@ struct Run32 {
   __device__ void add(const KG32 &g, Box32 &ds, int key, float U, int neg, float fx, float fy,
                       float fz, unsigned long long *rep) {
     if (key != cur) {
-      flush32(g, ds, cur, a, rep);
+      flush32(g, ds, cur, a, off, rep);
       reset();
       cur = key;
     }
@@ -456,13 +480,18 @@ __device__ inline void store32(const KP32 &p, int64_t n, int64_t i0, bool full, 
 // The FP32 particle phase in place: gather + push + wall BC (+ fixed-point
 // deposit).  Lost particles become holes (cell = -1).  ctr[0] counts losses,
 // ctr[1] survivors that left the cells (an error the host reports).
+// 5 waves per SIMD: at most 96 VGPRs and 32 KB of LDS per block (4 with a
+// B map, whose gather needs more registers).
 template <int BM>
-__global__ __launch_bounds__(BLOCK) void k_push32(
+__global__ __launch_bounds__(BLOCK, BM == 2 ? 4 : 5) void k_push32(
     KG32 g, KP32 p, int64_t n, const float4 *__restrict__ E4, KBF32 bf, float k1,
     const int *__restrict__ wt, KBC bp, float U, int neg, unsigned long long *rep_base,
-    int64_t nodes, int do_dep, double *p_mac, double *P_loss, unsigned long long *lost_ctr,
+    int64_t nodes, int do_dep, double *p_mac, double *wdrep, unsigned long long *lost_ctr,
     unsigned long long *esc_ctr) {
   __shared__ Box32 ds;
+  __shared__ WallDiagShared wd;
+  wd_begin(wd);
+  __syncthreads();
   unsigned long long *rep = do_dep ? rep_base + (int64_t)xcc_id() * nodes : nullptr;
   const int64_t i0 = (int64_t)blockIdx.x * CTILE + (int64_t)threadIdx.x * CPT;
   const bool full = i0 + CPT <= n;
@@ -487,16 +516,20 @@ __global__ __launch_bounds__(BLOCK) void k_push32(
       ++cnt;
     }
   }
-  if (do_dep) box32_begin(g, ds, lo, hi, sm, cnt);
   Run32 ra;
   ra.reset();
+  if (do_dep) {
+    box32_begin(g, ds, lo, hi, sm, cnt);
+    box32_offsets(ds, ra.off);
+  }
   int64_t eb = -1;
   float4 f1, f2, f3, f4, f5, f6, f7, f8;
   int lost_local = 0, esc_local = 0;
+  int nx[CPT], ny[CPT], nz[CPT];
+  // gather + velocity update + move of the lane's particles
 #pragma unroll
   for (int q = 0; q < CPT; ++q) {
-    bool lost = false;
-    int label = 0;
+    nx[q] = ny[q] = nz[q] = 0;
     if (c[q] >= 0) {
       float w[8];
       weights32(fx[q], fy[q], fz[q], w);
@@ -510,10 +543,22 @@ __global__ __launch_bounds__(BLOCK) void k_push32(
       float Ex, Ey, Ez;
       sum8(w, f1, f2, f3, f4, f5, f6, f7, f8, Ex, Ey, Ez);
       velocity_update32<BM>(g, bf, w, base, k1, Ex, Ey, Ez, vx[q], vy[q], vz[q]);
-      int nx = ix[q] + advance(fx[q], vx[q] * g.dtx);
-      int ny = iy[q] + advance(fy[q], vy[q] * g.dty);
-      int nz = iz[q] + advance(fz[q], vz[q] * g.dtz);
-      const int r = wall32(g, wt, bp, nx, ny, nz, fx[q], fy[q], fz[q], vx[q], label);
+      nx[q] = ix[q] + advance(fx[q], vx[q] * g.dtx);
+      ny[q] = iy[q] + advance(fy[q], vy[q] * g.dty);
+      nz[q] = iz[q] + advance(fz[q], vz[q] * g.dtz);
+    }
+  }
+  // the wall table at the new lower nodes, all loads issued together
+  int wc[CPT];
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) wc[q] = c[q] >= 0 ? wall_entry32(g, wt, nx[q], ny[q], nz[q]) : -1;
+  // wall BC, new cell, deposit
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) {
+    bool lost = false;
+    int label = 0;
+    if (c[q] >= 0) {
+      const int r = wall32(g, wc[q], bp, nx[q], ny[q], nz[q], fx[q], fy[q], fz[q], vx[q], label);
       if (r == 1 || r == 2) {
         lost = true;
         ++lost_local;
@@ -522,17 +567,19 @@ __global__ __launch_bounds__(BLOCK) void k_push32(
         ++esc_local;
         c[q] = -1;
       } else {
-        c[q] = (nx - 1) + g.n1 * ((ny - 1) + g.n2 * (nz - 1));
-        if (do_dep) ra.add(g, ds, key32(g, ds, nx, ny, nz), U, neg, fx[q], fy[q], fz[q], rep);
+        c[q] = (nx[q] - 1) + g.n1 * ((ny[q] - 1) + g.n2 * (nz[q] - 1));
+        if (do_dep)
+          ra.add(g, ds, key32(g, ds, nx[q], ny[q], nz[q]), U, neg, fx[q], fy[q], fz[q], rep);
       }
     }
-    wall_diag_wave(bp, lost, label, (double)vx[q], (double)vy[q], (double)vz[q], p_mac, P_loss);
+    if (lost) wd_lost(wd, bp, label, (double)vx[q], (double)vy[q], (double)vz[q], p_mac);
   }
   store32(p, n, i0, full, c, fx, fy, fz, vx, vy, vz);
   if (do_dep) {
-    flush32(g, ds, ra.cur, ra.a, rep);
+    flush32(g, ds, ra.cur, ra.a, ra.off, rep);
     box32_flush(g, ds, rep);
   }
+  wd_finish(wd, wdrep);
   int tot = lost_local, esc = esc_local;
   for (int d = 32; d > 0; d >>= 1) {
     tot += __shfl_xor(tot, d);
@@ -574,11 +621,12 @@ __global__ __launch_bounds__(BLOCK) void k_deposit32(KG32 g, KP32 p, int64_t n, 
   box32_begin(g, ds, lo, hi, sm, cnt);
   Run32 ra;
   ra.reset();
+  box32_offsets(ds, ra.off);
 #pragma unroll
   for (int q = 0; q < CPT; ++q)
     if (c[q] >= 0)
       ra.add(g, ds, key32(g, ds, ix[q], iy[q], iz[q]), U, neg, fx[q], fy[q], fz[q], rep);
-  flush32(g, ds, ra.cur, ra.a, rep);
+  flush32(g, ds, ra.cur, ra.a, ra.off, rep);
   box32_flush(g, ds, rep);
 }
 
@@ -1128,15 +1176,18 @@ int lepic_push32(lepic_ctx *ctx, const lepic_grid *g, lepic_pset32 *p, const flo
   switch (bmode) {
     case 0: hipLaunchKernelGGL((k_push32<0>), grid, block, 0, ctx->stream, kg, kp32(p), p->n,
                                reinterpret_cast<const float4 *>(E4f), kbf, k1, wt, kbc(bp), (float)U, neg,
-                               rep, nodes, dd, p_mac, P_loss, lc, ec); break;
+                               rep, nodes, dd, p_mac, ctx->wdiag, lc, ec); break;
     case 1: hipLaunchKernelGGL((k_push32<1>), grid, block, 0, ctx->stream, kg, kp32(p), p->n,
                                reinterpret_cast<const float4 *>(E4f), kbf, k1, wt, kbc(bp), (float)U, neg,
-                               rep, nodes, dd, p_mac, P_loss, lc, ec); break;
+                               rep, nodes, dd, p_mac, ctx->wdiag, lc, ec); break;
     default: hipLaunchKernelGGL((k_push32<2>), grid, block, 0, ctx->stream, kg, kp32(p), p->n,
                                 reinterpret_cast<const float4 *>(E4f), kbf, k1, wt, kbc(bp), (float)U, neg,
-                                rep, nodes, dd, p_mac, P_loss, lc, ec); break;
+                                rep, nodes, dd, p_mac, ctx->wdiag, lc, ec); break;
   }
   LEPIC_LAUNCH_CHECK(ctx, "k_push32");
+  hipLaunchKernelGGL(k_wd_flush, dim3(1), dim3(64), 0, ctx->stream, ctx->wdiag, p_mac, P_loss,
+                     std::min(bp->ngrid + 1, LEPIC_WD_LABELS));
+  LEPIC_LAUNCH_CHECK(ctx, "k_wd_flush");
   if (dep) {
     hipLaunchKernelGGL(k_rep_flush_i64, dim3(grid_blocks(nodes, 256)), dim3(256), 0, ctx->stream,
                        reinterpret_cast<long long *>(dep), rep, nodes);

@@ -78,6 +78,81 @@ __device__ inline void wall_diag_wave(const KBC &bp, bool lost, int label, doubl
   }
 }
 
+// Wall diagnostics summed per block (the production pushes): each lost
+// particle adds its p_mac and P_loss terms to the block's LDS sums (labels
+// below WD_LABELS; higher labels go straight to p_mac), and the block adds
+// its sums once into this XCD's replica of the diagnostics with L2-scope
+// atomics; k_wd_flush adds the replicas into p_mac / P_loss and clears them.
+// (Per-wave device-scope atomics on p_mac / P_loss serialised on those few
+// words and dominated the electron push, which loses ~10^5 particles per
+// step at 128^3.)
+constexpr int WD_LABELS = LEPIC_WD_LABELS;
+constexpr int WD_STRIDE = 2 * WD_LABELS + 1;   // p_mac pairs, then P_loss
+
+struct WallDiagShared {
+  double s[WD_STRIDE];
+};
+
+// Zeroes the block's sums; a __syncthreads must follow before wd_lost.
+__device__ inline void wd_begin(WallDiagShared &wd) {
+  for (int t = threadIdx.x; t < WD_STRIDE; t += blockDim.x) wd.s[t] = 0.0;
+}
+
+// One lost particle (mod_particleBC.f90:195-206): p_mac(1,label) += Nm q,
+// p_mac(2,label) += |Nm q| Ek[eV], P_loss += Nm Ek[J].
+__device__ inline void wd_lost(WallDiagShared &wd, const KBC &bp, int label, double vx, double vy,
+                               double vz, double *p_mac) {
+  const double v2 = vx * vx + vy * vy + vz * vz;
+  const double Ek_eV = 0.5 * bp.mass * v2 / 1.60217646e-19;
+  const double Ek_J = 0.5 * bp.mass * v2;
+  atomicAdd(&wd.s[2 * WD_LABELS], bp.Nm * Ek_J);
+  if (label > bp.ngrid) return;
+  if (label < WD_LABELS) {
+    atomicAdd(&wd.s[2 * label + 0], bp.qmacro);
+    atomicAdd(&wd.s[2 * label + 1], fabs(bp.qmacro) * Ek_eV);
+  } else {
+    atomicAdd(&p_mac[2 * label + 0], bp.qmacro);
+    atomicAdd(&p_mac[2 * label + 1], fabs(bp.qmacro) * Ek_eV);
+  }
+}
+
+// The block's sums into this XCD's replica (rep: LEPIC_XCDS x WD_STRIDE).
+// Every thread calls it after its last wd_lost.
+__device__ inline void wd_finish(WallDiagShared &wd, double *rep) {
+  __syncthreads();
+  double *r = rep + (int64_t)xcc_id() * WD_STRIDE;
+  for (int t = threadIdx.x; t < WD_STRIDE; t += blockDim.x) {
+    const double v = wd.s[t];
+    if (v != 0.0) l2_atomic_add(&r[t], v);
+  }
+}
+
+__global__ void k_wd_flush(double *rep, double *p_mac, double *P_loss, int npairs) {
+  const int t = threadIdx.x;
+  if (t >= WD_STRIDE) return;
+  double v = 0.0;
+  for (int x = 0; x < LEPIC_XCDS; ++x) {
+    v += rep[x * WD_STRIDE + t];
+    rep[x * WD_STRIDE + t] = 0.0;
+  }
+  if (t == 2 * WD_LABELS) {
+    if (v != 0.0) P_loss[0] += v;
+  } else if (t < 2 * npairs && v != 0.0) {
+    p_mac[t] += v;
+  }
+}
+
+// floor(t / d) for t, d < 2^16 by one multiply with m = ceil(2^32 / d): with
+// m d = 2^32 + e, 0 <= e < d, t m / 2^32 = t / d + t e / (d 2^32) and
+// t e < t d < 2^32, so the floor is t's quotient.  Used to decode box-local
+// node numbers without a division per node.
+__host__ __device__ inline uint64_t small_div_magic(uint32_t d) {
+  return ((1ull << 32) + d - 1) / d;
+}
+__device__ inline uint32_t small_div(uint32_t t, uint64_t m) {
+  return (uint32_t)(((uint64_t)t * m) >> 32);
+}
+
 // Offset of stencil node q (the reference's order w1..w8) from the lower node.
 __device__ inline int node_off(int q, int bx, int bxy) {
   const int r = q & 3;

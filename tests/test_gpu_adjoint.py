@@ -286,7 +286,7 @@ def test_gradient_matches_pure_torch_16cube(gpu, steps, checkpoint):
     sim = DiffPIC(case, parts, "cuda:0", checkpoint=checkpoint)
     L = sim.forward(20.0, 0.005, steps, keep=True, record_deaths=True)
     dV, dB = sim.backward()
-    sched = [d.cpu()[:len(p["x"])] for d, p in zip(sim.deaths, parts)]
+    sched = [torch.as_tensor(sim.by_input(s, d)) for s, d in enumerate(sim.deaths)]
     sp = [(s.charge, s.mass, nm) for s, nm in zip(case.tracked, case.Nm)]
     model = TorchPIC(case.n, case.h, case.bcnd, 2, sp, case.dt)
     xs = [torch.tensor(np.stack([p["x"], p["y"], p["z"]], 1)) for p in parts]
