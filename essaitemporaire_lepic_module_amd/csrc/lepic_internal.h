@@ -236,7 +236,9 @@ __host__ __device__ inline void fixed_split(double U, double ux, double uy, doub
 // FP32 with two fraction bits to spare, rint makes it an integer, and every
 // difference of integers below 2^23 is exact, so the 8 parts are integers
 // (int32) that sum to U exactly, at a fraction of the FP64 split's cost.
-#define LEPIC_FIXED32_MAX 4194304.0   // 2^22 units per full weight
+// The FP32 kernels take U up to 2^21 units, so that one block's sums at a node
+// (1024 particles at most) fit the uint32 of their LDS box.
+#define LEPIC_FIXED32_MAX 2097152.0   // 2^21 units per full weight
 __host__ __device__ inline void fixed_split32(float U, float ux, float uy, float uz, int a[8]) {
   const float x1 = rintf(U * ux), x0 = U - x1;
   const float x0y1 = rintf(x0 * uy), x0y0 = x0 - x0y1;

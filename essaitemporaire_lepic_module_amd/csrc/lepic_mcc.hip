@@ -987,7 +987,7 @@ static int collisions_impl(lepic_ctx *ctx, const lepic_grid *g, MccSets sets,
         const double unit = std::ldexp(std::fabs(k1), sets.frac_bits[t]);
         if (k1 != 0.0 && !(unit >= 1.0 && unit <= LEPIC_FIXED32_MAX))
           return lepic_fail(ctx, LEPIC_ERR_ARG,
-                            "collisions32: |Nm/(h1 h2 h3)| * 2^frac_bits must lie in [1, 2^22]");
+                            "collisions32: |Nm/(h1 h2 h3)| * 2^frac_bits must lie in [1, 2^21]");
         P32.P.dep[t] = reinterpret_cast<long long *>(sets.dep + (int64_t)t * nodes);
         P32.P.U[t] = std::rint(unit);
         P32.P.neg[t] = k1 < 0.0 ? 1 : 0;

@@ -24,7 +24,7 @@ namespace {
 constexpr int BLOCK = 256;
 constexpr int CPT = 4;              // particles per lane, moved with 16-byte accesses
 constexpr int CTILE = BLOCK * CPT;
-constexpr int BOX32 = 4000;   // int64 nodes of the per-block deposit box (with its header and the wall sums, 32 KB of LDS: 5 blocks per CU)
+constexpr int BOX32 = 8000;   // uint32 nodes of the per-block deposit box (with its header and the wall sums, 32 KB of LDS: 5 blocks per CU)
 
 struct KP32 {
   int32_t *cell;
@@ -255,18 +255,23 @@ __device__ inline int wall32(const KG32 &g, int c, const KBC &bp, int &ix, int &
 // widened by one node each way for motion within the step) and per-XCD int64
 // replicas for stencils outside it.
 // ---------------------------------------------------------------------------
+// The box holds magnitudes: a species' parts all carry the sign of its
+// charge (neg), applied when the box goes out to the replicas.  A node of the
+// box sums at most one full weight per particle of the block, 1024 x 2^21 =
+// 2^31 units, so uint32 holds it, and LDS atomics are 32-bit.
 struct Box32 {
   int mn[3], mx[3], sum[3], cnt;
   int x0, y0, z0, bx, by, bz, vol;   // vol <= 0: no box (everything to the replicas)
+  int neg;
   uint64_t mbx, mby;                 // small_div multipliers of bx and by
-  unsigned long long box[BOX32];
+  uint32_t box[BOX32];
 };
 
 // Box extents (nodes) used when the block's full range does not fit BOX32:
 // a cell-sorted tile spans a row of cells along x and one cell in y and z,
-// so the box keeps 36 x 10 x 10 nodes around the tile's mean lower node and
+// so the box keeps 36 x 14 x 14 nodes around the tile's mean lower node and
 // the few particles that drifted beyond go to the replicas.
-constexpr int BOXLIM_X = 36, BOXLIM_Y = 10, BOXLIM_Z = 10;
+constexpr int BOXLIM_X = 36, BOXLIM_Y = 14, BOXLIM_Z = 14;
 static_assert(BOXLIM_X * BOXLIM_Y * BOXLIM_Z <= BOX32, "box limits exceed the box");
 
 // lo / hi: the lane's lower nodes (1-based) before the move; sm / cnt: the
@@ -276,8 +281,9 @@ static_assert(BOXLIM_X * BOXLIM_Y * BOXLIM_Z <= BOX32, "box limits exceed the bo
 // BOXLIM around the mean lower node, and particles outside it deposit into
 // the replicas.
 __device__ inline void box32_begin(const KG32 &g, Box32 &ds, const int lo[3], const int hi[3],
-                                   const int sm[3], int cnt) {
+                                   const int sm[3], int cnt, int neg) {
   if (threadIdx.x == 0) {
+    ds.neg = neg;
     ds.mn[0] = ds.mn[1] = ds.mn[2] = 1 << 30;
     ds.mx[0] = ds.mx[1] = ds.mx[2] = -(1 << 30);
     ds.sum[0] = ds.sum[1] = ds.sum[2] = 0;
@@ -327,7 +333,7 @@ __device__ inline void box32_begin(const KG32 &g, Box32 &ds, const int lo[3], co
     ds.vol = vol;
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < ds.vol; t += BLOCK) ds.box[t] = 0ull;
+  for (int t = threadIdx.x; t < ds.vol; t += BLOCK) ds.box[t] = 0u;
   __syncthreads();
 }
 
@@ -342,9 +348,9 @@ __device__ inline int key32(const KG32 &g, const Box32 &ds, int ix, int iy, int 
   return -(int)node_of(g, ix, iy, iz) - 2;
 }
 
-// A run's 8 node sums (int32: a lane's run holds at most CPT particles of at
-// most 2^22 units) added to the box or, outside it, to this XCD's replica,
-// sign-extended into the int64 node sums.  In the box, lane l starts at
+// A run's 8 node sums (magnitudes below 2^23: a lane's run holds at most CPT
+// particles of at most 2^21 units) added to the box or, outside it, with the
+// species' sign to this XCD's int64 replica.  In the box, lane l starts at
 // stencil node l mod 8, so the lanes of one instruction that flush the same
 // cell (sorted data) hit different addresses; off[j] is the box offset of the
 // lane's j-th node (box32_offsets), and the sums are rotated to match by three
@@ -372,16 +378,16 @@ __device__ inline void flush32(const KG32 &g, Box32 &ds, int key, const int a[8]
       for (int j = 0; j < 8; ++j) b[j] = take ? t[j] : b[j];
     }
 #pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if (b[j] != 0) atomicAdd(&ds.box[key + off[j]], (unsigned long long)(long long)b[j]);
+    for (int j = 0; j < 8; ++j) atomicAdd(&ds.box[key + off[j]], (uint32_t)b[j]);
   } else if (key <= -2) {
     const int64_t b = (int64_t)(-(key + 2));
     const int64_t nd[8] = {b, b + 1, b + g.sy + 1, b + g.sy, b + g.sz, b + g.sz + 1,
                            b + g.sz + g.sy + 1, b + g.sz + g.sy};
+    const bool neg = ds.neg != 0;
 #pragma unroll
     for (int q = 0; q < 8; ++q)
       if (a[q] != 0)
-        __hip_atomic_fetch_add(&rep[nd[q]], (unsigned long long)(long long)a[q],
+        __hip_atomic_fetch_add(&rep[nd[q]], (unsigned long long)(long long)(neg ? -a[q] : a[q]),
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
 }
@@ -389,8 +395,10 @@ __device__ inline void flush32(const KG32 &g, Box32 &ds, int key, const int a[8]
 __device__ inline void box32_flush(const KG32 &g, Box32 &ds, unsigned long long *rep) {
   __syncthreads();
   for (int t = threadIdx.x; t < ds.vol; t += BLOCK) {
-    const unsigned long long v = ds.box[t];
-    if (v != 0ull) {
+    const uint32_t m = ds.box[t];
+    if (m != 0u) {
+      const unsigned long long v =
+          (unsigned long long)(ds.neg ? -(long long)m : (long long)m);
       const int r = (int)small_div((uint32_t)t, ds.mbx), lz = (int)small_div((uint32_t)r, ds.mby);
       const int lx = t - r * ds.bx, ly = r - lz * ds.by;
       __hip_atomic_fetch_add(&rep[node_of(g, ds.x0 + lx, ds.y0 + ly, ds.z0 + lz)], v,
@@ -409,8 +417,8 @@ struct Run32 {
 #pragma unroll
     for (int q = 0; q < 8; ++q) a[q] = 0;
   }
-  __device__ void add(const KG32 &g, Box32 &ds, int key, float U, int neg, float fx, float fy,
-                      float fz, unsigned long long *rep) {
+  __device__ void add(const KG32 &g, Box32 &ds, int key, float U, float fx, float fy, float fz,
+                      unsigned long long *rep) {
     if (key != cur) {
       flush32(g, ds, cur, a, off, rep);
       reset();
@@ -419,7 +427,7 @@ struct Run32 {
     int parts[8];
     fixed_split32(U, fx, fy, fz, parts);
 #pragma unroll
-    for (int q = 0; q < 8; ++q) a[q] += neg ? -parts[q] : parts[q];
+    for (int q = 0; q < 8; ++q) a[q] += parts[q];
   }
 };
 
@@ -519,7 +527,7 @@ __global__ __launch_bounds__(BLOCK, BM == 2 ? 4 : 5) void k_push32(
   Run32 ra;
   ra.reset();
   if (do_dep) {
-    box32_begin(g, ds, lo, hi, sm, cnt);
+    box32_begin(g, ds, lo, hi, sm, cnt, neg);
     box32_offsets(ds, ra.off);
   }
   int64_t eb = -1;
@@ -569,7 +577,7 @@ __global__ __launch_bounds__(BLOCK, BM == 2 ? 4 : 5) void k_push32(
       } else {
         c[q] = (nx[q] - 1) + g.n1 * ((ny[q] - 1) + g.n2 * (nz[q] - 1));
         if (do_dep)
-          ra.add(g, ds, key32(g, ds, nx[q], ny[q], nz[q]), U, neg, fx[q], fy[q], fz[q], rep);
+          ra.add(g, ds, key32(g, ds, nx[q], ny[q], nz[q]), U, fx[q], fy[q], fz[q], rep);
       }
     }
     if (lost) wd_lost(wd, bp, label, (double)vx[q], (double)vy[q], (double)vz[q], p_mac);
@@ -618,14 +626,14 @@ __global__ __launch_bounds__(BLOCK) void k_deposit32(KG32 g, KP32 p, int64_t n, 
       ++cnt;
     }
   }
-  box32_begin(g, ds, lo, hi, sm, cnt);
+  box32_begin(g, ds, lo, hi, sm, cnt, neg);
   Run32 ra;
   ra.reset();
   box32_offsets(ds, ra.off);
 #pragma unroll
   for (int q = 0; q < CPT; ++q)
     if (c[q] >= 0)
-      ra.add(g, ds, key32(g, ds, ix[q], iy[q], iz[q]), U, neg, fx[q], fy[q], fz[q], rep);
+      ra.add(g, ds, key32(g, ds, ix[q], iy[q], iz[q]), U, fx[q], fy[q], fz[q], rep);
   flush32(g, ds, ra.cur, ra.a, ra.off, rep);
   box32_flush(g, ds, rep);
 }
@@ -1059,7 +1067,7 @@ int check_grid32(lepic_ctx *ctx, const lepic_grid *g, const char *who) {
   return LEPIC_OK;
 }
 
-// Full weight in units of 2^-S: rint(|Nm / (h1 h2 h3)| 2^S) within [1, 2^22]
+// Full weight in units of 2^-S: rint(|Nm / (h1 h2 h3)| 2^S) within [1, 2^21]
 // (the FP32 split's exact range).
 int fixed_unit(lepic_ctx *ctx, const lepic_grid *g, double Nm, int32_t frac_bits, double &U,
                int &neg, const char *who) {
@@ -1072,7 +1080,7 @@ int fixed_unit(lepic_ctx *ctx, const lepic_grid *g, double Nm, int32_t frac_bits
   const double unit = std::ldexp(std::fabs(k1), frac_bits);
   if (frac_bits < -1000 || frac_bits > 1000 || !(unit >= 1.0 && unit <= LEPIC_FIXED32_MAX))
     return lepic_fail(ctx, LEPIC_ERR_ARG,
-                      std::string(who) + ": |Nm/(h1 h2 h3)| * 2^frac_bits must lie in [1, 2^22]");
+                      std::string(who) + ": |Nm/(h1 h2 h3)| * 2^frac_bits must lie in [1, 2^21]");
   U = std::rint(unit);
   return LEPIC_OK;
 }

@@ -57,12 +57,13 @@ def fixed_point_bits(n, h, Nm_species: float) -> int:
 
 def fixed_point_bits32(n, h, Nm_species: float) -> int:
     """Fraction bits for the FP32 particle phase's deposit: a full weight
-    |Nm/(h1 h2 h3)| maps to at most 2^22 units, the range in which its FP32
-    split into 8 node parts is exact."""
+    |Nm/(h1 h2 h3)| maps to at most 2^21 units (its FP32 split into 8 node
+    parts is exact up to 2^22, and one block's sums at a node stay within
+    uint32)."""
     k1 = abs(float(Nm_species) / (h[0] * h[1] * h[2]))
     if k1 == 0.0:
         return 0
-    return 22 - math.ceil(math.log2(k1))
+    return 21 - math.ceil(math.log2(k1))
 
 
 def deposit_particle_set_to_np_thread_fixed(part: ParticleSet, n, h, kq: torch.Tensor,

@@ -448,10 +448,10 @@ int lepic_round_field4(lepic_ctx *ctx, const double *F4, int64_t nodes, float *F
  * (Src/part_expmover.f90:51-246) + apply_particle_bc_legacy's wall test,
  * losses, Neumann reflection and periodic wrap (mod_particleBC.f90:42-259)
  * + the deposit (mod_chargeDeposition.f90:25-133) in fixed point: every
- * survivor's full weight rint(|Nm/(h1 h2 h3)| 2^frac_bits) split into 8
- * integer node parts as lepic_deposit_fixed splits it (upper shares = the
- * new offsets) and added to dep (device int64, one per node, NULL: no
- * deposit).  Constants are formed in FP64 and rounded once: k1 = dt q/(2m),
+ * survivor's full weight rint(|Nm/(h1 h2 h3)| 2^frac_bits) (in [1, 2^21],
+ * else LEPIC_ERR_ARG) split into 8 integer node parts as lepic_deposit_fixed
+ * splits it (upper shares = the new offsets) and added to dep (device int64,
+ * one per node, NULL: no deposit).  Constants are formed in FP64 and rounded once: k1 = dt q/(2m),
  * dt/h per axis, the uniform-B rotation factor.  Lost particles become holes
  * (cell = -1) and are added to *lost_acc (device); a survivor left outside
  * the cells (no wall around the domain) also becomes a hole and is added to

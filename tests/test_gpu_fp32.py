@@ -19,7 +19,7 @@ AMU = 1.66053886e-27
 
 
 def _bits(n, h, Nm):
-    return 22 - math.ceil(math.log2(abs(Nm / (h[0] * h[1] * h[2]))))
+    return 21 - math.ceil(math.log2(abs(Nm / (h[0] * h[1] * h[2]))))
 
 
 def _pair(oracle, P32, rng, n, h, count, vscale=5e5, outside=0.0, dead=None):
