@@ -63,7 +63,24 @@ struct KLevelT {
   int32_t *bc;
   // thread-number decoders: n1/2+1, n2+1, n1+1, n1+3, n1+2, n2/2+1
   MDiv d_h1, d_n2p1, d_n1p1, d_n1p3, d_n1p2, d_h2;
+  double acy;   // RN(1 / ac) for the FP64 levels' division (0: plain division)
 };
+
+// x / ac of the SOR update: for FP64 levels by the reciprocal and two FMA
+// corrections (lepic_qdiv.h: the correctly rounded quotient, identical to
+// the division), the plain division otherwise.
+template <class T>
+__device__ inline T div_ac(T x, const KLevelT<T> &L) {
+  return x / L.ac;
+}
+template <>
+__device__ inline double div_ac<double>(double x, const KLevelT<double> &L) {
+  if (L.acy == 0.0) return x / L.ac;
+  QDiv d;
+  d.h = L.ac;
+  d.y = L.acy;
+  return qdiv(x, d);
+}
 
 template <class T>
 inline void set_level_divs(KLevelT<T> &L) {
@@ -140,9 +157,9 @@ __device__ inline void sor_node(const KLevelT<T> &L, int colour, T omega, int64_
   T *u = L.u;
   if (bcv == -2) {
     u[c] = (T(1) - omega) * u[c] +
-           omega * (L.b[c] - L.an * u[c + g.sy] - L.as * u[c - g.sy] - T(2) * L.ae * u[c + 1] -
-                    L.at * u[c + g.sz] - L.ab * u[c - g.sz]) /
-               L.ac;
+           div_ac(omega * (L.b[c] - L.an * u[c + g.sy] - L.as * u[c - g.sy] -
+                           T(2) * L.ae * u[c + 1] - L.at * u[c + g.sz] - L.ab * u[c - g.sz]),
+                  L);
     return;
   }
   if (bcv == 0) {
@@ -150,9 +167,9 @@ __device__ inline void sor_node(const KLevelT<T> &L, int colour, T omega, int64_
     if (j == g.n2 + 1) return;
   }
   u[c] = (T(1) - omega) * u[c] +
-         omega * (L.b[c] - L.aw * u[c - 1] - L.an * u[c + g.sy] - L.as * u[c - g.sy] -
-                  L.ae * u[c + 1] - L.at * u[c + g.sz] - L.ab * u[c - g.sz]) /
-             L.ac;
+         div_ac(omega * (L.b[c] - L.aw * u[c - 1] - L.an * u[c + g.sy] - L.as * u[c - g.sy] -
+                         L.ae * u[c + 1] - L.at * u[c + g.sz] - L.ab * u[c - g.sz]),
+                L);
 }
 
 template <class T>
@@ -876,6 +893,7 @@ void set_coeffs(KLevelT<T> &L, double eps0) {
   const double ac = -(an + as + aw + ae + at + ab);
   L.aw = (T)aw; L.ae = (T)ae; L.an = (T)an; L.as = (T)as; L.at = (T)at; L.ab = (T)ab;
   L.ac = (T)ac;
+  L.acy = qdiv_divisor_ok(ac) ? 1.0 / ac : 0.0;
 }
 
 }  // namespace
