@@ -345,6 +345,139 @@ def _adjoint_cfg4(steps, n=64, ppc=8, checkpoint=20):
     return out
 
 
+def _adjoint_cfg5(n, ppc=32, steps=500, checkpoint=20, seed=0):
+    """The combined run's forward + adjoint of the ion current into the anode
+    with respect to the anode bias, on an n^3 grid of the same geometry (the
+    annular channel, anode at 300 V, radial Br(z) map, argon MCC every 10
+    steps frozen in the backward), FP64, `steps` steps with checkpoints
+    every `checkpoint`.  Particles: ppc per species in the loadable cells,
+    uniform positions and Maxwellian velocities from a seeded host draw.
+    Times one plain forward, one forward that keeps checkpoints and its
+    backward, after an untimed 20-step forward + backward."""
+    import torch
+    import numpy as np
+    from essaitemporaire_lepic_module_amd import config as cfg
+    from essaitemporaire_lepic_module_amd.adjoint import DiffPIC
+    case = cfg.cfg5_combined(n=n, np_cell=ppc, steps=steps)
+    rng = np.random.default_rng(seed)
+    h = np.array(case.h)
+    pl = np.asarray(case.bcnd) == -1
+    load = np.zeros((n, n, n), bool)
+    for dk in (1, 2):
+        for dj in (1, 2):
+            for di in (1, 2):
+                load |= pl[dk:dk + n, dj:dj + n, di:di + n]
+    cells = np.flatnonzero(load.reshape(-1))
+    ci = np.repeat(cells, ppc)
+    pos = np.stack([ci % n, (ci // n) % n, ci // (n * n)], 1).astype(np.float64)
+    pos = (pos + rng.random(pos.shape)) * h
+    parts = []
+    for sp in case.tracked:
+        vel = rng.standard_normal(pos.shape) * cfg.thermal_speed(sp.T_eV, sp.mass) / math.sqrt(2.0)
+        parts.append({"x": pos[:, 0], "y": pos[:, 1], "z": pos[:, 2],
+                      "vx": vel[:, 0], "vy": vel[:, 1], "vz": vel[:, 2]})
+    del pos, ci
+    sim = DiffPIC(case, parts, "cuda:0", checkpoint=checkpoint)
+    del parts
+    sim.forward(300.0, None, 20, keep=True)
+    sim.backward()
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, r
+
+    t_plain, loss = timed(lambda: sim.forward(300.0, None, steps, keep=False))
+    torch.cuda.reset_peak_memory_stats()
+    t_keep, _ = timed(lambda: sim.forward(300.0, None, steps, keep=True))
+    t_back, (dV, _) = timed(sim.backward)
+    ncoll = sum(int(ev["idx"].numel()) for evs in sim.mcc_events.values() for ev in evs)
+    out = {"forward_ms": t_plain * 1e3, "forward_checkpointed_ms": t_keep * 1e3,
+           "backward_ms": t_back * 1e3,
+           "adjoint_over_forward": (t_keep + t_back) / t_plain,
+           "loss_A": loss, "dL_dV_A_per_V": dV, "collided": ncoll,
+           "particles": 2 * int(len(cells)) * ppc,
+           "peak_device_GB": torch.cuda.max_memory_allocated() / 1e9,
+           "workload": (f"{n}^3 cells of the combined run's geometry (annular channel, anode "
+                        f"300 V, radial Br(z) map), {ppc} ppc x 2 species in the loadable "
+                        f"cells, argon MCC every 10 steps (frozen in the backward), {steps} "
+                        f"steps, checkpoint every {checkpoint}, FP64; loss: the mean ion "
+                        "current into the anode through z = h3")}
+    del sim
+    torch.cuda.empty_cache()
+    return out
+
+
+def _combined_cfg5(n, ppc, steps=10, warmup=3):
+    """The combined large run (cfg5 forward): n^3 cells, the annular channel
+    with the anode at 300 V, the radial Br(z) map, argon MCC every 10 steps,
+    in the precision form with the mixed-precision Poisson solve; `steps`
+    steps timed by wall clock after `warmup` (electrons re-sorted every 4),
+    then as many with every phase between HIP events."""
+    import torch
+    from essaitemporaire_lepic_module_amd import config as cfg
+    from essaitemporaire_lepic_module_amd.simulation import Simulation32
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats()
+    t0 = time.perf_counter()
+    case = cfg.cfg5_combined(n=n, np_cell=ppc)
+    cells = cfg.loadable_cells(case.bcnd, case.n)
+    sim = Simulation32(case, "cuda:0", capacity_factor=1.25, resort_every=[4, 0], poisson="mixed")
+    torch.cuda.synchronize()
+    t_init = time.perf_counter() - t0
+    istep = 1
+    sim.timed_phases = {"push_bc_deposit"}
+    holes = torch.zeros_like(sim.lost_acc)
+    for _ in range(warmup):
+        sim.timers = {}
+        holes += sim.lost_acc
+        sim.advance_one_step(istep)
+        istep += 1
+    n_before = sim.particle_count()
+    torch.cuda.synchronize()
+    sim.timers = {}
+    holes.zero_()
+    slots = 0
+    cycles = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        slots += sum(p.n for p in sim.parts)
+        holes += sim.lost_acc
+        sim.advance_one_step(istep)
+        cycles.append(sim.last_mg["cycles"])
+        istep += 1
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_phases = sim.phase_times()
+    sim.timers = {}
+    sim.timed_phases = None
+    for _ in range(steps):
+        sim.advance_one_step(istep)
+        istep += 1
+    torch.cuda.synchronize()
+    phases = sim.phase_times()
+    sim.check_escaped()
+    push_ms, push_launches, push_units = kernel_phases.get("push_bc_deposit", (0.0, 0, 0))
+    out = {"value": (slots - int(holes.sum().item())) / wall, "unit": "particle-steps/s",
+           "ms_per_step": wall * 1e3 / steps, "steps": steps, "warmup": warmup,
+           "ppc": ppc, "loadable_cells": cells, "particles": n_before,
+           "refinement_steps_per_solve": sum(cycles) / len(cycles),
+           "phase_ms_per_step": {k: v[0] / steps for k, v in phases.items()},
+           "push_GBs": (BYTES_PER_PARTICLE_STEP_F32 * push_units / (push_ms * 1e-3) / 1e9
+                        if push_ms else 0.0),
+           "init_s": t_init, "peak_device_GB": torch.cuda.max_memory_allocated() / 1e9,
+           "workload": (f"{n}^3 cells, annular channel (r 0.3-0.45 of the width), anode 300 V at "
+                        f"z = 0, radial Br(z) map (0.02 T at z = 0.7 L, width 0.1 L), {ppc} ppc "
+                        f"x 2 species in the {cells} loadable cells, argon MCC every 10 steps, "
+                        "FP32 particles, fixed-point deposit, mixed-precision Poisson to 1e-10")}
+    del sim
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -359,6 +492,15 @@ def main():
     ap.add_argument("--poisson-n", type=int, default=256,
                     help="grid of the mixed-precision Poisson comparison (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--combined-n", type=int, default=256,
+                    help="grid of the combined run (annular channel, B map, mixed Poisson; "
+                         "0: skip)")
+    ap.add_argument("--combined-box-count", type=int, default=1,
+                    help="also run the combined case with 32 ppc x 2 species of the whole box "
+                         "(~1.07B particles at 256^3) placed in the channel (0: skip)")
+    ap.add_argument("--combined-adjoint-n", type=int, default=64,
+                    help="grid of the combined geometry's 500-step forward + adjoint of the "
+                         "anode current (0: skip)")
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="steps of the differentiable 64^3 run whose forward+adjoint / forward "
                          "cost is reported (0: skip)")
@@ -431,9 +573,24 @@ def main():
     if args.poisson_n:
         line["poisson_mixed"] = _poisson_cfg3(args.poisson_n)
         print("[bench] poisson done", file=sys.stderr, flush=True)
+    if args.combined_n:
+        line["combined"] = _combined_cfg5(args.combined_n, 32)
+        print("[bench] combined done", file=sys.stderr, flush=True)
+        if args.combined_box_count:
+            # the same run with the particle count of 32 ppc x 2 species over
+            # every cell of the box (the channel holds about a third of them)
+            from essaitemporaire_lepic_module_amd import config as cfg
+            n = args.combined_n
+            cells = cfg.loadable_cells(cfg.annular_channel_bcnd((n,) * 3, (3.125e-4,) * 3),
+                                       (n,) * 3)
+            line["combined_box_count"] = _combined_cfg5(n, int(round(32 * n ** 3 / cells)))
+            print("[bench] combined (box count) done", file=sys.stderr, flush=True)
     if args.adjoint_steps:
         line["adjoint"] = _adjoint_cfg4(args.adjoint_steps)
         print("[bench] adjoint done", file=sys.stderr, flush=True)
+    if args.combined_adjoint_n:
+        line["combined_adjoint"] = _adjoint_cfg5(args.combined_adjoint_n)
+        print("[bench] combined adjoint done", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(case)
     print(json.dumps(line))

@@ -34,8 +34,9 @@ EXPORTS = (
     "lepic_plane_moments", "lepic_electron_heating", "lepic_dielectric_phi",
     "lepic_pset_to32", "lepic_pset_from32", "lepic_round_field4", "lepic_push32",
     "lepic_deposit32", "lepic_fixed_density", "lepic_sort32", "lepic_collisions32",
-    "lepic_poisson_solve_mixed", "lepic_adj_push", "lepic_adj_efield", "lepic_adj_poisson_rhs",
-    "lepic_adj_bias", "lepic_adj_deposit", "lepic_plane_flux", "lepic_apply_deaths",
+    "lepic_poisson_solve_mixed", "lepic_adj_push", "lepic_adj_push_map", "lepic_adj_efield",
+    "lepic_adj_poisson_rhs", "lepic_adj_bias", "lepic_adj_deposit", "lepic_plane_flux",
+    "lepic_plane_flux_axis", "lepic_apply_deaths",
 )
 
 
@@ -213,12 +214,16 @@ def load() -> C.CDLL:
                                              C.POINTER(MCCStats), vp, i32p]),
             "lepic_adj_push": (C.c_int, [vp, G, PS, AJ, vp, dp, C.c_double, C.c_double,
                                          C.c_double, vp, vp]),
+            "lepic_adj_push_map": (C.c_int, [vp, G, PS, AJ, vp, vp, C.c_int32, C.c_double,
+                                             C.c_double, C.c_double, vp]),
             "lepic_adj_efield": (C.c_int, [vp, G, vp, vp, vp]),
             "lepic_adj_poisson_rhs": (C.c_int, [vp, G, vp, vp, C.c_double, vp]),
             "lepic_adj_bias": (C.c_int, [vp, G, vp, C.c_int32, vp, vp, C.c_double, vp]),
             "lepic_adj_deposit": (C.c_int, [vp, G, PS, AJ, vp, vp, C.c_double, C.c_double]),
             "lepic_plane_flux": (C.c_int, [vp, G, PS, C.c_int32, C.c_double, vp, AJ,
                                            C.c_double]),
+            "lepic_plane_flux_axis": (C.c_int, [vp, G, PS, C.c_int32, C.c_int32, C.c_double, vp,
+                                                AJ, C.c_double]),
             "lepic_apply_deaths": (C.c_int, [vp, PS, vp, C.c_int64]),
         }
         for name, (res, args) in sig.items():

@@ -5,7 +5,8 @@ hand-written adjoint kernels (lepic_adj_*) run backward over the steps.
 The forward is the FP64 step on particles held in place (lost particles
 become holes and keep their slots, so each slot is one particle for the
 whole run): rho from the deposit, the multigrid potential with the biased
-electrode at V, E, the Boris push with B = (0, 0, B0) and the E gather, the
+electrode at V, E, the Boris push with B = (0, 0, B0) (or the case's B map,
+gathered like E) and the E gather, the
 wall test, the fused deposit for the next step, and on collision steps the
 null-collision MCC (mod_simulation.f90:340-472 without sorting; MCC
 products take slots reserved after the initial particles).  The loss is the
@@ -14,6 +15,9 @@ front of the biased wall,
 
     L = (1/K) sum_k q Nm / (Ly Lz h1) sum_p vx_p hat(x_p / h1 - (n1 - 1)),
 
+or, with the case's flux_plane (axis a, node plane ip, sign, area A), the
+same sum of sign q Nm / (A h_a) v_a hat(x_a / h_a - (ip - 1)) -- with A = 1
+the ion current into an electrode face (the annular channel's anode);
 a smooth stand-in for the charge the wall collects per unit time and area
 (apply_particle_bc_legacy's p_mac, mod_particleBC.f90:195-206, counts whole
 particles and has zero derivative almost everywhere).  The events are
@@ -82,8 +86,8 @@ class AdjointSet:
 class DiffPIC:
     """The differentiable run of a Case on one device.
 
-    case: plasma (-1) and electrode (>= 1) labels only, B uniform along z or
-    absent, collisions (PHILOX draws, keyed by philox_key and the step) or
+    case: plasma (-1) and electrode (>= 1) labels only, B uniform along z, a
+    B map on the field grid (B_map; dL/dB0 is then 0) or none, collisions (PHILOX draws, keyed by philox_key and the step) or
     none, no dielectrics or heating.  parts: the initial FP64 particle sets
     (host arrays dict per species: x y z vx vy vz), loaded into device sets
     of that many slots plus, with collisions, room for the products
@@ -95,8 +99,10 @@ class DiffPIC:
         b = np.asarray(case.bcnd)
         if np.any((b == 0) | (b == -2)) or case.flag_pbc or case.flag_nmn:
             raise ValueError("the adjoint E field takes plasma (-1) and electrode labels only")
-        if case.flag_die or case.heating is not None or case.B_map is not None:
-            raise ValueError("the differentiable step has no dielectrics, heating or B map")
+        if case.flag_die or case.heating is not None:
+            raise ValueError("the differentiable step has no dielectrics or heating")
+        if case.B_map is not None and case.B_grid is not None:
+            raise ValueError("the differentiable step takes a B map on the field grid")
         if case.B_uniform is not None and (case.B_uniform[0] or case.B_uniform[1]):
             raise ValueError("the differentiable step takes B along z")
         self.case = case
@@ -120,6 +126,12 @@ class DiffPIC:
         self.charge = [s.charge for s in tr]
         self.mass = [s.mass for s in tr]
         self.Nm = list(case.Nm[: self.ntype])
+        self.magnetized = [q <= 0 or case.ions_magnetized for q in self.charge]
+        # a B map (on the field grid) replaces the uniform B0 along z; dL/dB0
+        # is then 0
+        self.B4 = None
+        if case.B_map is not None:
+            self.B4 = torch.as_tensor(np.asarray(case.B_map, np.float64), device=dev).contiguous()
         shape = node_shape(self.n)
         self.shape = shape
         self.collide_every = int(case.nb_step_collisions) if case.chemistry is not None else 0
@@ -153,10 +165,14 @@ class DiffPIC:
             self.init.append((m, d, cap))
         self.mcc_events = {}
         n1, n2, n3 = self.n
-        self.plane = n1                       # node plane x = (n1 - 1) h1
-        area = n2 * self.h[1] * n3 * self.h[2]
-        self.flux_coef = self.charge[self.flux_species] * self.Nm[self.flux_species] / \
-            (area * self.h[0])
+        # the loss plane: the case's (axis, node plane, sign, area), else the
+        # node plane x = (n1 - 1) h1 over the x = L face's area
+        axis, plane, sign, area = case.flux_plane or (0, n1, 1.0, None)
+        if area is None:
+            area = math.prod(self.n[d] * self.h[d] for d in range(3) if d != axis)
+        self.flux_axis, self.plane = int(axis), int(plane)
+        self.flux_coef = float(sign) * self.charge[self.flux_species] * \
+            self.Nm[self.flux_species] / (area * self.h[axis])
         self.phi = torch.zeros(shape, dtype=torch.float64, device=dev)
         self.rho = torch.zeros(shape, dtype=torch.float64, device=dev)
         self.E4 = torch.zeros(shape + (4,), dtype=torch.float64, device=dev)
@@ -214,11 +230,20 @@ class DiffPIC:
                                c.omega, c.ng, 0, 0)
         F.calc_Efield_modular(self.n, self.h, self.phi, self.E4, self.bcnd)
 
+    def _bf(self, s: int, B0):
+        """The push's B for species s: the case's map, else B0 along z (None:
+        no B)."""
+        if self.B4 is not None:
+            return P.bfield(2, B4=self.B4, magnetized=self.magnetized[s])
+        if B0 is None:
+            return P.bfield(0)
+        return P.bfield(1, (0.0, 0.0, float(B0)), magnetized=self.magnetized[s])
+
     def _push(self, st, B0):
         c = self.case
-        bf = P.bfield(1, (0.0, 0.0, float(B0))) if B0 is not None else P.bfield(0)
         self.np_thread.zero_()
         for s, (m, w) in enumerate(st):
+            bf = self._bf(s, B0)
             p = _pset_of(w, m)
             bp = P.bc_params(self.n, self.h, 0, 0, s + 1, -1, self.Nm[s] * self.charge[s],
                              self.mass[s], self.Nm[s], c.ngrid)
@@ -231,8 +256,9 @@ class DiffPIC:
     def _flux(self, st, acc, adj=None, scale=0.0):
         m, w = st[self.flux_species]
         p = _pset_of(w, m)
-        self.ctx.check(self.ctx.lib.lepic_plane_flux(
-            self.ctx.h, C.byref(self.g), C.byref(p), int(self.plane), float(self.flux_coef),
+        self.ctx.check(self.ctx.lib.lepic_plane_flux_axis(
+            self.ctx.h, C.byref(self.g), C.byref(p), self.flux_axis, self.plane,
+            float(self.flux_coef),
             ptr(acc), C.byref(adj.struct()) if adj is not None else None, float(scale)))
 
     @staticmethod
@@ -300,9 +326,9 @@ class DiffPIC:
         moves, the slots recorded lost at step k become holes, the rest
         deposit (the map the adjoint differentiates)."""
         c = self.case
-        bf = P.bfield(1, (0.0, 0.0, float(B0))) if B0 is not None else P.bfield(0)
         self.np_thread.zero_()
         for s, (m, w) in enumerate(st):
+            bf = self._bf(s, B0)
             p = _pset_of(w, m)
             self.ctx.check(self.ctx.lib.lepic_move_boris(self.ctx.h, C.byref(self.g), C.byref(p),
                                                          ptr(self.E4), C.byref(bf), self.charge[s],
@@ -371,7 +397,7 @@ class DiffPIC:
         lamB = torch.zeros(3, dtype=torch.float64, device=dev)
         lamV = torch.zeros(1, dtype=torch.float64, device=dev)
         plasma = (self.bcnd <= 0)
-        B = None if B0 is None else (C.c_double * 3)(0.0, 0.0, B0)
+        B = None if B0 is None or self.B4 is not None else (C.c_double * 3)(0.0, 0.0, B0)
         c = self.case
         scale = grad_out / steps
         starts = sorted(self.ckpt)
@@ -410,9 +436,16 @@ class DiffPIC:
                 for s, (m, w) in enumerate(before):
                     p = _pset_of(w, m)
                     a = adj[s].struct()
-                    ctx.check(ctx.lib.lepic_adj_push(
-                        ctx.h, C.byref(g), C.byref(p), C.byref(a), ptr(Es[j]), B,
-                        self.charge[s], self.mass[s], c.dt, ptr(lamE), ptr(lamB)))
+                    if self.B4 is not None:
+                        ctx.check(ctx.lib.lepic_adj_push_map(
+                            ctx.h, C.byref(g), C.byref(p), C.byref(a), ptr(Es[j]), ptr(self.B4),
+                            int(self.magnetized[s]), self.charge[s], self.mass[s], c.dt,
+                            ptr(lamE)))
+                    else:
+                        ctx.check(ctx.lib.lepic_adj_push(
+                            ctx.h, C.byref(g), C.byref(p), C.byref(a), ptr(Es[j]),
+                            B if self.magnetized[s] else None, self.charge[s], self.mass[s],
+                            c.dt, ptr(lamE), ptr(lamB)))
                 ctx.check(ctx.lib.lepic_adj_efield(ctx.h, C.byref(g), ptr(self.bcnd), ptr(lamE),
                                                    ptr(lamphi)))
                 # A mu = dL/dphi on plasma nodes, mu = 0 on electrodes, solved
@@ -438,21 +471,25 @@ class DiffPIC:
                                                         self.Nm[s]))
             del states, Es
         self.adj = adj
-        return float(lamV.item()), (float(lamB[2].item()) if B0 is not None else 0.0)
+        return float(lamV.item()), (float(lamB[2].item())
+                                    if B0 is not None and self.B4 is None else 0.0)
 
 
 class PicLoss(torch.autograd.Function):
     """L(V, B0) of a DiffPIC run as a torch function: the forward runs the
-    steps keeping checkpoints, the backward runs the adjoint kernels."""
+    steps keeping checkpoints, the backward runs the adjoint kernels.  B0
+    None: no uniform field (the case's B map, or none)."""
 
     @staticmethod
-    def forward(ctx, V: torch.Tensor, B0: torch.Tensor, sim: DiffPIC, steps: int):
-        L = sim.forward(float(V.item()), float(B0.item()), steps, keep=True)
+    def forward(ctx, V: torch.Tensor, B0: torch.Tensor | None, sim: DiffPIC, steps: int):
+        L = sim.forward(float(V.item()), None if B0 is None else float(B0.item()), steps,
+                        keep=True)
         ctx.sim = sim
+        ctx.has_b = B0 is not None
         return torch.tensor(L, dtype=torch.float64)
 
     @staticmethod
     def backward(ctx, gL):
         dV, dB = ctx.sim.backward(float(gL.item()))
-        return (torch.tensor(dV, dtype=torch.float64), torch.tensor(dB, dtype=torch.float64),
-                None, None)
+        return (torch.tensor(dV, dtype=torch.float64),
+                torch.tensor(dB, dtype=torch.float64) if ctx.has_b else None, None, None)

@@ -115,6 +115,9 @@ class Case:
     flag_die: int = 0                    # dielectric surface charging
     dtype: np.ndarray | None = None      # wall type per label 0..ngrid (flag_die)
     Ca: float = 0.0                      # dielectric capacitance per area (F/m^2)
+    # loss plane of a differentiable run: (axis, 1-based node plane, sign,
+    # area in m^2 or None for the whole face); None: x = (n1 - 1) h1, sign +1
+    flux_plane: tuple | None = None
 
     @property
     def tracked(self):
@@ -214,6 +217,85 @@ def cfg4_differentiable(n: int = 64, np_cell: int = 8, Vb: float = 20.0, B0: flo
              np_cell, 0.1 / omega_pe(n0), steps, eps=1e-10, ng=ng, B_uniform=(0.0, 0.0, B0),
              ngas=ngas, nb_step_collisions=int(mcc_every))
     c.chemistry = argon_chemistry(sp, ngas)
+    return c
+
+
+def loadable_cells(bcnd: np.ndarray, n) -> int:
+    """Cells with a plasma (-1) node among their 8 corners: the cells the
+    loaders fill (mod_particle_loader.f90:75-91)."""
+    pl = np.asarray(bcnd) == -1
+    n1, n2, n3 = (int(v) for v in n)
+    load = np.zeros((n3, n2, n1), bool)
+    for dk in (0, 1):
+        for dj in (0, 1):
+            for di in (0, 1):
+                load |= pl[1 + dk:n3 + 1 + dk, 1 + dj:n2 + 1 + dj, 1 + di:n1 + 1 + di]
+    return int(load.sum())
+
+
+def annular_channel_bcnd(n, h, r_in: float = 0.3, r_out: float = 0.45, anode_label: int = 2):
+    """A grounded box (label 1) holding a cylindrical annular channel along z
+    about the box's centre line: plasma (-1) on the interior nodes whose
+    distance r from the axis satisfies r_in L <= r <= r_out L (L the box
+    width along x), label 1 elsewhere; the channel's cross-section on the
+    z = 0 face (node planes 0 and 1) is the anode (anode_label)."""
+    b = grounded_box_bcnd(n, 1)
+    L = n[0] * h[0]
+    x = (np.arange(n[0] + 3) - 1) * h[0] - 0.5 * L
+    y = (np.arange(n[1] + 3) - 1) * h[1] - 0.5 * n[1] * h[1]
+    r = np.sqrt(x[None, :] ** 2 + y[:, None] ** 2)          # (j, i)
+    inside = (r >= r_in * L) & (r <= r_out * L)
+    b[:, ~inside] = 1
+    b[0:2][:, inside] = anode_label
+    return b
+
+
+def radial_bfield_map(n, h, B0: float = 0.02, z_peak: float = 0.7, width: float = 0.1):
+    """Radial field B = Br(z) r_hat about the box's centre line, as a map of 4
+    values per node (Bx, By, Bz, |B|) on the field grid: Br(z) = B0 exp(-(z -
+    z_peak L)^2 / (2 (width L)^2)), L the box length along z (the Gaussian
+    profile form of gaussian_Bfield, mod_magneticField.f90:410-428); zero on
+    the axis."""
+    Lx, Ly, Lz = (n[d] * h[d] for d in range(3))
+    x = (np.arange(n[0] + 3) - 1) * h[0] - 0.5 * Lx
+    y = (np.arange(n[1] + 3) - 1) * h[1] - 0.5 * Ly
+    z = (np.arange(n[2] + 3) - 1) * h[2]
+    br = B0 * np.exp(-(z - z_peak * Lz) ** 2 / (2.0 * (width * Lz) ** 2))      # (k,)
+    r = np.sqrt(x[None, :] ** 2 + y[:, None] ** 2)                             # (j, i)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        cx = np.where(r > 0.0, x[None, :] / r, 0.0)
+        cy = np.where(r > 0.0, y[:, None] / r, 0.0)
+    B4 = np.zeros(node_shape(n) + (4,))
+    B4[..., 0] = br[:, None, None] * cx[None]
+    B4[..., 1] = br[:, None, None] * cy[None]
+    B4[..., 3] = np.sqrt(B4[..., 0] ** 2 + B4[..., 1] ** 2 + B4[..., 2] ** 2)
+    return B4
+
+
+def cfg5_combined(n: int = 256, np_cell: int = 32, steps: int = 500, V_anode: float = 300.0,
+                  B0: float = 0.02, pressure_Pa: float = 1.0, ng: int | None = None) -> Case:
+    """Combined large run: n^3 cells (h = 3.125e-4 m), the annular channel of
+    annular_channel_bcnd (inner radius 0.3, outer 0.45 of the box width) with
+    the anode at z = 0 biased at V_anode, the radial Br(z) map of
+    radial_bfield_map (peak B0 at z = 0.7 L, width 0.1 L) acting on both
+    species with the |B| > 1e-5 gate, e- and Ar+ at np_cell particles per
+    loadable cell and species, the analytic argon MCC of the PIC-MCC case
+    every 10 steps, sort every 20, dt = 0.1/omega_pe, Poisson to 1e-10."""
+    nn = (n, n, n)
+    h = (3.125e-4,) * 3
+    n0 = 1e15
+    if ng is None:
+        ng = max(2, int(round(math.log2(n))))
+    sp = argon_species()
+    ngas = pressure_Pa / (KB * 300.0)
+    c = Case("combined", nn, h, annular_channel_bcnd(nn, h), 2, {1: 0.0, 2: V_anode}, sp, n0,
+             np_cell, 0.1 / omega_pe(n0), steps, eps=1e-10, ng=ng, ngas=ngas,
+             nb_step_sort=20, nb_step_collisions=10)
+    c.B_map = radial_bfield_map(nn, h, B0)
+    c.chemistry = argon_chemistry(sp, ngas)
+    # the differentiable run's loss: the ion current (A) into the anode,
+    # through the node plane z = h3 one cell in front of it
+    c.flux_plane = (2, 2, -1.0, 1.0)
     return c
 
 

@@ -77,6 +77,8 @@ struct KBAdj {
   double B[3];
   double k2;        // k1 * 2 / (1 + (k1 |B|)^2), formed as the forward forms it
   double dk2;       // dk2 / d(|B|^2)
+  const double4 *B4;   // map on the E grid (MAP kernels): 4 doubles per node
+  int magnetized;      // map: 0 when this species ignores B
 };
 
 // Adjoint of the push of one particle: given dL/dx', dL/dv' of the state
@@ -101,6 +103,11 @@ __device__ inline void adj_flush(const KGrid &g, double *dst, int64_t b, const d
   }
 }
 
+// MAP: B is a map on the E grid gathered with the particle's stencil and
+// gated at its lower node (|B| > 1e-5, Src/part_expmover.f90:158), so B,
+// k2 and dk2 are per particle and the B gather adds its own position term;
+// dL/dB of a map is not accumulated (the map is not a parameter here).
+template <bool MAP>
 __global__ __launch_bounds__(BLOCK) void k_adj_push(KGrid g, lepic_pset st, KAdj a, int64_t n,
                                                     const double4 *__restrict__ E4, KBAdj bf,
                                                     double k1, double dt, double *rep,
@@ -136,31 +143,57 @@ __global__ __launch_bounds__(BLOCK) void k_adj_push(KGrid g, lepic_pset st, KAdj
       E[2] += w[c] * Ec[c].z;
     }
     double lv[3], lE[3];
-    if (bf.use_b) {
-      const double *B = bf.B;
+    bool use_b = bf.use_b != 0;
+    double Bm[3] = {0.0, 0.0, 0.0}, k2 = bf.k2, dk2 = bf.dk2;
+    double4 Bc[8];
+    if (MAP) {
+      use_b = bf.magnetized && bf.B4[nd[0]].w > 1.0e-5;
+      if (use_b) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          Bc[c] = bf.B4[nd[c]];
+          Bm[0] += w[c] * Bc[c].x;
+          Bm[1] += w[c] * Bc[c].y;
+          Bm[2] += w[c] * Bc[c].z;
+        }
+        const double B2 = Bm[0] * Bm[0] + Bm[1] * Bm[1] + Bm[2] * Bm[2];
+        const double Bt = sqrt(B2);
+        k2 = k1 * (2.0 / (1.0 + (k1 * Bt) * (k1 * Bt)));
+        const double den = 1.0 + k1 * k1 * B2;
+        dk2 = -2.0 * k1 * k1 * k1 / (den * den);
+      }
+    }
+    double pb0 = 0.0, pb1 = 0.0, pb2 = 0.0;   // dL/dB at this particle
+    if (use_b) {
+      const double *B = MAP ? Bm : bf.B;
       const double v1[3] = {v[0] + k1 * E[0], v[1] + k1 * E[1], v[2] + k1 * E[2]};
       const double v3[3] = {v1[0] + k1 * (v1[1] * B[2] - v1[2] * B[1]),
                             v1[1] + k1 * (v1[2] * B[0] - v1[0] * B[2]),
                             v1[2] + k1 * (v1[0] * B[1] - v1[1] * B[0])};
       // v2 = v1 + k2 (v3 x B), v' = v2 + k1 E
       const double *l2 = lam;
-      const double l3[3] = {bf.k2 * (B[1] * l2[2] - B[2] * l2[1]),
-                            bf.k2 * (B[2] * l2[0] - B[0] * l2[2]),
-                            bf.k2 * (B[0] * l2[1] - B[1] * l2[0])};
+      const double l3[3] = {k2 * (B[1] * l2[2] - B[2] * l2[1]),
+                            k2 * (B[2] * l2[0] - B[0] * l2[2]),
+                            k2 * (B[0] * l2[1] - B[1] * l2[0])};
       const double lk2 = l2[0] * (v3[1] * B[2] - v3[2] * B[1]) +
                          l2[1] * (v3[2] * B[0] - v3[0] * B[2]) +
                          l2[2] * (v3[0] * B[1] - v3[1] * B[0]);
       // dL/dB from v3 x B (through k2's factor) and from k2(|B|^2)
-      lb0 += bf.k2 * (l2[1] * v3[2] - l2[2] * v3[1]) + lk2 * bf.dk2 * 2.0 * B[0];
-      lb1 += bf.k2 * (l2[2] * v3[0] - l2[0] * v3[2]) + lk2 * bf.dk2 * 2.0 * B[1];
-      lb2 += bf.k2 * (l2[0] * v3[1] - l2[1] * v3[0]) + lk2 * bf.dk2 * 2.0 * B[2];
+      pb0 = k2 * (l2[1] * v3[2] - l2[2] * v3[1]) + lk2 * dk2 * 2.0 * B[0];
+      pb1 = k2 * (l2[2] * v3[0] - l2[0] * v3[2]) + lk2 * dk2 * 2.0 * B[1];
+      pb2 = k2 * (l2[0] * v3[1] - l2[1] * v3[0]) + lk2 * dk2 * 2.0 * B[2];
       // v3 = v1 + k1 (v1 x B)
       const double l1[3] = {l2[0] + l3[0] + k1 * (B[1] * l3[2] - B[2] * l3[1]),
                             l2[1] + l3[1] + k1 * (B[2] * l3[0] - B[0] * l3[2]),
                             l2[2] + l3[2] + k1 * (B[0] * l3[1] - B[1] * l3[0])};
-      lb0 += k1 * (l3[1] * v1[2] - l3[2] * v1[1]);
-      lb1 += k1 * (l3[2] * v1[0] - l3[0] * v1[2]);
-      lb2 += k1 * (l3[0] * v1[1] - l3[1] * v1[0]);
+      pb0 += k1 * (l3[1] * v1[2] - l3[2] * v1[1]);
+      pb1 += k1 * (l3[2] * v1[0] - l3[0] * v1[2]);
+      pb2 += k1 * (l3[0] * v1[1] - l3[1] * v1[0]);
+      if (!MAP) {
+        lb0 += pb0;
+        lb1 += pb1;
+        lb2 += pb2;
+      }
       for (int d = 0; d < 3; ++d) {
         lv[d] = l1[d];
         lE[d] = k1 * lam[d] + k1 * l1[d];
@@ -183,6 +216,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_push(KGrid g, lepic_pset st, KAdj
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       e[c] = Ec[c].x * lE[0] + Ec[c].y * lE[1] + Ec[c].z * lE[2];
+      if (MAP && use_b) e[c] += Bc[c].x * pb0 + Bc[c].y * pb1 + Bc[c].z * pb2;
       acc[c][0] += w[c] * lE[0];
       acc[c][1] += w[c] * lE[1];
       acc[c][2] += w[c] * lE[2];
@@ -196,7 +230,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_push(KGrid g, lepic_pset st, KAdj
     a.lvx[i] = lv[0]; a.lvy[i] = lv[1]; a.lvz[i] = lv[2];
   }
   adj_flush(g, dst, cur, acc);
-  if (bf.use_b) {
+  if (!MAP && bf.use_b) {
     __shared__ double red[3][BLOCK / LEPIC_WAVE];
     lb0 = wave_sum_d(lb0); lb1 = wave_sum_d(lb1); lb2 = wave_sum_d(lb2);
     const int w = threadIdx.x / LEPIC_WAVE;
@@ -380,24 +414,30 @@ __global__ __launch_bounds__(BLOCK) void k_adj_deposit(KGrid g, lepic_pset st, K
   a.lz[i] -= coef * gz / g.h3;
 }
 
-// The loss term of one step: coef * sum over live particles of vx * S(x),
-// S the cloud-in-cell hat of node plane ip (x = (ip - 1) h1) over h1;
-// with a != NULL also dL/dx, dL/dv += scale * its derivatives.
-__global__ __launch_bounds__(BLOCK) void k_plane_flux(KGrid g, lepic_pset st, int64_t n, int ip,
-                                                      double coef, double *acc, KAdj a,
+// The loss term of one step: coef * sum over live particles of v_a * S(x_a),
+// a the axis (0 x, 1 y, 2 z), S the cloud-in-cell hat of node plane ip
+// (x_a = (ip - 1) h_a) over h_a; with a != NULL also dL/dx_a, dL/dv_a +=
+// scale * its derivatives.
+__global__ __launch_bounds__(BLOCK) void k_plane_flux(KGrid g, lepic_pset st, int64_t n, int axis,
+                                                      int ip, double coef, double *acc, KAdj a,
                                                       int adj, double scale) {
   const int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+  const double *pos = axis == 0 ? st.x : axis == 1 ? st.y : st.z;
+  const double *vel = axis == 0 ? st.vx : axis == 1 ? st.vy : st.vz;
+  const double h = axis == 0 ? g.h1 : axis == 1 ? g.h2 : g.h3;
   double v = 0.0;
   if (i < n && st.dead[i] == 0) {
-    const double u = st.x[i] / g.h1 - (double)(ip - 1);
+    const double u = pos[i] / h - (double)(ip - 1);
     const double au = fabs(u);
     if (au < 1.0) {
       const double hat = 1.0 - au;
-      const double vx = st.vx[i];
-      v = coef * vx * hat;
+      const double va = vel[i];
+      v = coef * va * hat;
       if (adj) {
-        a.lvx[i] += scale * coef * hat;
-        a.lx[i] += scale * coef * vx * (u > 0.0 ? -1.0 : 1.0) / g.h1;
+        double *lp = axis == 0 ? a.lx : axis == 1 ? a.ly : a.lz;
+        double *lv = axis == 0 ? a.lvx : axis == 1 ? a.lvy : a.lvz;
+        lv[i] += scale * coef * hat;
+        lp[i] += scale * coef * va * (u > 0.0 ? -1.0 : 1.0) / h;
       }
     }
   }
@@ -456,12 +496,44 @@ int lepic_adj_push(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *state,
   bf.k2 = k1 * (2.0 / (1.0 + (k1 * Bt) * (k1 * Bt)));
   const double den = 1.0 + k1 * k1 * B2;
   bf.dk2 = -2.0 * k1 * k1 * k1 / (den * den);
+  bf.B4 = nullptr;
+  bf.magnetized = 1;
   double *rep = lepic_replicas(ctx, 4 * nodes);
   if (!rep) return lepic_fail(ctx, LEPIC_ERR_HIP, "adj_push: replicas");
-  hipLaunchKernelGGL(k_adj_push, dim3(grid_blocks(n, BLOCK * APT)), dim3(BLOCK), 0, ctx->stream,
-                     make_kgrid(g), *state, kadj(adj), n, reinterpret_cast<const double4 *>(E4),
-                     bf, k1, dt, rep, nodes, lamB);
+  hipLaunchKernelGGL(k_adj_push<false>, dim3(grid_blocks(n, BLOCK * APT)), dim3(BLOCK), 0,
+                     ctx->stream, make_kgrid(g), *state, kadj(adj), n,
+                     reinterpret_cast<const double4 *>(E4), bf, k1, dt, rep, nodes, lamB);
   LEPIC_LAUNCH_CHECK(ctx, "k_adj_push");
+  hipLaunchKernelGGL(k_adj_rep_flush, dim3(grid_blocks(4 * nodes, 256)), dim3(256), 0,
+                     ctx->stream, lamE4, rep, 4 * nodes);
+  LEPIC_LAUNCH_CHECK(ctx, "k_adj_rep_flush");
+  return LEPIC_OK;
+}
+
+int lepic_adj_push_map(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *state,
+                       lepic_adjoint_set *adj, const double *E4, const double *B4,
+                       int32_t magnetized, double q, double m, double dt, double *lamE4) {
+  if (!ctx) return LEPIC_ERR_ARG;
+  int st;
+  if ((st = check_grid(ctx, g)) || (st = check_adj(ctx, state, adj, "adj_push_map"))) return st;
+  if (!E4 || !B4 || !lamE4) return lepic_fail(ctx, LEPIC_ERR_ARG, "adj_push_map: null argument");
+  const int64_t n = state->n;
+  if (n == 0) return LEPIC_OK;
+  const int64_t nodes = grid_nodes(g);
+  const double k1 = dt * q / (2.0 * m);
+  KBAdj bf;
+  bf.use_b = 0;
+  bf.B[0] = bf.B[1] = bf.B[2] = 0.0;
+  bf.k2 = 0.0;
+  bf.dk2 = 0.0;
+  bf.B4 = reinterpret_cast<const double4 *>(B4);
+  bf.magnetized = magnetized ? 1 : 0;
+  double *rep = lepic_replicas(ctx, 4 * nodes);
+  if (!rep) return lepic_fail(ctx, LEPIC_ERR_HIP, "adj_push_map: replicas");
+  hipLaunchKernelGGL(k_adj_push<true>, dim3(grid_blocks(n, BLOCK * APT)), dim3(BLOCK), 0,
+                     ctx->stream, make_kgrid(g), *state, kadj(adj), n,
+                     reinterpret_cast<const double4 *>(E4), bf, k1, dt, rep, nodes, nullptr);
+  LEPIC_LAUNCH_CHECK(ctx, "k_adj_push_map");
   hipLaunchKernelGGL(k_adj_rep_flush, dim3(grid_blocks(4 * nodes, 256)), dim3(256), 0,
                      ctx->stream, lamE4, rep, 4 * nodes);
   LEPIC_LAUNCH_CHECK(ctx, "k_adj_rep_flush");
@@ -534,23 +606,33 @@ int lepic_adj_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *sta
   return LEPIC_OK;
 }
 
-int lepic_plane_flux(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *state, int32_t plane,
-                     double coef, double *acc, lepic_adjoint_set *adj, double scale) {
+int lepic_plane_flux_axis(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *state,
+                          int32_t axis, int32_t plane, double coef, double *acc,
+                          lepic_adjoint_set *adj, double scale) {
   if (!ctx) return LEPIC_ERR_ARG;
   int st;
   if ((st = check_grid(ctx, g))) return st;
-  if (!state || state->n < 0 || (state->n > 0 && (!state->x || !state->vx || !state->dead)))
+  if (axis < 0 || axis > 2) return lepic_fail(ctx, LEPIC_ERR_ARG, "plane_flux: axis");
+  if (!state || state->n < 0 ||
+      (state->n > 0 && (!state->x || !state->y || !state->z || !state->vx || !state->vy ||
+                        !state->vz || !state->dead)))
     return lepic_fail(ctx, LEPIC_ERR_ARG, "plane_flux: bad particle set");
-  if (plane < 1 || plane > g->n[0] + 1) return lepic_fail(ctx, LEPIC_ERR_ARG, "plane_flux: plane");
+  if (plane < 1 || plane > g->n[axis] + 1)
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "plane_flux: plane");
   if (adj && (st = check_adj(ctx, state, adj, "plane_flux"))) return st;
   if (!acc && !adj) return LEPIC_OK;
   if (state->n == 0) return LEPIC_OK;
   KAdj a = adj ? kadj(adj) : KAdj{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   hipLaunchKernelGGL(k_plane_flux, dim3(grid_blocks(state->n, BLOCK)), dim3(BLOCK), 0,
-                     ctx->stream, make_kgrid(g), *state, state->n, plane, coef, acc, a,
+                     ctx->stream, make_kgrid(g), *state, state->n, (int)axis, plane, coef, acc, a,
                      adj ? 1 : 0, scale);
   LEPIC_LAUNCH_CHECK(ctx, "k_plane_flux");
   return LEPIC_OK;
+}
+
+int lepic_plane_flux(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *state, int32_t plane,
+                     double coef, double *acc, lepic_adjoint_set *adj, double scale) {
+  return lepic_plane_flux_axis(ctx, g, state, 0, plane, coef, acc, adj, scale);
 }
 
 int lepic_apply_deaths(lepic_ctx *ctx, lepic_pset *p, const int64_t *deaths, int64_t step) {

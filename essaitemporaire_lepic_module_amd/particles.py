@@ -41,9 +41,10 @@ class ParticleSet:
         return d
 
     def allocate_pset(self, nmax: int):
-        """allocate_pset (mod_particles.f90:39-72)."""
+        """allocate_pset (mod_particles.f90:39-72).  The second buffer (the
+        target of out-of-place passes) is allocated at its first use."""
         nmax = max(0, int(nmax))
-        self._buf = [self._alloc(nmax), self._alloc(nmax)]
+        self._buf = [self._alloc(nmax), None]
         self._cur = 0
         self.n = 0
         self.nmax = nmax
@@ -59,23 +60,36 @@ class ParticleSet:
         if old is not None and self.n:
             for k in cur:
                 cur[k][: self.n].copy_(old[k][: self.n])
-        self._buf = [cur, self._alloc(new)]
+        self._buf = [cur, None]
         self._cur = 0
         self.nmax = new
 
+    def _buffer(self, i: int) -> dict:
+        if self._buf[i] is None:
+            self._buf[i] = self._alloc(self.nmax)
+        return self._buf[i]
+
+    def release(self):
+        """Drops both buffers (the set is empty afterwards)."""
+        self._buf = [None, None]
+        self._cur = 0
+        self.n = 0
+        self.nmax = 0
+
     def swap(self):
+        self._buffer(self._cur ^ 1)
         self._cur ^= 1
 
     def arrays(self, which: int | None = None) -> dict:
-        return self._buf[self._cur if which is None else which]
+        return self._buffer(self._cur if which is None else which)
 
     def __getattr__(self, name):
         if name in FIELDS or name in ("cell", "dead"):
-            return self._buf[self._cur][name][: self.n]
+            return self._buffer(self._cur)[name][: self.n]
         raise AttributeError(name)
 
     def pset(self, other: bool = False) -> _native.PSet:
-        b = self._buf[self._cur ^ (1 if other else 0)]
+        b = self._buffer(self._cur ^ (1 if other else 0))
         p = _native.PSet()
         for f in FIELDS + ("cell", "dead"):
             setattr(p, f, b[f].data_ptr())
@@ -88,7 +102,7 @@ class ParticleSet:
         m = len(x)
         if m > self.nmax:
             self.allocate_pset(m)
-        b = self._buf[self._cur]
+        b = self._buffer(self._cur)
         for f, v in zip(FIELDS, (x, y, z, vx, vy, vz)):
             b[f][:m].copy_(torch.as_tensor(np.asarray(v, np.float64)))
         b["dead"].zero_()
@@ -100,7 +114,7 @@ class ParticleSet:
         return self
 
     def to_host(self) -> dict:
-        b = self._buf[self._cur]
+        b = self._buffer(self._cur)
         return {k: b[k][: self.n].cpu().numpy() for k in FIELDS + ("cell", "dead")}
 
 

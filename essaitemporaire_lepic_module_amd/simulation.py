@@ -69,8 +69,9 @@ class Simulation:
         self.Nm = case.Nm[: self.ntype]
         self.np_red = torch.zeros((self.ntype,) + shape, dtype=torch.float64, device=dev)
         self.iseed = [P.seed_initialization(1, 0)]
-        ncells = n[0] * n[1] * n[2]
-        cap = int(capacity_factor * abs(case.np_cell) * ncells) + 1024
+        # room for np_cell particles in every cell the loader fills, with headroom
+        from .config import loadable_cells
+        cap = int(capacity_factor * abs(case.np_cell) * loadable_cells(case.bcnd, n)) + 1024
         if parts is not None:
             self.parts, self.np_thread = parts, np_thread
         elif loader == "host":
@@ -372,6 +373,8 @@ class Simulation32(Simulation):
             q, out = P32.ParticleSet32.from_pset64(p, n, h, cap=max(p.nmax, 16))
             if out:
                 raise ValueError(f"{out} particles lie outside the cells")
+            if parts64 is None:
+                p.release()                # the FP64 load is only the source
             P32.sort32(q, n, h)
             parts32.append(q)
         self.parts = parts32

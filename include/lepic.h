@@ -509,6 +509,15 @@ typedef struct {
 int lepic_adj_push(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *state,
                    lepic_adjoint_set *adj, const double *E4, const double *Bunif, double q,
                    double m, double dt, double *lamE4, double *lamB);
+/* lepic_adj_push for a B map on the E grid (lepic_bfield mode 2: B4, 4
+ * doubles per node, the |B| > 1e-5 gate at the particle's lower node,
+ * Src/part_expmover.f90:149-212): B, the rotation factor and its
+ * derivative are the particle's own, and the B gather's position term is
+ * added to dL/dx.  magnetized = 0: this species ignores B.  The map is not
+ * a parameter, so no dL/dB is returned. */
+int lepic_adj_push_map(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *state,
+                       lepic_adjoint_set *adj, const double *E4, const double *B4,
+                       int32_t magnetized, double q, double m, double dt, double *lamE4);
 /* lamphi = D^T lamE4, D the E field of calc_Efield_modular
  * (mod_electricField.f90:9-164) on a grid of plasma (-1) and electrode
  * (>= 1) labels (no periodic or Neumann labels); lamphi is overwritten. */
@@ -537,6 +546,13 @@ int lepic_adj_deposit(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *sta
  * dL/dv += scale * the term's derivatives. */
 int lepic_plane_flux(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *state, int32_t plane,
                      double coef, double *acc, lepic_adjoint_set *adj, double scale);
+/* The same term through a node plane normal to any axis (0 x, 1 y, 2 z):
+ * coef * sum over live particles of v_a * hat(x_a / h_a - (plane - 1)),
+ * 1 <= plane <= n_a + 1 (the ion current into an electrode face normal to
+ * y or z, e.g. the anode at z = 0 of the annular channel). */
+int lepic_plane_flux_axis(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset *state,
+                          int32_t axis, int32_t plane, double coef, double *acc,
+                          lepic_adjoint_set *adj, double scale);
 
 /* Frozen wall events for replays of a differentiable run: the slots whose
  * recorded loss step (deaths, device int64 per slot, 0: never) equals `step`

@@ -13,11 +13,13 @@ doing the differentiation:
 - E = -grad(phi): central differences on plasma nodes, the wall nodes'
   extrapolation blocks in W, E, S, N, B, T order (mod_electricField.f90);
 - gather with the same weights, the Boris push with uniform B = (0, 0, B0)
-  (Src/part_expmover.f90:214-246), x += dt v;
+  or a B map gathered like E and gated at the lower node
+  (Src/part_expmover.f90:151-246), x += dt v;
 - wall absorption frozen: a particle dies at the step the schedule says;
 - loss: the time-averaged ion flux through the node plane one cell in front
   of the biased x = L wall, q Nm vx S(x) / (Ly Lz) with S the plane's
-  cloud-in-cell hat over h.
+  cloud-in-cell hat over h (or through another axis's node plane, with a
+  sign and an area).
 
 Test infrastructure only."""
 from __future__ import annotations
@@ -181,6 +183,35 @@ class TorchPIC:
         Ef = E.reshape(-1, 3)
         return (w[:, :, None] * Ef[nodes]).sum(1)
 
+    def gather_b(self, B4, x):
+        """A B map (node_shape + (4,): Bx, By, Bz, |B|) at x, gated at the
+        stencil's lower node, |B| > 1e-5 T (Src/part_expmover.f90:151-207):
+        the gathered (N, 3) field and the gate."""
+        nodes, w = self.stencil(x)
+        Bf = B4.reshape(-1, 4)
+        use = Bf[nodes[:, 0], 3] > 1.0e-5
+        return (w[:, :, None] * Bf[nodes][:, :, :3]).sum(1), use
+
+    def boris_map(self, v, Ep, Bp, use, q, m):
+        """Boris with a per-particle B (N, 3); leapfrog where use is False."""
+        k1 = self.dt * q / (2.0 * m)
+        Bx, By, Bz = Bp[:, 0], Bp[:, 1], Bp[:, 2]
+        B2 = Bx * Bx + By * By + Bz * Bz
+        # |B| without sqrt's infinite slope at 0 (where B vanishes the
+        # particle is gated off and its Boris branch is not taken)
+        pos = B2 > 0.0
+        Bt = torch.where(pos, torch.sqrt(torch.where(pos, B2, torch.ones_like(B2))), 0.0)
+        k2 = k1 * (2.0 / (1.0 + (k1 * Bt) * (k1 * Bt)))
+        v1 = v + k1 * Ep
+        v3x = v1[:, 0] + k1 * (v1[:, 1] * Bz - v1[:, 2] * By)
+        v3y = v1[:, 1] + k1 * (v1[:, 2] * Bx - v1[:, 0] * Bz)
+        v3z = v1[:, 2] + k1 * (v1[:, 0] * By - v1[:, 1] * Bx)
+        v2x = v1[:, 0] + k2 * (v3y * Bz - v3z * By)
+        v2y = v1[:, 1] + k2 * (v3z * Bx - v3x * Bz)
+        v2z = v1[:, 2] + k2 * (v3x * By - v3y * Bx)
+        vb = torch.stack([v2x, v2y, v2z], 1) + k1 * Ep
+        return torch.where(use[:, None], vb, v + 2.0 * k1 * Ep)
+
     def boris(self, v, Ep, B0, q, m):
         k1 = self.dt * q / (2.0 * m)
         Bx = torch.zeros((), dtype=torch.float64)
@@ -197,20 +228,25 @@ class TorchPIC:
         v2z = v1[:, 2] + k2 * (v3x * By - v3y * Bx)
         return torch.stack([v2x, v2y, v2z], 1) + k1 * Ep
 
-    def plane_flux(self, x, v, alive, q, Nm):
-        n1, n2, n3 = self.n
+    def plane_flux(self, x, v, alive, q, Nm, flux=None):
+        """flux: (axis, 1-based node plane, sign, area or None); None: the
+        node plane x = (n1 - 1) h1 over the x = L face."""
+        axis, plane, sign, A = flux or (0, self.n[0], 1.0, None)
         h = self.h
-        xp = (n1 - 1) * h[0]
-        S = torch.clamp(1.0 - torch.abs(x[:, 0] - xp) / h[0], min=0.0) / h[0]
-        A = n2 * h[1] * n3 * h[2]
-        return (q * Nm / A) * (v[:, 0] * S * alive.double()).sum()
+        if A is None:
+            A = float(np.prod([self.n[d] * h[d] for d in range(3) if d != axis]))
+        xp = (plane - 1) * h[axis]
+        S = torch.clamp(1.0 - torch.abs(x[:, axis] - xp) / h[axis], min=0.0) / h[axis]
+        return (sign * q * Nm / A) * (v[:, axis] * S * alive.double()).sum()
 
     # -- the run ----------------------------------------------------------
-    def run(self, xs, vs, Vb, B0, steps, schedule=None):
+    def run(self, xs, vs, Vb, B0, steps, schedule=None, B4=None, magnetized=None, flux=None):
         """xs, vs: per species (N, 3) FP64.  schedule: per species, the step
         (1-based) at which each particle dies (0: never); None records it
-        from the wall test (outside [0, L) per axis).  Returns the loss and
-        the schedule."""
+        from the wall test (outside [0, L) per axis).  B4: a B map on the
+        nodes (replaces B0) acting on the species whose magnetized entry is
+        true (default all); flux: the loss plane of plane_flux.  Returns the
+        loss and the schedule."""
         L = torch.tensor([self.n[d] * self.h[d] for d in range(3)], dtype=torch.float64)
         xs = [x.clone() for x in xs]
         vs = [v.clone() for v in vs]
@@ -229,8 +265,15 @@ class TorchPIC:
             E = self.efield(phi)
             for s, (q, m, Nm) in enumerate(self.species):
                 a = alive[s]
-                Ep = self.gather(E, torch.where(a[:, None], xs[s], L / 2))
-                vn = self.boris(vs[s], Ep, B0, q, m)
+                xa = torch.where(a[:, None], xs[s], L / 2)
+                Ep = self.gather(E, xa)
+                if B4 is not None:
+                    Bp, use = self.gather_b(B4, xa)
+                    if magnetized is not None and not magnetized[s]:
+                        use = torch.zeros_like(use)
+                    vn = self.boris_map(vs[s], Ep, Bp, use, q, m)
+                else:
+                    vn = self.boris(vs[s], Ep, B0, q, m)
                 xn = xs[s] + self.dt * vn
                 vs[s] = torch.where(a[:, None], vn, vs[s])
                 xs[s] = torch.where(a[:, None], xn, xs[s])
@@ -239,6 +282,6 @@ class TorchPIC:
                     schedule[s] = torch.where(out, torch.full_like(schedule[s], k), schedule[s])
                 dead_now = schedule[s] == k
                 if s == self.ion:
-                    loss = loss + self.plane_flux(xs[s], vs[s], a & ~dead_now, q, Nm)
+                    loss = loss + self.plane_flux(xs[s], vs[s], a & ~dead_now, q, Nm, flux)
                 alive[s] = a & ~dead_now
         return loss / steps, schedule
