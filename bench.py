@@ -10,9 +10,11 @@ Particles are synthetic (device loader).
 The headline is the step in its precision form, as the workload specifies
 it: FP32 particles (int32 cell + FP32 in-cell offsets + FP32 velocities),
 FP32 gather and push, int64 fixed-point deposit, FP64 fields.  The FP64 step
-is measured beside it in the same run, and the mixed-precision Poisson solve
-(FP32 red-black V-cycles inside FP64 refinement) against the FP64 multigrid
-on its own 256^3 problem.
+is measured beside it in the same run; so are the mixed-precision Poisson
+solves (FP32 red-black V-cycles inside FP64 refinement, and FP64 conjugate
+gradients preconditioned with them) against the FP64 multigrid on their own
+256^3 problem, the combined annular-channel run at 256^3 (392M and 1.07B
+particles) and the forward + adjoint costs of the differentiable runs.
 
 Prints one JSON line: particle-steps/s of the whole step, the roofline of the
 fused particle kernel (HIP events around its launches, on the stream it runs
@@ -256,21 +258,25 @@ def _step_bench(args, precision):
 def _poisson_cfg3(n):
     """The mixed-precision Poisson workload: n^3 cells, six faces biased at
     U[-100, 100] V, 64 Gaussian blobs of charge, solved to 1e-10 (L1 / alpha)
-    from phi = 0 inside, once with the reference's FP64 multigrid and once
-    with FP32 red-black V-cycles inside FP64 refinement; each solve timed
+    from phi = 0 inside, once with the reference's FP64 multigrid, once
+    with FP32 red-black V-cycles inside FP64 refinement and once with FP64
+    conjugate gradients preconditioned with one FP32 V-cycle; each solve timed
     after one untimed solve of the same problem (graph capture).  Errors of
     phi and E (max abs over nodes, relative to the FP64 solution's max)."""
     import torch
     from essaitemporaire_lepic_module_amd import config as cfg
     from essaitemporaire_lepic_module_amd import fields as F
     n3, h, bc, phi0, rhs = cfg.cfg3_poisson(n=n)
+    # preconditioned conjugate gradients with one FP32 V-cycle per iteration
+    pcg = lambda *a: F.solve_poisson_pcg(*a[:9])
     dev = "cuda:0"
     bcg = torch.as_tensor(bc, device=dev)
     rg = torch.as_tensor(rhs, device=dev)
     ng = max(2, int(round(math.log2(n))))
     res = {}
     phis = {}
-    for name, solve in (("fp64", F.solve_poisson_legacy), ("mixed", F.solve_poisson_mixed)):
+    for name, solve in (("fp64", F.solve_poisson_legacy), ("mixed", F.solve_poisson_mixed),
+                        ("pcg", pcg)):
         for rep in range(2):
             phi = torch.as_tensor(phi0, device=dev).clone()
             torch.cuda.synchronize()
@@ -290,6 +296,9 @@ def _poisson_cfg3(n):
     res["phi_err_rel"] = float((phis["mixed"] - phis["fp64"]).abs().max()) / pmax
     res["E_err_rel"] = float((E["mixed"][..., :3] - E["fp64"][..., :3]).abs().max()) / emax
     res["speedup"] = res["fp64"]["ms_per_solve"] / res["mixed"]["ms_per_solve"]
+    res["pcg"]["phi_err_rel"] = float((phis["pcg"] - phis["fp64"]).abs().max()) / pmax
+    res["pcg"]["E_err_rel"] = float((E["pcg"][..., :3] - E["fp64"][..., :3]).abs().max()) / emax
+    res["pcg"]["speedup"] = res["fp64"]["ms_per_solve"] / res["pcg"]["ms_per_solve"]
     res["workload"] = (f"{n}^3 cells, Dirichlet faces U[-100,100] V, 64 Gaussian blobs, "
                        f"tolerance 1e-10, {ng} levels, from phi = 0")
     del phis, E, bcg, rg
@@ -377,7 +386,7 @@ def _adjoint_cfg5(n, ppc=32, steps=500, checkpoint=20, seed=0):
         parts.append({"x": pos[:, 0], "y": pos[:, 1], "z": pos[:, 2],
                       "vx": vel[:, 0], "vy": vel[:, 1], "vz": vel[:, 2]})
     del pos, ci
-    sim = DiffPIC(case, parts, "cuda:0", checkpoint=checkpoint)
+    sim = DiffPIC(case, parts, "cuda:0", checkpoint=checkpoint, poisson="pcg")
     del parts
     sim.forward(300.0, None, 20, keep=True)
     sim.backward()
@@ -403,8 +412,10 @@ def _adjoint_cfg5(n, ppc=32, steps=500, checkpoint=20, seed=0):
            "workload": (f"{n}^3 cells of the combined run's geometry (annular channel, anode "
                         f"300 V, radial Br(z) map), {ppc} ppc x 2 species in the loadable "
                         f"cells, argon MCC every 10 steps (frozen in the backward), {steps} "
-                        f"steps, checkpoint every {checkpoint}, FP64; loss: the mean ion "
-                        "current into the anode through z = h3")}
+                        f"steps, checkpoint every {checkpoint}, FP64 (forward and adjoint "
+                        "Poisson solves by conjugate gradients preconditioned with one FP32 "
+                        "V-cycle, to 1e-10); loss: the mean ion current into the anode through "
+                        "z = h3")}
     del sim
     torch.cuda.empty_cache()
     return out
@@ -413,7 +424,8 @@ def _adjoint_cfg5(n, ppc=32, steps=500, checkpoint=20, seed=0):
 def _combined_cfg5(n, ppc, steps=10, warmup=3):
     """The combined large run (cfg5 forward): n^3 cells, the annular channel
     with the anode at 300 V, the radial Br(z) map, argon MCC every 10 steps,
-    in the precision form with the mixed-precision Poisson solve; `steps`
+    in the precision form with the Poisson solve by conjugate gradients
+    preconditioned with FP32 V-cycles; `steps`
     steps timed by wall clock after `warmup` (electrons re-sorted every 4),
     then as many with every phase between HIP events."""
     import torch
@@ -424,7 +436,7 @@ def _combined_cfg5(n, ppc, steps=10, warmup=3):
     t0 = time.perf_counter()
     case = cfg.cfg5_combined(n=n, np_cell=ppc)
     cells = cfg.loadable_cells(case.bcnd, case.n)
-    sim = Simulation32(case, "cuda:0", capacity_factor=1.25, resort_every=[4, 0], poisson="mixed")
+    sim = Simulation32(case, "cuda:0", capacity_factor=1.25, resort_every=[4, 0], poisson="pcg")
     torch.cuda.synchronize()
     t_init = time.perf_counter() - t0
     istep = 1
@@ -463,7 +475,7 @@ def _combined_cfg5(n, ppc, steps=10, warmup=3):
     out = {"value": (slots - int(holes.sum().item())) / wall, "unit": "particle-steps/s",
            "ms_per_step": wall * 1e3 / steps, "steps": steps, "warmup": warmup,
            "ppc": ppc, "loadable_cells": cells, "particles": n_before,
-           "refinement_steps_per_solve": sum(cycles) / len(cycles),
+           "pcg_iterations_per_solve": sum(cycles) / len(cycles),
            "phase_ms_per_step": {k: v[0] / steps for k, v in phases.items()},
            "push_GBs": (BYTES_PER_PARTICLE_STEP_F32 * push_units / (push_ms * 1e-3) / 1e9
                         if push_ms else 0.0),
@@ -471,7 +483,8 @@ def _combined_cfg5(n, ppc, steps=10, warmup=3):
            "workload": (f"{n}^3 cells, annular channel (r 0.3-0.45 of the width), anode 300 V at "
                         f"z = 0, radial Br(z) map (0.02 T at z = 0.7 L, width 0.1 L), {ppc} ppc "
                         f"x 2 species in the {cells} loadable cells, argon MCC every 10 steps, "
-                        "FP32 particles, fixed-point deposit, mixed-precision Poisson to 1e-10")}
+                        "FP32 particles, fixed-point deposit, Poisson to 1e-10 by FP64 conjugate "
+                        "gradients preconditioned with one FP32 V-cycle")}
     del sim
     torch.cuda.synchronize()
     torch.cuda.empty_cache()

@@ -34,9 +34,9 @@ EXPORTS = (
     "lepic_plane_moments", "lepic_electron_heating", "lepic_dielectric_phi",
     "lepic_pset_to32", "lepic_pset_from32", "lepic_round_field4", "lepic_push32",
     "lepic_deposit32", "lepic_fixed_density", "lepic_sort32", "lepic_collisions32",
-    "lepic_poisson_solve_mixed", "lepic_adj_push", "lepic_adj_push_map", "lepic_adj_efield",
-    "lepic_adj_poisson_rhs", "lepic_adj_bias", "lepic_adj_deposit", "lepic_plane_flux",
-    "lepic_plane_flux_axis", "lepic_apply_deaths",
+    "lepic_poisson_solve_mixed", "lepic_poisson_solve_pcg", "lepic_adj_push",
+    "lepic_adj_push_map", "lepic_adj_efield", "lepic_adj_poisson_rhs", "lepic_adj_bias",
+    "lepic_adj_deposit", "lepic_plane_flux", "lepic_plane_flux_axis", "lepic_apply_deaths",
 )
 
 
@@ -175,6 +175,8 @@ def load() -> C.CDLL:
             "lepic_poisson_solve": (C.c_int, [vp, G, vp, vp, vp, C.POINTER(MGParams),
                                               C.POINTER(MGStats)]),
             "lepic_poisson_solve_mixed": (C.c_int, [vp, G, vp, vp, vp, C.POINTER(MGParams),
+                                                    C.POINTER(MGMixParams), C.POINTER(MGStats)]),
+            "lepic_poisson_solve_pcg": (C.c_int, [vp, G, vp, vp, vp, C.POINTER(MGParams),
                                                     C.POINTER(MGMixParams), C.POINTER(MGStats)]),
             "lepic_efield": (C.c_int, [vp, G, vp, vp, vp]),
             "lepic_efield_noalloc": (C.c_int, [vp, G, vp, vp, vp]),

@@ -88,14 +88,20 @@ class DiffPIC:
 
     case: plasma (-1) and electrode (>= 1) labels only, B uniform along z, a
     B map on the field grid (B_map; dL/dB0 is then 0) or none, collisions (PHILOX draws, keyed by philox_key and the step) or
-    none, no dielectrics or heating.  parts: the initial FP64 particle sets
+    none, no dielectrics or heating.  poisson: "fp64" (the reference's
+    multigrid) or "pcg" (conjugate gradients preconditioned with FP32
+    V-cycles, to the same tolerance) for the forward and adjoint solves.
+    parts: the initial FP64 particle sets
     (host arrays dict per species: x y z vx vy vz), loaded into device sets
     of that many slots plus, with collisions, room for the products
     (product_room of the initial count, at least 1024 slots)."""
 
     def __init__(self, case: Case, parts, device=None, bias_label: int = 2, flux_species: int = 1,
                  checkpoint: int = 20, adjoint_eps: float = 1e-10, philox_key: int = 0x5EED5EED,
-                 product_room: float = 1.0 / 16):
+                 product_room: float = 1.0 / 16, poisson: str = "fp64"):
+        if poisson not in ("fp64", "pcg"):
+            raise ValueError(f"unknown Poisson solve {poisson!r}")
+        self.poisson = poisson
         b = np.asarray(case.bcnd)
         if np.any((b == 0) | (b == -2)) or case.flag_pbc or case.flag_nmn:
             raise ValueError("the adjoint E field takes plasma (-1) and electrode labels only")
@@ -226,9 +232,19 @@ class DiffPIC:
         F.reduce_species_density(self.n, self.bcnd, self.np_thread, self.ntype, self.np_red)
         F.build_rho_from_np(self.n, self.np_red, self.charge, self.ntype, self.rho)
         self._set_electrodes(self.phi, V)
-        F.solve_poisson_legacy(self.phi, self.bcnd, self.rho, self.h, self.n, c.ncycl, c.eps,
-                               c.omega, c.ng, 0, 0)
+        self._solve(self.phi, self.rho, c.eps)
         F.calc_Efield_modular(self.n, self.h, self.phi, self.E4, self.bcnd)
+
+    def _solve(self, u, rhs, eps):
+        """A u = rhs on plasma nodes to eps (warm start from u): the FP64
+        multigrid, or the conjugate gradients preconditioned with FP32
+        V-cycles (poisson="pcg")."""
+        c = self.case
+        if self.poisson == "pcg":
+            F.solve_poisson_pcg(u, self.bcnd, rhs, self.h, self.n, c.ncycl, eps, c.omega, c.ng)
+        else:
+            F.solve_poisson_legacy(u, self.bcnd, rhs, self.h, self.n, c.ncycl, eps, c.omega,
+                                   c.ng, 0, 0)
 
     def _bf(self, s: int, B0):
         """The push's B for species s: the case's map, else B0 along z (None:
@@ -456,8 +472,7 @@ class DiffPIC:
                     ctx.check(ctx.lib.lepic_adj_poisson_rhs(ctx.h, C.byref(g), ptr(self.bcnd),
                                                             ptr(lamphi), sc, ptr(rhs)))
                     mu.mul_(sc)          # the previous step's solution as the first guess
-                    F.solve_poisson_legacy(mu, self.bcnd, rhs, self.h, self.n, c.ncycl,
-                                           self.adj_eps, c.omega, c.ng, 0, 0)
+                    self._solve(mu, rhs, self.adj_eps)
                     mu.mul_(1.0 / sc)
                 else:
                     mu.zero_()

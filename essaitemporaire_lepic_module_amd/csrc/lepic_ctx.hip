@@ -109,6 +109,7 @@ int lepic_destroy(lepic_ctx *ctx) {
     if (e) (void)hipGraphExecDestroy(e);
   if (ctx->mgx_mem) (void)hipFree(ctx->mgx_mem);
   if (ctx->mgx_exec) (void)hipGraphExecDestroy(ctx->mgx_exec);
+  if (ctx->pcg_exec) (void)hipGraphExecDestroy(ctx->pcg_exec);
   if (ctx->capture_stream) (void)hipStreamDestroy(ctx->capture_stream);
   delete ctx;
   return LEPIC_OK;

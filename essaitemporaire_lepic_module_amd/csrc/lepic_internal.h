@@ -46,6 +46,9 @@ struct lepic_ctx {
   size_t mgx_bytes = 0;
   hipGraphExec_t mgx_exec = nullptr;
   std::vector<unsigned char> mgx_key;
+  // the captured iteration of the preconditioned conjugate-gradient solve
+  hipGraphExec_t pcg_exec = nullptr;
+  std::vector<unsigned char> pcg_key;
   // per-XCD deposit replicas (LEPIC_XCDS x nodes doubles, kept zeroed)
   double *rep = nullptr;
   int64_t rep_nodes = 0;

@@ -1140,22 +1140,40 @@ __global__ void k_mix_correct(double *phi, const float *e32, const int32_t *bc, 
 
 }  // namespace
 
-extern "C" int lepic_poisson_solve_mixed(lepic_ctx *ctx, const lepic_grid *g, double *phi,
-                                         const double *rho, const int32_t *bcnd,
-                                         const lepic_mg_params *mp, const lepic_mgmix_params *xp,
-                                         lepic_mg_stats *stats) {
-  if (!ctx) return LEPIC_ERR_ARG;
+// ---------------------------------------------------------------------------
+// Set-up shared by the two solves that smooth in FP32: the FP32 correction
+// hierarchy and the FP64 work arrays on ctx->mgx_mem, laid out as
+// [MIX_COLS x MIX_BLOCKS partial sums][8 scalars][nf64 FP64 node arrays]
+// [e32, r32][periodic residual plane][coarse u, b][coarse labels].
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int MIX_COLS = 4;
+
+struct MixSetup {
+  Solver<float> S;
+  KLevel F64;          // the FP64 fine level: phi, rho, labels
+  double *partials = nullptr, *scal = nullptr, *f64 = nullptr;
+  float *e32 = nullptr, *r32 = nullptr;
+  int64_t nodes = 0;
+  double alpha = 0.0;  // pdesolver's residual normalisation
+  unsigned rb = 0;     // blocks of the FP64 residual kernels
+};
+
+int mix_setup(lepic_ctx *ctx, const lepic_grid *g, double *phi, const double *rho,
+              const int32_t *bcnd, const lepic_mg_params *mp, const lepic_mgmix_params *xp,
+              int nf64, const char *who, MixSetup &M) {
+  const std::string w(who);
   if (!g || !phi || !rho || !bcnd || !mp || !xp)
-    return lepic_fail(ctx, LEPIC_ERR_ARG, "poisson_mixed: null argument");
+    return lepic_fail(ctx, LEPIC_ERR_ARG, w + ": null argument");
   const int ng = mp->ng;
-  if (ng < 1 || ng > MAX_LEVELS)
-    return lepic_fail(ctx, LEPIC_ERR_ARG, "poisson_mixed: ng outside [1,12]");
+  if (ng < 1 || ng > MAX_LEVELS) return lepic_fail(ctx, LEPIC_ERR_ARG, w + ": ng outside [1,12]");
   if (xp->inner_cycles < 1 || xp->inner_cycles > 64 || xp->nu < 1 || xp->nu > 16 ||
       xp->nu_coarsest < 1 || xp->nu_coarsest > 100000 || !(xp->omega > 0.0 && xp->omega < 2.0))
-    return lepic_fail(ctx, LEPIC_ERR_ARG, "poisson_mixed: inner V-cycle parameters out of range");
+    return lepic_fail(ctx, LEPIC_ERR_ARG, w + ": inner V-cycle parameters out of range");
   if (grid_nodes(g) > INT32_MAX)   // thread numbers are decoded in 32 bits
-    return lepic_fail(ctx, LEPIC_ERR_ARG, "poisson_mixed: more than 2^31 nodes");
-  Solver<float> S;
+    return lepic_fail(ctx, LEPIC_ERR_ARG, w + ": more than 2^31 nodes");
+  Solver<float> &S = M.S;
   S.ctx = ctx;
   S.ng = ng;
   S.omega = (float)xp->omega;
@@ -1164,7 +1182,6 @@ extern "C" int lepic_poisson_solve_mixed(lepic_ctx *ctx, const lepic_grid *g, do
   S.fixed = 1;
   S.nr_fixed = xp->nu;
   S.nr_coarsest = xp->nu_coarsest;
-  KLevel F64;   // the FP64 fine level: phi, rho, labels
   int64_t coarse_nodes = 0;
   for (int l = 0; l < ng; ++l) {
     lepic_grid gl;
@@ -1176,23 +1193,24 @@ extern "C" int lepic_poisson_solve_mixed(lepic_ctx *ctx, const lepic_grid *g, do
       for (int d = 0; d < 3; ++d)
         if (gl.n[d] < 2 || (gl.n[d] % 2) != 0)
           return lepic_fail(ctx, LEPIC_ERR_ARG,
-                            "poisson_mixed: every level above the coarsest needs even n >= 2");
+                            w + ": every level above the coarsest needs even n >= 2");
     if (gl.n[0] < 1 || gl.n[1] < 1 || gl.n[2] < 1)
-      return lepic_fail(ctx, LEPIC_ERR_ARG, "poisson_mixed: too many levels for this grid");
+      return lepic_fail(ctx, LEPIC_ERR_ARG, w + ": too many levels for this grid");
     S.L[l].g = make_kgrid(&gl);
     set_level_divs(S.L[l]);
     set_coeffs(S.L[l], mp->eps0);
     if (l == 0) {
-      F64.g = S.L[0].g;
-      set_level_divs(F64);
-      set_coeffs(F64, mp->eps0);
+      M.F64.g = S.L[0].g;
+      set_level_divs(M.F64);
+      set_coeffs(M.F64, mp->eps0);
     } else {
       coarse_nodes += grid_nodes(&gl);
     }
   }
   const int64_t nodes = grid_nodes(g);
   const int64_t rp_len = ((int64_t)g->n[0] + 2) * ((int64_t)g->n[1] + 2);
-  const size_t need = sizeof(double) * (size_t)(MIX_BLOCKS + 8) +
+  const size_t need = sizeof(double) * (size_t)(MIX_COLS * MIX_BLOCKS + 8) +
+                      sizeof(double) * (size_t)nf64 * (size_t)nodes +
                       sizeof(float) * (size_t)(2 * nodes + 2 * coarse_nodes + rp_len) +
                       sizeof(int32_t) * (size_t)coarse_nodes + 4096;
   if (!ctx->mgx_mem || ctx->mgx_bytes < need) {
@@ -1202,21 +1220,24 @@ extern "C" int lepic_poisson_solve_mixed(lepic_ctx *ctx, const lepic_grid *g, do
       ctx->mgx_mem = nullptr;
       ctx->mgx_bytes = 0;
       ctx->mgx_key.clear();
+      ctx->pcg_key.clear();
     }
     LEPIC_HIP(ctx, hipMalloc(&ctx->mgx_mem, need));
     LEPIC_HIP(ctx, hipMemsetAsync(ctx->mgx_mem, 0, need, ctx->stream));
     ctx->mgx_bytes = need;
   }
-  double *partials = reinterpret_cast<double *>(ctx->mgx_mem);
-  double *res_dev = partials + MIX_BLOCKS;
-  float *fmem = reinterpret_cast<float *>(res_dev + 8);
-  float *e32 = fmem, *r32 = fmem + nodes;
-  float *lv = r32 + nodes;
+  M.partials = reinterpret_cast<double *>(ctx->mgx_mem);
+  M.scal = M.partials + MIX_COLS * MIX_BLOCKS;
+  M.f64 = M.scal + 8;
+  float *fmem = reinterpret_cast<float *>(M.f64 + (size_t)nf64 * (size_t)nodes);
+  M.e32 = fmem;
+  M.r32 = fmem + nodes;
+  float *lv = M.r32 + nodes;
   S.rp = lv;
   lv += rp_len;
   int32_t *lb = reinterpret_cast<int32_t *>(lv + 2 * coarse_nodes);
-  S.L[0].u = e32;
-  S.L[0].b = r32;
+  S.L[0].u = M.e32;
+  S.L[0].b = M.r32;
   S.L[0].bc = const_cast<int32_t *>(bcnd);
   for (int l = 1; l < ng; ++l) {
     const int64_t nn = Solver<float>::grid_nodes_k(S.L[l].g);
@@ -1226,15 +1247,38 @@ extern "C" int lepic_poisson_solve_mixed(lepic_ctx *ctx, const lepic_grid *g, do
     S.L[l].bc = lb;
     lb += nn;
   }
-  S.partials = partials;   // unused with fixed sweeps
+  S.partials = M.partials;   // unused with fixed sweeps
   S.ctl = reinterpret_cast<MGCtl *>(ctx->d_ctrl + CTRL_MG);
   S.choose_lc();
-  F64.u = phi;
-  F64.b = const_cast<double *>(rho);
-  F64.bc = const_cast<int32_t *>(bcnd);
-  const double alpha =
-      std::fabs((double)((g->n[0] + 1) * (g->n[1] + 1) * (g->n[2] + 1)) * F64.ac) * 1.0;
-  const unsigned rb = (unsigned)std::min<int64_t>(grid_blocks(nodes, BLOCK), MIX_BLOCKS);
+  M.F64.u = phi;
+  M.F64.b = const_cast<double *>(rho);
+  M.F64.bc = const_cast<int32_t *>(bcnd);
+  M.nodes = nodes;
+  M.alpha =
+      std::fabs((double)((g->n[0] + 1) * (g->n[1] + 1) * (g->n[2] + 1)) * M.F64.ac) * 1.0;
+  M.rb = (unsigned)std::min<int64_t>(grid_blocks(nodes, BLOCK), MIX_BLOCKS);
+  return LEPIC_OK;
+}
+
+}  // namespace
+
+extern "C" int lepic_poisson_solve_mixed(lepic_ctx *ctx, const lepic_grid *g, double *phi,
+                                         const double *rho, const int32_t *bcnd,
+                                         const lepic_mg_params *mp, const lepic_mgmix_params *xp,
+                                         lepic_mg_stats *stats) {
+  if (!ctx) return LEPIC_ERR_ARG;
+  MixSetup M;
+  int st0 = mix_setup(ctx, g, phi, rho, bcnd, mp, xp, 0, "poisson_mixed", M);
+  if (st0) return st0;
+  Solver<float> &S = M.S;
+  KLevel &F64 = M.F64;
+  const int ng = mp->ng;
+  const int64_t nodes = M.nodes;
+  double *partials = M.partials;
+  double *res_dev = M.scal;
+  float *e32 = M.e32, *r32 = M.r32;
+  const double alpha = M.alpha;
+  const unsigned rb = M.rb;
   // the inner V-cycles and the correction of one refinement step: one
   // captured graph per solve configuration
   auto inner = [&]() -> int {
@@ -1334,5 +1378,314 @@ extern "C" int lepic_poisson_solve_mixed(lepic_ctx *ctx, const lepic_grid *g, do
     stats->ktot = ktot;
     stats->res = res;
   }
+  return status;
+}
+
+// ---------------------------------------------------------------------------
+// Preconditioned conjugate gradients: the FP32 V-cycles of the mixed solve
+// used as the preconditioner of an FP64 flexible CG (Notay's FCG with one
+// previous direction kept) on the plasma nodes.  With plasma (-1) and
+// electrode (> 0) labels and no periodic or Neumann walls the operator is
+// the symmetric 7-point Dirichlet operator of sors.f90 restricted to plasma
+// nodes; the V-cycle is applied to A z = r from z = 0 with fixed sweeps.  An
+// iteration: z = M(r); p = z - (<z, A p_old> / <p_old, A p_old>) p_old;
+// q = A p; phi += a p, r -= a q with a = <p, r> / <p, q>.  It stops on the
+// reference's criterion (L1 residual over plasma nodes / alpha < eps),
+// confirmed on the residual recomputed from phi.
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ inline double block_sum_d(double a, double *red) {
+  for (int d = LEPIC_WAVE / 2; d > 0; d >>= 1) a = a + __shfl_down(a, d);
+  if (__lane_id() == 0) red[threadIdx.x / LEPIC_WAVE] = a;
+  __syncthreads();
+  double b = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < BLOCK / LEPIC_WAVE; ++w) b = b + red[w];
+  __syncthreads();
+  return b;
+}
+
+// The residual from phi (start and restarts): r = b - A phi on plasma nodes,
+// 0 elsewhere; the directions p, q cleared; r32 = r, e32 = 0; the L1 sum to
+// partials column 0; a physical node labelled 0 or -2, or a ghost node
+// labelled -1, raises *bad.
+__global__ __launch_bounds__(BLOCK) void k_pcg_resid(KLevel L, MDiv dy, double *r, double *p,
+                                                     double *q, float *r32, float *e32,
+                                                     double *partials, int *bad, int64_t nodes) {
+  __shared__ double red[BLOCK / LEPIC_WAVE];
+  const KGrid &g = L.g;
+  double a = 0.0;
+  int flag = 0;
+  const int64_t stride = (int64_t)gridDim.x * BLOCK;
+  for (int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x; t < nodes; t += stride) {
+    const uint32_t tt = (uint32_t)t, row = mdiv_q(tt, L.d_n1p3), kk = mdiv_q(row, dy);
+    const int i = (int)(tt - row * (uint32_t)(g.n1 + 3));
+    const int j = (int)(row - kk * (uint32_t)(g.n2 + 3)), k = (int)kk;
+    double rv = 0.0;
+    if (i >= 1 && i <= g.n1 + 1 && j >= 1 && j <= g.n2 + 1 && k >= 1 && k <= g.n3 + 1) {
+      const int bcv = L.bc[t];
+      const double *u = L.u;
+      if (bcv == -1) {
+        rv = L.b[t] - (L.aw * u[t - 1] + L.an * u[t + g.sy] + L.as * u[t - g.sy] +
+                       L.ae * u[t + 1] + L.ac * u[t] + L.at * u[t + g.sz] + L.ab * u[t - g.sz]);
+        a = a + fabs(rv);
+      } else if (bcv == 0 || bcv == -2) {
+        flag = 1;
+      }
+    } else if (L.bc[t] == -1) {
+      flag = 1;   // a plasma label on a ghost node: its stencil would leave the array
+    }
+    r[t] = rv;
+    p[t] = 0.0;
+    q[t] = 0.0;
+    r32[t] = (float)rv;
+    e32[t] = 0.0f;
+  }
+  const double s = block_sum_d(a, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+  if (__syncthreads_or(flag) && threadIdx.x == 0) atomicOr(bad, 1);
+}
+
+// <z, q_old> (z = e32, zero off the plasma nodes) to partials column 1.
+__global__ __launch_bounds__(BLOCK) void k_pcg_zq(const float *e32, const double *q,
+                                                  double *partials, int64_t nodes) {
+  __shared__ double red[BLOCK / LEPIC_WAVE];
+  double a = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * BLOCK;
+  for (int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x; t < nodes; t += stride)
+    a = a + (double)e32[t] * q[t];
+  const double s = block_sum_d(a, red);
+  if (threadIdx.x == 0) partials[MIX_BLOCKS + blockIdx.x] = s;
+}
+
+// The new direction p_new = z - beta p (beta = scal[1] / scal[2], the
+// previous <p, q>) on plasma nodes, 0 elsewhere, and q = A p_new from the
+// neighbours' p_new formed on the fly (z and p are 0 off the plasma nodes);
+// <p_new, q> and <p_new, r> to partials columns 2 and 3.
+__global__ __launch_bounds__(BLOCK) void k_pcg_dir(KLevel L, const float *e32, const double *p,
+                                                   double *pn_out, double *q, const double *r,
+                                                   const double *scal, double *partials,
+                                                   int64_t nodes) {
+  __shared__ double red[BLOCK / LEPIC_WAVE];
+  const KGrid &g = L.g;
+  const double beta = scal[2] != 0.0 ? scal[1] / scal[2] : 0.0;
+  double apq = 0.0, apr = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * BLOCK;
+  for (int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x; t < nodes; t += stride) {
+    double pn = 0.0, qv = 0.0;
+    if (L.bc[t] == -1) {
+      auto P = [&](int64_t u) { return (double)e32[u] - beta * p[u]; };
+      pn = P(t);
+      qv = L.aw * P(t - 1) + L.an * P(t + g.sy) + L.as * P(t - g.sy) + L.ae * P(t + 1) +
+           L.ac * pn + L.at * P(t + g.sz) + L.ab * P(t - g.sz);
+      apq = apq + pn * qv;
+      apr = apr + pn * r[t];
+    }
+    pn_out[t] = pn;
+    q[t] = qv;
+  }
+  const double s1 = block_sum_d(apq, red);
+  const double s2 = block_sum_d(apr, red);
+  if (threadIdx.x == 0) {
+    partials[2 * MIX_BLOCKS + blockIdx.x] = s1;
+    partials[3 * MIX_BLOCKS + blockIdx.x] = s2;
+  }
+}
+
+// phi += a p_new, r -= a q (a = scal[3] / scal[2]) on plasma nodes; r32 = r,
+// e32 = 0, p = p_new; the L1 sum of r to partials column 0.
+__global__ __launch_bounds__(BLOCK) void k_pcg_update(double *phi, const int32_t *bc,
+                                                      const double *pn, const double *q,
+                                                      double *p, double *r, float *r32,
+                                                      float *e32, const double *scal,
+                                                      double *partials, int64_t nodes) {
+  __shared__ double red[BLOCK / LEPIC_WAVE];
+  const double a = scal[2] != 0.0 ? scal[3] / scal[2] : 0.0;
+  double l1 = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * BLOCK;
+  for (int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x; t < nodes; t += stride) {
+    const double pv = pn[t];
+    if (bc[t] == -1) {
+      phi[t] = phi[t] + a * pv;
+      const double rv = r[t] - a * q[t];
+      r[t] = rv;
+      r32[t] = (float)rv;
+      l1 = l1 + fabs(rv);
+    }
+    p[t] = pv;
+    e32[t] = 0.0f;
+  }
+  const double s = block_sum_d(l1, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// scal[first + c] = the sum of partials column first + c (one block each).
+__global__ __launch_bounds__(BLOCK) void k_pcg_sum(const double *partials, int64_t nparts,
+                                                   int first, double *scal) {
+  __shared__ double red[BLOCK];
+  const int col = first + (int)blockIdx.x;
+  const double *pc = partials + (int64_t)col * MIX_BLOCKS;
+  double a = 0.0;
+  for (int64_t i = threadIdx.x; i < nparts; i += BLOCK) a = a + pc[i];
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = BLOCK / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) scal[col] = red[0];
+}
+
+__global__ void k_pcg_clear_flag(int *bad) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *bad = 0;
+}
+
+}  // namespace
+
+extern "C" int lepic_poisson_solve_pcg(lepic_ctx *ctx, const lepic_grid *g, double *phi,
+                                       const double *rho, const int32_t *bcnd,
+                                       const lepic_mg_params *mp, const lepic_mgmix_params *xp,
+                                       lepic_mg_stats *stats) {
+  if (!ctx) return LEPIC_ERR_ARG;
+  if (mp && (mp->flag_pbc || mp->flag_nmn))
+    return lepic_fail(ctx, LEPIC_ERR_ARG,
+                      "poisson_pcg: periodic or Neumann walls (the operator is not symmetric)");
+  MixSetup M;
+  // FP64 arrays: r, p, p_new, q
+  int st0 = mix_setup(ctx, g, phi, rho, bcnd, mp, xp, 4, "poisson_pcg", M);
+  if (st0) return st0;
+  Solver<float> &S = M.S;
+  KLevel &F64 = M.F64;
+  const int ng = mp->ng;
+  const int64_t nodes = M.nodes;
+  double *r = M.f64, *p = r + nodes, *pn = p + nodes, *q = pn + nodes;
+  double *scal = M.scal;
+  int *bad = reinterpret_cast<int *>(scal + 4);
+  const unsigned rb = M.rb;
+  const MDiv dy = mdiv((uint32_t)(g->n[1] + 3));
+  hipStream_t &s = ctx->stream;
+  auto iteration = [&]() -> int {
+    for (int c = 0; c < xp->inner_cycles; ++c) {
+      int st = S.vcycle(2, 0.0);
+      if (st) return st;
+    }
+    hipLaunchKernelGGL(k_pcg_zq, dim3(rb), dim3(BLOCK), 0, s, M.e32, q, M.partials, nodes);
+    hipLaunchKernelGGL(k_pcg_sum, dim3(1), dim3(BLOCK), 0, s, M.partials, (int64_t)rb, 1, scal);
+    hipLaunchKernelGGL(k_pcg_dir, dim3(rb), dim3(BLOCK), 0, s, F64, M.e32, p, pn, q, r, scal,
+                       M.partials, nodes);
+    hipLaunchKernelGGL(k_pcg_sum, dim3(2), dim3(BLOCK), 0, s, M.partials, (int64_t)rb, 2, scal);
+    hipLaunchKernelGGL(k_pcg_update, dim3(rb), dim3(BLOCK), 0, s, phi, bcnd, pn, q, p, r, M.r32,
+                       M.e32, scal, M.partials, nodes);
+    hipLaunchKernelGGL(k_pcg_sum, dim3(1), dim3(BLOCK), 0, s, M.partials, (int64_t)rb, 0, scal);
+    return LEPIC_OK;
+  };
+  const bool replay = mp->graph != 0;
+  if (replay) {
+    const int32_t ints[10] = {ng, g->n[0], g->n[1], g->n[2], 0, 0,
+                              xp->inner_cycles, xp->nu, xp->nu_coarsest, S.lc};
+    const void *ptrs[4] = {phi, bcnd, ctx->mgx_mem, ctx->d_ctrl};
+    const double dbl[5] = {g->h[0], g->h[1], g->h[2], xp->omega, mp->eps0};
+    std::vector<unsigned char> key(sizeof(ints) + sizeof(ptrs) + sizeof(dbl));
+    std::memcpy(key.data(), ints, sizeof(ints));
+    std::memcpy(key.data() + sizeof(ints), ptrs, sizeof(ptrs));
+    std::memcpy(key.data() + sizeof(ints) + sizeof(ptrs), dbl, sizeof(dbl));
+    if (key != ctx->pcg_key || !ctx->pcg_exec) {
+      if (ctx->pcg_exec) {
+        LEPIC_HIP(ctx, hipGraphExecDestroy(ctx->pcg_exec));
+        ctx->pcg_exec = nullptr;
+      }
+      ctx->pcg_key.clear();
+      if (!ctx->capture_stream)
+        LEPIC_HIP(ctx, hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+      const hipStream_t eager = ctx->stream;
+      ctx->stream = ctx->capture_stream;   // the Solver and the lambda launch on ctx->stream
+      int st = LEPIC_OK;
+      hipGraph_t graph = nullptr;
+      hipError_t e = hipStreamBeginCapture(ctx->capture_stream, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        st = iteration();
+        e = hipStreamEndCapture(ctx->capture_stream, &graph);
+      }
+      if (e == hipSuccess && st == LEPIC_OK)
+        e = hipGraphInstantiate(&ctx->pcg_exec, graph, nullptr, nullptr, 0);
+      if (graph) (void)hipGraphDestroy(graph);
+      ctx->stream = eager;
+      if (st) return st;
+      if (e != hipSuccess) return lepic_fail(ctx, LEPIC_ERR_HIP, hipGetErrorString(e));
+      ctx->pcg_key = key;
+    }
+  }
+  // r from phi (and the directions cleared), read back with the label flag
+  auto restart = [&]() -> int {
+    hipLaunchKernelGGL(k_pcg_clear_flag, dim3(1), dim3(64), 0, s, bad);
+    hipLaunchKernelGGL(k_pcg_resid, dim3(rb), dim3(BLOCK), 0, s, F64, dy, r, p, q, M.r32, M.e32,
+                       M.partials, bad, nodes);
+    hipLaunchKernelGGL(k_pcg_sum, dim3(1), dim3(BLOCK), 0, s, M.partials, (int64_t)rb, 0, scal);
+    LEPIC_LAUNCH_CHECK(ctx, "pcg residual");
+    return LEPIC_OK;
+  };
+  auto read_back = [&](double &res, int &flag) -> int {
+    LEPIC_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, scal, 5 * sizeof(double), hipMemcpyDeviceToHost,
+                                  s));
+    LEPIC_HIP(ctx, hipStreamSynchronize(s));
+    double v;
+    std::memcpy(&v, &ctx->h_pinned[0], sizeof(double));
+    std::memcpy(&flag, &ctx->h_pinned[4], sizeof(int));
+    res = v / M.alpha;
+    return LEPIC_OK;
+  };
+  hipLaunchKernelGGL(k_reset_totals, dim3(1), dim3(64), 0, s, S.ctl, MAX_LEVELS);
+  int st = restart();
+  if (st) return st;
+  double res = 0.0;
+  int flag = 0;
+  if ((st = read_back(res, flag))) return st;
+  if (flag)
+    return lepic_fail(ctx, LEPIC_ERR_ARG,
+                      "poisson_pcg: labels 0 or -2 (the operator is not symmetric), or plasma "
+                      "on a ghost node");
+  const double res_s = res;
+  int k = 0, restarts = 0;
+  bool fresh = true;   // r was just formed from phi
+  int status = LEPIC_OK;
+  for (;;) {
+    if (res < mp->eps) {
+      if (fresh) break;
+      if ((st = restart())) return st;   // confirm on the residual from phi
+      if ((st = read_back(res, flag))) return st;
+      fresh = true;
+      ++restarts;
+      if (res < mp->eps) break;
+    }
+    if (k == mp->ncycl) break;
+    ++k;
+    if (replay) {
+      LEPIC_HIP(ctx, hipGraphLaunch(ctx->pcg_exec, s));
+    } else if ((st = iteration())) {
+      return st;
+    }
+    LEPIC_LAUNCH_CHECK(ctx, "pcg iteration");
+    if ((st = read_back(res, flag))) return st;
+    fresh = false;
+    if (!(res <= 10.0 * res_s)) {
+      status = lepic_fail(ctx, LEPIC_ERR_DIVERGED, "poisson_pcg: res/res_s > 10 (diverging)");
+      break;
+    }
+  }
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    double ktot = 0.0;
+    for (int l = 0; l < ng; ++l) {
+      const int64_t per = (l == ng - 1) ? xp->nu_coarsest : 2 * xp->nu;
+      const int64_t tot = per * (int64_t)xp->inner_cycles * k;
+      stats->sweeps[l] = tot;
+      ktot = ktot + (double)tot / std::pow(2.0, 2.0 * l);
+    }
+    stats->cycles = k;
+    stats->ktot = ktot;
+    stats->res = res;
+  }
+  (void)restarts;
   return status;
 }

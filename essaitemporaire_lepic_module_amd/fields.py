@@ -151,6 +151,34 @@ def solve_poisson_mixed(phi: torch.Tensor, bcnd: torch.Tensor, rhs: torch.Tensor
             "sweeps": [st.sweeps[i] for i in range(int(ng))]}
 
 
+def solve_poisson_pcg(phi: torch.Tensor, bcnd: torch.Tensor, rhs: torch.Tensor, h, n,
+                      ncycl: int, eps: float, omega: float, ng: int, flag_pbc: int = 0,
+                      flag_nmn: int = 0, eps0: float = EPS0, inner_cycles: int = 1, nu: int = 2,
+                      nu_coarsest: int = 64, omega_smooth: float = 1.15,
+                      graph: bool = True) -> dict:
+    """The Poisson solve by FP64 conjugate gradients preconditioned with the
+    mixed solve's FP32 V-cycles (inner_cycles per iteration), stopped as
+    solve_poisson_legacy stops; plasma and electrode labels only, no
+    periodic or Neumann walls.  Returns the iterations ("cycles"), equivalent
+    fine sweeps, the residual and the FP32 sweeps per level."""
+    if flag_pbc or flag_nmn:
+        raise ValueError("the conjugate-gradient solve takes no periodic or Neumann walls")
+    ctx = context(phi.device)
+    g = _native.grid(n, h)
+    mp = _native.MGParams()
+    mp.flag_pbc, mp.flag_nmn, mp.eps0 = 0, 0, float(eps0)
+    mp.ncycl, mp.eps, mp.omega, mp.ng = int(ncycl), float(eps), float(omega), int(ng)
+    mp.graph = 1 if graph else 0
+    xp = _native.MGMixParams()
+    xp.inner_cycles, xp.nu, xp.nu_coarsest = int(inner_cycles), int(nu), int(nu_coarsest)
+    xp.omega = float(omega_smooth)
+    st = _native.MGStats()
+    ctx.check(ctx.lib.lepic_poisson_solve_pcg(ctx.h, C.byref(g), ptr(phi), ptr(rhs), ptr(bcnd),
+                                              C.byref(mp), C.byref(xp), C.byref(st)))
+    return {"cycles": st.cycles, "ktot": st.ktot, "res": st.res,
+            "sweeps": [st.sweeps[i] for i in range(int(ng))]}
+
+
 def apply_dielectric_bc_to_phi(n, h, bcnd: torch.Tensor, dtype: torch.Tensor, ngrid: int,
                                sum_q_xz: torch.Tensor, sum_q_yz: torch.Tensor, Ca_cell: float,
                                phi: torch.Tensor):

@@ -257,6 +257,22 @@ int lepic_poisson_solve_mixed(lepic_ctx *ctx, const lepic_grid *g, double *phi, 
                               const int32_t *bcnd, const lepic_mg_params *mp,
                               const lepic_mgmix_params *xp, lepic_mg_stats *stats);
 
+/* Preconditioned conjugate gradients in FP64 with the FP32 V-cycles above
+ * as the preconditioner (a flexible CG keeping one previous direction): each
+ * iteration applies inner_cycles fixed-sweep FP32 V-cycles to A z = r from
+ * z = 0, makes the new direction A-orthogonal to the previous one, and
+ * updates phi and r along it.  Stops on the same criterion as the solves
+ * above (FP64 L1 residual over plasma nodes / alpha < mp->eps, confirmed on
+ * the residual recomputed from phi) or after mp->ncycl iterations;
+ * stats->cycles counts iterations.  The operator must be symmetric: plasma
+ * (-1) and electrode (> 0) labels only, no periodic or Neumann walls
+ * (LEPIC_ERR_ARG otherwise).  Fewer V-cycles than the refinement above where
+ * the electrodes' shapes slow the multigrid (curved or internal
+ * electrodes).  (No counterpart in the reference.) */
+int lepic_poisson_solve_pcg(lepic_ctx *ctx, const lepic_grid *g, double *phi, const double *rho,
+                            const int32_t *bcnd, const lepic_mg_params *mp,
+                            const lepic_mgmix_params *xp, lepic_mg_stats *stats);
+
 /* ---------------- a9: E field ---------------- */
 /* calc_Efield_modular (mod_electricField.f90:9-164). */
 int lepic_efield(lepic_ctx *ctx, const lepic_grid *g, const double *phi, const int32_t *bcnd,

@@ -529,3 +529,23 @@ def test_annular_map_gradient_matches_central_differences(gpu):
     print(f"collided {ncoll}; dL/dV adjoint {dV:.10e} frozen fd {fV:.10e} free fd {gV:.10e}")
     assert fV != 0.0
     assert abs(dV - fV) <= 1e-4 * abs(fV)
+
+
+def test_pcg_solves_leave_gradient_unchanged(gpu):
+    """The differentiable run with its forward and adjoint Poisson solves by
+    the preconditioned conjugate gradients (to the same 1e-10): the loss and
+    dL/dV of the annular channel agree with the multigrid run's to the
+    solves' tolerance."""
+    from essaitemporaire_lepic_module_amd.adjoint import DiffPIC
+    rng = np.random.default_rng(8)
+    case = cfg.cfg5_combined(n=16, np_cell=4, steps=10)
+    parts = _initial_channel(rng, case)
+    out = []
+    for poisson in ("fp64", "pcg"):
+        sim = DiffPIC(case, parts, "cuda:0", checkpoint=4, poisson=poisson)
+        L = sim.forward(300.0, None, 10, keep=True)
+        dV, _ = sim.backward()
+        out.append((L, dV))
+    print(out)
+    assert abs(out[1][0] - out[0][0]) <= 1e-7 * abs(out[0][0])
+    assert abs(out[1][1] - out[0][1]) <= 1e-6 * abs(out[0][1])
