@@ -498,10 +498,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--ppc", type=int, default=64)
-    ap.add_argument("--precision", choices=("f32", "f64", "both"), default="both",
+    ap.add_argument("--precision", choices=("f32", "f64", "both", "none"), default="both",
                     help="f32: the step's precision form (FP32 particles, fixed-point deposit, "
                          "FP64 Poisson) as the headline; f64: the FP64 step; both: f32 headline "
-                         "with the FP64 step measured beside it")
+                         "with the FP64 step measured beside it; none: no cfg2 step, only the "
+                         "other legs (for profiling them alone; the line then has no value)")
     ap.add_argument("--poisson-n", type=int, default=256,
                     help="grid of the mixed-precision Poisson comparison (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -535,6 +536,11 @@ def main():
     from essaitemporaire_lepic_module_amd import config as cfg
 
     torch.cuda.init()
+    if args.precision == "none":
+        line = {"metric": "legs only (no cfg2 step)", "value": None}
+        _other_legs(args, line)
+        print(json.dumps(line))
+        return
     head = "f64" if args.precision == "f64" else "f32"
     main_run = _step_bench(args, head)
     print(f"[bench] {head} step: {main_run['ms_per_step']:.2f} ms/step", file=sys.stderr,
@@ -583,6 +589,16 @@ def main():
     if other is not None:
         other["dtype"] = "f64"
         line["fp64_step"] = other
+    _other_legs(args, line)
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(case)
+    print(json.dumps(line))
+
+
+def _other_legs(args, line):
+    """The legs beside the cfg2 step: cfg3's Poisson solves, cfg5's combined
+    run and its forward + adjoint, cfg4's forward + adjoint."""
+    from essaitemporaire_lepic_module_amd import config as cfg
     if args.poisson_n:
         line["poisson_mixed"] = _poisson_cfg3(args.poisson_n)
         print("[bench] poisson done", file=sys.stderr, flush=True)
@@ -592,7 +608,6 @@ def main():
         if args.combined_box_count:
             # the same run with the particle count of 32 ppc x 2 species over
             # every cell of the box (the channel holds about a third of them)
-            from essaitemporaire_lepic_module_amd import config as cfg
             n = args.combined_n
             cells = cfg.loadable_cells(cfg.annular_channel_bcnd((n,) * 3, (3.125e-4,) * 3),
                                        (n,) * 3)
@@ -604,9 +619,6 @@ def main():
     if args.combined_adjoint_n:
         line["combined_adjoint"] = _adjoint_cfg5(args.combined_adjoint_n)
         print("[bench] combined adjoint done", file=sys.stderr, flush=True)
-    if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(case)
-    print(json.dumps(line))
 
 
 if __name__ == "__main__":
