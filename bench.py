@@ -171,7 +171,8 @@ def _step_bench(args, precision):
     resort = None if args.exact_order or args.resort_electrons <= 0 else \
         [args.resort_electrons] + [0] * (len(case.tracked) - 1)
     if precision == "f32":
-        sim = Simulation32(case, "cuda:0", capacity_factor=1.25, resort_every=resort)
+        sim = Simulation32(case, "cuda:0", capacity_factor=1.25, resort_every=resort,
+                           poisson=args.step_poisson)
         kernel, bpp = "k_push32", BYTES_PER_PARTICLE_STEP_F32
     else:
         sim = Simulation(case, "cuda:0", fused=True, loader="device",
@@ -503,6 +504,10 @@ def main():
                          "FP64 Poisson) as the headline; f64: the FP64 step; both: f32 headline "
                          "with the FP64 step measured beside it; none: no cfg2 step, only the "
                          "other legs (for profiling them alone; the line then has no value)")
+    ap.add_argument("--step-poisson", choices=("fp64", "pcg64"), default="fp64",
+                    help="the precision-form step's Poisson solve: the reference's FP64 "
+                         "multigrid, or FP64 conjugate gradients preconditioned with one FP64 "
+                         "V-cycle of the same multigrid (FP64 throughout, same stopping rule)")
     ap.add_argument("--poisson-n", type=int, default=256,
                     help="grid of the mixed-precision Poisson comparison (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -549,14 +554,23 @@ def main():
     if args.precision == "both":
         other = _step_bench(args, "f64")
         print(f"[bench] f64 step: {other['ms_per_step']:.2f} ms/step", file=sys.stderr, flush=True)
+    alt = None
+    if args.precision == "both" and args.step_poisson == "fp64":
+        # the headline's step with its Poisson solve by FP64 conjugate gradients
+        alt_args = argparse.Namespace(**{**vars(args), "step_poisson": "pcg64"})
+        alt = _step_bench(alt_args, "f32")
+        print(f"[bench] f32 step, CG solve: {alt['ms_per_step']:.2f} ms/step", file=sys.stderr,
+              flush=True)
     if args.copy_probe:
         a = torch.ones(COPY_PROBE_BYTES // 8, dtype=torch.float64, device="cuda:0")
         torch.empty_like(a).copy_(a)
         torch.cuda.synchronize()
         del a
     case = cfg.cfg2_pic_mcc(n=args.n, np_cell=args.ppc)
-    prec = ("FP32 particles (int32 cell + FP32 offsets), int64 fixed-point deposit, FP64 "
-            "Poisson" if head == "f32" else "FP64 particles and fields")
+    solve = ("FP64 multigrid" if args.step_poisson == "fp64" else
+             "FP64 conjugate gradients preconditioned with one FP64 V-cycle")
+    prec = (f"FP32 particles (int32 cell + FP32 offsets), int64 fixed-point deposit, FP64 "
+            f"Poisson ({solve}, to 1e-8)" if head == "f32" else "FP64 particles and fields")
     line = {
         "metric": "particle-steps/s of the 128^3 x 64 ppc x 2 species PIC-MCC step",
         "value": main_run["value"],
@@ -589,6 +603,13 @@ def main():
     if other is not None:
         other["dtype"] = "f64"
         line["fp64_step"] = other
+    if alt is not None:
+        line["f32_step_cg_poisson"] = {
+            k: alt[k] for k in ("value", "ms_per_step", "step_ms_median", "mg_solve_ms",
+                                "mg_vcycles_per_solve", "phase_ms_per_step")}
+        line["f32_step_cg_poisson"]["poisson"] = (
+            "FP64 conjugate gradients preconditioned with one FP64 V-cycle, the same L1 "
+            "residual criterion (1e-8); mg_vcycles_per_solve counts CG iterations")
     _other_legs(args, line)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(case)

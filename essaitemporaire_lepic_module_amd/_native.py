@@ -105,7 +105,7 @@ class MGParams(C.Structure):
 
 class MGMixParams(C.Structure):
     _fields_ = [("inner_cycles", C.c_int32), ("nu", C.c_int32), ("nu_coarsest", C.c_int32),
-                ("omega", C.c_double)]
+                ("omega", C.c_double), ("precond_fp64", C.c_int32)]
 
 
 class MGStats(C.Structure):

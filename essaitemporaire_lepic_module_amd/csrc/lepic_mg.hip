@@ -1150,19 +1150,21 @@ namespace {
 
 constexpr int MIX_COLS = 4;
 
+template <class T>
 struct MixSetup {
-  Solver<float> S;
+  Solver<T> S;         // the correction hierarchy (T: FP32, or FP64 for the CG preconditioner)
   KLevel F64;          // the FP64 fine level: phi, rho, labels
   double *partials = nullptr, *scal = nullptr, *f64 = nullptr;
-  float *e32 = nullptr, *r32 = nullptr;
+  T *e32 = nullptr, *r32 = nullptr;
   int64_t nodes = 0;
   double alpha = 0.0;  // pdesolver's residual normalisation
   unsigned rb = 0;     // blocks of the FP64 residual kernels
 };
 
+template <class T>
 int mix_setup(lepic_ctx *ctx, const lepic_grid *g, double *phi, const double *rho,
               const int32_t *bcnd, const lepic_mg_params *mp, const lepic_mgmix_params *xp,
-              int nf64, const char *who, MixSetup &M) {
+              int nf64, const char *who, MixSetup<T> &M) {
   const std::string w(who);
   if (!g || !phi || !rho || !bcnd || !mp || !xp)
     return lepic_fail(ctx, LEPIC_ERR_ARG, w + ": null argument");
@@ -1173,10 +1175,10 @@ int mix_setup(lepic_ctx *ctx, const lepic_grid *g, double *phi, const double *rh
     return lepic_fail(ctx, LEPIC_ERR_ARG, w + ": inner V-cycle parameters out of range");
   if (grid_nodes(g) > INT32_MAX)   // thread numbers are decoded in 32 bits
     return lepic_fail(ctx, LEPIC_ERR_ARG, w + ": more than 2^31 nodes");
-  Solver<float> &S = M.S;
+  Solver<T> &S = M.S;
   S.ctx = ctx;
   S.ng = ng;
-  S.omega = (float)xp->omega;
+  S.omega = (T)xp->omega;
   S.flag_pbc = mp->flag_pbc;
   S.flag_nmn = mp->flag_nmn;
   S.fixed = 1;
@@ -1211,7 +1213,7 @@ int mix_setup(lepic_ctx *ctx, const lepic_grid *g, double *phi, const double *rh
   const int64_t rp_len = ((int64_t)g->n[0] + 2) * ((int64_t)g->n[1] + 2);
   const size_t need = sizeof(double) * (size_t)(MIX_COLS * MIX_BLOCKS + 8) +
                       sizeof(double) * (size_t)nf64 * (size_t)nodes +
-                      sizeof(float) * (size_t)(2 * nodes + 2 * coarse_nodes + rp_len) +
+                      sizeof(T) * (size_t)(2 * nodes + 2 * coarse_nodes + rp_len) +
                       sizeof(int32_t) * (size_t)coarse_nodes + 4096;
   if (!ctx->mgx_mem || ctx->mgx_bytes < need) {
     if (ctx->mgx_mem) {
@@ -1229,10 +1231,10 @@ int mix_setup(lepic_ctx *ctx, const lepic_grid *g, double *phi, const double *rh
   M.partials = reinterpret_cast<double *>(ctx->mgx_mem);
   M.scal = M.partials + MIX_COLS * MIX_BLOCKS;
   M.f64 = M.scal + 8;
-  float *fmem = reinterpret_cast<float *>(M.f64 + (size_t)nf64 * (size_t)nodes);
+  T *fmem = reinterpret_cast<T *>(M.f64 + (size_t)nf64 * (size_t)nodes);
   M.e32 = fmem;
   M.r32 = fmem + nodes;
-  float *lv = M.r32 + nodes;
+  T *lv = M.r32 + nodes;
   S.rp = lv;
   lv += rp_len;
   int32_t *lb = reinterpret_cast<int32_t *>(lv + 2 * coarse_nodes);
@@ -1240,7 +1242,7 @@ int mix_setup(lepic_ctx *ctx, const lepic_grid *g, double *phi, const double *rh
   S.L[0].b = M.r32;
   S.L[0].bc = const_cast<int32_t *>(bcnd);
   for (int l = 1; l < ng; ++l) {
-    const int64_t nn = Solver<float>::grid_nodes_k(S.L[l].g);
+    const int64_t nn = Solver<T>::grid_nodes_k(S.L[l].g);
     S.L[l].u = lv;
     S.L[l].b = lv + nn;
     lv += 2 * nn;
@@ -1267,7 +1269,7 @@ extern "C" int lepic_poisson_solve_mixed(lepic_ctx *ctx, const lepic_grid *g, do
                                          const lepic_mg_params *mp, const lepic_mgmix_params *xp,
                                          lepic_mg_stats *stats) {
   if (!ctx) return LEPIC_ERR_ARG;
-  MixSetup M;
+  MixSetup<float> M;
   int st0 = mix_setup(ctx, g, phi, rho, bcnd, mp, xp, 0, "poisson_mixed", M);
   if (st0) return st0;
   Solver<float> &S = M.S;
@@ -1410,8 +1412,9 @@ __device__ inline double block_sum_d(double a, double *red) {
 // 0 elsewhere; the directions p, q cleared; r32 = r, e32 = 0; the L1 sum to
 // partials column 0; a physical node labelled 0 or -2, or a ghost node
 // labelled -1, raises *bad.
+template <class T>
 __global__ __launch_bounds__(BLOCK) void k_pcg_resid(KLevel L, MDiv dy, double *r, double *p,
-                                                     double *q, float *r32, float *e32,
+                                                     double *q, T *r32, T *e32,
                                                      double *partials, int *bad, int64_t nodes) {
   __shared__ double red[BLOCK / LEPIC_WAVE];
   const KGrid &g = L.g;
@@ -1439,8 +1442,8 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_resid(KLevel L, MDiv dy, double *
     r[t] = rv;
     p[t] = 0.0;
     q[t] = 0.0;
-    r32[t] = (float)rv;
-    e32[t] = 0.0f;
+    r32[t] = (T)rv;
+    e32[t] = T(0);
   }
   const double s = block_sum_d(a, red);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
@@ -1448,7 +1451,8 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_resid(KLevel L, MDiv dy, double *
 }
 
 // <z, q_old> (z = e32, zero off the plasma nodes) to partials column 1.
-__global__ __launch_bounds__(BLOCK) void k_pcg_zq(const float *e32, const double *q,
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_pcg_zq(const T *e32, const double *q,
                                                   double *partials, int64_t nodes) {
   __shared__ double red[BLOCK / LEPIC_WAVE];
   double a = 0.0;
@@ -1463,7 +1467,8 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_zq(const float *e32, const double
 // previous <p, q>) on plasma nodes, 0 elsewhere, and q = A p_new from the
 // neighbours' p_new formed on the fly (z and p are 0 off the plasma nodes);
 // <p_new, q> and <p_new, r> to partials columns 2 and 3.
-__global__ __launch_bounds__(BLOCK) void k_pcg_dir(KLevel L, const float *e32, const double *p,
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_pcg_dir(KLevel L, const T *e32, const double *p,
                                                    double *pn_out, double *q, const double *r,
                                                    const double *scal, double *partials,
                                                    int64_t nodes) {
@@ -1495,10 +1500,11 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_dir(KLevel L, const float *e32, c
 
 // phi += a p_new, r -= a q (a = scal[3] / scal[2]) on plasma nodes; r32 = r,
 // e32 = 0, p = p_new; the L1 sum of r to partials column 0.
+template <class T>
 __global__ __launch_bounds__(BLOCK) void k_pcg_update(double *phi, const int32_t *bc,
                                                       const double *pn, const double *q,
-                                                      double *p, double *r, float *r32,
-                                                      float *e32, const double *scal,
+                                                      double *p, double *r, T *r32,
+                                                      T *e32, const double *scal,
                                                       double *partials, int64_t nodes) {
   __shared__ double red[BLOCK / LEPIC_WAVE];
   const double a = scal[2] != 0.0 ? scal[3] / scal[2] : 0.0;
@@ -1510,11 +1516,11 @@ __global__ __launch_bounds__(BLOCK) void k_pcg_update(double *phi, const int32_t
       phi[t] = phi[t] + a * pv;
       const double rv = r[t] - a * q[t];
       r[t] = rv;
-      r32[t] = (float)rv;
+      r32[t] = (T)rv;
       l1 = l1 + fabs(rv);
     }
     p[t] = pv;
-    e32[t] = 0.0f;
+    e32[t] = T(0);
   }
   const double s = block_sum_d(l1, red);
   if (threadIdx.x == 0) partials[blockIdx.x] = s;
@@ -1543,19 +1549,17 @@ __global__ void k_pcg_clear_flag(int *bad) {
 
 }  // namespace
 
-extern "C" int lepic_poisson_solve_pcg(lepic_ctx *ctx, const lepic_grid *g, double *phi,
-                                       const double *rho, const int32_t *bcnd,
-                                       const lepic_mg_params *mp, const lepic_mgmix_params *xp,
-                                       lepic_mg_stats *stats) {
-  if (!ctx) return LEPIC_ERR_ARG;
-  if (mp && (mp->flag_pbc || mp->flag_nmn))
-    return lepic_fail(ctx, LEPIC_ERR_ARG,
-                      "poisson_pcg: periodic or Neumann walls (the operator is not symmetric)");
-  MixSetup M;
+namespace {
+
+template <class T>
+int pcg_solve(lepic_ctx *ctx, const lepic_grid *g, double *phi, const double *rho,
+              const int32_t *bcnd, const lepic_mg_params *mp, const lepic_mgmix_params *xp,
+              lepic_mg_stats *stats) {
+  MixSetup<T> M;
   // FP64 arrays: r, p, p_new, q
   int st0 = mix_setup(ctx, g, phi, rho, bcnd, mp, xp, 4, "poisson_pcg", M);
   if (st0) return st0;
-  Solver<float> &S = M.S;
+  Solver<T> &S = M.S;
   KLevel &F64 = M.F64;
   const int ng = mp->ng;
   const int64_t nodes = M.nodes;
@@ -1570,19 +1574,19 @@ extern "C" int lepic_poisson_solve_pcg(lepic_ctx *ctx, const lepic_grid *g, doub
       int st = S.vcycle(2, 0.0);
       if (st) return st;
     }
-    hipLaunchKernelGGL(k_pcg_zq, dim3(rb), dim3(BLOCK), 0, s, M.e32, q, M.partials, nodes);
+    hipLaunchKernelGGL(k_pcg_zq<T>, dim3(rb), dim3(BLOCK), 0, s, M.e32, q, M.partials, nodes);
     hipLaunchKernelGGL(k_pcg_sum, dim3(1), dim3(BLOCK), 0, s, M.partials, (int64_t)rb, 1, scal);
-    hipLaunchKernelGGL(k_pcg_dir, dim3(rb), dim3(BLOCK), 0, s, F64, M.e32, p, pn, q, r, scal,
+    hipLaunchKernelGGL(k_pcg_dir<T>, dim3(rb), dim3(BLOCK), 0, s, F64, M.e32, p, pn, q, r, scal,
                        M.partials, nodes);
     hipLaunchKernelGGL(k_pcg_sum, dim3(2), dim3(BLOCK), 0, s, M.partials, (int64_t)rb, 2, scal);
-    hipLaunchKernelGGL(k_pcg_update, dim3(rb), dim3(BLOCK), 0, s, phi, bcnd, pn, q, p, r, M.r32,
+    hipLaunchKernelGGL(k_pcg_update<T>, dim3(rb), dim3(BLOCK), 0, s, phi, bcnd, pn, q, p, r, M.r32,
                        M.e32, scal, M.partials, nodes);
     hipLaunchKernelGGL(k_pcg_sum, dim3(1), dim3(BLOCK), 0, s, M.partials, (int64_t)rb, 0, scal);
     return LEPIC_OK;
   };
   const bool replay = mp->graph != 0;
   if (replay) {
-    const int32_t ints[10] = {ng, g->n[0], g->n[1], g->n[2], 0, 0,
+    const int32_t ints[10] = {ng, g->n[0], g->n[1], g->n[2], 0, (int32_t)sizeof(T),
                               xp->inner_cycles, xp->nu, xp->nu_coarsest, S.lc};
     const void *ptrs[4] = {phi, bcnd, ctx->mgx_mem, ctx->d_ctrl};
     const double dbl[5] = {g->h[0], g->h[1], g->h[2], xp->omega, mp->eps0};
@@ -1619,7 +1623,7 @@ extern "C" int lepic_poisson_solve_pcg(lepic_ctx *ctx, const lepic_grid *g, doub
   // r from phi (and the directions cleared), read back with the label flag
   auto restart = [&]() -> int {
     hipLaunchKernelGGL(k_pcg_clear_flag, dim3(1), dim3(64), 0, s, bad);
-    hipLaunchKernelGGL(k_pcg_resid, dim3(rb), dim3(BLOCK), 0, s, F64, dy, r, p, q, M.r32, M.e32,
+    hipLaunchKernelGGL(k_pcg_resid<T>, dim3(rb), dim3(BLOCK), 0, s, F64, dy, r, p, q, M.r32, M.e32,
                        M.partials, bad, nodes);
     hipLaunchKernelGGL(k_pcg_sum, dim3(1), dim3(BLOCK), 0, s, M.partials, (int64_t)rb, 0, scal);
     LEPIC_LAUNCH_CHECK(ctx, "pcg residual");
@@ -1688,4 +1692,21 @@ extern "C" int lepic_poisson_solve_pcg(lepic_ctx *ctx, const lepic_grid *g, doub
   }
   (void)restarts;
   return status;
+}
+
+}  // namespace
+
+extern "C" int lepic_poisson_solve_pcg(lepic_ctx *ctx, const lepic_grid *g, double *phi,
+                                       const double *rho, const int32_t *bcnd,
+                                       const lepic_mg_params *mp, const lepic_mgmix_params *xp,
+                                       lepic_mg_stats *stats) {
+  if (!ctx) return LEPIC_ERR_ARG;
+  if (!mp || !xp) return lepic_fail(ctx, LEPIC_ERR_ARG, "poisson_pcg: null argument");
+  if (mp->flag_pbc || mp->flag_nmn)
+    return lepic_fail(ctx, LEPIC_ERR_ARG,
+                      "poisson_pcg: periodic or Neumann walls (the operator is not symmetric)");
+  if (xp->precond_fp64 != 0 && xp->precond_fp64 != 1)
+    return lepic_fail(ctx, LEPIC_ERR_ARG, "poisson_pcg: precond_fp64 must be 0 or 1");
+  return xp->precond_fp64 ? pcg_solve<double>(ctx, g, phi, rho, bcnd, mp, xp, stats)
+                          : pcg_solve<float>(ctx, g, phi, rho, bcnd, mp, xp, stats);
 }

@@ -155,12 +155,13 @@ def solve_poisson_pcg(phi: torch.Tensor, bcnd: torch.Tensor, rhs: torch.Tensor, 
                       ncycl: int, eps: float, omega: float, ng: int, flag_pbc: int = 0,
                       flag_nmn: int = 0, eps0: float = EPS0, inner_cycles: int = 1, nu: int = 2,
                       nu_coarsest: int = 64, omega_smooth: float = 1.15,
-                      graph: bool = True) -> dict:
+                      graph: bool = True, precond_fp64: bool = False) -> dict:
     """The Poisson solve by FP64 conjugate gradients preconditioned with the
-    mixed solve's FP32 V-cycles (inner_cycles per iteration), stopped as
+    mixed solve's fixed-sweep V-cycles (inner_cycles per iteration), in FP32
+    or, with precond_fp64, in FP64 (the whole solve FP64), stopped as
     solve_poisson_legacy stops; plasma and electrode labels only, no
     periodic or Neumann walls.  Returns the iterations ("cycles"), equivalent
-    fine sweeps, the residual and the FP32 sweeps per level."""
+    fine sweeps, the residual and the preconditioner's sweeps per level."""
     if flag_pbc or flag_nmn:
         raise ValueError("the conjugate-gradient solve takes no periodic or Neumann walls")
     ctx = context(phi.device)
@@ -172,6 +173,7 @@ def solve_poisson_pcg(phi: torch.Tensor, bcnd: torch.Tensor, rhs: torch.Tensor, 
     xp = _native.MGMixParams()
     xp.inner_cycles, xp.nu, xp.nu_coarsest = int(inner_cycles), int(nu), int(nu_coarsest)
     xp.omega = float(omega_smooth)
+    xp.precond_fp64 = 1 if precond_fp64 else 0
     st = _native.MGStats()
     ctx.check(ctx.lib.lepic_poisson_solve_pcg(ctx.h, C.byref(g), ptr(phi), ptr(rhs), ptr(bcnd),
                                               C.byref(mp), C.byref(xp), C.byref(st)))

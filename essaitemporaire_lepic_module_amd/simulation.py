@@ -348,7 +348,8 @@ class Simulation32(Simulation):
     poisson: "fp64" (the reference's multigrid), "mixed" (FP32 smoothing
     inside FP64 iterative refinement, fields.solve_poisson_mixed) or "pcg"
     (FP64 conjugate gradients preconditioned with the FP32 V-cycles,
-    fields.solve_poisson_pcg; plasma and electrode labels only)."""
+    fields.solve_poisson_pcg; plasma and electrode labels only) or "pcg64"
+    (the same with FP64 V-cycles: FP64 throughout)."""
 
     def __init__(self, case: Case, device=None, philox_key: int = 0x5EED5EED,
                  capacity_factor: float = 1.3, resort_every=None, poisson: str = "fp64",
@@ -364,7 +365,7 @@ class Simulation32(Simulation):
                          parts=parts64, np_thread=None if parts64 is None else torch.zeros(
                              (len(case.tracked),) + node_shape(case.n), dtype=torch.float64,
                              device=device if device is not None else "cuda:0"))
-        if poisson not in ("fp64", "mixed", "pcg"):
+        if poisson not in ("fp64", "mixed", "pcg", "pcg64"):
             raise ValueError(f"unknown Poisson solve {poisson!r}")
         self.poisson = poisson
         self.resort_every32 = list(resort_every) if resort_every is not None else None
@@ -443,9 +444,10 @@ class Simulation32(Simulation):
             if self.poisson == "mixed":
                 self.last_mg = F.solve_poisson_mixed(self.phi, self.bcnd, self.rho, h, n, c.ncycl,
                                                      c.eps, c.omega, c.ng, c.flag_pbc, c.flag_nmn)
-            elif self.poisson == "pcg":
+            elif self.poisson in ("pcg", "pcg64"):
                 self.last_mg = F.solve_poisson_pcg(self.phi, self.bcnd, self.rho, h, n, c.ncycl,
-                                                   c.eps, c.omega, c.ng, c.flag_pbc, c.flag_nmn)
+                                                   c.eps, c.omega, c.ng, c.flag_pbc, c.flag_nmn,
+                                                   precond_fp64=self.poisson == "pcg64")
             else:
                 self.last_mg = F.solve_poisson_legacy(self.phi, self.bcnd, self.rho, h, n, c.ncycl,
                                                       c.eps, c.omega, c.ng, c.flag_pbc, c.flag_nmn)

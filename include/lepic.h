@@ -251,6 +251,8 @@ typedef struct {
   int32_t nu;              /* sweeps per level per leg (1..16)           */
   int32_t nu_coarsest;     /* sweeps on the coarsest level               */
   double omega;            /* over-relaxation of the FP32 smoother       */
+  int32_t precond_fp64;    /* lepic_poisson_solve_pcg only: 1 runs the
+                              preconditioner's V-cycles in FP64 (0: FP32) */
 } lepic_mgmix_params;
 
 int lepic_poisson_solve_mixed(lepic_ctx *ctx, const lepic_grid *g, double *phi, const double *rho,
@@ -258,8 +260,9 @@ int lepic_poisson_solve_mixed(lepic_ctx *ctx, const lepic_grid *g, double *phi, 
                               const lepic_mgmix_params *xp, lepic_mg_stats *stats);
 
 /* Preconditioned conjugate gradients in FP64 with the FP32 V-cycles above
- * as the preconditioner (a flexible CG keeping one previous direction): each
- * iteration applies inner_cycles fixed-sweep FP32 V-cycles to A z = r from
+ * as the preconditioner (a flexible CG keeping one previous direction), or
+ * with the same V-cycles in FP64 (xp->precond_fp64 = 1: FP64 throughout):
+ * each iteration applies inner_cycles fixed-sweep V-cycles to A z = r from
  * z = 0, makes the new direction A-orthogonal to the previous one, and
  * updates phi and r along it.  Stops on the same criterion as the solves
  * above (FP64 L1 residual over plasma nodes / alpha < mp->eps, confirmed on

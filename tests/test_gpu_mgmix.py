@@ -97,14 +97,16 @@ def _pcg_vs_oracle(oracle, gpu, n, h, bc, phi0, rhs, ng, eps, **kw):
     return phi_o, phi_g.cpu().numpy(), info
 
 
-@pytest.mark.parametrize("n,ng,inner", [(32, 5, 1), (64, 6, 1), (64, 6, 2)])
-def test_pcg_matches_fp64_faces_and_blobs(gpu, oracle, n, ng, inner):
-    """cfg3's problem by the preconditioned conjugate gradients: the same
-    tolerance, phi and E agreement and untouched electrodes as the mixed
-    solve."""
+@pytest.mark.parametrize("n,ng,inner,fp64", [(32, 5, 1, False), (64, 6, 1, False),
+                                              (64, 6, 2, False), (32, 5, 1, True),
+                                              (64, 6, 1, True)])
+def test_pcg_matches_fp64_faces_and_blobs(gpu, oracle, n, ng, inner, fp64):
+    """cfg3's problem by the preconditioned conjugate gradients (FP32 or FP64
+    V-cycles as the preconditioner): the same tolerance, phi and E agreement
+    and untouched electrodes as the mixed solve."""
     n3, h, bc, phi0, rhs = cfg.cfg3_poisson(n=n, seed=5, nblobs=16)
     phi_o, phi_g, info = _pcg_vs_oracle(oracle, gpu, n3, h, bc, phi0, rhs, ng, 1e-10,
-                                        inner_cycles=inner)
+                                        inner_cycles=inner, precond_fp64=fp64)
     assert info["res"] < 1e-10 and info["cycles"] >= 1
     scale = np.abs(phi_o).max()
     assert np.abs(phi_g - phi_o).max() <= 1e-8 * scale
@@ -114,8 +116,8 @@ def test_pcg_matches_fp64_faces_and_blobs(gpu, oracle, n, ng, inner):
     assert np.abs(E_g - E_o).max() <= 1e-6 * np.abs(E_o).max()
 
 
-@pytest.mark.parametrize("n,ng", [(32, 5), (64, 6)])
-def test_pcg_annular_channel(gpu, oracle, rng, n, ng):
+@pytest.mark.parametrize("n,ng,fp64", [(32, 5, False), (64, 6, False), (64, 6, True)])
+def test_pcg_annular_channel(gpu, oracle, rng, n, ng, fp64):
     """The combined run's channel (curved inner and outer electrodes, the
     anode at 300 V on z = 0) with a random charge density in the plasma:
     the conjugate-gradient solve against the oracle's FP64 multigrid, in
@@ -125,7 +127,8 @@ def test_pcg_annular_channel(gpu, oracle, rng, n, ng):
     bc, h, n3 = c.bcnd, c.h, c.n
     phi0 = c.phi0()
     rhs = np.where(bc == -1, 1e-4 * rng.standard_normal(bc.shape), 0.0)
-    phi_o, phi_g, info = _pcg_vs_oracle(oracle, gpu, n3, h, bc, phi0, rhs, ng, 1e-10)
+    phi_o, phi_g, info = _pcg_vs_oracle(oracle, gpu, n3, h, bc, phi0, rhs, ng, 1e-10,
+                                        precond_fp64=fp64)
     assert info["res"] < 1e-10
     assert np.abs(phi_g - phi_o).max() <= 1e-8 * np.abs(phi_o).max()
     np.testing.assert_array_equal(phi_g[bc > 0], phi0[bc > 0])
@@ -145,18 +148,21 @@ def test_pcg_warm_start_graph_and_refusals(gpu):
     from essaitemporaire_lepic_module_amd import fields as F
     n3, h, bc, phi0, rhs = cfg.cfg3_poisson(n=32, seed=9, nblobs=8)
     bcg, rg = torch.as_tensor(bc, device=gpu), torch.as_tensor(rhs, device=gpu)
-    outs = []
-    for graph in (True, False):
-        phi = torch.as_tensor(phi0.copy(), device=gpu)
-        for rep in range(3):
-            info = F.solve_poisson_pcg(phi, bcg, rg, h, n3, 200, 1e-10 if rep == 0 else 1e-11,
-                                       1.4, 5, graph=graph)
-            assert info["res"] < (1e-10 if rep == 0 else 1e-11)
-            torch.cuda.synchronize()
-        outs.append(phi.cpu().numpy())
-        info = F.solve_poisson_pcg(phi, bcg, rg, h, n3, 20000, 1e-10, 1.4, 5, graph=graph)
-        assert info["cycles"] == 0 and info["res"] < 1e-10
-    np.testing.assert_array_equal(outs[0], outs[1])
+    for fp64 in (False, True):
+        outs = []
+        for graph in (True, False):
+            phi = torch.as_tensor(phi0.copy(), device=gpu)
+            for rep in range(3):
+                info = F.solve_poisson_pcg(phi, bcg, rg, h, n3, 200,
+                                           1e-10 if rep == 0 else 1e-11, 1.4, 5, graph=graph,
+                                           precond_fp64=fp64)
+                assert info["res"] < (1e-10 if rep == 0 else 1e-11)
+                torch.cuda.synchronize()
+            outs.append(phi.cpu().numpy())
+            info = F.solve_poisson_pcg(phi, bcg, rg, h, n3, 20000, 1e-10, 1.4, 5, graph=graph,
+                                       precond_fp64=fp64)
+            assert info["cycles"] == 0 and info["res"] < 1e-10
+        np.testing.assert_array_equal(outs[0], outs[1])
     with pytest.raises(ValueError):
         F.solve_poisson_pcg(phi, bcg, rg, h, n3, 20000, 1e-10, 1.4, 5, flag_pbc=1)
     bad = bc.copy()
