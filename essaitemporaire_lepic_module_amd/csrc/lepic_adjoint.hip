@@ -52,15 +52,20 @@ __device__ inline void weight_grads(double px, double py, double pz, const doubl
        px * qy * (e[3] - e[7]);
 }
 
+// The mover's stencil lower node (make_stencil): ix = int(x/h) + 1 clamped
+// to [0, n + 1] per axis.
+__device__ inline void stencil_lo(const KGrid &g, double xp, double yp, double zp, int &ix,
+                                  int &iy, int &iz) {
+  ix = imax(0, imin(g.n1 + 1, (int)qdiv(xp, g.q1) + 1));
+  iy = imax(0, imin(g.n2 + 1, (int)qdiv(yp, g.q2) + 1));
+  iz = imax(0, imin(g.n3 + 1, (int)qdiv(zp, g.q3) + 1));
+}
+
 // The mover's stencil (make_stencil) with its fractions kept.
 __device__ inline void stencil_fracs(const KGrid &g, double xp, double yp, double zp,
-                                     int64_t &base, double &px, double &py, double &pz) {
-  int ix = (int)qdiv(xp, g.q1) + 1;
-  int iy = (int)qdiv(yp, g.q2) + 1;
-  int iz = (int)qdiv(zp, g.q3) + 1;
-  ix = imax(0, imin(g.n1 + 1, ix));
-  iy = imax(0, imin(g.n2 + 1, iy));
-  iz = imax(0, imin(g.n3 + 1, iz));
+                                     int64_t &base, double &px, double &py, double &pz,
+                                     int &ix, int &iy, int &iz) {
+  stencil_lo(g, xp, yp, zp, ix, iy, iz);
   px = qdiv((double)ix * g.h1 - xp, g.q1);
   py = qdiv((double)iy * g.h2 - yp, g.q2);
   pz = qdiv((double)iz * g.h3 - zp, g.q3);
@@ -87,7 +92,8 @@ struct KBAdj {
 // scatters dL/dE to the node replicas and accumulates dL/dB.  A lane takes
 // APT consecutive particles and sums the dL/dE of consecutive particles with
 // the same stencil in registers (one cell of cell-sorted particles), so a run
-// costs one set of 24 atomics instead of one per particle.
+// costs one set of 24 atomics instead of one per particle, and those go to
+// the block's LDS box (AdjBox) when the run's stencil lies in it.
 constexpr int APT = 4;
 
 __device__ inline void adj_flush(const KGrid &g, double *dst, int64_t b, const double acc[8][3]) {
@@ -103,6 +109,115 @@ __device__ inline void adj_flush(const KGrid &g, double *dst, int64_t b, const d
   }
 }
 
+// Per-block LDS box of dL/dE (3 doubles per node) over the block's stencils.
+// Global atomics on the replicas, not arithmetic, bound the push adjoint
+// (it runs 6x faster with them removed); cell-sorted particles keep a
+// block's stencils within a few cells, so their runs add into LDS and each
+// box node goes to the replica once.  Runs outside the box (a block spread
+// wider than the box, drifted particles) go to the replica directly.
+constexpr int ABOX = 1536;                            // nodes: 36 KB of LDS
+constexpr int ABOX_X = 24, ABOX_Y = 8, ABOX_Z = 8;    // cut extents, product <= ABOX
+static_assert(ABOX_X * ABOX_Y * ABOX_Z <= ABOX, "adjoint box limits exceed the box");
+struct AdjBox {
+  int mn[3], mx[3], sum[3], cnt;
+  int x0, y0, z0, ex, ey, ez, vol;   // vol 0: no box
+  double v[3 * ABOX];
+};
+
+// lo / hi / sm / cnt: the lane's lower nodes' bounds, sum and count.  The
+// box spans the block's lower nodes and one node above (the stencils), cut
+// to ABOX_* around the mean lower node when that exceeds ABOX nodes.
+__device__ inline void adj_box_begin(const KGrid &g, AdjBox &bx, const int lo[3], const int hi[3],
+                                     const int sm[3], int cnt) {
+  if (threadIdx.x == 0) {
+    bx.mn[0] = bx.mn[1] = bx.mn[2] = 1 << 30;
+    bx.mx[0] = bx.mx[1] = bx.mx[2] = -(1 << 30);
+    bx.sum[0] = bx.sum[1] = bx.sum[2] = 0;
+    bx.cnt = 0;
+  }
+  __syncthreads();
+  const int a0 = wave_min(lo[0]), a1 = wave_min(lo[1]), a2 = wave_min(lo[2]);
+  const int b0 = wave_max(hi[0]), b1 = wave_max(hi[1]), b2 = wave_max(hi[2]);
+  int s0 = sm[0], s1 = sm[1], s2 = sm[2], c = cnt;
+  for (int d = 32; d > 0; d >>= 1) {
+    s0 += __shfl_xor(s0, d); s1 += __shfl_xor(s1, d); s2 += __shfl_xor(s2, d);
+    c += __shfl_xor(c, d);
+  }
+  if (__lane_id() == 0) {
+    atomicMin(&bx.mn[0], a0); atomicMin(&bx.mn[1], a1); atomicMin(&bx.mn[2], a2);
+    atomicMax(&bx.mx[0], b0); atomicMax(&bx.mx[1], b1); atomicMax(&bx.mx[2], b2);
+    atomicAdd(&bx.sum[0], s0); atomicAdd(&bx.sum[1], s1); atomicAdd(&bx.sum[2], s2);
+    atomicAdd(&bx.cnt, c);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int vol = 0;
+    if (bx.cnt > 0) {
+      const int top[3] = {g.n1 + 2, g.n2 + 2, g.n3 + 2};
+      const int lim[3] = {ABOX_X, ABOX_Y, ABOX_Z};
+      int a[3], e[3];
+      for (int d = 0; d < 3; ++d) {
+        a[d] = bx.mn[d];
+        e[d] = min(bx.mx[d] + 1, top[d]) - a[d] + 1;
+      }
+      if ((int64_t)e[0] * e[1] * e[2] > ABOX) {
+        for (int d = 0; d < 3; ++d) {
+          if (e[d] <= lim[d]) continue;
+          int st = bx.sum[d] / bx.cnt - lim[d] / 2;
+          st = min(max(st, a[d]), a[d] + e[d] - lim[d]);
+          a[d] = st;
+          e[d] = lim[d];
+        }
+      }
+      bx.x0 = a[0]; bx.y0 = a[1]; bx.z0 = a[2];
+      bx.ex = e[0]; bx.ey = e[1]; bx.ez = e[2];
+      vol = e[0] * e[1] * e[2];
+    }
+    bx.vol = vol;
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < 3 * bx.vol; t += BLOCK) bx.v[t] = 0.0;
+  __syncthreads();
+}
+
+// A run's 8 nodes x 3 components: into the box when its stencil lies inside
+// it, else to the replica.
+__device__ inline void adj_flush_box(const KGrid &g, AdjBox &bx, double *dst, int64_t b, int ix,
+                                     int iy, int iz, const double acc[8][3]) {
+  if (b < 0) return;
+  const int lx = ix - bx.x0, ly = iy - bx.y0, lz = iz - bx.z0;
+  if (bx.vol > 0 && lx >= 0 && ly >= 0 && lz >= 0 && lx + 1 < bx.ex && ly + 1 < bx.ey &&
+      lz + 1 < bx.ez) {
+    const int l = lx + bx.ex * (ly + bx.ey * lz);
+    const int sy = bx.ex, sz = bx.ex * bx.ey;
+    const int o[8] = {0, 1, sy + 1, sy, sz, sz + 1, sz + sy + 1, sz + sy};
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      double *d = &bx.v[3 * (l + o[c])];
+      atomicAdd(d + 0, acc[c][0]);
+      atomicAdd(d + 1, acc[c][1]);
+      atomicAdd(d + 2, acc[c][2]);
+    }
+  } else {
+    adj_flush(g, dst, b, acc);
+  }
+}
+
+// The box's nonzero sums to this XCD's replica.
+__device__ inline void adj_box_flush(const KGrid &g, AdjBox &bx, double *dst) {
+  __syncthreads();
+  const int exy = bx.ex * bx.ey;
+  for (int t = threadIdx.x; t < bx.vol; t += BLOCK) {
+    const int lz = t / exy, r = t - lz * exy, ly = r / bx.ex, lx = r - ly * bx.ex;
+    double *d = dst + 4 * gidx(g, bx.x0 + lx, bx.y0 + ly, bx.z0 + lz);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double v = bx.v[3 * t + k];
+      if (v != 0.0) l2_atomic_add(d + k, v);
+    }
+  }
+}
+
 // MAP: B is a map on the E grid gathered with the particle's stencil and
 // gated at its lower node (|B| > 1e-5, Src/part_expmover.f90:158), so B,
 // k2 and dk2 are per particle and the B gather adds its own position term;
@@ -112,11 +227,29 @@ __global__ __launch_bounds__(BLOCK) void k_adj_push(KGrid g, lepic_pset st, KAdj
                                                     const double4 *__restrict__ E4, KBAdj bf,
                                                     double k1, double dt, double *rep,
                                                     int64_t nodes, double *lamB) {
+  __shared__ AdjBox bx;
   const int64_t i0 = (blockIdx.x * (int64_t)BLOCK + threadIdx.x) * APT;
   double *dst = rep + (int64_t)xcc_id() * 4 * nodes;
+  {
+    int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-(1 << 30), -(1 << 30), -(1 << 30)};
+    int sm[3] = {0, 0, 0}, cnt = 0;
+#pragma unroll
+    for (int q = 0; q < APT; ++q) {
+      const int64_t i = i0 + q;
+      if (i >= n || st.dead[i] != 0) continue;
+      int jx, jy, jz;
+      stencil_lo(g, st.x[i], st.y[i], st.z[i], jx, jy, jz);
+      lo[0] = min(lo[0], jx); lo[1] = min(lo[1], jy); lo[2] = min(lo[2], jz);
+      hi[0] = max(hi[0], jx); hi[1] = max(hi[1], jy); hi[2] = max(hi[2], jz);
+      sm[0] += jx; sm[1] += jy; sm[2] += jz;
+      ++cnt;
+    }
+    adj_box_begin(g, bx, lo, hi, sm, cnt);
+  }
   double lb0 = 0.0, lb1 = 0.0, lb2 = 0.0;
   double acc[8][3];
   int64_t cur = -1;
+  int cx = 0, cy = 0, cz = 0;   // the run's lower node
 #pragma unroll
   for (int q = 0; q < APT; ++q) {
     const int64_t i = i0 + q;
@@ -127,7 +260,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_push(KGrid g, lepic_pset st, KAdj
     double lam[3] = {a.lvx[i] + dt * lx, a.lvy[i] + dt * ly, a.lvz[i] + dt * lz};
     int64_t b;
     double px, py, pz;
-    stencil_fracs(g, x, y, z, b, px, py, pz);
+    int jx, jy, jz;
+    stencil_fracs(g, x, y, z, b, px, py, pz, jx, jy, jz);
     int64_t nd[8];
     node8(g, b, nd);
     const double qx = 1.0 - px, qy = 1.0 - py, qz = 1.0 - pz;
@@ -207,8 +341,9 @@ __global__ __launch_bounds__(BLOCK) void k_adj_push(KGrid g, lepic_pset st, KAdj
     // gather transpose: dL/dE_c += w_c dL/dE_p, summed over the run;
     // position term of the weights
     if (b != cur) {
-      adj_flush(g, dst, cur, acc);
+      adj_flush_box(g, bx, dst, cur, cx, cy, cz, acc);
       cur = b;
+      cx = jx; cy = jy; cz = jz;
 #pragma unroll
       for (int c = 0; c < 8; ++c) acc[c][0] = acc[c][1] = acc[c][2] = 0.0;
     }
@@ -229,7 +364,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_push(KGrid g, lepic_pset st, KAdj
     a.lx[i] = lx; a.ly[i] = ly; a.lz[i] = lz;
     a.lvx[i] = lv[0]; a.lvy[i] = lv[1]; a.lvz[i] = lv[2];
   }
-  adj_flush(g, dst, cur, acc);
+  adj_flush_box(g, bx, dst, cur, cx, cy, cz, acc);
+  adj_box_flush(g, bx, dst);
   if (!MAP && bf.use_b) {
     __shared__ double red[3][BLOCK / LEPIC_WAVE];
     lb0 = wave_sum_d(lb0); lb1 = wave_sum_d(lb1); lb2 = wave_sum_d(lb2);

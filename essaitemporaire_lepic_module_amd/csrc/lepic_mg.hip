@@ -172,10 +172,11 @@ __device__ inline void sor_node(const KLevelT<T> &L, int colour, T omega, int64_
                 L);
 }
 
+// ctl == nullptr: a fixed sweep count (the level is always active).
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_sor(KLevelT<T> L, int colour, T omega,
                                                const MGCtl *ctl) {
-  if (ctl->active == 0) return;
+  if (ctl && ctl->active == 0) return;
   sor_node(L, colour, omega, blockIdx.x * (int64_t)BLOCK + threadIdx.x);
 }
 
@@ -208,13 +209,13 @@ __device__ inline void per_z_node(const KLevelT<T> &L, int64_t t) {
 
 template <class T>
 __global__ void k_per_y(KLevelT<T> L, const MGCtl *ctl) {
-  if (ctl->active == 0) return;
+  if (ctl && ctl->active == 0) return;
   per_y_node(L, blockIdx.x * (int64_t)blockDim.x + threadIdx.x);
 }
 
 template <class T>
 __global__ void k_per_z(KLevelT<T> L, const MGCtl *ctl) {
-  if (ctl->active == 0) return;
+  if (ctl && ctl->active == 0) return;
   per_z_node(L, blockIdx.x * (int64_t)blockDim.x + threadIdx.x);
 }
 
@@ -618,7 +619,7 @@ __device__ void wg_sor_rb(const CoarseArgs<T> &A, const CoarseLDS<T> &S, int l, 
     res_s = res;
     if (stop || ksor >= nr) break;
   }
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0 && !A.fixed) {
     MGCtl *c = &A.ctl[l];
     c->active = 0;
     c->ksor = ksor;
@@ -757,13 +758,14 @@ struct Solver {
     unsigned bs = grid_blocks(nsor, BLOCK);
     const unsigned br = std::min<unsigned>(grid_blocks(nres, BLOCK), RESID_BLOCKS);
     hipStream_t s = ctx->stream;
+    const MGCtl *c = fixed ? nullptr : &ctl[l];   // fixed counts: no exit, no control block
     for (int colour = 0; colour < 2; ++colour) {
-      hipLaunchKernelGGL(k_sor<T>, dim3(bs), dim3(BLOCK), 0, s, Lv, colour, omega, &ctl[l]);
+      hipLaunchKernelGGL(k_sor<T>, dim3(bs), dim3(BLOCK), 0, s, Lv, colour, omega, c);
       if (flag_pbc) {
         hipLaunchKernelGGL(k_per_y<T>, dim3(grid_blocks((int64_t)(g.n1 + 3) * (g.n3 + 3), BLOCK)),
-                           dim3(BLOCK), 0, s, Lv, &ctl[l]);
+                           dim3(BLOCK), 0, s, Lv, c);
         hipLaunchKernelGGL(k_per_z<T>, dim3(grid_blocks((int64_t)(g.n1 + 3) * (g.n2 + 3), BLOCK)),
-                           dim3(BLOCK), 0, s, Lv, &ctl[l]);
+                           dim3(BLOCK), 0, s, Lv, c);
       }
     }
     if (fixed) return;   // a fixed sweep count: no residual, the level stays active
@@ -774,11 +776,11 @@ struct Solver {
 
   // sor_rb: returns LEPIC_OK; sweeps are counted on the device.
   int sor_rb(int l, double eps, int kmg, int dig) {
-    hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, ctx->stream, &ctl[l]);
     if (fixed) {
       for (int c = 0; c < (dig == 0 ? nr_coarsest : nr_fixed); ++c) sor_sweep(l, eps, dig, 4);
       return LEPIC_OK;
     }
+    hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, ctx->stream, &ctl[l]);
     if (kmg == 1 && dig == 0) {
       // first V-cycle's coarsest level: up to 1e6 sweeps to res <= eps.
       const int64_t nr = 1000000;

@@ -154,12 +154,13 @@ def solve_poisson_mixed(phi: torch.Tensor, bcnd: torch.Tensor, rhs: torch.Tensor
 def solve_poisson_pcg(phi: torch.Tensor, bcnd: torch.Tensor, rhs: torch.Tensor, h, n,
                       ncycl: int, eps: float, omega: float, ng: int, flag_pbc: int = 0,
                       flag_nmn: int = 0, eps0: float = EPS0, inner_cycles: int = 1, nu: int = 2,
-                      nu_coarsest: int = 64, omega_smooth: float = 1.15,
+                      nu_coarsest: int = 16, omega_smooth: float = 1.15,
                       graph: bool = True, precond_fp64: bool = False) -> dict:
     """The Poisson solve by FP64 conjugate gradients preconditioned with the
     mixed solve's fixed-sweep V-cycles (inner_cycles per iteration), in FP32
-    or, with precond_fp64, in FP64 (the whole solve FP64), stopped as
-    solve_poisson_legacy stops; plasma and electrode labels only, no
+    or, with precond_fp64, in FP64 (the whole solve FP64), 16 sweeps on the
+    coarsest level (the CG corrects what a shorter coarse solve leaves),
+    stopped as solve_poisson_legacy stops; plasma and electrode labels only, no
     periodic or Neumann walls.  Returns the iterations ("cycles"), equivalent
     fine sweeps, the residual and the preconditioner's sweeps per level."""
     if flag_pbc or flag_nmn:
