@@ -75,6 +75,15 @@ class AdjointSet:
         for v in self.a.values():
             v.zero_()
 
+    def unpermute_(self, perm: torch.Tensor):
+        """Slot perm[j] takes slot j's values (j < len(perm)); every other
+        slot is zero."""
+        m = perm.numel()
+        for k, v in self.a.items():
+            out = torch.zeros_like(v)
+            out.index_copy_(0, perm, v[:m])
+            self.a[k] = out
+
     def struct(self) -> _native.AdjSet:
         s = _native.AdjSet()
         for k, v in self.a.items():
@@ -91,6 +100,11 @@ class DiffPIC:
     none, no dielectrics or heating.  poisson: "fp64" (the reference's
     multigrid) or "pcg" (conjugate gradients preconditioned with FP32
     V-cycles, to the same tolerance) for the forward and adjoint solves.
+    resort: every `resort` steps (0: never) the live particles are sorted by
+    cell at the start of the step and the holes dropped, so that gathers,
+    deposits and the adjoint's scatter stay local; the slot orders are
+    recorded with the run's events, replayed by frozen runs and undone on
+    the adjoint in the backward.
     parts: the initial FP64 particle sets
     (host arrays dict per species: x y z vx vy vz), loaded into device sets
     of that many slots plus, with collisions, room for the products
@@ -98,10 +112,12 @@ class DiffPIC:
 
     def __init__(self, case: Case, parts, device=None, bias_label: int = 2, flux_species: int = 1,
                  checkpoint: int = 20, adjoint_eps: float = 1e-10, philox_key: int = 0x5EED5EED,
-                 product_room: float = 1.0 / 16, poisson: str = "fp64"):
+                 product_room: float = 1.0 / 16, poisson: str = "fp64", resort: int = 20):
         if poisson not in ("fp64", "pcg"):
             raise ValueError(f"unknown Poisson solve {poisson!r}")
         self.poisson = poisson
+        self.R = max(0, int(resort))
+        self._track = True
         b = np.asarray(case.bcnd)
         if np.any((b == 0) | (b == -2)) or case.flag_pbc or case.flag_nmn:
             raise ValueError("the adjoint E field takes plasma (-1) and electrode labels only")
@@ -196,9 +212,14 @@ class DiffPIC:
         """Working particle arrays from the initial sets ([slots in use,
         arrays] per species) and the initial deposit."""
         st = []
-        for m, d, _ in self.init:
+        self.ident, self.next_id = [], []
+        for m, d, cap in self.init:
             w = {k: v.clone() for k, v in d.items()}
             st.append([m, w])
+            # the particle each slot holds: its input index, products
+            # numbered from m on in order of creation
+            self.ident.append(torch.arange(cap, dtype=torch.int64, device=self.dev))
+            self.next_id.append(m)
         self.np_thread.zero_()
         for s, (m, w) in enumerate(st):
             p = _pset_of(w, m)
@@ -277,6 +298,43 @@ class DiffPIC:
             float(self.flux_coef),
             ptr(acc), C.byref(adj.struct()) if adj is not None else None, float(scale)))
 
+    def _sorts_at(self, k: int) -> bool:
+        return self.R > 0 and k > 1 and (k - 1) % self.R == 0
+
+    def _sort_perms(self, st):
+        """Per species, the live slots in cell order (stable): the new
+        order's old slot numbers."""
+        n1, n2, n3 = self.n
+        perms = []
+        for m, w in st:
+            live = (w["dead"][:m] == 0).nonzero().squeeze(1)
+            cx = torch.clamp((w["x"][live] / self.h[0]).floor().long(), 0, n1 - 1)
+            cy = torch.clamp((w["y"][live] / self.h[1]).floor().long(), 0, n2 - 1)
+            cz = torch.clamp((w["z"][live] / self.h[2]).floor().long(), 0, n3 - 1)
+            order = torch.argsort(cx + n1 * (cy + n2 * cz), stable=True)
+            perms.append(live[order])
+        return perms
+
+    def _apply_sort(self, st, perms):
+        """Slots j < len(perm) take the particle of slot perm[j]; the rest
+        are unused."""
+        for s, perm in enumerate(perms):
+            m, w = st[s]
+            m1 = perm.numel()
+            for f in ("x", "y", "z", "vx", "vy", "vz"):
+                w[f][:m1] = w[f].index_select(0, perm)
+            w["dead"][:m1] = 0
+            w["dead"][m1:m] = 1
+            if self._track:
+                self.ident[s][:m1] = self.ident[s].index_select(0, perm)
+            st[s][0] = m1
+
+    def _new_products(self, s: int, m0: int, m1: int):
+        if self._track and m1 > m0:
+            self.ident[s][m0:m1] = torch.arange(self.next_id[s], self.next_id[s] + m1 - m0,
+                                                dtype=torch.int64, device=self.dev)
+            self.next_id[s] += m1 - m0
+
     @staticmethod
     def _snap(st):
         return [[m, {k: v.clone() for k, v in w.items()}] for m, w in st]
@@ -307,6 +365,7 @@ class DiffPIC:
                     p = _pset_of(ev["prod"], b - a)
                     ctx.check(ctx.lib.lepic_deposit(ctx.h, C.byref(g), C.byref(p), ptr(self.kq),
                                                     float(self.Nm[s]), ptr(self.np_thread[s])))
+                self._new_products(s, a, b)
                 st[s][0] = b
             return events
         c = self.case
@@ -334,6 +393,7 @@ class DiffPIC:
             for f in ("vx", "vy", "vz"):
                 ev[f] = w[f].index_select(0, idx)
             out.append(ev)
+            self._new_products(s, m, m1)
             st[s][0] = m1
         return out
 
@@ -349,32 +409,48 @@ class DiffPIC:
             self.ctx.check(self.ctx.lib.lepic_move_boris(self.ctx.h, C.byref(self.g), C.byref(p),
                                                          ptr(self.E4), C.byref(bf), self.charge[s],
                                                          self.mass[s], c.dt))
+            per_slot = deaths[s].index_select(0, self.ident[s][:m]).contiguous()
             self.ctx.check(self.ctx.lib.lepic_apply_deaths(self.ctx.h, C.byref(p),
-                                                           ptr(deaths[s]), int(k)))
+                                                           ptr(per_slot), int(k)))
             self.ctx.check(self.ctx.lib.lepic_deposit(self.ctx.h, C.byref(self.g), C.byref(p),
                                                       ptr(self.kq), float(self.Nm[s]),
                                                       ptr(self.np_thread[s])))
 
     def forward(self, V: float, B0, steps: int, keep: bool = True,
-                record_deaths: bool = False, frozen=None, frozen_mcc=None) -> float:
+                record_deaths: bool = False, frozen=None, frozen_mcc=None,
+                frozen_sorts=None) -> float:
         """The loss of `steps` steps at bias V and field B0 (None: no B);
         keep: store the checkpoints the backward needs; record_deaths: keep
-        in self.deaths, per species, the step at which each slot was lost
-        (0: never); frozen: such a record, whose wall events replace the
-        wall test; frozen_mcc: a run's self.mcc_events, whose collisions
-        replace MCC (with both, the run the adjoint differentiates, for
-        checks by finite differences).  Without frozen_mcc the collisions
-        are drawn and recorded in self.mcc_events."""
+        in self.deaths, per species, the step at which each particle was
+        lost (0: never), indexed by particle (input index, then products in
+        order of creation); frozen: such a record, whose wall events replace
+        the wall test; frozen_mcc: a run's self.mcc_events, whose collisions
+        replace MCC, and frozen_sorts its self.sorts (default: self.sorts),
+        whose slot orders replace the sorts (all three: the run the adjoint
+        differentiates, for checks by finite differences).  Without
+        frozen_mcc the collisions are drawn and recorded in self.mcc_events
+        and the sorts in self.sorts."""
+        self._track = True
         st = self._fresh()
         self.phi.zero_()
         self.acc.zero_()
         self.ckpt = {}
         if frozen_mcc is None:
             self.mcc_events = {}
+            self.sorts = {}
+            sorts = None
+        else:
+            sorts = frozen_sorts if frozen_sorts is not None else self.sorts
         if record_deaths:
-            self.deaths = [torch.zeros(max(w["x"].numel(), 1), dtype=torch.int64,
+            self.deaths = [torch.zeros(max(2 * w["x"].numel(), 1), dtype=torch.int64,
                                        device=self.dev) for _, w in st]
         for k in range(1, steps + 1):
+            if self._sorts_at(k):
+                if sorts is None:
+                    self.sorts[k] = self._sort_perms(st)
+                    self._apply_sort(st, self.sorts[k])
+                else:
+                    self._apply_sort(st, sorts[k])
             if keep and (k - 1) % self.K == 0:
                 self.ckpt[k] = (self._snap(st), self.np_thread.clone(), self.phi.clone())
             self._fields(V)
@@ -390,8 +466,13 @@ class DiffPIC:
             self._flux(st, self.acc)
             if record_deaths:
                 for s, (m, w) in enumerate(st):
-                    d = self.deaths[s][:m]
-                    d.copy_(torch.where((w["dead"][:m] != 0) & (d == 0), k, d))
+                    if self.next_id[s] > self.deaths[s].numel():
+                        self.deaths[s] = torch.cat([self.deaths[s],
+                                                    torch.zeros_like(self.deaths[s])])
+                    ids = self.ident[s][:m]
+                    d = self.deaths[s].index_select(0, ids)
+                    lost = (w["dead"][:m] != 0) & (d == 0)
+                    self.deaths[s].index_copy_(0, ids, torch.where(lost, k, d))
         self.steps = steps
         self.theta = (float(V), None if B0 is None else float(B0))
         return float(self.acc.item()) / steps
@@ -401,6 +482,7 @@ class DiffPIC:
         """(dL/dV, dL/dB0) of the last forward, times grad_out."""
         if not getattr(self, "ckpt", None):
             raise RuntimeError("backward needs a forward with keep=True")
+        self._track = False   # the recomputed segments leave the forward's records alone
         V, B0 = self.theta
         steps = self.steps
         ctx, g = self.ctx, self.g
@@ -424,7 +506,11 @@ class DiffPIC:
             self.np_thread.copy_(np_c)
             self.phi.copy_(phi_c)
             states, Es = [], []
+            presort = {}    # the state before a sort inside the segment: the previous step's end
             for k in range(s0, s1 + 1):
+                if k != s0 and k in self.sorts:
+                    presort[k] = self._snap(st)
+                    self._apply_sort(st, self.sorts[k])
                 states.append(self._snap(st))
                 self._fields(V)
                 Es.append(self.E4.clone())
@@ -433,8 +519,8 @@ class DiffPIC:
                     self._collide(st, k, self.mcc_events[k])
             states.append(self._snap(st))          # after the segment's last step
             for j in range(s1 - s0, -1, -1):
-                after, before = states[j + 1], states[j]
                 k = s0 + j
+                after, before = presort.get(k + 1, states[j + 1]), states[j]
                 # loss term of this step (state after the push and MCC)
                 self._flux(after, None, adj[self.flux_species], scale)
                 if self._collides(k):
@@ -484,7 +570,12 @@ class DiffPIC:
                     ctx.check(ctx.lib.lepic_adj_deposit(ctx.h, C.byref(g), C.byref(p), C.byref(a),
                                                         ptr(self.kq), ptr(mu), self.charge[s],
                                                         self.Nm[s]))
-            del states, Es
+                if k in self.sorts:
+                    # back to the slot order before the sort at the start of
+                    # step k; the dropped holes carry no derivative
+                    for s, perm in enumerate(self.sorts[k]):
+                        adj[s].unpermute_(perm)
+            del states, Es, presort
         self.adj = adj
         return float(lamV.item()), (float(lamB[2].item())
                                     if B0 is not None and self.B4 is None else 0.0)

@@ -356,15 +356,17 @@ def _initial(rng, case):
     return parts
 
 
-@pytest.mark.parametrize("steps,checkpoint", [(12, 5)])
-def test_gradient_matches_pure_torch_16cube(gpu, steps, checkpoint):
+@pytest.mark.parametrize("steps,checkpoint,resort", [(12, 5, 0), (12, 5, 5), (12, 4, 5)])
+def test_gradient_matches_pure_torch_16cube(gpu, steps, checkpoint, resort):
     """The whole backward at 16^3 x 4 ppc against autograd of the pure-torch
-    run from the same particles, with the wall events the GPU forward saw."""
+    run from the same particles, with the wall events the GPU forward saw;
+    with the particles re-sorted by cell at steps 6 and 11 (at checkpoints,
+    and inside the segments of 4 steps)."""
     from essaitemporaire_lepic_module_amd.adjoint import DiffPIC
     rng = np.random.default_rng(4)
     case = cfg.cfg4_differentiable(n=16, np_cell=4, steps=steps, mcc_every=0)
     parts = _initial(rng, case)
-    sim = DiffPIC(case, parts, "cuda:0", checkpoint=checkpoint)
+    sim = DiffPIC(case, parts, "cuda:0", checkpoint=checkpoint, resort=resort)
     L = sim.forward(20.0, 0.005, steps, keep=True, record_deaths=True)
     dV, dB = sim.backward()
     sched = [torch.as_tensor(sim.by_input(s, d)) for s, d in enumerate(sim.deaths)]
@@ -381,16 +383,19 @@ def test_gradient_matches_pure_torch_16cube(gpu, steps, checkpoint):
     assert abs(dB - gB.item()) <= 1e-6 * abs(gB.item()), (dB, gB.item())
 
 
-def test_checkpoint_interval_leaves_gradient_unchanged(gpu):
+@pytest.mark.parametrize("resort", [0, 5])
+def test_checkpoint_interval_leaves_gradient_unchanged(gpu, resort):
     """Checkpoints every 1, 4 and 12 steps give one gradient; MCC at step 10
-    (recomputed segments replay the recorded collisions)."""
+    (recomputed segments replay the recorded collisions); with resort 5 the
+    particles are re-sorted at steps 6 and 11, at checkpoints or inside the
+    segments (the backward replays the recorded slot orders)."""
     from essaitemporaire_lepic_module_amd.adjoint import DiffPIC
     rng = np.random.default_rng(5)
     case = cfg.cfg4_differentiable(n=16, np_cell=4, steps=12)
     parts = _initial(rng, case)
     out = []
     for K in (1, 4, 12):
-        sim = DiffPIC(case, parts, "cuda:0", checkpoint=K)
+        sim = DiffPIC(case, parts, "cuda:0", checkpoint=K, resort=resort)
         sim.forward(20.0, 0.005, 12)
         assert sum(int(ev["idx"].numel()) for ev in sim.mcc_events[10]) > 0
         out.append(sim.backward())
@@ -400,8 +405,8 @@ def test_checkpoint_interval_leaves_gradient_unchanged(gpu):
 
 
 def test_gradient_matches_central_differences(gpu):
-    """32^3 x 8 ppc, 20 steps, MCC at steps 10 and 20: dL/dV and dL/dB0
-    against central differences (steps 1e-3 and 1e-4 relative) of the run with
+    """32^3 x 8 ppc, 20 steps, MCC at steps 10 and 20, particles re-sorted by
+    cell at steps 9 and 17: dL/dV and dL/dB0 against central differences (steps 1e-3 and 1e-4 relative) of the run with
     the nominal wall events and collisions replayed (the map the adjoint
     differentiates).  Central differences of the free
     run, whose wall events move with the parameters, are printed beside
@@ -412,7 +417,7 @@ def test_gradient_matches_central_differences(gpu):
     steps = 20
     case = cfg.cfg4_differentiable(n=32, np_cell=8, steps=steps)
     parts = _initial(rng, case)
-    sim = DiffPIC(case, parts, "cuda:0", checkpoint=10)
+    sim = DiffPIC(case, parts, "cuda:0", checkpoint=10, resort=8)   # re-sorts at steps 9, 17
     V = torch.tensor(20.0, dtype=torch.float64, requires_grad=True)
     B0 = torch.tensor(0.005, dtype=torch.float64, requires_grad=True)
     L = PicLoss.apply(V, B0, sim, steps)
@@ -420,7 +425,7 @@ def test_gradient_matches_central_differences(gpu):
     dV, dB = V.grad.item(), B0.grad.item()
     sim.forward(20.0, 0.005, steps, keep=False, record_deaths=True)
     deaths = [d.clone() for d in sim.deaths]
-    events = sim.mcc_events
+    events, sorts = sim.mcc_events, sim.sorts
     ncoll = sum(int(ev["idx"].numel()) for evs in events.values() for ev in evs)
     nprod = sum(ev["m1"] - ev["m0"] for evs in events.values() for ev in evs)
     print(f"collided {ncoll} products {nprod}")
@@ -433,7 +438,8 @@ def test_gradient_matches_central_differences(gpu):
 
     def fd(frozen):
         f = lambda a, b: sim.forward(a, b, steps, keep=False, frozen=frozen,
-                                     frozen_mcc=events if frozen is not None else None)
+                                     frozen_mcc=events if frozen is not None else None,
+                                     frozen_sorts=sorts if frozen is not None else None)
         return ((f(20.0 + hV, 0.005) - f(20.0 - hV, 0.005)) / (2 * hV),
                 (f(20.0, 0.005 + hB) - f(20.0, 0.005 - hB)) / (2 * hB))
 
@@ -511,19 +517,20 @@ def test_annular_map_gradient_matches_central_differences(gpu):
     steps = 20
     case = cfg.cfg5_combined(n=32, np_cell=8, steps=steps)
     parts = _initial_channel(rng, case)
-    sim = DiffPIC(case, parts, "cuda:0", checkpoint=10)
+    sim = DiffPIC(case, parts, "cuda:0", checkpoint=10, resort=8)   # re-sorts at steps 9, 17
     V = torch.tensor(300.0, dtype=torch.float64, requires_grad=True)
     L = PicLoss.apply(V, None, sim, steps)
     L.backward()
     dV = V.grad.item()
     sim.forward(300.0, None, steps, keep=False, record_deaths=True)
     deaths = [d.clone() for d in sim.deaths]
-    events = sim.mcc_events
+    events, sorts = sim.mcc_events, sim.sorts
     ncoll = sum(int(ev["idx"].numel()) for evs in events.values() for ev in evs)
     assert ncoll > 0
     hV = 300.0 * 1e-4
     f = lambda a, fr: sim.forward(a, None, steps, keep=False, frozen=fr,
-                                  frozen_mcc=events if fr is not None else None)
+                                  frozen_mcc=events if fr is not None else None,
+                                  frozen_sorts=sorts if fr is not None else None)
     fV = (f(300.0 + hV, deaths) - f(300.0 - hV, deaths)) / (2 * hV)
     gV = (f(300.0 + hV, None) - f(300.0 - hV, None)) / (2 * hV)
     print(f"collided {ncoll}; dL/dV adjoint {dV:.10e} frozen fd {fV:.10e} free fd {gV:.10e}")
