@@ -98,8 +98,9 @@ class DiffPIC:
     case: plasma (-1) and electrode (>= 1) labels only, B uniform along z, a
     B map on the field grid (B_map; dL/dB0 is then 0) or none, collisions (PHILOX draws, keyed by philox_key and the step) or
     none, no dielectrics or heating.  poisson: "fp64" (the reference's
-    multigrid) or "pcg" (conjugate gradients preconditioned with FP32
-    V-cycles, to the same tolerance) for the forward and adjoint solves.
+    multigrid), "pcg" (conjugate gradients preconditioned with FP32
+    V-cycles, to the same tolerance) or "pcg64" (the same with FP64
+    V-cycles) for the forward and adjoint solves.
     resort: every `resort` steps (0: never) the live particles are sorted by
     cell at the start of the step and the holes dropped, so that gathers,
     deposits and the adjoint's scatter stay local; the slot orders are
@@ -113,7 +114,7 @@ class DiffPIC:
     def __init__(self, case: Case, parts, device=None, bias_label: int = 2, flux_species: int = 1,
                  checkpoint: int = 20, adjoint_eps: float = 1e-10, philox_key: int = 0x5EED5EED,
                  product_room: float = 1.0 / 16, poisson: str = "fp64", resort: int = 20):
-        if poisson not in ("fp64", "pcg"):
+        if poisson not in ("fp64", "pcg", "pcg64"):
             raise ValueError(f"unknown Poisson solve {poisson!r}")
         self.poisson = poisson
         self.R = max(0, int(resort))
@@ -261,8 +262,9 @@ class DiffPIC:
         multigrid, or the conjugate gradients preconditioned with FP32
         V-cycles (poisson="pcg")."""
         c = self.case
-        if self.poisson == "pcg":
-            F.solve_poisson_pcg(u, self.bcnd, rhs, self.h, self.n, c.ncycl, eps, c.omega, c.ng)
+        if self.poisson in ("pcg", "pcg64"):
+            F.solve_poisson_pcg(u, self.bcnd, rhs, self.h, self.n, c.ncycl, eps, c.omega, c.ng,
+                                precond_fp64=self.poisson == "pcg64")
         else:
             F.solve_poisson_legacy(u, self.bcnd, rhs, self.h, self.n, c.ncycl, eps, c.omega,
                                    c.ng, 0, 0)
