@@ -356,14 +356,15 @@ def _adjoint_cfg4(steps, n=64, ppc=8, checkpoint=20, poisson="fp64"):
            "loss": loss, "dL_dV": dV, "dL_dB0": dB, "collided": ncoll,
            "workload": (f"{n}^3 cells, {ppc} ppc x 2 species, wall biased at 20 V, Bz 0.005 T, "
                         f"argon MCC every 10 steps (events frozen in the backward), {steps} "
-                        f"steps, checkpoint every {checkpoint}, re-sort every 20, FP64, Poisson "
-                        f"solves by {SOLVES[poisson]}")}
+                        f"steps, checkpoint every {checkpoint}, re-sort every 20, FP64, forward "
+                        f"Poisson solves by {SOLVES[poisson]}, adjoint solves by "
+                        f"{SOLVES['pcg']}")}
     del sim
     torch.cuda.empty_cache()
     return out
 
 
-def _adjoint_cfg5(n, ppc=32, steps=500, checkpoint=20, seed=0, poisson="pcg"):
+def _adjoint_cfg5(n, ppc=32, steps=500, checkpoint=20, seed=0, poisson="pcg", resort=20):
     """The combined run's forward + adjoint of the ion current into the anode
     with respect to the anode bias, on an n^3 grid of the same geometry (the
     annular channel, anode at 300 V, radial Br(z) map, argon MCC every 10
@@ -396,7 +397,7 @@ def _adjoint_cfg5(n, ppc=32, steps=500, checkpoint=20, seed=0, poisson="pcg"):
         parts.append({"x": pos[:, 0], "y": pos[:, 1], "z": pos[:, 2],
                       "vx": vel[:, 0], "vy": vel[:, 1], "vz": vel[:, 2]})
     del pos, ci
-    sim = DiffPIC(case, parts, "cuda:0", checkpoint=checkpoint, poisson=poisson)
+    sim = DiffPIC(case, parts, "cuda:0", checkpoint=checkpoint, poisson=poisson, resort=resort)
     del parts
     sim.forward(300.0, None, 20, keep=True)
     sim.backward()
@@ -424,8 +425,9 @@ def _adjoint_cfg5(n, ppc=32, steps=500, checkpoint=20, seed=0, poisson="pcg"):
                         f"300 V, radial Br(z) map), {ppc} ppc x 2 species in the loadable "
                         f"cells, argon MCC every 10 steps (frozen in the backward), {steps} "
                         f"steps, checkpoint every {checkpoint}, re-sort every 20, FP64, forward "
-                        f"and adjoint Poisson solves by {SOLVES[poisson]} to 1e-10; loss: the "
-                        "mean ion current into the anode through z = h3")}
+                        f"Poisson solves by {SOLVES[poisson]}, adjoint solves by "
+                        f"{SOLVES['pcg']}, to 1e-10; loss: the mean ion current into the anode "
+                        "through z = h3")}
     del sim
     torch.cuda.empty_cache()
     return out

@@ -100,7 +100,11 @@ class DiffPIC:
     none, no dielectrics or heating.  poisson: "fp64" (the reference's
     multigrid), "pcg" (conjugate gradients preconditioned with FP32
     V-cycles, to the same tolerance) or "pcg64" (the same with FP64
-    V-cycles) for the forward and adjoint solves.
+    V-cycles) for the forward's solves (the backward's recomputed
+    segments use the same); adjoint_poisson: the same choice for the
+    adjoint solves A mu = dL/dphi, which have no counterpart in the
+    reference (default: FP64 conjugate gradients preconditioned with FP32
+    V-cycles).
     resort: every `resort` steps (0: never) the live particles are sorted by
     cell at the start of the step and the holes dropped, so that gathers,
     deposits and the adjoint's scatter stay local; the slot orders are
@@ -113,10 +117,13 @@ class DiffPIC:
 
     def __init__(self, case: Case, parts, device=None, bias_label: int = 2, flux_species: int = 1,
                  checkpoint: int = 20, adjoint_eps: float = 1e-10, philox_key: int = 0x5EED5EED,
-                 product_room: float = 1.0 / 16, poisson: str = "fp64", resort: int = 20):
-        if poisson not in ("fp64", "pcg", "pcg64"):
-            raise ValueError(f"unknown Poisson solve {poisson!r}")
+                 product_room: float = 1.0 / 16, poisson: str = "fp64", resort: int = 20,
+                 adjoint_poisson: str = "pcg"):
+        for sv in (poisson, adjoint_poisson):
+            if sv not in ("fp64", "pcg", "pcg64"):
+                raise ValueError(f"unknown Poisson solve {sv!r}")
         self.poisson = poisson
+        self.adjoint_poisson = adjoint_poisson
         self.R = max(0, int(resort))
         self._track = True
         b = np.asarray(case.bcnd)
@@ -254,17 +261,17 @@ class DiffPIC:
         F.reduce_species_density(self.n, self.bcnd, self.np_thread, self.ntype, self.np_red)
         F.build_rho_from_np(self.n, self.np_red, self.charge, self.ntype, self.rho)
         self._set_electrodes(self.phi, V)
-        self._solve(self.phi, self.rho, c.eps)
+        self._solve(self.phi, self.rho, c.eps, self.poisson)
         F.calc_Efield_modular(self.n, self.h, self.phi, self.E4, self.bcnd)
 
-    def _solve(self, u, rhs, eps):
+    def _solve(self, u, rhs, eps, how):
         """A u = rhs on plasma nodes to eps (warm start from u): the FP64
-        multigrid, or the conjugate gradients preconditioned with FP32
-        V-cycles (poisson="pcg")."""
+        multigrid ("fp64"), or the conjugate gradients preconditioned with
+        FP32 ("pcg") or FP64 ("pcg64") V-cycles."""
         c = self.case
-        if self.poisson in ("pcg", "pcg64"):
+        if how in ("pcg", "pcg64"):
             F.solve_poisson_pcg(u, self.bcnd, rhs, self.h, self.n, c.ncycl, eps, c.omega, c.ng,
-                                precond_fp64=self.poisson == "pcg64")
+                                precond_fp64=how == "pcg64")
         else:
             F.solve_poisson_legacy(u, self.bcnd, rhs, self.h, self.n, c.ncycl, eps, c.omega,
                                    c.ng, 0, 0)
@@ -560,7 +567,7 @@ class DiffPIC:
                     ctx.check(ctx.lib.lepic_adj_poisson_rhs(ctx.h, C.byref(g), ptr(self.bcnd),
                                                             ptr(lamphi), sc, ptr(rhs)))
                     mu.mul_(sc)          # the previous step's solution as the first guess
-                    self._solve(mu, rhs, self.adj_eps)
+                    self._solve(mu, rhs, self.adj_eps, self.adjoint_poisson)
                     mu.mul_(1.0 / sc)
                 else:
                     mu.zero_()
