@@ -433,12 +433,13 @@ def _adjoint_cfg5(n, ppc=32, steps=500, checkpoint=20, seed=0, poisson="pcg", re
     return out
 
 
-def _combined_cfg5(n, ppc, steps=10, warmup=3):
+def _combined_cfg5(n, ppc, steps=10, warmup=3, resort=4):
     """The combined large run (cfg5 forward): n^3 cells, the annular channel
     with the anode at 300 V, the radial Br(z) map, argon MCC every 10 steps,
     in the precision form with the Poisson solve by conjugate gradients
     preconditioned with FP32 V-cycles; `steps`
-    steps timed by wall clock after `warmup` (electrons re-sorted every 4),
+    steps timed by wall clock after `warmup` (electrons re-sorted every
+    `resort` steps, the --resort-electrons of the cfg2 step),
     then as many with every phase between HIP events."""
     import torch
     from essaitemporaire_lepic_module_amd import config as cfg
@@ -448,7 +449,8 @@ def _combined_cfg5(n, ppc, steps=10, warmup=3):
     t0 = time.perf_counter()
     case = cfg.cfg5_combined(n=n, np_cell=ppc)
     cells = cfg.loadable_cells(case.bcnd, case.n)
-    sim = Simulation32(case, "cuda:0", capacity_factor=1.25, resort_every=[4, 0], poisson="pcg")
+    sim = Simulation32(case, "cuda:0", capacity_factor=1.25,
+                       resort_every=[resort, 0] if resort > 0 else None, poisson="pcg")
     torch.cuda.synchronize()
     t_init = time.perf_counter() - t0
     istep = 1
@@ -487,6 +489,7 @@ def _combined_cfg5(n, ppc, steps=10, warmup=3):
     out = {"value": (slots - int(holes.sum().item())) / wall, "unit": "particle-steps/s",
            "ms_per_step": wall * 1e3 / steps, "steps": steps, "warmup": warmup,
            "ppc": ppc, "loadable_cells": cells, "particles": n_before,
+           "electron_resort_every": resort,
            "pcg_iterations_per_solve": sum(cycles) / len(cycles),
            "phase_ms_per_step": {k: v[0] / steps for k, v in phases.items()},
            "push_GBs": (BYTES_PER_PARTICLE_STEP_F32 * push_units / (push_ms * 1e-3) / 1e9
@@ -635,7 +638,7 @@ def _other_legs(args, line):
         line["poisson_mixed"] = _poisson_cfg3(args.poisson_n)
         print("[bench] poisson done", file=sys.stderr, flush=True)
     if args.combined_n:
-        line["combined"] = _combined_cfg5(args.combined_n, 32)
+        line["combined"] = _combined_cfg5(args.combined_n, 32, resort=args.resort_electrons)
         print("[bench] combined done", file=sys.stderr, flush=True)
         if args.combined_box_count:
             # the same run with the particle count of 32 ppc x 2 species over
@@ -643,7 +646,8 @@ def _other_legs(args, line):
             n = args.combined_n
             cells = cfg.loadable_cells(cfg.annular_channel_bcnd((n,) * 3, (3.125e-4,) * 3),
                                        (n,) * 3)
-            line["combined_box_count"] = _combined_cfg5(n, int(round(32 * n ** 3 / cells)))
+            line["combined_box_count"] = _combined_cfg5(n, int(round(32 * n ** 3 / cells)),
+                                                        resort=args.resort_electrons)
             print("[bench] combined (box count) done", file=sys.stderr, flush=True)
     if args.adjoint_steps:
         line["adjoint"] = _adjoint_cfg4(args.adjoint_steps)

@@ -1652,7 +1652,7 @@ int pcg_solve(lepic_ctx *ctx, const lepic_grid *g, double *phi, const double *rh
                       "poisson_pcg: labels 0 or -2 (the operator is not symmetric), or plasma "
                       "on a ghost node");
   const double res_s = res;
-  int k = 0, restarts = 0;
+  int k = 0;
   bool fresh = true;   // r was just formed from phi
   int status = LEPIC_OK;
   for (;;) {
@@ -1661,7 +1661,6 @@ int pcg_solve(lepic_ctx *ctx, const lepic_grid *g, double *phi, const double *rh
       if ((st = restart())) return st;   // confirm on the residual from phi
       if ((st = read_back(res, flag))) return st;
       fresh = true;
-      ++restarts;
       if (res < mp->eps) break;
     }
     if (k == mp->ncycl) break;
@@ -1674,8 +1673,10 @@ int pcg_solve(lepic_ctx *ctx, const lepic_grid *g, double *phi, const double *rh
     LEPIC_LAUNCH_CHECK(ctx, "pcg iteration");
     if ((st = read_back(res, flag))) return st;
     fresh = false;
-    if (!(res <= 10.0 * res_s)) {
-      status = lepic_fail(ctx, LEPIC_ERR_DIVERGED, "poisson_pcg: res/res_s > 10 (diverging)");
+    // the residual of conjugate gradients need not fall at every iteration:
+    // only a non-finite or exploding one stops the solve
+    if (!(res <= 1.0e6 * res_s)) {
+      status = lepic_fail(ctx, LEPIC_ERR_DIVERGED, "poisson_pcg: res/res_s > 1e6 (diverging)");
       break;
     }
   }
@@ -1692,7 +1693,6 @@ int pcg_solve(lepic_ctx *ctx, const lepic_grid *g, double *phi, const double *rh
     stats->ktot = ktot;
     stats->res = res;
   }
-  (void)restarts;
   return status;
 }
 
