@@ -36,6 +36,10 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_PARTICLE_STEP = 104   # FP64 fused gather+push+BC+deposit, state read and written once
 BYTES_PER_PARTICLE_STEP_F32 = 56   # FP32: int32 cell + 3 FP32 offsets + 3 FP32 velocities, r + w
 COPY_PROBE_BYTES = 1 << 31
+# FP64 multigrid, per node and executed sweep: u read and written, b and the
+# int32 label read (28 B), and the sweep's residual pass (u, b, label: 20 B);
+# restriction and prolongation not counted
+MG_BYTES_PER_NODE_SWEEP = 48
 SOLVES = {"fp64": "the FP64 multigrid",
           "pcg64": "FP64 conjugate gradients preconditioned with one FP64 V-cycle",
           "pcg": "FP64 conjugate gradients preconditioned with one FP32 V-cycle"}
@@ -223,9 +227,14 @@ def _step_bench(args, precision):
     kernel_phases = sim.phase_times()
     sim.timers = {}
     sim.timed_phases = None
+    mg_node_sweeps = 0.0
     for _ in range(args.steps):
         sim.advance_one_step(istep)
         istep += 1
+        # sweeps per level of this solve (each with its residual pass) x the
+        # level's nodes, (n / 2^l + 1)^3
+        mg_node_sweeps += sum(sw * ((args.n >> l) + 1) ** 3
+                              for l, sw in enumerate(sim.last_mg["sweeps"]))
     torch.cuda.synchronize()
     phases = sim.phase_times()
     if precision == "f32":
@@ -233,6 +242,8 @@ def _step_bench(args, precision):
     fused_ms, fused_launches, fused_units = kernel_phases.get("push_bc_deposit", (0.0, 0, 0))
     achieved = bpp * fused_units / (fused_ms * 1e-3) / 1e9 if fused_ms else 0.0
     mg_ms, mg_calls, _ = phases.get("poisson", (0.0, 1, 0))
+    mg_bytes = MG_BYTES_PER_NODE_SWEEP * mg_node_sweeps
+    mg_gbs = mg_bytes / (mg_ms * 1e-3) / 1e9 if mg_ms else 0.0
     out = {
         "value": particle_steps / wall,
         "ms_per_step": wall * 1e3 / args.steps,
@@ -244,6 +255,10 @@ def _step_bench(args, precision):
         "electron_resort_every": 0 if resort is None else resort[0],
         "mg_solve_ms": mg_ms / max(mg_calls, 1),
         "mg_vcycles_per_solve": sum(mg_cycles) / max(len(mg_cycles), 1),
+        "mg_roofline": {"bound": "hbm", "bytes_per_node_sweep": MG_BYTES_PER_NODE_SWEEP,
+                        "algorithmic_GB_per_solve": mg_bytes / max(mg_calls, 1) / 1e9,
+                        "achieved": mg_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": mg_gbs / HBM_PEAK_GBS},
         "phase_ms_per_step": {k: v[0] / args.steps for k, v in phases.items()},
         "roofline": {"bound": "hbm", "kernel": kernel,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -609,6 +624,7 @@ def main():
                    "parallelism": "1 GPU"},
         "mg_solve_ms": main_run["mg_solve_ms"],
         "mg_vcycles_per_solve": main_run["mg_vcycles_per_solve"],
+        "mg_roofline": main_run["mg_roofline"],
         "phase_ms_per_step": main_run["phase_ms_per_step"],
         "phase_timing": "second pass of the same number of steps, every phase between HIP "
                         "events (the timed pass brackets only the fused particle kernel)",
