@@ -6,6 +6,7 @@ liblepic_hip.so on torch's current stream.
 from __future__ import annotations
 
 import ctypes as C
+import threading
 
 import numpy as np
 import torch
@@ -22,14 +23,18 @@ def require_gpu(device=None) -> torch.device:
 
 
 def context(device=None) -> _native.Context:
-    """The lepic context of `device`, bound to torch's current stream."""
+    """The lepic context of `device` for the calling thread, bound to torch's
+    current stream (which is per thread too).  A context holds the library's
+    scratch and control buffers, so threads that drive the device at once
+    (a sort on a side stream beside the field solve) each get their own."""
     dev = require_gpu(device)
     idx = dev.index if dev.index is not None else torch.cuda.current_device()
-    ctx = _ctx.get(idx)
+    key = (idx, threading.get_ident())
+    ctx = _ctx.get(key)
     if ctx is None:
         with torch.cuda.device(idx):
             ctx = _native.Context(idx)
-        _ctx[idx] = ctx
+        _ctx[key] = ctx
     ctx.bind_stream(torch.cuda.current_stream(idx).cuda_stream)
     return ctx
 

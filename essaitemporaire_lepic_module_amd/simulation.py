@@ -353,7 +353,7 @@ class Simulation32(Simulation):
 
     def __init__(self, case: Case, device=None, philox_key: int = 0x5EED5EED,
                  capacity_factor: float = 1.3, resort_every=None, poisson: str = "fp64",
-                 parts64=None):
+                 parts64=None, overlap_sort: bool = True):
         from . import particles32 as P32
         if case.flag_die or case.heating is not None:
             raise ValueError("the FP32 step has no dielectric charging or electron heating")
@@ -368,6 +368,13 @@ class Simulation32(Simulation):
         if poisson not in ("fp64", "mixed", "pcg", "pcg64"):
             raise ValueError(f"unknown Poisson solve {poisson!r}")
         self.poisson = poisson
+        # the step's sorts touch only the particles and the field phase only
+        # the grid: with overlap_sort the sorts run on a side stream, driven
+        # by a worker thread with its own lepic context, while this thread
+        # drives the density, the solve and E (unless the sort phase is timed)
+        self.overlap_sort = overlap_sort
+        self._side = None
+        self._pool = None
         self.resort_every32 = list(resort_every) if resort_every is not None else None
         n, h = self.n, self.h
         shape = node_shape(n)
@@ -423,20 +430,47 @@ class Simulation32(Simulation):
         if esc:
             raise RuntimeError(f"{esc} particles left the domain without meeting a wall")
 
+    def _sorts_due(self, istep: int) -> list:
+        if istep % self.case.nb_step_sort == 1:
+            return list(range(self.ntype))
+        if self.resort_every32 is None:
+            return []
+        return [s for s in range(self.ntype)
+                if self.resort_every32[s] and (istep - 1) % self.resort_every32[s] == 0]
+
+    def _start_sorts(self, due: list):
+        """The sorts of `due` on the side stream, after the work already on
+        this thread's stream; returns the future of their completion event."""
+        import concurrent.futures
+        if self._pool is None:
+            self._side = torch.cuda.Stream(self.dev)
+            self._pool = concurrent.futures.ThreadPoolExecutor(1)
+        ready = torch.cuda.Event()
+        ready.record(torch.cuda.current_stream(self.dev))
+        side, dev = self._side, self.dev
+
+        def work():
+            with torch.cuda.device(dev), torch.cuda.stream(side):
+                side.wait_event(ready)
+                for s in due:
+                    self._compact(s)
+                done = torch.cuda.Event()
+                done.record(side)
+            return done
+        return self._pool.submit(work)
+
     def advance_one_step(self, istep: int):
         c = self.case
         n, h = self.n, self.h
         P32 = self._P32
-        if istep % c.nb_step_sort == 1:
+        due = self._sorts_due(istep)
+        sorting = None
+        if due and self.overlap_sort and not self._timing("sort"):
+            sorting = self._start_sorts(due)
+        elif due:
             with self._phase("sort"):
-                for s in range(self.ntype):
+                for s in due:
                     self._compact(s)
-        elif self.resort_every32 is not None:
-            with self._phase("sort"):
-                for s in range(self.ntype):
-                    k = self.resort_every32[s]
-                    if k and (istep - 1) % k == 0:
-                        self._compact(s)
         with self._phase("rho"):
             F.reduce_species_density(n, self.bcnd, self.np_thread, self.ntype, self.np_red)
             F.build_rho_from_np(n, self.np_red, self.charge, self.ntype, self.rho)
@@ -457,7 +491,16 @@ class Simulation32(Simulation):
         collide = (self.tables is not None and c.nb_step_collisions > 0
                    and istep % c.nb_step_collisions == 0)
         self.dep.zero_()
-        for s, p in enumerate(self.parts):
+        # the species being sorted push last, so that their sorts also
+        # overlap the other species' pushes
+        order = list(range(self.ntype))
+        if sorting is not None:
+            order = [s for s in order if s not in due] + [s for s in order if s in due]
+        for s in order:
+            if sorting is not None and s in due:
+                torch.cuda.current_stream(self.dev).wait_event(sorting.result())
+                sorting = None
+            p = self.parts[s]
             units = (p.n, self.lost_acc[s].clone()) if self._timing("push_bc_deposit") else 0
             with self._phase("push_bc_deposit", units):
                 P32.push32(p, n, h, self.E4f, self.bf32[s], self.charge[s], self.mass[s], c.dt,
