@@ -335,6 +335,19 @@ class Simulation:
         return float(sum(q * nm * p.n for q, nm, p in zip(self.charge, self.Nm, self.parts)))
 
 
+_SIDE = {}
+
+
+def _side_worker(dev):
+    """The process's one worker thread for side-stream work (it gets its own
+    lepic context) and the side stream of `dev` (default priority)."""
+    import concurrent.futures
+    key = torch.device(dev).index or 0
+    if key not in _SIDE:
+        _SIDE[key] = (concurrent.futures.ThreadPoolExecutor(1), torch.cuda.Stream(dev))
+    return _SIDE[key]
+
+
 class Simulation32(Simulation):
     """The step in its precision form: particles as int32 cell + FP32 in-cell
     offsets + FP32 velocities (particles32.ParticleSet32), the FP32 gather +
@@ -373,8 +386,6 @@ class Simulation32(Simulation):
         # by a worker thread with its own lepic context, while this thread
         # drives the density, the solve and E (unless the sort phase is timed)
         self.overlap_sort = overlap_sort
-        self._side = None
-        self._pool = None
         self.resort_every32 = list(resort_every) if resort_every is not None else None
         n, h = self.n, self.h
         shape = node_shape(n)
@@ -441,13 +452,10 @@ class Simulation32(Simulation):
     def _start_sorts(self, due: list):
         """The sorts of `due` on the side stream, after the work already on
         this thread's stream; returns the future of their completion event."""
-        import concurrent.futures
-        if self._pool is None:
-            self._side = torch.cuda.Stream(self.dev)
-            self._pool = concurrent.futures.ThreadPoolExecutor(1)
+        pool, side = _side_worker(self.dev)
         ready = torch.cuda.Event()
         ready.record(torch.cuda.current_stream(self.dev))
-        side, dev = self._side, self.dev
+        dev = self.dev
 
         def work():
             with torch.cuda.device(dev), torch.cuda.stream(side):
@@ -457,7 +465,7 @@ class Simulation32(Simulation):
                 done = torch.cuda.Event()
                 done.record(side)
             return done
-        return self._pool.submit(work)
+        return pool.submit(work)
 
     def advance_one_step(self, istep: int):
         c = self.case
