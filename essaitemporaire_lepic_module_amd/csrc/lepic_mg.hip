@@ -117,13 +117,12 @@ __device__ inline T getres(const KLevelT<T> &L, int ir, int jr, int kr, int flag
   return urhs - (aw * uw + ae * ue + L.an * un + L.as * us + L.ac * uij + L.at * ut + L.ab * ub);
 }
 
-__global__ void k_begin(MGCtl *ctl) {
-  if (threadIdx.x == 0) {
-    ctl->active = 1;
-    ctl->ksor = 0;
-    ctl->res = 0.0;
-    ctl->res_s = 0.0;
-  }
+// sor_rb's start (the level active, no sweep counted yet).
+__device__ inline void ctl_begin(MGCtl *ctl) {
+  ctl->active = 1;
+  ctl->ksor = 0;
+  ctl->res = 0.0;
+  ctl->res_s = 0.0;
 }
 
 __global__ void k_reset_totals(MGCtl *ctl, int n) {
@@ -173,10 +172,16 @@ __device__ inline void sor_node(const KLevelT<T> &L, int colour, T omega, int64_
 }
 
 // ctl == nullptr: a fixed sweep count (the level is always active).
+// begin != nullptr: the first red sweep of an sor_rb, which always runs and
+// arms the level's control block for the kernels after it.
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_sor(KLevelT<T> L, int colour, T omega,
-                                               const MGCtl *ctl) {
-  if (ctl && ctl->active == 0) return;
+                                               const MGCtl *ctl, MGCtl *begin) {
+  if (begin) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl_begin(begin);
+  } else if (ctl && ctl->active == 0) {
+    return;
+  }
   sor_node(L, colour, omega, blockIdx.x * (int64_t)BLOCK + threadIdx.x);
 }
 
@@ -269,7 +274,9 @@ __global__ __launch_bounds__(BLOCK) void k_resid(KLevelT<T> L, double *partials,
 
 // Sum the partials in a fixed order and apply sor_rb's exit rules
 // (sors.f90:446-454).  A separate one-block launch: the kernel boundary
-// makes every block's partial visible without a fence per block.
+// makes every block's partial visible without a fence per block (a
+// last-block-done reduction inside k_resid, with an agent-scope
+// acquire-release count per block, measured slower than this launch).
 __global__ __launch_bounds__(BLOCK) void k_decide(const double *partials, int64_t nparts,
                                                   MGCtl *ctl, double eps, int dig, int64_t nr) {
   if (ctl->active == 0) return;
@@ -343,9 +350,11 @@ __global__ void k_rpbc(KLevelT<T> F, T *rp, int flag_nmn) {
 
 // Restriction (mg.f90:413-718): coarse labels injected from the fine grid and
 // the 27-point full-weighting residual with the reference's neighbour rules.
+// R != nullptr: getres of every node of the fine residual range, formed
+// beforehand by k_getres (the same expression, so the same bits).
 template <class T>
 __device__ inline void restrict_node(const KLevelT<T> &F, const KLevelT<T> &C, const T *rp,
-                                     int flag_nmn, int64_t t) {
+                                     int flag_nmn, int64_t t, const T *R = nullptr) {
   const KGrid &g = F.g;
   const KGrid &gc = C.g;
   const int m = g.n3;
@@ -373,7 +382,10 @@ __device__ inline void restrict_node(const KLevelT<T> &F, const KLevelT<T> &C, c
     const int n2 = g.n2;
     const int64_t rs = g.n1 + 2;
     auto B = [&](int a, int b_, int c) { return bc[gidx(g, a, b_, c)]; };
-    auto GR = [&](int a, int b_, int c) { return getres(F, a, b_, c, flag_nmn); };
+    auto GR = [&](int a, int b_, int c) {
+      const bool in = a >= 1 && a <= N1 && b_ >= 1 && b_ <= N2 && c >= 1 && c <= M1;
+      return (R && in) ? R[gidx(g, a, b_, c)] : getres(F, a, b_, c, flag_nmn);
+    };
     auto RP = [&](int a, int b_) { return rp ? rp[(int64_t)a + rs * b_] : T(0); };
     const bool kb = (kr == 1);
     T r000 = 0, rpp0 = 0, rp00 = 0, rpm0 = 0, r0m0 = 0, rmm0 = 0, rm00 = 0, rmp0 = 0,
@@ -475,8 +487,28 @@ __device__ inline void restrict_node(const KLevelT<T> &F, const KLevelT<T> &C, c
 
 template <class T>
 __global__ __launch_bounds__(BLOCK) void k_restrict(KLevelT<T> F, KLevelT<T> C, const T *rp,
-                                                    int flag_nmn) {
-  restrict_node(F, C, rp, flag_nmn, blockIdx.x * (int64_t)BLOCK + threadIdx.x);
+                                                    int flag_nmn, const T *R) {
+  restrict_node(F, C, rp, flag_nmn, blockIdx.x * (int64_t)BLOCK + threadIdx.x, R);
+}
+
+// The restriction's first half: getres (mg.f90:809-848) at every node of
+// the fine level's residual range (1..n+1 in each direction) into R, once
+// per node instead of once per coarse neighbour that reads it, and the
+// coarse correction u zeroed (nc nodes).  The restriction's few getres
+// calls outside that range (the n2 wrap at i = n1 + 2) still go to getres.
+template <class T>
+__global__ __launch_bounds__(BLOCK) void k_getres(KLevelT<T> F, T *R, T *cu, int64_t nc,
+                                                  int flag_nmn) {
+  const int64_t t = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+  if (t < nc) cu[t] = T(0);
+  const KGrid &g = F.g;
+  const int64_t nx = g.n1 + 1, ny = g.n2 + 1, nz = g.n3 + 1;
+  if (t >= nx * ny * nz) return;
+  const uint32_t tt = (uint32_t)t, r = mdiv_q(tt, F.d_n1p1), rk = mdiv_q(r, F.d_n2p1);
+  const int i = 1 + (int)(tt - r * (uint32_t)nx);
+  const int j = 1 + (int)(r - rk * (uint32_t)ny);
+  const int k = 1 + (int)rk;
+  R[gidx(g, i, j, k)] = getres(F, i, j, k, flag_nmn);
 }
 
 // Prolongation (mg.f90:886-935): one thread per coarse node updates its 8
@@ -731,6 +763,9 @@ struct Solver {
   // fixed sweep counts (the inner smoother of the mixed-precision solve):
   // nr_fixed per sor_rb, nr_coarsest on the coarsest level, no residuals
   int fixed = 0, nr_fixed = 2, nr_coarsest = 64;
+  // the fine residual of a restriction (the finest level's ghosted size), or
+  // nullptr: the restriction forms it per coarse node
+  T *resbuf = nullptr;
 
   void coarse(double eps, int kmg) {
     CoarseArgs<T> A;
@@ -750,7 +785,8 @@ struct Solver {
     hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(CBLOCK), 0, ctx->stream, A);
   }
 
-  void sor_sweep(int l, double eps, int dig, int64_t nr) {
+  // One sweep; first: the sor_rb's first, whose red half arms ctl[l].
+  void sor_sweep(int l, double eps, int dig, int64_t nr, bool first) {
     KLevelT<T> &Lv = L[l];
     const KGrid &g = Lv.g;
     int64_t nsor = (int64_t)(g.n1 / 2 + 1) * (g.n2 + 1) * (g.n3 + 1);
@@ -760,7 +796,8 @@ struct Solver {
     hipStream_t s = ctx->stream;
     const MGCtl *c = fixed ? nullptr : &ctl[l];   // fixed counts: no exit, no control block
     for (int colour = 0; colour < 2; ++colour) {
-      hipLaunchKernelGGL(k_sor<T>, dim3(bs), dim3(BLOCK), 0, s, Lv, colour, omega, c);
+      MGCtl *begin = (first && colour == 0 && !fixed) ? &ctl[l] : nullptr;
+      hipLaunchKernelGGL(k_sor<T>, dim3(bs), dim3(BLOCK), 0, s, Lv, colour, omega, c, begin);
       if (flag_pbc) {
         hipLaunchKernelGGL(k_per_y<T>, dim3(grid_blocks((int64_t)(g.n1 + 3) * (g.n3 + 3), BLOCK)),
                            dim3(BLOCK), 0, s, Lv, c);
@@ -777,17 +814,16 @@ struct Solver {
   // sor_rb: returns LEPIC_OK; sweeps are counted on the device.
   int sor_rb(int l, double eps, int kmg, int dig) {
     if (fixed) {
-      for (int c = 0; c < (dig == 0 ? nr_coarsest : nr_fixed); ++c) sor_sweep(l, eps, dig, 4);
+      for (int c = 0; c < (dig == 0 ? nr_coarsest : nr_fixed); ++c) sor_sweep(l, eps, dig, 4, false);
       return LEPIC_OK;
     }
-    hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, ctx->stream, &ctl[l]);
     if (kmg == 1 && dig == 0) {
       // first V-cycle's coarsest level: up to 1e6 sweeps to res <= eps.
       const int64_t nr = 1000000;
       int64_t done = 0;
       while (done < nr) {
         int chunk = 32;
-        for (int c = 0; c < chunk; ++c) sor_sweep(l, eps, dig, nr);
+        for (int c = 0; c < chunk; ++c) sor_sweep(l, eps, dig, nr, done == 0 && c == 0);
         done += chunk;
         LEPIC_HIP(ctx, hipMemcpyAsync(ctx->h_pinned, &ctl[l], sizeof(MGCtl), hipMemcpyDeviceToHost,
                                       ctx->stream));
@@ -795,7 +831,7 @@ struct Solver {
         if (ctx->h_pinned[0] == 0) break;
       }
     } else {
-      for (int c = 0; c < 4; ++c) sor_sweep(l, eps, dig, 4);
+      for (int c = 0; c < 4; ++c) sor_sweep(l, eps, dig, 4, c == 0);
     }
     return LEPIC_OK;
   }
@@ -805,8 +841,14 @@ struct Solver {
     KLevelT<T> &C = L[l + 1];
     const KGrid &g = F.g;
     int64_t nc = grid_nodes_k(C.g);
-    hipLaunchKernelGGL(k_fill_d<T>, dim3(grid_blocks(nc, BLOCK)), dim3(BLOCK), 0, ctx->stream, C.u,
-                       nc, T(0));
+    if (resbuf) {
+      const int64_t nres = (int64_t)(g.n1 + 1) * (g.n2 + 1) * (g.n3 + 1);
+      hipLaunchKernelGGL(k_getres<T>, dim3(grid_blocks(std::max(nc, nres), BLOCK)), dim3(BLOCK), 0,
+                         ctx->stream, F, resbuf, C.u, nc, flag_nmn);
+    } else {
+      hipLaunchKernelGGL(k_fill_d<T>, dim3(grid_blocks(nc, BLOCK)), dim3(BLOCK), 0, ctx->stream,
+                         C.u, nc, T(0));
+    }
     const T *rpp = nullptr;
     if (flag_pbc) {
       int64_t nent = (int64_t)(g.n1 + 2) * (g.n2 + 2);
@@ -816,7 +858,7 @@ struct Solver {
     }
     int64_t nt = (int64_t)(g.n1 / 2 + 1) * (g.n2 / 2 + 1) * (g.n3 / 2 + 1);
     hipLaunchKernelGGL(k_restrict<T>, dim3(grid_blocks(nt, BLOCK)), dim3(BLOCK), 0, ctx->stream, F,
-                       C, rpp, flag_nmn);
+                       C, rpp, flag_nmn, (const T *)resbuf);
   }
 
   void prolongation(int l) {  // L[l] += P(L[l+1])
@@ -939,7 +981,8 @@ extern "C" int lepic_poisson_solve(lepic_ctx *ctx, const lepic_grid *g, double *
   }
   // coarse-level storage (kept across calls; labels zeroed once)
   const int64_t rp_len = ((int64_t)g->n[0] + 2) * ((int64_t)g->n[1] + 2);
-  size_t need = sizeof(double) * (size_t)(2 * coarse_nodes + max_parts + rp_len) +
+  const int64_t fine_nodes = ng > 1 ? grid_nodes(g) : 0;   // the restriction's residual
+  size_t need = sizeof(double) * (size_t)(2 * coarse_nodes + max_parts + rp_len + fine_nodes) +
                 sizeof(int32_t) * (size_t)coarse_nodes + 4096;
   bool same = ctx->mg_mem && ctx->mg_ng == ng && ctx->mg_n[0] == g->n[0] &&
               ctx->mg_n[1] == g->n[1] && ctx->mg_n[2] == g->n[2] && ctx->mg_bytes >= need;
@@ -960,7 +1003,8 @@ extern "C" int lepic_poisson_solve(lepic_ctx *ctx, const lepic_grid *g, double *
   S.partials = dmem;
   S.rp = dmem + max_parts;
   double *lv = S.rp + rp_len;
-  int32_t *lb = reinterpret_cast<int32_t *>(lv + 2 * coarse_nodes);
+  S.resbuf = fine_nodes ? lv + 2 * coarse_nodes : nullptr;
+  int32_t *lb = reinterpret_cast<int32_t *>(lv + 2 * coarse_nodes + fine_nodes);
   S.L[0].u = phi;
   S.L[0].b = const_cast<double *>(rho);
   S.L[0].bc = const_cast<int32_t *>(bcnd);
@@ -1216,6 +1260,7 @@ int mix_setup(lepic_ctx *ctx, const lepic_grid *g, double *phi, const double *rh
   const size_t need = sizeof(double) * (size_t)(MIX_COLS * MIX_BLOCKS + 8) +
                       sizeof(double) * (size_t)nf64 * (size_t)nodes +
                       sizeof(T) * (size_t)(2 * nodes + 2 * coarse_nodes + rp_len) +
+                      sizeof(T) * (size_t)(ng > 1 ? nodes : 0) +   // the restriction's residual
                       sizeof(int32_t) * (size_t)coarse_nodes + 4096;
   if (!ctx->mgx_mem || ctx->mgx_bytes < need) {
     if (ctx->mgx_mem) {
@@ -1239,7 +1284,8 @@ int mix_setup(lepic_ctx *ctx, const lepic_grid *g, double *phi, const double *rh
   T *lv = M.r32 + nodes;
   S.rp = lv;
   lv += rp_len;
-  int32_t *lb = reinterpret_cast<int32_t *>(lv + 2 * coarse_nodes);
+  S.resbuf = ng > 1 ? lv + 2 * coarse_nodes : nullptr;
+  int32_t *lb = reinterpret_cast<int32_t *>(lv + 2 * coarse_nodes + (ng > 1 ? nodes : 0));
   S.L[0].u = M.e32;
   S.L[0].b = M.r32;
   S.L[0].bc = const_cast<int32_t *>(bcnd);
