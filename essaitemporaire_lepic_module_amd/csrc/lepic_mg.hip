@@ -584,20 +584,19 @@ struct CoarseArgs {
   int nr_fixed, nr_coarsest;  // those counts (fixed = 1)
   T *rp;
   MGCtl *ctl;
+  int lds_off[MAX_LEVELS];    // level l's u at lds + lds_off[l], b lds_nn[l] after it
+  int lds_nn[MAX_LEVELS];
 };
 
-// Level l as the workgroup sees it: u and b in LDS.
+// Level l as the workgroup sees it: u and b in LDS.  The pointers are formed
+// from the workgroup's LDS array itself, never stored and reloaded, so the
+// compiler keeps them in the LDS address space (ds_read/ds_write rather than
+// flat accesses, which wait on the vector memory counter too).
 template <class T>
-struct CoarseLDS {
-  T *u[MAX_LEVELS];
-  T *b[MAX_LEVELS];
-};
-
-template <class T>
-__device__ inline KLevelT<T> lds_level(const CoarseArgs<T> &A, const CoarseLDS<T> &S, int l) {
+__device__ inline KLevelT<T> lds_level(const CoarseArgs<T> &A, T *lds, int l) {
   KLevelT<T> v = A.L[l];
-  v.u = S.u[l];
-  v.b = S.b[l];
+  v.u = lds + A.lds_off[l];
+  v.b = lds + A.lds_off[l] + A.lds_nn[l];
   return v;
 }
 
@@ -617,7 +616,7 @@ __device__ inline double wg_sum(double a, double *red) {
 // sor_rb (sors.f90:1-535) on level l: at most nr sweeps, exit rules as k_decide
 // (with A.fixed, exactly nr sweeps and no residual).
 template <class T>
-__device__ void wg_sor_rb(const CoarseArgs<T> &A, const CoarseLDS<T> &S, int l, int dig, int64_t nr,
+__device__ void wg_sor_rb(const CoarseArgs<T> &A, T *S, int l, int dig, int64_t nr,
                           double *red) {
   const KLevelT<T> Lv = lds_level(A, S, l);
   const KGrid &g = Lv.g;
@@ -662,7 +661,7 @@ __device__ void wg_sor_rb(const CoarseArgs<T> &A, const CoarseLDS<T> &S, int l, 
 }
 
 template <class T>
-__device__ void wg_restriction(const CoarseArgs<T> &A, const CoarseLDS<T> &S, int l) {
+__device__ void wg_restriction(const CoarseArgs<T> &A, T *S, int l) {
   const KLevelT<T> F = lds_level(A, S, l);
   const KLevelT<T> C = lds_level(A, S, l + 1);
   const KGrid &g = F.g;
@@ -681,7 +680,7 @@ __device__ void wg_restriction(const CoarseArgs<T> &A, const CoarseLDS<T> &S, in
 }
 
 template <class T>
-__device__ void wg_prolongation(const CoarseArgs<T> &A, const CoarseLDS<T> &S, int l) {  // L[l] += P(L[l+1])
+__device__ void wg_prolongation(const CoarseArgs<T> &A, T *S, int l) {  // L[l] += P(L[l+1])
   const KLevelT<T> F = lds_level(A, S, l);
   const KLevelT<T> C = lds_level(A, S, l + 1);
   const KGrid &g = F.g;
@@ -702,22 +701,14 @@ template <class T>
 __global__ __launch_bounds__(CBLOCK) void k_mg_coarse(CoarseArgs<T> A) {
   __shared__ double red[CBLOCK / LEPIC_WAVE];
   __shared__ T lds[2 * COARSE_LDS_NODES];
-  __shared__ CoarseLDS<T> S;   // level pointers into lds, uniform
+  T *const S = lds;
   const int ng = A.ng;
-  {
-    T *p = lds;
-    for (int l = A.lc; l < ng; ++l) {
-      const KGrid &g = A.L[l].g;
-      const int64_t nn = ((int64_t)g.n1 + 3) * ((int64_t)g.n2 + 3) * ((int64_t)g.n3 + 3);
-      if (threadIdx.x == 0) {
-        S.u[l] = p;
-        S.b[l] = p + nn;
-      }
-      for (int64_t t = threadIdx.x; t < nn; t += CBLOCK) {
-        p[t] = A.L[l].u[t];
-        p[nn + t] = A.L[l].b[t];
-      }
-      p += 2 * nn;
+  for (int l = A.lc; l < ng; ++l) {
+    T *p = lds + A.lds_off[l];
+    const int nn = A.lds_nn[l];
+    for (int t = threadIdx.x; t < nn; t += CBLOCK) {
+      p[t] = A.L[l].u[t];
+      p[nn + t] = A.L[l].b[t];
     }
   }
   __syncthreads();
@@ -739,7 +730,7 @@ __global__ __launch_bounds__(CBLOCK) void k_mg_coarse(CoarseArgs<T> A) {
   {
     const KGrid &g = A.L[A.lc].g;
     const int64_t nn = ((int64_t)g.n1 + 3) * ((int64_t)g.n2 + 3) * ((int64_t)g.n3 + 3);
-    for (int64_t t = threadIdx.x; t < nn; t += CBLOCK) A.L[A.lc].u[t] = S.u[A.lc][t];
+    for (int64_t t = threadIdx.x; t < nn; t += CBLOCK) A.L[A.lc].u[t] = lds[A.lds_off[A.lc] + t];
   }
 }
 
@@ -782,6 +773,15 @@ struct Solver {
     A.nr_coarsest = nr_coarsest;
     A.rp = rp;
     A.ctl = ctl;
+    int off = 0;
+    for (int l = 0; l < MAX_LEVELS; ++l) {
+      A.lds_off[l] = 0;
+      A.lds_nn[l] = 0;
+      if (l < lc || l >= ng) continue;
+      A.lds_off[l] = off;
+      A.lds_nn[l] = (int)grid_nodes_k(L[l].g);
+      off += 2 * A.lds_nn[l];
+    }
     hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(CBLOCK), 0, ctx->stream, A);
   }
 
