@@ -54,13 +54,16 @@ __device__ inline uint32_t mdiv_q(uint32_t x, const MDiv &v) {
   return v.d <= 1 ? x : (uint32_t)__umul64hi((uint64_t)x, v.m);
 }
 
-template <class T>
+// B: the label type (int32_t in global memory; int16_t in the coarse
+// workgroup's LDS, labels saturated to that range, which keeps every test
+// the solver makes of them: sign, 0, -1, -2).
+template <class T, class B = int32_t>
 struct KLevelT {
   KGrid g;
   T aw, ae, an, as, at, ab, ac;
   T *u;
   T *b;
-  int32_t *bc;
+  B *bc;
   // thread-number decoders: n1/2+1, n2+1, n1+1, n1+3, n1+2, n2/2+1
   MDiv d_h1, d_n2p1, d_n1p1, d_n1p3, d_n1p2, d_h2;
   double acy;   // RN(1 / ac) for the FP64 levels' division (0: plain division)
@@ -69,12 +72,12 @@ struct KLevelT {
 // x / ac of the SOR update: for FP64 levels by the reciprocal and two FMA
 // corrections (lepic_qdiv.h: the correctly rounded quotient, identical to
 // the division), the plain division otherwise.
-template <class T>
-__device__ inline T div_ac(T x, const KLevelT<T> &L) {
+template <class T, class B>
+__device__ inline T div_ac(T x, const KLevelT<T, B> &L) {
   return x / L.ac;
 }
-template <>
-__device__ inline double div_ac<double>(double x, const KLevelT<double> &L) {
+template <class B>
+__device__ inline double div_ac(double x, const KLevelT<double, B> &L) {
   if (L.acy == 0.0) return x / L.ac;
   QDiv d;
   d.h = L.ac;
@@ -94,8 +97,8 @@ inline void set_level_divs(KLevelT<T> &L) {
 }
 using KLevel = KLevelT<double>;
 
-template <class T>
-__device__ inline T getres(const KLevelT<T> &L, int ir, int jr, int kr, int flag_nmn) {
+template <class T, class B>
+__device__ inline T getres(const KLevelT<T, B> &L, int ir, int jr, int kr, int flag_nmn) {
   const KGrid &g = L.g;
   if (ir < 0 || ir > g.n1 + 1 || jr < 0 || jr > g.n2 + 1 || kr < 0 || kr > g.n3 + 1) return T(0);
   const T *u = L.u;
@@ -132,8 +135,8 @@ __global__ void k_reset_totals(MGCtl *ctl, int n) {
 
 // One colour of sor_rb (sors.f90:87-140 red, 249-302 black).
 // colour 0 = red = (i+j+k) odd, 1 = black.  Threads over (i2, j, k).
-template <class T>
-__device__ inline void sor_node(const KLevelT<T> &L, int colour, T omega, int64_t t) {
+template <class T, class B>
+__device__ inline void sor_node(const KLevelT<T, B> &L, int colour, T omega, int64_t t) {
   const KGrid &g = L.g;
   const int ni2 = g.n1 / 2 + 1;
   int64_t total = (int64_t)ni2 * (g.n2 + 1) * (g.n3 + 1);
@@ -187,8 +190,8 @@ __global__ __launch_bounds__(BLOCK) void k_sor(KLevelT<T> L, int colour, T omega
 
 // Periodic ghost copies (sors.f90:184-239, single rank): y planes for every
 // (i, k), then z planes for every (i, j).
-template <class T>
-__device__ inline void per_y_node(const KLevelT<T> &L, int64_t t) {
+template <class T, class B>
+__device__ inline void per_y_node(const KLevelT<T, B> &L, int64_t t) {
   const KGrid &g = L.g;
   int64_t ni = g.n1 + 3, nk = g.n3 + 3;
   if (t >= ni * nk) return;
@@ -199,8 +202,8 @@ __device__ inline void per_y_node(const KLevelT<T> &L, int64_t t) {
   if (b2 == 0 || b2 == -2) L.u[gidx(g, i, g.n2 + 1, k)] = L.u[gidx(g, i, 1, k)];
 }
 
-template <class T>
-__device__ inline void per_z_node(const KLevelT<T> &L, int64_t t) {
+template <class T, class B>
+__device__ inline void per_z_node(const KLevelT<T, B> &L, int64_t t) {
   const KGrid &g = L.g;
   int64_t ni = g.n1 + 3, nj = g.n2 + 3;
   if (t >= ni * nj) return;
@@ -226,8 +229,8 @@ __global__ void k_per_z(KLevelT<T> L, const MGCtl *ctl) {
 
 // L1 residual over bcnd == -1 nodes (sors.f90:409-431), block partials.
 // |b - A u| at interior node t of the residual range (0 outside plasma).
-template <class T>
-__device__ inline double resid_node(const KLevelT<T> &L, int64_t t) {
+template <class T, class B>
+__device__ inline double resid_node(const KLevelT<T, B> &L, int64_t t) {
   const KGrid &g = L.g;
   int64_t nx = g.n1 + 1, ny = g.n2 + 1, nz = g.n3 + 1;
   double a = 0.0;
@@ -309,8 +312,8 @@ __global__ __launch_bounds__(BLOCK) void k_decide(const double *partials, int64_
 // winning; here one thread per entry finds that last writer (largest jr,
 // then largest ir) and applies its branch: getres when bcnd(ir,jr,m+1) <= 0,
 // else 0.  The two branches guard the (ir, jr-1) entry differently.
-template <class T>
-__device__ inline void rpbc_entry(const KLevelT<T> &F, T *rp, int flag_nmn, int64_t t) {
+template <class T, class B>
+__device__ inline void rpbc_entry(const KLevelT<T, B> &F, T *rp, int flag_nmn, int64_t t) {
   const KGrid &g = F.g;
   const int64_t rs = g.n1 + 2;
   const int64_t total = rs * (g.n2 + 2);
@@ -352,8 +355,8 @@ __global__ void k_rpbc(KLevelT<T> F, T *rp, int flag_nmn) {
 // the 27-point full-weighting residual with the reference's neighbour rules.
 // R != nullptr: getres of every node of the fine residual range, formed
 // beforehand by k_getres (the same expression, so the same bits).
-template <class T>
-__device__ inline void restrict_node(const KLevelT<T> &F, const KLevelT<T> &C, const T *rp,
+template <class T, class LB>
+__device__ inline void restrict_node(const KLevelT<T, LB> &F, const KLevelT<T, LB> &C, const T *rp,
                                      int flag_nmn, int64_t t, const T *R = nullptr) {
   const KGrid &g = F.g;
   const KGrid &gc = C.g;
@@ -367,9 +370,9 @@ __device__ inline void restrict_node(const KLevelT<T> &F, const KLevelT<T> &C, c
   int k = 1 + (int)rk;
   const int ir = 2 * i - 1, jr = 2 * j - 1, kr = 2 * k - 1;
   const int N1 = g.n1 + 1, N2 = g.n2 + 1, M1 = m + 1;
-  const int32_t *bc = F.bc;
-  int32_t *bc2 = C.bc;
-  int32_t lab = bc[gidx(g, ir, jr, kr)];
+  const LB *bc = F.bc;
+  LB *bc2 = C.bc;
+  const LB lab = bc[gidx(g, ir, jr, kr)];
   bc2[gidx(gc, i, j, k)] = lab;
   if (ir == 1) bc2[gidx(gc, 0, j, k)] = bc[gidx(g, 0, jr, kr)];
   if (ir == N1) bc2[gidx(gc, g.n1 / 2 + 2, j, k)] = bc[gidx(g, g.n1 + 2, jr, kr)];
@@ -513,8 +516,8 @@ __global__ __launch_bounds__(BLOCK) void k_getres(KLevelT<T> F, T *R, T *cu, int
 
 // Prolongation (mg.f90:886-935): one thread per coarse node updates its 8
 // fine nodes (disjoint between threads).
-template <class T>
-__device__ inline void prolong_node(const KLevelT<T> &F, const KLevelT<T> &C, int64_t t) {
+template <class T, class B>
+__device__ inline void prolong_node(const KLevelT<T, B> &F, const KLevelT<T, B> &C, int64_t t) {
   const KGrid &g = F.g;
   const KGrid &gc = C.g;
   const int kp = g.n3 / 2 + 1;
@@ -531,7 +534,7 @@ __device__ inline void prolong_node(const KLevelT<T> &F, const KLevelT<T> &C, in
          rmmm = e[gidx(gc, i - 1, j - 1, k - 1)], rm0m = e[gidx(gc, i - 1, j, k - 1)];
   const int ir = 2 * i - 1, jr = 2 * j - 1, kr = 2 * k - 1;
   T *u = F.u;
-  const int32_t *bc = F.bc;
+  const B *bc = F.bc;
   int64_t c;
   c = gidx(g, ir, jr, kr);
   if (bc[c] <= 0) u[c] = u[c] + r000;
@@ -566,9 +569,9 @@ __global__ __launch_bounds__(BLOCK) void k_prolong(KLevelT<T> F, KLevelT<T> C) {
 // evaluated in the workgroup.
 // --------------------------------------------------------------------------
 constexpr int CBLOCK = 1024;
-// u and b of every level the workgroup handles live in its LDS during the
-// launch (2 x 8704 doubles, 136 KB): each barrier-separated phase then waits
-// on LDS instead of L2.  lc is the first level from which the remaining
+// u, b and the labels of every level the workgroup handles live in its LDS
+// during the launch (2 x 8704 doubles and 8704 int16 labels, 153 KB): each
+// barrier-separated phase then waits on LDS instead of L2.  lc is the first level from which the remaining
 // levels fit.
 constexpr int64_t COARSE_LDS_NODES = 8704;
 
@@ -586,17 +589,32 @@ struct CoarseArgs {
   MGCtl *ctl;
   int lds_off[MAX_LEVELS];    // level l's u at lds + lds_off[l], b lds_nn[l] after it
   int lds_nn[MAX_LEVELS];
+  int lab_off[MAX_LEVELS];    // level l's labels at labs + lab_off[l]
 };
+
+// A label as the coarse workgroup keeps it: saturated to int16, which keeps
+// its sign and the values 0, -1 and -2, all the solver tests.
+__device__ inline int16_t label16(int32_t v) {
+  return (int16_t)(v > 32767 ? 32767 : (v < -32768 ? -32768 : v));
+}
 
 // Level l as the workgroup sees it: u and b in LDS.  The pointers are formed
 // from the workgroup's LDS array itself, never stored and reloaded, so the
 // compiler keeps them in the LDS address space (ds_read/ds_write rather than
 // flat accesses, which wait on the vector memory counter too).
 template <class T>
-__device__ inline KLevelT<T> lds_level(const CoarseArgs<T> &A, T *lds, int l) {
-  KLevelT<T> v = A.L[l];
+__device__ inline KLevelT<T, int16_t> lds_level(const CoarseArgs<T> &A, T *lds, int16_t *labs,
+                                                int l) {
+  const KLevelT<T> &G = A.L[l];
+  KLevelT<T, int16_t> v;
+  v.g = G.g;
+  v.aw = G.aw; v.ae = G.ae; v.an = G.an; v.as = G.as; v.at = G.at; v.ab = G.ab; v.ac = G.ac;
   v.u = lds + A.lds_off[l];
   v.b = lds + A.lds_off[l] + A.lds_nn[l];
+  v.bc = labs + A.lab_off[l];
+  v.d_h1 = G.d_h1; v.d_n2p1 = G.d_n2p1; v.d_n1p1 = G.d_n1p1;
+  v.d_n1p3 = G.d_n1p3; v.d_n1p2 = G.d_n1p2; v.d_h2 = G.d_h2;
+  v.acy = G.acy;
   return v;
 }
 
@@ -616,9 +634,9 @@ __device__ inline double wg_sum(double a, double *red) {
 // sor_rb (sors.f90:1-535) on level l: at most nr sweeps, exit rules as k_decide
 // (with A.fixed, exactly nr sweeps and no residual).
 template <class T>
-__device__ void wg_sor_rb(const CoarseArgs<T> &A, T *S, int l, int dig, int64_t nr,
+__device__ void wg_sor_rb(const CoarseArgs<T> &A, T *S, int16_t *Lb, int l, int dig, int64_t nr,
                           double *red) {
-  const KLevelT<T> Lv = lds_level(A, S, l);
+  const KLevelT<T, int16_t> Lv = lds_level(A, S, Lb, l);
   const KGrid &g = Lv.g;
   const int64_t nsor = (int64_t)(g.n1 / 2 + 1) * (g.n2 + 1) * (g.n3 + 1);
   const int64_t nres = (int64_t)(g.n1 + 1) * (g.n2 + 1) * (g.n3 + 1);
@@ -661,9 +679,9 @@ __device__ void wg_sor_rb(const CoarseArgs<T> &A, T *S, int l, int dig, int64_t 
 }
 
 template <class T>
-__device__ void wg_restriction(const CoarseArgs<T> &A, T *S, int l) {
-  const KLevelT<T> F = lds_level(A, S, l);
-  const KLevelT<T> C = lds_level(A, S, l + 1);
+__device__ void wg_restriction(const CoarseArgs<T> &A, T *S, int16_t *Lb, int l) {
+  const KLevelT<T, int16_t> F = lds_level(A, S, Lb, l);
+  const KLevelT<T, int16_t> C = lds_level(A, S, Lb, l + 1);
   const KGrid &g = F.g;
   const int64_t nc = ((int64_t)C.g.n1 + 3) * ((int64_t)C.g.n2 + 3) * ((int64_t)C.g.n3 + 3);
   for (int64_t t = threadIdx.x; t < nc; t += CBLOCK) C.u[t] = T(0);
@@ -680,9 +698,9 @@ __device__ void wg_restriction(const CoarseArgs<T> &A, T *S, int l) {
 }
 
 template <class T>
-__device__ void wg_prolongation(const CoarseArgs<T> &A, T *S, int l) {  // L[l] += P(L[l+1])
-  const KLevelT<T> F = lds_level(A, S, l);
-  const KLevelT<T> C = lds_level(A, S, l + 1);
+__device__ void wg_prolongation(const CoarseArgs<T> &A, T *S, int16_t *Lb, int l) {  // L[l] += P(L[l+1])
+  const KLevelT<T, int16_t> F = lds_level(A, S, Lb, l);
+  const KLevelT<T, int16_t> C = lds_level(A, S, Lb, l + 1);
   const KGrid &g = F.g;
   const int64_t nt = (int64_t)(g.n1 / 2 + 1) * (g.n2 / 2 + 1) * (g.n3 / 2 + 1);
   for (int64_t t = threadIdx.x; t < nt; t += CBLOCK) prolong_node(F, C, t);
@@ -701,30 +719,38 @@ template <class T>
 __global__ __launch_bounds__(CBLOCK) void k_mg_coarse(CoarseArgs<T> A) {
   __shared__ double red[CBLOCK / LEPIC_WAVE];
   __shared__ T lds[2 * COARSE_LDS_NODES];
+  __shared__ int16_t labs[COARSE_LDS_NODES];
   T *const S = lds;
+  int16_t *const Lb = labs;
   const int ng = A.ng;
+  // level lc's labels are the ones the last global restriction wrote; the
+  // coarser levels' global labels are never written (0, as at allocation),
+  // and the workgroup's restrictions write their values into LDS before any
+  // read, leaving 0 where none writes
   for (int l = A.lc; l < ng; ++l) {
     T *p = lds + A.lds_off[l];
+    int16_t *q = labs + A.lab_off[l];
     const int nn = A.lds_nn[l];
     for (int t = threadIdx.x; t < nn; t += CBLOCK) {
       p[t] = A.L[l].u[t];
       p[nn + t] = A.L[l].b[t];
+      q[t] = label16(A.L[l].bc[t]);
     }
   }
   __syncthreads();
   const int64_t nr = A.fixed ? A.nr_fixed : 4;
   for (int l = A.lc; l <= ng - 2; ++l) {
-    wg_sor_rb(A, S, l, l + 1 - ng, nr, red);
+    wg_sor_rb(A, S, Lb, l, l + 1 - ng, nr, red);
     __syncthreads();
-    wg_restriction(A, S, l);
+    wg_restriction(A, S, Lb, l);
   }
-  wg_sor_rb(A, S, ng - 1, 0, A.fixed ? A.nr_coarsest : (A.first_cycle ? 1000000 : 4), red);
+  wg_sor_rb(A, S, Lb, ng - 1, 0, A.fixed ? A.nr_coarsest : (A.first_cycle ? 1000000 : 4), red);
   __syncthreads();
-  if (ng - 2 >= A.lc) wg_prolongation(A, S, ng - 2);
+  if (ng - 2 >= A.lc) wg_prolongation(A, S, Lb, ng - 2);
   for (int l = ng - 2; l >= A.lc; --l) {
-    wg_sor_rb(A, S, l, ng - 1 - l, nr, red);
+    wg_sor_rb(A, S, Lb, l, ng - 1 - l, nr, red);
     __syncthreads();
-    if (l - 1 >= A.lc) wg_prolongation(A, S, l - 1);
+    if (l - 1 >= A.lc) wg_prolongation(A, S, Lb, l - 1);
   }
   __syncthreads();
   {
@@ -777,9 +803,11 @@ struct Solver {
     for (int l = 0; l < MAX_LEVELS; ++l) {
       A.lds_off[l] = 0;
       A.lds_nn[l] = 0;
+      A.lab_off[l] = 0;
       if (l < lc || l >= ng) continue;
       A.lds_off[l] = off;
       A.lds_nn[l] = (int)grid_nodes_k(L[l].g);
+      A.lab_off[l] = off / 2;
       off += 2 * A.lds_nn[l];
     }
     hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(CBLOCK), 0, ctx->stream, A);
