@@ -156,21 +156,21 @@ __device__ inline void sor_node(const KLevelT<T, B> &L, int colour, T omega, int
   const int64_t c = gidx(g, i, j, k);
   const int bcv = L.bc[c];
   if (bcv > 0) return;
+  if (bcv == 0 && (k == g.n3 + 1 || j == g.n2 + 1)) return;
   T *u = L.u;
+  // every operand read before any arithmetic, so the reads overlap (the
+  // ghost layer keeps u[c - 1] in bounds where the -2 rows do not use it)
+  const T uc = u[c], uw = u[c - 1], ue = u[c + 1], un = u[c + g.sy], us = u[c - g.sy],
+          ut = u[c + g.sz], ub = u[c - g.sz], bb = L.b[c];
   if (bcv == -2) {
-    u[c] = (T(1) - omega) * u[c] +
-           div_ac(omega * (L.b[c] - L.an * u[c + g.sy] - L.as * u[c - g.sy] -
-                           T(2) * L.ae * u[c + 1] - L.at * u[c + g.sz] - L.ab * u[c - g.sz]),
+    u[c] = (T(1) - omega) * uc +
+           div_ac(omega * (bb - L.an * un - L.as * us - T(2) * L.ae * ue - L.at * ut - L.ab * ub),
                   L);
     return;
   }
-  if (bcv == 0) {
-    if (k == g.n3 + 1) return;
-    if (j == g.n2 + 1) return;
-  }
-  u[c] = (T(1) - omega) * u[c] +
-         div_ac(omega * (L.b[c] - L.aw * u[c - 1] - L.an * u[c + g.sy] - L.as * u[c - g.sy] -
-                         L.ae * u[c + 1] - L.at * u[c + g.sz] - L.ab * u[c - g.sz]),
+  u[c] = (T(1) - omega) * uc +
+         div_ac(omega * (bb - L.aw * uw - L.an * un - L.as * us - L.ae * ue - L.at * ut -
+                         L.ab * ub),
                 L);
 }
 
@@ -242,9 +242,10 @@ __device__ inline double resid_node(const KLevelT<T, B> &L, int64_t t) {
     const int64_t c = gidx(g, i, j, k);
     if (L.bc[c] == -1) {
       const T *u = L.u;
-      T rij = L.b[c] - (L.aw * u[c - 1] + L.an * u[c + g.sy] + L.as * u[c - g.sy] +
-                        L.ae * u[c + 1] + L.ac * u[c] + L.at * u[c + g.sz] +
-                        L.ab * u[c - g.sz]);
+      const T uc = u[c], uw = u[c - 1], ue = u[c + 1], un = u[c + g.sy], us = u[c - g.sy],
+              ut = u[c + g.sz], ub = u[c - g.sz], bb = L.b[c];
+      T rij = bb - (L.aw * uw + L.an * un + L.as * us + L.ae * ue + L.ac * uc + L.at * ut +
+                    L.ab * ub);
       a = fabs((double)rij);
     }
   }
