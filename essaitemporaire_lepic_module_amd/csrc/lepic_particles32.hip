@@ -1021,15 +1021,38 @@ __global__ __launch_bounds__(BLOCK) void k_sort_rank(const int32_t *__restrict__
   }
 }
 
+// Four destination slots per thread: one 16-byte read of the source slots,
+// the 28 scattered reads in flight together, 16-byte stores (sidx and the
+// destination arrays are 16-byte aligned at multiples of 4 slots).
 __global__ __launch_bounds__(BLOCK) void k_sort_gather(KP32 src, KP32 dst,
                                                        const int32_t *__restrict__ sidx,
                                                        int64_t m) {
-  const int64_t d = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
-  if (d >= m) return;
-  const int64_t j = sidx[d];
-  dst.cell[d] = src.cell[j];
-  dst.fx[d] = src.fx[j]; dst.fy[d] = src.fy[j]; dst.fz[d] = src.fz[j];
-  dst.vx[d] = src.vx[j]; dst.vy[d] = src.vy[j]; dst.vz[d] = src.vz[j];
+  const int64_t d0 = (blockIdx.x * (int64_t)BLOCK + threadIdx.x) * 4;
+  if (d0 >= m) return;
+  if (d0 + 4 <= m) {
+    const int4 j = *reinterpret_cast<const int4 *>(sidx + d0);
+    const int4 c = make_int4(src.cell[j.x], src.cell[j.y], src.cell[j.z], src.cell[j.w]);
+    const float4 fx = make_float4(src.fx[j.x], src.fx[j.y], src.fx[j.z], src.fx[j.w]);
+    const float4 fy = make_float4(src.fy[j.x], src.fy[j.y], src.fy[j.z], src.fy[j.w]);
+    const float4 fz = make_float4(src.fz[j.x], src.fz[j.y], src.fz[j.z], src.fz[j.w]);
+    const float4 vx = make_float4(src.vx[j.x], src.vx[j.y], src.vx[j.z], src.vx[j.w]);
+    const float4 vy = make_float4(src.vy[j.x], src.vy[j.y], src.vy[j.z], src.vy[j.w]);
+    const float4 vz = make_float4(src.vz[j.x], src.vz[j.y], src.vz[j.z], src.vz[j.w]);
+    *reinterpret_cast<int4 *>(dst.cell + d0) = c;
+    *reinterpret_cast<float4 *>(dst.fx + d0) = fx;
+    *reinterpret_cast<float4 *>(dst.fy + d0) = fy;
+    *reinterpret_cast<float4 *>(dst.fz + d0) = fz;
+    *reinterpret_cast<float4 *>(dst.vx + d0) = vx;
+    *reinterpret_cast<float4 *>(dst.vy + d0) = vy;
+    *reinterpret_cast<float4 *>(dst.vz + d0) = vz;
+    return;
+  }
+  for (int64_t d = d0; d < m; ++d) {
+    const int64_t j = sidx[d];
+    dst.cell[d] = src.cell[j];
+    dst.fx[d] = src.fx[j]; dst.fy[d] = src.fy[j]; dst.fz[d] = src.fz[j];
+    dst.vx[d] = src.vx[j]; dst.vy[d] = src.vy[j]; dst.vz[d] = src.vz[j];
+  }
 }
 
 __global__ void k_zero_i32(int32_t *a, int64_t n) {
@@ -1263,7 +1286,7 @@ int lepic_sort32(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset32 *src, l
   int32_t *count = w, *start = w + ncells, *cursor = w + 2 * ncells, *bsum = w + 3 * ncells;
   int32_t *idx = bsum + nb + 16;
   idx += (16 - (reinterpret_cast<uintptr_t>(idx) / 4) % 16) % 16;
-  int32_t *sidx = idx + n;
+  int32_t *sidx = idx + ((n + 15) & ~(int64_t)15);   // 64-byte aligned, like idx
   int64_t *total = ctx->d_ctrl + CTRL_COUNT_OUT;
   const KG32 kg = make_kg32(g, 0.0);
   const hipStream_t s = ctx->stream;
@@ -1286,8 +1309,8 @@ int lepic_sort32(lepic_ctx *ctx, const lepic_grid *g, const lepic_pset32 *src, l
   const int64_t m = ctx->h_pinned[0];
   if (m < 0 || m > n) return lepic_fail(ctx, LEPIC_ERR_HIP, "sort32: inconsistent count");
   if (m > 0)
-    hipLaunchKernelGGL(k_sort_gather, dim3(grid_blocks(m, BLOCK)), dim3(BLOCK), 0, s, kp32(src),
-                       kp32(dst), sidx, m);
+    hipLaunchKernelGGL(k_sort_gather, dim3(grid_blocks((m + 3) / 4, BLOCK)), dim3(BLOCK), 0, s,
+                       kp32(src), kp32(dst), sidx, m);
   LEPIC_LAUNCH_CHECK(ctx, "k_sort_gather");
   dst->n = m;
   return LEPIC_OK;
